@@ -1,0 +1,230 @@
+/*
+ * oracle/ref_harness_2d.cpp -- extern "C" wrappers around the reference's own D2Q9 headers.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Compiled (by oracle/Makefile, target `ref`) against the read-only
+ * reference tree, plain path (no _USE_AVX_DEFINES, no _USE_MPI_DEFINES), into
+ * oracle/_ref/libref_panslbm2.so.  It holds no reference code: each wrapper builds a reference
+ * lattice object, loads populations in the reference's AoS layout, calls ONE reference function
+ * and copies the results out.  Boundary lambdas read node-indexed arrays, so a wrapper evaluates
+ * exactly what a driver's lambda would.  Used to write the golden fixtures under tests/golden/
+ * and, when present, as a live cross-check of the oracle.
+ */
+#include <cstring>
+
+#include "src/particle/d2q9.h"
+#include "src/equation/navierstokes.h"
+#include "src/equation/advection.h"
+#include "src/equation/adjointnavierstokes.h"
+#include "src/equation/adjointadvection.h"
+#include "src/utility/residual.h"
+#include "src/utility/normalize.h"
+#include "src/utility/densityfilter.h"
+
+using namespace PANSLBM2;
+
+namespace {
+template <class L>
+void load(L &p, const double *f0, const double *f) {
+    std::memcpy(p.f0, f0, sizeof(double) * p.nxyz);
+    std::memcpy(p.f, f, sizeof(double) * p.nxyz * (L::nc - 1));
+}
+template <class L>
+void save(const L &p, double *f0, double *f) {
+    std::memcpy(f0, p.f0, sizeof(double) * p.nxyz);
+    std::memcpy(f, p.f, sizeof(double) * p.nxyz * (L::nc - 1));
+}
+struct N2 {
+    int lx;
+    const int *a;
+    int operator()(int i, int j) const { return a[i + lx * j]; }
+};
+struct N3 {
+    int lx, ly;
+    const int *a;
+    int operator()(int i, int j, int k) const { return a[i + lx * j + lx * ly * k]; }
+};
+struct V2 {
+    int lx;
+    const double *a;
+    double operator()(int i, int j) const { return a[i + lx * j]; }
+};
+struct V3 {
+    int lx, ly;
+    const double *a;
+    double operator()(int i, int j, int k) const { return a[i + lx * j + lx * ly * k]; }
+};
+}  // namespace
+
+
+extern "C" {
+
+void ref2_stream(int lx, int ly, int lz, double *f0, double *f, int adjoint) {
+    D2Q9<double> p(lx, ly);
+    { load(p, f0, f); if (adjoint) p.iStream(); else p.Stream(); save(p, f0, f); }
+}
+void ref2_bc(int lx, int ly, int lz, double *f0, double *f, const int *type, int adjoint) {
+    D2Q9<double> p(lx, ly);
+    { load(p, f0, f); N2 b{lx, type}; if (adjoint) p.iBoundaryCondition(b); else p.BoundaryCondition(b); save(p, f0, f); }
+}
+void ref2_smooth(int lx, int ly, int lz, double *f0, double *f) {
+    D2Q9<double> p(lx, ly);
+    { load(p, f0, f); p.SmoothCorner(); save(p, f0, f); }
+}
+
+void ref2_ns_set_u(int lx, int ly, int lz, double *f0, double *f, const int *mask, const double *ux,
+                  const double *uy, const double *uz) {
+    D2Q9<double> p(lx, ly);
+    { load(p, f0, f); NS::BoundaryConditionSetU(p, V2{lx, ux}, V2{lx, uy}, N2{lx, mask}); save(p, f0, f); }
+}
+void ref2_ns_set_rho(int lx, int ly, int lz, double *f0, double *f, const int *mask, const double *rho,
+                    const double *us, const double *ut) {
+    D2Q9<double> p(lx, ly);
+    { load(p, f0, f); NS::BoundaryConditionSetRho(p, V2{lx, rho}, V2{lx, us}, N2{lx, mask}); save(p, f0, f); }
+}
+void ref2_ad_set_t(int lx, int ly, int lz, double *g0, double *g, const int *mask, const double *tem,
+                  const double *ux, const double *uy, const double *uz) {
+    D2Q9<double> p(lx, ly);
+    { load(p, g0, g); AD::BoundaryConditionSetT(p, V2{lx, tem}, ux, uy, N2{lx, mask}); save(p, g0, g); }
+}
+void ref2_ad_set_q(int lx, int ly, int lz, double *g0, double *g, const int *mask, const double *qn,
+                  const double *ux, const double *uy, const double *uz, const double *diff_field, double diff_const) {
+    D2Q9<double> p(lx, ly);
+    {
+                 load(p, g0, g);
+                 if (diff_field) AD::BoundaryConditionSetQ(p, V2{lx, qn}, ux, uy, diff_field, N2{lx, mask});
+                 else AD::BoundaryConditionSetQ(p, V2{lx, qn}, ux, uy, diff_const, N2{lx, mask});
+                 save(p, g0, g);
+             }
+}
+void ref2_ans_iset_u(int lx, int ly, int lz, double *f0, double *f, const int *mask, const double *ux,
+                    const double *uy, const double *uz, double eps) {
+    D2Q9<double> p(lx, ly);
+    { load(p, f0, f); ANS::iBoundaryConditionSetU(p, V2{lx, ux}, V2{lx, uy}, N2{lx, mask}, eps); save(p, f0, f); }
+}
+void ref2_ans_iset_rho(int lx, int ly, int lz, double *f0, double *f, const int *mask) {
+    D2Q9<double> p(lx, ly);
+    { load(p, f0, f); ANS::iBoundaryConditionSetRho2D(p, N2{lx, mask}); save(p, f0, f); }
+}
+void ref2_aad_iset_t(int lx, int ly, int lz, double *g0, double *g, const int *mask, const double *ux,
+                    const double *uy, const double *uz) {
+    D2Q9<double> p(lx, ly);
+    { load(p, g0, g); AAD::iBoundaryConditionSetT(p, ux, uy, N2{lx, mask}); save(p, g0, g); }
+}
+void ref2_aad_iset_q(int lx, int ly, int lz, double *g0, double *g, const int *mask, const double *ux,
+                    const double *uy, const double *uz, double eps) {
+    D2Q9<double> p(lx, ly);
+    { load(p, g0, g); AAD::iBoundaryConditionSetQ(p, ux, uy, N2{lx, mask}, eps); save(p, g0, g); }
+}
+void ref2_aad_iset_rho(int lx, int ly, int lz, double *f0, double *f, double *g0, double *g, const int *type,
+                      const double *rho, const double *ux, const double *uy, const double *uz, const double *tem,
+                      double eps) {
+    {
+        D2Q9<double> p(lx, ly), q(lx, ly);
+        load(p, f0, f); load(q, g0, g);
+        AAD::iBoundaryConditionSetRho(p, q, rho, ux, uy, tem, N2{lx, type}, eps);
+        save(p, f0, f); save(q, g0, g);
+    }
+}
+
+void ref2_ns_init(int lx, int ly, int lz, double *f0, double *f, const double *rho, const double *ux,
+                 const double *uy, const double *uz) {
+    D2Q9<double> p(lx, ly);
+    { NS::InitialCondition(p, rho, ux, uy); save(p, f0, f); }
+}
+void ref2_ad_init(int lx, int ly, int lz, double *g0, double *g, const double *tem, const double *ux,
+                 const double *uy, const double *uz) {
+    D2Q9<double> p(lx, ly);
+    { AD::InitialCondition(p, tem, ux, uy); save(p, g0, g); }
+}
+void ref2_ans_init(int lx, int ly, int lz, double *f0, double *f, const double *ux, const double *uy,
+                  const double *uz, const double *ip, const double *iux, const double *iuy, const double *iuz) {
+    D2Q9<double> p(lx, ly);
+    { ANS::InitialCondition(p, ux, uy, ip, iux, iuy); save(p, f0, f); }
+}
+void ref2_aad_init(int lx, int ly, int lz, double *g0, double *g, const double *ux, const double *uy,
+                  const double *uz, const double *item, const double *iqx, const double *iqy, const double *iqz) {
+    D2Q9<double> p(lx, ly);
+    { AAD::InitialCondition(p, ux, uy, item, iqx, iqy); save(p, g0, g); }
+}
+
+void ref2_ns_macro_collide(int lx, int ly, int lz, double *f0, double *f, double *rho, double *ux, double *uy,
+                          double *uz, double nu, int issave) {
+    D2Q9<double> p(lx, ly);
+    { load(p, f0, f); NS::MacroCollide(p, rho, ux, uy, nu, issave != 0); save(p, f0, f); }
+}
+void ref2_ns_macro_brinkman_collide(int lx, int ly, int lz, double *f0, double *f, double *rho, double *ux,
+                                   double *uy, double *uz, double nu, const double *alpha, int issave) {
+    D2Q9<double> p(lx, ly);
+    { load(p, f0, f); NS::MacroBrinkmanCollide(p, rho, ux, uy, nu, alpha, issave != 0); save(p, f0, f); }
+}
+void ref2_ad_collide_fc(int lx, int ly, int lz, double *f0, double *f, double *g0, double *g, double *rho,
+                       double *ux, double *uy, double *uz, const double *alpha, double nu, double *tem, double *qx,
+                       double *qy, double *qz, const double *diff, int issave, double *gsnap) {
+    {
+        D2Q9<double> p(lx, ly), q(lx, ly);
+        load(p, f0, f); load(q, g0, g);
+        AD::MacroBrinkmanCollideForceConvection(p, rho, ux, uy, alpha, nu, q, tem, qx, qy, diff, issave != 0, gsnap);
+        save(p, f0, f); save(q, g0, g);
+    }
+}
+void ref2_ans_collide(int lx, int ly, int lz, double *f0, double *f, const double *rho, const double *ux,
+                     const double *uy, const double *uz, double *ip, double *iux, double *iuy, double *iuz, double *imx,
+                     double *imy, double *imz, double nu, const double *alpha, int issave) {
+    D2Q9<double> p(lx, ly);
+    { load(p, f0, f); ANS::MacroBrinkmanCollide(p, rho, ux, uy, ip, iux, iuy, imx, imy, nu, alpha, issave != 0); save(p, f0, f); }
+}
+void ref2_aad_collide_fc(int lx, int ly, int lz, double *f0, double *f, double *g0, double *g,
+                        const double *rho, const double *ux, const double *uy, const double *uz, double *ip, double *iux,
+                        double *iuy, double *iuz, double *imx, double *imy, double *imz, const double *alpha, double nu,
+                        const double *tem, double *item, double *iqx, double *iqy, double *iqz, const double *diff,
+                        int issave, double *igsnap) {
+    {
+        D2Q9<double> p(lx, ly), q(lx, ly);
+        load(p, f0, f); load(q, g0, g);
+        AAD::MacroBrinkmanCollideForceConvection(p, rho, ux, uy, ip, iux, iuy, imx, imy, alpha, nu, q, tem, item, iqx,
+                                                 iqy, diff, issave != 0, igsnap);
+        save(p, f0, f); save(q, g0, g);
+    }
+}
+
+/* snapshots g / ig in the reference layout nc*idx + c */
+void ref2_sens_brinkman(int lx, int ly, int lz, double *dfds, const double *ux, const double *uy,
+                       const double *uz, const double *imx, const double *imy, const double *imz, const double *dads) {
+    D2Q9<double> p(lx, ly);
+    { ANS::SensitivityBrinkman(p, dfds, ux, uy, imx, imy, dads); }
+}
+void ref2_sens_brinkman_diffusivity(int lx, int ly, int lz, double *dfds, const double *ux, const double *uy,
+                                   const double *uz, const double *imx, const double *imy, const double *imz,
+                                   const double *dads, const double *tem, const double *item, const double *iqx,
+                                   const double *iqy, const double *iqz, const double *g, const double *ig,
+                                   const double *diff, const double *dkds) {
+    D2Q9<double> p(lx, ly);
+    { AAD::SensitivityBrinkmanDiffusivity(p, dfds, ux, uy, imx, imy, dads, tem, item, iqx, iqy, g, ig, diff, dkds); }
+}
+/* 2D only: the plain 3D overload reads out of bounds (argument order at adjointadvection.h:1537-1542) */
+void ref2_sens_temp_heat_source(int lx, int ly, double *dfds, const double *ux, const double *uy, const double *imx,
+                                  const double *imy, const double *dads, const double *tem, const double *item,
+                                  const double *iqx, const double *iqy, const double *g, const double *ig,
+                                  const double *diff, const double *dkds, const int *mask, const double *qn) {
+    D2Q9<double> p(lx, ly);
+    AAD::SensitivityTemperatureAtHeatSource(p, dfds, ux, uy, imx, imy, dads, tem, item, iqx, iqy, g, ig, diff, dkds,
+                                            V2{lx, qn}, N2{lx, mask});
+}
+
+double ref2_residual(int ncomp, const double *a0, const double *a1, const double *a2, const double *b0, const double *b1,
+                    const double *b2, int n) {
+    if (ncomp == 1) return Residual(a0, b0, n);
+    if (ncomp == 2) return Residual(a0, a1, b0, b1, n);
+    return Residual(a0, a1, a2, b0, b1, b2, n);
+}
+void ref2_normalize(double *v, int n) { Normalize(v, n); }
+void ref2_density_filter(int lx, int ly, int lz, double R, const double *v, double *out) {
+    D2Q9<double> p(lx, ly);
+    {
+                 std::vector<double> in(v, v + p.nxyz);
+                 std::vector<double> r = DensityFilter::GetFilteredValue(p, R, in);
+                 std::memcpy(out, r.data(), sizeof(double) * p.nxyz);
+             }
+}
+
+}  // extern "C"

@@ -1,0 +1,215 @@
+/*
+ * oracle/ref_harness_3d.cpp -- extern "C" wrappers around the reference's own D3Q15 headers.
+ * Not wrapped: 3D AAD::iBoundaryConditionSetRho, whose template does not compile when
+ * instantiated (adjointadvection.h:583 names an undeclared _bctype), and the plain 3D
+ * AAD::SensitivityTemperatureAtHeatSource, which reads out of bounds (argument order at
+ * adjointadvection.h:1537-1542 vs :805).
+ *
+ * TEST INFRASTRUCTURE ONLY.  Compiled (by oracle/Makefile, target `ref`) against the read-only
+ * reference tree, plain path (no _USE_AVX_DEFINES, no _USE_MPI_DEFINES), into
+ * oracle/_ref/libref_panslbm2.so.  It holds no reference code: each wrapper builds a reference
+ * lattice object, loads populations in the reference's AoS layout, calls ONE reference function
+ * and copies the results out.  Boundary lambdas read node-indexed arrays, so a wrapper evaluates
+ * exactly what a driver's lambda would.  Used to write the golden fixtures under tests/golden/
+ * and, when present, as a live cross-check of the oracle.
+ */
+#include <cstring>
+
+#include "src/particle/d3q15.h"
+#include "src/equation/navierstokes.h"
+#include "src/equation/advection.h"
+#include "src/equation/adjointnavierstokes.h"
+#include "src/equation/adjointadvection.h"
+#include "src/utility/residual.h"
+#include "src/utility/normalize.h"
+#include "src/utility/densityfilter.h"
+
+using namespace PANSLBM2;
+
+namespace {
+template <class L>
+void load(L &p, const double *f0, const double *f) {
+    std::memcpy(p.f0, f0, sizeof(double) * p.nxyz);
+    std::memcpy(p.f, f, sizeof(double) * p.nxyz * (L::nc - 1));
+}
+template <class L>
+void save(const L &p, double *f0, double *f) {
+    std::memcpy(f0, p.f0, sizeof(double) * p.nxyz);
+    std::memcpy(f, p.f, sizeof(double) * p.nxyz * (L::nc - 1));
+}
+struct N2 {
+    int lx;
+    const int *a;
+    int operator()(int i, int j) const { return a[i + lx * j]; }
+};
+struct N3 {
+    int lx, ly;
+    const int *a;
+    int operator()(int i, int j, int k) const { return a[i + lx * j + lx * ly * k]; }
+};
+struct V2 {
+    int lx;
+    const double *a;
+    double operator()(int i, int j) const { return a[i + lx * j]; }
+};
+struct V3 {
+    int lx, ly;
+    const double *a;
+    double operator()(int i, int j, int k) const { return a[i + lx * j + lx * ly * k]; }
+};
+}  // namespace
+
+
+extern "C" {
+
+void ref3_stream(int lx, int ly, int lz, double *f0, double *f, int adjoint) {
+    D3Q15<double> p(lx, ly, lz);
+    { load(p, f0, f); if (adjoint) p.iStream(); else p.Stream(); save(p, f0, f); }
+}
+void ref3_bc(int lx, int ly, int lz, double *f0, double *f, const int *type, int adjoint) {
+    D3Q15<double> p(lx, ly, lz);
+    { load(p, f0, f); N3 b{lx, ly, type}; if (adjoint) p.iBoundaryCondition(b); else p.BoundaryCondition(b); save(p, f0, f); }
+}
+void ref3_smooth(int lx, int ly, int lz, double *f0, double *f) {
+    D3Q15<double> p(lx, ly, lz);
+    { load(p, f0, f); p.SmoothCorner(); save(p, f0, f); }
+}
+
+void ref3_ns_set_u(int lx, int ly, int lz, double *f0, double *f, const int *mask, const double *ux,
+                  const double *uy, const double *uz) {
+    D3Q15<double> p(lx, ly, lz);
+    { load(p, f0, f); NS::BoundaryConditionSetU(p, V3{lx, ly, ux}, V3{lx, ly, uy}, V3{lx, ly, uz}, N3{lx, ly, mask}); save(p, f0, f); }
+}
+void ref3_ns_set_rho(int lx, int ly, int lz, double *f0, double *f, const int *mask, const double *rho,
+                    const double *us, const double *ut) {
+    D3Q15<double> p(lx, ly, lz);
+    { load(p, f0, f); NS::BoundaryConditionSetRho(p, V3{lx, ly, rho}, V3{lx, ly, us}, V3{lx, ly, ut}, N3{lx, ly, mask}); save(p, f0, f); }
+}
+void ref3_ad_set_t(int lx, int ly, int lz, double *g0, double *g, const int *mask, const double *tem,
+                  const double *ux, const double *uy, const double *uz) {
+    D3Q15<double> p(lx, ly, lz);
+    { load(p, g0, g); AD::BoundaryConditionSetT(p, V3{lx, ly, tem}, ux, uy, uz, N3{lx, ly, mask}); save(p, g0, g); }
+}
+void ref3_ad_set_q(int lx, int ly, int lz, double *g0, double *g, const int *mask, const double *qn,
+                  const double *ux, const double *uy, const double *uz, const double *diff_field, double diff_const) {
+    D3Q15<double> p(lx, ly, lz);
+    {
+                 load(p, g0, g);
+                 if (diff_field) AD::BoundaryConditionSetQ(p, V3{lx, ly, qn}, ux, uy, uz, diff_field, N3{lx, ly, mask});
+                 else AD::BoundaryConditionSetQ(p, V3{lx, ly, qn}, ux, uy, uz, diff_const, N3{lx, ly, mask});
+                 save(p, g0, g);
+             }
+}
+void ref3_ans_iset_u(int lx, int ly, int lz, double *f0, double *f, const int *mask, const double *ux,
+                    const double *uy, const double *uz, double eps) {
+    D3Q15<double> p(lx, ly, lz);
+    { load(p, f0, f); ANS::iBoundaryConditionSetU(p, V3{lx, ly, ux}, V3{lx, ly, uy}, V3{lx, ly, uz}, N3{lx, ly, mask}, eps); save(p, f0, f); }
+}
+void ref3_ans_iset_rho(int lx, int ly, int lz, double *f0, double *f, const int *mask) {
+    D3Q15<double> p(lx, ly, lz);
+    { load(p, f0, f); ANS::iBoundaryConditionSetRho3D(p, N3{lx, ly, mask}); save(p, f0, f); }
+}
+void ref3_aad_iset_t(int lx, int ly, int lz, double *g0, double *g, const int *mask, const double *ux,
+                    const double *uy, const double *uz) {
+    D3Q15<double> p(lx, ly, lz);
+    { load(p, g0, g); AAD::iBoundaryConditionSetT(p, ux, uy, uz, N3{lx, ly, mask}); save(p, g0, g); }
+}
+void ref3_aad_iset_q(int lx, int ly, int lz, double *g0, double *g, const int *mask, const double *ux,
+                    const double *uy, const double *uz, double eps) {
+    D3Q15<double> p(lx, ly, lz);
+    { load(p, g0, g); AAD::iBoundaryConditionSetQ(p, ux, uy, uz, N3{lx, ly, mask}, eps); save(p, g0, g); }
+}
+
+void ref3_ns_init(int lx, int ly, int lz, double *f0, double *f, const double *rho, const double *ux,
+                 const double *uy, const double *uz) {
+    D3Q15<double> p(lx, ly, lz);
+    { NS::InitialCondition(p, rho, ux, uy, uz); save(p, f0, f); }
+}
+void ref3_ad_init(int lx, int ly, int lz, double *g0, double *g, const double *tem, const double *ux,
+                 const double *uy, const double *uz) {
+    D3Q15<double> p(lx, ly, lz);
+    { AD::InitialCondition(p, tem, ux, uy, uz); save(p, g0, g); }
+}
+void ref3_ans_init(int lx, int ly, int lz, double *f0, double *f, const double *ux, const double *uy,
+                  const double *uz, const double *ip, const double *iux, const double *iuy, const double *iuz) {
+    D3Q15<double> p(lx, ly, lz);
+    { ANS::InitialCondition(p, ux, uy, uz, ip, iux, iuy, iuz); save(p, f0, f); }
+}
+void ref3_aad_init(int lx, int ly, int lz, double *g0, double *g, const double *ux, const double *uy,
+                  const double *uz, const double *item, const double *iqx, const double *iqy, const double *iqz) {
+    D3Q15<double> p(lx, ly, lz);
+    { AAD::InitialCondition(p, ux, uy, uz, item, iqx, iqy, iqz); save(p, g0, g); }
+}
+
+void ref3_ns_macro_collide(int lx, int ly, int lz, double *f0, double *f, double *rho, double *ux, double *uy,
+                          double *uz, double nu, int issave) {
+    D3Q15<double> p(lx, ly, lz);
+    { load(p, f0, f); NS::MacroCollide(p, rho, ux, uy, uz, nu, issave != 0); save(p, f0, f); }
+}
+void ref3_ns_macro_brinkman_collide(int lx, int ly, int lz, double *f0, double *f, double *rho, double *ux,
+                                   double *uy, double *uz, double nu, const double *alpha, int issave) {
+    D3Q15<double> p(lx, ly, lz);
+    { load(p, f0, f); NS::MacroBrinkmanCollide(p, rho, ux, uy, uz, nu, alpha, issave != 0); save(p, f0, f); }
+}
+void ref3_ad_collide_fc(int lx, int ly, int lz, double *f0, double *f, double *g0, double *g, double *rho,
+                       double *ux, double *uy, double *uz, const double *alpha, double nu, double *tem, double *qx,
+                       double *qy, double *qz, const double *diff, int issave, double *gsnap) {
+    {
+        D3Q15<double> p(lx, ly, lz), q(lx, ly, lz);
+        load(p, f0, f); load(q, g0, g);
+        AD::MacroBrinkmanCollideForceConvection(p, rho, ux, uy, uz, alpha, nu, q, tem, qx, qy, qz, diff, issave != 0, gsnap);
+        save(p, f0, f); save(q, g0, g);
+    }
+}
+void ref3_ans_collide(int lx, int ly, int lz, double *f0, double *f, const double *rho, const double *ux,
+                     const double *uy, const double *uz, double *ip, double *iux, double *iuy, double *iuz, double *imx,
+                     double *imy, double *imz, double nu, const double *alpha, int issave) {
+    D3Q15<double> p(lx, ly, lz);
+    { load(p, f0, f); ANS::MacroBrinkmanCollide(p, rho, ux, uy, uz, ip, iux, iuy, iuz, imx, imy, imz, nu, alpha, issave != 0); save(p, f0, f); }
+}
+void ref3_aad_collide_fc(int lx, int ly, int lz, double *f0, double *f, double *g0, double *g,
+                        const double *rho, const double *ux, const double *uy, const double *uz, double *ip, double *iux,
+                        double *iuy, double *iuz, double *imx, double *imy, double *imz, const double *alpha, double nu,
+                        const double *tem, double *item, double *iqx, double *iqy, double *iqz, const double *diff,
+                        int issave, double *igsnap) {
+    {
+        D3Q15<double> p(lx, ly, lz), q(lx, ly, lz);
+        load(p, f0, f); load(q, g0, g);
+        AAD::MacroBrinkmanCollideForceConvection(p, rho, ux, uy, uz, ip, iux, iuy, iuz, imx, imy, imz, alpha, nu, q,
+                                                 tem, item, iqx, iqy, iqz, diff, issave != 0, igsnap);
+        save(p, f0, f); save(q, g0, g);
+    }
+}
+
+/* snapshots g / ig in the reference layout nc*idx + c */
+void ref3_sens_brinkman(int lx, int ly, int lz, double *dfds, const double *ux, const double *uy,
+                       const double *uz, const double *imx, const double *imy, const double *imz, const double *dads) {
+    D3Q15<double> p(lx, ly, lz);
+    { ANS::SensitivityBrinkman(p, dfds, ux, uy, uz, imx, imy, imz, dads); }
+}
+void ref3_sens_brinkman_diffusivity(int lx, int ly, int lz, double *dfds, const double *ux, const double *uy,
+                                   const double *uz, const double *imx, const double *imy, const double *imz,
+                                   const double *dads, const double *tem, const double *item, const double *iqx,
+                                   const double *iqy, const double *iqz, const double *g, const double *ig,
+                                   const double *diff, const double *dkds) {
+    D3Q15<double> p(lx, ly, lz);
+    { AAD::SensitivityBrinkmanDiffusivity(p, dfds, ux, uy, uz, imx, imy, imz, dads, tem, item, iqx, iqy, iqz, g, ig, diff, dkds); }
+}
+
+double ref3_residual(int ncomp, const double *a0, const double *a1, const double *a2, const double *b0, const double *b1,
+                    const double *b2, int n) {
+    if (ncomp == 1) return Residual(a0, b0, n);
+    if (ncomp == 2) return Residual(a0, a1, b0, b1, n);
+    return Residual(a0, a1, a2, b0, b1, b2, n);
+}
+void ref3_normalize(double *v, int n) { Normalize(v, n); }
+void ref3_density_filter(int lx, int ly, int lz, double R, const double *v, double *out) {
+    D3Q15<double> p(lx, ly, lz);
+    {
+                 std::vector<double> in(v, v + p.nxyz);
+                 std::vector<double> r = DensityFilter::GetFilteredValue(p, R, in);
+                 std::memcpy(out, r.data(), sizeof(double) * p.nxyz);
+             }
+}
+
+}  // extern "C"

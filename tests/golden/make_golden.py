@@ -1,0 +1,46 @@
+"""Write the golden fixtures in tests/golden/ from the compiled reference harness.
+
+Run in a container that has the reference tree (to build oracle/_ref) :
+    make -C oracle ref && python tests/golden/make_golden.py
+
+For every case in tests/refcases.py it stores the seeded inputs and, per reference function,
+the outputs the reference produced.  The fixtures are data only (inputs and expected outputs).
+Also stores short driver-sequence runs (lid-driven cavity, coupled forced convection forward and
+adjoint) composed from reference calls in the order of test/cavityflow.cpp:44-67 and
+test/nsadsens.cpp:67-133.
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+
+import refcases  # noqa: E402
+import sequences  # noqa: E402
+from _oracle import ref  # noqa: E402
+
+
+def main():
+    if ref() is None:
+        raise SystemExit("reference harness not built: make -C oracle ref")
+    for case in refcases.CASES_2D + refcases.CASES_3D:
+        inp = refcases.make_inputs(case)
+        blob = {"in/" + k: np.asarray(v) for k, v in inp.items()}
+        for op in refcases.ops_for(case):
+            out = refcases.run_ref(case, op, inp)
+            for k, v in out.items():
+                blob["out/%s/%s" % (op, k)] = np.asarray(v)
+        path = os.path.join(HERE, case.name + ".npz")
+        np.savez_compressed(path, **blob)
+        print("wrote", path, os.path.getsize(path), "bytes")
+    for seq in sequences.SEQUENCES:
+        out = seq.run(sequences.RefBackend())
+        path = os.path.join(HERE, "seq_" + seq.name + ".npz")
+        np.savez_compressed(path, **out)
+        print("wrote", path, os.path.getsize(path), "bytes")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,475 @@
+"""Driver sequences written once against a small backend interface, so the same time-step loop
+runs on the compiled reference harness, on the CPU oracle and on the HIP path (tests/gpu_backend.py).
+
+Fields are addressed by name; each backend owns their storage (numpy here, device memory on the
+GPU), which is how the reference drivers pass raw arrays and swap them (heatsink.cpp:170-173).
+Boundary lambdas are given as node-indexed arrays.
+"""
+import ctypes
+
+import numpy as np
+
+from _oracle import D2Q9, NC, ND, Lattice, aos_to_soa, lib, ptr, ref, soa_to_aos, to_face
+
+D = ctypes.c_double
+
+
+class _FieldStore:
+    def __init__(self):
+        self.fields = {}
+
+    def put(self, name, arr):
+        self.fields[name] = np.array(arr, dtype=np.float64)
+
+    def zeros(self, name, n):
+        self.fields[name] = np.zeros(n)
+
+    def get(self, name):
+        return self.fields[name].copy()
+
+    def swap(self, a, b):
+        self.fields[a], self.fields[b] = self.fields[b], self.fields[a]
+
+    def f(self, name):
+        return None if name is None else self.fields[name]
+
+
+class OracleBackend(_FieldStore):
+    name = "oracle"
+
+    def lattice(self, kind, lx, ly, lz=1):
+        return Lattice(kind, lx, ly, lz)
+
+    def pops(self, h):
+        return h.get()
+
+    def set_pops(self, h, soa):
+        h.set(soa)
+
+    def _face(self, h, a, dt=None):
+        return to_face(h.kind, h.lx, h.ly, h.lz, a, dt)
+
+    def stream(self, h):
+        lib().ol_stream(h.h)
+
+    def istream(self, h):
+        lib().ol_istream(h.h)
+
+    def bc(self, h, type_node, adjoint=False):
+        lib().ol_bc(h.h, ptr(self._face(h, type_node, np.int32)), int(adjoint))
+
+    def smooth(self, h):
+        lib().ol_smooth_corner(h.h)
+
+    def ns_set_u(self, h, mask, ux, uy, uz=None):
+        uz = np.zeros_like(ux) if uz is None else uz
+        lib().ol_ns_set_u(h.h, ptr(self._face(h, mask, np.int32)), ptr(self._face(h, ux)), ptr(self._face(h, uy)),
+                          ptr(self._face(h, uz)))
+
+    def ns_set_rho(self, h, mask, rho, us, ut=None):
+        ut = np.zeros_like(us) if ut is None else ut
+        lib().ol_ns_set_rho(h.h, ptr(self._face(h, mask, np.int32)), ptr(self._face(h, rho)), ptr(self._face(h, us)),
+                            ptr(self._face(h, ut)))
+
+    def ad_set_t(self, h, mask, tem, ux, uy, uz=None):
+        lib().ol_ad_set_t(h.h, ptr(self._face(h, mask, np.int32)), ptr(self._face(h, tem)), ptr(self.f(ux)),
+                          ptr(self.f(uy)), ptr(self.f(uz)))
+
+    def ad_set_q(self, h, mask, qn, ux, uy, uz, diff):
+        field = isinstance(diff, str)
+        lib().ol_ad_set_q(h.h, ptr(self._face(h, mask, np.int32)), ptr(self._face(h, qn)), ptr(self.f(ux)),
+                          ptr(self.f(uy)), ptr(self.f(uz)), ptr(self.f(diff)) if field else None,
+                          D(0.0 if field else float(diff)))
+
+    def ans_iset_u(self, h, mask, ux, uy, uz=None, eps=0.0):
+        uz = np.zeros_like(ux) if uz is None else uz
+        lib().ol_ans_iset_u(h.h, ptr(self._face(h, mask, np.int32)), ptr(self._face(h, ux)), ptr(self._face(h, uy)),
+                            ptr(self._face(h, uz)), D(eps), 1)
+
+    def ans_iset_rho(self, h, mask):
+        lib().ol_ans_iset_rho(h.h, ptr(self._face(h, mask, np.int32)))
+
+    def aad_iset_t(self, h, mask, ux, uy, uz=None):
+        lib().ol_aad_iset_t(h.h, ptr(self._face(h, mask, np.int32)), ptr(self.f(ux)), ptr(self.f(uy)), ptr(self.f(uz)))
+
+    def aad_iset_q(self, h, mask, ux, uy, uz=None, eps=0.0):
+        lib().ol_aad_iset_q(h.h, ptr(self._face(h, mask, np.int32)), ptr(self.f(ux)), ptr(self.f(uy)),
+                            ptr(self.f(uz)), D(eps))
+
+    def aad_iset_rho(self, p, q, type_node, rho, ux, uy, uz, tem, eps=0.0):
+        lib().ol_aad_iset_rho(p.h, q.h, ptr(self._face(p, type_node, np.int32)), ptr(self.f(rho)), ptr(self.f(ux)),
+                              ptr(self.f(uy)), ptr(self.f(uz)), ptr(self.f(tem)), D(eps))
+
+    def ns_init(self, h, rho, ux, uy, uz=None):
+        lib().ol_ns_init(h.h, ptr(self.f(rho)), ptr(self.f(ux)), ptr(self.f(uy)), ptr(self.f(uz)))
+
+    def ad_init(self, h, tem, ux, uy, uz=None):
+        lib().ol_ad_init(h.h, ptr(self.f(tem)), ptr(self.f(ux)), ptr(self.f(uy)), ptr(self.f(uz)))
+
+    def ans_init(self, h, ux, uy, uz, ip, iux, iuy, iuz):
+        lib().ol_ans_init(h.h, *[ptr(self.f(n)) for n in (ux, uy, uz, ip, iux, iuy, iuz)])
+
+    def aad_init(self, h, ux, uy, uz, item, iqx, iqy, iqz):
+        lib().ol_aad_init(h.h, *[ptr(self.f(n)) for n in (ux, uy, uz, item, iqx, iqy, iqz)])
+
+    def ns_collide(self, h, rho, ux, uy, uz, nu, save=True):
+        lib().ol_ns_macro_collide(h.h, *[ptr(self.f(n)) for n in (rho, ux, uy, uz)], D(nu), int(save))
+
+    def ns_brinkman_collide(self, h, rho, ux, uy, uz, nu, alpha, save=True):
+        lib().ol_ns_macro_brinkman_collide(h.h, *[ptr(self.f(n)) for n in (rho, ux, uy, uz)], D(nu),
+                                           ptr(self.f(alpha)), int(save))
+
+    def ad_collide_fc(self, p, rho, ux, uy, uz, alpha, nu, q, tem, qx, qy, qz, diff, save=True, gsnap=None,
+                      heat=None):
+        lib().ol_ad_collide_fc(p.h, *[ptr(self.f(n)) for n in (rho, ux, uy, uz, alpha)], D(nu), q.h,
+                               *[ptr(self.f(n)) for n in (tem, qx, qy, qz, diff)], int(save), ptr(self.f(gsnap)),
+                               ptr(self.f(heat)))
+
+    def ans_collide(self, h, rho, ux, uy, uz, ip, iux, iuy, iuz, imx, imy, imz, nu, alpha, save=True):
+        lib().ol_ans_collide(h.h, *[ptr(self.f(n)) for n in (rho, ux, uy, uz, ip, iux, iuy, iuz, imx, imy, imz)],
+                             D(nu), ptr(self.f(alpha)), int(save))
+
+    def aad_collide_fc(self, p, rho, ux, uy, uz, ip, iux, iuy, iuz, imx, imy, imz, alpha, nu, q, tem, item, iqx, iqy,
+                       iqz, diff, save=True, igsnap=None):
+        lib().ol_aad_collide_fc(p.h, *[ptr(self.f(n)) for n in (rho, ux, uy, uz, ip, iux, iuy, iuz, imx, imy, imz,
+                                                                alpha)], D(nu), q.h,
+                                *[ptr(self.f(n)) for n in (tem, item, iqx, iqy, iqz, diff)], int(save),
+                                ptr(self.f(igsnap)))
+
+    def sens_temp_heat_source(self, q, dfds, ux, uy, uz, imx, imy, imz, dads, tem, item, iqx, iqy, iqz, g, ig, diff,
+                              dkds, mask, qn):
+        lib().ol_sens_temp_heat_source(q.h, *[ptr(self.f(n)) for n in (dfds, ux, uy, uz, imx, imy, imz, dads, tem,
+                                                                       item, iqx, iqy, iqz, g, ig, diff, dkds)],
+                                       ptr(self._face(q, mask, np.int32)), ptr(self._face(q, qn)))
+
+    def residual(self, a, b):
+        n = len(a)
+        aa = [self.f(x) for x in a] + [None] * (3 - n)
+        bb = [self.f(x) for x in b] + [None] * (3 - n)
+        N = aa[0].shape[0]
+        return lib().ol_residual(n, *[ptr(x) for x in aa], *[ptr(x) for x in bb], N)
+
+
+class _RefLat:
+    def __init__(self, kind, lx, ly, lz):
+        self.kind, self.lx, self.ly, self.lz = kind, lx, ly, (lz if ND[kind] == 3 else 1)
+        self.N = lx * ly * self.lz
+        self.f0 = np.zeros(self.N)
+        self.f = np.zeros((NC[kind] - 1) * self.N)
+
+    @property
+    def L(self):
+        return (self.lx, self.ly, self.lz)
+
+    @property
+    def pre(self):
+        return "ref2_" if ND[self.kind] == 2 else "ref3_"
+
+
+class RefBackend(_FieldStore):
+    """Each call runs ONE reference function on a freshly built reference lattice (d2q9.h ctor)."""
+
+    name = "reference"
+
+    def _r(self, h, n):
+        return getattr(ref(), h.pre + n)
+
+    def lattice(self, kind, lx, ly, lz=1):
+        return _RefLat(kind, lx, ly, lz)
+
+    def pops(self, h):
+        return aos_to_soa(h.f0, h.f, NC[h.kind])
+
+    def set_pops(self, h, soa):
+        h.f0[:], h.f[:] = soa_to_aos(soa)
+
+    def _i(self, a):
+        return np.ascontiguousarray(a, dtype=np.int32)
+
+    def _d(self, a):
+        return np.ascontiguousarray(a, dtype=np.float64)
+
+    def stream(self, h):
+        self._r(h, "stream")(*h.L, ptr(h.f0), ptr(h.f), 0)
+
+    def istream(self, h):
+        self._r(h, "stream")(*h.L, ptr(h.f0), ptr(h.f), 1)
+
+    def bc(self, h, type_node, adjoint=False):
+        self._r(h, "bc")(*h.L, ptr(h.f0), ptr(h.f), ptr(self._i(type_node)), int(adjoint))
+
+    def smooth(self, h):
+        self._r(h, "smooth")(*h.L, ptr(h.f0), ptr(h.f))
+
+    def ns_set_u(self, h, mask, ux, uy, uz=None):
+        uz = np.zeros_like(ux) if uz is None else uz
+        self._r(h, "ns_set_u")(*h.L, ptr(h.f0), ptr(h.f), ptr(self._i(mask)), ptr(self._d(ux)), ptr(self._d(uy)),
+                               ptr(self._d(uz)))
+
+    def ns_set_rho(self, h, mask, rho, us, ut=None):
+        ut = np.zeros_like(us) if ut is None else ut
+        self._r(h, "ns_set_rho")(*h.L, ptr(h.f0), ptr(h.f), ptr(self._i(mask)), ptr(self._d(rho)), ptr(self._d(us)),
+                                 ptr(self._d(ut)))
+
+    def ad_set_t(self, h, mask, tem, ux, uy, uz=None):
+        self._r(h, "ad_set_t")(*h.L, ptr(h.f0), ptr(h.f), ptr(self._i(mask)), ptr(self._d(tem)), ptr(self.f(ux)),
+                               ptr(self.f(uy)), ptr(self.f(uz)))
+
+    def ad_set_q(self, h, mask, qn, ux, uy, uz, diff):
+        field = isinstance(diff, str)
+        self._r(h, "ad_set_q")(*h.L, ptr(h.f0), ptr(h.f), ptr(self._i(mask)), ptr(self._d(qn)), ptr(self.f(ux)),
+                               ptr(self.f(uy)), ptr(self.f(uz)), ptr(self.f(diff)) if field else None,
+                               D(0.0 if field else float(diff)))
+
+    def ans_iset_u(self, h, mask, ux, uy, uz=None, eps=0.0):
+        uz = np.zeros_like(ux) if uz is None else uz
+        self._r(h, "ans_iset_u")(*h.L, ptr(h.f0), ptr(h.f), ptr(self._i(mask)), ptr(self._d(ux)), ptr(self._d(uy)),
+                                 ptr(self._d(uz)), D(eps))
+
+    def ans_iset_rho(self, h, mask):
+        self._r(h, "ans_iset_rho")(*h.L, ptr(h.f0), ptr(h.f), ptr(self._i(mask)))
+
+    def aad_iset_t(self, h, mask, ux, uy, uz=None):
+        self._r(h, "aad_iset_t")(*h.L, ptr(h.f0), ptr(h.f), ptr(self._i(mask)), ptr(self.f(ux)), ptr(self.f(uy)),
+                                 ptr(self.f(uz)))
+
+    def aad_iset_q(self, h, mask, ux, uy, uz=None, eps=0.0):
+        self._r(h, "aad_iset_q")(*h.L, ptr(h.f0), ptr(h.f), ptr(self._i(mask)), ptr(self.f(ux)), ptr(self.f(uy)),
+                                 ptr(self.f(uz)), D(eps))
+
+    def aad_iset_rho(self, p, q, type_node, rho, ux, uy, uz, tem, eps=0.0):
+        self._r(p, "aad_iset_rho")(*p.L, ptr(p.f0), ptr(p.f), ptr(q.f0), ptr(q.f), ptr(self._i(type_node)),
+                                   ptr(self.f(rho)), ptr(self.f(ux)), ptr(self.f(uy)), ptr(self.f(uz)),
+                                   ptr(self.f(tem)), D(eps))
+
+    def ns_init(self, h, rho, ux, uy, uz=None):
+        self._r(h, "ns_init")(*h.L, ptr(h.f0), ptr(h.f), *[ptr(self.f(n)) for n in (rho, ux, uy, uz)])
+
+    def ad_init(self, h, tem, ux, uy, uz=None):
+        self._r(h, "ad_init")(*h.L, ptr(h.f0), ptr(h.f), *[ptr(self.f(n)) for n in (tem, ux, uy, uz)])
+
+    def ans_init(self, h, ux, uy, uz, ip, iux, iuy, iuz):
+        self._r(h, "ans_init")(*h.L, ptr(h.f0), ptr(h.f), *[ptr(self.f(n)) for n in (ux, uy, uz, ip, iux, iuy, iuz)])
+
+    def aad_init(self, h, ux, uy, uz, item, iqx, iqy, iqz):
+        self._r(h, "aad_init")(*h.L, ptr(h.f0), ptr(h.f),
+                               *[ptr(self.f(n)) for n in (ux, uy, uz, item, iqx, iqy, iqz)])
+
+    def ns_collide(self, h, rho, ux, uy, uz, nu, save=True):
+        self._r(h, "ns_macro_collide")(*h.L, ptr(h.f0), ptr(h.f), *[ptr(self.f(n)) for n in (rho, ux, uy, uz)], D(nu),
+                                       int(save))
+
+    def ns_brinkman_collide(self, h, rho, ux, uy, uz, nu, alpha, save=True):
+        self._r(h, "ns_macro_brinkman_collide")(*h.L, ptr(h.f0), ptr(h.f), *[ptr(self.f(n)) for n in (rho, ux, uy, uz)],
+                                                D(nu), ptr(self.f(alpha)), int(save))
+
+    def _snap_out(self, name, q, buf):
+        if name is not None:
+            self.fields[name] = buf.reshape(q.N, NC[q.kind]).T.copy()
+
+    def ad_collide_fc(self, p, rho, ux, uy, uz, alpha, nu, q, tem, qx, qy, qz, diff, save=True, gsnap=None,
+                      heat=None):
+        assert heat is None, "the reference has no volumetric heat source"
+        buf = np.zeros(NC[q.kind] * q.N)
+        self._r(p, "ad_collide_fc")(*p.L, ptr(p.f0), ptr(p.f), ptr(q.f0), ptr(q.f),
+                                    *[ptr(self.f(n)) for n in (rho, ux, uy, uz, alpha)], D(nu),
+                                    *[ptr(self.f(n)) for n in (tem, qx, qy, qz, diff)], int(save),
+                                    ptr(buf) if gsnap else None)
+        if save:
+            self._snap_out(gsnap, q, buf)
+
+    def ans_collide(self, h, rho, ux, uy, uz, ip, iux, iuy, iuz, imx, imy, imz, nu, alpha, save=True):
+        self._r(h, "ans_collide")(*h.L, ptr(h.f0), ptr(h.f),
+                                  *[ptr(self.f(n)) for n in (rho, ux, uy, uz, ip, iux, iuy, iuz, imx, imy, imz)],
+                                  D(nu), ptr(self.f(alpha)), int(save))
+
+    def aad_collide_fc(self, p, rho, ux, uy, uz, ip, iux, iuy, iuz, imx, imy, imz, alpha, nu, q, tem, item, iqx, iqy,
+                       iqz, diff, save=True, igsnap=None):
+        buf = np.zeros(NC[q.kind] * q.N)
+        self._r(p, "aad_collide_fc")(*p.L, ptr(p.f0), ptr(p.f), ptr(q.f0), ptr(q.f),
+                                     *[ptr(self.f(n)) for n in (rho, ux, uy, uz, ip, iux, iuy, iuz, imx, imy, imz,
+                                                                alpha)], D(nu),
+                                     *[ptr(self.f(n)) for n in (tem, item, iqx, iqy, iqz, diff)], int(save),
+                                     ptr(buf) if igsnap else None)
+        if save:
+            self._snap_out(igsnap, q, buf)
+
+    def sens_temp_heat_source(self, q, dfds, ux, uy, uz, imx, imy, imz, dads, tem, item, iqx, iqy, iqz, g, ig, diff,
+                              dkds, mask, qn):
+        assert ND[q.kind] == 2
+        gs = np.ascontiguousarray(self.f(g).T).reshape(-1)
+        igs = np.ascontiguousarray(self.f(ig).T).reshape(-1)
+        ref().ref2_sens_temp_heat_source(q.lx, q.ly, *[ptr(self.f(n)) for n in (dfds, ux, uy, imx, imy, dads, tem,
+                                                                               item, iqx, iqy)],
+                                         ptr(gs), ptr(igs), ptr(self.f(diff)), ptr(self.f(dkds)),
+                                         ptr(self._i(mask)), ptr(self._d(qn)))
+
+    def residual(self, a, b):
+        n = len(a)
+        aa = [self.f(x) for x in a] + [None] * (3 - n)
+        bb = [self.f(x) for x in b] + [None] * (3 - n)
+        fr = ref().ref2_residual
+        fr.restype = D
+        return fr(n, *[ptr(x) for x in aa], *[ptr(x) for x in bb], aa[0].shape[0])
+
+
+# ------------------------------------------------------------------------------------------------
+class Cavity:
+    """test/cavityflow.cpp:44-67 (D2Q9 BGK, bounce-back walls, Zou-He lid, SmoothCorner)."""
+
+    def __init__(self, name, lx, ly, nt, nu=0.1, u0=0.1):
+        self.name, self.lx, self.ly, self.nt, self.nu, self.u0 = name, lx, ly, nt, nu, u0
+
+    def masks(self):
+        lx, ly = self.lx, self.ly
+        i, j = np.meshgrid(np.arange(lx), np.arange(ly), indexing="xy")
+        i, j = i.reshape(-1), j.reshape(-1)
+        wall = ((i == 0) | (i == lx - 1) | (j == 0)).astype(np.int32)
+        lid = (j == ly - 1).astype(np.int32)
+        return wall, lid
+
+    def run(self, b):
+        N = self.lx * self.ly
+        pf = b.lattice(D2Q9, self.lx, self.ly)
+        b.put("rho", np.ones(N))
+        b.zeros("ux", N)
+        b.zeros("uy", N)
+        wall, lid = self.masks()
+        b.ns_init(pf, "rho", "ux", "uy")
+        for _ in range(self.nt):
+            b.ns_collide(pf, "rho", "ux", "uy", None, self.nu, True)
+            b.stream(pf)
+            b.bc(pf, wall)
+            b.ns_set_u(pf, lid, np.full(N, self.u0), np.zeros(N))
+            b.smooth(pf)
+        return {"f": b.pops(pf), "rho": b.get("rho"), "ux": b.get("ux"), "uy": b.get("uy")}
+
+
+class ForcedConvection2D:
+    """Coupled NS + AD forced convection in a channel, forward then adjoint, then the heat-source
+    sensitivity: the call order of test/nsadsens.cpp:67-133 with the ForceConvection collides
+    (advection.h:814, adjointadvection.h:997) and the heat-flux objective of
+    production/heatsink.cpp:156-166,200-232."""
+
+    def __init__(self, name, lx, ly, nt, nu=0.1, pr=6.0, umax=0.05, qn=1e-2, L=None, seed=5, check=None):
+        self.name, self.lx, self.ly, self.nt = name, lx, ly, nt
+        self.nu, self.pr, self.umax, self.qn = nu, pr, umax, qn
+        self.L = L if L is not None else lx // 2
+        self.seed = seed
+        self.check = check
+
+    def setup(self):
+        lx, ly = self.lx, self.ly
+        N = lx * ly
+        i, j = np.meshgrid(np.arange(lx), np.arange(ly), indexing="xy")
+        i, j = i.reshape(-1), j.reshape(-1)
+        rng = np.random.default_rng(self.seed)
+        s = rng.uniform(0.0, 1.0, N)
+        qf, qg, amax = 0.01, 1.0, 1.0
+        dfl = self.nu / self.pr
+        dso = 10.0 * dfl
+        d = dict(
+            i=i, j=j, s=s,
+            alpha=amax * qf * (1.0 - s) / (s + qf),
+            dads=-amax * qf * (1.0 + qf) / (s + qf) ** 2,
+            diff=dso + (dfl - dso) * s * (1.0 + qg) / (s + qg),
+            dkds=(dfl - dso) * qg * (1.0 + qg) / (s + qg) ** 2,
+        )
+        H = ly - 1.0
+        d["uin"] = 4.0 * self.umax * j * (H - j) / H ** 2
+        d["wall_f"] = np.where(j == 0, 1, np.where(j == ly - 1, 2, 0)).astype(np.int32)
+        d["inlet"] = ((i == 0) & (j > 0) & (j < ly - 1)).astype(np.int32)
+        d["outlet"] = ((i == lx - 1) & (j > 0) & (j < ly - 1)).astype(np.int32)
+        d["wall_g"] = np.where(j == ly - 1, 2, 0).astype(np.int32)
+        d["heat_edge"] = ((j == 0) & (i < self.L)).astype(np.int32)
+        d["flux_bc"] = (((j == 0) | (i == lx - 1)) & ~((i == 0) & (j > 0))).astype(np.int32)
+        d["qn"] = np.where((j == 0) & (i < self.L), self.qn, 0.0)
+        d["outlet_type"] = np.where(d["outlet"] == 1, 2, 0).astype(np.int32)
+        return d
+
+    def run(self, b):
+        lx, ly, N = self.lx, self.ly, self.lx * self.ly
+        d = self.setup()
+        pf = b.lattice(D2Q9, lx, ly)
+        pg = b.lattice(D2Q9, lx, ly)
+        for k in ("alpha", "dads", "diff", "dkds"):
+            b.put(k, d[k])
+        b.put("rho", np.ones(N))
+        for k in ("ux", "uy", "uxp", "uyp", "tem", "qx", "qy", "qxp", "qyp", "ip", "iux", "iuy", "iuxp", "iuyp",
+                  "imx", "imy", "item", "iqx", "iqy", "iqxp", "iqyp", "dfds"):
+            b.zeros(k, N)
+        b.zeros("g", (NC[D2Q9], N))
+        b.zeros("ig", (NC[D2Q9], N))
+        zero = np.zeros(N)
+        b.ns_init(pf, "rho", "ux", "uy")
+        b.ad_init(pg, "tem", "ux", "uy")
+        res = []
+        for t in range(1, self.nt + 1):
+            b.ad_collide_fc(pf, "rho", "ux", "uy", None, "alpha", self.nu, pg, "tem", "qx", "qy", None, "diff",
+                            True, "g")
+            if self.check and t % self.check == 0:
+                res.append((b.residual(["ux", "uy"], ["uxp", "uyp"]), b.residual(["qx", "qy"], ["qxp", "qyp"])))
+            b.stream(pf)
+            b.stream(pg)
+            b.bc(pf, d["wall_f"])
+            b.ns_set_u(pf, d["inlet"], d["uin"], zero)
+            b.ns_set_rho(pf, d["outlet"], np.ones(N), zero)
+            b.smooth(pf)
+            b.bc(pg, d["wall_g"])
+            b.ad_set_t(pg, d["inlet"], zero, "ux", "uy")
+            b.ad_set_q(pg, d["flux_bc"], d["qn"], "ux", "uy", None, "diff")
+            b.smooth(pg)
+            b.swap("ux", "uxp")
+            b.swap("uy", "uyp")
+            b.swap("qx", "qxp")
+            b.swap("qy", "qyp")
+        # after the final swap the step-nt fields sit in the "p" buffers (heatsink.cpp:170-173)
+        b.swap("ux", "uxp")
+        b.swap("uy", "uyp")
+        b.swap("qx", "qxp")
+        b.swap("qy", "qyp")
+        out = {"f": b.pops(pf), "g": b.pops(pg)}
+        for k in ("rho", "ux", "uy", "tem", "qx", "qy", "g"):
+            out["fwd_" + k] = b.get(k)
+        b.ans_init(pf, "ux", "uy", None, "ip", "iux", "iuy", None)
+        b.aad_init(pg, "ux", "uy", None, "item", "iqx", "iqy", None)
+        for t in range(1, self.nt + 1):
+            b.aad_collide_fc(pf, "rho", "ux", "uy", None, "ip", "iux", "iuy", None, "imx", "imy", None, "alpha",
+                             self.nu, pg, "tem", "item", "iqx", "iqy", None, "diff", True, "ig")
+            if self.check and t % self.check == 0:
+                res.append((b.residual(["iux", "iuy"], ["iuxp", "iuyp"]), b.residual(["iqx", "iqy"], ["iqxp", "iqyp"])))
+            b.istream(pf)
+            b.istream(pg)
+            b.aad_iset_t(pg, d["inlet"], "ux", "uy")
+            b.aad_iset_q(pg, d["flux_bc"], "ux", "uy")
+            b.aad_iset_q(pg, d["heat_edge"], "ux", "uy", None, 1.0)
+            b.bc(pg, d["wall_g"], adjoint=True)
+            b.smooth(pg)
+            b.bc(pf, d["wall_f"], adjoint=True)
+            b.ans_iset_u(pf, d["inlet"], d["uin"], zero)
+            b.aad_iset_rho(pf, pg, d["outlet_type"], "rho", "ux", "uy", None, "tem")
+            b.smooth(pf)
+            b.swap("iux", "iuxp")
+            b.swap("iuy", "iuyp")
+            b.swap("iqx", "iqxp")
+            b.swap("iqy", "iqyp")
+        b.swap("iux", "iuxp")
+        b.swap("iuy", "iuyp")
+        b.swap("iqx", "iqxp")
+        b.swap("iqy", "iqyp")
+        b.sens_temp_heat_source(pg, "dfds", "ux", "uy", None, "imx", "imy", None, "dads", "tem", "item", "iqx", "iqy",
+                                None, "g", "ig", "diff", "dkds", d["heat_edge"], d["qn"])
+        out.update({"adj_f": b.pops(pf), "adj_g": b.pops(pg), "dfds": b.get("dfds")})
+        for k in ("ip", "iux", "iuy", "imx", "imy", "item", "iqx", "iqy", "ig"):
+            out["adj_" + k] = b.get(k)
+        tem = out["fwd_tem"]
+        out["objective"] = np.array(sum(tem[i] for i in range(lx) if i < self.L) / self.L)
+        if res:
+            out["residuals"] = np.array(res)
+        return out
+
+
+SEQUENCES = [
+    Cavity("cavity_21x21", 21, 21, 300),
+    ForcedConvection2D("fc2d_24x16", 24, 16, 200, check=100),
+]
