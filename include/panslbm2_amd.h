@@ -1,0 +1,228 @@
+/*
+ * include/panslbm2_amd.h -- C ABI of the MI355X lattice Boltzmann hot path (libpanslbm2_hip.so).
+ *
+ * The reference (PANSLBM2) exposes this path as C++ templates, not an FFI: a lattice class
+ * (src/particle/d2q9.h:24-158, d3q15.h:24-249) and free functions in namespaces NS, AD, ANS,
+ * AAD (the headers under src/equation/) that take the lattice, raw field arrays and host lambdas.  Each entry
+ * point below replaces one of those functions and names it (file:line).  Conventions:
+ *
+ *  - Every call returns 0 (LBM_OK) or a negative LBM_ERR_*; lbm_last_error() returns a
+ *    thread-local message.  The reference has no error returns (asserts only, d2q9.h:37).
+ *  - All work is ordered on the context's single HIP stream; calls return when the work is
+ *    queued unless they return host data (residual, downloads) or say otherwise.
+ *  - Field arrays (rho, ux, ..., alpha, diffusivity, snapshots) are DEVICE pointers of N doubles
+ *    (N = lx*ly*lz, node idx = i + lx*j + lx*ly*k as in d2q9.h:80-84); allocate them with
+ *    lbm_device_alloc.  Population snapshots g/ig (the reference's `T *g`, layout nc*idx + c,
+ *    advection.h:844-850) are nc*N doubles here in plane order: population c of node idx at
+ *    [c*N + idx].
+ *  - Boundary-condition calls take the reference's lambdas tabulated on the domain faces: HOST
+ *    arrays with one entry per face node, faces in the reference's call order xmin, xmax, ymin,
+ *    ymax[, zmin, zmax] (navierstokes.h:576-592).  2D: x-faces hold ly entries (index j),
+ *    y-faces lx entries (index i).  3D: x-faces ly*lz (j + ly*k), y-faces lx*lz (i + lx*k),
+ *    z-faces lx*ly (i + lx*j).  lbm_lattice_nface() gives the total length.  A zero mask/type
+ *    entry skips that node, like a lambda returning 0/false.
+ *  - Populations are kept on the device in plane order (population c of node idx at c*N + idx).
+ *    lbm_lattice_upload/download move them in that order; the *_aos variants use the
+ *    reference's f0[N] + f[(nc-1)*idx + c-1] layout (d2q9.h:88-90).
+ *
+ * Recording: between lbm_record_begin and lbm_record_end, lbm_stream/lbm_istream, the boundary
+ * calls and lbm_smooth_corner do not run; they are recorded, in order, into a program.  Passing
+ * that program to a *_collide_p entry point runs "stream + boundary calls + corner smoothing +
+ * collide" as one pass over memory.  Field pointers named in recorded calls are kept by
+ * reference in up to 4 field sets (set k = the k-th distinct pointer tuple recorded) and can be
+ * rebound between runs with lbm_program_bind_fields, which is how a driver that swaps its
+ * velocity arrays each step (production/heatsink.cpp:170-173) reuses one program.
+ */
+#ifndef PANSLBM2_AMD_H
+#define PANSLBM2_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBM_OK 0
+#define LBM_ERR_ARG (-1)
+#define LBM_ERR_HIP (-2)
+#define LBM_ERR_STATE (-3)
+#define LBM_ERR_UNSUPPORTED (-4)
+
+/* lattice kinds: the reference's D2Q9 (d2q9.h:160-163) and D3Q15 (d3q15.h:251-254), plus the
+ * advection-diffusion lattices D2Q5 and D3Q7 (c_s^2 = 1/3), which the reference lacks */
+#define LBM_D2Q9 0
+#define LBM_D3Q15 1
+#define LBM_D2Q5 2
+#define LBM_D3Q7 3
+
+/* boundary types of lbm_bc: d2q9.h:19-22 */
+#define LBM_BARRIER 1
+#define LBM_MIRROR 2
+/* boundary types of lbm_aad_iset_rho: adjointadvection.h:15-18 */
+#define LBM_SET_T 1
+#define LBM_SET_Q 2
+
+typedef struct lbm_context lbm_context;
+typedef struct lbm_lattice lbm_lattice;
+typedef struct lbm_program lbm_program;
+
+const char *lbm_last_error(void);
+int lbm_version(void);
+
+/* ---- context, device memory (replaces the reference's new[]/_mm_malloc, d2q9.h:38-46) ---- */
+int lbm_context_create(int device, lbm_context **out);
+int lbm_context_destroy(lbm_context *ctx);
+int lbm_context_synchronize(lbm_context *ctx);
+int lbm_device_alloc(lbm_context *ctx, size_t bytes, void **out);
+int lbm_device_free(lbm_context *ctx, void *ptr);
+int lbm_copy_to_device(lbm_context *ctx, void *dst, const void *src, size_t bytes);
+int lbm_copy_to_host(lbm_context *ctx, void *dst, const void *src, size_t bytes);
+int lbm_device_fill(lbm_context *ctx, double *dst, double value, size_t count);
+int lbm_device_copy(lbm_context *ctx, void *dst, const void *src, size_t bytes);
+/* HIP events on the context stream, for timing: event 0..7 */
+int lbm_event_record(lbm_context *ctx, int event);
+int lbm_event_elapsed_ms(lbm_context *ctx, int event_start, int event_stop, float *ms);
+
+/* ---- lattice: D2Q9<T>(lx, ly) / D3Q15<T>(lx, ly, lz), d2q9.h:28-58, d3q15.h:28-86 ---- */
+int lbm_lattice_create(lbm_context *ctx, int kind, int lx, int ly, int lz, lbm_lattice **out);
+int lbm_lattice_destroy(lbm_lattice *lat);
+int lbm_lattice_nc(const lbm_lattice *lat);
+int lbm_lattice_nxyz(const lbm_lattice *lat);
+int lbm_lattice_nface(const lbm_lattice *lat);
+int lbm_lattice_upload(lbm_lattice *lat, const double *soa_host);
+int lbm_lattice_download(lbm_lattice *lat, double *soa_host);
+int lbm_lattice_upload_aos(lbm_lattice *lat, const double *f0_host, const double *f_host);
+int lbm_lattice_download_aos(lbm_lattice *lat, double *f0_host, double *f_host);
+/* device pointer of the current population buffer (nc*N doubles, plane order) */
+double *lbm_lattice_populations(lbm_lattice *lat);
+
+/* ---- streaming and lattice boundaries ---- */
+int lbm_stream(lbm_lattice *lat);                                    /* d2q9.h:165-295, d3q15.h:256-617 */
+int lbm_istream(lbm_lattice *lat);                                   /* d2q9.h:297-427, d3q15.h:619-980 */
+int lbm_bc(lbm_lattice *lat, const int *type_face);                  /* BoundaryCondition, d2q9.h:113-119 */
+int lbm_ibc(lbm_lattice *lat, const int *type_face);                 /* iBoundaryCondition, d2q9.h:120-126 */
+int lbm_smooth_corner(lbm_lattice *lat);                             /* SmoothCorner, d2q9.h:127-132 */
+
+/* ---- NS (src/equation/navierstokes.h) ---- */
+int lbm_ns_initial_condition(lbm_lattice *p, const double *rho, const double *ux, const double *uy,
+                             const double *uz);                      /* :550-572 */
+int lbm_ns_macro_collide(lbm_lattice *p, double *rho, double *ux, double *uy, double *uz, double viscosity,
+                         int issave);                                /* :432-483 */
+int lbm_ns_macro_brinkman_collide(lbm_lattice *p, double *rho, double *ux, double *uy, double *uz,
+                                  double viscosity, const double *alpha, int issave); /* :487-546 */
+int lbm_ns_set_u(lbm_lattice *p, const int *mask_face, const double *ux_face, const double *uy_face,
+                 const double *uz_face);                             /* BoundaryConditionSetU :576-592 */
+int lbm_ns_set_rho(lbm_lattice *p, const int *mask_face, const double *rho_face, const double *us_face,
+                   const double *ut_face);                           /* BoundaryConditionSetRho :596-612 */
+
+/* ---- AD (src/equation/advection.h) ---- */
+int lbm_ad_initial_condition(lbm_lattice *q, const double *tem, const double *ux, const double *uy,
+                             const double *uz);                      /* :1048-1070 */
+/* MacroBrinkmanCollideForceConvection :814-926.  heat (may be NULL) adds w_c*heat to g after the
+ * Brinkman force, a volumetric source with no reference counterpart. */
+int lbm_ad_macro_brinkman_collide_force_convection(lbm_lattice *p, double *rho, double *ux, double *uy, double *uz,
+                                                   const double *alpha, double viscosity, lbm_lattice *q, double *tem,
+                                                   double *qx, double *qy, double *qz, const double *diffusivity,
+                                                   int issave, double *g, const double *heat);
+int lbm_ad_set_t(lbm_lattice *q, const int *mask_face, const double *tem_face, const double *ux, const double *uy,
+                 const double *uz);                                  /* BoundaryConditionSetT :1074-1090 */
+/* BoundaryConditionSetQ :1094-1130; diffusivity NULL selects the constant-D overload */
+int lbm_ad_set_q(lbm_lattice *q, const int *mask_face, const double *qn_face, const double *ux, const double *uy,
+                 const double *uz, const double *diffusivity, double diffusivity_const);
+
+/* ---- ANS (src/equation/adjointnavierstokes.h) ---- */
+int lbm_ans_initial_condition(lbm_lattice *p, const double *ux, const double *uy, const double *uz, const double *ip,
+                              const double *iux, const double *iuy, const double *iuz); /* :474-498 */
+int lbm_ans_macro_brinkman_collide(lbm_lattice *p, const double *rho, const double *ux, const double *uy,
+                                   const double *uz, double *ip, double *iux, double *iuy, double *iuz, double *imx,
+                                   double *imy, double *imz, double viscosity, const double *alpha,
+                                   int issave);                      /* :398-470 */
+/* iBoundaryConditionSetU :502-518.  compat = 1 keeps the reference's y-edge formula, which uses
+ * ux where the x-edge formula uses the normal velocity (:130, :135); 0 uses uy there. */
+int lbm_ans_iset_u(lbm_lattice *p, const int *mask_face, const double *ux_face, const double *uy_face,
+                   const double *uz_face, double eps, int compat);
+int lbm_ans_iset_rho(lbm_lattice *p, const int *mask_face);          /* iBoundaryConditionSetRho2D/3D :522-538 */
+
+/* ---- AAD (src/equation/adjointadvection.h) ---- */
+int lbm_aad_initial_condition(lbm_lattice *q, const double *ux, const double *uy, const double *uz,
+                              const double *item, const double *iqx, const double *iqy,
+                              const double *iqz);                    /* :1359-1381 */
+int lbm_aad_macro_brinkman_collide_force_convection(
+    lbm_lattice *p, const double *rho, const double *ux, const double *uy, const double *uz, double *ip, double *iux,
+    double *iuy, double *iuz, double *imx, double *imy, double *imz, const double *alpha, double viscosity,
+    lbm_lattice *q, const double *tem, double *item, double *iqx, double *iqy, double *iqz, const double *diffusivity,
+    int issave, double *ig);                                         /* :997-1109 */
+int lbm_aad_iset_t(lbm_lattice *q, const int *mask_face, const double *ux, const double *uy,
+                   const double *uz);                                /* iBoundaryConditionSetT :1385-1401 */
+int lbm_aad_iset_q(lbm_lattice *q, const int *mask_face, const double *ux, const double *uy, const double *uz,
+                   double eps);                                      /* iBoundaryConditionSetQ :1405-1421 */
+/* iBoundaryConditionSetRho :1425-1441; type_face entries LBM_SET_T / LBM_SET_Q / 0 */
+int lbm_aad_iset_rho(lbm_lattice *p, lbm_lattice *q, const int *type_face, const double *rho, const double *ux,
+                     const double *uy, const double *uz, const double *tem, double eps);
+
+/* ---- design sensitivities (accumulate += into dfds, like the reference) ---- */
+int lbm_sensitivity_brinkman(lbm_lattice *p, double *dfds, const double *ux, const double *uy, const double *uz,
+                             const double *imx, const double *imy, const double *imz,
+                             const double *dads);                    /* adjointnavierstokes.h:542-554 */
+int lbm_sensitivity_brinkman_diffusivity(lbm_lattice *q, double *dfds, const double *ux, const double *uy,
+                                         const double *uz, const double *imx, const double *imy, const double *imz,
+                                         const double *dads, const double *tem, const double *item, const double *iqx,
+                                         const double *iqy, const double *iqz, const double *g, const double *ig,
+                                         const double *diffusivity, const double *dkds); /* adjointadvection.h:1461-1493 */
+/* adjointadvection.h:1497-1517 (2D); mask_face / qn_face tabulate the reference's bctype / qnbc lambdas */
+int lbm_sensitivity_temperature_at_heat_source(lbm_lattice *q, double *dfds, const double *ux, const double *uy,
+                                               const double *uz, const double *imx, const double *imy,
+                                               const double *imz, const double *dads, const double *tem,
+                                               const double *item, const double *iqx, const double *iqy,
+                                               const double *iqz, const double *g, const double *ig,
+                                               const double *diffusivity, const double *dkds, const int *mask_face,
+                                               const double *qn_face);
+
+/* ---- reductions (FP64, deterministic fixed-order trees) ---- */
+/* Residual, residual.h:9-51: sqrt(sum|a-b|^2 / sum|a|^2) over ncomp (1..3) components */
+int lbm_residual(lbm_context *ctx, int ncomp, const double *a0, const double *a1, const double *a2, const double *b0,
+                 const double *b1, const double *b2, int n, double *out_host);
+int lbm_normalize(lbm_context *ctx, double *v, int n);              /* Normalize, normalize.h:9-24 */
+/* sum of v[idx[t]] for t < count (host index list), used for boundary-mean objectives
+ * (production/heatsink.cpp:215-226) */
+int lbm_gather_sum(lbm_context *ctx, const double *v, const int *idx_host, int count, double *out_host);
+/* DensityFilter::GetFilteredValue with the default hat weight, densityfilter.h:390-420,512-515 */
+int lbm_density_filter(lbm_context *ctx, int lx, int ly, int lz, double R, const double *v, double *out);
+
+/* ---- recording and fused steps ---- */
+int lbm_record_begin(lbm_context *ctx);
+int lbm_record_end(lbm_context *ctx, lbm_program **out);
+int lbm_program_destroy(lbm_program *prog);
+/* run the recorded calls on their own, exactly as if they had been made (no collide) */
+int lbm_program_apply(lbm_program *prog);
+/* rebind field set k (0..3) of a program; NULL keeps a pointer */
+int lbm_program_bind_fields(lbm_program *prog, int set, const double *rho, const double *ux, const double *uy,
+                            const double *uz, const double *tem, const double *diffusivity);
+int lbm_program_num_field_sets(const lbm_program *prog);
+/* whether a program runs fused with a collide (1) or falls back to apply + collide (0) */
+int lbm_program_fusable(const lbm_program *prog);
+
+/* "program, then collide" in one pass; prog may be NULL (plain collide) */
+int lbm_ns_macro_collide_p(lbm_program *prog, lbm_lattice *p, double *rho, double *ux, double *uy, double *uz,
+                           double viscosity, int issave);
+int lbm_ns_macro_brinkman_collide_p(lbm_program *prog, lbm_lattice *p, double *rho, double *ux, double *uy,
+                                    double *uz, double viscosity, const double *alpha, int issave);
+int lbm_ad_macro_brinkman_collide_force_convection_p(lbm_program *prog, lbm_lattice *p, double *rho, double *ux,
+                                                     double *uy, double *uz, const double *alpha, double viscosity,
+                                                     lbm_lattice *q, double *tem, double *qx, double *qy, double *qz,
+                                                     const double *diffusivity, int issave, double *g,
+                                                     const double *heat);
+int lbm_ans_macro_brinkman_collide_p(lbm_program *prog, lbm_lattice *p, const double *rho, const double *ux,
+                                     const double *uy, const double *uz, double *ip, double *iux, double *iuy,
+                                     double *iuz, double *imx, double *imy, double *imz, double viscosity,
+                                     const double *alpha, int issave);
+int lbm_aad_macro_brinkman_collide_force_convection_p(
+    lbm_program *prog, lbm_lattice *p, const double *rho, const double *ux, const double *uy, const double *uz,
+    double *ip, double *iux, double *iuy, double *iuz, double *imx, double *imy, double *imz, const double *alpha,
+    double viscosity, lbm_lattice *q, const double *tem, double *item, double *iqx, double *iqy, double *iqz,
+    const double *diffusivity, int issave, double *ig);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1061 @@
+// panslbm2_amd/csrc/capi.hip -- host implementation of include/panslbm2_amd.h.
+//
+// Owns contexts (device + one stream), lattices (two population buffers, plane order), and
+// boundary programs.  Every reference call maps to one entry point; boundary calls build a
+// program (per-node operator lists in call order) that either runs immediately (eager) or is
+// recorded for a fused "stream + boundaries + collide" pass.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/panslbm2_amd.h"
+#include "device.h"
+
+using lbm::FieldSet;
+using lbm::Grid;
+using lbm::Op;
+using lbm::ProgView;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+#define HIPCHK(expr)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess) return fail(LBM_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_));       \
+    } while (0)
+
+int kind_nc(int kind) {
+    switch (kind) {
+    case LBM_D2Q9: return 9;
+    case LBM_D3Q15: return 15;
+    case LBM_D2Q5: return 5;
+    case LBM_D3Q7: return 7;
+    }
+    return 0;
+}
+int kind_nd(int kind) { return (kind == LBM_D3Q15 || kind == LBM_D3Q7) ? 3 : 2; }
+bool is_ns_kind(int kind) { return kind == LBM_D2Q9 || kind == LBM_D3Q15; }
+
+// grow-only device buffer
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t bytes) {
+        if (bytes <= cap) return 0;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        if (hipMalloc(&p, bytes) != hipSuccess) return -1;
+        cap = bytes;
+        return 0;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+}  // namespace
+
+struct lbm_program;
+
+struct lbm_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[8] = {};
+    lbm_program *recording = nullptr;
+    double *d_part = nullptr;  // reduction partials
+    double *d_out = nullptr;   // reduction results
+    double *h_out = nullptr;   // pinned
+    DevBuf scratch;            // index lists for gather / filter weights
+};
+
+struct lbm_lattice {
+    lbm_context *ctx;
+    int kind, nc, nd, lx, ly, lz, N;
+    double *buf[2] = {nullptr, nullptr};
+    int cur = 0;
+    // smoothing lists for the eager path (in-place kernels): edges (3D) then corners
+    std::vector<int4> sm_edge, sm_corner;
+    int4 *d_sm_edge = nullptr, *d_sm_corner = nullptr;
+    Grid grid() const { return Grid{lx, ly, lz, N}; }
+};
+
+struct HostOp {
+    int node;
+    Op op;
+};
+
+struct lbm_program {
+    lbm_context *ctx = nullptr;
+    lbm_lattice *lat[2] = {nullptr, nullptr};
+    int nlat = 0;
+    int pull[2] = {0, 0};
+    int smoothed[2] = {0, 0};
+    int smooth_seg[2] = {-1, -1};
+    std::vector<int> seg_closer;  // lattice slot whose SmoothCorner closed segment k
+    std::vector<HostOp> ops;
+    std::vector<FieldSet> fsets;
+    int seg = 0;
+    bool ordered_ok = true;  // no operator on a lattice after its SmoothCorner
+    // device tables
+    bool uploaded = false;
+    DevBuf d_slot_entry, d_entry_node, d_entry_ops, d_ops, d_fix_corner, d_fix_edge;
+    int nentry = 0, nslot = 0, n_fix_corner = 0, n_fix_edge = 0;
+    bool owned_by_ctx = false;  // eager scratch program
+    ~lbm_program() {
+        d_slot_entry.release();
+        d_entry_node.release();
+        d_entry_ops.release();
+        d_ops.release();
+        d_fix_corner.release();
+        d_fix_edge.release();
+    }
+};
+
+namespace {
+
+// ---------------------------------------------------------------------------------------------
+// face layout (include/panslbm2_amd.h): node of face-array entry t on face `face`
+// ---------------------------------------------------------------------------------------------
+int face_len(const lbm_lattice *L, int face) {
+    int ax = face / 2;
+    if (L->nd == 2) return ax == 0 ? L->ly : L->lx;
+    return ax == 0 ? L->ly * L->lz : (ax == 1 ? L->lx * L->lz : L->lx * L->ly);
+}
+int face_node(const lbm_lattice *L, int face, int t) {
+    int ax = face / 2, hi = face % 2, i, j, k = 0;
+    if (L->nd == 2) {
+        if (ax == 0) { i = hi ? L->lx - 1 : 0; j = t; }
+        else { j = hi ? L->ly - 1 : 0; i = t; }
+    } else {
+        if (ax == 0) { i = hi ? L->lx - 1 : 0; j = t % L->ly; k = t / L->ly; }
+        else if (ax == 1) { j = hi ? L->ly - 1 : 0; i = t % L->lx; k = t / L->lx; }
+        else { k = hi ? L->lz - 1 : 0; i = t % L->lx; j = t / L->lx; }
+    }
+    return i + L->lx * (j + L->ly * k);
+}
+int nface_total(const lbm_lattice *L) {
+    int s = 0;
+    for (int f = 0; f < 2 * L->nd; ++f) s += face_len(L, f);
+    return s;
+}
+void node_coords(const lbm_lattice *L, int n, int &i, int &j, int &k) {
+    i = n % L->lx;
+    j = (n / L->lx) % L->ly;
+    k = n / (L->lx * L->ly);
+}
+int nboundary(const lbm_lattice *L, int n) {
+    int i, j, k;
+    node_coords(L, n, i, j, k);
+    int c = (i == 0 || i == L->lx - 1) + (j == 0 || j == L->ly - 1);
+    if (L->nd == 3) c += (k == 0 || k == L->lz - 1);
+    return c;
+}
+
+// Smoothed nodes and their inward neighbours (axis order), d2q9.h:127-132, d3q15.h:199-220.
+// 3D edge lists exclude the corners, which the reference overwrites afterwards anyway.
+void build_smooth_lists(lbm_lattice *L) {
+    L->sm_edge.clear();
+    L->sm_corner.clear();
+    int lx = L->lx, ly = L->ly, lz = L->lz;
+    auto id = [&](int i, int j, int k) { return i + lx * (j + ly * k); };
+    for (int k = 0; k < (L->nd == 3 ? lz : 1); ++k)
+        for (int j = 0; j < ly; ++j)
+            for (int i = 0; i < lx; ++i) {
+                int n = id(i, j, k);
+                int nb = nboundary(L, n);
+                if (nb < 2) continue;
+                int ci[3] = {i, j, k}, cn[3] = {lx, ly, lz}, nbr[3] = {-1, -1, -1}, cnt = 0;
+                for (int ax = 0; ax < L->nd; ++ax) {
+                    int d = ci[ax] == 0 ? 1 : (ci[ax] == cn[ax] - 1 ? -1 : 0);
+                    if (!d) continue;
+                    int c2[3] = {i, j, k};
+                    c2[ax] += d;
+                    nbr[cnt++] = id(c2[0], c2[1], c2[2]);
+                }
+                int4 e = make_int4(n, nbr[0], nbr[1], cnt == 3 ? nbr[2] : -1);
+                if (L->nd == 3 && nb == 2) L->sm_edge.push_back(e);
+                else L->sm_corner.push_back(e);
+            }
+}
+
+int smooth_eager(lbm_lattice *L) {
+    if (L->lx < 3 || L->ly < 3 || (L->nd == 3 && L->lz < 3))
+        return fail(LBM_ERR_UNSUPPORTED, "SmoothCorner needs at least 3 nodes per axis");
+    hipStream_t s = L->ctx->stream;
+    HIPCHK(lbm::launch_smooth(L->kind, L->buf[L->cur], L->N, L->d_sm_edge, (int)L->sm_edge.size(), s));
+    HIPCHK(lbm::launch_smooth(L->kind, L->buf[L->cur], L->N, L->d_sm_corner, (int)L->sm_corner.size(), s));
+    return LBM_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// program construction
+// ---------------------------------------------------------------------------------------------
+int prog_slot(lbm_program *P, lbm_lattice *L) {
+    for (int s = 0; s < P->nlat; ++s)
+        if (P->lat[s] == L) return s;
+    if (P->nlat == 2) return fail(LBM_ERR_UNSUPPORTED, "a program spans at most two lattices");
+    if (P->nlat == 1) {
+        lbm_lattice *A = P->lat[0];
+        if (A->lx != L->lx || A->ly != L->ly || A->lz != L->lz || A->nd != L->nd)
+            return fail(LBM_ERR_ARG, "lattices of one program must have the same grid");
+    }
+    P->lat[P->nlat] = L;
+    return P->nlat++;
+}
+int prog_fset(lbm_program *P, const FieldSet &fs) {
+    for (size_t k = 0; k < P->fsets.size(); ++k)
+        if (!memcmp(&P->fsets[k], &fs, sizeof fs)) return (int)k;
+    if ((int)P->fsets.size() == lbm::kMaxFieldSets)
+        return fail(LBM_ERR_UNSUPPORTED, "more than %d distinct field sets in one program", lbm::kMaxFieldSets);
+    P->fsets.push_back(fs);
+    return (int)P->fsets.size() - 1;
+}
+
+// Record one boundary call: for every face node whose selector is nonzero, append an operator.
+// vals(t, op) fills the operator's values for face entry t.
+template <class Vals>
+int prog_record(lbm_program *P, lbm_lattice *L, uint8_t code, const int *sel, const FieldSet *fs, Vals vals) {
+    if (!sel) return fail(LBM_ERR_ARG, "null face selector array");
+    int slot = prog_slot(P, L);
+    if (slot < 0) return slot;
+    int fset = 0;
+    if (fs) {
+        fset = prog_fset(P, *fs);
+        if (fset < 0) return fset;
+    }
+    if (P->smoothed[slot] && code != lbm::OP_SMOOTH) P->ordered_ok = false;
+    int off = 0;
+    for (int face = 0; face < 2 * L->nd; ++face) {
+        int n = face_len(L, face);
+        for (int t = 0; t < n; ++t) {
+            int s = sel[off + t];
+            if (!s) continue;
+            HostOp h{};
+            h.node = face_node(L, face, t);
+            h.op.code = code;
+            h.op.face = (uint8_t)face;
+            h.op.lat = (uint8_t)slot;
+            h.op.arg = (uint8_t)s;
+            h.op.fset = (uint8_t)fset;
+            h.op.seg = (uint8_t)P->seg;
+            vals(off + t, h.op);
+            P->ops.push_back(h);
+        }
+        off += n;
+    }
+    P->uploaded = false;
+    return LBM_OK;
+}
+
+int prog_record_smooth(lbm_program *P, lbm_lattice *L) {
+    if (L->lx < 3 || L->ly < 3 || (L->nd == 3 && L->lz < 3))
+        return fail(LBM_ERR_UNSUPPORTED, "SmoothCorner needs at least 3 nodes per axis");
+    int slot = prog_slot(P, L);
+    if (slot < 0) return slot;
+    for (const int4 &e : L->sm_edge) {
+        HostOp h{};
+        h.node = e.x;
+        h.op.code = lbm::OP_SMOOTH;
+        h.op.lat = (uint8_t)slot;
+        h.op.seg = (uint8_t)P->seg;
+        P->ops.push_back(h);
+    }
+    for (const int4 &e : L->sm_corner) {
+        HostOp h{};
+        h.node = e.x;
+        h.op.code = lbm::OP_SMOOTH;
+        h.op.lat = (uint8_t)slot;
+        h.op.seg = (uint8_t)P->seg;
+        P->ops.push_back(h);
+    }
+    P->smoothed[slot] = 1;
+    P->smooth_seg[slot] = P->seg;
+    P->seg_closer.push_back(slot);
+    P->seg++;
+    if (P->seg > 250) return fail(LBM_ERR_UNSUPPORTED, "too many SmoothCorner calls in one program");
+    P->uploaded = false;
+    return LBM_OK;
+}
+
+int prog_upload(lbm_program *P) {
+    if (P->uploaded) return LBM_OK;
+    lbm_lattice *L = P->lat[0];
+    if (!L) return fail(LBM_ERR_STATE, "empty program");
+    // group operators by node, keeping record order within a node
+    std::map<int, std::vector<int>> by_node;
+    for (size_t t = 0; t < P->ops.size(); ++t) by_node[P->ops[t].node].push_back((int)t);
+    std::vector<int> entry_node;
+    std::vector<int2> entry_ops;
+    std::vector<Op> ops;
+    for (auto &kv : by_node) {
+        entry_node.push_back(kv.first);
+        entry_ops.push_back(make_int2((int)ops.size(), (int)kv.second.size()));
+        for (int t : kv.second) ops.push_back(P->ops[t].op);
+    }
+    P->nentry = (int)entry_node.size();
+    P->nslot = lbm::boundary_slot_count(L->lx, L->ly, L->lz, L->nd);
+    std::vector<int> slot_entry(P->nslot, -1);
+    for (int e = 0; e < P->nentry; ++e) {
+        int i, j, k;
+        node_coords(L, entry_node[e], i, j, k);
+        int s = lbm::boundary_slot(i, j, k, L->lx, L->ly, L->lz, L->nd);
+        if (s < 0) return fail(LBM_ERR_STATE, "boundary operator on an interior node");
+        slot_entry[s] = e;
+    }
+    std::vector<int> fix_corner, fix_edge;
+    if (P->smoothed[0] || P->smoothed[1]) {
+        for (const int4 &e : L->sm_corner) fix_corner.push_back(e.x);
+        for (const int4 &e : L->sm_edge) fix_edge.push_back(e.x);
+    }
+    P->n_fix_corner = (int)fix_corner.size();
+    P->n_fix_edge = (int)fix_edge.size();
+    auto up = [&](DevBuf &b, const void *src, size_t bytes) -> int {
+        if (b.ensure(bytes ? bytes : 8)) return fail(LBM_ERR_HIP, "device allocation of %zu bytes failed", bytes);
+        if (bytes) HIPCHK(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
+        return LBM_OK;
+    };
+    int rc;
+    if ((rc = up(P->d_slot_entry, slot_entry.data(), slot_entry.size() * sizeof(int)))) return rc;
+    if ((rc = up(P->d_entry_node, entry_node.data(), entry_node.size() * sizeof(int)))) return rc;
+    if ((rc = up(P->d_entry_ops, entry_ops.data(), entry_ops.size() * sizeof(int2)))) return rc;
+    if ((rc = up(P->d_ops, ops.data(), ops.size() * sizeof(Op)))) return rc;
+    if ((rc = up(P->d_fix_corner, fix_corner.data(), fix_corner.size() * sizeof(int)))) return rc;
+    if ((rc = up(P->d_fix_edge, fix_edge.data(), fix_edge.size() * sizeof(int)))) return rc;
+    P->uploaded = true;
+    return LBM_OK;
+}
+
+ProgView prog_view(const lbm_program *P) {
+    ProgView v{};
+    v.slot_entry = (const int *)P->d_slot_entry.p;
+    v.entry_node = (const int *)P->d_entry_node.p;
+    v.entry_ops = (const int2 *)P->d_entry_ops.p;
+    v.ops = (const Op *)P->d_ops.p;
+    v.nentry = P->nentry;
+    v.nslot = P->nslot;
+    for (size_t k = 0; k < P->fsets.size() && k < (size_t)lbm::kMaxFieldSets; ++k) v.fs[k] = P->fsets[k];
+    return v;
+}
+
+void swap_buf(lbm_lattice *L) { L->cur ^= 1; }
+
+// Exact, unfused execution: streams, then per segment the in-place operator pass followed by the
+// SmoothCorner that closed the segment.
+int prog_apply(lbm_program *P) {
+    int rc = prog_upload(P);
+    if (rc) return rc;
+    hipStream_t s = P->ctx->stream;
+    for (int k = 0; k < P->nlat; ++k) {
+        lbm_lattice *L = P->lat[k];
+        if (P->pull[k]) {
+            HIPCHK(lbm::launch_stream(L->kind, L->buf[L->cur], L->buf[L->cur ^ 1], L->grid(), P->pull[k], s));
+            swap_buf(L);
+        }
+    }
+    lbm_lattice *A = P->lat[0], *B = P->nlat > 1 ? P->lat[1] : nullptr;
+    ProgView v = prog_view(P);
+    for (int seg = 0; seg <= P->seg; ++seg) {
+        HIPCHK(lbm::launch_program_apply(A->kind, B ? B->kind : -1, A->buf[A->cur], B ? B->buf[B->cur] : nullptr, A->N,
+                                         v, seg, s));
+        if (seg < (int)P->seg_closer.size()) {
+            rc = smooth_eager(P->lat[P->seg_closer[seg]]);
+            if (rc) return rc;
+        }
+    }
+    return LBM_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// eager boundary calls: record into the context's scratch program (or the active recording)
+// ---------------------------------------------------------------------------------------------
+template <class Rec>
+int boundary_call(lbm_lattice *L, Rec rec) {
+    if (!L) return fail(LBM_ERR_ARG, "null lattice");
+    lbm_context *C = L->ctx;
+    if (C->recording) return rec(C->recording);
+    lbm_program P;
+    P.ctx = C;
+    int rc = rec(&P);
+    if (rc) return rc;
+    if (P.ops.empty()) return LBM_OK;
+    rc = prog_apply(&P);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(C->stream));  // P's device tables are freed on return
+    return LBM_OK;
+}
+
+bool fields_ok(std::initializer_list<const void *> ps) {
+    for (const void *p : ps)
+        if (!p) return false;
+    return true;
+}
+
+// ---------------------------------------------------------------------------------------------
+// collide dispatch: plain, fused, or program-then-plain
+// ---------------------------------------------------------------------------------------------
+template <class Launch>
+int collide_call(lbm_program *P, lbm_lattice *p, lbm_lattice *q, Launch launch) {
+    if (!p) return fail(LBM_ERR_ARG, "null lattice");
+    if (q && (q->lx != p->lx || q->ly != p->ly || q->lz != p->lz))
+        return fail(LBM_ERR_ARG, "coupled lattices must have the same grid");
+    lbm_context *C = p->ctx;
+    if (C->recording) return fail(LBM_ERR_STATE, "collide called while recording a program");
+    lbm::CollideLaunch cl{};
+    cl.ka = p->kind;
+    cl.kb = q ? q->kind : -1;
+    cl.g = p->grid();
+    cl.stream = C->stream;
+    bool fused = false;
+    if (P && P->nlat == 0) P = nullptr;
+    if (P) {
+        bool subset = true;
+        for (int k = 0; k < P->nlat; ++k) subset &= (P->lat[k] == p || P->lat[k] == q);
+        bool dims = !(P->smoothed[0] || P->smoothed[1]) || (p->lx >= 3 && p->ly >= 3 && (p->nd == 2 || p->lz >= 3));
+        fused = subset && P->ordered_ok && dims && P->nlat > 0;
+        if (!fused) {
+            int rc = prog_apply(P);
+            if (rc) return rc;
+        }
+    }
+    int rc;
+    if (!fused) {
+        cl.da = p->buf[p->cur];
+        cl.db = q ? q->buf[q->cur] : nullptr;
+        cl.pv = nullptr;
+        rc = launch(cl);
+        return rc;
+    }
+    if ((rc = prog_upload(P))) return rc;
+    ProgView v = prog_view(P);
+    v.latswap = (P->lat[0] == q) ? 1 : 0;
+    v.pull[0] = v.pull[1] = 0;
+    for (int k = 0; k < P->nlat; ++k) {
+        if (P->lat[k] == p) v.pull[0] = P->pull[k];
+        else v.pull[1] = P->pull[k];
+    }
+    cl.sa = p->buf[p->cur];
+    cl.da = p->buf[p->cur ^ 1];
+    cl.sb = q ? q->buf[q->cur] : nullptr;
+    cl.db = q ? q->buf[q->cur ^ 1] : nullptr;
+    cl.pv = &v;
+    cl.fix_corner = (const int *)P->d_fix_corner.p;
+    cl.n_fix_corner = P->n_fix_corner;
+    cl.fix_edge = (const int *)P->d_fix_edge.p;
+    cl.n_fix_edge = P->n_fix_edge;
+    cl.any_smooth = (P->smoothed[0] || P->smoothed[1]) ? 1 : 0;
+    rc = launch(cl);
+    if (rc) return rc;
+    swap_buf(p);
+    if (q) swap_buf(q);
+    return LBM_OK;
+}
+
+int hip_result(hipError_t e, const char *what) {
+    if (e != hipSuccess) return fail(LBM_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+    return LBM_OK;
+}
+
+}  // namespace
+
+// =============================================================================================
+extern "C" {
+
+const char *lbm_last_error(void) { return g_err.c_str(); }
+int lbm_version(void) { return 1; }
+
+int lbm_context_create(int device, lbm_context **out) {
+    if (!out) return fail(LBM_ERR_ARG, "null out");
+    *out = nullptr;
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(LBM_ERR_ARG, "device %d not present (%d devices)", device, n);
+    HIPCHK(hipSetDevice(device));
+    lbm_context *C = new lbm_context;
+    C->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete C;
+        return fail(LBM_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    for (auto &ev : C->ev) HIPCHK(hipEventCreate(&ev));
+    HIPCHK(hipMalloc(&C->d_part, sizeof(double) * 2 * lbm::reduction_blocks()));
+    HIPCHK(hipMalloc(&C->d_out, sizeof(double) * 4));
+    HIPCHK(hipHostMalloc(&C->h_out, sizeof(double) * 4, hipHostMallocDefault));
+    *out = C;
+    return LBM_OK;
+}
+int lbm_context_destroy(lbm_context *C) {
+    if (!C) return LBM_OK;
+    (void)hipSetDevice(C->device);
+    (void)hipStreamSynchronize(C->stream);
+    for (auto &ev : C->ev) (void)hipEventDestroy(ev);
+    (void)hipFree(C->d_part);
+    (void)hipFree(C->d_out);
+    (void)hipHostFree(C->h_out);
+    C->scratch.release();
+    (void)hipStreamDestroy(C->stream);
+    delete C;
+    return LBM_OK;
+}
+int lbm_context_synchronize(lbm_context *C) {
+    if (!C) return fail(LBM_ERR_ARG, "null context");
+    HIPCHK(hipStreamSynchronize(C->stream));
+    return LBM_OK;
+}
+int lbm_device_alloc(lbm_context *C, size_t bytes, void **out) {
+    if (!C || !out) return fail(LBM_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(C->device));
+    HIPCHK(hipMalloc(out, bytes ? bytes : 8));
+    return LBM_OK;
+}
+int lbm_device_free(lbm_context *C, void *p) {
+    if (!C) return fail(LBM_ERR_ARG, "null context");
+    HIPCHK(hipStreamSynchronize(C->stream));
+    HIPCHK(hipFree(p));
+    return LBM_OK;
+}
+int lbm_copy_to_device(lbm_context *C, void *dst, const void *src, size_t bytes) {
+    if (!C) return fail(LBM_ERR_ARG, "null context");
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, C->stream));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    return LBM_OK;
+}
+int lbm_copy_to_host(lbm_context *C, void *dst, const void *src, size_t bytes) {
+    if (!C) return fail(LBM_ERR_ARG, "null context");
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, C->stream));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    return LBM_OK;
+}
+int lbm_device_fill(lbm_context *C, double *dst, double value, size_t count) {
+    if (!C) return fail(LBM_ERR_ARG, "null context");
+    return hip_result(lbm::launch_fill(dst, count, value, C->stream), "fill");
+}
+int lbm_device_copy(lbm_context *C, void *dst, const void *src, size_t bytes) {
+    if (!C) return fail(LBM_ERR_ARG, "null context");
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, C->stream));
+    return LBM_OK;
+}
+int lbm_event_record(lbm_context *C, int e) {
+    if (!C || e < 0 || e >= 8) return fail(LBM_ERR_ARG, "bad event");
+    HIPCHK(hipEventRecord(C->ev[e], C->stream));
+    return LBM_OK;
+}
+int lbm_event_elapsed_ms(lbm_context *C, int a, int b, float *ms) {
+    if (!C || a < 0 || a >= 8 || b < 0 || b >= 8 || !ms) return fail(LBM_ERR_ARG, "bad event");
+    HIPCHK(hipEventSynchronize(C->ev[b]));
+    HIPCHK(hipEventElapsedTime(ms, C->ev[a], C->ev[b]));
+    return LBM_OK;
+}
+
+// ---- lattice ------------------------------------------------------------------------------
+int lbm_lattice_create(lbm_context *C, int kind, int lx, int ly, int lz, lbm_lattice **out) {
+    if (!C || !out) return fail(LBM_ERR_ARG, "null argument");
+    *out = nullptr;
+    int nc = kind_nc(kind);
+    if (!nc) return fail(LBM_ERR_ARG, "unknown lattice kind %d", kind);
+    int nd = kind_nd(kind);
+    if (nd == 2) lz = 1;
+    if (lx < 1 || ly < 1 || lz < 1) return fail(LBM_ERR_ARG, "bad grid %dx%dx%d", lx, ly, lz);
+    long N = (long)lx * ly * lz;
+    if (N * nc > (1L << 31) - 1) return fail(LBM_ERR_UNSUPPORTED, "grid too large for 32-bit plane indexing");
+    lbm_lattice *L = new lbm_lattice;
+    L->ctx = C;
+    L->kind = kind;
+    L->nc = nc;
+    L->nd = nd;
+    L->lx = lx;
+    L->ly = ly;
+    L->lz = lz;
+    L->N = (int)N;
+    for (int b = 0; b < 2; ++b) {
+        hipError_t e = hipMalloc(&L->buf[b], sizeof(double) * nc * N);
+        if (e != hipSuccess) {
+            for (int c = 0; c < b; ++c) (void)hipFree(L->buf[c]);
+            delete L;
+            return fail(LBM_ERR_HIP, "hipMalloc populations: %s", hipGetErrorString(e));
+        }
+    }
+    HIPCHK(lbm::launch_fill(L->buf[0], (size_t)nc * N, 0.0, C->stream));
+    build_smooth_lists(L);
+    if (!L->sm_edge.empty()) {
+        HIPCHK(hipMalloc(&L->d_sm_edge, sizeof(int4) * L->sm_edge.size()));
+        HIPCHK(hipMemcpy(L->d_sm_edge, L->sm_edge.data(), sizeof(int4) * L->sm_edge.size(), hipMemcpyHostToDevice));
+    }
+    if (!L->sm_corner.empty()) {
+        HIPCHK(hipMalloc(&L->d_sm_corner, sizeof(int4) * L->sm_corner.size()));
+        HIPCHK(hipMemcpy(L->d_sm_corner, L->sm_corner.data(), sizeof(int4) * L->sm_corner.size(), hipMemcpyHostToDevice));
+    }
+    *out = L;
+    return LBM_OK;
+}
+int lbm_lattice_destroy(lbm_lattice *L) {
+    if (!L) return LBM_OK;
+    (void)hipStreamSynchronize(L->ctx->stream);
+    (void)hipFree(L->buf[0]);
+    (void)hipFree(L->buf[1]);
+    if (L->d_sm_edge) (void)hipFree(L->d_sm_edge);
+    if (L->d_sm_corner) (void)hipFree(L->d_sm_corner);
+    delete L;
+    return LBM_OK;
+}
+int lbm_lattice_nc(const lbm_lattice *L) { return L ? L->nc : fail(LBM_ERR_ARG, "null lattice"); }
+int lbm_lattice_nxyz(const lbm_lattice *L) { return L ? L->N : fail(LBM_ERR_ARG, "null lattice"); }
+int lbm_lattice_nface(const lbm_lattice *L) { return L ? nface_total(L) : fail(LBM_ERR_ARG, "null lattice"); }
+double *lbm_lattice_populations(lbm_lattice *L) { return L ? L->buf[L->cur] : nullptr; }
+int lbm_lattice_upload(lbm_lattice *L, const double *soa) {
+    if (!L || !soa) return fail(LBM_ERR_ARG, "null argument");
+    return lbm_copy_to_device(L->ctx, L->buf[L->cur], soa, sizeof(double) * L->nc * L->N);
+}
+int lbm_lattice_download(lbm_lattice *L, double *soa) {
+    if (!L || !soa) return fail(LBM_ERR_ARG, "null argument");
+    return lbm_copy_to_host(L->ctx, soa, L->buf[L->cur], sizeof(double) * L->nc * L->N);
+}
+int lbm_lattice_upload_aos(lbm_lattice *L, const double *f0, const double *f) {
+    if (!L || !f0 || !f) return fail(LBM_ERR_ARG, "null argument");
+    std::vector<double> soa((size_t)L->nc * L->N);
+    for (int n = 0; n < L->N; ++n) {
+        soa[n] = f0[n];
+        for (int c = 1; c < L->nc; ++c) soa[(size_t)c * L->N + n] = f[(size_t)(L->nc - 1) * n + (c - 1)];
+    }
+    return lbm_lattice_upload(L, soa.data());
+}
+int lbm_lattice_download_aos(lbm_lattice *L, double *f0, double *f) {
+    if (!L || !f0 || !f) return fail(LBM_ERR_ARG, "null argument");
+    std::vector<double> soa((size_t)L->nc * L->N);
+    int rc = lbm_lattice_download(L, soa.data());
+    if (rc) return rc;
+    for (int n = 0; n < L->N; ++n) {
+        f0[n] = soa[n];
+        for (int c = 1; c < L->nc; ++c) f[(size_t)(L->nc - 1) * n + (c - 1)] = soa[(size_t)c * L->N + n];
+    }
+    return LBM_OK;
+}
+
+// ---- streaming, lattice boundaries, smoothing --------------------------------------------
+static int stream_call(lbm_lattice *L, int pull) {
+    if (!L) return fail(LBM_ERR_ARG, "null lattice");
+    lbm_context *C = L->ctx;
+    if (C->recording) {
+        lbm_program *P = C->recording;
+        int slot = prog_slot(P, L);
+        if (slot < 0) return slot;
+        for (const HostOp &h : P->ops)
+            if (h.op.lat == slot) return fail(LBM_ERR_UNSUPPORTED, "stream recorded after boundary calls on that lattice");
+        if (P->pull[slot]) return fail(LBM_ERR_UNSUPPORTED, "one stream per lattice per program");
+        P->pull[slot] = pull;
+        return LBM_OK;
+    }
+    HIPCHK(lbm::launch_stream(L->kind, L->buf[L->cur], L->buf[L->cur ^ 1], L->grid(), pull, C->stream));
+    swap_buf(L);
+    return LBM_OK;
+}
+int lbm_stream(lbm_lattice *L) { return stream_call(L, 1); }
+int lbm_istream(lbm_lattice *L) { return stream_call(L, -1); }
+
+static int bc_call(lbm_lattice *L, const int *type, uint8_t code) {
+    return boundary_call(L, [&](lbm_program *P) {
+        // only types 1 (BARRIER) and 2 (MIRROR) act, d2q9.h:436-441
+        std::vector<int> sel(type, type + (L ? nface_total(L) : 0));
+        for (int &s : sel) s = (s == LBM_BARRIER || s == LBM_MIRROR) ? s : 0;
+        return prog_record(P, L, code, sel.data(), nullptr, [](int, Op &) {});
+    });
+}
+int lbm_bc(lbm_lattice *L, const int *type) {
+    if (!type) return fail(LBM_ERR_ARG, "null type array");
+    return bc_call(L, type, lbm::OP_BOUNCE);
+}
+int lbm_ibc(lbm_lattice *L, const int *type) {
+    if (!type) return fail(LBM_ERR_ARG, "null type array");
+    return bc_call(L, type, lbm::OP_IBOUNCE);
+}
+int lbm_smooth_corner(lbm_lattice *L) {
+    if (!L) return fail(LBM_ERR_ARG, "null lattice");
+    if (L->ctx->recording) return prog_record_smooth(L->ctx->recording, L);
+    return smooth_eager(L);
+}
+
+// ---- NS ---------------------------------------------------------------------------------------
+static bool ns_lattice(const lbm_lattice *L) { return L && is_ns_kind(L->kind); }
+static bool ad_lattice(const lbm_lattice *L) { return L != nullptr; }  // AD runs on any lattice kind
+static int unsupported_3d(const lbm_lattice *L, const char *what) {
+    return fail(LBM_ERR_UNSUPPORTED, "%s on a 3D lattice is not available in this build", what);
+}
+
+int lbm_ns_initial_condition(lbm_lattice *p, const double *rho, const double *ux, const double *uy, const double *uz) {
+    if (!ns_lattice(p) || !fields_ok({rho, ux, uy}) || (p->nd == 3 && !uz)) return fail(LBM_ERR_ARG, "bad arguments");
+    const double *a[7] = {rho, ux, uy, uz, nullptr, nullptr, nullptr};
+    return hip_result(lbm::launch_init(p->kind, p->buf[p->cur], p->N, 0, a, p->ctx->stream), "ns init");
+}
+int lbm_ns_macro_collide_p(lbm_program *P, lbm_lattice *p, double *rho, double *ux, double *uy, double *uz, double nu,
+                           int issave) {
+    if (!ns_lattice(p)) return fail(LBM_ERR_ARG, "NS collide needs a D2Q9 or D3Q15 lattice");
+    if (issave && (!fields_ok({rho, ux, uy}) || (p->nd == 3 && !uz))) return fail(LBM_ERR_ARG, "null field");
+    return collide_call(P, p, nullptr, [&](const lbm::CollideLaunch &cl) {
+        lbm::NSArgs a{rho, ux, uy, uz, nullptr, nu, issave};
+        return hip_result(lbm::launch_ns_collide(cl, a), "ns collide");
+    });
+}
+int lbm_ns_macro_collide(lbm_lattice *p, double *rho, double *ux, double *uy, double *uz, double nu, int issave) {
+    return lbm_ns_macro_collide_p(nullptr, p, rho, ux, uy, uz, nu, issave);
+}
+int lbm_ns_macro_brinkman_collide_p(lbm_program *P, lbm_lattice *p, double *rho, double *ux, double *uy, double *uz,
+                                    double nu, const double *alpha, int issave) {
+    if (!ns_lattice(p) || !alpha) return fail(LBM_ERR_ARG, "bad arguments");
+    if (issave && (!fields_ok({rho, ux, uy}) || (p->nd == 3 && !uz))) return fail(LBM_ERR_ARG, "null field");
+    return collide_call(P, p, nullptr, [&](const lbm::CollideLaunch &cl) {
+        lbm::NSArgs a{rho, ux, uy, uz, alpha, nu, issave};
+        return hip_result(lbm::launch_ns_collide(cl, a), "ns brinkman collide");
+    });
+}
+int lbm_ns_macro_brinkman_collide(lbm_lattice *p, double *rho, double *ux, double *uy, double *uz, double nu,
+                                  const double *alpha, int issave) {
+    return lbm_ns_macro_brinkman_collide_p(nullptr, p, rho, ux, uy, uz, nu, alpha, issave);
+}
+int lbm_ns_set_u(lbm_lattice *p, const int *mask, const double *ux, const double *uy, const double *uz) {
+    if (!ns_lattice(p) || !mask || !ux || !uy || (p->nd == 3 && !uz)) return fail(LBM_ERR_ARG, "bad arguments");
+    if (p->nd == 3) return unsupported_3d(p, "BoundaryConditionSetU");
+    return boundary_call(p, [&](lbm_program *P) {
+        return prog_record(P, p, lbm::OP_NS_SET_U, mask, nullptr, [&](int t, Op &o) {
+            o.v[0] = ux[t];
+            o.v[1] = uy[t];
+            o.v[2] = uz ? uz[t] : 0.0;
+        });
+    });
+}
+int lbm_ns_set_rho(lbm_lattice *p, const int *mask, const double *rho, const double *us, const double *ut) {
+    if (!ns_lattice(p) || !mask || !rho || !us || (p->nd == 3 && !ut)) return fail(LBM_ERR_ARG, "bad arguments");
+    if (p->nd == 3) return unsupported_3d(p, "BoundaryConditionSetRho");
+    return boundary_call(p, [&](lbm_program *P) {
+        return prog_record(P, p, lbm::OP_NS_SET_RHO, mask, nullptr, [&](int t, Op &o) {
+            o.v[0] = rho[t];
+            o.v[1] = us[t];
+            o.v[2] = ut ? ut[t] : 0.0;
+        });
+    });
+}
+
+// ---- AD -------------------------------------------------------------------------------------
+int lbm_ad_initial_condition(lbm_lattice *q, const double *tem, const double *ux, const double *uy, const double *uz) {
+    if (!ad_lattice(q) || !fields_ok({tem, ux, uy}) || (q->nd == 3 && !uz)) return fail(LBM_ERR_ARG, "bad arguments");
+    const double *a[7] = {tem, ux, uy, uz, nullptr, nullptr, nullptr};
+    return hip_result(lbm::launch_init(q->kind, q->buf[q->cur], q->N, 1, a, q->ctx->stream), "ad init");
+}
+int lbm_ad_macro_brinkman_collide_force_convection_p(lbm_program *P, lbm_lattice *p, double *rho, double *ux,
+                                                     double *uy, double *uz, const double *alpha, double nu,
+                                                     lbm_lattice *q, double *tem, double *qx, double *qy, double *qz,
+                                                     const double *diff, int issave, double *g, const double *heat) {
+    if (!ns_lattice(p) || !q || !alpha || !diff) return fail(LBM_ERR_ARG, "bad arguments");
+    if (issave && (!fields_ok({rho, ux, uy, tem, qx, qy}) || (p->nd == 3 && (!uz || !qz))))
+        return fail(LBM_ERR_ARG, "null field");
+    return collide_call(P, p, q, [&](const lbm::CollideLaunch &cl) {
+        lbm::ADArgs a{rho, ux, uy, uz, tem, qx, qy, qz, issave ? g : nullptr, alpha, diff, heat, nu, issave};
+        return hip_result(lbm::launch_ad_collide(cl, a), "ad collide");
+    });
+}
+int lbm_ad_macro_brinkman_collide_force_convection(lbm_lattice *p, double *rho, double *ux, double *uy, double *uz,
+                                                   const double *alpha, double nu, lbm_lattice *q, double *tem,
+                                                   double *qx, double *qy, double *qz, const double *diff, int issave,
+                                                   double *g, const double *heat) {
+    return lbm_ad_macro_brinkman_collide_force_convection_p(nullptr, p, rho, ux, uy, uz, alpha, nu, q, tem, qx, qy, qz,
+                                                            diff, issave, g, heat);
+}
+int lbm_ad_set_t(lbm_lattice *q, const int *mask, const double *tem, const double *ux, const double *uy,
+                 const double *uz) {
+    if (!q || !mask || !tem || !ux || !uy || (q->nd == 3 && !uz)) return fail(LBM_ERR_ARG, "bad arguments");
+    if (q->nd == 3) return unsupported_3d(q, "BoundaryConditionSetT");
+    FieldSet fs{nullptr, ux, uy, uz, nullptr, nullptr};
+    return boundary_call(q, [&](lbm_program *P) {
+        return prog_record(P, q, lbm::OP_AD_SET_T, mask, &fs, [&](int t, Op &o) { o.v[0] = tem[t]; });
+    });
+}
+int lbm_ad_set_q(lbm_lattice *q, const int *mask, const double *qn, const double *ux, const double *uy,
+                 const double *uz, const double *diff, double diff_const) {
+    if (!q || !mask || !qn || !ux || !uy || (q->nd == 3 && !uz)) return fail(LBM_ERR_ARG, "bad arguments");
+    if (q->nd == 3) return unsupported_3d(q, "BoundaryConditionSetQ");
+    FieldSet fs{nullptr, ux, uy, uz, nullptr, diff};
+    return boundary_call(q, [&](lbm_program *P) {
+        return prog_record(P, q, lbm::OP_AD_SET_Q, mask, &fs, [&](int t, Op &o) {
+            o.v[0] = qn[t];
+            o.v[1] = diff_const;
+        });
+    });
+}
+
+// ---- ANS ------------------------------------------------------------------------------------
+int lbm_ans_initial_condition(lbm_lattice *p, const double *ux, const double *uy, const double *uz, const double *ip,
+                              const double *iux, const double *iuy, const double *iuz) {
+    if (!ns_lattice(p) || !fields_ok({ux, uy, ip, iux, iuy}) || (p->nd == 3 && (!uz || !iuz)))
+        return fail(LBM_ERR_ARG, "bad arguments");
+    const double *a[7] = {ux, uy, uz, ip, iux, iuy, iuz};
+    return hip_result(lbm::launch_init(p->kind, p->buf[p->cur], p->N, 2, a, p->ctx->stream), "ans init");
+}
+int lbm_ans_macro_brinkman_collide_p(lbm_program *P, lbm_lattice *p, const double *rho, const double *ux,
+                                     const double *uy, const double *uz, double *ip, double *iux, double *iuy,
+                                     double *iuz, double *imx, double *imy, double *imz, double nu,
+                                     const double *alpha, int issave) {
+    if (!ns_lattice(p) || !fields_ok({rho, ux, uy, alpha}) || (p->nd == 3 && !uz))
+        return fail(LBM_ERR_ARG, "bad arguments");
+    if (issave && (!fields_ok({ip, iux, iuy, imx, imy}) || (p->nd == 3 && (!iuz || !imz))))
+        return fail(LBM_ERR_ARG, "null field");
+    return collide_call(P, p, nullptr, [&](const lbm::CollideLaunch &cl) {
+        lbm::ANSArgs a{rho, ux, uy, uz, alpha, ip, iux, iuy, iuz, imx, imy, imz, nu, issave};
+        return hip_result(lbm::launch_ans_collide(cl, a), "ans collide");
+    });
+}
+int lbm_ans_macro_brinkman_collide(lbm_lattice *p, const double *rho, const double *ux, const double *uy,
+                                   const double *uz, double *ip, double *iux, double *iuy, double *iuz, double *imx,
+                                   double *imy, double *imz, double nu, const double *alpha, int issave) {
+    return lbm_ans_macro_brinkman_collide_p(nullptr, p, rho, ux, uy, uz, ip, iux, iuy, iuz, imx, imy, imz, nu, alpha,
+                                            issave);
+}
+int lbm_ans_iset_u(lbm_lattice *p, const int *mask, const double *ux, const double *uy, const double *uz, double eps,
+                   int compat) {
+    if (!ns_lattice(p) || !mask || !ux || !uy || (p->nd == 3 && !uz)) return fail(LBM_ERR_ARG, "bad arguments");
+    if (p->nd == 3) return unsupported_3d(p, "iBoundaryConditionSetU");
+    return boundary_call(p, [&](lbm_program *P) {
+        return prog_record(P, p, lbm::OP_ANS_ISET_U, mask, nullptr, [&](int t, Op &o) {
+            o.v[0] = ux[t];
+            o.v[1] = uy[t];
+            o.v[2] = uz ? uz[t] : 0.0;
+            o.v3 = eps;
+            o.arg = (uint8_t)(compat ? 1 : 0);
+        });
+    });
+}
+int lbm_ans_iset_rho(lbm_lattice *p, const int *mask) {
+    if (!ns_lattice(p) || !mask) return fail(LBM_ERR_ARG, "bad arguments");
+    if (p->nd == 3) return unsupported_3d(p, "iBoundaryConditionSetRho3D");
+    return boundary_call(p, [&](lbm_program *P) {
+        return prog_record(P, p, lbm::OP_ANS_ISET_RHO, mask, nullptr, [](int, Op &) {});
+    });
+}
+
+// ---- AAD ------------------------------------------------------------------------------------
+int lbm_aad_initial_condition(lbm_lattice *q, const double *ux, const double *uy, const double *uz,
+                              const double *item, const double *iqx, const double *iqy, const double *iqz) {
+    if (!q || !fields_ok({ux, uy, item, iqx, iqy}) || (q->nd == 3 && (!uz || !iqz)))
+        return fail(LBM_ERR_ARG, "bad arguments");
+    const double *a[7] = {ux, uy, uz, item, iqx, iqy, iqz};
+    return hip_result(lbm::launch_init(q->kind, q->buf[q->cur], q->N, 3, a, q->ctx->stream), "aad init");
+}
+int lbm_aad_macro_brinkman_collide_force_convection_p(lbm_program *P, lbm_lattice *p, const double *rho,
+                                                      const double *ux, const double *uy, const double *uz, double *ip,
+                                                      double *iux, double *iuy, double *iuz, double *imx, double *imy,
+                                                      double *imz, const double *alpha, double nu, lbm_lattice *q,
+                                                      const double *tem, double *item, double *iqx, double *iqy,
+                                                      double *iqz, const double *diff, int issave, double *ig) {
+    if (!ns_lattice(p) || !q || !fields_ok({rho, ux, uy, alpha, tem, diff}) || (p->nd == 3 && !uz))
+        return fail(LBM_ERR_ARG, "bad arguments");
+    if (issave && (!fields_ok({ip, iux, iuy, imx, imy, item, iqx, iqy}) || (p->nd == 3 && (!iuz || !imz || !iqz))))
+        return fail(LBM_ERR_ARG, "null field");
+    return collide_call(P, p, q, [&](const lbm::CollideLaunch &cl) {
+        lbm::AADArgs a{rho, ux, uy, uz, alpha, tem, diff, ip, iux, iuy, iuz, imx, imy, imz, item, iqx, iqy, iqz,
+                       issave ? ig : nullptr, nu, issave};
+        return hip_result(lbm::launch_aad_collide(cl, a), "aad collide");
+    });
+}
+int lbm_aad_macro_brinkman_collide_force_convection(lbm_lattice *p, const double *rho, const double *ux,
+                                                    const double *uy, const double *uz, double *ip, double *iux,
+                                                    double *iuy, double *iuz, double *imx, double *imy, double *imz,
+                                                    const double *alpha, double nu, lbm_lattice *q, const double *tem,
+                                                    double *item, double *iqx, double *iqy, double *iqz,
+                                                    const double *diff, int issave, double *ig) {
+    return lbm_aad_macro_brinkman_collide_force_convection_p(nullptr, p, rho, ux, uy, uz, ip, iux, iuy, iuz, imx, imy,
+                                                             imz, alpha, nu, q, tem, item, iqx, iqy, iqz, diff, issave,
+                                                             ig);
+}
+int lbm_aad_iset_t(lbm_lattice *q, const int *mask, const double *ux, const double *uy, const double *uz) {
+    if (!q || !mask || !ux || !uy || (q->nd == 3 && !uz)) return fail(LBM_ERR_ARG, "bad arguments");
+    if (q->nd == 3) return unsupported_3d(q, "iBoundaryConditionSetT");
+    FieldSet fs{nullptr, ux, uy, uz, nullptr, nullptr};
+    return boundary_call(q, [&](lbm_program *P) {
+        return prog_record(P, q, lbm::OP_AAD_ISET_T, mask, &fs, [](int, Op &) {});
+    });
+}
+int lbm_aad_iset_q(lbm_lattice *q, const int *mask, const double *ux, const double *uy, const double *uz, double eps) {
+    if (!q || !mask || !ux || !uy || (q->nd == 3 && !uz)) return fail(LBM_ERR_ARG, "bad arguments");
+    if (q->nd == 3) return unsupported_3d(q, "iBoundaryConditionSetQ");
+    FieldSet fs{nullptr, ux, uy, uz, nullptr, nullptr};
+    return boundary_call(q, [&](lbm_program *P) {
+        return prog_record(P, q, lbm::OP_AAD_ISET_Q, mask, &fs, [&](int, Op &o) { o.v3 = eps; });
+    });
+}
+int lbm_aad_iset_rho(lbm_lattice *p, lbm_lattice *q, const int *type, const double *rho, const double *ux,
+                     const double *uy, const double *uz, const double *tem, double eps) {
+    if (!ns_lattice(p) || !q || !type || !fields_ok({rho, ux, uy, tem}) || (p->nd == 3 && !uz))
+        return fail(LBM_ERR_ARG, "bad arguments");
+    if (p->nd == 3) return unsupported_3d(p, "AAD::iBoundaryConditionSetRho");
+    FieldSet fs{rho, ux, uy, uz, tem, nullptr};
+    return boundary_call(p, [&](lbm_program *P) {
+        // the operator reads q's populations: q must belong to the same program
+        int sq = prog_slot(P, q);
+        if (sq < 0) return sq;
+        std::vector<int> sel(type, type + nface_total(p));
+        for (int &s : sel) s = (s == LBM_SET_T || s == LBM_SET_Q) ? s : 0;
+        return prog_record(P, p, lbm::OP_AAD_ISET_RHO, sel.data(), &fs, [&](int, Op &o) { o.v3 = eps; });
+    });
+}
+
+// ---- sensitivities ----------------------------------------------------------------------------
+int lbm_sensitivity_brinkman(lbm_lattice *p, double *dfds, const double *ux, const double *uy, const double *uz,
+                             const double *imx, const double *imy, const double *imz, const double *dads) {
+    if (!p || !fields_ok({dfds, ux, uy, imx, imy, dads}) || (p->nd == 3 && (!uz || !imz)))
+        return fail(LBM_ERR_ARG, "bad arguments");
+    const double *a[7] = {ux, uy, uz, imx, imy, imz, dads};
+    return hip_result(lbm::launch_sens_brinkman(p->nd, dfds, p->N, a, p->ctx->stream), "sensitivity brinkman");
+}
+int lbm_sensitivity_brinkman_diffusivity(lbm_lattice *q, double *dfds, const double *ux, const double *uy,
+                                         const double *uz, const double *imx, const double *imy, const double *imz,
+                                         const double *dads, const double *tem, const double *item, const double *iqx,
+                                         const double *iqy, const double *iqz, const double *g, const double *ig,
+                                         const double *diff, const double *dkds) {
+    if (!q || !fields_ok({dfds, ux, uy, imx, imy, dads, tem, item, iqx, iqy, g, ig, diff, dkds}) ||
+        (q->nd == 3 && (!uz || !imz || !iqz)))
+        return fail(LBM_ERR_ARG, "bad arguments");
+    const double *a[16] = {ux, uy, uz, imx, imy, imz, dads, tem, item, iqx, iqy, iqz, g, ig, diff, dkds};
+    return hip_result(lbm::launch_sens_bd(q->kind, dfds, q->N, a, q->ctx->stream), "sensitivity brinkman diffusivity");
+}
+int lbm_sensitivity_temperature_at_heat_source(lbm_lattice *q, double *dfds, const double *ux, const double *uy,
+                                               const double *uz, const double *imx, const double *imy,
+                                               const double *imz, const double *dads, const double *tem,
+                                               const double *item, const double *iqx, const double *iqy,
+                                               const double *iqz, const double *g, const double *ig,
+                                               const double *diff, const double *dkds, const int *mask,
+                                               const double *qn) {
+    if (!q || !mask || !qn) return fail(LBM_ERR_ARG, "bad arguments");
+    if (q->kind != LBM_D2Q9)
+        return fail(LBM_ERR_UNSUPPORTED, "the heat-source boundary term exists for D2Q9 only in this build");
+    int rc = lbm_sensitivity_brinkman_diffusivity(q, dfds, ux, uy, uz, imx, imy, imz, dads, tem, item, iqx, iqy, iqz, g,
+                                                  ig, diff, dkds);
+    if (rc) return rc;
+    lbm_program P;
+    P.ctx = q->ctx;
+    rc = prog_record(&P, q, lbm::OP_NONE, mask, nullptr, [&](int t, Op &o) { o.v[0] = qn[t]; });
+    if (rc) return rc;
+    if (P.ops.empty()) return LBM_OK;
+    if ((rc = prog_upload(&P))) return rc;
+    ProgView v = prog_view(&P);
+    HIPCHK(lbm::launch_sens_hs_boundary(dfds, q->N, v, ux, uy, ig, diff, dkds, q->ctx->stream));
+    HIPCHK(hipStreamSynchronize(q->ctx->stream));
+    return LBM_OK;
+}
+
+// ---- reductions -------------------------------------------------------------------------------
+int lbm_residual(lbm_context *C, int ncomp, const double *a0, const double *a1, const double *a2, const double *b0,
+                 const double *b1, const double *b2, int n, double *out) {
+    if (!C || !out || ncomp < 1 || ncomp > 3 || !a0 || !b0 || (ncomp > 1 && (!a1 || !b1)) || (ncomp > 2 && (!a2 || !b2)))
+        return fail(LBM_ERR_ARG, "bad arguments");
+    const double *a[3] = {a0, a1, a2}, *b[3] = {b0, b1, b2};
+    HIPCHK(lbm::launch_residual(ncomp, a, b, n, C->d_part, C->d_out, C->stream));
+    HIPCHK(hipMemcpyAsync(C->h_out, C->d_out, 2 * sizeof(double), hipMemcpyDeviceToHost, C->stream));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    *out = std::sqrt(C->h_out[0] / C->h_out[1]);
+    return LBM_OK;
+}
+int lbm_normalize(lbm_context *C, double *v, int n) {
+    if (!C || !v) return fail(LBM_ERR_ARG, "bad arguments");
+    return hip_result(lbm::launch_normalize(v, n, C->d_part, C->d_out, C->stream), "normalize");
+}
+int lbm_gather_sum(lbm_context *C, const double *v, const int *idx, int count, double *out) {
+    if (!C || !v || !out || (count > 0 && !idx)) return fail(LBM_ERR_ARG, "bad arguments");
+    if (C->scratch.ensure(sizeof(int) * (count > 0 ? count : 1))) return fail(LBM_ERR_HIP, "scratch allocation failed");
+    HIPCHK(hipStreamSynchronize(C->stream));
+    if (count > 0) HIPCHK(hipMemcpy(C->scratch.p, idx, sizeof(int) * count, hipMemcpyHostToDevice));
+    HIPCHK(lbm::launch_gather_sum(v, (const int *)C->scratch.p, count, C->d_out, C->stream));
+    HIPCHK(hipMemcpyAsync(C->h_out, C->d_out, sizeof(double), hipMemcpyDeviceToHost, C->stream));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    *out = C->h_out[0];
+    return LBM_OK;
+}
+int lbm_density_filter(lbm_context *C, int lx, int ly, int lz, double R, const double *v, double *out) {
+    if (!C || !v || !out || R <= 0.0 || lx < 1 || ly < 1 || lz < 1) return fail(LBM_ERR_ARG, "bad arguments");
+    int nR = (int)R, W = 2 * nR + 1, nd = lz > 1 ? 3 : 2;
+    // hat weights, densityfilter.h:403-406,512-515 (same libm calls as the reference); -1 marks
+    // offsets outside the radius
+    std::vector<double> w((size_t)W * W * W);
+    for (int dk = -nR; dk <= nR; ++dk)
+        for (int dj = -nR; dj <= nR; ++dj)
+            for (int di = -nR; di <= nR; ++di) {
+                double dist = std::sqrt(std::pow(-di, 2.0) + std::pow(-dj, 2.0) + std::pow(-dk, 2.0));
+                double wt = dist <= R ? (R - std::sqrt(std::pow(-di, 2.0) + std::pow(-dj, 2.0) + std::pow(-dk, 2.0))) / R : -1.0;
+                w[(di + nR) + W * ((dj + nR) + W * (dk + nR))] = wt;
+            }
+    if (C->scratch.ensure(sizeof(double) * w.size())) return fail(LBM_ERR_HIP, "scratch allocation failed");
+    HIPCHK(hipStreamSynchronize(C->stream));
+    HIPCHK(hipMemcpy(C->scratch.p, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice));
+    HIPCHK(lbm::launch_density_filter(v, out, lx, ly, lz, nR, nd, (const double *)C->scratch.p, C->stream));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    return LBM_OK;
+}
+
+// ---- recording ------------------------------------------------------------------------------
+int lbm_record_begin(lbm_context *C) {
+    if (!C) return fail(LBM_ERR_ARG, "null context");
+    if (C->recording) return fail(LBM_ERR_STATE, "already recording");
+    C->recording = new lbm_program;
+    C->recording->ctx = C;
+    return LBM_OK;
+}
+int lbm_record_end(lbm_context *C, lbm_program **out) {
+    if (!C || !out) return fail(LBM_ERR_ARG, "null argument");
+    if (!C->recording) return fail(LBM_ERR_STATE, "not recording");
+    lbm_program *P = C->recording;
+    C->recording = nullptr;
+    if (P->nlat > 0) {
+        int rc = prog_upload(P);
+        if (rc) {
+            delete P;
+            return rc;
+        }
+    }
+    *out = P;
+    return LBM_OK;
+}
+int lbm_program_destroy(lbm_program *P) {
+    if (P) {
+        (void)hipStreamSynchronize(P->ctx->stream);
+        delete P;
+    }
+    return LBM_OK;
+}
+int lbm_program_apply(lbm_program *P) {
+    if (!P) return fail(LBM_ERR_ARG, "null program");
+    if (P->nlat == 0) return LBM_OK;
+    return prog_apply(P);
+}
+int lbm_program_bind_fields(lbm_program *P, int set, const double *rho, const double *ux, const double *uy,
+                            const double *uz, const double *tem, const double *diff) {
+    if (!P || set < 0 || set >= (int)P->fsets.size()) return fail(LBM_ERR_ARG, "bad field set %d", set);
+    FieldSet &f = P->fsets[set];
+    if (rho) f.rho = rho;
+    if (ux) f.ux = ux;
+    if (uy) f.uy = uy;
+    if (uz) f.uz = uz;
+    if (tem) f.tem = tem;
+    if (diff) f.diff = diff;
+    return LBM_OK;
+}
+int lbm_program_num_field_sets(const lbm_program *P) { return P ? (int)P->fsets.size() : fail(LBM_ERR_ARG, "null"); }
+int lbm_program_fusable(const lbm_program *P) { return P ? (P->ordered_ok ? 1 : 0) : fail(LBM_ERR_ARG, "null"); }
+
+}  // extern "C"

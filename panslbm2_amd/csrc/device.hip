@@ -1,0 +1,762 @@
+// panslbm2_amd/csrc/device.hip -- HIP kernels for gfx950 and their typed launchers.
+//
+// Memory-bound stencil kernels: one thread per lattice node, populations in plane order so each
+// plane access is a coalesced 64-lane stream; no LDS and no MFMA (nothing here is GEMM-shaped).
+// Launch shapes: 256-thread workgroups; the fused kernels remap workgroups so that consecutive
+// node tiles stay on one XCD (xcd_tile).
+#include "device.h"
+
+#include "kernels.h"
+
+namespace lbm {
+
+constexpr int kBlock = 256;
+
+inline int nblocks(long n, int b = kBlock) { return (int)((n + b - 1) / b); }
+
+// ------------------------------------------------------------------------------------------
+// Streaming: d2q9.h:282-291 / d3q15.h:602-613 (pull from x - c), iStream :414-423 / :965-976
+// (pull from x + c).  The rest population is copied through.
+// ------------------------------------------------------------------------------------------
+template <class L>
+__global__ void __launch_bounds__(kBlock) k_stream(const double *__restrict__ src, double *__restrict__ dst, Grid g,
+                                                   int pull) {
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.N) return;
+    int i, j, k;
+    coords(n, g, i, j, k);
+    double f[L::nc];
+    load_pull<L>(src, g, i, j, k, n, pull, f);
+    store_node<L>(dst, g.N, n, f);
+}
+
+// ------------------------------------------------------------------------------------------
+// Boundary program, applied in place: one thread per boundary node runs that node's operators of
+// segment `seg` in call order (segments are separated by SmoothCorner calls).
+// ------------------------------------------------------------------------------------------
+template <class LA, class LB>
+__global__ void __launch_bounds__(kBlock) k_program_apply(double *a, double *b, int N, ProgView pv, int seg) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= pv.nentry) return;
+    int n = pv.entry_node[e];
+    int2 r = pv.entry_ops[e];
+    double fa[LA::nc], fb[LB::nc > 0 ? LB::nc : 1];
+    load_node<LA>(a, N, n, fa);
+    load_node<LB>(b, N, n, fb);
+    for (int t = r.x; t < r.x + r.y; ++t) {
+        Op op = pv.ops[t];
+        if (op.code != OP_SMOOTH && (int)op.seg == seg) apply_op<LA, LB>(op, fa, fb, pv, n);
+    }
+    store_node<LA>(a, N, n, fa);
+    store_node<LB>(b, N, n, fb);
+}
+
+// Corner / edge smoothing in place, d2q9.h:577-587, d3q15.h:1241-1303.  Entry = (node, nb0,
+// nb1, nb2 or -1); neighbours are listed in axis order.
+template <class L>
+__global__ void __launch_bounds__(kBlock) k_smooth(double *f, int N, const int4 *__restrict__ list, int count) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= count) return;
+    int4 s = list[e];
+    sfor<0, L::nc>([&](auto c) {
+        constexpr int C = decltype(c)::value;
+        double *p = f + (size_t)C * N;
+        if (s.w < 0) p[s.x] = 0.5 * (p[s.y] + p[s.z]);
+        else p[s.x] = (p[s.y] + p[s.z] + p[s.w]) / 3.0;
+    });
+}
+
+// ------------------------------------------------------------------------------------------
+// Collide bodies: act on one node's populations in registers.  Save flags and field pointers as
+// in the reference signatures.
+// ------------------------------------------------------------------------------------------
+// NS::MacroCollide / NS::MacroBrinkmanCollide, navierstokes.h:432-546
+template <class LA, class LB>
+struct NSBody {
+    double *rho, *ux, *uy, *uz;
+    const double *alpha;
+    double omega, iomega;
+    int save, brinkman;
+    __device__ __forceinline__ void operator()(double *f, double *, int n) const {
+        double r, vx, vy, vz, feq[LA::nc];
+        ns_macro<LA>(f, r, vx, vy, vz);
+        if (brinkman) {
+            ns_brinkman<LA>(f, r, vx, vy, vz, alpha[n]);
+            ns_macro<LA>(f, r, vx, vy, vz);
+        }
+        if (save) {
+            rho[n] = r;
+            ux[n] = vx;
+            uy[n] = vy;
+            if constexpr (LA::nd == 3) uz[n] = vz;
+        }
+        ns_eq<LA>(feq, r, vx, vy, vz);
+        bgk<LA>(f, feq, omega, iomega);
+    }
+};
+
+// AD::MacroBrinkmanCollideForceConvection, advection.h:814-926 (+ optional volumetric source)
+template <class LA, class LB>
+struct ADBody {
+    double *rho, *ux, *uy, *uz, *tem, *qx, *qy, *qz, *gsnap;
+    const double *alpha, *diff, *heat;
+    double omegaf, iomegaf;
+    int save, N;
+    __device__ __forceinline__ void operator()(double *f, double *g, int n) const {
+        double omegag = 1.0 / (3.0 * diff[n] + 0.5), iomegag = 1.0 - omegag;
+        double r, vx, vy, vz, T, hx, hy, hz;
+        ns_macro<LA>(f, r, vx, vy, vz);
+        ad_macro<LB>(g, vx, vy, vz, omegag, T, hx, hy, hz);
+        ns_brinkman<LA>(f, r, vx, vy, vz, alpha[n]);
+        if (heat) {
+            double h = heat[n];
+            sfor<0, LB::nc>([&](auto c) {
+                constexpr int C = decltype(c)::value;
+                g[C] += LB::w[C] * h;
+            });
+        }
+        ns_macro<LA>(f, r, vx, vy, vz);
+        ad_macro<LB>(g, vx, vy, vz, omegag, T, hx, hy, hz);
+        if (save) {
+            rho[n] = r;
+            ux[n] = vx;
+            uy[n] = vy;
+            tem[n] = T;
+            qx[n] = hx;
+            qy[n] = hy;
+            if constexpr (LA::nd == 3) {
+                uz[n] = vz;
+                qz[n] = hz;
+            }
+            if (gsnap) store_node<LB>(gsnap, N, n, g);
+        }
+        {
+            double feq[LA::nc];
+            ns_eq<LA>(feq, r, vx, vy, vz);
+            bgk<LA>(f, feq, omegaf, iomegaf);
+        }
+        {
+            double geq[LB::nc];
+            ad_eq<LB>(geq, T, vx, vy, vz);
+            bgk<LB>(g, geq, omegag, iomegag);
+        }
+    }
+};
+
+// ANS::MacroBrinkmanCollide, adjointnavierstokes.h:398-470
+template <class LA, class LB>
+struct ANSBody {
+    const double *rho, *ux, *uy, *uz, *alpha;
+    double *ip, *iux, *iuy, *iuz, *imx, *imy, *imz;
+    double omega, iomega;
+    int save;
+    __device__ __forceinline__ void operator()(double *f, double *, int n) const {
+        double vx = ux[n], vy = uy[n], vz = LA::nd == 3 ? uz[n] : 0.0;
+        double a, bx, by, bz, mx, my, mz, feq[LA::nc];
+        ans_macro<LA>(f, vx, vy, vz, a, bx, by, bz, mx, my, mz);
+        ans_brinkman<LA>(f, rho[n], vx, vy, vz, mx, my, mz, alpha[n]);
+        ans_macro<LA>(f, vx, vy, vz, a, bx, by, bz, mx, my, mz);
+        if (save) {
+            ip[n] = a;
+            iux[n] = bx;
+            iuy[n] = by;
+            imx[n] = mx;
+            imy[n] = my;
+            if constexpr (LA::nd == 3) {
+                iuz[n] = bz;
+                imz[n] = mz;
+            }
+        }
+        ans_eq<LA>(feq, vx, vy, vz, a, bx, by, bz);
+        bgk<LA>(f, feq, omega, iomega);
+    }
+};
+
+// AAD::MacroBrinkmanCollideForceConvection, adjointadvection.h:997-1109
+template <class LA, class LB>
+struct AADBody {
+    const double *rho, *ux, *uy, *uz, *alpha, *tem, *diff;
+    double *ip, *iux, *iuy, *iuz, *imx, *imy, *imz, *item, *iqx, *iqy, *iqz, *igsnap;
+    double omegaf, iomegaf;
+    int save, N;
+    __device__ __forceinline__ void operator()(double *f, double *g, int n) const {
+        double omegag = 1.0 / (3.0 * diff[n] + 0.5), iomegag = 1.0 - omegag;
+        double vx = ux[n], vy = uy[n], vz = LA::nd == 3 ? uz[n] : 0.0;
+        double a, bx, by, bz, mx, my, mz, it, hx, hy, hz;
+        ans_macro<LA>(f, vx, vy, vz, a, bx, by, bz, mx, my, mz);
+        aad_macro<LB>(g, it, hx, hy, hz);
+        aad_brinkman<LA>(f, rho[n], vx, vy, vz, mx, my, mz, tem[n], hx, hy, hz, omegag, alpha[n]);
+        ans_macro<LA>(f, vx, vy, vz, a, bx, by, bz, mx, my, mz);
+        if (save) {
+            ip[n] = a;
+            iux[n] = bx;
+            iuy[n] = by;
+            imx[n] = mx;
+            imy[n] = my;
+            item[n] = it;
+            iqx[n] = hx;
+            iqy[n] = hy;
+            if constexpr (LA::nd == 3) {
+                iuz[n] = bz;
+                imz[n] = mz;
+                iqz[n] = hz;
+            }
+            if (igsnap) store_node<LB>(igsnap, N, n, g);
+        }
+        {
+            double feq[LA::nc];
+            ans_eq<LA>(feq, vx, vy, vz, a, bx, by, bz);
+            bgk<LA>(f, feq, omegaf, iomegaf);
+        }
+        double ge = aad_eq<LB>(it, hx, hy, hz, vx, vy, vz);
+        sfor<0, LB::nc>([&](auto c) {
+            constexpr int C = decltype(c)::value;
+            g[C] = iomegag * g[C] + omegag * ge;
+        });
+    }
+};
+
+// ------------------------------------------------------------------------------------------
+// Collide kernels.  In place: the reference's own collide (no pending stream / boundaries).
+// Fused: pull + boundary program + collide, reading `s*` and writing `d*`; nodes owned by the
+// smoothing fix-up kernels are skipped.  Fix-up: the smoothed nodes (2D corners; 3D edges and
+// corners), whose state averages neighbours' post-boundary states.
+// ------------------------------------------------------------------------------------------
+template <class LA, class LB, class Body>
+__global__ void __launch_bounds__(kBlock) k_collide_inplace(double *a, double *b, int N, Body body) {
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    double fa[LA::nc], fb[LB::nc > 0 ? LB::nc : 1];
+    load_node<LA>(a, N, n, fa);
+    load_node<LB>(b, N, n, fb);
+    body(fa, fb, n);
+    store_node<LA>(a, N, n, fa);
+    store_node<LB>(b, N, n, fb);
+}
+
+template <class LA, class LB, class Body>
+__global__ void __launch_bounds__(kBlock) k_collide_fused(const double *__restrict__ sa, const double *__restrict__ sb,
+                                                          double *__restrict__ da, double *__restrict__ db, Grid g,
+                                                          ProgView pv, int any_smooth, Body body) {
+    int tile = xcd_tile(blockIdx.x, gridDim.x);
+    int n = tile * blockDim.x + threadIdx.x;
+    if (n >= g.N) return;
+    if (any_smooth) {
+        int i, j, k;
+        coords(n, g, i, j, k);
+        if (nboundary(i, j, k, g, LA::nd) >= 2) return;
+    }
+    double fa[LA::nc], fb[LB::nc > 0 ? LB::nc : 1];
+    node_state<LA, LB>(sa, sb, g, pv, n, fa, fb);
+    body(fa, fb, n);
+    store_node<LA>(da, g.N, n, fa);
+    store_node<LB>(db, g.N, n, fb);
+}
+
+// State of node n with SMOOTH operators applied; LEVEL bounds the neighbour recursion (2D corners
+// and 3D edges: 1, 3D corners: 2).
+template <class LA, class LB, int LEVEL>
+__device__ void smoothed_state(const double *sa, const double *sb, const Grid &g, const ProgView &pv, int n, double *a,
+                               double *b) {
+    if constexpr (LEVEL == 0) {
+        node_state<LA, LB>(sa, sb, g, pv, n, a, b);
+    } else {
+        int i, j, k;
+        coords(n, g, i, j, k);
+        load_pull<LA>(sa, g, i, j, k, n, pv.pull[0], a);
+        load_pull<LB>(sb, g, i, j, k, n, pv.pull[1], b);
+        int slot = boundary_slot(i, j, k, g.nx, g.ny, g.nz, LA::nd);
+        int e = (slot >= 0 && slot < pv.nslot) ? pv.slot_entry[slot] : -1;
+        if (e < 0) return;
+        int2 r = pv.entry_ops[e];
+        for (int t = r.x; t < r.x + r.y; ++t) {
+            Op op = pv.ops[t];
+            if (op.code != OP_SMOOTH) {
+                apply_op<LA, LB>(op, a, b, pv, n);
+                continue;
+            }
+            // inward neighbours in axis order
+            int nb[3], cnt = 0;
+            int ci[3] = {i, j, k}, cn[3] = {g.nx, g.ny, g.nz};
+            for (int ax = 0; ax < LA::nd; ++ax) {
+                int d = ci[ax] == 0 ? 1 : (ci[ax] == cn[ax] - 1 ? -1 : 0);
+                if (d == 0) continue;
+                int c2[3] = {i, j, k};
+                c2[ax] += d;
+                nb[cnt++] = c2[0] + g.nx * (c2[1] + g.ny * c2[2]);
+            }
+            const bool onA = (op.lat ^ pv.latswap) == 0;
+            constexpr int NCM = LA::nc > LB::nc ? LA::nc : LB::nc;
+            double s0[NCM], s1[NCM], s2[NCM];
+            for (int q = 0; q < cnt; ++q) {
+                double ta[LA::nc], tb[LB::nc > 0 ? LB::nc : 1];
+                smoothed_state<LA, LB, LEVEL - 1>(sa, sb, g, pv, nb[q], ta, tb);
+                double *dst = q == 0 ? s0 : (q == 1 ? s1 : s2);
+                if (onA) {
+                    sfor<0, LA::nc>([&](auto c) { dst[decltype(c)::value] = ta[decltype(c)::value]; });
+                } else if constexpr (LB::nc > 0) {
+                    sfor<0, LB::nc>([&](auto c) { dst[decltype(c)::value] = tb[decltype(c)::value]; });
+                }
+            }
+            if (onA) {
+                sfor<0, LA::nc>([&](auto c) {
+                    constexpr int C = decltype(c)::value;
+                    a[C] = cnt == 2 ? 0.5 * (s0[C] + s1[C]) : (s0[C] + s1[C] + s2[C]) / 3.0;
+                });
+            } else if constexpr (LB::nc > 0) {
+                sfor<0, LB::nc>([&](auto c) {
+                    constexpr int C = decltype(c)::value;
+                    b[C] = cnt == 2 ? 0.5 * (s0[C] + s1[C]) : (s0[C] + s1[C] + s2[C]) / 3.0;
+                });
+            }
+        }
+    }
+}
+
+template <class LA, class LB, class Body, int LEVEL>
+__global__ void __launch_bounds__(kBlock) k_collide_fixup(const double *__restrict__ sa, const double *__restrict__ sb,
+                                                          double *__restrict__ da, double *__restrict__ db, Grid g,
+                                                          ProgView pv, const int *__restrict__ nodes, int count,
+                                                          Body body) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= count) return;
+    int n = nodes[e];
+    double fa[LA::nc], fb[LB::nc > 0 ? LB::nc : 1];
+    smoothed_state<LA, LB, LEVEL>(sa, sb, g, pv, n, fa, fb);
+    body(fa, fb, n);
+    store_node<LA>(da, g.N, n, fa);
+    store_node<LB>(db, g.N, n, fb);
+}
+
+// ------------------------------------------------------------------------------------------
+// Initial conditions (equilibria of given fields): navierstokes.h:550-572, advection.h:1048-1070,
+// adjointnavierstokes.h:474-498, adjointadvection.h:1359-1381
+// ------------------------------------------------------------------------------------------
+template <class L>
+__global__ void __launch_bounds__(kBlock) k_init(double *f, int N, int which, const double *a0, const double *a1,
+                                                 const double *a2, const double *a3, const double *a4, const double *a5,
+                                                 const double *a6) {
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    double feq[L::nc];
+    auto z = [&](const double *p) { return p ? p[n] : 0.0; };
+    if (which == 0) {  // NS: rho, ux, uy, uz
+        ns_eq<L>(feq, a0[n], a1[n], a2[n], z(a3));
+    } else if (which == 1) {  // AD: tem, ux, uy, uz
+        ad_eq<L>(feq, a0[n], a1[n], a2[n], z(a3));
+    } else if (which == 2) {  // ANS: ux, uy, uz, ip, iux, iuy, iuz
+        ans_eq<L>(feq, a0[n], a1[n], z(a2), a3[n], a4[n], a5[n], z(a6));
+    } else {  // AAD: ux, uy, uz, item, iqx, iqy, iqz
+        double ge = aad_eq<L>(a3[n], a4[n], a5[n], z(a6), a0[n], a1[n], z(a2));
+        sfor<0, L::nc>([&](auto c) { feq[decltype(c)::value] = ge; });
+    }
+    store_node<L>(f, N, n, feq);
+}
+
+// ------------------------------------------------------------------------------------------
+// Sensitivities (accumulate into dfds).  adjointnavierstokes.h:542-554,
+// adjointadvection.h:1461-1493 (the reference squares with pow(x, 2.0); x*x here).
+// ------------------------------------------------------------------------------------------
+template <int ND>
+__global__ void __launch_bounds__(kBlock) k_sens_brinkman(double *dfds, int N, const double *ux, const double *uy,
+                                                          const double *uz, const double *imx, const double *imy,
+                                                          const double *imz, const double *dads) {
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    if constexpr (ND == 2) dfds[n] += 3.0 * dads[n] * (ux[n] * imx[n] + uy[n] * imy[n]);
+    else dfds[n] += 3.0 * dads[n] * (ux[n] * imx[n] + uy[n] * imy[n] + uz[n] * imz[n]);
+}
+
+template <class L>
+__global__ void __launch_bounds__(kBlock) k_sens_bd(double *dfds, int N, const double *ux, const double *uy,
+                                                    const double *uz, const double *imx, const double *imy,
+                                                    const double *imz, const double *dads, const double *tem,
+                                                    const double *item, const double *iqx, const double *iqy,
+                                                    const double *iqz, const double *g, const double *ig,
+                                                    const double *diff, const double *dkds) {
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    double d = dfds[n];
+    if constexpr (L::nd == 2) d += 3.0 * dads[n] * (ux[n] * imx[n] + uy[n] * imy[n]);
+    else d += 3.0 * dads[n] * (ux[n] * imx[n] + uy[n] * imy[n] + uz[n] * imz[n]);
+    double sumg = 0.0;
+    sfor<0, L::nc>([&](auto c) {
+        constexpr int C = decltype(c)::value;
+        sumg += g[(size_t)C * N + n] * ig[(size_t)C * N + n];
+    });
+    double uq;
+    if constexpr (L::nd == 2) uq = ux[n] * iqx[n] + uy[n] * iqy[n];
+    else uq = ux[n] * iqx[n] + uy[n] * iqy[n] + uz[n] * iqz[n];
+    double s = 3.0 * diff[n] + 0.5;
+    d += -3.0 / (s * s) * dkds[n] * (sumg - tem[n] * (item[n] + 3.0 * uq));
+    dfds[n] = d;
+}
+
+// adjointadvection.h:755-801: boundary term of the heat-source sensitivity (D2Q9), one thread per
+// boundary node; the node's face terms are added in the reference's face order.
+__global__ void __launch_bounds__(kBlock) k_sens_hs_boundary(double *dfds, int N, ProgView pv, const double *ux,
+                                                             const double *uy, const double *ig, const double *diff,
+                                                             const double *dkds) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= pv.nentry) return;
+    int n = pv.entry_node[e];
+    int2 r = pv.entry_ops[e];
+    double igl[9];
+    sfor<0, 9>([&](auto c) { igl[decltype(c)::value] = ig[(size_t)decltype(c)::value * N + n]; });
+    double vx = ux[n], vy = uy[n], D2 = diff[n] * diff[n], dk = dkds[n], acc_ = dfds[n];
+    for (int t = r.x; t < r.x + r.y; ++t) {
+        Op op = pv.ops[t];
+        double qn = op.v[0], term;
+        switch (op.face) {
+        case 0: term = qn * dk * ((1.0 + 3.0 * vx) * (-6.0 + 4.0 * igl[1] + igl[5] + igl[8]) + 3.0 * vy * (igl[5] - igl[8])) / (36.0 * (1.0 - 3.0 * vx) * D2); break;
+        case 1: term = qn * dk * ((1.0 - 3.0 * vx) * (-6.0 + 4.0 * igl[3] + igl[6] + igl[7]) + 3.0 * vy * (igl[6] - igl[7])) / (36.0 * (1.0 + 3.0 * vx) * D2); break;
+        case 2: term = qn * dk * ((1.0 + 3.0 * vy) * (-6.0 + 4.0 * igl[2] + igl[5] + igl[6]) + 3.0 * vx * (igl[5] - igl[6])) / (36.0 * (1.0 - 3.0 * vy) * D2); break;
+        default: term = qn * dk * ((1.0 - 3.0 * vy) * (-6.0 + 4.0 * igl[4] + igl[7] + igl[8]) + 3.0 * vx * (igl[8] - igl[7])) / (36.0 * (1.0 + 3.0 * vy) * D2); break;
+        }
+        acc_ += term;
+    }
+    dfds[n] = acc_;
+}
+
+// ------------------------------------------------------------------------------------------
+// Reductions.  Fixed grid and fixed tree order, so results do not depend on scheduling.
+// ------------------------------------------------------------------------------------------
+constexpr int kRedBlocks = 512;
+
+__global__ void __launch_bounds__(kBlock) k_residual_partial(int ncomp, const double *a0, const double *a1,
+                                                             const double *a2, const double *b0, const double *b1,
+                                                             const double *b2, int n, double *part) {
+    __shared__ double s0[kBlock], s1[kBlock];
+    double x0 = 0.0, x1 = 0.0;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        double d0 = a0[t] - b0[t], e = d0 * d0, q = a0[t] * a0[t];
+        if (ncomp > 1) {
+            double d1 = a1[t] - b1[t];
+            e += d1 * d1;
+            q += a1[t] * a1[t];
+        }
+        if (ncomp > 2) {
+            double d2 = a2[t] - b2[t];
+            e += d2 * d2;
+            q += a2[t] * a2[t];
+        }
+        x0 += e;
+        x1 += q;
+    }
+    s0[threadIdx.x] = x0;
+    s1[threadIdx.x] = x1;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            s0[threadIdx.x] += s0[threadIdx.x + w];
+            s1[threadIdx.x] += s1[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = s0[0];
+        part[2 * blockIdx.x + 1] = s1[0];
+    }
+}
+__global__ void __launch_bounds__(kBlock) k_pair_final(const double *part, int nb, double *out) {
+    __shared__ double s0[kBlock], s1[kBlock];
+    double x0 = 0.0, x1 = 0.0;
+    for (int t = threadIdx.x; t < nb; t += blockDim.x) {
+        x0 += part[2 * t];
+        x1 += part[2 * t + 1];
+    }
+    s0[threadIdx.x] = x0;
+    s1[threadIdx.x] = x1;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            s0[threadIdx.x] += s0[threadIdx.x + w];
+            s1[threadIdx.x] += s1[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = s0[0];
+        out[1] = s1[0];
+    }
+}
+// max |v| (order-independent, so exact), then v /= max: normalize.h:9-24
+__global__ void __launch_bounds__(kBlock) k_absmax_partial(const double *v, int n, double *part) {
+    __shared__ double s[kBlock];
+    double m = 0.0;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) m = fmax(m, fabs(v[t]));
+    s[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+__global__ void __launch_bounds__(kBlock) k_absmax_final(const double *part, int nb, double *out) {
+    __shared__ double s[kBlock];
+    double m = 0.0;
+    for (int t = threadIdx.x; t < nb; t += blockDim.x) m = fmax(m, part[t]);
+    s[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = s[0];
+}
+__global__ void __launch_bounds__(kBlock) k_divide(double *v, int n, const double *by) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) v[t] /= by[0];
+}
+// sequential sum over an index list, in list order (the reference's serial loop)
+__global__ void k_gather_sum(const double *v, const int *idx, int count, double *out) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        double s = 0.0;
+        for (int t = 0; t < count; ++t) s += v[idx[t]];
+        out[0] = s;
+    }
+}
+__global__ void __launch_bounds__(kBlock) k_fill(double *p, size_t n, double v) {
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+
+// densityfilter.h:390-420 with precomputed hat weights w[(di+nR) + (2nR+1)*((dj+nR) + (2nR+1)*(dk+nR))]
+// (0 where the distance exceeds R); neighbour loop order i2, j2, k2 as in the reference.
+__global__ void __launch_bounds__(kBlock) k_density_filter(const double *v, double *out, int lx, int ly, int lz, int nR,
+                                                           int nd, const double *w) {
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= lx * ly * lz) return;
+    int i1 = n % lx, j1 = (n / lx) % ly, k1 = n / (lx * ly);
+    int W = 2 * nR + 1;
+    double ws = 0.0, acc_ = 0.0;
+    for (int i2 = i1 - nR; i2 <= i1 + nR; ++i2)
+        for (int j2 = j1 - nR; j2 <= j1 + nR; ++j2)
+            for (int k2 = k1 - nR; k2 <= k1 + nR; ++k2) {
+                double wt = w[(i2 - i1 + nR) + W * ((j2 - j1 + nR) + W * (k2 - k1 + nR))];
+                if (wt == -1.0) continue;  // outside the radius
+                if (0 <= i2 && i2 < lx && 0 <= j2 && j2 < ly && 0 <= k2 && k2 < lz) {
+                    ws += wt;
+                    acc_ += wt * v[i2 + lx * (j2 + ly * k2)];
+                }
+            }
+    out[n] = acc_ / ws;
+}
+
+// ==========================================================================================
+// typed launchers
+// ==========================================================================================
+#define LBM_LAUNCH(kernel, grid, stream, ...) hipLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), 0, stream, __VA_ARGS__)
+
+template <class L>
+static hipError_t launch_stream_t(const double *src, double *dst, Grid g, int pull, hipStream_t s) {
+    LBM_LAUNCH((k_stream<L>), nblocks(g.N), s, src, dst, g, pull);
+    return hipGetLastError();
+}
+hipError_t launch_stream(int kind, const double *src, double *dst, Grid g, int pull, hipStream_t s) {
+    switch (kind) {
+    case kD2Q9: return launch_stream_t<D2Q9>(src, dst, g, pull, s);
+    case kD3Q15: return launch_stream_t<D3Q15>(src, dst, g, pull, s);
+    case kD2Q5: return launch_stream_t<D2Q5>(src, dst, g, pull, s);
+    case kD3Q7: return launch_stream_t<D3Q7>(src, dst, g, pull, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+template <class L>
+static hipError_t launch_smooth_t(double *f, int N, const int4 *list, int count, hipStream_t s) {
+    if (count > 0) LBM_LAUNCH((k_smooth<L>), nblocks(count), s, f, N, list, count);
+    return hipGetLastError();
+}
+hipError_t launch_smooth(int kind, double *f, int N, const int4 *list, int count, hipStream_t s) {
+    switch (kind) {
+    case kD2Q9: return launch_smooth_t<D2Q9>(f, N, list, count, s);
+    case kD3Q15: return launch_smooth_t<D3Q15>(f, N, list, count, s);
+    case kD2Q5: return launch_smooth_t<D2Q5>(f, N, list, count, s);
+    case kD3Q7: return launch_smooth_t<D3Q7>(f, N, list, count, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+// Dispatch a functor over the supported lattices (single) or (A, B) lattice pairs (coupled).
+template <class F>
+static hipError_t dispatch_single(int ka, F &&f) {
+    switch (ka) {
+    case kD2Q9: return f(D2Q9{}, NoLat{});
+    case kD3Q15: return f(D3Q15{}, NoLat{});
+    case kD2Q5: return f(D2Q5{}, NoLat{});
+    case kD3Q7: return f(D3Q7{}, NoLat{});
+    }
+    return hipErrorInvalidValue;
+}
+template <class F>
+static hipError_t dispatch_coupled(int ka, int kb, F &&f) {
+    if (ka == kD2Q9) {
+        if (kb == kD2Q9) return f(D2Q9{}, D2Q9{});
+        if (kb == kD2Q5) return f(D2Q9{}, D2Q5{});
+    } else if (ka == kD3Q15) {
+        if (kb == kD3Q15) return f(D3Q15{}, D3Q15{});
+        if (kb == kD3Q7) return f(D3Q15{}, D3Q7{});
+    }
+    return hipErrorInvalidValue;
+}
+template <class F>
+static hipError_t dispatch_pair(int ka, int kb, F &&f) {
+    return kb < 0 ? dispatch_single(ka, f) : dispatch_coupled(ka, kb, f);
+}
+
+hipError_t launch_program_apply(int ka, int kb, double *a, double *b, int N, const ProgView &pv, int seg,
+                                hipStream_t s) {
+    if (pv.nentry == 0) return hipSuccess;
+    return dispatch_pair(ka, kb, [&](auto la, auto lb) {
+        using LA = decltype(la);
+        using LB = decltype(lb);
+        LBM_LAUNCH((k_program_apply<LA, LB>), nblocks(pv.nentry), s, a, b, N, pv, seg);
+        return hipGetLastError();
+    });
+}
+
+// Collide launch: body built per (LA, LB) by `make`.
+template <template <class, class> class Body, bool COUPLED, class Make>
+static hipError_t launch_collide(const CollideLaunch &cl, Make &&make) {
+    auto go = [&](auto la, auto lb) -> hipError_t {
+        using LA = decltype(la);
+        using LB = decltype(lb);
+        Body<LA, LB> body = make(la, lb);
+        const Grid &g = cl.g;
+        if (!cl.pv) {
+            LBM_LAUNCH((k_collide_inplace<LA, LB, Body<LA, LB>>), nblocks(g.N), cl.stream, cl.da, cl.db, g.N, body);
+            return hipGetLastError();
+        }
+        const ProgView &pv = *cl.pv;
+        if (cl.n_fix_corner > 0) {
+            if constexpr (LA::nd == 3)
+                LBM_LAUNCH((k_collide_fixup<LA, LB, Body<LA, LB>, 2>), nblocks(cl.n_fix_corner), cl.stream, cl.sa, cl.sb,
+                           cl.da, cl.db, g, pv, cl.fix_corner, cl.n_fix_corner, body);
+            else
+                LBM_LAUNCH((k_collide_fixup<LA, LB, Body<LA, LB>, 1>), nblocks(cl.n_fix_corner), cl.stream, cl.sa, cl.sb,
+                           cl.da, cl.db, g, pv, cl.fix_corner, cl.n_fix_corner, body);
+        }
+        if (cl.n_fix_edge > 0) {
+            LBM_LAUNCH((k_collide_fixup<LA, LB, Body<LA, LB>, 1>), nblocks(cl.n_fix_edge), cl.stream, cl.sa, cl.sb,
+                       cl.da, cl.db, g, pv, cl.fix_edge, cl.n_fix_edge, body);
+        }
+        LBM_LAUNCH((k_collide_fused<LA, LB, Body<LA, LB>>), nblocks(g.N), cl.stream, cl.sa, cl.sb, cl.da, cl.db, g, pv,
+                   cl.any_smooth, body);
+        return hipGetLastError();
+    };
+    if constexpr (COUPLED) return dispatch_coupled(cl.ka, cl.kb, go);
+    else return cl.kb < 0 ? dispatch_single(cl.ka, go) : hipErrorInvalidValue;
+}
+
+hipError_t launch_ns_collide(const CollideLaunch &cl, const NSArgs &a) {
+    double omega = 1.0 / (3.0 * a.nu + 0.5);
+    return launch_collide<NSBody, false>(cl, [&](auto la, auto lb) {
+        NSBody<decltype(la), decltype(lb)> b;
+        b.rho = a.rho; b.ux = a.ux; b.uy = a.uy; b.uz = a.uz; b.alpha = a.alpha;
+        b.omega = omega; b.iomega = 1.0 - omega; b.save = a.save; b.brinkman = a.alpha != nullptr;
+        return b;
+    });
+}
+hipError_t launch_ad_collide(const CollideLaunch &cl, const ADArgs &a) {
+    double omega = 1.0 / (3.0 * a.nu + 0.5);
+    return launch_collide<ADBody, true>(cl, [&](auto la, auto lb) {
+        ADBody<decltype(la), decltype(lb)> b;
+        b.rho = a.rho; b.ux = a.ux; b.uy = a.uy; b.uz = a.uz; b.tem = a.tem; b.qx = a.qx; b.qy = a.qy; b.qz = a.qz;
+        b.gsnap = a.gsnap; b.alpha = a.alpha; b.diff = a.diff; b.heat = a.heat;
+        b.omegaf = omega; b.iomegaf = 1.0 - omega; b.save = a.save; b.N = cl.g.N;
+        return b;
+    });
+}
+hipError_t launch_ans_collide(const CollideLaunch &cl, const ANSArgs &a) {
+    double omega = 1.0 / (3.0 * a.nu + 0.5);
+    return launch_collide<ANSBody, false>(cl, [&](auto la, auto lb) {
+        ANSBody<decltype(la), decltype(lb)> b;
+        b.rho = a.rho; b.ux = a.ux; b.uy = a.uy; b.uz = a.uz; b.alpha = a.alpha;
+        b.ip = a.ip; b.iux = a.iux; b.iuy = a.iuy; b.iuz = a.iuz; b.imx = a.imx; b.imy = a.imy; b.imz = a.imz;
+        b.omega = omega; b.iomega = 1.0 - omega; b.save = a.save;
+        return b;
+    });
+}
+hipError_t launch_aad_collide(const CollideLaunch &cl, const AADArgs &a) {
+    double omega = 1.0 / (3.0 * a.nu + 0.5);
+    return launch_collide<AADBody, true>(cl, [&](auto la, auto lb) {
+        AADBody<decltype(la), decltype(lb)> b;
+        b.rho = a.rho; b.ux = a.ux; b.uy = a.uy; b.uz = a.uz; b.alpha = a.alpha; b.tem = a.tem; b.diff = a.diff;
+        b.ip = a.ip; b.iux = a.iux; b.iuy = a.iuy; b.iuz = a.iuz; b.imx = a.imx; b.imy = a.imy; b.imz = a.imz;
+        b.item = a.item; b.iqx = a.iqx; b.iqy = a.iqy; b.iqz = a.iqz; b.igsnap = a.igsnap;
+        b.omegaf = omega; b.iomegaf = 1.0 - omega; b.save = a.save; b.N = cl.g.N;
+        return b;
+    });
+}
+
+template <class L>
+static hipError_t launch_init_t(double *f, int N, int which, const double *const *p, hipStream_t s) {
+    LBM_LAUNCH((k_init<L>), nblocks(N), s, f, N, which, p[0], p[1], p[2], p[3], p[4], p[5], p[6]);
+    return hipGetLastError();
+}
+hipError_t launch_init(int kind, double *f, int N, int which, const double *const *p, hipStream_t s) {
+    switch (kind) {
+    case kD2Q9: return launch_init_t<D2Q9>(f, N, which, p, s);
+    case kD3Q15: return launch_init_t<D3Q15>(f, N, which, p, s);
+    case kD2Q5: return launch_init_t<D2Q5>(f, N, which, p, s);
+    case kD3Q7: return launch_init_t<D3Q7>(f, N, which, p, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_sens_brinkman(int nd, double *dfds, int N, const double *const *p, hipStream_t s) {
+    if (nd == 2) LBM_LAUNCH((k_sens_brinkman<2>), nblocks(N), s, dfds, N, p[0], p[1], p[2], p[3], p[4], p[5], p[6]);
+    else LBM_LAUNCH((k_sens_brinkman<3>), nblocks(N), s, dfds, N, p[0], p[1], p[2], p[3], p[4], p[5], p[6]);
+    return hipGetLastError();
+}
+hipError_t launch_sens_bd(int kind, double *dfds, int N, const double *const *p, hipStream_t s) {
+    auto go = [&](auto l) {
+        using L = decltype(l);
+        LBM_LAUNCH((k_sens_bd<L>), nblocks(N), s, dfds, N, p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], p[9],
+                   p[10], p[11], p[12], p[13], p[14], p[15]);
+        return hipGetLastError();
+    };
+    switch (kind) {
+    case kD2Q9: return go(D2Q9{});
+    case kD3Q15: return go(D3Q15{});
+    case kD2Q5: return go(D2Q5{});
+    case kD3Q7: return go(D3Q7{});
+    }
+    return hipErrorInvalidValue;
+}
+hipError_t launch_sens_hs_boundary(double *dfds, int N, const ProgView &pv, const double *ux, const double *uy,
+                                   const double *ig, const double *diff, const double *dkds, hipStream_t s) {
+    if (pv.nentry > 0) LBM_LAUNCH(k_sens_hs_boundary, nblocks(pv.nentry), s, dfds, N, pv, ux, uy, ig, diff, dkds);
+    return hipGetLastError();
+}
+
+hipError_t launch_residual(int ncomp, const double *const *a, const double *const *b, int n, double *part,
+                           double *out, hipStream_t s) {
+    LBM_LAUNCH(k_residual_partial, kRedBlocks, s, ncomp, a[0], a[1], a[2], b[0], b[1], b[2], n, part);
+    LBM_LAUNCH(k_pair_final, 1, s, part, kRedBlocks, out);
+    return hipGetLastError();
+}
+hipError_t launch_normalize(double *v, int n, double *part, double *out, hipStream_t s) {
+    LBM_LAUNCH(k_absmax_partial, kRedBlocks, s, v, n, part);
+    LBM_LAUNCH(k_absmax_final, 1, s, part, kRedBlocks, out);
+    LBM_LAUNCH(k_divide, nblocks(n), s, v, n, out);
+    return hipGetLastError();
+}
+hipError_t launch_gather_sum(const double *v, const int *idx, int count, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_gather_sum, dim3(1), dim3(64), 0, s, v, idx, count, out);
+    return hipGetLastError();
+}
+hipError_t launch_fill(double *p, size_t n, double v, hipStream_t s) {
+    if (n) LBM_LAUNCH(k_fill, nblocks((long)n), s, p, n, v);
+    return hipGetLastError();
+}
+hipError_t launch_density_filter(const double *v, double *out, int lx, int ly, int lz, int nR, int nd, const double *w,
+                                 hipStream_t s) {
+    LBM_LAUNCH(k_density_filter, nblocks((long)lx * ly * lz), s, v, out, lx, ly, lz, nR, nd, w);
+    return hipGetLastError();
+}
+int reduction_blocks() { return kRedBlocks; }
+
+}  // namespace lbm

@@ -1,0 +1,202 @@
+// panslbm2_amd/csrc/kernels.h -- node-state assembly shared by the stream, boundary-program,
+// corner-smoothing and collide kernels.
+//
+// A node's post-stream state is pulled from the source buffer (periodic wrap as in d2q9.h:80-84),
+// then the node's boundary operators run in call order.  Kernels that own smoothed nodes
+// (SmoothCorner) rebuild the neighbours' states the same way and average them.
+#pragma once
+#include "bc2d.h"
+#include "bc3d.h"
+#include "lattice.h"
+#include "physics.h"
+#include "program.h"
+
+namespace lbm {
+
+// Placeholder for "no second lattice".
+struct NoLat {
+    static constexpr int kind = -1, nc = 0, nd = 0;
+    static constexpr int cx[1] = {0}, cy[1] = {0}, cz[1] = {0};
+    static constexpr double w[1] = {0.0};
+};
+
+// compile-time velocity-set helpers
+template <class L>
+constexpr int comp_of(int c, int ax) {
+    return ax == 0 ? L::cx[c] : (ax == 1 ? L::cy[c] : L::cz[c]);
+}
+template <class L>
+constexpr int opposite_of(int c) {
+    for (int e = 0; e < L::nc; ++e)
+        if (L::cx[e] == -L::cx[c] && L::cy[e] == -L::cy[c] && L::cz[e] == -L::cz[c]) return e;
+    return 0;
+}
+template <class L>
+constexpr int mirror_of(int c, int ax) {
+    int v0 = L::cx[c], v1 = L::cy[c], v2 = L::cz[c];
+    if (ax == 0) v0 = -v0;
+    else if (ax == 1) v1 = -v1;
+    else v2 = -v2;
+    for (int e = 0; e < L::nc; ++e)
+        if (L::cx[e] == v0 && L::cy[e] == v1 && L::cz[e] == v2) return e;
+    return 0;
+}
+
+// Bounce-back (type 1) or specular reflection (type 2) of the populations streaming left
+// undefined on face FACE: incoming ones (c_axis == -dir) after Stream, outgoing ones
+// (c_axis == dir) after iStream.  d2q9.h:429-575, d3q15.h:982-1239.
+template <class L, int FACE, bool ADJ>
+__device__ __forceinline__ void bounce_face(double *f, int type) {
+    constexpr int AX = FACE / 2;
+    constexpr int DIR = (FACE % 2) ? 1 : -1;
+    constexpr int WANT = ADJ ? DIR : -DIR;
+    sfor<1, L::nc>([&](auto c) {
+        constexpr int C = decltype(c)::value;
+        if constexpr (comp_of<L>(C, AX) == WANT) {
+            constexpr int OPP = opposite_of<L>(C);
+            constexpr int MIR = mirror_of<L>(C, AX);
+            f[C] = (type == 1) ? f[OPP] : f[MIR];
+        }
+    });
+}
+template <class L, bool ADJ>
+__device__ __forceinline__ void bounce(double *f, int face, int type) {
+    switch (face) {
+    case 0: bounce_face<L, 0, ADJ>(f, type); break;
+    case 1: bounce_face<L, 1, ADJ>(f, type); break;
+    case 2: bounce_face<L, 2, ADJ>(f, type); break;
+    case 3: bounce_face<L, 3, ADJ>(f, type); break;
+    default:
+        if constexpr (L::nd == 3) {
+            if (face == 4) bounce_face<L, 4, ADJ>(f, type);
+            else bounce_face<L, 5, ADJ>(f, type);
+        }
+    }
+}
+
+// Apply one boundary operator to lattice L's populations `f`; `o` holds the other lattice's
+// populations (lattice LO), read by the coupled adjoint pressure operator.
+template <class L, class LO>
+__device__ __forceinline__ void apply_on(const Op &op, double *f, const double *o, const FieldSet &s, int n) {
+    switch (op.code) {
+    case OP_BOUNCE: bounce<L, false>(f, op.face, op.arg); break;
+    case OP_IBOUNCE: bounce<L, true>(f, op.face, op.arg); break;
+    case OP_NS_SET_U:
+        if constexpr (L::kind == kD2Q9) ns_set_u_2d(f, op.face, op.v[0], op.v[1]);
+        else if constexpr (L::kind == kD3Q15) ns_set_u_3d(f, op.face, op.v[0], op.v[1], op.v[2]);
+        break;
+    case OP_NS_SET_RHO:
+        if constexpr (L::kind == kD2Q9) ns_set_rho_2d(f, op.face, op.v[0], op.v[1]);
+        else if constexpr (L::kind == kD3Q15) ns_set_rho_3d(f, op.face, op.v[0], op.v[1], op.v[2]);
+        break;
+    case OP_AD_SET_T:
+        if constexpr (L::nd == 2) ad_set_t_2d<L>(f, op.face, op.v[0], s.ux[n], s.uy[n]);
+        else if constexpr (L::nd == 3) ad_set_t_3d<L>(f, op.face, op.v[0], s.ux[n], s.uy[n], s.uz[n]);
+        break;
+    case OP_AD_SET_Q: {
+        double d = s.diff ? s.diff[n] : op.v[1];
+        if constexpr (L::nd == 2) ad_set_q_2d<L>(f, op.face, op.v[0], s.ux[n], s.uy[n], d);
+        else if constexpr (L::nd == 3) ad_set_q_3d<L>(f, op.face, op.v[0], s.ux[n], s.uy[n], s.uz[n], d);
+    } break;
+    case OP_ANS_ISET_U:
+        if constexpr (L::kind == kD2Q9) ans_iset_u_2d(f, op.face, op.v[0], op.v[1], op.v3, op.arg);
+        else if constexpr (L::kind == kD3Q15) ans_iset_u_3d(f, op.face, op.v[0], op.v[1], op.v[2], op.v3);
+        break;
+    case OP_ANS_ISET_RHO:
+        if constexpr (L::kind == kD2Q9) ans_iset_rho_2d(f, op.face);
+        else if constexpr (L::kind == kD3Q15) ans_iset_rho_3d(f, op.face);
+        break;
+    case OP_AAD_ISET_T:
+        if constexpr (L::nd == 2) aad_iset_t_2d<L>(f, op.face, s.ux[n], s.uy[n]);
+        else if constexpr (L::nd == 3) aad_iset_t_3d<L>(f, op.face, s.ux[n], s.uy[n], s.uz[n]);
+        break;
+    case OP_AAD_ISET_Q:
+        if constexpr (L::nd == 2) aad_iset_q_2d<L>(f, op.face, s.ux[n], s.uy[n], op.v3);
+        else if constexpr (L::nd == 3) aad_iset_q_3d<L>(f, op.face, s.ux[n], s.uy[n], s.uz[n], op.v3);
+        break;
+    case OP_AAD_ISET_RHO:
+        if constexpr (L::kind == kD2Q9 && LO::nd == 2)
+            aad_iset_rho_2d<LO>(f, o, op.face, op.arg, s.rho[n], s.ux[n], s.uy[n], s.tem[n], op.v3);
+        else if constexpr (L::kind == kD3Q15 && LO::nd == 3)
+            aad_iset_rho_3d<LO>(f, o, op.face, op.arg, s.rho[n], s.ux[n], s.uy[n], s.uz[n], s.tem[n], op.v3);
+        break;
+    default: break;
+    }
+}
+
+template <class LA, class LB>
+__device__ __forceinline__ void apply_op(const Op &op, double *a, double *b, const ProgView &pv, int n) {
+    const FieldSet &s = pv.fs[op.fset];
+    if ((op.lat ^ pv.latswap) == 0) apply_on<LA, LB>(op, a, b, s, n);
+    else if constexpr (LB::nc > 0) apply_on<LB, LA>(op, b, a, s, n);
+}
+
+// Pull node n's populations: pull = +1 from x - c (Stream), -1 from x + c (iStream), 0 in place.
+template <class L>
+__device__ __forceinline__ void load_pull(const double *__restrict__ src, const Grid &g, int i, int j, int k, int n,
+                                          int pull, double *f) {
+    if constexpr (L::nc > 0) {
+        f[0] = src[n];
+        sfor<1, L::nc>([&](auto c) {
+            constexpr int C = decltype(c)::value;
+            int s = n;
+            if (pull != 0) {
+                int ii = wrap(i, -pull * L::cx[C], g.nx);
+                int jj = wrap(j, -pull * L::cy[C], g.ny);
+                int kk = L::nd == 3 ? wrap(k, -pull * L::cz[C], g.nz) : 0;
+                s = ii + g.nx * (jj + g.ny * kk);
+            }
+            f[C] = src[(size_t)C * g.N + s];
+        });
+    }
+}
+template <class L>
+__device__ __forceinline__ void load_node(const double *__restrict__ src, int N, int n, double *f) {
+    sfor<0, L::nc>([&](auto c) {
+        constexpr int C = decltype(c)::value;
+        f[C] = src[(size_t)C * N + n];
+    });
+}
+template <class L>
+__device__ __forceinline__ void store_node(double *__restrict__ dst, int N, int n, const double *f) {
+    sfor<0, L::nc>([&](auto c) {
+        constexpr int C = decltype(c)::value;
+        dst[(size_t)C * N + n] = f[C];
+    });
+}
+
+__device__ __forceinline__ void coords(int n, const Grid &g, int &i, int &j, int &k) {
+    i = n % g.nx;
+    int t = n / g.nx;
+    j = t % g.ny;
+    k = t / g.ny;
+}
+__device__ __forceinline__ int nboundary(int i, int j, int k, const Grid &g, int nd) {
+    int c = (i == 0 || i == g.nx - 1) + (j == 0 || j == g.ny - 1);
+    if (nd == 3) c += (k == 0 || k == g.nz - 1);
+    return c;
+}
+
+// Post-stream, post-boundary state of node n; SMOOTH operators are skipped (callers that own
+// smoothed nodes apply them).  Returns nothing for interior nodes beyond the pull.
+template <class LA, class LB>
+__device__ __forceinline__ void node_state(const double *sa, const double *sb, const Grid &g, const ProgView &pv, int n,
+                                           double *a, double *b) {
+    int i, j, k;
+    coords(n, g, i, j, k);
+    load_pull<LA>(sa, g, i, j, k, n, pv.pull[0], a);
+    load_pull<LB>(sb, g, i, j, k, n, pv.pull[1], b);
+    int slot = boundary_slot(i, j, k, g.nx, g.ny, g.nz, LA::nd);
+    if (slot >= 0 && slot < pv.nslot) {
+        int e = pv.slot_entry[slot];
+        if (e >= 0) {
+            int2 r = pv.entry_ops[e];
+            for (int t = r.x; t < r.x + r.y; ++t) {
+                Op op = pv.ops[t];
+                if (op.code != OP_SMOOTH) apply_op<LA, LB>(op, a, b, pv, n);
+            }
+        }
+    }
+}
+
+}  // namespace lbm
