@@ -1,0 +1,196 @@
+#!/usr/bin/env python3
+"""bench.py -- design evaluations of the 2D forced-convection heatsink on one MI355X.
+
+One step = one design evaluation: forward solve to a 1e-6 step-to-step residual (checked every
+100 steps), adjoint solve to the same residual, design sensitivity (see
+panslbm2_amd/heatsink2d.py).  Workload: 200 x 400 nodes, D2Q9 Navier-Stokes with Brinkman drag
+coupled to D2Q5 advection-diffusion with a volumetric heat source, FP64, seeded synthetic design.
+
+value = MLUPS over the job: nodes x (forward + adjoint time steps) / wall seconds, a coupled
+NS+AD node update counting once.  Inputs are resident on the device before timing starts.
+
+Also reported:
+  roofline      -- the fused forward step kernel (pull + boundary program + collide) timed with
+                   HIP events on the library's stream over 100-step graph launches, against the
+                   8 TB/s HBM peak, for this workload (cache resident: ~20 MB per step) and for a
+                   4096 x 4096 grid (4.2 GB per step, HBM bound).
+  cpu_baseline  -- the reference's own plain C++ (oracle/_ref, OpenMP) running the same forward
+                   and adjoint time steps on D2Q9 + D2Q9 (the reference has no D2Q5) on the host
+                   cores, a bounded number of steps.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per node update of the fused forward step, FP64: populations read once and
+# written once (D2Q9 + D2Q5 = 14 doubles each way) plus alpha, diffusivity and heat source read.
+BYTES_FWD_Q9Q5 = 2 * (9 + 5) * 8 + 3 * 8
+BYTES_FWD_Q9Q9 = 2 * (9 + 9) * 8 + 3 * 8
+
+
+def kernel_rate(hs, steps=100, reps=5):
+    """Average duration (ms) of one fused forward step, from HIP events around graph launches."""
+    # make sure the forward chunk graph exists and state is mid-solve
+    hs._chunk("fwd", 100, hs._fwd_step)
+    hs.ctx.sync()
+    hs.ctx.record(0)
+    for _ in range(reps):
+        hs._chunk("fwd", 100, hs._fwd_step)
+    hs.ctx.record(1)
+    ms = hs.ctx.elapsed_ms(0, 1)
+    return ms / (reps * hs.check_every)
+
+
+def big_grid_roofline(ctx, lx=4096, ly=4096):
+    from panslbm2_amd import D2Q5
+    from panslbm2_amd.heatsink2d import Heatsink2D
+
+    hs = Heatsink2D(ctx, lx=lx, ly=ly, ad_kind=D2Q5, max_steps=200, filter_radius=1.0)
+    L, d = hs.L, hs.d
+    from panslbm2_amd import check
+
+    check(L.lbm_ns_initial_condition(hs.pf.h, d["one"].ptr, d["ux0"].ptr, d["uy0"].ptr, None))
+    check(L.lbm_ad_initial_condition(hs.pg.h, d["tem"].ptr, d["ux0"].ptr, d["uy0"].ptr, None))
+    for s in range(1, 101):
+        hs._fwd_step(s)
+    ms = kernel_rate(hs, reps=2)
+    hs.close()
+    return ms, lx * ly
+
+
+def cpu_baseline(hs, target_seconds=15.0):
+    """The reference's plain OpenMP path on the host cores (oracle/_ref), D2Q9 + D2Q9."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
+
+    from _oracle import lib as oracle_lib
+    from _oracle import ref
+
+    R = ref()
+    if R is None:
+        return None
+    lx, ly, N = hs.lx, hs.ly, hs.N
+    i, j = hs.i, hs.j
+    walls = ((j == 0) | (j == ly - 1)).astype(np.int32)
+    inlet = ((i == 0) & (j > 0) & (j < ly - 1)).astype(np.int32)
+    outlet = ((i == lx - 1) & (j > 0) & (j < ly - 1)).astype(np.int32)
+    flux = (walls | outlet).astype(np.int32)
+    objedge = ((j == 0) & (i < lx // 2)).astype(np.int32)
+    H = ly - 1.0
+    uin = np.ascontiguousarray(4.0 * 0.05 * j * (H - j) / H ** 2)
+    alpha = np.ascontiguousarray(hs.d["alpha"].download())
+    diff = np.ascontiguousarray(hs.d["diff"].download())
+    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    secs = np.zeros(2)
+    f = R.ref2_heatsink_steps
+    f.argtypes = [ctypes.c_int] * 4 + [ctypes.c_double] + [ctypes.c_void_p] * 9
+    nt = 20
+    f(lx, ly, nt, nt, 0.1, P(alpha), P(diff), P(walls), P(inlet), P(outlet), P(flux), P(objedge), P(uin), P(secs))
+    per = secs.sum() / (2 * nt)
+    nt = max(20, int(target_seconds / max(per, 1e-9) / 2))
+    f(lx, ly, nt, nt, 0.1, P(alpha), P(diff), P(walls), P(inlet), P(outlet), P(flux), P(objedge), P(uin), P(secs))
+    mlups = N * 2 * nt / secs.sum() / 1e6
+    return {"value": round(mlups, 3), "unit": "MLUPS", "cores": int(oracle_lib().ol_num_threads()),
+            "kind": "reference",
+            "sample": "%d forward + %d adjoint steps of the same 200x400 heatsink with the reference's plain "
+                      "OpenMP functions on D2Q9+D2Q9 (the reference has no D2Q5): fwd %.2f s, adj %.2f s"
+                      % (nt, nt, secs[0], secs[1])}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--lx", type=int, default=200)
+    ap.add_argument("--ly", type=int, default=400)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-big", action="store_true")
+    ap.add_argument("--max-steps", type=int, default=400000, help="cap on forward and on adjoint steps")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (requested --gpus %d)" % a.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    from panslbm2_amd import D2Q5, D2Q9, Context
+    from panslbm2_amd.heatsink2d import Heatsink2D
+
+    ctx = Context(0)
+    hs = Heatsink2D(ctx, lx=a.lx, ly=a.ly, ad_kind=D2Q5, max_steps=a.max_steps)
+    print("bench: heatsink %dx%d D2Q9+D2Q5 FP64, warmup %d, steps %d" % (a.lx, a.ly, a.warmup, a.steps),
+          file=sys.stderr, flush=True)
+    for w in range(a.warmup):
+        r = hs.evaluate()
+        print("warmup %d: %.3f s fwd %d adj %d" % (w, r["seconds"], r["fwd_steps"], r["adj_steps"]), file=sys.stderr,
+              flush=True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    total_updates = 0
+    runs = []
+    for s in range(a.steps):
+        r = hs.evaluate()
+        runs.append(r)
+        total_updates += hs.N * (r["fwd_steps"] + r["adj_steps"])
+        print("step %d: %.3f s fwd %d adj %d obj %.6e" % (s, r["seconds"], r["fwd_steps"], r["adj_steps"],
+                                                           r["objective"]), file=sys.stderr, flush=True)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    mlups = total_updates / wall / 1e6
+
+    ms_step = kernel_rate(hs)
+    achieved = BYTES_FWD_Q9Q5 * hs.N / (ms_step * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_collide_fused<D2Q9,D2Q5,ADBody> + corner fix-up, per fused forward step",
+                "bytes_per_launch": BYTES_FWD_Q9Q5 * hs.N, "us_per_launch": round(ms_step * 1e3, 3),
+                "note": "working set ~20 MB per step: resident in the 256 MB Infinity Cache"}
+    extra = {}
+    if not a.no_big:
+        ms_big, nbig = big_grid_roofline(ctx)
+        ach_big = BYTES_FWD_Q9Q5 * nbig / (ms_big * 1e-3) / 1e9
+        extra["roofline_hbm_4096x4096"] = {"achieved": round(ach_big, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                           "frac": round(ach_big / HBM_PEAK_GBS, 4),
+                                           "us_per_launch": round(ms_big * 1e3, 1),
+                                           "mlups": round(nbig / (ms_big * 1e-3) / 1e6, 1)}
+    # like-for-like with the reference's lattice pairing (D2Q9 advection-diffusion)
+    hs9 = Heatsink2D(ctx, lx=a.lx, ly=a.ly, ad_kind=D2Q9, max_steps=a.max_steps)
+    r9 = hs9.evaluate()
+    extra["mlups_d2q9_d2q9"] = round(hs9.N * (r9["fwd_steps"] + r9["adj_steps"]) / r9["seconds"] / 1e6, 1)
+    cpu = None if a.no_cpu else cpu_baseline(hs9)
+    hs9.close()
+
+    out = {
+        "metric": "MLUPS (fwd+adjoint) & % HBM peak; wall-time to steady, warm vs cold",
+        "value": round(mlups, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(wall / a.steps * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded uniform design field, density-filtered)",
+        "config": {"workload": "2D heatsink design evaluation (forward+adjoint to 1e-6, sensitivity), cold start",
+                   "grid": [a.lx, a.ly], "lattices": "D2Q9 NS + D2Q5 AD", "precision": "fp64",
+                   "fwd_steps": runs[-1]["fwd_steps"], "adj_steps": runs[-1]["adj_steps"],
+                   "wall_s_to_steady_cold": round(runs[-1]["seconds"], 3)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    out.update(extra)
+    hs.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -21,6 +21,10 @@
  *    y-faces lx entries (index i).  3D: x-faces ly*lz (j + ly*k), y-faces lx*lz (i + lx*k),
  *    z-faces lx*ly (i + lx*j).  lbm_lattice_nface() gives the total length.  A zero mask/type
  *    entry skips that node, like a lambda returning 0/false.
+ *  - issave of the collide calls: 0 writes no fields, 1 writes every node (the reference's
+ *    `issave = true`), 2 writes the fields of boundary (face) nodes only -- all the next step's
+ *    boundary calls read, since they read the saved velocity at their own node
+ *    (advection.h:107) -- and no snapshot.  Values written are the same in modes 1 and 2.
  *  - Populations are kept on the device in plane order (population c of node idx at c*N + idx).
  *    lbm_lattice_upload/download move them in that order; the *_aos variants use the
  *    reference's f0[N] + f[(nc-1)*idx + c-1] layout (d2q9.h:88-90).
@@ -221,6 +225,16 @@ int lbm_aad_macro_brinkman_collide_force_convection_p(
     double *ip, double *iux, double *iuy, double *iuz, double *imx, double *imy, double *imz, const double *alpha,
     double viscosity, lbm_lattice *q, const double *tem, double *item, double *iqx, double *iqy, double *iqz,
     const double *diffusivity, int issave, double *ig);
+
+/* ---- HIP graphs of the context stream (no reference counterpart; the reference loops on the
+ * host).  Calls between begin and end are captured, not run; they must not return host data.
+ * Trap: a captured sequence must leave every buffer binding as it found it (e.g. an even number
+ * of steps when fields alternate). ---- */
+typedef struct lbm_graph lbm_graph;
+int lbm_graph_begin(lbm_context *ctx);
+int lbm_graph_end(lbm_context *ctx, lbm_graph **out);
+int lbm_graph_launch(lbm_graph *graph);
+int lbm_graph_destroy(lbm_graph *graph);
 
 #ifdef __cplusplus
 }

@@ -9,7 +9,9 @@
  * exactly what a driver's lambda would.  Used to write the golden fixtures under tests/golden/
  * and, when present, as a live cross-check of the oracle.
  */
+#include <chrono>
 #include <cstring>
+#include <vector>
 
 #include "src/particle/d2q9.h"
 #include "src/equation/navierstokes.h"
@@ -225,6 +227,80 @@ void ref2_density_filter(int lx, int ly, int lz, double R, const double *v, doub
                  std::vector<double> r = DensityFilter::GetFilteredValue(p, R, in);
                  std::memcpy(out, r.data(), sizeof(double) * p.nxyz);
              }
+}
+
+
+/* Timed loop of the reference's own functions for the CPU baseline: the forced-convection
+ * heatsink step of production/heatsink.cpp:142-174 (forward) and :183-212 (adjoint) with the
+ * ForceConvection collides (advection.h:814, adjointadvection.h:997) on D2Q9 + D2Q9, the
+ * boundary set of test/nsadsens.cpp:79-131, fixed step counts (no residual test).  Node arrays
+ * give the lambdas: walls, inlet, outlet, flux (0/1), uin (inlet velocity), objective edge.
+ * Returns wall seconds of each phase in secs[0..1]. */
+void ref2_heatsink_steps(int lx, int ly, int nt_fwd, int nt_adj, double nu, const double *alpha, const double *diff,
+                         const int *walls, const int *inlet, const int *outlet, const int *flux, const int *objedge,
+                         const double *uin, double *secs) {
+    D2Q9<double> pf(lx, ly), pg(lx, ly);
+    int N = pf.nxyz;
+    std::vector<double> rho(N, 1.0), ux(N, 0.0), uy(N, 0.0), uxp(N, 0.0), uyp(N, 0.0), tem(N, 0.0), qx(N, 0.0),
+        qy(N, 0.0), qxp(N, 0.0), qyp(N, 0.0), g(9 * N, 0.0), ig(9 * N, 0.0);
+    std::vector<double> ip(N, 0.0), iux(N, 0.0), iuy(N, 0.0), iuxp(N, 0.0), iuyp(N, 0.0), imx(N, 0.0), imy(N, 0.0),
+        item(N, 0.0), iqx(N, 0.0), iqy(N, 0.0), iqxp(N, 0.0), iqyp(N, 0.0);
+    N2 bw{lx, walls}, bi{lx, inlet}, bo{lx, outlet}, bq{lx, flux}, be{lx, objedge};
+    V2 vin{lx, uin};
+    auto zero = [](int, int) { return 0.0; };
+    auto one = [](int, int) { return 1.0; };
+    std::vector<int> otype(N);
+    for (int n = 0; n < N; ++n) otype[n] = outlet[n] ? 2 : 0;
+    N2 bot{lx, otype.data()};
+    double *pux = ux.data(), *puy = uy.data(), *puxp = uxp.data(), *puyp = uyp.data();
+    double *pqx = qx.data(), *pqy = qy.data(), *pqxp = qxp.data(), *pqyp = qyp.data();
+    auto t0 = std::chrono::steady_clock::now();
+    NS::InitialCondition(pf, rho.data(), pux, puy);
+    AD::InitialCondition(pg, tem.data(), pux, puy);
+    for (int t = 1; t <= nt_fwd; ++t) {
+        AD::MacroBrinkmanCollideForceConvection(pf, rho.data(), pux, puy, alpha, nu, pg, tem.data(), pqx, pqy, diff,
+                                                true, g.data());
+        pf.Stream();
+        pg.Stream();
+        pf.BoundaryCondition(bw);
+        NS::BoundaryConditionSetU(pf, vin, zero, bi);
+        NS::BoundaryConditionSetRho(pf, one, zero, bo);
+        pf.SmoothCorner();
+        AD::BoundaryConditionSetT(pg, zero, pux, puy, bi);
+        AD::BoundaryConditionSetQ(pg, zero, pux, puy, diff, bq);
+        pg.SmoothCorner();
+        std::swap(pux, puxp);
+        std::swap(puy, puyp);
+        std::swap(pqx, pqxp);
+        std::swap(pqy, pqyp);
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    ANS::InitialCondition(pf, pux, puy, ip.data(), iux.data(), iuy.data());
+    AAD::InitialCondition(pg, pux, puy, item.data(), iqx.data(), iqy.data());
+    double *piux = iux.data(), *piuy = iuy.data(), *piuxp = iuxp.data(), *piuyp = iuyp.data();
+    double *piqx = iqx.data(), *piqy = iqy.data(), *piqxp = iqxp.data(), *piqyp = iqyp.data();
+    for (int t = 1; t <= nt_adj; ++t) {
+        AAD::MacroBrinkmanCollideForceConvection(pf, rho.data(), pux, puy, ip.data(), piux, piuy, imx.data(),
+                                                 imy.data(), alpha, nu, pg, tem.data(), item.data(), piqx, piqy, diff,
+                                                 true, ig.data());
+        pf.iStream();
+        pg.iStream();
+        AAD::iBoundaryConditionSetT(pg, pux, puy, bi);
+        AAD::iBoundaryConditionSetQ(pg, pux, puy, bq);
+        AAD::iBoundaryConditionSetQ(pg, pux, puy, be, 1.0);
+        pg.SmoothCorner();
+        pf.iBoundaryCondition(bw);
+        ANS::iBoundaryConditionSetU(pf, vin, zero, bi);
+        AAD::iBoundaryConditionSetRho(pf, pg, rho.data(), pux, puy, tem.data(), bot);
+        pf.SmoothCorner();
+        std::swap(piux, piuxp);
+        std::swap(piuy, piuyp);
+        std::swap(piqx, piqxp);
+        std::swap(piqy, piqyp);
+    }
+    auto t2 = std::chrono::steady_clock::now();
+    secs[0] = std::chrono::duration<double>(t1 - t0).count();
+    secs[1] = std::chrono::duration<double>(t2 - t1).count();
 }
 
 }  // extern "C"

@@ -97,6 +97,10 @@ def load():
         "lbm_program_bind_fields": [VP, I] + [VP] * 6,
         "lbm_program_num_field_sets": [VP],
         "lbm_program_fusable": [VP],
+        "lbm_graph_begin": [VP],
+        "lbm_graph_end": [VP, ctypes.POINTER(VP)],
+        "lbm_graph_launch": [VP],
+        "lbm_graph_destroy": [VP],
         "lbm_ns_macro_collide_p": [VP] * 6 + [D, I],
         "lbm_ns_macro_brinkman_collide_p": [VP] * 6 + [D, VP, I],
         "lbm_ad_macro_brinkman_collide_force_convection_p": [VP] * 7 + [D] + [VP] * 6 + [I, VP, VP],

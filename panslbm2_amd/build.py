@@ -18,8 +18,12 @@ SOURCES = ["device.hip", "capi.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -ffp-contract=off: no FMA contraction, so FP64 kernels reproduce the reference's plain
 # x86-64 arithmetic.  Packed FP32 instructions are disabled for all device code.
+# -simplifycfg-sink-common=false: the per-face boundary operators write different populations;
+# sinking those stores into one store through a selected address forces the node's populations
+# into scratch memory.  Without the sinking they stay in registers.
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
-         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-command-line-argument"]
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-command-line-argument",
+         "-mllvm", "-simplifycfg-sink-common=false"]
 
 
 def _run(cmd, cwd=None):

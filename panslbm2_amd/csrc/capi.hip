@@ -1055,6 +1055,43 @@ int lbm_program_bind_fields(lbm_program *P, int set, const double *rho, const do
     if (diff) f.diff = diff;
     return LBM_OK;
 }
+struct lbm_graph {
+    lbm_context *ctx;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+};
+int lbm_graph_begin(lbm_context *C) {
+    if (!C) return fail(LBM_ERR_ARG, "null context");
+    if (C->recording) return fail(LBM_ERR_STATE, "graph capture while recording a program");
+    HIPCHK(hipStreamBeginCapture(C->stream, hipStreamCaptureModeRelaxed));
+    return LBM_OK;
+}
+int lbm_graph_end(lbm_context *C, lbm_graph **out) {
+    if (!C || !out) return fail(LBM_ERR_ARG, "null argument");
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamEndCapture(C->stream, &g));
+    hipGraphExec_t ex = nullptr;
+    hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return fail(LBM_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    *out = new lbm_graph{C, g, ex};
+    return LBM_OK;
+}
+int lbm_graph_launch(lbm_graph *G) {
+    if (!G) return fail(LBM_ERR_ARG, "null graph");
+    HIPCHK(hipGraphLaunch(G->exec, G->ctx->stream));
+    return LBM_OK;
+}
+int lbm_graph_destroy(lbm_graph *G) {
+    if (!G) return LBM_OK;
+    (void)hipStreamSynchronize(G->ctx->stream);
+    (void)hipGraphExecDestroy(G->exec);
+    (void)hipGraphDestroy(G->graph);
+    delete G;
+    return LBM_OK;
+}
 int lbm_program_num_field_sets(const lbm_program *P) { return P ? (int)P->fsets.size() : fail(LBM_ERR_ARG, "null"); }
 int lbm_program_fusable(const lbm_program *P) { return P ? (P->ordered_ok ? 1 : 0) : fail(LBM_ERR_ARG, "null"); }
 

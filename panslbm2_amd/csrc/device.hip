@@ -77,14 +77,14 @@ struct NSBody {
     const double *alpha;
     double omega, iomega;
     int save, brinkman;
-    __device__ __forceinline__ void operator()(double *f, double *, int n) const {
+    __device__ __forceinline__ void operator()(double *f, double *, int n, bool bnd) const {
         double r, vx, vy, vz, feq[LA::nc];
         ns_macro<LA>(f, r, vx, vy, vz);
         if (brinkman) {
             ns_brinkman<LA>(f, r, vx, vy, vz, alpha[n]);
             ns_macro<LA>(f, r, vx, vy, vz);
         }
-        if (save) {
+        if (save == 1 || (save == 2 && bnd)) {
             rho[n] = r;
             ux[n] = vx;
             uy[n] = vy;
@@ -102,7 +102,7 @@ struct ADBody {
     const double *alpha, *diff, *heat;
     double omegaf, iomegaf;
     int save, N;
-    __device__ __forceinline__ void operator()(double *f, double *g, int n) const {
+    __device__ __forceinline__ void operator()(double *f, double *g, int n, bool bnd) const {
         double omegag = 1.0 / (3.0 * diff[n] + 0.5), iomegag = 1.0 - omegag;
         double r, vx, vy, vz, T, hx, hy, hz;
         ns_macro<LA>(f, r, vx, vy, vz);
@@ -117,7 +117,7 @@ struct ADBody {
         }
         ns_macro<LA>(f, r, vx, vy, vz);
         ad_macro<LB>(g, vx, vy, vz, omegag, T, hx, hy, hz);
-        if (save) {
+        if (save == 1 || (save == 2 && bnd)) {
             rho[n] = r;
             ux[n] = vx;
             uy[n] = vy;
@@ -128,7 +128,7 @@ struct ADBody {
                 uz[n] = vz;
                 qz[n] = hz;
             }
-            if (gsnap) store_node<LB>(gsnap, N, n, g);
+            if (gsnap && save == 1) store_node<LB>(gsnap, N, n, g);
         }
         {
             double feq[LA::nc];
@@ -150,13 +150,13 @@ struct ANSBody {
     double *ip, *iux, *iuy, *iuz, *imx, *imy, *imz;
     double omega, iomega;
     int save;
-    __device__ __forceinline__ void operator()(double *f, double *, int n) const {
+    __device__ __forceinline__ void operator()(double *f, double *, int n, bool bnd) const {
         double vx = ux[n], vy = uy[n], vz = LA::nd == 3 ? uz[n] : 0.0;
         double a, bx, by, bz, mx, my, mz, feq[LA::nc];
         ans_macro<LA>(f, vx, vy, vz, a, bx, by, bz, mx, my, mz);
         ans_brinkman<LA>(f, rho[n], vx, vy, vz, mx, my, mz, alpha[n]);
         ans_macro<LA>(f, vx, vy, vz, a, bx, by, bz, mx, my, mz);
-        if (save) {
+        if (save == 1 || (save == 2 && bnd)) {
             ip[n] = a;
             iux[n] = bx;
             iuy[n] = by;
@@ -179,7 +179,7 @@ struct AADBody {
     double *ip, *iux, *iuy, *iuz, *imx, *imy, *imz, *item, *iqx, *iqy, *iqz, *igsnap;
     double omegaf, iomegaf;
     int save, N;
-    __device__ __forceinline__ void operator()(double *f, double *g, int n) const {
+    __device__ __forceinline__ void operator()(double *f, double *g, int n, bool bnd) const {
         double omegag = 1.0 / (3.0 * diff[n] + 0.5), iomegag = 1.0 - omegag;
         double vx = ux[n], vy = uy[n], vz = LA::nd == 3 ? uz[n] : 0.0;
         double a, bx, by, bz, mx, my, mz, it, hx, hy, hz;
@@ -187,7 +187,7 @@ struct AADBody {
         aad_macro<LB>(g, it, hx, hy, hz);
         aad_brinkman<LA>(f, rho[n], vx, vy, vz, mx, my, mz, tem[n], hx, hy, hz, omegag, alpha[n]);
         ans_macro<LA>(f, vx, vy, vz, a, bx, by, bz, mx, my, mz);
-        if (save) {
+        if (save == 1 || (save == 2 && bnd)) {
             ip[n] = a;
             iux[n] = bx;
             iuy[n] = by;
@@ -201,7 +201,7 @@ struct AADBody {
                 imz[n] = mz;
                 iqz[n] = hz;
             }
-            if (igsnap) store_node<LB>(igsnap, N, n, g);
+            if (igsnap && save == 1) store_node<LB>(igsnap, N, n, g);
         }
         {
             double feq[LA::nc];
@@ -223,13 +223,16 @@ struct AADBody {
 // corners), whose state averages neighbours' post-boundary states.
 // ------------------------------------------------------------------------------------------
 template <class LA, class LB, class Body>
-__global__ void __launch_bounds__(kBlock) k_collide_inplace(double *a, double *b, int N, Body body) {
+__global__ void __launch_bounds__(kBlock) k_collide_inplace(double *a, double *b, Grid g, Body body) {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
+    int N = g.N;
     if (n >= N) return;
+    int i, j, k;
+    coords(n, g, i, j, k);
     double fa[LA::nc], fb[LB::nc > 0 ? LB::nc : 1];
     load_node<LA>(a, N, n, fa);
     load_node<LB>(b, N, n, fb);
-    body(fa, fb, n);
+    body(fa, fb, n, nboundary(i, j, k, g, LA::nd) > 0);
     store_node<LA>(a, N, n, fa);
     store_node<LB>(b, N, n, fb);
 }
@@ -241,14 +244,13 @@ __global__ void __launch_bounds__(kBlock) k_collide_fused(const double *__restri
     int tile = xcd_tile(blockIdx.x, gridDim.x);
     int n = tile * blockDim.x + threadIdx.x;
     if (n >= g.N) return;
-    if (any_smooth) {
-        int i, j, k;
-        coords(n, g, i, j, k);
-        if (nboundary(i, j, k, g, LA::nd) >= 2) return;
-    }
+    int i, j, k;
+    coords(n, g, i, j, k);
+    int nb = nboundary(i, j, k, g, LA::nd);
+    if (any_smooth && nb >= 2) return;
     double fa[LA::nc], fb[LB::nc > 0 ? LB::nc : 1];
     node_state<LA, LB>(sa, sb, g, pv, n, fa, fb);
-    body(fa, fb, n);
+    body(fa, fb, n, nb > 0);
     store_node<LA>(da, g.N, n, fa);
     store_node<LB>(db, g.N, n, fb);
 }
@@ -323,7 +325,7 @@ __global__ void __launch_bounds__(kBlock) k_collide_fixup(const double *__restri
     int n = nodes[e];
     double fa[LA::nc], fb[LB::nc > 0 ? LB::nc : 1];
     smoothed_state<LA, LB, LEVEL>(sa, sb, g, pv, n, fa, fb);
-    body(fa, fb, n);
+    body(fa, fb, n, true);
     store_node<LA>(da, g.N, n, fa);
     store_node<LB>(db, g.N, n, fb);
 }
@@ -626,7 +628,7 @@ static hipError_t launch_collide(const CollideLaunch &cl, Make &&make) {
         Body<LA, LB> body = make(la, lb);
         const Grid &g = cl.g;
         if (!cl.pv) {
-            LBM_LAUNCH((k_collide_inplace<LA, LB, Body<LA, LB>>), nblocks(g.N), cl.stream, cl.da, cl.db, g.N, body);
+            LBM_LAUNCH((k_collide_inplace<LA, LB, Body<LA, LB>>), nblocks(g.N), cl.stream, cl.da, cl.db, g, body);
             return hipGetLastError();
         }
         const ProgView &pv = *cl.pv;

@@ -124,9 +124,16 @@ __device__ __forceinline__ void apply_on(const Op &op, double *f, const double *
     }
 }
 
+// Field set k of a program.  Selected with constant indices: a runtime index into the by-value
+// kernel argument would make the compiler copy the argument to private (scratch) memory.
+__device__ __forceinline__ FieldSet field_set(const ProgView &pv, int k) {
+    static_assert(kMaxFieldSets == 4, "field_set covers four sets");
+    return k == 0 ? pv.fs[0] : (k == 1 ? pv.fs[1] : (k == 2 ? pv.fs[2] : pv.fs[3]));
+}
+
 template <class LA, class LB>
 __device__ __forceinline__ void apply_op(const Op &op, double *a, double *b, const ProgView &pv, int n) {
-    const FieldSet &s = pv.fs[op.fset];
+    const FieldSet s = field_set(pv, op.fset);
     if ((op.lat ^ pv.latswap) == 0) apply_on<LA, LB>(op, a, b, s, n);
     else if constexpr (LB::nc > 0) apply_on<LB, LA>(op, b, a, s, n);
 }

@@ -1,0 +1,296 @@
+"""2D forced-convection heatsink design evaluation on the HIP path: forward steady solve, adjoint
+steady solve, design sensitivity.
+
+Step order and convergence test follow production/heatsink.cpp:136-232 (collide with saving,
+Residual every `check` steps on the step-to-step change of u and q, stream, boundary calls,
+SmoothCorner, swap of the saved-velocity arrays), with the forced-convection collides of
+advection.h:814 and adjointadvection.h:997 and the boundary set of test/nsadsens.cpp:79-131.
+
+Workload (one design evaluation):
+  * NS on D2Q9 with Brinkman drag alpha = alpha_max/(ly-1)*qf*(1-gamma)/(qf+gamma), the
+    reference's scaling (heatsink.cpp:110; with alpha_max = 1e4 unscaled the reference's adjoint
+    Brinkman terms diverge within a few hundred steps); AD on D2Q5 (or on
+    D2Q9, the reference's own pairing) with diffusivity D(gamma) interpolated between fluid
+    D_f = nu/Pr and solid 10*D_f (heatsink.cpp:46,109) and a volumetric heat source
+    Q*(1-gamma) in the solid.
+  * Parabolic inlet (u_max) on x = 0, pressure outlet (rho = 1) on x = lx-1, bounce-back walls
+    on y = 0 and y = ly-1; T = 0 at the inlet, zero heat flux on walls and outlet.
+  * gamma: seeded uniform[0,1] smoothed by the density filter (densityfilter.h:512-515).
+  * Objective: mean temperature on the bottom-wall segment x < lx/2 (heatsink.cpp:215-226),
+    injected into the adjoint through the flux condition with eps = 1 (heatsink.cpp:202).
+
+Every time step is one fused kernel pass (pull stream + boundary program + corner smoothing +
+collide); 100-step chunks between residual checks run as one HIP graph.
+"""
+import ctypes
+import time
+
+import numpy as np
+
+from . import D2Q5, D2Q9, NC, Context, Lattice, check, hptr, to_face
+
+D = ctypes.c_double
+VP = ctypes.c_void_p
+
+
+class Heatsink2D:
+    def __init__(self, ctx=None, lx=200, ly=400, ad_kind=D2Q5, seed=0, nu=0.1, pr=6.0, umax=0.05, heat=1e-3,
+                 alpha_max=1e4, qf=0.01, qg=1.0, filter_radius=3.0, eps=1e-6, check_every=100, max_steps=400000,
+                 use_graphs=True):
+        self.ctx = ctx or Context(0)
+        self.L = self.ctx.lib
+        self.lx, self.ly, self.N = lx, ly, lx * ly
+        self.ad_kind = ad_kind
+        self.nu, self.eps, self.check_every, self.max_steps = nu, eps, check_every, max_steps
+        assert check_every % 2 == 0, "chunks must leave the alternating buffers where they found them"
+        self.use_graphs = use_graphs
+        N = self.N
+        c = self.ctx
+        i, j = np.meshgrid(np.arange(lx), np.arange(ly), indexing="xy")
+        i, j = i.reshape(-1), j.reshape(-1)
+        self.i, self.j = i, j
+
+        # design field: seeded uniform[0,1], density filter on the device
+        rng = np.random.default_rng(seed)
+        raw = c.array((N,), data=rng.uniform(0.0, 1.0, N))
+        gam = c.array((N,), fill=0.0)
+        check(self.L.lbm_density_filter(c.h, lx, ly, 1, D(filter_radius), raw.ptr, gam.ptr))
+        s = gam.download()
+        self.gamma = s
+        dfl = nu / pr
+        dso = 10.0 * dfl
+        coef = {
+            "alpha": alpha_max / (ly - 1.0) * qf * (1.0 - s) / (s + qf),
+            "dads": -alpha_max / (ly - 1.0) * qf * (1.0 + qf) / (s + qf) ** 2,
+            "diff": dso + (dfl - dso) * s * (1.0 + qg) / (s + qg),
+            "dkds": (dfl - dso) * qg * (1.0 + qg) / (s + qg) ** 2,
+            "heat": heat * (1.0 - s),
+        }
+        self.d = {k: c.array((N,), data=v) for k, v in coef.items()}
+        # fields: saved values alternate between two buffers per quantity (u at t and t-1)
+        for k in ("rho", "tem", "ip", "imx", "imy", "item", "dfds"):
+            self.d[k] = c.array((N,), fill=0.0)
+        for k in ("ux", "uy", "qx", "qy", "iux", "iuy", "iqx", "iqy"):
+            self.d[k + "0"] = c.array((N,), fill=0.0)
+            self.d[k + "1"] = c.array((N,), fill=0.0)
+        self.d["g"] = c.array((NC[ad_kind], N), fill=0.0)
+        self.d["ig"] = c.array((NC[ad_kind], N), fill=0.0)
+        self.d["one"] = c.array((N,), fill=1.0)
+        self.d["zero"] = c.array((N,), fill=0.0)
+
+        self.pf = Lattice(c, D2Q9, lx, ly)
+        self.pg = Lattice(c, ad_kind, lx, ly)
+
+        # boundary lambdas, tabulated
+        H = ly - 1.0
+        uin = 4.0 * umax * j * (H - j) / H ** 2
+        inlet = ((i == 0) & (j > 0) & (j < ly - 1)).astype(np.int32)
+        outlet = ((i == lx - 1) & (j > 0) & (j < ly - 1)).astype(np.int32)
+        walls = ((j == 0) | (j == ly - 1)).astype(np.int32)
+        flux = (walls | outlet).astype(np.int32)
+        self.obj_nodes = np.nonzero((j == 0) & (i < lx // 2))[0].astype(np.int32)
+        objedge = ((j == 0) & (i < lx // 2)).astype(np.int32)
+        F = lambda kind, a, dt=np.float64: to_face(kind, lx, ly, 1, a, dt)
+        self.faces = dict(
+            walls_f=F(D2Q9, walls, np.int32), inlet_f=F(D2Q9, inlet, np.int32), outlet_f=F(D2Q9, outlet, np.int32),
+            uin_f=F(D2Q9, uin), zero_f=F(D2Q9, np.zeros(N)), one_f=F(D2Q9, np.ones(N)),
+            inlet_g=F(ad_kind, inlet, np.int32), flux_g=F(ad_kind, flux, np.int32), obj_g=F(ad_kind, objedge, np.int32),
+            zero_g=F(ad_kind, np.zeros(N)), outlet_type=F(D2Q9, outlet * 2, np.int32),
+        )
+        self.fwd_prog = self._record_forward()
+        self.adj_prog = None
+        self._graphs = {}
+
+    # ---- programs ---------------------------------------------------------------------------
+    def _record_forward(self):
+        L, c, d, fc = self.L, self.ctx, self.d, self.faces
+        check(L.lbm_record_begin(c.h))
+        check(L.lbm_stream(self.pf.h))
+        check(L.lbm_stream(self.pg.h))
+        check(L.lbm_bc(self.pf.h, hptr(fc["walls_f"])))
+        check(L.lbm_ns_set_u(self.pf.h, hptr(fc["inlet_f"]), hptr(fc["uin_f"]), hptr(fc["zero_f"]), hptr(fc["zero_f"])))
+        check(L.lbm_ns_set_rho(self.pf.h, hptr(fc["outlet_f"]), hptr(fc["one_f"]), hptr(fc["zero_f"]),
+                               hptr(fc["zero_f"])))
+        check(L.lbm_smooth_corner(self.pf.h))
+        check(L.lbm_ad_set_t(self.pg.h, hptr(fc["inlet_g"]), hptr(fc["zero_g"]), d["ux0"].ptr, d["uy0"].ptr, None))
+        check(L.lbm_ad_set_q(self.pg.h, hptr(fc["flux_g"]), hptr(fc["zero_g"]), d["ux0"].ptr, d["uy0"].ptr, None,
+                             d["diff"].ptr, D(0.0)))
+        check(L.lbm_smooth_corner(self.pg.h))
+        prog = VP()
+        check(L.lbm_record_end(c.h, ctypes.byref(prog)))
+        assert L.lbm_program_fusable(prog) == 1
+        return prog
+
+    def _record_adjoint(self, b):
+        """b: buffer holding the converged forward velocity."""
+        L, c, d, fc = self.L, self.ctx, self.d, self.faces
+        ux, uy = d["ux%d" % b].ptr, d["uy%d" % b].ptr
+        check(L.lbm_record_begin(c.h))
+        check(L.lbm_istream(self.pf.h))
+        check(L.lbm_istream(self.pg.h))
+        check(L.lbm_aad_iset_t(self.pg.h, hptr(fc["inlet_g"]), ux, uy, None))
+        check(L.lbm_aad_iset_q(self.pg.h, hptr(fc["flux_g"]), ux, uy, None, D(0.0)))
+        check(L.lbm_aad_iset_q(self.pg.h, hptr(fc["obj_g"]), ux, uy, None, D(1.0)))
+        check(L.lbm_smooth_corner(self.pg.h))
+        check(L.lbm_ibc(self.pf.h, hptr(fc["walls_f"])))
+        check(L.lbm_ans_iset_u(self.pf.h, hptr(fc["inlet_f"]), hptr(fc["uin_f"]), hptr(fc["zero_f"]),
+                               hptr(fc["zero_f"]), D(0.0), 1))
+        check(L.lbm_aad_iset_rho(self.pf.h, self.pg.h, hptr(fc["outlet_type"]), d["rho"].ptr, ux, uy, None,
+                                 d["tem"].ptr, D(0.0)))
+        check(L.lbm_smooth_corner(self.pf.h))
+        prog = VP()
+        check(L.lbm_record_end(c.h, ctypes.byref(prog)))
+        assert L.lbm_program_fusable(prog) == 1
+        return prog
+
+    # ---- steps ------------------------------------------------------------------------------
+    def _save_mode(self, t, adjoint):
+        r = t % self.check_every
+        if r == 0 or r == self.check_every - 1:
+            return 1
+        return 0 if adjoint else 2
+
+    def _fwd_step(self, t):
+        """Forward step t (t >= 1): saves into buffer t % 2; boundary calls read buffer (t-1) % 2."""
+        L, d = self.L, self.d
+        b, pb = t % 2, (t - 1) % 2
+        prog = None
+        if t > 1:
+            prog = self.fwd_prog
+            for s in range(L.lbm_program_num_field_sets(prog)):
+                check(L.lbm_program_bind_fields(prog, s, None, d["ux%d" % pb].ptr, d["uy%d" % pb].ptr, None, None,
+                                                None))
+        mode = self._save_mode(t, False)
+        check(L.lbm_ad_macro_brinkman_collide_force_convection_p(
+            prog, self.pf.h, d["rho"].ptr, d["ux%d" % b].ptr, d["uy%d" % b].ptr, None, d["alpha"].ptr, D(self.nu),
+            self.pg.h, d["tem"].ptr, d["qx%d" % b].ptr, d["qy%d" % b].ptr, None, d["diff"].ptr, mode, d["g"].ptr,
+            d["heat"].ptr))
+
+    def _adj_step(self, t, fb):
+        L, d = self.L, self.d
+        b = t % 2
+        prog = self.adj_prog if t > 1 else None
+        mode = self._save_mode(t, True)
+        check(L.lbm_aad_macro_brinkman_collide_force_convection_p(
+            prog, self.pf.h, d["rho"].ptr, d["ux%d" % fb].ptr, d["uy%d" % fb].ptr, None, d["ip"].ptr,
+            d["iux%d" % b].ptr, d["iuy%d" % b].ptr, None, d["imx"].ptr, d["imy"].ptr, None, d["alpha"].ptr, D(self.nu),
+            self.pg.h, d["tem"].ptr, d["item"].ptr, d["iqx%d" % b].ptr, d["iqy%d" % b].ptr, None, d["diff"].ptr, mode,
+            d["ig"].ptr))
+
+    def _chunk(self, key, t0, step):
+        """Steps t0+1 .. t0+check_every, captured once as a HIP graph (t0 even, so bindings repeat)."""
+        if not self.use_graphs:
+            for t in range(t0 + 1, t0 + self.check_every + 1):
+                step(t)
+            return
+        g = self._graphs.get(key)
+        if g is None:
+            check(self.L.lbm_graph_begin(self.ctx.h))
+            for t in range(t0 + 1, t0 + self.check_every + 1):
+                step(t)
+            g = VP()
+            check(self.L.lbm_graph_end(self.ctx.h, ctypes.byref(g)))
+            self._graphs[key] = g
+        check(self.L.lbm_graph_launch(g))
+
+    def _residual(self, a0, a1, b0, b1):
+        r = ctypes.c_double()
+        check(self.L.lbm_residual(self.ctx.h, 2, a0.ptr, a1.ptr, None, b0.ptr, b1.ptr, None, self.N, ctypes.byref(r)))
+        return r.value
+
+    def forward(self):
+        L, d = self.L, self.d
+        for k in ("ux0", "uy0"):
+            d[k].fill(0.0)
+        d["tem"].fill(0.0)
+        check(L.lbm_ns_initial_condition(self.pf.h, d["one"].ptr, d["ux0"].ptr, d["uy0"].ptr, None))
+        check(L.lbm_ad_initial_condition(self.pg.h, d["tem"].ptr, d["ux0"].ptr, d["uy0"].ptr, None))
+        ce = self.check_every
+        t = 0
+        while t < self.max_steps:
+            if t == 0:
+                for s in range(1, ce + 1):
+                    self._fwd_step(s)
+            else:
+                self._chunk("fwd", t, self._fwd_step)
+            t += ce
+            b, pb = t % 2, (t - 1) % 2
+            ru = self._residual(d["ux%d" % b], d["uy%d" % b], d["ux%d" % pb], d["uy%d" % pb])
+            rq = self._residual(d["qx%d" % b], d["qy%d" % b], d["qx%d" % pb], d["qy%d" % pb])
+            if ru < self.eps and rq < self.eps:
+                break
+        self.fwd_steps = t
+        self.fwd_buf = t % 2
+        self.fwd_res = (ru, rq)
+        return t
+
+    def adjoint(self):
+        L, d = self.L, self.d
+        fb = self.fwd_buf
+        if self.adj_prog is None or getattr(self, "_adj_fb", None) != fb:
+            if self.adj_prog is not None:
+                L.lbm_program_destroy(self.adj_prog)
+            self.adj_prog = self._record_adjoint(fb)
+            self._adj_fb = fb
+            self._graphs.pop("adj", None)
+        for k in ("ip", "item"):
+            d[k].fill(0.0)
+        for k in ("iux0", "iuy0", "iqx0", "iqy0"):
+            d[k].fill(0.0)
+        check(L.lbm_ans_initial_condition(self.pf.h, d["ux%d" % fb].ptr, d["uy%d" % fb].ptr, None, d["ip"].ptr,
+                                          d["iux0"].ptr, d["iuy0"].ptr, None))
+        check(L.lbm_aad_initial_condition(self.pg.h, d["ux%d" % fb].ptr, d["uy%d" % fb].ptr, None, d["item"].ptr,
+                                          d["iqx0"].ptr, d["iqy0"].ptr, None))
+        ce = self.check_every
+        step = lambda s: self._adj_step(s, fb)
+        t = 0
+        while t < self.max_steps:
+            if t == 0:
+                for s in range(1, ce + 1):
+                    step(s)
+            else:
+                self._chunk("adj", t, step)
+            t += ce
+            b, pb = t % 2, (t - 1) % 2
+            ru = self._residual(d["iux%d" % b], d["iuy%d" % b], d["iux%d" % pb], d["iuy%d" % pb])
+            rq = self._residual(d["iqx%d" % b], d["iqy%d" % b], d["iqx%d" % pb], d["iqy%d" % pb])
+            if ru < self.eps and rq < self.eps:
+                break
+        self.adj_steps = t
+        self.adj_buf = t % 2
+        self.adj_res = (ru, rq)
+        return t
+
+    def sensitivity(self):
+        """dF/dgamma: Brinkman and diffusivity terms (adjointadvection.h:1461-1475)."""
+        L, d = self.L, self.d
+        fb, ab = self.fwd_buf, self.adj_buf
+        d["dfds"].fill(0.0)
+        check(L.lbm_sensitivity_brinkman_diffusivity(
+            self.pg.h, d["dfds"].ptr, d["ux%d" % fb].ptr, d["uy%d" % fb].ptr, None, d["imx"].ptr, d["imy"].ptr, None,
+            d["dads"].ptr, d["tem"].ptr, d["item"].ptr, d["iqx%d" % ab].ptr, d["iqy%d" % ab].ptr, None, d["g"].ptr,
+            d["ig"].ptr, d["diff"].ptr, d["dkds"].ptr))
+        out = ctypes.c_double()
+        check(L.lbm_gather_sum(self.ctx.h, d["tem"].ptr, hptr(self.obj_nodes), len(self.obj_nodes), ctypes.byref(out)))
+        self.objective = out.value / len(self.obj_nodes)
+        return self.objective
+
+    def evaluate(self):
+        """One design evaluation; returns wall seconds and step counts."""
+        t0 = time.perf_counter()
+        self.forward()
+        self.adjoint()
+        self.sensitivity()
+        self.ctx.sync()
+        dt = time.perf_counter() - t0
+        return {"seconds": dt, "fwd_steps": self.fwd_steps, "adj_steps": self.adj_steps,
+                "objective": self.objective, "fwd_residual": self.fwd_res, "adj_residual": self.adj_res}
+
+    def close(self):
+        for g in self._graphs.values():
+            self.L.lbm_graph_destroy(g)
+        self._graphs = {}
+        for p in (self.fwd_prog, self.adj_prog):
+            if p is not None:
+                self.L.lbm_program_destroy(p)
+        self.fwd_prog = self.adj_prog = None

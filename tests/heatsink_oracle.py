@@ -1,0 +1,113 @@
+"""CPU-oracle replica of panslbm2_amd/heatsink2d.py (same workload, same step order, same
+residual test), used to check the HIP driver value for value."""
+import ctypes
+
+import numpy as np
+
+from _oracle import D2Q9, NC, Lattice, lib, ptr, to_face
+
+D = ctypes.c_double
+
+
+def run(lx, ly, ad_kind, seed=0, nu=0.1, pr=6.0, umax=0.05, heat=1e-3, alpha_max=1e4, qf=0.01, qg=1.0,
+        filter_radius=3.0, eps=1e-6, check_every=100, max_steps=400000):
+    O = lib()
+    N = lx * ly
+    i, j = np.meshgrid(np.arange(lx), np.arange(ly), indexing="xy")
+    i, j = i.reshape(-1), j.reshape(-1)
+    rng = np.random.default_rng(seed)
+    raw = rng.uniform(0.0, 1.0, N)
+    s = np.zeros(N)
+    O.ol_density_filter(lx, ly, 1, D(filter_radius), ptr(raw), ptr(s))
+    dfl = nu / pr
+    dso = 10.0 * dfl
+    alpha = alpha_max / (ly - 1.0) * qf * (1.0 - s) / (s + qf)
+    dads = -alpha_max / (ly - 1.0) * qf * (1.0 + qf) / (s + qf) ** 2
+    diff = dso + (dfl - dso) * s * (1.0 + qg) / (s + qg)
+    dkds = (dfl - dso) * qg * (1.0 + qg) / (s + qg) ** 2
+    hsrc = heat * (1.0 - s)
+    H = ly - 1.0
+    uin = 4.0 * umax * j * (H - j) / H ** 2
+    inlet = ((i == 0) & (j > 0) & (j < ly - 1)).astype(np.int32)
+    outlet = ((i == lx - 1) & (j > 0) & (j < ly - 1)).astype(np.int32)
+    walls = ((j == 0) | (j == ly - 1)).astype(np.int32)
+    flux = (walls | outlet).astype(np.int32)
+    objedge = ((j == 0) & (i < lx // 2)).astype(np.int32)
+    Ff = lambda k, a, dt=np.float64: to_face(k, lx, ly, 1, a, dt)
+    pf, pg = Lattice(D2Q9, lx, ly), Lattice(ad_kind, lx, ly)
+    z = lambda: np.zeros(N)
+    rho, tem = np.ones(N), z()
+    U = [[z(), z()], [z(), z()]]  # U[b] = (ux, uy)
+    Q = [[z(), z()], [z(), z()]]
+    g = np.zeros((NC[ad_kind], N))
+    O.ol_ns_init(pf.h, ptr(rho), ptr(U[0][0]), ptr(U[0][1]), None)
+    O.ol_ad_init(pg.h, ptr(tem), ptr(U[0][0]), ptr(U[0][1]), None)
+    t = 0
+    while True:
+        t += 1
+        b = t % 2
+        O.ol_ad_collide_fc(pf.h, ptr(rho), ptr(U[b][0]), ptr(U[b][1]), None, ptr(alpha), D(nu), pg.h, ptr(tem),
+                           ptr(Q[b][0]), ptr(Q[b][1]), None, ptr(diff), 1, ptr(g) if t % check_every == 0 else None,
+                           ptr(hsrc))
+        if t % check_every == 0:
+            ru = O.ol_residual(2, ptr(U[b][0]), ptr(U[b][1]), None, ptr(U[1 - b][0]), ptr(U[1 - b][1]), None, N)
+            rq = O.ol_residual(2, ptr(Q[b][0]), ptr(Q[b][1]), None, ptr(Q[1 - b][0]), ptr(Q[1 - b][1]), None, N)
+            if (ru < eps and rq < eps) or t >= max_steps:
+                break
+        O.ol_stream(pf.h)
+        O.ol_stream(pg.h)
+        O.ol_bc(pf.h, ptr(Ff(D2Q9, walls, np.int32)), 0)
+        O.ol_ns_set_u(pf.h, ptr(Ff(D2Q9, inlet, np.int32)), ptr(Ff(D2Q9, uin)), ptr(Ff(D2Q9, z())), ptr(Ff(D2Q9, z())))
+        O.ol_ns_set_rho(pf.h, ptr(Ff(D2Q9, outlet, np.int32)), ptr(Ff(D2Q9, np.ones(N))), ptr(Ff(D2Q9, z())),
+                        ptr(Ff(D2Q9, z())))
+        O.ol_smooth_corner(pf.h)
+        O.ol_ad_set_t(pg.h, ptr(Ff(ad_kind, inlet, np.int32)), ptr(Ff(ad_kind, z())), ptr(U[b][0]), ptr(U[b][1]), None)
+        O.ol_ad_set_q(pg.h, ptr(Ff(ad_kind, flux, np.int32)), ptr(Ff(ad_kind, z())), ptr(U[b][0]), ptr(U[b][1]), None,
+                      ptr(diff), D(0.0))
+        O.ol_smooth_corner(pg.h)
+    fwd_steps, fb = t, t % 2
+    out = {"fwd_f": pf.get(), "fwd_g": pg.get(), "rho": rho.copy(), "tem": tem.copy(), "ux": U[fb][0].copy(),
+           "uy": U[fb][1].copy(), "g": g.copy(), "fwd_steps": fwd_steps, "fwd_res": (ru, rq)}
+    ux, uy = U[fb]
+    ip, imx, imy, item = z(), z(), z(), z()
+    IU = [[z(), z()], [z(), z()]]
+    IQ = [[z(), z()], [z(), z()]]
+    ig = np.zeros((NC[ad_kind], N))
+    O.ol_ans_init(pf.h, ptr(ux), ptr(uy), None, ptr(ip), ptr(IU[0][0]), ptr(IU[0][1]), None)
+    O.ol_aad_init(pg.h, ptr(ux), ptr(uy), None, ptr(item), ptr(IQ[0][0]), ptr(IQ[0][1]), None)
+    otype = Ff(D2Q9, outlet * 2, np.int32)
+    t = 0
+    while True:
+        t += 1
+        b = t % 2
+        O.ol_aad_collide_fc(pf.h, ptr(rho), ptr(ux), ptr(uy), None, ptr(ip), ptr(IU[b][0]), ptr(IU[b][1]), None,
+                            ptr(imx), ptr(imy), None, ptr(alpha), D(nu), pg.h, ptr(tem), ptr(item), ptr(IQ[b][0]),
+                            ptr(IQ[b][1]), None, ptr(diff), 1, ptr(ig) if t % check_every == 0 else None)
+        if t % check_every == 0:
+            ru = O.ol_residual(2, ptr(IU[b][0]), ptr(IU[b][1]), None, ptr(IU[1 - b][0]), ptr(IU[1 - b][1]), None, N)
+            rq = O.ol_residual(2, ptr(IQ[b][0]), ptr(IQ[b][1]), None, ptr(IQ[1 - b][0]), ptr(IQ[1 - b][1]), None, N)
+            if (ru < eps and rq < eps) or t >= max_steps:
+                break
+        O.ol_istream(pf.h)
+        O.ol_istream(pg.h)
+        O.ol_aad_iset_t(pg.h, ptr(Ff(ad_kind, inlet, np.int32)), ptr(ux), ptr(uy), None)
+        O.ol_aad_iset_q(pg.h, ptr(Ff(ad_kind, flux, np.int32)), ptr(ux), ptr(uy), None, D(0.0))
+        O.ol_aad_iset_q(pg.h, ptr(Ff(ad_kind, objedge, np.int32)), ptr(ux), ptr(uy), None, D(1.0))
+        O.ol_smooth_corner(pg.h)
+        O.ol_bc(pf.h, ptr(Ff(D2Q9, walls, np.int32)), 1)
+        O.ol_ans_iset_u(pf.h, ptr(Ff(D2Q9, inlet, np.int32)), ptr(Ff(D2Q9, uin)), ptr(Ff(D2Q9, z())),
+                        ptr(Ff(D2Q9, z())), D(0.0), 1)
+        O.ol_aad_iset_rho(pf.h, pg.h, ptr(otype), ptr(rho), ptr(ux), ptr(uy), None, ptr(tem), D(0.0))
+        O.ol_smooth_corner(pf.h)
+    ab = t % 2
+    dfds = z()
+    O.ol_sens_brinkman_diffusivity(pg.h, ptr(dfds), ptr(ux), ptr(uy), None, ptr(imx), ptr(imy), None, ptr(dads),
+                                   ptr(tem), ptr(item), ptr(IQ[ab][0]), ptr(IQ[ab][1]), None, ptr(g), ptr(ig),
+                                   ptr(diff), ptr(dkds))
+    objsum = 0.0
+    for n in np.nonzero(objedge)[0]:
+        objsum += tem[n]
+    out.update({"adj_f": pf.get(), "adj_g": pg.get(), "ip": ip, "imx": imx, "imy": imy, "item": item,
+                "iqx": IQ[ab][0].copy(), "iqy": IQ[ab][1].copy(), "ig": ig, "dfds": dfds,
+                "objective": objsum / int(objedge.sum()), "adj_steps": t, "adj_res": (ru, rq), "gamma": s})
+    return out
