@@ -104,6 +104,48 @@ def cpu_baseline(hs, target_seconds=15.0):
                       % (nt, nt, secs[0], secs[1])}
 
 
+def kernel_only(ctx, a):
+    """Launch one kernel kind a.steps times (eager, no graphs) and print its event-timed average."""
+    from panslbm2_amd import D2Q5, D2Q9, Lattice, check
+    from panslbm2_amd.heatsink2d import Heatsink2D
+
+    if a.kernel_only == "stream":
+        lat = Lattice(ctx, D2Q9, a.lx, a.ly)
+        check(ctx.lib.lbm_stream(lat.h))
+        ctx.sync()
+        ctx.record(0)
+        for _ in range(a.steps):
+            check(ctx.lib.lbm_stream(lat.h))
+        ctx.record(1)
+        ms = ctx.elapsed_ms(0, 1) / a.steps
+        nbytes = 2 * 9 * 8 * a.lx * a.ly
+    else:
+        hs = Heatsink2D(ctx, lx=a.lx, ly=a.ly, ad_kind=D2Q5, max_steps=a.steps, use_graphs=False, filter_radius=1.0)
+        L, d = hs.L, hs.d
+        check(L.lbm_ns_initial_condition(hs.pf.h, d["one"].ptr, d["ux0"].ptr, d["uy0"].ptr, None))
+        check(L.lbm_ad_initial_condition(hs.pg.h, d["tem"].ptr, d["ux0"].ptr, d["uy0"].ptr, None))
+        if a.kernel_only == "adj":
+            hs.fwd_buf = 0
+            hs.adj_prog = hs._record_adjoint(0)
+            step = lambda t: hs._adj_step(t, 0)
+            nbytes = (2 * (9 + 5) * 8 + 2 * 8 + 4 * 8) * hs.N
+        else:
+            step = hs._fwd_step
+            nbytes = BYTES_FWD_Q9Q5 * hs.N
+        for t in range(1, 3):
+            step(t)
+        ctx.sync()
+        ctx.record(0)
+        for t in range(3, 3 + a.steps):
+            step(t)
+        ctx.record(1)
+        ms = ctx.elapsed_ms(0, 1) / a.steps
+        hs.close()
+    print(json.dumps({"kernel_only": a.kernel_only, "grid": [a.lx, a.ly], "launches": a.steps,
+                      "us_per_launch": round(ms * 1e3, 3), "algorithmic_bytes": nbytes,
+                      "achieved_GBs": round(nbytes / (ms * 1e-3) / 1e9, 1)}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,6 +156,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-big", action="store_true")
     ap.add_argument("--max-steps", type=int, default=400000, help="cap on forward and on adjoint steps")
+    ap.add_argument("--no-graphs", action="store_true", help="launch every step (for profilers)")
+    ap.add_argument("--kernel-only", choices=["fwd", "adj", "stream"],
+                    help="launch only the fused forward step, the fused adjoint step or the pull-stream copy "
+                         "--steps times on --lx x --ly (profiling and counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (requested --gpus %d)" % a.gpus,
@@ -124,7 +170,9 @@ def main():
     from panslbm2_amd.heatsink2d import Heatsink2D
 
     ctx = Context(0)
-    hs = Heatsink2D(ctx, lx=a.lx, ly=a.ly, ad_kind=D2Q5, max_steps=a.max_steps)
+    if a.kernel_only:
+        return kernel_only(ctx, a)
+    hs = Heatsink2D(ctx, lx=a.lx, ly=a.ly, ad_kind=D2Q5, max_steps=a.max_steps, use_graphs=not a.no_graphs)
     print("bench: heatsink %dx%d D2Q9+D2Q5 FP64, warmup %d, steps %d" % (a.lx, a.ly, a.warmup, a.steps),
           file=sys.stderr, flush=True)
     for w in range(a.warmup):
