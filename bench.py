@@ -48,11 +48,11 @@ def kernel_rate(hs, steps=100, reps=5):
     return ms / (reps * hs.check_every)
 
 
-def big_grid_roofline(ctx, lx=4096, ly=4096):
+def big_grid_roofline(ctx, lx=4096, ly=4096, use_graphs=True):
     from panslbm2_amd import D2Q5
     from panslbm2_amd.heatsink2d import Heatsink2D
 
-    hs = Heatsink2D(ctx, lx=lx, ly=ly, ad_kind=D2Q5, max_steps=200, filter_radius=1.0)
+    hs = Heatsink2D(ctx, lx=lx, ly=ly, ad_kind=D2Q5, max_steps=200, filter_radius=1.0, use_graphs=use_graphs)
     L, d = hs.L, hs.d
     from panslbm2_amd import check
 
@@ -202,14 +202,14 @@ def main():
                 "note": "working set ~20 MB per step: resident in the 256 MB Infinity Cache"}
     extra = {}
     if not a.no_big:
-        ms_big, nbig = big_grid_roofline(ctx)
+        ms_big, nbig = big_grid_roofline(ctx, use_graphs=not a.no_graphs)
         ach_big = BYTES_FWD_Q9Q5 * nbig / (ms_big * 1e-3) / 1e9
         extra["roofline_hbm_4096x4096"] = {"achieved": round(ach_big, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                            "frac": round(ach_big / HBM_PEAK_GBS, 4),
                                            "us_per_launch": round(ms_big * 1e3, 1),
                                            "mlups": round(nbig / (ms_big * 1e-3) / 1e6, 1)}
     # like-for-like with the reference's lattice pairing (D2Q9 advection-diffusion)
-    hs9 = Heatsink2D(ctx, lx=a.lx, ly=a.ly, ad_kind=D2Q9, max_steps=a.max_steps)
+    hs9 = Heatsink2D(ctx, lx=a.lx, ly=a.ly, ad_kind=D2Q9, max_steps=a.max_steps, use_graphs=not a.no_graphs)
     r9 = hs9.evaluate()
     extra["mlups_d2q9_d2q9"] = round(hs9.N * (r9["fwd_steps"] + r9["adj_steps"]) / r9["seconds"] / 1e6, 1)
     cpu = None if a.no_cpu else cpu_baseline(hs9)

@@ -205,6 +205,8 @@ int lbm_program_bind_fields(lbm_program *prog, int set, const double *rho, const
 int lbm_program_num_field_sets(const lbm_program *prog);
 /* whether a program runs fused with a collide (1) or falls back to apply + collide (0) */
 int lbm_program_fusable(const lbm_program *prog);
+/* 1 when two programs record the same calls with the same values and field pointers */
+int lbm_program_equal(const lbm_program *a, const lbm_program *b);
 
 /* "program, then collide" in one pass; prog may be NULL (plain collide) */
 int lbm_ns_macro_collide_p(lbm_program *prog, lbm_lattice *p, double *rho, double *ux, double *uy, double *uz,

@@ -97,6 +97,7 @@ def load():
         "lbm_program_bind_fields": [VP, I] + [VP] * 6,
         "lbm_program_num_field_sets": [VP],
         "lbm_program_fusable": [VP],
+        "lbm_program_equal": [VP, VP],
         "lbm_graph_begin": [VP],
         "lbm_graph_end": [VP, ctypes.POINTER(VP)],
         "lbm_graph_launch": [VP],

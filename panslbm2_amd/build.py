@@ -1,6 +1,7 @@
 """Build the HIP library (gfx950) and the test-only CPU oracle.
 
     python -m panslbm2_amd.build            # library + oracle (+ reference harness if present)
+                                            # + the C++ drivers of tests/cpp
 
 The library is built in-tree (panslbm2_amd/libpanslbm2_hip.so) so that it travels with the tree.
 """
@@ -60,9 +61,17 @@ def build_oracle(verbose=False):
         print(out)
 
 
+def build_drivers(verbose=False):
+    """C++ drivers written against include/panslbm2_amd.hpp (tests/cpp), linked to the library."""
+    out = _run(["make", "-B", "-C", os.path.join(ROOT, "tests", "cpp")])
+    if verbose:
+        print(out)
+
+
 def main():
     build_oracle(verbose=True)
     build_library(verbose=True)
+    build_drivers(verbose=True)
 
 
 if __name__ == "__main__":

@@ -1093,6 +1093,22 @@ int lbm_graph_destroy(lbm_graph *G) {
     return LBM_OK;
 }
 int lbm_program_num_field_sets(const lbm_program *P) { return P ? (int)P->fsets.size() : fail(LBM_ERR_ARG, "null"); }
+int lbm_program_equal(const lbm_program *a, const lbm_program *b) {
+    if (!a || !b) return fail(LBM_ERR_ARG, "null program");
+    if (a->nlat != b->nlat || a->seg != b->seg || a->ops.size() != b->ops.size() || a->fsets.size() != b->fsets.size())
+        return 0;
+    for (int k = 0; k < a->nlat; ++k)
+        if (a->lat[k] != b->lat[k] || a->pull[k] != b->pull[k] || a->smoothed[k] != b->smoothed[k]) return 0;
+    for (int k = 0; k < a->nlat; ++k)
+        if (a->smooth_seg[k] != b->smooth_seg[k]) return 0;
+    if (a->seg_closer != b->seg_closer) return 0;
+    if (!a->fsets.empty() && memcmp(a->fsets.data(), b->fsets.data(), sizeof(FieldSet) * a->fsets.size())) return 0;
+    for (size_t t = 0; t < a->ops.size(); ++t) {
+        const HostOp &x = a->ops[t], &y = b->ops[t];
+        if (x.node != y.node || memcmp(&x.op, &y.op, sizeof(Op))) return 0;
+    }
+    return 1;
+}
 int lbm_program_fusable(const lbm_program *P) { return P ? (P->ordered_ok ? 1 : 0) : fail(LBM_ERR_ARG, "null"); }
 
 }  // extern "C"

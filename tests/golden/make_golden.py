@@ -7,9 +7,12 @@ For every case in tests/refcases.py it stores the seeded inputs and, per referen
 the outputs the reference produced.  The fixtures are data only (inputs and expected outputs).
 Also stores short driver-sequence runs (lid-driven cavity, coupled forced convection forward and
 adjoint) composed from reference calls in the order of test/cavityflow.cpp:44-67 and
-test/nsadsens.cpp:67-133.
+test/nsadsens.cpp:67-133, and the output of the C++ heat-sink driver tests/cpp/heatsink_driver.cpp
+compiled against the reference headers (oracle/_ref/heatsink_driver_ref).
 """
 import os
+import shutil
+import subprocess
 import sys
 
 import numpy as np
@@ -40,7 +43,22 @@ def main():
         path = os.path.join(HERE, "seq_" + seq.name + ".npz")
         np.savez_compressed(path, **out)
         print("wrote", path, os.path.getsize(path), "bytes")
+    write_driver_golden()
+
+
+def write_driver_golden():
+    drv = os.path.join(os.path.dirname(os.path.dirname(HERE)), "oracle", "_ref", "heatsink_driver_ref")
+    if not os.path.exists(drv):
+        raise SystemExit("reference driver not built: make -C oracle ref")
+    path = os.path.join(HERE, "shim_heatsink_30x17.bin")
+    tmp = path + ".tmp"
+    subprocess.run([drv, tmp], check=True)
+    shutil.move(tmp, path)
+    print("wrote", path, os.path.getsize(path), "bytes")
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["driver"]:
+        write_driver_golden()
+    else:
+        main()
