@@ -118,6 +118,7 @@ public:
             Mirror &m = mirrors_.at(h);
             m.device_owned = true;
             m.device_newer = true;
+            m.stale = m.recheck = false;  // the device copy is now the newest
             if (coherence == Coherence::Host || force_download) download(h, m);
         }
     }
@@ -278,6 +279,7 @@ private:
         }
         m.shadow.assign(h, h + m.count);
         m.device_newer = false;
+        m.stale = false;
     }
 
     lbm_context *ctx_ = nullptr;
