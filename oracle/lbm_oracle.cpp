@@ -546,6 +546,208 @@ void aad_iset_rho_2d(const LatDesc &dq, double *f, const double *g, int face, in
     } break;
     }
 }
+/* ---- per-node boundary operators, D3Q15 / D3Q7 (3D) --------------------------------------
+ * Face numbering 0..5 = xmin, xmax, ymin, ymax, zmin, zmax; axis a = face / 2, direction
+ * dd = -1 on a min face and +1 on a max face.  The reference spells every face out; its sums run
+ * over populations in ascending index order and its velocity terms in axis order x, y, z (or the
+ * cyclic order noted below), which is the order kept here. */
+
+/* 1/w_c as the reference writes it in its denominators (9.0, 36.0, 72.0; 6.0 for D2Q5/D3Q7) */
+double inv_w(const LatDesc &d, int c) {
+    int nz = (d.cx[c] != 0) + (d.cy[c] != 0) + (d.cz[c] != 0);
+    if (d.nc == 5 || d.nc == 7) return 6.0;
+    if (nz == 1) return 9.0;
+    return d.nc == 9 ? 36.0 : 72.0;
+}
+/* the axis population with component s (+1 / -1) along axis a */
+int axis_pop(const LatDesc &d, int a, int s) {
+    for (int c = 1; c < d.nc; ++c)
+        if (comp(d, c, a) == s && (d.cx[c] != 0) + (d.cy[c] != 0) + (d.cz[c] != 0) == 1) return c;
+    return -1;
+}
+bool is_diag(const LatDesc &d, int c) { return (d.cx[c] != 0) + (d.cy[c] != 0) + (d.cz[c] != 0) > 1; }
+/* sum over the diagonal populations with c_a == s of sign(c_t) * f_c, ascending (the first term
+ * carries a + sign in every face the reference writes) */
+double signed_diag_sum(const LatDesc &d, const double *f, int a, int s, int t) {
+    double r = 0.0;
+    bool first = true;
+    for (int c = 1; c < d.nc; ++c) {
+        if (!is_diag(d, c) || comp(d, c, a) != s) continue;
+        int sg = comp(d, c, t);
+        if (first) { r = sg > 0 ? f[c] : -f[c]; first = false; }
+        else r = sg > 0 ? r + f[c] : r - f[c];
+    }
+    return r;
+}
+/* 8 * f_axis + the diagonal populations, all with c_a == s, ascending */
+double normal_sum8(const LatDesc &d, const double *f, int a, int s) {
+    double r = 8.0 * f[axis_pop(d, a, s)];
+    for (int c = 1; c < d.nc; ++c)
+        if (is_diag(d, c) && comp(d, c, a) == s) r = r + f[c];
+    return r;
+}
+/* the Zou-He unknowns (c_a == -dd): axis f_opp +/- 8 m_a, diagonals f_opp + c_a m_a - c_t m_t */
+void zou_he_unknowns_3d(const LatDesc &d, double *f, int a, int dd, const double m[3]) {
+    for (int c = 1; c < d.nc; ++c) {
+        if (comp(d, c, a) != -dd) continue;
+        double r = f[opposite(d, c)];
+        if (!is_diag(d, c)) {
+            f[c] = comp(d, c, a) > 0 ? r + 8.0 * m[a] : r - 8.0 * m[a];
+            continue;
+        }
+        for (int ax = 0; ax < 3; ++ax) {
+            int cc = comp(d, c, ax);
+            if (cc == 0) continue;
+            int sg = ax == a ? cc : -cc;
+            r = sg > 0 ? r + m[ax] : r - m[ax];
+        }
+        f[c] = r;
+    }
+}
+/* f0 + sum_{c_a == 0} f_c + 2 sum_{c_a == dd} f_c, as (f0 + ... ) + 2.0*( ... ) */
+double zou_he_known_3d(const LatDesc &d, const double *f, int a, int dd) {
+    double s0 = f[0], s1 = 0.0;
+    bool first = true;
+    for (int c = 1; c < d.nc; ++c)
+        if (comp(d, c, a) == 0) s0 = s0 + f[c];
+    for (int c = 1; c < d.nc; ++c)
+        if (comp(d, c, a) == dd) {
+            if (first) { s1 = f[c]; first = false; }
+            else s1 = s1 + f[c];
+        }
+    return s0 + 2.0 * s1;
+}
+/* navierstokes.h:150-257 (D3Q15 Zou-He velocity) */
+void ns_set_u_3d(const LatDesc &d, double *f, int face, double ux, double uy, double uz) {
+    int a = face / 2, dd = face % 2 ? 1 : -1;
+    double u[3] = {ux, uy, uz};
+    double rho0 = zou_he_known_3d(d, f, a, dd) / (dd < 0 ? 1.0 - u[a] : 1.0 + u[a]);
+    double m[3];
+    for (int ax = 0; ax < 3; ++ax) {
+        if (ax == a) m[ax] = rho0 * u[ax] / 12.0;
+        else m[ax] = 0.25 * (f[axis_pop(d, ax, 1)] - f[axis_pop(d, ax, -1)] - rho0 * u[ax]);
+    }
+    zou_he_unknowns_3d(d, f, a, dd, m);
+}
+/* navierstokes.h:319-426 (D3Q15 Zou-He pressure); the two tangential velocities are given in
+ * cyclic order: x-face (uy, uz), y-face (uz, ux), z-face (ux, uy) */
+void ns_set_rho_3d(const LatDesc &d, double *f, int face, double rho, double us, double ut) {
+    int a = face / 2, dd = face % 2 ? 1 : -1;
+    double u[3];
+    u[(a + 1) % 3] = us;
+    u[(a + 2) % 3] = ut;
+    double known = zou_he_known_3d(d, f, a, dd);
+    u[a] = dd < 0 ? 1.0 - known / rho : -1.0 + known / rho;
+    double m[3];
+    for (int ax = 0; ax < 3; ++ax) {
+        if (ax == a) m[ax] = rho * u[ax] / 12.0;
+        else m[ax] = 0.25 * (f[axis_pop(d, ax, 1)] - f[axis_pop(d, ax, -1)] - rho * u[ax]);
+    }
+    zou_he_unknowns_3d(d, f, a, dd, m);
+}
+/* unknown AD populations (c_a == -dd): tem0 (1 + 3 c.u) / (1/w_c), velocity terms in axis order */
+void ad_unknowns_3d(const LatDesc &d, double *g, int a, int dd, double tem0, const double u[3]) {
+    for (int c = 1; c < d.nc; ++c) {
+        if (comp(d, c, a) != -dd) continue;
+        double e = 1.0;
+        for (int ax = 0; ax < 3; ++ax) {
+            int cc = comp(d, c, ax);
+            if (cc > 0) e = e + 3.0 * u[ax];
+            else if (cc < 0) e = e - 3.0 * u[ax];
+        }
+        g[c] = tem0 * e / inv_w(d, c);
+    }
+}
+/* advection.h:149-238 (D3Q15 Dirichlet temperature); D3Q7: the same construction */
+void ad_set_t_3d(const LatDesc &d, double *g, int face, double tem, double ux, double uy, double uz) {
+    int a = face / 2, dd = face % 2 ? 1 : -1;
+    double u[3] = {ux, uy, uz};
+    double r = tem;
+    for (int c = 0; c < d.nc; ++c)
+        if (comp(d, c, a) != -dd) r = r - g[c];
+    double tem0 = 6.0 * r / (dd < 0 ? 1.0 + 3.0 * u[a] : 1.0 - 3.0 * u[a]);
+    ad_unknowns_3d(d, g, a, dd, tem0, u);
+}
+/* advection.h:292-381 (constant D) and :435-524 (field D); D3Q7: the same construction */
+void ad_set_q_3d(const LatDesc &d, double *g, int face, double qn, double ux, double uy, double uz, double diff) {
+    int a = face / 2, dd = face % 2 ? 1 : -1;
+    double u[3] = {ux, uy, uz};
+    double r = (1.0 + 1.0 / (6.0 * diff)) * qn;
+    for (int c = 1; c < d.nc; ++c)
+        if (comp(d, c, a) == dd) r = r + g[c];
+    double tem0 = 6.0 * r / (dd < 0 ? 1.0 - 3.0 * u[a] : 1.0 + 3.0 * u[a]);
+    ad_unknowns_3d(d, g, a, dd, tem0, u);
+}
+/* adjointnavierstokes.h:147-254 (D3Q15 adjoint velocity) */
+void ans_iset_u_3d(const LatDesc &d, double *f, int face, double ux, double uy, double uz, double eps) {
+    int a = face / 2, dd = face % 2 ? 1 : -1;
+    double u[3] = {ux, uy, uz};
+    double r = -4.0 * eps;
+    for (int ax = 0; ax < 3; ++ax) {
+        if (ax == a) {
+            double sn = normal_sum8(d, f, a, -dd);
+            r = -dd > 0 ? r + u[a] * sn : r - u[a] * sn;
+        } else {
+            r = r + 3.0 * u[ax] * signed_diag_sum(d, f, a, -dd, ax);
+        }
+    }
+    double rho0 = r / (6.0 * (dd < 0 ? 1.0 - u[a] : 1.0 + u[a]));
+    for (int c = 1; c < d.nc; ++c)
+        if (comp(d, c, a) == dd) f[c] = f[opposite(d, c)] + rho0;
+}
+/* adjointnavierstokes.h:306-392 */
+void ans_iset_rho_3d(const LatDesc &d, double *f, int face) {
+    int a = face / 2, dd = face % 2 ? 1 : -1;
+    double rho0 = normal_sum8(d, f, a, -dd) / 6.0;
+    for (int c = 1; c < d.nc; ++c)
+        if (comp(d, c, a) == dd) f[c] = f[opposite(d, c)] - rho0;
+}
+/* adjointadvection.h:202-300 (D3Q15): tangential terms in cyclic order (x-face y, z; y-face z, x;
+ * z-face x, y).  D3Q7 image: the single known population, as the D2Q5 form. */
+void aad_iset_t_3d(const LatDesc &d, double *g, int face, double ux, double uy, double uz) {
+    int a = face / 2, dd = face % 2 ? 1 : -1;
+    double u[3] = {ux, uy, uz};
+    double den = dd < 0 ? 1.0 + 3.0 * u[a] : 1.0 - 3.0 * u[a];
+    if (d.nc == 7) {
+        int k = axis_pop(d, a, -dd), c = axis_pop(d, a, dd);
+        g[c] = -(den * g[k]) / den;
+        return;
+    }
+    double r = -normal_sum8(d, g, a, -dd) / 12.0;
+    for (int q = 1; q <= 2; ++q) {
+        int t = (a + q) % 3;
+        r = r - u[t] * signed_diag_sum(d, g, a, -dd, t) / (4.0 * den);
+    }
+    for (int c = 1; c < d.nc; ++c)
+        if (comp(d, c, a) == dd) g[c] = r;
+}
+/* adjointadvection.h:368-484 (D3Q15), tangential terms in cyclic order.  The reference scales the
+ * tangential terms by 3.0 on the xmin, xmax and ymin faces only (:378-379, :390-391, :418-419) and
+ * not on ymax, zmin, zmax (:430-431, :458-459, :470-471); kept as written.  D3Q7 image: the D2Q5
+ * form. */
+void aad_iset_q_3d(const LatDesc &d, double *g, int face, double ux, double uy, double uz, double eps) {
+    int a = face / 2, dd = face % 2 ? 1 : -1;
+    double u[3] = {ux, uy, uz};
+    double top = dd < 0 ? 1.0 + 3.0 * u[a] : 1.0 - 3.0 * u[a];
+    double bot = dd < 0 ? 1.0 - 3.0 * u[a] : 1.0 + 3.0 * u[a];
+    if (d.nc == 7) {
+        int k = axis_pop(d, a, -dd), c = axis_pop(d, a, dd);
+        g[c] = (top * g[k] - 2.0 * eps) / bot;
+        return;
+    }
+    const bool scaled = face <= 2;
+    double r = top * normal_sum8(d, g, a, -dd);
+    for (int q = 1; q <= 2; ++q) {
+        int t = (a + q) % 3;
+        double ts = signed_diag_sum(d, g, a, -dd, t);
+        r = scaled ? r + 3.0 * u[t] * ts : r + u[t] * ts;
+    }
+    r = r - 24.0 * eps;
+    double rho0 = r / (12.0 * bot);
+    for (int c = 1; c < d.nc; ++c)
+        if (comp(d, c, a) == dd) g[c] = rho0;
+}
+
 /* adjointadvection.h:755-801 (D2Q9 boundary term of the heat-source sensitivity); ig is SoA */
 double sens_hs_boundary_2d(int face, const double *ig /* ig[c] at node */, double qn, double dkds, double ux, double uy,
                            double diff) {
@@ -757,25 +959,25 @@ void ol_ns_macro_brinkman_collide(ol_lattice *p, double *rho, double *ux, double
 }
 /* navierstokes.h:576-592 with :92-257 */
 void ol_ns_set_u(ol_lattice *p, const int *mask, const double *ux, const double *uy, const double *uz) {
-    (void)uz;
-    if (p->d->nd != 2) return; /* 3D face formulas: see lbm_oracle3d.cpp */
+    const bool d3 = p->d->nd == 3;
     for_faces(p, [&](int face, int t, int idx) {
         if (!mask[t]) return;
         double f[27];
         load(p, idx, f);
-        ns_set_u_2d(f, face, ux[t], uy[t]);
+        if (d3) ns_set_u_3d(*p->d, f, face, ux[t], uy[t], uz[t]);
+        else ns_set_u_2d(f, face, ux[t], uy[t]);
         store(p, idx, f);
     });
 }
 /* navierstokes.h:596-612 with :261-426 */
 void ol_ns_set_rho(ol_lattice *p, const int *mask, const double *rho, const double *us, const double *ut) {
-    (void)ut;
-    if (p->d->nd != 2) return;
+    const bool d3 = p->d->nd == 3;
     for_faces(p, [&](int face, int t, int idx) {
         if (!mask[t]) return;
         double f[27];
         load(p, idx, f);
-        ns_set_rho_2d(f, face, rho[t], us[t]);
+        if (d3) ns_set_rho_3d(*p->d, f, face, rho[t], us[t], ut[t]);
+        else ns_set_rho_2d(f, face, rho[t], us[t]);
         store(p, idx, f);
     });
 }
@@ -828,26 +1030,27 @@ void ol_ad_collide_fc(ol_lattice *p, double *rho, double *ux, double *uy, double
 /* advection.h:1074-1090 with :99-238 */
 void ol_ad_set_t(ol_lattice *q, const int *mask, const double *tem, const double *ux, const double *uy,
                  const double *uz) {
-    (void)uz;
-    if (q->d->nd != 2) return;
+    const bool d3 = q->d->nd == 3;
     for_faces(q, [&](int face, int t, int idx) {
         if (!mask[t]) return;
         double g[27];
         load(q, idx, g);
-        ad_set_t_2d(*q->d, g, face, tem[t], ux[idx], uy[idx]);
+        if (d3) ad_set_t_3d(*q->d, g, face, tem[t], ux[idx], uy[idx], uz[idx]);
+        else ad_set_t_2d(*q->d, g, face, tem[t], ux[idx], uy[idx]);
         store(q, idx, g);
     });
 }
 /* advection.h:1094-1130 with :242-524 */
 void ol_ad_set_q(ol_lattice *q, const int *mask, const double *qn, const double *ux, const double *uy, const double *uz,
                  const double *diff_field, double diff_const) {
-    (void)uz;
-    if (q->d->nd != 2) return;
+    const bool d3 = q->d->nd == 3;
     for_faces(q, [&](int face, int t, int idx) {
         if (!mask[t]) return;
         double g[27];
         load(q, idx, g);
-        ad_set_q_2d(*q->d, g, face, qn[t], ux[idx], uy[idx], diff_field ? diff_field[idx] : diff_const);
+        double dv = diff_field ? diff_field[idx] : diff_const;
+        if (d3) ad_set_q_3d(*q->d, g, face, qn[t], ux[idx], uy[idx], uz[idx], dv);
+        else ad_set_q_2d(*q->d, g, face, qn[t], ux[idx], uy[idx], dv);
         store(q, idx, g);
     });
 }
@@ -889,24 +1092,25 @@ void ol_ans_collide(ol_lattice *p, const double *rho, const double *ux, const do
 /* adjointnavierstokes.h:502-518 */
 void ol_ans_iset_u(ol_lattice *p, const int *mask, const double *ux, const double *uy, const double *uz, double eps,
                    int compat) {
-    (void)uz;
-    if (p->d->nd != 2) return;
+    const bool d3 = p->d->nd == 3;
     for_faces(p, [&](int face, int t, int idx) {
         if (!mask[t]) return;
         double f[27];
         load(p, idx, f);
-        ans_iset_u_2d(f, face, ux[t], uy[t], eps, compat);
+        if (d3) ans_iset_u_3d(*p->d, f, face, ux[t], uy[t], uz[t], eps);
+        else ans_iset_u_2d(f, face, ux[t], uy[t], eps, compat);
         store(p, idx, f);
     });
 }
 /* adjointnavierstokes.h:522-538 */
 void ol_ans_iset_rho(ol_lattice *p, const int *mask) {
-    if (p->d->nd != 2) return;
+    const bool d3 = p->d->nd == 3;
     for_faces(p, [&](int face, int t, int idx) {
         if (!mask[t]) return;
         double f[27];
         load(p, idx, f);
-        ans_iset_rho_2d(f, face);
+        if (d3) ans_iset_rho_3d(*p->d, f, face);
+        else ans_iset_rho_2d(f, face);
         store(p, idx, f);
     });
 }
@@ -941,7 +1145,8 @@ void ol_aad_collide_fc(ol_lattice *p, const double *rho, const double *ux, const
         aad_brinkman(dp, f, rho[idx], vx, vy, vz, mx, my, mz, tem[idx], hx, hy, hz, omegag, alpha[idx]);
         ans_macro(dp, f, vx, vy, vz, a, bx, by, bz, mx, my, mz);
         if (issave) {
-            ip[idx] = a; iux[idx] = bx; iuy[idx] = by; put(iuz, idx, bz);
+            /* the reference's 3D save writes no iuz (adjointadvection.h:1074-1084); kept */
+            ip[idx] = a; iux[idx] = bx; iuy[idx] = by;
             imx[idx] = mx; imy[idx] = my; put(imz, idx, mz);
             item[idx] = it; iqx[idx] = hx; iqy[idx] = hy; put(iqz, idx, hz);
             if (igsnap)
@@ -957,25 +1162,25 @@ void ol_aad_collide_fc(ol_lattice *p, const double *rho, const double *ux, const
 }
 /* adjointadvection.h:1385-1401 */
 void ol_aad_iset_t(ol_lattice *q, const int *mask, const double *ux, const double *uy, const double *uz) {
-    (void)uz;
-    if (q->d->nd != 2) return;
+    const bool d3 = q->d->nd == 3;
     for_faces(q, [&](int face, int t, int idx) {
         if (!mask[t]) return;
         double g[27];
         load(q, idx, g);
-        aad_iset_t_2d(*q->d, g, face, ux[idx], uy[idx]);
+        if (d3) aad_iset_t_3d(*q->d, g, face, ux[idx], uy[idx], uz[idx]);
+        else aad_iset_t_2d(*q->d, g, face, ux[idx], uy[idx]);
         store(q, idx, g);
     });
 }
 /* adjointadvection.h:1405-1421 */
 void ol_aad_iset_q(ol_lattice *q, const int *mask, const double *ux, const double *uy, const double *uz, double eps) {
-    (void)uz;
-    if (q->d->nd != 2) return;
+    const bool d3 = q->d->nd == 3;
     for_faces(q, [&](int face, int t, int idx) {
         if (!mask[t]) return;
         double g[27];
         load(q, idx, g);
-        aad_iset_q_2d(*q->d, g, face, ux[idx], uy[idx], eps);
+        if (d3) aad_iset_q_3d(*q->d, g, face, ux[idx], uy[idx], uz[idx], eps);
+        else aad_iset_q_2d(*q->d, g, face, ux[idx], uy[idx], eps);
         store(q, idx, g);
     });
 }

@@ -118,16 +118,21 @@ struct lbm_program {
     bool ordered_ok = true;  // no operator on a lattice after its SmoothCorner
     // device tables
     bool uploaded = false;
-    DevBuf d_slot_entry, d_entry_node, d_entry_ops, d_ops, d_fix_corner, d_fix_edge;
-    int nentry = 0, nslot = 0, n_fix_corner = 0, n_fix_edge = 0;
+    DevBuf d_slot_entry, d_entry_node, d_entry_ops, d_ops, d_sops, d_fix_corner, d_fix_edge;
+    DevBuf d_side_of_slot, d_corner2d, d_side;  // 2D corner smoothing after the fused pass
+    int nentry = 0, nslot = 0, maxops = 0, n_fix_corner = 0, n_fix_edge = 0, n_corner2d = 0;
     bool owned_by_ctx = false;  // eager scratch program
     ~lbm_program() {
         d_slot_entry.release();
         d_entry_node.release();
         d_entry_ops.release();
         d_ops.release();
+        d_sops.release();
         d_fix_corner.release();
         d_fix_edge.release();
+        d_side_of_slot.release();
+        d_corner2d.release();
+        d_side.release();
     }
 };
 
@@ -322,11 +327,47 @@ int prog_upload(lbm_program *P) {
         if (s < 0) return fail(LBM_ERR_STATE, "boundary operator on an interior node");
         slot_entry[s] = e;
     }
+    // slot-major copy for the fused kernels: slot s's operators at [s * maxops, ...), OP_NONE-padded
+    int most = 0;
+    for (const int2 &r : entry_ops) most = std::max(most, r.y);
+    P->maxops = (most + lbm::kOpBatch - 1) / lbm::kOpBatch * lbm::kOpBatch;
+    std::vector<Op> sops((size_t)P->nslot * P->maxops);
+    for (int s = 0; s < P->nslot; ++s) {
+        int e = slot_entry[s];
+        if (e < 0) continue;
+        for (int t = 0; t < entry_ops[e].y; ++t) sops[(size_t)s * P->maxops + t] = ops[entry_ops[e].x + t];
+    }
     std::vector<int> fix_corner, fix_edge;
-    if (P->smoothed[0] || P->smoothed[1]) {
+    // 2D: corners are done after the fused pass from their neighbours' saved states; 3D: edges and
+    // corners are rebuilt by fix-up kernels before it
+    std::vector<int> side_of_slot;
+    std::vector<int4> corner2d;
+    if ((P->smoothed[0] || P->smoothed[1]) && L->nd == 2) {
+        std::map<int, int> side;
+        auto side_id = [&](int node) {
+            auto it = side.find(node);
+            if (it != side.end()) return it->second;
+            int id = (int)side.size();
+            side[node] = id;
+            return id;
+        };
+        for (const int4 &e : L->sm_corner) {
+            int a = side_id(e.y), b = side_id(e.z);
+            corner2d.push_back(make_int4(e.x, a, b, -1));
+        }
+        side_of_slot.assign(P->nslot, -1);
+        for (auto &kv : side) {
+            int i, j, k;
+            node_coords(L, kv.first, i, j, k);
+            side_of_slot[lbm::boundary_slot(i, j, k, L->lx, L->ly, L->lz, L->nd)] = kv.second;
+        }
+        if (P->d_side.ensure(sizeof(double) * lbm::kSideStride * std::max<size_t>(side.size(), 1)))
+            return fail(LBM_ERR_HIP, "device allocation of the side states failed");
+    } else if (P->smoothed[0] || P->smoothed[1]) {
         for (const int4 &e : L->sm_corner) fix_corner.push_back(e.x);
         for (const int4 &e : L->sm_edge) fix_edge.push_back(e.x);
     }
+    P->n_corner2d = (int)corner2d.size();
     P->n_fix_corner = (int)fix_corner.size();
     P->n_fix_edge = (int)fix_edge.size();
     auto up = [&](DevBuf &b, const void *src, size_t bytes) -> int {
@@ -339,8 +380,11 @@ int prog_upload(lbm_program *P) {
     if ((rc = up(P->d_entry_node, entry_node.data(), entry_node.size() * sizeof(int)))) return rc;
     if ((rc = up(P->d_entry_ops, entry_ops.data(), entry_ops.size() * sizeof(int2)))) return rc;
     if ((rc = up(P->d_ops, ops.data(), ops.size() * sizeof(Op)))) return rc;
+    if ((rc = up(P->d_sops, sops.data(), sops.size() * sizeof(Op)))) return rc;
     if ((rc = up(P->d_fix_corner, fix_corner.data(), fix_corner.size() * sizeof(int)))) return rc;
     if ((rc = up(P->d_fix_edge, fix_edge.data(), fix_edge.size() * sizeof(int)))) return rc;
+    if ((rc = up(P->d_side_of_slot, side_of_slot.data(), side_of_slot.size() * sizeof(int)))) return rc;
+    if ((rc = up(P->d_corner2d, corner2d.data(), corner2d.size() * sizeof(int4)))) return rc;
     P->uploaded = true;
     return LBM_OK;
 }
@@ -351,6 +395,10 @@ ProgView prog_view(const lbm_program *P) {
     v.entry_node = (const int *)P->d_entry_node.p;
     v.entry_ops = (const int2 *)P->d_entry_ops.p;
     v.ops = (const Op *)P->d_ops.p;
+    v.sops = (const Op *)P->d_sops.p;
+    v.maxops = P->maxops;
+    v.side_of_slot = P->n_corner2d > 0 ? (const int *)P->d_side_of_slot.p : nullptr;
+    v.side = P->n_corner2d > 0 ? (double *)P->d_side.p : nullptr;
     v.nentry = P->nentry;
     v.nslot = P->nslot;
     for (size_t k = 0; k < P->fsets.size() && k < (size_t)lbm::kMaxFieldSets; ++k) v.fs[k] = P->fsets[k];
@@ -462,6 +510,8 @@ int collide_call(lbm_program *P, lbm_lattice *p, lbm_lattice *q, Launch launch) 
     cl.n_fix_corner = P->n_fix_corner;
     cl.fix_edge = (const int *)P->d_fix_edge.p;
     cl.n_fix_edge = P->n_fix_edge;
+    cl.corner2d = (const int4 *)P->d_corner2d.p;
+    cl.n_corner2d = P->n_corner2d;
     cl.any_smooth = (P->smoothed[0] || P->smoothed[1]) ? 1 : 0;
     rc = launch(cl);
     if (rc) return rc;
@@ -698,7 +748,8 @@ int lbm_smooth_corner(lbm_lattice *L) {
 static bool ns_lattice(const lbm_lattice *L) { return L && is_ns_kind(L->kind); }
 static bool ad_lattice(const lbm_lattice *L) { return L != nullptr; }  // AD runs on any lattice kind
 static int unsupported_3d(const lbm_lattice *L, const char *what) {
-    return fail(LBM_ERR_UNSUPPORTED, "%s on a 3D lattice is not available in this build", what);
+    (void)L;
+    return fail(LBM_ERR_UNSUPPORTED, "%s on a 3D lattice has no reference behaviour to follow", what);
 }
 
 int lbm_ns_initial_condition(lbm_lattice *p, const double *rho, const double *ux, const double *uy, const double *uz) {
@@ -733,7 +784,6 @@ int lbm_ns_macro_brinkman_collide(lbm_lattice *p, double *rho, double *ux, doubl
 }
 int lbm_ns_set_u(lbm_lattice *p, const int *mask, const double *ux, const double *uy, const double *uz) {
     if (!ns_lattice(p) || !mask || !ux || !uy || (p->nd == 3 && !uz)) return fail(LBM_ERR_ARG, "bad arguments");
-    if (p->nd == 3) return unsupported_3d(p, "BoundaryConditionSetU");
     return boundary_call(p, [&](lbm_program *P) {
         return prog_record(P, p, lbm::OP_NS_SET_U, mask, nullptr, [&](int t, Op &o) {
             o.v[0] = ux[t];
@@ -744,7 +794,6 @@ int lbm_ns_set_u(lbm_lattice *p, const int *mask, const double *ux, const double
 }
 int lbm_ns_set_rho(lbm_lattice *p, const int *mask, const double *rho, const double *us, const double *ut) {
     if (!ns_lattice(p) || !mask || !rho || !us || (p->nd == 3 && !ut)) return fail(LBM_ERR_ARG, "bad arguments");
-    if (p->nd == 3) return unsupported_3d(p, "BoundaryConditionSetRho");
     return boundary_call(p, [&](lbm_program *P) {
         return prog_record(P, p, lbm::OP_NS_SET_RHO, mask, nullptr, [&](int t, Op &o) {
             o.v[0] = rho[t];
@@ -782,7 +831,6 @@ int lbm_ad_macro_brinkman_collide_force_convection(lbm_lattice *p, double *rho, 
 int lbm_ad_set_t(lbm_lattice *q, const int *mask, const double *tem, const double *ux, const double *uy,
                  const double *uz) {
     if (!q || !mask || !tem || !ux || !uy || (q->nd == 3 && !uz)) return fail(LBM_ERR_ARG, "bad arguments");
-    if (q->nd == 3) return unsupported_3d(q, "BoundaryConditionSetT");
     FieldSet fs{nullptr, ux, uy, uz, nullptr, nullptr};
     return boundary_call(q, [&](lbm_program *P) {
         return prog_record(P, q, lbm::OP_AD_SET_T, mask, &fs, [&](int t, Op &o) { o.v[0] = tem[t]; });
@@ -791,7 +839,6 @@ int lbm_ad_set_t(lbm_lattice *q, const int *mask, const double *tem, const doubl
 int lbm_ad_set_q(lbm_lattice *q, const int *mask, const double *qn, const double *ux, const double *uy,
                  const double *uz, const double *diff, double diff_const) {
     if (!q || !mask || !qn || !ux || !uy || (q->nd == 3 && !uz)) return fail(LBM_ERR_ARG, "bad arguments");
-    if (q->nd == 3) return unsupported_3d(q, "BoundaryConditionSetQ");
     FieldSet fs{nullptr, ux, uy, uz, nullptr, diff};
     return boundary_call(q, [&](lbm_program *P) {
         return prog_record(P, q, lbm::OP_AD_SET_Q, mask, &fs, [&](int t, Op &o) {
@@ -831,7 +878,6 @@ int lbm_ans_macro_brinkman_collide(lbm_lattice *p, const double *rho, const doub
 int lbm_ans_iset_u(lbm_lattice *p, const int *mask, const double *ux, const double *uy, const double *uz, double eps,
                    int compat) {
     if (!ns_lattice(p) || !mask || !ux || !uy || (p->nd == 3 && !uz)) return fail(LBM_ERR_ARG, "bad arguments");
-    if (p->nd == 3) return unsupported_3d(p, "iBoundaryConditionSetU");
     return boundary_call(p, [&](lbm_program *P) {
         return prog_record(P, p, lbm::OP_ANS_ISET_U, mask, nullptr, [&](int t, Op &o) {
             o.v[0] = ux[t];
@@ -844,7 +890,6 @@ int lbm_ans_iset_u(lbm_lattice *p, const int *mask, const double *ux, const doub
 }
 int lbm_ans_iset_rho(lbm_lattice *p, const int *mask) {
     if (!ns_lattice(p) || !mask) return fail(LBM_ERR_ARG, "bad arguments");
-    if (p->nd == 3) return unsupported_3d(p, "iBoundaryConditionSetRho3D");
     return boundary_call(p, [&](lbm_program *P) {
         return prog_record(P, p, lbm::OP_ANS_ISET_RHO, mask, nullptr, [](int, Op &) {});
     });
@@ -886,7 +931,6 @@ int lbm_aad_macro_brinkman_collide_force_convection(lbm_lattice *p, const double
 }
 int lbm_aad_iset_t(lbm_lattice *q, const int *mask, const double *ux, const double *uy, const double *uz) {
     if (!q || !mask || !ux || !uy || (q->nd == 3 && !uz)) return fail(LBM_ERR_ARG, "bad arguments");
-    if (q->nd == 3) return unsupported_3d(q, "iBoundaryConditionSetT");
     FieldSet fs{nullptr, ux, uy, uz, nullptr, nullptr};
     return boundary_call(q, [&](lbm_program *P) {
         return prog_record(P, q, lbm::OP_AAD_ISET_T, mask, &fs, [](int, Op &) {});
@@ -894,7 +938,6 @@ int lbm_aad_iset_t(lbm_lattice *q, const int *mask, const double *ux, const doub
 }
 int lbm_aad_iset_q(lbm_lattice *q, const int *mask, const double *ux, const double *uy, const double *uz, double eps) {
     if (!q || !mask || !ux || !uy || (q->nd == 3 && !uz)) return fail(LBM_ERR_ARG, "bad arguments");
-    if (q->nd == 3) return unsupported_3d(q, "iBoundaryConditionSetQ");
     FieldSet fs{nullptr, ux, uy, uz, nullptr, nullptr};
     return boundary_call(q, [&](lbm_program *P) {
         return prog_record(P, q, lbm::OP_AAD_ISET_Q, mask, &fs, [&](int, Op &o) { o.v3 = eps; });

@@ -18,6 +18,8 @@ struct CollideLaunch {
     int n_fix_corner;
     const int *fix_edge;
     int n_fix_edge;
+    const int4 *corner2d;      // 2D smoothed corners (node, side index of x-neighbour, of y-neighbour, -1),
+    int n_corner2d;            // done after the fused pass from the side states it saved
     int any_smooth;
 };
 

@@ -1,18 +1,10 @@
-// panslbm2_amd/csrc/device.hip -- HIP kernels for gfx950 and their typed launchers.
-//
-// Memory-bound stencil kernels: one thread per lattice node, populations in plane order so each
-// plane access is a coalesced 64-lane stream; no LDS and no MFMA (nothing here is GEMM-shaped).
-// Launch shapes: 256-thread workgroups; the fused kernels remap workgroups so that consecutive
-// node tiles stay on one XCD (xcd_tile).
-#include "device.h"
-
-#include "kernels.h"
+// panslbm2_amd/csrc/device.hip -- HIP kernels for gfx950 other than the collides (streaming,
+// eager boundary programs, corner smoothing, initial conditions, sensitivities, reductions,
+// filters) and their typed launchers.  The collide kernels live in collide.h and are launched
+// from collide_{ns,ad,ans,aad}.hip.
+#include "collide.h"
 
 namespace lbm {
-
-constexpr int kBlock = 256;
-
-inline int nblocks(long n, int b = kBlock) { return (int)((n + b - 1) / b); }
 
 // ------------------------------------------------------------------------------------------
 // Streaming: d2q9.h:282-291 / d3q15.h:602-613 (pull from x - c), iStream :414-423 / :965-976
@@ -43,9 +35,10 @@ __global__ void __launch_bounds__(kBlock) k_program_apply(double *a, double *b, 
     double fa[LA::nc], fb[LB::nc > 0 ? LB::nc : 1];
     load_node<LA>(a, N, n, fa);
     load_node<LB>(b, N, n, fb);
+    NodeFieldCache fc = prefetch_fields(pv, n);
     for (int t = r.x; t < r.x + r.y; ++t) {
         Op op = pv.ops[t];
-        if (op.code != OP_SMOOTH && (int)op.seg == seg) apply_op<LA, LB>(op, fa, fb, pv, n);
+        if (op.code != OP_SMOOTH && (int)op.seg == seg) apply_op<LA, LB>(op, fa, fb, pv, n, fc);
     }
     store_node<LA>(a, N, n, fa);
     store_node<LB>(b, N, n, fb);
@@ -64,270 +57,6 @@ __global__ void __launch_bounds__(kBlock) k_smooth(double *f, int N, const int4 
         if (s.w < 0) p[s.x] = 0.5 * (p[s.y] + p[s.z]);
         else p[s.x] = (p[s.y] + p[s.z] + p[s.w]) / 3.0;
     });
-}
-
-// ------------------------------------------------------------------------------------------
-// Collide bodies: act on one node's populations in registers.  Save flags and field pointers as
-// in the reference signatures.
-// ------------------------------------------------------------------------------------------
-// NS::MacroCollide / NS::MacroBrinkmanCollide, navierstokes.h:432-546
-template <class LA, class LB>
-struct NSBody {
-    double *rho, *ux, *uy, *uz;
-    const double *alpha;
-    double omega, iomega;
-    int save, brinkman;
-    __device__ __forceinline__ void operator()(double *f, double *, int n, bool bnd) const {
-        double r, vx, vy, vz, feq[LA::nc];
-        ns_macro<LA>(f, r, vx, vy, vz);
-        if (brinkman) {
-            ns_brinkman<LA>(f, r, vx, vy, vz, alpha[n]);
-            ns_macro<LA>(f, r, vx, vy, vz);
-        }
-        if (save == 1 || (save == 2 && bnd)) {
-            rho[n] = r;
-            ux[n] = vx;
-            uy[n] = vy;
-            if constexpr (LA::nd == 3) uz[n] = vz;
-        }
-        ns_eq<LA>(feq, r, vx, vy, vz);
-        bgk<LA>(f, feq, omega, iomega);
-    }
-};
-
-// AD::MacroBrinkmanCollideForceConvection, advection.h:814-926 (+ optional volumetric source)
-template <class LA, class LB>
-struct ADBody {
-    double *rho, *ux, *uy, *uz, *tem, *qx, *qy, *qz, *gsnap;
-    const double *alpha, *diff, *heat;
-    double omegaf, iomegaf;
-    int save, N;
-    __device__ __forceinline__ void operator()(double *f, double *g, int n, bool bnd) const {
-        double omegag = 1.0 / (3.0 * diff[n] + 0.5), iomegag = 1.0 - omegag;
-        double r, vx, vy, vz, T, hx, hy, hz;
-        ns_macro<LA>(f, r, vx, vy, vz);
-        ad_macro<LB>(g, vx, vy, vz, omegag, T, hx, hy, hz);
-        ns_brinkman<LA>(f, r, vx, vy, vz, alpha[n]);
-        if (heat) {
-            double h = heat[n];
-            sfor<0, LB::nc>([&](auto c) {
-                constexpr int C = decltype(c)::value;
-                g[C] += LB::w[C] * h;
-            });
-        }
-        ns_macro<LA>(f, r, vx, vy, vz);
-        ad_macro<LB>(g, vx, vy, vz, omegag, T, hx, hy, hz);
-        if (save == 1 || (save == 2 && bnd)) {
-            rho[n] = r;
-            ux[n] = vx;
-            uy[n] = vy;
-            tem[n] = T;
-            qx[n] = hx;
-            qy[n] = hy;
-            if constexpr (LA::nd == 3) {
-                uz[n] = vz;
-                qz[n] = hz;
-            }
-            if (gsnap && save == 1) store_node<LB>(gsnap, N, n, g);
-        }
-        {
-            double feq[LA::nc];
-            ns_eq<LA>(feq, r, vx, vy, vz);
-            bgk<LA>(f, feq, omegaf, iomegaf);
-        }
-        {
-            double geq[LB::nc];
-            ad_eq<LB>(geq, T, vx, vy, vz);
-            bgk<LB>(g, geq, omegag, iomegag);
-        }
-    }
-};
-
-// ANS::MacroBrinkmanCollide, adjointnavierstokes.h:398-470
-template <class LA, class LB>
-struct ANSBody {
-    const double *rho, *ux, *uy, *uz, *alpha;
-    double *ip, *iux, *iuy, *iuz, *imx, *imy, *imz;
-    double omega, iomega;
-    int save;
-    __device__ __forceinline__ void operator()(double *f, double *, int n, bool bnd) const {
-        double vx = ux[n], vy = uy[n], vz = LA::nd == 3 ? uz[n] : 0.0;
-        double a, bx, by, bz, mx, my, mz, feq[LA::nc];
-        ans_macro<LA>(f, vx, vy, vz, a, bx, by, bz, mx, my, mz);
-        ans_brinkman<LA>(f, rho[n], vx, vy, vz, mx, my, mz, alpha[n]);
-        ans_macro<LA>(f, vx, vy, vz, a, bx, by, bz, mx, my, mz);
-        if (save == 1 || (save == 2 && bnd)) {
-            ip[n] = a;
-            iux[n] = bx;
-            iuy[n] = by;
-            imx[n] = mx;
-            imy[n] = my;
-            if constexpr (LA::nd == 3) {
-                iuz[n] = bz;
-                imz[n] = mz;
-            }
-        }
-        ans_eq<LA>(feq, vx, vy, vz, a, bx, by, bz);
-        bgk<LA>(f, feq, omega, iomega);
-    }
-};
-
-// AAD::MacroBrinkmanCollideForceConvection, adjointadvection.h:997-1109
-template <class LA, class LB>
-struct AADBody {
-    const double *rho, *ux, *uy, *uz, *alpha, *tem, *diff;
-    double *ip, *iux, *iuy, *iuz, *imx, *imy, *imz, *item, *iqx, *iqy, *iqz, *igsnap;
-    double omegaf, iomegaf;
-    int save, N;
-    __device__ __forceinline__ void operator()(double *f, double *g, int n, bool bnd) const {
-        double omegag = 1.0 / (3.0 * diff[n] + 0.5), iomegag = 1.0 - omegag;
-        double vx = ux[n], vy = uy[n], vz = LA::nd == 3 ? uz[n] : 0.0;
-        double a, bx, by, bz, mx, my, mz, it, hx, hy, hz;
-        ans_macro<LA>(f, vx, vy, vz, a, bx, by, bz, mx, my, mz);
-        aad_macro<LB>(g, it, hx, hy, hz);
-        aad_brinkman<LA>(f, rho[n], vx, vy, vz, mx, my, mz, tem[n], hx, hy, hz, omegag, alpha[n]);
-        ans_macro<LA>(f, vx, vy, vz, a, bx, by, bz, mx, my, mz);
-        if (save == 1 || (save == 2 && bnd)) {
-            ip[n] = a;
-            iux[n] = bx;
-            iuy[n] = by;
-            imx[n] = mx;
-            imy[n] = my;
-            item[n] = it;
-            iqx[n] = hx;
-            iqy[n] = hy;
-            if constexpr (LA::nd == 3) {
-                iuz[n] = bz;
-                imz[n] = mz;
-                iqz[n] = hz;
-            }
-            if (igsnap && save == 1) store_node<LB>(igsnap, N, n, g);
-        }
-        {
-            double feq[LA::nc];
-            ans_eq<LA>(feq, vx, vy, vz, a, bx, by, bz);
-            bgk<LA>(f, feq, omegaf, iomegaf);
-        }
-        double ge = aad_eq<LB>(it, hx, hy, hz, vx, vy, vz);
-        sfor<0, LB::nc>([&](auto c) {
-            constexpr int C = decltype(c)::value;
-            g[C] = iomegag * g[C] + omegag * ge;
-        });
-    }
-};
-
-// ------------------------------------------------------------------------------------------
-// Collide kernels.  In place: the reference's own collide (no pending stream / boundaries).
-// Fused: pull + boundary program + collide, reading `s*` and writing `d*`; nodes owned by the
-// smoothing fix-up kernels are skipped.  Fix-up: the smoothed nodes (2D corners; 3D edges and
-// corners), whose state averages neighbours' post-boundary states.
-// ------------------------------------------------------------------------------------------
-template <class LA, class LB, class Body>
-__global__ void __launch_bounds__(kBlock) k_collide_inplace(double *a, double *b, Grid g, Body body) {
-    int n = blockIdx.x * blockDim.x + threadIdx.x;
-    int N = g.N;
-    if (n >= N) return;
-    int i, j, k;
-    coords(n, g, i, j, k);
-    double fa[LA::nc], fb[LB::nc > 0 ? LB::nc : 1];
-    load_node<LA>(a, N, n, fa);
-    load_node<LB>(b, N, n, fb);
-    body(fa, fb, n, nboundary(i, j, k, g, LA::nd) > 0);
-    store_node<LA>(a, N, n, fa);
-    store_node<LB>(b, N, n, fb);
-}
-
-template <class LA, class LB, class Body>
-__global__ void __launch_bounds__(kBlock) k_collide_fused(const double *__restrict__ sa, const double *__restrict__ sb,
-                                                          double *__restrict__ da, double *__restrict__ db, Grid g,
-                                                          ProgView pv, int any_smooth, Body body) {
-    int tile = xcd_tile(blockIdx.x, gridDim.x);
-    int n = tile * blockDim.x + threadIdx.x;
-    if (n >= g.N) return;
-    int i, j, k;
-    coords(n, g, i, j, k);
-    int nb = nboundary(i, j, k, g, LA::nd);
-    if (any_smooth && nb >= 2) return;
-    double fa[LA::nc], fb[LB::nc > 0 ? LB::nc : 1];
-    node_state<LA, LB>(sa, sb, g, pv, n, fa, fb);
-    body(fa, fb, n, nb > 0);
-    store_node<LA>(da, g.N, n, fa);
-    store_node<LB>(db, g.N, n, fb);
-}
-
-// State of node n with SMOOTH operators applied; LEVEL bounds the neighbour recursion (2D corners
-// and 3D edges: 1, 3D corners: 2).
-template <class LA, class LB, int LEVEL>
-__device__ void smoothed_state(const double *sa, const double *sb, const Grid &g, const ProgView &pv, int n, double *a,
-                               double *b) {
-    if constexpr (LEVEL == 0) {
-        node_state<LA, LB>(sa, sb, g, pv, n, a, b);
-    } else {
-        int i, j, k;
-        coords(n, g, i, j, k);
-        load_pull<LA>(sa, g, i, j, k, n, pv.pull[0], a);
-        load_pull<LB>(sb, g, i, j, k, n, pv.pull[1], b);
-        int slot = boundary_slot(i, j, k, g.nx, g.ny, g.nz, LA::nd);
-        int e = (slot >= 0 && slot < pv.nslot) ? pv.slot_entry[slot] : -1;
-        if (e < 0) return;
-        int2 r = pv.entry_ops[e];
-        for (int t = r.x; t < r.x + r.y; ++t) {
-            Op op = pv.ops[t];
-            if (op.code != OP_SMOOTH) {
-                apply_op<LA, LB>(op, a, b, pv, n);
-                continue;
-            }
-            // inward neighbours in axis order
-            int nb[3], cnt = 0;
-            int ci[3] = {i, j, k}, cn[3] = {g.nx, g.ny, g.nz};
-            for (int ax = 0; ax < LA::nd; ++ax) {
-                int d = ci[ax] == 0 ? 1 : (ci[ax] == cn[ax] - 1 ? -1 : 0);
-                if (d == 0) continue;
-                int c2[3] = {i, j, k};
-                c2[ax] += d;
-                nb[cnt++] = c2[0] + g.nx * (c2[1] + g.ny * c2[2]);
-            }
-            const bool onA = (op.lat ^ pv.latswap) == 0;
-            constexpr int NCM = LA::nc > LB::nc ? LA::nc : LB::nc;
-            double s0[NCM], s1[NCM], s2[NCM];
-            for (int q = 0; q < cnt; ++q) {
-                double ta[LA::nc], tb[LB::nc > 0 ? LB::nc : 1];
-                smoothed_state<LA, LB, LEVEL - 1>(sa, sb, g, pv, nb[q], ta, tb);
-                double *dst = q == 0 ? s0 : (q == 1 ? s1 : s2);
-                if (onA) {
-                    sfor<0, LA::nc>([&](auto c) { dst[decltype(c)::value] = ta[decltype(c)::value]; });
-                } else if constexpr (LB::nc > 0) {
-                    sfor<0, LB::nc>([&](auto c) { dst[decltype(c)::value] = tb[decltype(c)::value]; });
-                }
-            }
-            if (onA) {
-                sfor<0, LA::nc>([&](auto c) {
-                    constexpr int C = decltype(c)::value;
-                    a[C] = cnt == 2 ? 0.5 * (s0[C] + s1[C]) : (s0[C] + s1[C] + s2[C]) / 3.0;
-                });
-            } else if constexpr (LB::nc > 0) {
-                sfor<0, LB::nc>([&](auto c) {
-                    constexpr int C = decltype(c)::value;
-                    b[C] = cnt == 2 ? 0.5 * (s0[C] + s1[C]) : (s0[C] + s1[C] + s2[C]) / 3.0;
-                });
-            }
-        }
-    }
-}
-
-template <class LA, class LB, class Body, int LEVEL>
-__global__ void __launch_bounds__(kBlock) k_collide_fixup(const double *__restrict__ sa, const double *__restrict__ sb,
-                                                          double *__restrict__ da, double *__restrict__ db, Grid g,
-                                                          ProgView pv, const int *__restrict__ nodes, int count,
-                                                          Body body) {
-    int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= count) return;
-    int n = nodes[e];
-    double fa[LA::nc], fb[LB::nc > 0 ? LB::nc : 1];
-    smoothed_state<LA, LB, LEVEL>(sa, sb, g, pv, n, fa, fb);
-    body(fa, fb, n, true);
-    store_node<LA>(da, g.N, n, fa);
-    store_node<LB>(db, g.N, n, fb);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -549,7 +278,6 @@ __global__ void __launch_bounds__(kBlock) k_density_filter(const double *v, doub
 // ==========================================================================================
 // typed launchers
 // ==========================================================================================
-#define LBM_LAUNCH(kernel, grid, stream, ...) hipLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), 0, stream, __VA_ARGS__)
 
 template <class L>
 static hipError_t launch_stream_t(const double *src, double *dst, Grid g, int pull, hipStream_t s) {
@@ -581,33 +309,6 @@ hipError_t launch_smooth(int kind, double *f, int N, const int4 *list, int count
     return hipErrorInvalidValue;
 }
 
-// Dispatch a functor over the supported lattices (single) or (A, B) lattice pairs (coupled).
-template <class F>
-static hipError_t dispatch_single(int ka, F &&f) {
-    switch (ka) {
-    case kD2Q9: return f(D2Q9{}, NoLat{});
-    case kD3Q15: return f(D3Q15{}, NoLat{});
-    case kD2Q5: return f(D2Q5{}, NoLat{});
-    case kD3Q7: return f(D3Q7{}, NoLat{});
-    }
-    return hipErrorInvalidValue;
-}
-template <class F>
-static hipError_t dispatch_coupled(int ka, int kb, F &&f) {
-    if (ka == kD2Q9) {
-        if (kb == kD2Q9) return f(D2Q9{}, D2Q9{});
-        if (kb == kD2Q5) return f(D2Q9{}, D2Q5{});
-    } else if (ka == kD3Q15) {
-        if (kb == kD3Q15) return f(D3Q15{}, D3Q15{});
-        if (kb == kD3Q7) return f(D3Q15{}, D3Q7{});
-    }
-    return hipErrorInvalidValue;
-}
-template <class F>
-static hipError_t dispatch_pair(int ka, int kb, F &&f) {
-    return kb < 0 ? dispatch_single(ka, f) : dispatch_coupled(ka, kb, f);
-}
-
 hipError_t launch_program_apply(int ka, int kb, double *a, double *b, int N, const ProgView &pv, int seg,
                                 hipStream_t s) {
     if (pv.nentry == 0) return hipSuccess;
@@ -616,80 +317,6 @@ hipError_t launch_program_apply(int ka, int kb, double *a, double *b, int N, con
         using LB = decltype(lb);
         LBM_LAUNCH((k_program_apply<LA, LB>), nblocks(pv.nentry), s, a, b, N, pv, seg);
         return hipGetLastError();
-    });
-}
-
-// Collide launch: body built per (LA, LB) by `make`.
-template <template <class, class> class Body, bool COUPLED, class Make>
-static hipError_t launch_collide(const CollideLaunch &cl, Make &&make) {
-    auto go = [&](auto la, auto lb) -> hipError_t {
-        using LA = decltype(la);
-        using LB = decltype(lb);
-        Body<LA, LB> body = make(la, lb);
-        const Grid &g = cl.g;
-        if (!cl.pv) {
-            LBM_LAUNCH((k_collide_inplace<LA, LB, Body<LA, LB>>), nblocks(g.N), cl.stream, cl.da, cl.db, g, body);
-            return hipGetLastError();
-        }
-        const ProgView &pv = *cl.pv;
-        if (cl.n_fix_corner > 0) {
-            if constexpr (LA::nd == 3)
-                LBM_LAUNCH((k_collide_fixup<LA, LB, Body<LA, LB>, 2>), nblocks(cl.n_fix_corner), cl.stream, cl.sa, cl.sb,
-                           cl.da, cl.db, g, pv, cl.fix_corner, cl.n_fix_corner, body);
-            else
-                LBM_LAUNCH((k_collide_fixup<LA, LB, Body<LA, LB>, 1>), nblocks(cl.n_fix_corner), cl.stream, cl.sa, cl.sb,
-                           cl.da, cl.db, g, pv, cl.fix_corner, cl.n_fix_corner, body);
-        }
-        if (cl.n_fix_edge > 0) {
-            LBM_LAUNCH((k_collide_fixup<LA, LB, Body<LA, LB>, 1>), nblocks(cl.n_fix_edge), cl.stream, cl.sa, cl.sb,
-                       cl.da, cl.db, g, pv, cl.fix_edge, cl.n_fix_edge, body);
-        }
-        LBM_LAUNCH((k_collide_fused<LA, LB, Body<LA, LB>>), nblocks(g.N), cl.stream, cl.sa, cl.sb, cl.da, cl.db, g, pv,
-                   cl.any_smooth, body);
-        return hipGetLastError();
-    };
-    if constexpr (COUPLED) return dispatch_coupled(cl.ka, cl.kb, go);
-    else return cl.kb < 0 ? dispatch_single(cl.ka, go) : hipErrorInvalidValue;
-}
-
-hipError_t launch_ns_collide(const CollideLaunch &cl, const NSArgs &a) {
-    double omega = 1.0 / (3.0 * a.nu + 0.5);
-    return launch_collide<NSBody, false>(cl, [&](auto la, auto lb) {
-        NSBody<decltype(la), decltype(lb)> b;
-        b.rho = a.rho; b.ux = a.ux; b.uy = a.uy; b.uz = a.uz; b.alpha = a.alpha;
-        b.omega = omega; b.iomega = 1.0 - omega; b.save = a.save; b.brinkman = a.alpha != nullptr;
-        return b;
-    });
-}
-hipError_t launch_ad_collide(const CollideLaunch &cl, const ADArgs &a) {
-    double omega = 1.0 / (3.0 * a.nu + 0.5);
-    return launch_collide<ADBody, true>(cl, [&](auto la, auto lb) {
-        ADBody<decltype(la), decltype(lb)> b;
-        b.rho = a.rho; b.ux = a.ux; b.uy = a.uy; b.uz = a.uz; b.tem = a.tem; b.qx = a.qx; b.qy = a.qy; b.qz = a.qz;
-        b.gsnap = a.gsnap; b.alpha = a.alpha; b.diff = a.diff; b.heat = a.heat;
-        b.omegaf = omega; b.iomegaf = 1.0 - omega; b.save = a.save; b.N = cl.g.N;
-        return b;
-    });
-}
-hipError_t launch_ans_collide(const CollideLaunch &cl, const ANSArgs &a) {
-    double omega = 1.0 / (3.0 * a.nu + 0.5);
-    return launch_collide<ANSBody, false>(cl, [&](auto la, auto lb) {
-        ANSBody<decltype(la), decltype(lb)> b;
-        b.rho = a.rho; b.ux = a.ux; b.uy = a.uy; b.uz = a.uz; b.alpha = a.alpha;
-        b.ip = a.ip; b.iux = a.iux; b.iuy = a.iuy; b.iuz = a.iuz; b.imx = a.imx; b.imy = a.imy; b.imz = a.imz;
-        b.omega = omega; b.iomega = 1.0 - omega; b.save = a.save;
-        return b;
-    });
-}
-hipError_t launch_aad_collide(const CollideLaunch &cl, const AADArgs &a) {
-    double omega = 1.0 / (3.0 * a.nu + 0.5);
-    return launch_collide<AADBody, true>(cl, [&](auto la, auto lb) {
-        AADBody<decltype(la), decltype(lb)> b;
-        b.rho = a.rho; b.ux = a.ux; b.uy = a.uy; b.uz = a.uz; b.alpha = a.alpha; b.tem = a.tem; b.diff = a.diff;
-        b.ip = a.ip; b.iux = a.iux; b.iuy = a.iuy; b.iuz = a.iuz; b.imx = a.imx; b.imy = a.imy; b.imz = a.imz;
-        b.item = a.item; b.iqx = a.iqx; b.iqy = a.iqy; b.iqz = a.iqz; b.igsnap = a.igsnap;
-        b.omegaf = omega; b.iomegaf = 1.0 - omega; b.save = a.save; b.N = cl.g.N;
-        return b;
     });
 }
 

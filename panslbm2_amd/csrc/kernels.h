@@ -74,51 +74,68 @@ __device__ __forceinline__ void bounce(double *f, int face, int type) {
     }
 }
 
+// Field values one node's boundary operators read (one field set, at the node).
+struct NodeFields {
+    double rho, ux, uy, uz, tem, diff;
+    bool has_diff;  // the set has a diffusivity field (else AD_SET_Q uses its constant)
+};
+__device__ __forceinline__ NodeFields load_fields(const FieldSet &s, int n) {
+    NodeFields v;
+    v.rho = s.rho ? s.rho[n] : 0.0;
+    v.ux = s.ux ? s.ux[n] : 0.0;
+    v.uy = s.uy ? s.uy[n] : 0.0;
+    v.uz = s.uz ? s.uz[n] : 0.0;
+    v.tem = s.tem ? s.tem[n] : 0.0;
+    v.diff = s.diff ? s.diff[n] : 0.0;
+    v.has_diff = s.diff != nullptr;
+    return v;
+}
+
 // Apply one boundary operator to lattice L's populations `f`; `o` holds the other lattice's
 // populations (lattice LO), read by the coupled adjoint pressure operator.
 template <class L, class LO>
-__device__ __forceinline__ void apply_on(const Op &op, double *f, const double *o, const FieldSet &s, int n) {
+__device__ __forceinline__ void apply_on(const Op &op, double *f, const double *o, const NodeFields &s) {
     switch (op.code) {
     case OP_BOUNCE: bounce<L, false>(f, op.face, op.arg); break;
     case OP_IBOUNCE: bounce<L, true>(f, op.face, op.arg); break;
     case OP_NS_SET_U:
         if constexpr (L::kind == kD2Q9) ns_set_u_2d(f, op.face, op.v[0], op.v[1]);
-        else if constexpr (L::kind == kD3Q15) ns_set_u_3d(f, op.face, op.v[0], op.v[1], op.v[2]);
+        else if constexpr (L::kind == kD3Q15) ns_set_u_3d<L>(f, op.face, op.v[0], op.v[1], op.v[2]);
         break;
     case OP_NS_SET_RHO:
         if constexpr (L::kind == kD2Q9) ns_set_rho_2d(f, op.face, op.v[0], op.v[1]);
-        else if constexpr (L::kind == kD3Q15) ns_set_rho_3d(f, op.face, op.v[0], op.v[1], op.v[2]);
+        else if constexpr (L::kind == kD3Q15) ns_set_rho_3d<L>(f, op.face, op.v[0], op.v[1], op.v[2]);
         break;
     case OP_AD_SET_T:
-        if constexpr (L::nd == 2) ad_set_t_2d<L>(f, op.face, op.v[0], s.ux[n], s.uy[n]);
-        else if constexpr (L::nd == 3) ad_set_t_3d<L>(f, op.face, op.v[0], s.ux[n], s.uy[n], s.uz[n]);
+        if constexpr (L::nd == 2) ad_set_t_2d<L>(f, op.face, op.v[0], s.ux, s.uy);
+        else if constexpr (L::nd == 3) ad_set_t_3d<L>(f, op.face, op.v[0], s.ux, s.uy, s.uz);
         break;
     case OP_AD_SET_Q: {
-        double d = s.diff ? s.diff[n] : op.v[1];
-        if constexpr (L::nd == 2) ad_set_q_2d<L>(f, op.face, op.v[0], s.ux[n], s.uy[n], d);
-        else if constexpr (L::nd == 3) ad_set_q_3d<L>(f, op.face, op.v[0], s.ux[n], s.uy[n], s.uz[n], d);
+        double d = s.has_diff ? s.diff : op.v[1];
+        if constexpr (L::nd == 2) ad_set_q_2d<L>(f, op.face, op.v[0], s.ux, s.uy, d);
+        else if constexpr (L::nd == 3) ad_set_q_3d<L>(f, op.face, op.v[0], s.ux, s.uy, s.uz, d);
     } break;
     case OP_ANS_ISET_U:
         if constexpr (L::kind == kD2Q9) ans_iset_u_2d(f, op.face, op.v[0], op.v[1], op.v3, op.arg);
-        else if constexpr (L::kind == kD3Q15) ans_iset_u_3d(f, op.face, op.v[0], op.v[1], op.v[2], op.v3);
+        else if constexpr (L::kind == kD3Q15) ans_iset_u_3d<L>(f, op.face, op.v[0], op.v[1], op.v[2], op.v3);
         break;
     case OP_ANS_ISET_RHO:
         if constexpr (L::kind == kD2Q9) ans_iset_rho_2d(f, op.face);
-        else if constexpr (L::kind == kD3Q15) ans_iset_rho_3d(f, op.face);
+        else if constexpr (L::kind == kD3Q15) ans_iset_rho_3d<L>(f, op.face);
         break;
     case OP_AAD_ISET_T:
-        if constexpr (L::nd == 2) aad_iset_t_2d<L>(f, op.face, s.ux[n], s.uy[n]);
-        else if constexpr (L::nd == 3) aad_iset_t_3d<L>(f, op.face, s.ux[n], s.uy[n], s.uz[n]);
+        if constexpr (L::nd == 2) aad_iset_t_2d<L>(f, op.face, s.ux, s.uy);
+        else if constexpr (L::nd == 3) aad_iset_t_3d<L>(f, op.face, s.ux, s.uy, s.uz);
         break;
     case OP_AAD_ISET_Q:
-        if constexpr (L::nd == 2) aad_iset_q_2d<L>(f, op.face, s.ux[n], s.uy[n], op.v3);
-        else if constexpr (L::nd == 3) aad_iset_q_3d<L>(f, op.face, s.ux[n], s.uy[n], s.uz[n], op.v3);
+        if constexpr (L::nd == 2) aad_iset_q_2d<L>(f, op.face, s.ux, s.uy, op.v3);
+        else if constexpr (L::nd == 3) aad_iset_q_3d<L>(f, op.face, s.ux, s.uy, s.uz, op.v3);
         break;
     case OP_AAD_ISET_RHO:
         if constexpr (L::kind == kD2Q9 && LO::nd == 2)
-            aad_iset_rho_2d<LO>(f, o, op.face, op.arg, s.rho[n], s.ux[n], s.uy[n], s.tem[n], op.v3);
+            aad_iset_rho_2d<LO>(f, o, op.face, op.arg, s.rho, s.ux, s.uy, s.tem, op.v3);
         else if constexpr (L::kind == kD3Q15 && LO::nd == 3)
-            aad_iset_rho_3d<LO>(f, o, op.face, op.arg, s.rho[n], s.ux[n], s.uy[n], s.uz[n], s.tem[n], op.v3);
+            aad_iset_rho_3d<LO>(f, o, op.face, op.arg, s.rho, s.ux, s.uy, s.uz, s.tem, op.v3);
         break;
     default: break;
     }
@@ -131,11 +148,38 @@ __device__ __forceinline__ FieldSet field_set(const ProgView &pv, int k) {
     return k == 0 ? pv.fs[0] : (k == 1 ? pv.fs[1] : (k == 2 ? pv.fs[2] : pv.fs[3]));
 }
 
+// The field sets a node's operators read most often (0 and 1) are loaded once per node, with the
+// operators; operators on sets 2 and 3 load theirs at use.
+struct NodeFieldCache {
+    NodeFields f0, f1;
+};
+__device__ __forceinline__ NodeFieldCache prefetch_fields(const ProgView &pv, int n) {
+    return NodeFieldCache{load_fields(pv.fs[0], n), load_fields(pv.fs[1], n)};
+}
+
 template <class LA, class LB>
-__device__ __forceinline__ void apply_op(const Op &op, double *a, double *b, const ProgView &pv, int n) {
-    const FieldSet s = field_set(pv, op.fset);
-    if ((op.lat ^ pv.latswap) == 0) apply_on<LA, LB>(op, a, b, s, n);
-    else if constexpr (LB::nc > 0) apply_on<LB, LA>(op, b, a, s, n);
+__device__ __forceinline__ void apply_op(const Op &op, double *a, double *b, const ProgView &pv, int n,
+                                         const NodeFieldCache &fc) {
+    NodeFields s = op.fset == 0 ? fc.f0 : (op.fset == 1 ? fc.f1 : load_fields(field_set(pv, op.fset), n));
+    if ((op.lat ^ pv.latswap) == 0) apply_on<LA, LB>(op, a, b, s);
+    else if constexpr (LB::nc > 0) apply_on<LB, LA>(op, b, a, s);
+}
+
+// Run the operators of boundary slot `slot` (node n) except SMOOTH, in record order.  The slot's
+// operators sit at sops[slot * maxops ...], padded with OP_NONE to a multiple of kOpBatch, so a
+// batch is fetched with independent loads together with the node's fields.
+template <class LA, class LB>
+__device__ __forceinline__ void run_slot_ops(const ProgView &pv, int slot, int n, double *a, double *b) {
+    const Op *p = pv.sops + (size_t)slot * pv.maxops;
+    NodeFieldCache fc = prefetch_fields(pv, n);
+    for (int base = 0; base < pv.maxops; base += kOpBatch) {
+        Op o[kOpBatch];
+        sfor<0, kOpBatch>([&](auto q) { o[decltype(q)::value] = p[base + decltype(q)::value]; });
+        sfor<0, kOpBatch>([&](auto q) {
+            const Op &op = o[decltype(q)::value];
+            if (op.code != OP_NONE && op.code != OP_SMOOTH) apply_op<LA, LB>(op, a, b, pv, n, fc);
+        });
+    }
 }
 
 // Pull node n's populations: pull = +1 from x - c (Stream), -1 from x + c (iStream), 0 in place.
@@ -194,16 +238,7 @@ __device__ __forceinline__ void node_state(const double *sa, const double *sb, c
     load_pull<LA>(sa, g, i, j, k, n, pv.pull[0], a);
     load_pull<LB>(sb, g, i, j, k, n, pv.pull[1], b);
     int slot = boundary_slot(i, j, k, g.nx, g.ny, g.nz, LA::nd);
-    if (slot >= 0 && slot < pv.nslot) {
-        int e = pv.slot_entry[slot];
-        if (e >= 0) {
-            int2 r = pv.entry_ops[e];
-            for (int t = r.x; t < r.x + r.y; ++t) {
-                Op op = pv.ops[t];
-                if (op.code != OP_SMOOTH) apply_op<LA, LB>(op, a, b, pv, n);
-            }
-        }
-    }
+    if (slot >= 0 && slot < pv.nslot) run_slot_ops<LA, LB>(pv, slot, n, a, b);
 }
 
 }  // namespace lbm

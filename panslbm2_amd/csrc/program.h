@@ -29,7 +29,7 @@ enum OpCode : uint8_t {
     OP_SMOOTH = 12,       // d2q9.h:577-587, d3q15.h:1241-1303 (fix-up kernels only)
 };
 
-// One operator at one node.  32 bytes, naturally aligned for 8-byte loads.
+// One operator at one node.  40 bytes, naturally aligned for 8-byte loads.
 struct Op {
     uint8_t code;
     uint8_t face;  // 0 xmin, 1 xmax, 2 ymin, 3 ymax, 4 zmin, 5 zmax
@@ -48,6 +48,7 @@ struct FieldSet {
     const double *rho, *ux, *uy, *uz, *tem, *diff;
 };
 constexpr int kMaxFieldSets = 4;
+constexpr int kOpBatch = 4;  // operators fetched per batch by the fused kernels
 
 // Device view of a program.
 struct ProgView {
@@ -55,12 +56,20 @@ struct ProgView {
     const int *entry_node;  // entry -> node index
     const int2 *entry_ops;  // entry -> (first op, op count)
     const Op *ops;
+    const Op *sops;         // slot-major copy: operators of slot s at sops[s * maxops + k], OP_NONE-padded
+    int maxops;             // operators per slot in sops, a multiple of kOpBatch (0: no operators)
     int nentry;
     int nslot;
     FieldSet fs[kMaxFieldSets];
     int pull[2];            // per kernel lattice: +1 stream (pull from x - c), -1 iStream (x + c), 0 none
     int latswap;            // 1 when the program's lattice 0 is the kernel's second lattice
+    // 2D corner smoothing: the fused pass saves the post-boundary state of each corner's inward
+    // neighbours here (kSideStride doubles each: the kernel's first lattice, then its second) and a
+    // corner kernel launched after it averages them.  nullptr when unused.
+    const int *side_of_slot;  // boundary slot -> side-state index or -1
+    double *side;
 };
+constexpr int kSideStride = 32;  // >= nc(first) + nc(second) of any lattice pair
 
 // Boundary slot of node (i, j, k) on an nx*ny*nz grid, -1 for interior nodes.  Surface nodes are
 // numbered z-faces first, then the y-faces of the remaining layers, then the x-faces.

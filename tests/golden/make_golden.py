@@ -25,10 +25,13 @@ import sequences  # noqa: E402
 from _oracle import ref  # noqa: E402
 
 
-def main():
+def main(only=None):
+    """Write every fixture, or (only = list of case names) just those cases' fixtures."""
     if ref() is None:
         raise SystemExit("reference harness not built: make -C oracle ref")
     for case in refcases.CASES_2D + refcases.CASES_3D:
+        if only and case.name not in only:
+            continue
         inp = refcases.make_inputs(case)
         blob = {"in/" + k: np.asarray(v) for k, v in inp.items()}
         for op in refcases.ops_for(case):
@@ -38,6 +41,8 @@ def main():
         path = os.path.join(HERE, case.name + ".npz")
         np.savez_compressed(path, **blob)
         print("wrote", path, os.path.getsize(path), "bytes")
+    if only:
+        return
     for seq in sequences.SEQUENCES:
         out = seq.run(sequences.RefBackend())
         path = os.path.join(HERE, "seq_" + seq.name + ".npz")
@@ -61,4 +66,4 @@ if __name__ == "__main__":
     if sys.argv[1:] == ["driver"]:
         write_driver_golden()
     else:
-        main()
+        main(sys.argv[1:] or None)

@@ -356,12 +356,27 @@ POW_OPS = {"sens_brinkman_diffusivity", "sens_temp_heat_source", "residual", "de
 
 CASES_2D = [Case("d2q9_12x10", D2Q9, (12, 10), 11, "D2Q9"), Case("d2q9_7x9", D2Q9, (7, 9), 12, "D2Q9")]
 
-# 3D cases are filled in once the D3Q15 boundary operators are restated.
-CASES_3D = []
+CASES_3D = [Case("d3q15_5x4x6", D3Q15, (5, 4, 6), 21, "D3Q15"), Case("d3q15_4x6x5", D3Q15, (4, 6, 5), 22, "D3Q15")]
+
+# D3Q15.  AAD::iBoundaryConditionSetRho has no 3D case: the reference's D3Q15 overload does not
+# compile (adjointadvection.h:583 names an undeclared `_bctype`); the 3D boundary term of
+# SensitivityTemperatureAtHeatSource is not restated.
+OPS_3D = {
+    "stream": "d3q15.h:602-613", "istream": "d3q15.h:965-976", "bc": "d3q15.h:982-1110",
+    "ibc": "d3q15.h:1112-1239", "smooth": "d3q15.h:199-220,1241-1303",
+    "ns_set_u": "navierstokes.h:150-257", "ns_set_rho": "navierstokes.h:319-426",
+    "ans_iset_u": "adjointnavierstokes.h:147-254", "ans_iset_rho": "adjointnavierstokes.h:306-392",
+    "ad_set_t": "advection.h:149-238", "ad_set_q_field": "advection.h:435-524", "ad_set_q_const": "advection.h:292-381",
+    "aad_iset_t": "adjointadvection.h:202-300", "aad_iset_q": "adjointadvection.h:368-484",
+    "ns_init": "navierstokes.h:563-572", "ad_init": "advection.h:1061-1070", "ans_init": "adjointnavierstokes.h:488-498",
+    "aad_init": "adjointadvection.h:1372-1381",
+    "ns_collide": "navierstokes.h:459-483", "ns_brinkman_collide": "navierstokes.h:518-546",
+    "ad_collide_fc": "advection.h:871-926", "ans_collide": "adjointnavierstokes.h:435-470",
+    "aad_collide_fc": "adjointadvection.h:1054-1109",
+    "sens_brinkman": "adjointnavierstokes.h:550-554", "sens_brinkman_diffusivity": "adjointadvection.h:1478-1493",
+    "residual": "residual.h:39-51", "normalize": "normalize.h:9-24", "density_filter": "densityfilter.h:390-420,512-515",
+}
 
 
 def ops_for(case):
     return list(OPS_2D) if ND[case.kind] == 2 else list(OPS_3D)
-
-
-OPS_3D = {}
