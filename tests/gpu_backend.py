@@ -153,6 +153,12 @@ class GpuBackend:
             q.h, *[self.f(n) for n in (dfds, ux, uy, uz, imx, imy, imz, dads, tem, item, iqx, iqy, iqz, g, ig, diff,
                                        dkds)], hptr(self._fi(q, mask)), hptr(self._face(q, qn))))
 
+    def sens_brinkman_diffusivity(self, q, dfds, ux, uy, uz, imx, imy, imz, dads, tem, item, iqx, iqy, iqz, g, ig,
+                                  diff, dkds):
+        check(self.L.lbm_sensitivity_brinkman_diffusivity(
+            q.h, *[self.f(n) for n in (dfds, ux, uy, uz, imx, imy, imz, dads, tem, item, iqx, iqy, iqz, g, ig, diff,
+                                       dkds)]))
+
     def residual(self, a, b):
         n = len(a)
         aa = [self.f(x) for x in a] + [None] * (3 - n)
@@ -240,3 +246,7 @@ class FusedGpuBackend(GpuBackend):
     def sens_temp_heat_source(self, *a):
         self._flush()
         return super().sens_temp_heat_source(*a)
+
+    def sens_brinkman_diffusivity(self, *a):
+        self._flush()
+        return super().sens_brinkman_diffusivity(*a)

@@ -9,7 +9,7 @@ import ctypes
 
 import numpy as np
 
-from _oracle import D2Q9, NC, ND, Lattice, aos_to_soa, lib, ptr, ref, soa_to_aos, to_face
+from _oracle import D2Q9, D3Q15, NC, ND, Lattice, aos_to_soa, lib, ptr, ref, soa_to_aos, to_face
 
 D = ctypes.c_double
 
@@ -141,6 +141,11 @@ class OracleBackend(_FieldStore):
         lib().ol_sens_temp_heat_source(q.h, *[ptr(self.f(n)) for n in (dfds, ux, uy, uz, imx, imy, imz, dads, tem,
                                                                        item, iqx, iqy, iqz, g, ig, diff, dkds)],
                                        ptr(self._face(q, mask, np.int32)), ptr(self._face(q, qn)))
+
+    def sens_brinkman_diffusivity(self, q, dfds, ux, uy, uz, imx, imy, imz, dads, tem, item, iqx, iqy, iqz, g, ig,
+                                  diff, dkds):
+        lib().ol_sens_brinkman_diffusivity(q.h, *[ptr(self.f(n)) for n in (dfds, ux, uy, uz, imx, imy, imz, dads, tem,
+                                                                           item, iqx, iqy, iqz, g, ig, diff, dkds)])
 
     def residual(self, a, b):
         n = len(a)
@@ -303,6 +308,14 @@ class RefBackend(_FieldStore):
                                                                                item, iqx, iqy)],
                                          ptr(gs), ptr(igs), ptr(self.f(diff)), ptr(self.f(dkds)),
                                          ptr(self._i(mask)), ptr(self._d(qn)))
+
+    def sens_brinkman_diffusivity(self, q, dfds, ux, uy, uz, imx, imy, imz, dads, tem, item, iqx, iqy, iqz, g, ig,
+                                  diff, dkds):
+        gs = np.ascontiguousarray(self.f(g).T).reshape(-1)  # reference snapshot layout nc*idx + c
+        igs = np.ascontiguousarray(self.f(ig).T).reshape(-1)
+        self._r(q, "sens_brinkman_diffusivity")(*q.L, *[ptr(self.f(n)) for n in (dfds, ux, uy, uz, imx, imy, imz,
+                                                                                  dads, tem, item, iqx, iqy, iqz)],
+                                                ptr(gs), ptr(igs), ptr(self.f(diff)), ptr(self.f(dkds)))
 
     def residual(self, a, b):
         n = len(a)
@@ -469,7 +482,130 @@ class ForcedConvection2D:
         return out
 
 
+class ForcedConvection3D:
+    """D3Q15 NS + D3Q15 AD forced convection in a duct, forward then adjoint, then the Brinkman and
+    diffusivity sensitivity: the call order of test/nsadsens.cpp:67-133 on the 3D overloads
+    (advection.h:871, adjointadvection.h:1054, adjointadvection.h:1478).  Walls y = 0, ly-1 and
+    z = 0, lz-1 bounce back; inlet x = 0 Zou-He velocity, outlet x = lx-1 Zou-He pressure; T = 0 at
+    the inlet, heat flux into the y = 0 wall for x < L, adiabatic elsewhere.  The adjoint outlet is
+    ANS::iBoundaryConditionSetRho3D (the reference's 3D AAD::iBoundaryConditionSetRho does not
+    compile)."""
+
+    def __init__(self, name, lx, ly, lz, nt, nu=0.1, pr=6.0, umax=0.05, qn=1e-2, seed=6, check=None):
+        self.name, self.lx, self.ly, self.lz, self.nt = name, lx, ly, lz, nt
+        self.nu, self.pr, self.umax, self.qn = nu, pr, umax, qn
+        self.L = lx // 2
+        self.seed = seed
+        self.check = check
+
+    def setup(self):
+        lx, ly, lz = self.lx, self.ly, self.lz
+        N = lx * ly * lz
+        n = np.arange(N)
+        i, j, k = n % lx, (n // lx) % ly, n // (lx * ly)
+        rng = np.random.default_rng(self.seed)
+        s = rng.uniform(0.0, 1.0, N)
+        qf, qg, amax = 0.01, 1.0, 1.0
+        dfl = self.nu / self.pr
+        dso = 10.0 * dfl
+        d = dict(
+            alpha=amax * qf * (1.0 - s) / (s + qf),
+            dads=-amax * qf * (1.0 + qf) / (s + qf) ** 2,
+            diff=dso + (dfl - dso) * s * (1.0 + qg) / (s + qg),
+            dkds=(dfl - dso) * qg * (1.0 + qg) / (s + qg) ** 2,
+        )
+        Hy, Hz = ly - 1.0, lz - 1.0
+        d["uin"] = 16.0 * self.umax * j * (Hy - j) * k * (Hz - k) / (Hy ** 2 * Hz ** 2)
+        side = (j == 0) | (j == ly - 1) | (k == 0) | (k == lz - 1)
+        inner = (j > 0) & (j < ly - 1) & (k > 0) & (k < lz - 1)
+        d["wall_f"] = side.astype(np.int32)
+        d["inlet"] = ((i == 0) & inner).astype(np.int32)
+        d["outlet"] = ((i == lx - 1) & inner).astype(np.int32)
+        d["wall_g"] = np.where(j == ly - 1, 2, 0).astype(np.int32)
+        d["heat_face"] = ((j == 0) & (i < self.L)).astype(np.int32)
+        d["flux_bc"] = (((j == 0) | (k == 0) | (k == lz - 1) | (i == lx - 1)) & (i > 0)).astype(np.int32)
+        d["qn"] = np.where((j == 0) & (i < self.L), self.qn, 0.0)
+        return d
+
+    def run(self, b):
+        lx, ly, lz = self.lx, self.ly, self.lz
+        N = lx * ly * lz
+        d = self.setup()
+        pf = b.lattice(D3Q15, lx, ly, lz)
+        pg = b.lattice(D3Q15, lx, ly, lz)
+        for k in ("alpha", "dads", "diff", "dkds"):
+            b.put(k, d[k])
+        b.put("rho", np.ones(N))
+        for k in ("ux", "uy", "uz", "uxp", "uyp", "uzp", "tem", "qx", "qy", "qz", "qxp", "qyp", "qzp",
+                  "ip", "iux", "iuy", "iuz", "iuxp", "iuyp", "iuzp", "imx", "imy", "imz",
+                  "item", "iqx", "iqy", "iqz", "iqxp", "iqyp", "iqzp", "dfds"):
+            b.zeros(k, N)
+        b.zeros("g", (NC[D3Q15], N))
+        b.zeros("ig", (NC[D3Q15], N))
+        zero = np.zeros(N)
+        b.ns_init(pf, "rho", "ux", "uy", "uz")
+        b.ad_init(pg, "tem", "ux", "uy", "uz")
+        res = []
+        vel, flx = ("ux", "uy", "uz"), ("qx", "qy", "qz")
+        for t in range(1, self.nt + 1):
+            b.ad_collide_fc(pf, "rho", "ux", "uy", "uz", "alpha", self.nu, pg, "tem", "qx", "qy", "qz", "diff",
+                            True, "g")
+            if self.check and t % self.check == 0:
+                res.append((b.residual(list(vel), [v + "p" for v in vel]), b.residual(list(flx), [v + "p" for v in flx])))
+            b.stream(pf)
+            b.stream(pg)
+            b.bc(pf, d["wall_f"])
+            b.ns_set_u(pf, d["inlet"], d["uin"], zero, zero)
+            b.ns_set_rho(pf, d["outlet"], np.ones(N), zero, zero)
+            b.smooth(pf)
+            b.bc(pg, d["wall_g"])
+            b.ad_set_t(pg, d["inlet"], zero, "ux", "uy", "uz")
+            b.ad_set_q(pg, d["flux_bc"], d["qn"], "ux", "uy", "uz", "diff")
+            b.smooth(pg)
+            for v in vel + flx:
+                b.swap(v, v + "p")
+        for v in vel + flx:
+            b.swap(v, v + "p")
+        out = {"f": b.pops(pf), "g": b.pops(pg)}
+        for k in ("rho", "ux", "uy", "uz", "tem", "qx", "qy", "qz", "g"):
+            out["fwd_" + k] = b.get(k)
+        b.ans_init(pf, "ux", "uy", "uz", "ip", "iux", "iuy", "iuz")
+        b.aad_init(pg, "ux", "uy", "uz", "item", "iqx", "iqy", "iqz")
+        ivel, iflx = ("iux", "iuy", "iuz"), ("iqx", "iqy", "iqz")
+        for t in range(1, self.nt + 1):
+            b.aad_collide_fc(pf, "rho", "ux", "uy", "uz", "ip", "iux", "iuy", "iuz", "imx", "imy", "imz", "alpha",
+                             self.nu, pg, "tem", "item", "iqx", "iqy", "iqz", "diff", True, "ig")
+            if self.check and t % self.check == 0:
+                res.append((b.residual(list(ivel), [v + "p" for v in ivel]),
+                            b.residual(list(iflx), [v + "p" for v in iflx])))
+            b.istream(pf)
+            b.istream(pg)
+            b.aad_iset_t(pg, d["inlet"], "ux", "uy", "uz")
+            b.aad_iset_q(pg, d["flux_bc"], "ux", "uy", "uz")
+            b.aad_iset_q(pg, d["heat_face"], "ux", "uy", "uz", 1.0)
+            b.bc(pg, d["wall_g"], adjoint=True)
+            b.smooth(pg)
+            b.bc(pf, d["wall_f"], adjoint=True)
+            b.ans_iset_u(pf, d["inlet"], d["uin"], zero, zero)
+            b.ans_iset_rho(pf, d["outlet"])
+            b.smooth(pf)
+            for v in ivel + iflx:
+                b.swap(v, v + "p")
+        for v in ivel + iflx:
+            b.swap(v, v + "p")
+        b.sens_brinkman_diffusivity(pg, "dfds", "ux", "uy", "uz", "imx", "imy", "imz", "dads", "tem", "item", "iqx",
+                                    "iqy", "iqz", "g", "ig", "diff", "dkds")
+        out.update({"adj_f": b.pops(pf), "adj_g": b.pops(pg), "dfds": b.get("dfds")})
+        # the 3D adjoint collide leaves iuz as it found it (adjointadvection.h:1074-1084)
+        for k in ("ip", "iux", "iuy", "iuz", "imx", "imy", "imz", "item", "iqx", "iqy", "iqz", "ig"):
+            out["adj_" + k] = b.get(k)
+        if res:
+            out["residuals"] = np.array(res)
+        return out
+
+
 SEQUENCES = [
     Cavity("cavity_21x21", 21, 21, 300),
     ForcedConvection2D("fc2d_24x16", 24, 16, 200, check=100),
+    ForcedConvection3D("fc3d_7x5x5", 7, 5, 5, 60, check=30),
 ]
