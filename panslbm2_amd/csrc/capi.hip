@@ -119,8 +119,9 @@ struct lbm_program {
     // device tables
     bool uploaded = false;
     DevBuf d_slot_entry, d_entry_node, d_entry_ops, d_ops, d_sops, d_fix_corner, d_fix_edge;
-    DevBuf d_side_of_slot, d_corner2d, d_side;  // 2D corner smoothing after the fused pass
-    int nentry = 0, nslot = 0, maxops = 0, n_fix_corner = 0, n_fix_edge = 0, n_corner2d = 0;
+    DevBuf d_side_of_slot, d_corner2d, d_side, d_side_nodes;  // 2D corner smoothing
+    int nentry = 0, nslot = 0, maxops = 0, n_fix_corner = 0, n_fix_edge = 0, n_corner2d = 0, n_side = 0;
+    bool corner_block_ok = false;  // the corners may be done by the fused pass's last workgroup
     bool owned_by_ctx = false;  // eager scratch program
     ~lbm_program() {
         d_slot_entry.release();
@@ -133,6 +134,7 @@ struct lbm_program {
         d_side_of_slot.release();
         d_corner2d.release();
         d_side.release();
+        d_side_nodes.release();
     }
 };
 
@@ -340,8 +342,9 @@ int prog_upload(lbm_program *P) {
     std::vector<int> fix_corner, fix_edge;
     // 2D: corners are done after the fused pass from their neighbours' saved states; 3D: edges and
     // corners are rebuilt by fix-up kernels before it
-    std::vector<int> side_of_slot;
+    std::vector<int> side_of_slot, side_nodes;
     std::vector<int4> corner2d;
+    P->corner_block_ok = false;
     if ((P->smoothed[0] || P->smoothed[1]) && L->nd == 2) {
         std::map<int, int> side;
         auto side_id = [&](int node) {
@@ -351,10 +354,20 @@ int prog_upload(lbm_program *P) {
             side[node] = id;
             return id;
         };
+        const int smoothed_bits = (P->smoothed[0] ? 1 : 0) | (P->smoothed[1] ? 2 : 0);
         for (const int4 &e : L->sm_corner) {
             int a = side_id(e.y), b = side_id(e.z);
-            corner2d.push_back(make_int4(e.x, a, b, -1));
+            corner2d.push_back(make_int4(e.x, a, b, smoothed_bits));
         }
+        side_nodes.assign(side.size(), 0);
+        for (auto &kv : side) side_nodes[kv.second] = kv.first;
+        // The corner workgroup gives a corner, per lattice, either the neighbours' mean (smoothed)
+        // or its own pull + operators: exact when no operator at a corner reads the other lattice.
+        bool cross = false;
+        for (const int4 &e : L->sm_corner)
+            for (const HostOp &h : P->ops)
+                if (h.node == e.x && h.op.code == lbm::OP_AAD_ISET_RHO) cross = true;
+        P->corner_block_ok = !cross && (int)side.size() <= lbm::kMaxSide && corner2d.size() + side.size() <= 256;
         side_of_slot.assign(P->nslot, -1);
         for (auto &kv : side) {
             int i, j, k;
@@ -385,6 +398,8 @@ int prog_upload(lbm_program *P) {
     if ((rc = up(P->d_fix_edge, fix_edge.data(), fix_edge.size() * sizeof(int)))) return rc;
     if ((rc = up(P->d_side_of_slot, side_of_slot.data(), side_of_slot.size() * sizeof(int)))) return rc;
     if ((rc = up(P->d_corner2d, corner2d.data(), corner2d.size() * sizeof(int4)))) return rc;
+    if ((rc = up(P->d_side_nodes, side_nodes.data(), side_nodes.size() * sizeof(int)))) return rc;
+    P->n_side = (int)side_nodes.size();
     P->uploaded = true;
     return LBM_OK;
 }
@@ -461,8 +476,11 @@ bool fields_ok(std::initializer_list<const void *> ps) {
 // ---------------------------------------------------------------------------------------------
 // collide dispatch: plain, fused, or program-then-plain
 // ---------------------------------------------------------------------------------------------
+// `saved` lists the field arrays the collide writes: a program whose boundary operators read one
+// of them cannot have its corners done concurrently with the pass.
 template <class Launch>
-int collide_call(lbm_program *P, lbm_lattice *p, lbm_lattice *q, Launch launch) {
+int collide_call(lbm_program *P, lbm_lattice *p, lbm_lattice *q, std::initializer_list<const double *> saved,
+                 Launch launch) {
     if (!p) return fail(LBM_ERR_ARG, "null lattice");
     if (q && (q->lx != p->lx || q->ly != p->ly || q->lz != p->lz))
         return fail(LBM_ERR_ARG, "coupled lattices must have the same grid");
@@ -512,6 +530,22 @@ int collide_call(lbm_program *P, lbm_lattice *p, lbm_lattice *q, Launch launch) 
     cl.n_fix_edge = P->n_fix_edge;
     cl.corner2d = (const int4 *)P->d_corner2d.p;
     cl.n_corner2d = P->n_corner2d;
+    if (P->corner_block_ok && P->n_corner2d > 0) {
+        bool alias = false;
+        for (const FieldSet &fs : P->fsets)
+            for (const double *r : {fs.rho, fs.ux, fs.uy, fs.uz, fs.tem, fs.diff})
+                for (const double *w : saved) alias |= (r != nullptr && r == w);
+        if (!alias) {
+            v.corner_block = 1;
+            v.nside = P->n_side;
+            v.ncorner = P->n_corner2d;
+            v.side_nodes = (const int *)P->d_side_nodes.p;
+            v.corners = (const int4 *)P->d_corner2d.p;
+            v.side_of_slot = nullptr;
+            v.side = nullptr;
+            cl.n_corner2d = 0;
+        }
+    }
     cl.any_smooth = (P->smoothed[0] || P->smoothed[1]) ? 1 : 0;
     rc = launch(cl);
     if (rc) return rc;
@@ -761,7 +795,7 @@ int lbm_ns_macro_collide_p(lbm_program *P, lbm_lattice *p, double *rho, double *
                            int issave) {
     if (!ns_lattice(p)) return fail(LBM_ERR_ARG, "NS collide needs a D2Q9 or D3Q15 lattice");
     if (issave && (!fields_ok({rho, ux, uy}) || (p->nd == 3 && !uz))) return fail(LBM_ERR_ARG, "null field");
-    return collide_call(P, p, nullptr, [&](const lbm::CollideLaunch &cl) {
+    return collide_call(P, p, nullptr, {rho, ux, uy, uz}, [&](const lbm::CollideLaunch &cl) {
         lbm::NSArgs a{rho, ux, uy, uz, nullptr, nu, issave};
         return hip_result(lbm::launch_ns_collide(cl, a), "ns collide");
     });
@@ -773,7 +807,7 @@ int lbm_ns_macro_brinkman_collide_p(lbm_program *P, lbm_lattice *p, double *rho,
                                     double nu, const double *alpha, int issave) {
     if (!ns_lattice(p) || !alpha) return fail(LBM_ERR_ARG, "bad arguments");
     if (issave && (!fields_ok({rho, ux, uy}) || (p->nd == 3 && !uz))) return fail(LBM_ERR_ARG, "null field");
-    return collide_call(P, p, nullptr, [&](const lbm::CollideLaunch &cl) {
+    return collide_call(P, p, nullptr, {rho, ux, uy, uz}, [&](const lbm::CollideLaunch &cl) {
         lbm::NSArgs a{rho, ux, uy, uz, alpha, nu, issave};
         return hip_result(lbm::launch_ns_collide(cl, a), "ns brinkman collide");
     });
@@ -816,7 +850,7 @@ int lbm_ad_macro_brinkman_collide_force_convection_p(lbm_program *P, lbm_lattice
     if (!ns_lattice(p) || !q || !alpha || !diff) return fail(LBM_ERR_ARG, "bad arguments");
     if (issave && (!fields_ok({rho, ux, uy, tem, qx, qy}) || (p->nd == 3 && (!uz || !qz))))
         return fail(LBM_ERR_ARG, "null field");
-    return collide_call(P, p, q, [&](const lbm::CollideLaunch &cl) {
+    return collide_call(P, p, q, {rho, ux, uy, uz, tem, qx, qy, qz}, [&](const lbm::CollideLaunch &cl) {
         lbm::ADArgs a{rho, ux, uy, uz, tem, qx, qy, qz, issave ? g : nullptr, alpha, diff, heat, nu, issave};
         return hip_result(lbm::launch_ad_collide(cl, a), "ad collide");
     });
@@ -864,7 +898,7 @@ int lbm_ans_macro_brinkman_collide_p(lbm_program *P, lbm_lattice *p, const doubl
         return fail(LBM_ERR_ARG, "bad arguments");
     if (issave && (!fields_ok({ip, iux, iuy, imx, imy}) || (p->nd == 3 && (!iuz || !imz))))
         return fail(LBM_ERR_ARG, "null field");
-    return collide_call(P, p, nullptr, [&](const lbm::CollideLaunch &cl) {
+    return collide_call(P, p, nullptr, {ip, iux, iuy, iuz, imx, imy, imz}, [&](const lbm::CollideLaunch &cl) {
         lbm::ANSArgs a{rho, ux, uy, uz, alpha, ip, iux, iuy, iuz, imx, imy, imz, nu, issave};
         return hip_result(lbm::launch_ans_collide(cl, a), "ans collide");
     });
@@ -913,7 +947,7 @@ int lbm_aad_macro_brinkman_collide_force_convection_p(lbm_program *P, lbm_lattic
         return fail(LBM_ERR_ARG, "bad arguments");
     if (issave && (!fields_ok({ip, iux, iuy, imx, imy, item, iqx, iqy}) || (p->nd == 3 && (!iuz || !imz || !iqz))))
         return fail(LBM_ERR_ARG, "null field");
-    return collide_call(P, p, q, [&](const lbm::CollideLaunch &cl) {
+    return collide_call(P, p, q, {ip, iux, iuy, iuz, imx, imy, imz, item, iqx, iqy, iqz}, [&](const lbm::CollideLaunch &cl) {
         lbm::AADArgs a{rho, ux, uy, uz, alpha, tem, diff, ip, iux, iuy, iuz, imx, imy, imz, item, iqx, iqy, iqz,
                        issave ? ig : nullptr, nu, issave};
         return hip_result(lbm::launch_aad_collide(cl, a), "aad collide");

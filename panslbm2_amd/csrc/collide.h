@@ -191,18 +191,54 @@ template <class LA, class LB, class Body>
 __global__ void __launch_bounds__(kBlock) k_collide_fused(const double *__restrict__ sa, const double *__restrict__ sb,
                                                           double *__restrict__ da, double *__restrict__ db, Grid g,
                                                           ProgView pv, int any_smooth, Body body) {
-    int tile = xcd_tile(blockIdx.x, gridDim.x);
-    int n = tile * blockDim.x + threadIdx.x;
-    if (n >= g.N) return;
-    int i, j, k;
-    coords(n, g, i, j, k);
-    int nb = nboundary(i, j, k, g, LA::nd);
-    if (any_smooth && nb >= 2) return;
-    int side = -1;
-    if (pv.side_of_slot != nullptr && nb == 1) side = pv.side_of_slot[boundary_slot(i, j, k, g.nx, g.ny, g.nz, LA::nd)];
-    double fa[LA::nc], fb[LB::nc > 0 ? LB::nc : 1];
-    node_state<LA, LB>(sa, sb, g, pv, n, fa, fb);
-    if (side >= 0) {  // a smoothed corner's neighbour: keep its post-boundary state for the corner
+    constexpr int NB = LB::nc > 0 ? LB::nc : 1;
+    // the corner workgroup (last in the grid when pv.corner_block): lanes [0, nside) build the
+    // corners' inward neighbours' states, lanes [nside, nside + ncorner) the corners
+    const bool cblk = pv.corner_block && blockIdx.x == gridDim.x - 1;
+    int n, nb, side = -1;
+    if (cblk) {
+        int t = threadIdx.x;
+        n = t < pv.nside ? pv.side_nodes[t] : (t < pv.nside + pv.ncorner ? pv.corners[t - pv.nside].x : -1);
+        nb = 2;
+    } else {
+        int tile = xcd_tile(blockIdx.x, gridDim.x - pv.corner_block);
+        n = tile * blockDim.x + threadIdx.x;
+        if (n >= g.N) return;
+        int i, j, k;
+        coords(n, g, i, j, k);
+        nb = nboundary(i, j, k, g, LA::nd);
+        if (any_smooth && nb >= 2) return;
+        if (pv.side_of_slot != nullptr && nb == 1)
+            side = pv.side_of_slot[boundary_slot(i, j, k, g.nx, g.ny, g.nz, LA::nd)];
+    }
+    double fa[LA::nc], fb[NB];
+    if (n >= 0) node_state<LA, LB>(sa, sb, g, pv, n, fa, fb);
+    if (cblk) {
+        __shared__ double sst[kMaxSide * kSideStride];
+        const int t = threadIdx.x;
+        if (t < pv.nside) {
+            sfor<0, LA::nc>([&](auto c) { sst[t * kSideStride + decltype(c)::value] = fa[decltype(c)::value]; });
+            if constexpr (LB::nc > 0)
+                sfor<0, LB::nc>([&](auto c) { sst[t * kSideStride + LA::nc + decltype(c)::value] = fb[decltype(c)::value]; });
+        }
+        __syncthreads();
+        if (t < pv.nside || t >= pv.nside + pv.ncorner) return;
+        // d2q9.h:577-587: a smoothed lattice's corner state is the mean of its two neighbours' states
+        const int4 e = pv.corners[t - pv.nside];
+        const bool smA = (e.w >> (0 ^ pv.latswap)) & 1, smB = (e.w >> (1 ^ pv.latswap)) & 1;
+        if (smA)
+            sfor<0, LA::nc>([&](auto c) {
+                constexpr int C = decltype(c)::value;
+                fa[C] = 0.5 * (sst[e.y * kSideStride + C] + sst[e.z * kSideStride + C]);
+            });
+        if constexpr (LB::nc > 0) {
+            if (smB)
+                sfor<0, LB::nc>([&](auto c) {
+                    constexpr int C = decltype(c)::value;
+                    fb[C] = 0.5 * (sst[e.y * kSideStride + LA::nc + C] + sst[e.z * kSideStride + LA::nc + C]);
+                });
+        }
+    } else if (side >= 0) {  // a smoothed corner's neighbour: keep its post-boundary state for the corner
         double *sp = pv.side + (size_t)side * kSideStride;
         sfor<0, LA::nc>([&](auto c) { sp[decltype(c)::value] = fa[decltype(c)::value]; });
         if constexpr (LB::nc > 0) sfor<0, LB::nc>([&](auto c) { sp[LA::nc + decltype(c)::value] = fb[decltype(c)::value]; });
@@ -391,8 +427,8 @@ static hipError_t launch_collide(const CollideLaunch &cl, Make &&make) {
             LBM_LAUNCH((k_collide_fixup<LA, LB, Body<LA, LB>, 1>), nblocks(cl.n_fix_edge), cl.stream, cl.sa, cl.sb,
                        cl.da, cl.db, g, pv, cl.fix_edge, cl.n_fix_edge, body);
         }
-        LBM_LAUNCH((k_collide_fused<LA, LB, Body<LA, LB>>), nblocks(g.N), cl.stream, cl.sa, cl.sb, cl.da, cl.db, g, pv,
-                   cl.any_smooth, body);
+        LBM_LAUNCH((k_collide_fused<LA, LB, Body<LA, LB>>), nblocks(g.N) + pv.corner_block, cl.stream, cl.sa, cl.sb,
+                   cl.da, cl.db, g, pv, cl.any_smooth, body);
         if (cl.n_corner2d > 0) {
             LBM_LAUNCH((k_collide_corner2d<LA, LB, Body<LA, LB>>), nblocks(cl.n_corner2d), cl.stream, cl.sa, cl.sb,
                        cl.da, cl.db, g, pv, cl.corner2d, cl.n_corner2d, body);

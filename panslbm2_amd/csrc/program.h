@@ -68,7 +68,15 @@ struct ProgView {
     // corner kernel launched after it averages them.  nullptr when unused.
     const int *side_of_slot;  // boundary slot -> side-state index or -1
     double *side;
+    // 2D corner smoothing inside the fused pass, used when the boundary operators read no array the
+    // collide saves: the grid's last workgroup rebuilds the corners' inward neighbours' states, swaps
+    // them through LDS and finishes the corners, concurrently with the rest of the pass.
+    int corner_block;         // 1: the last workgroup does the corners
+    int nside, ncorner;       // neighbour states (<= kMaxSide) and corners
+    const int *side_nodes;    // node of each neighbour state
+    const int4 *corners;      // (node, state of x-neighbour, state of y-neighbour, smoothed-lattice bits)
 };
+constexpr int kMaxSide = 8;
 constexpr int kSideStride = 32;  // >= nc(first) + nc(second) of any lattice pair
 
 // Boundary slot of node (i, j, k) on an nx*ny*nz grid, -1 for interior nodes.  Surface nodes are

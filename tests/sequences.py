@@ -364,12 +364,15 @@ class ForcedConvection2D:
     (advection.h:814, adjointadvection.h:997) and the heat-flux objective of
     production/heatsink.cpp:156-166,200-232."""
 
-    def __init__(self, name, lx, ly, nt, nu=0.1, pr=6.0, umax=0.05, qn=1e-2, L=None, seed=5, check=None):
+    def __init__(self, name, lx, ly, nt, nu=0.1, pr=6.0, umax=0.05, qn=1e-2, L=None, seed=5, check=None, swap=True):
         self.name, self.lx, self.ly, self.nt = name, lx, ly, nt
         self.nu, self.pr, self.umax, self.qn = nu, pr, umax, qn
         self.L = L if L is not None else lx // 2
         self.seed = seed
         self.check = check
+        # swap = False: one array per field, as test/nsadsens.cpp:70-101 (the boundary calls then
+        # read the very arrays the next collide saves into)
+        self.swap_fields = swap
 
     def setup(self):
         lx, ly = self.lx, self.ly
@@ -432,15 +435,13 @@ class ForcedConvection2D:
             b.ad_set_t(pg, d["inlet"], zero, "ux", "uy")
             b.ad_set_q(pg, d["flux_bc"], d["qn"], "ux", "uy", None, "diff")
             b.smooth(pg)
-            b.swap("ux", "uxp")
-            b.swap("uy", "uyp")
-            b.swap("qx", "qxp")
-            b.swap("qy", "qyp")
+            if self.swap_fields:
+                for v in ("ux", "uy", "qx", "qy"):
+                    b.swap(v, v + "p")
         # after the final swap the step-nt fields sit in the "p" buffers (heatsink.cpp:170-173)
-        b.swap("ux", "uxp")
-        b.swap("uy", "uyp")
-        b.swap("qx", "qxp")
-        b.swap("qy", "qyp")
+        if self.swap_fields:
+            for v in ("ux", "uy", "qx", "qy"):
+                b.swap(v, v + "p")
         out = {"f": b.pops(pf), "g": b.pops(pg)}
         for k in ("rho", "ux", "uy", "tem", "qx", "qy", "g"):
             out["fwd_" + k] = b.get(k)
@@ -462,14 +463,12 @@ class ForcedConvection2D:
             b.ans_iset_u(pf, d["inlet"], d["uin"], zero)
             b.aad_iset_rho(pf, pg, d["outlet_type"], "rho", "ux", "uy", None, "tem")
             b.smooth(pf)
-            b.swap("iux", "iuxp")
-            b.swap("iuy", "iuyp")
-            b.swap("iqx", "iqxp")
-            b.swap("iqy", "iqyp")
-        b.swap("iux", "iuxp")
-        b.swap("iuy", "iuyp")
-        b.swap("iqx", "iqxp")
-        b.swap("iqy", "iqyp")
+            if self.swap_fields:
+                for v in ("iux", "iuy", "iqx", "iqy"):
+                    b.swap(v, v + "p")
+        if self.swap_fields:
+            for v in ("iux", "iuy", "iqx", "iqy"):
+                b.swap(v, v + "p")
         b.sens_temp_heat_source(pg, "dfds", "ux", "uy", None, "imx", "imy", None, "dads", "tem", "item", "iqx", "iqy",
                                 None, "g", "ig", "diff", "dkds", d["heat_edge"], d["qn"])
         out.update({"adj_f": b.pops(pf), "adj_g": b.pops(pg), "dfds": b.get("dfds")})
@@ -608,4 +607,5 @@ SEQUENCES = [
     Cavity("cavity_21x21", 21, 21, 300),
     ForcedConvection2D("fc2d_24x16", 24, 16, 200, check=100),
     ForcedConvection3D("fc3d_7x5x5", 7, 5, 5, 60, check=30),
+    ForcedConvection2D("fc2d_noswap_20x12", 20, 12, 120, swap=False),
 ]
