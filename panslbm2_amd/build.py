@@ -15,7 +15,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpanslbm2_hip.so")
 BUILD = os.path.join(HERE, "_build")
-SOURCES = ["collide_ad.hip", "collide_aad.hip", "collide_ns.hip", "collide_ans.hip", "device.hip", "capi.hip"]
+SOURCES = ["collide_ad3.hip", "collide_aad3.hip", "collide_ns3.hip", "collide_ans3.hip", "collide_ad2.hip",
+           "collide_aad2.hip", "collide_ns2.hip", "collide_ans2.hip", "device.hip", "capi.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -ffp-contract=off: no FMA contraction, so FP64 kernels reproduce the reference's plain
 # x86-64 arithmetic.  Packed FP32 instructions are disabled for all device code.

@@ -248,6 +248,10 @@ __global__ void __launch_bounds__(kBlock) k_collide_fused(const double *__restri
     store_node<LB>(db, g.N, n, fb);
 }
 
+template <class LA, class LB, int LEVEL>
+__device__ void smoothed_state_call(const double *sa, const double *sb, const Grid &g, const ProgView &pv, int n,
+                                    double *a, double *b);
+
 // State of node n with SMOOTH operators applied; LEVEL bounds the neighbour recursion (2D corners
 // and 3D edges: 1, 3D corners: 2).  The inward neighbours' states do not depend on this node's, so
 // they are built first, once, and their loads overlap this node's; a SMOOTH operator then averages
@@ -272,7 +276,7 @@ __device__ __forceinline__ void smoothed_state(const double *sa, const double *s
         // (an axis off the faces evaluates the node itself, unused, to keep the arrays in registers)
         sfor<0, LA::nd>([&](auto x) {
             constexpr int AX = decltype(x)::value;
-            smoothed_state<LA, LB, LEVEL - 1>(sa, sb, g, pv, n + dax[AX] * stride[AX], tx[AX], ty[AX]);
+            smoothed_state_call<LA, LB, LEVEL - 1>(sa, sb, g, pv, n + dax[AX] * stride[AX], tx[AX], ty[AX]);
         });
         // the two or three states in axis order: with two, the axis off the faces is skipped
         const int cnt = (dax[0] != 0) + (dax[1] != 0) + (dax[2] != 0);
@@ -310,6 +314,15 @@ __device__ __forceinline__ void smoothed_state(const double *sa, const double *s
             });
         }
     }
+}
+
+// The neighbour states of a smoothed node, out of line: a 3D corner needs 3 smoothed edges of 3
+// node states each, and inlining all of them multiplies the kernel's code without making the few
+// fix-up nodes faster.
+template <class LA, class LB, int LEVEL>
+__device__ __noinline__ void smoothed_state_call(const double *sa, const double *sb, const Grid &g, const ProgView &pv,
+                                                 int n, double *a, double *b) {
+    smoothed_state<LA, LB, LEVEL>(sa, sb, g, pv, n, a, b);
 }
 
 template <class LA, class LB, class Body, int LEVEL>
@@ -403,7 +416,32 @@ static hipError_t dispatch_pair(int ka, int kb, F &&f) {
 }
 
 // Collide launch: body built per (LA, LB) by `make`.
-template <template <class, class> class Body, bool COUPLED, class Make>
+// Dispatch over the lattices (single) or lattice pairs (coupled) of one dimension, so that each
+// translation unit instantiates the kernels of one dimension only.
+template <int ND, class F>
+static hipError_t dispatch_single_nd(int ka, F &&f) {
+    if constexpr (ND == 2) {
+        if (ka == kD2Q9) return f(D2Q9{}, NoLat{});
+        if (ka == kD2Q5) return f(D2Q5{}, NoLat{});
+    } else {
+        if (ka == kD3Q15) return f(D3Q15{}, NoLat{});
+        if (ka == kD3Q7) return f(D3Q7{}, NoLat{});
+    }
+    return hipErrorInvalidValue;
+}
+template <int ND, class F>
+static hipError_t dispatch_coupled_nd(int ka, int kb, F &&f) {
+    if constexpr (ND == 2) {
+        if (ka == kD2Q9 && kb == kD2Q9) return f(D2Q9{}, D2Q9{});
+        if (ka == kD2Q9 && kb == kD2Q5) return f(D2Q9{}, D2Q5{});
+    } else {
+        if (ka == kD3Q15 && kb == kD3Q15) return f(D3Q15{}, D3Q15{});
+        if (ka == kD3Q15 && kb == kD3Q7) return f(D3Q15{}, D3Q7{});
+    }
+    return hipErrorInvalidValue;
+}
+
+template <template <class, class> class Body, bool COUPLED, int ND, class Make>
 static hipError_t launch_collide(const CollideLaunch &cl, Make &&make) {
     auto go = [&](auto la, auto lb) -> hipError_t {
         using LA = decltype(la);
@@ -435,8 +473,55 @@ static hipError_t launch_collide(const CollideLaunch &cl, Make &&make) {
         }
         return hipGetLastError();
     };
-    if constexpr (COUPLED) return dispatch_coupled(cl.ka, cl.kb, go);
-    else return cl.kb < 0 ? dispatch_single(cl.ka, go) : hipErrorInvalidValue;
+    if constexpr (COUPLED) return dispatch_coupled_nd<ND>(cl.ka, cl.kb, go);
+    else return cl.kb < 0 ? dispatch_single_nd<ND>(cl.ka, go) : hipErrorInvalidValue;
 }
 
+
+// The collides of one equation and one dimension; each is instantiated in its own translation
+// unit (collide_<eq><nd>.hip) and dispatched by launch_<eq>_collide (device.hip).
+template <int ND>
+hipError_t ns_collide_nd(const CollideLaunch &cl, const NSArgs &a) {
+    double omega = 1.0 / (3.0 * a.nu + 0.5);
+    return launch_collide<NSBody, false, ND>(cl, [&](auto la, auto lb) {
+        NSBody<decltype(la), decltype(lb)> b;
+        b.rho = a.rho; b.ux = a.ux; b.uy = a.uy; b.uz = a.uz; b.alpha = a.alpha;
+        b.omega = omega; b.iomega = 1.0 - omega; b.save = a.save; b.brinkman = a.alpha != nullptr;
+        return b;
+    });
+}
+template <int ND>
+hipError_t ad_collide_nd(const CollideLaunch &cl, const ADArgs &a) {
+    double omega = 1.0 / (3.0 * a.nu + 0.5);
+    return launch_collide<ADBody, true, ND>(cl, [&](auto la, auto lb) {
+        ADBody<decltype(la), decltype(lb)> b;
+        b.rho = a.rho; b.ux = a.ux; b.uy = a.uy; b.uz = a.uz; b.tem = a.tem; b.qx = a.qx; b.qy = a.qy; b.qz = a.qz;
+        b.gsnap = a.gsnap; b.alpha = a.alpha; b.diff = a.diff; b.heat = a.heat;
+        b.omegaf = omega; b.iomegaf = 1.0 - omega; b.save = a.save; b.N = cl.g.N;
+        return b;
+    });
+}
+template <int ND>
+hipError_t ans_collide_nd(const CollideLaunch &cl, const ANSArgs &a) {
+    double omega = 1.0 / (3.0 * a.nu + 0.5);
+    return launch_collide<ANSBody, false, ND>(cl, [&](auto la, auto lb) {
+        ANSBody<decltype(la), decltype(lb)> b;
+        b.rho = a.rho; b.ux = a.ux; b.uy = a.uy; b.uz = a.uz; b.alpha = a.alpha;
+        b.ip = a.ip; b.iux = a.iux; b.iuy = a.iuy; b.iuz = a.iuz; b.imx = a.imx; b.imy = a.imy; b.imz = a.imz;
+        b.omega = omega; b.iomega = 1.0 - omega; b.save = a.save;
+        return b;
+    });
+}
+template <int ND>
+hipError_t aad_collide_nd(const CollideLaunch &cl, const AADArgs &a) {
+    double omega = 1.0 / (3.0 * a.nu + 0.5);
+    return launch_collide<AADBody, true, ND>(cl, [&](auto la, auto lb) {
+        AADBody<decltype(la), decltype(lb)> b;
+        b.rho = a.rho; b.ux = a.ux; b.uy = a.uy; b.uz = a.uz; b.alpha = a.alpha; b.tem = a.tem; b.diff = a.diff;
+        b.ip = a.ip; b.iux = a.iux; b.iuy = a.iuy; b.iuz = a.iuz; b.imx = a.imx; b.imy = a.imy; b.imz = a.imz;
+        b.item = a.item; b.iqx = a.iqx; b.iqy = a.iqy; b.iqz = a.iqz; b.igsnap = a.igsnap;
+        b.omegaf = omega; b.iomegaf = 1.0 - omega; b.save = a.save; b.N = cl.g.N;
+        return b;
+    });
+}
 }  // namespace lbm

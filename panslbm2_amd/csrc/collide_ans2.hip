@@ -1,0 +1,7 @@
+// panslbm2_amd/csrc/collide_ans2.hip -- the ANS collides on the 2D lattices (one translation unit
+// per equation and dimension so the kernel instantiations compile in parallel).
+#include "collide.h"
+
+namespace lbm {
+template hipError_t ans_collide_nd<2>(const CollideLaunch &cl, const ANSArgs &a);
+}  // namespace lbm

@@ -52,6 +52,18 @@ hipError_t launch_stream(int kind, const double *src, double *dst, Grid g, int p
 hipError_t launch_smooth(int kind, double *f, int N, const int4 *list, int count, hipStream_t s);
 hipError_t launch_program_apply(int ka, int kb, double *a, double *b, int N, const ProgView &pv, int seg,
                                 hipStream_t s);
+template <int ND> hipError_t ns_collide_nd(const CollideLaunch &cl, const NSArgs &a);
+template <int ND> hipError_t ad_collide_nd(const CollideLaunch &cl, const ADArgs &a);
+template <int ND> hipError_t ans_collide_nd(const CollideLaunch &cl, const ANSArgs &a);
+template <int ND> hipError_t aad_collide_nd(const CollideLaunch &cl, const AADArgs &a);
+extern template hipError_t ns_collide_nd<2>(const CollideLaunch &, const NSArgs &);
+extern template hipError_t ns_collide_nd<3>(const CollideLaunch &, const NSArgs &);
+extern template hipError_t ad_collide_nd<2>(const CollideLaunch &, const ADArgs &);
+extern template hipError_t ad_collide_nd<3>(const CollideLaunch &, const ADArgs &);
+extern template hipError_t ans_collide_nd<2>(const CollideLaunch &, const ANSArgs &);
+extern template hipError_t ans_collide_nd<3>(const CollideLaunch &, const ANSArgs &);
+extern template hipError_t aad_collide_nd<2>(const CollideLaunch &, const AADArgs &);
+extern template hipError_t aad_collide_nd<3>(const CollideLaunch &, const AADArgs &);
 hipError_t launch_ns_collide(const CollideLaunch &cl, const NSArgs &a);
 hipError_t launch_ad_collide(const CollideLaunch &cl, const ADArgs &a);
 hipError_t launch_ans_collide(const CollideLaunch &cl, const ANSArgs &a);

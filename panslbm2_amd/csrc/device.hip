@@ -320,6 +320,21 @@ hipError_t launch_program_apply(int ka, int kb, double *a, double *b, int N, con
     });
 }
 
+// collides: by the dimension of the first lattice (kinds D2Q9/D2Q5 are 2D, D3Q15/D3Q7 3D)
+static bool is_3d_kind(int k) { return k == kD3Q15 || k == kD3Q7; }
+hipError_t launch_ns_collide(const CollideLaunch &cl, const NSArgs &a) {
+    return is_3d_kind(cl.ka) ? ns_collide_nd<3>(cl, a) : ns_collide_nd<2>(cl, a);
+}
+hipError_t launch_ad_collide(const CollideLaunch &cl, const ADArgs &a) {
+    return is_3d_kind(cl.ka) ? ad_collide_nd<3>(cl, a) : ad_collide_nd<2>(cl, a);
+}
+hipError_t launch_ans_collide(const CollideLaunch &cl, const ANSArgs &a) {
+    return is_3d_kind(cl.ka) ? ans_collide_nd<3>(cl, a) : ans_collide_nd<2>(cl, a);
+}
+hipError_t launch_aad_collide(const CollideLaunch &cl, const AADArgs &a) {
+    return is_3d_kind(cl.ka) ? aad_collide_nd<3>(cl, a) : aad_collide_nd<2>(cl, a);
+}
+
 template <class L>
 static hipError_t launch_init_t(double *f, int N, int which, const double *const *p, hipStream_t s) {
     LBM_LAUNCH((k_init<L>), nblocks(N), s, f, N, which, p[0], p[1], p[2], p[3], p[4], p[5], p[6]);
