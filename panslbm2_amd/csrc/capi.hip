@@ -118,9 +118,9 @@ struct lbm_program {
     bool ordered_ok = true;  // no operator on a lattice after its SmoothCorner
     // device tables
     bool uploaded = false;
-    DevBuf d_slot_entry, d_entry_node, d_entry_ops, d_ops, d_sops, d_fix_corner, d_fix_edge;
+    DevBuf d_slot_entry, d_entry_node, d_entry_ops, d_ops, d_sops;
     DevBuf d_side_of_slot, d_corner2d, d_side, d_side_nodes;  // 2D corner smoothing
-    int nentry = 0, nslot = 0, maxops = 0, n_fix_corner = 0, n_fix_edge = 0, n_corner2d = 0, n_side = 0;
+    int nentry = 0, nslot = 0, maxops = 0, n_corner2d = 0, n_side = 0;
     bool corner_block_ok = false;  // the corners may be done by the fused pass's last workgroup
     bool owned_by_ctx = false;  // eager scratch program
     ~lbm_program() {
@@ -129,8 +129,6 @@ struct lbm_program {
         d_entry_ops.release();
         d_ops.release();
         d_sops.release();
-        d_fix_corner.release();
-        d_fix_edge.release();
         d_side_of_slot.release();
         d_corner2d.release();
         d_side.release();
@@ -339,9 +337,9 @@ int prog_upload(lbm_program *P) {
         if (e < 0) continue;
         for (int t = 0; t < entry_ops[e].y; ++t) sops[(size_t)s * P->maxops + t] = ops[entry_ops[e].x + t];
     }
-    std::vector<int> fix_corner, fix_edge;
-    // 2D: corners are done after the fused pass from their neighbours' saved states; 3D: edges and
-    // corners are rebuilt by fix-up kernels before it
+    // 2D corners are done from their neighbours' post-boundary states, by the fused pass's last
+    // workgroup or, when the boundary operators read arrays the collide saves, by a corner kernel
+    // after the pass from states the pass stores in a side buffer (3D smoothed programs are not fused)
     std::vector<int> side_of_slot, side_nodes;
     std::vector<int4> corner2d;
     P->corner_block_ok = false;
@@ -376,13 +374,8 @@ int prog_upload(lbm_program *P) {
         }
         if (P->d_side.ensure(sizeof(double) * lbm::kSideStride * std::max<size_t>(side.size(), 1)))
             return fail(LBM_ERR_HIP, "device allocation of the side states failed");
-    } else if (P->smoothed[0] || P->smoothed[1]) {
-        for (const int4 &e : L->sm_corner) fix_corner.push_back(e.x);
-        for (const int4 &e : L->sm_edge) fix_edge.push_back(e.x);
     }
     P->n_corner2d = (int)corner2d.size();
-    P->n_fix_corner = (int)fix_corner.size();
-    P->n_fix_edge = (int)fix_edge.size();
     auto up = [&](DevBuf &b, const void *src, size_t bytes) -> int {
         if (b.ensure(bytes ? bytes : 8)) return fail(LBM_ERR_HIP, "device allocation of %zu bytes failed", bytes);
         if (bytes) HIPCHK(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
@@ -394,8 +387,6 @@ int prog_upload(lbm_program *P) {
     if ((rc = up(P->d_entry_ops, entry_ops.data(), entry_ops.size() * sizeof(int2)))) return rc;
     if ((rc = up(P->d_ops, ops.data(), ops.size() * sizeof(Op)))) return rc;
     if ((rc = up(P->d_sops, sops.data(), sops.size() * sizeof(Op)))) return rc;
-    if ((rc = up(P->d_fix_corner, fix_corner.data(), fix_corner.size() * sizeof(int)))) return rc;
-    if ((rc = up(P->d_fix_edge, fix_edge.data(), fix_edge.size() * sizeof(int)))) return rc;
     if ((rc = up(P->d_side_of_slot, side_of_slot.data(), side_of_slot.size() * sizeof(int)))) return rc;
     if ((rc = up(P->d_corner2d, corner2d.data(), corner2d.size() * sizeof(int4)))) return rc;
     if ((rc = up(P->d_side_nodes, side_nodes.data(), side_nodes.size() * sizeof(int)))) return rc;
@@ -496,7 +487,10 @@ int collide_call(lbm_program *P, lbm_lattice *p, lbm_lattice *q, std::initialize
     if (P) {
         bool subset = true;
         for (int k = 0; k < P->nlat; ++k) subset &= (P->lat[k] == p || P->lat[k] == q);
-        bool dims = !(P->smoothed[0] || P->smoothed[1]) || (p->lx >= 3 && p->ly >= 3 && (p->nd == 2 || p->lz >= 3));
+        // SmoothCorner fuses in 2D only (corners from the pass's own neighbour states); a 3D
+        // program with SmoothCorner runs unfused: stream, in-place boundary passes, smoothing, collide.
+        bool smoothed = P->smoothed[0] || P->smoothed[1];
+        bool dims = !smoothed || (p->nd == 2 && p->lx >= 3 && p->ly >= 3);
         fused = subset && P->ordered_ok && dims && P->nlat > 0;
         if (!fused) {
             int rc = prog_apply(P);
@@ -524,10 +518,6 @@ int collide_call(lbm_program *P, lbm_lattice *p, lbm_lattice *q, std::initialize
     cl.sb = q ? q->buf[q->cur] : nullptr;
     cl.db = q ? q->buf[q->cur ^ 1] : nullptr;
     cl.pv = &v;
-    cl.fix_corner = (const int *)P->d_fix_corner.p;
-    cl.n_fix_corner = P->n_fix_corner;
-    cl.fix_edge = (const int *)P->d_fix_edge.p;
-    cl.n_fix_edge = P->n_fix_edge;
     cl.corner2d = (const int4 *)P->d_corner2d.p;
     cl.n_corner2d = P->n_corner2d;
     if (P->corner_block_ok && P->n_corner2d > 0) {

@@ -248,98 +248,6 @@ __global__ void __launch_bounds__(kBlock) k_collide_fused(const double *__restri
     store_node<LB>(db, g.N, n, fb);
 }
 
-template <class LA, class LB, int LEVEL>
-__device__ void smoothed_state_call(const double *sa, const double *sb, const Grid &g, const ProgView &pv, int n,
-                                    double *a, double *b);
-
-// State of node n with SMOOTH operators applied; LEVEL bounds the neighbour recursion (2D corners
-// and 3D edges: 1, 3D corners: 2).  The inward neighbours' states do not depend on this node's, so
-// they are built first, once, and their loads overlap this node's; a SMOOTH operator then averages
-// the lattice it names.
-template <class LA, class LB, int LEVEL>
-__device__ __forceinline__ void smoothed_state(const double *sa, const double *sb, const Grid &g, const ProgView &pv, int n, double *a,
-                               double *b) {
-    if constexpr (LEVEL == 0) {
-        node_state<LA, LB>(sa, sb, g, pv, n, a, b);
-    } else {
-        constexpr int NB = LB::nc > 0 ? LB::nc : 1;
-        int i, j, k;
-        coords(n, g, i, j, k);
-        // inward neighbour along each axis on which the node lies on a face (step +1 off a min
-        // face, -1 off a max face); the average runs over those, in axis order
-        const int stride[3] = {1, g.nx, g.nx * g.ny};
-        int dax[3] = {0, 0, 0};
-        dax[0] = i == 0 ? 1 : (i == g.nx - 1 ? -1 : 0);
-        dax[1] = j == 0 ? 1 : (j == g.ny - 1 ? -1 : 0);
-        if (LA::nd == 3) dax[2] = k == 0 ? 1 : (k == g.nz - 1 ? -1 : 0);
-        double tx[LA::nd][LA::nc], ty[LA::nd][NB];
-        // (an axis off the faces evaluates the node itself, unused, to keep the arrays in registers)
-        sfor<0, LA::nd>([&](auto x) {
-            constexpr int AX = decltype(x)::value;
-            smoothed_state_call<LA, LB, LEVEL - 1>(sa, sb, g, pv, n + dax[AX] * stride[AX], tx[AX], ty[AX]);
-        });
-        // the two or three states in axis order: with two, the axis off the faces is skipped
-        const int cnt = (dax[0] != 0) + (dax[1] != 0) + (dax[2] != 0);
-        // first boundary axis: 0 unless x is off the faces; second: 1 unless x or y is off them
-        const bool skip0 = dax[0] == 0, skip01 = dax[0] == 0 || dax[1] == 0;
-        load_pull<LA>(sa, g, i, j, k, n, pv.pull[0], a);
-        load_pull<LB>(sb, g, i, j, k, n, pv.pull[1], b);
-        int slot = boundary_slot(i, j, k, g.nx, g.ny, g.nz, LA::nd);
-        if (slot < 0 || slot >= pv.nslot) return;
-        const Op *p = pv.sops + (size_t)slot * pv.maxops;
-        NodeFieldCache fc = prefetch_fields(pv, n);
-        for (int base = 0; base < pv.maxops; base += kOpBatch) {
-            Op o[kOpBatch];
-            sfor<0, kOpBatch>([&](auto q) { o[decltype(q)::value] = p[base + decltype(q)::value]; });
-            sfor<0, kOpBatch>([&](auto q) {
-                const Op &op = o[decltype(q)::value];
-                if (op.code == OP_NONE) return;
-                if (op.code != OP_SMOOTH) {
-                    apply_op<LA, LB>(op, a, b, pv, n, fc);
-                } else if ((op.lat ^ pv.latswap) == 0) {
-                    sfor<0, LA::nc>([&](auto c) {
-                        constexpr int C = decltype(c)::value;
-                        double s0 = skip0 ? tx[1][C] : tx[0][C];
-                        double s1 = skip01 ? tx[LA::nd - 1][C] : tx[1][C];
-                        a[C] = cnt == 2 ? 0.5 * (s0 + s1) : (s0 + s1 + tx[LA::nd - 1][C]) / 3.0;
-                    });
-                } else if constexpr (LB::nc > 0) {
-                    sfor<0, LB::nc>([&](auto c) {
-                        constexpr int C = decltype(c)::value;
-                        double s0 = skip0 ? ty[1][C] : ty[0][C];
-                        double s1 = skip01 ? ty[LA::nd - 1][C] : ty[1][C];
-                        b[C] = cnt == 2 ? 0.5 * (s0 + s1) : (s0 + s1 + ty[LA::nd - 1][C]) / 3.0;
-                    });
-                }
-            });
-        }
-    }
-}
-
-// The neighbour states of a smoothed node, out of line: a 3D corner needs 3 smoothed edges of 3
-// node states each, and inlining all of them multiplies the kernel's code without making the few
-// fix-up nodes faster.
-template <class LA, class LB, int LEVEL>
-__device__ __noinline__ void smoothed_state_call(const double *sa, const double *sb, const Grid &g, const ProgView &pv,
-                                                 int n, double *a, double *b) {
-    smoothed_state<LA, LB, LEVEL>(sa, sb, g, pv, n, a, b);
-}
-
-template <class LA, class LB, class Body, int LEVEL>
-__global__ void __launch_bounds__(kBlock) k_collide_fixup(const double *__restrict__ sa, const double *__restrict__ sb,
-                                                          double *__restrict__ da, double *__restrict__ db, Grid g,
-                                                          ProgView pv, const int *__restrict__ nodes, int count,
-                                                          Body body) {
-    int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= count) return;
-    int n = nodes[e];
-    double fa[LA::nc], fb[LB::nc > 0 ? LB::nc : 1];
-    smoothed_state<LA, LB, LEVEL>(sa, sb, g, pv, n, fa, fb);
-    body(fa, fb, n, true);
-    store_node<LA>(da, g.N, n, fa);
-    store_node<LB>(db, g.N, n, fb);
-}
-
 // 2D smoothed corners after the fused pass: pull, the corner's own operators in record order, a
 // SMOOTH operator averaging the two inward neighbours' post-boundary states that the fused pass
 // saved (d2q9.h:577-587), then the collide.  The corner's fields are untouched by the fused pass,
@@ -453,18 +361,6 @@ static hipError_t launch_collide(const CollideLaunch &cl, Make &&make) {
             return hipGetLastError();
         }
         const ProgView &pv = *cl.pv;
-        if (cl.n_fix_corner > 0) {
-            if constexpr (LA::nd == 3)
-                LBM_LAUNCH((k_collide_fixup<LA, LB, Body<LA, LB>, 2>), nblocks(cl.n_fix_corner), cl.stream, cl.sa, cl.sb,
-                           cl.da, cl.db, g, pv, cl.fix_corner, cl.n_fix_corner, body);
-            else
-                LBM_LAUNCH((k_collide_fixup<LA, LB, Body<LA, LB>, 1>), nblocks(cl.n_fix_corner), cl.stream, cl.sa, cl.sb,
-                           cl.da, cl.db, g, pv, cl.fix_corner, cl.n_fix_corner, body);
-        }
-        if (cl.n_fix_edge > 0) {
-            LBM_LAUNCH((k_collide_fixup<LA, LB, Body<LA, LB>, 1>), nblocks(cl.n_fix_edge), cl.stream, cl.sa, cl.sb,
-                       cl.da, cl.db, g, pv, cl.fix_edge, cl.n_fix_edge, body);
-        }
         LBM_LAUNCH((k_collide_fused<LA, LB, Body<LA, LB>>), nblocks(g.N) + pv.corner_block, cl.stream, cl.sa, cl.sb,
                    cl.da, cl.db, g, pv, cl.any_smooth, body);
         if (cl.n_corner2d > 0) {

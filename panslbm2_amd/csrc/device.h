@@ -14,10 +14,6 @@ struct CollideLaunch {
     const double *sa, *sb;     // sources (fused) -- unused in place
     double *da, *db;           // destinations (fused) or the in-place buffers
     const ProgView *pv;        // nullptr: plain in-place collide
-    const int *fix_corner;     // smoothed nodes owned by the fix-up kernels
-    int n_fix_corner;
-    const int *fix_edge;
-    int n_fix_edge;
     const int4 *corner2d;      // 2D smoothed corners (node, side index of x-neighbour, of y-neighbour, -1),
     int n_corner2d;            // done after the fused pass from the side states it saved
     int any_smooth;
