@@ -128,6 +128,13 @@ int lbm_ad_macro_brinkman_collide_force_convection(lbm_lattice *p, double *rho, 
                                                    const double *alpha, double viscosity, lbm_lattice *q, double *tem,
                                                    double *qx, double *qy, double *qz, const double *diffusivity,
                                                    int issave, double *g, const double *heat);
+/* MacroBrinkmanCollideNaturalConvection :930-1044: buoyancy f_c += 3 w_c (c.g)(T - tem0) ahead of the
+ * Brinkman force; gz is ignored on 2D lattices. */
+int lbm_ad_macro_brinkman_collide_natural_convection(lbm_lattice *p, double *rho, double *ux, double *uy, double *uz,
+                                                     const double *alpha, double viscosity, lbm_lattice *q,
+                                                     double *tem, double *qx, double *qy, double *qz,
+                                                     const double *diffusivity, double gx, double gy, double gz,
+                                                     double tem0, int issave, double *g);
 int lbm_ad_set_t(lbm_lattice *q, const int *mask_face, const double *tem_face, const double *ux, const double *uy,
                  const double *uz);                                  /* BoundaryConditionSetT :1074-1090 */
 /* BoundaryConditionSetQ :1094-1130; diffusivity NULL selects the constant-D overload */
@@ -156,6 +163,13 @@ int lbm_aad_macro_brinkman_collide_force_convection(
     double *iuy, double *iuz, double *imx, double *imy, double *imz, const double *alpha, double viscosity,
     lbm_lattice *q, const double *tem, double *item, double *iqx, double *iqy, double *iqz, const double *diffusivity,
     int issave, double *ig);                                         /* :997-1109 */
+/* MacroBrinkmanCollideNaturalConvection :1113-1230: g_c += 3 (im.g) after the ANS re-macro, then the
+ * AAD macro again; gz is ignored on 2D lattices. */
+int lbm_aad_macro_brinkman_collide_natural_convection(
+    lbm_lattice *p, const double *rho, const double *ux, const double *uy, const double *uz, double *ip, double *iux,
+    double *iuy, double *iuz, double *imx, double *imy, double *imz, const double *alpha, double viscosity,
+    lbm_lattice *q, const double *tem, double *item, double *iqx, double *iqy, double *iqz, const double *diffusivity,
+    double gx, double gy, double gz, int issave, double *ig);
 int lbm_aad_iset_t(lbm_lattice *q, const int *mask_face, const double *ux, const double *uy,
                    const double *uz);                                /* iBoundaryConditionSetT :1385-1401 */
 int lbm_aad_iset_q(lbm_lattice *q, const int *mask_face, const double *ux, const double *uy, const double *uz,
@@ -218,6 +232,11 @@ int lbm_ad_macro_brinkman_collide_force_convection_p(lbm_program *prog, lbm_latt
                                                      lbm_lattice *q, double *tem, double *qx, double *qy, double *qz,
                                                      const double *diffusivity, int issave, double *g,
                                                      const double *heat);
+int lbm_ad_macro_brinkman_collide_natural_convection_p(lbm_program *prog, lbm_lattice *p, double *rho, double *ux,
+                                                       double *uy, double *uz, const double *alpha, double viscosity,
+                                                       lbm_lattice *q, double *tem, double *qx, double *qy,
+                                                       double *qz, const double *diffusivity, double gx, double gy,
+                                                       double gz, double tem0, int issave, double *g);
 int lbm_ans_macro_brinkman_collide_p(lbm_program *prog, lbm_lattice *p, const double *rho, const double *ux,
                                      const double *uy, const double *uz, double *ip, double *iux, double *iuy,
                                      double *iuz, double *imx, double *imy, double *imz, double viscosity,
@@ -227,6 +246,11 @@ int lbm_aad_macro_brinkman_collide_force_convection_p(
     double *ip, double *iux, double *iuy, double *iuz, double *imx, double *imy, double *imz, const double *alpha,
     double viscosity, lbm_lattice *q, const double *tem, double *item, double *iqx, double *iqy, double *iqz,
     const double *diffusivity, int issave, double *ig);
+int lbm_aad_macro_brinkman_collide_natural_convection_p(
+    lbm_program *prog, lbm_lattice *p, const double *rho, const double *ux, const double *uy, const double *uz,
+    double *ip, double *iux, double *iuy, double *iuz, double *imx, double *imy, double *imz, const double *alpha,
+    double viscosity, lbm_lattice *q, const double *tem, double *item, double *iqx, double *iqy, double *iqz,
+    const double *diffusivity, double gx, double gy, double gz, int issave, double *ig);
 
 /* ---- HIP graphs of the context stream (no reference counterpart; the reference loops on the
  * host).  Calls between begin and end are captured, not run; they must not return host data.

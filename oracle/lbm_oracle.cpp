@@ -1027,6 +1027,44 @@ void ol_ad_collide_fc(ol_lattice *p, double *rho, double *ux, double *uy, double
         store(q, idx, g);
     }
 }
+/* advection.h:930-1044 (MacroBrinkmanCollideNaturalConvection) with ExternalForceNaturalConvection
+ * (advection.h:73-87): after the first macro, f_c += 3 w_c (c.g) (T - tem0) for c >= 1, then the
+ * Brinkman force and the re-macro as in the forced-convection step.  gz is unused in 2D. */
+void ol_ad_collide_nc(ol_lattice *p, double *rho, double *ux, double *uy, double *uz, const double *alpha, double nu,
+                      ol_lattice *q, double *tem, double *qx, double *qy, double *qz, const double *diff, double gx,
+                      double gy, double gz, double tem0, int issave, double *gsnap) {
+    const LatDesc &dp = *p->d, &dq = *q->d;
+    const bool d3 = dp.nd == 3;
+    double omegaf = 1.0 / (3.0 * nu + 0.5), iomegaf = 1.0 - omegaf;
+#pragma omp parallel for
+    for (int idx = 0; idx < p->N; ++idx) {
+        double omegag = 1.0 / (3.0 * diff[idx] + 0.5), iomegag = 1.0 - omegag;
+        double f[27], g[27], feq[27], geq[27], r, vx, vy, vz, T, hx, hy, hz;
+        load(p, idx, f);
+        load(q, idx, g);
+        ns_macro(dp, f, r, vx, vy, vz);
+        ad_macro(dq, g, vx, vy, vz, omegag, T, hx, hy, hz);
+        for (int c = 1; c < dp.nc; ++c) {
+            double cg = d3 ? dp.cx[c] * gx + dp.cy[c] * gy + dp.cz[c] * gz : dp.cx[c] * gx + dp.cy[c] * gy;
+            f[c] += 3.0 * dp.w[c] * cg * (T - tem0);
+        }
+        ns_brinkman(dp, f, r, vx, vy, vz, alpha[idx]);
+        ns_macro(dp, f, r, vx, vy, vz);
+        ad_macro(dq, g, vx, vy, vz, omegag, T, hx, hy, hz);
+        if (issave) {
+            rho[idx] = r; ux[idx] = vx; uy[idx] = vy; put(uz, idx, vz);
+            tem[idx] = T; qx[idx] = hx; qy[idx] = hy; put(qz, idx, hz);
+            if (gsnap)
+                for (int c = 0; c < dq.nc; ++c) gsnap[(size_t)c * q->N + idx] = g[c];
+        }
+        ns_eq(dp, feq, r, vx, vy, vz);
+        bgk(dp, f, feq, omegaf, iomegaf);
+        ad_eq(dq, geq, T, vx, vy, vz);
+        bgk(dq, g, geq, omegag, iomegag);
+        store(p, idx, f);
+        store(q, idx, g);
+    }
+}
 /* advection.h:1074-1090 with :99-238 */
 void ol_ad_set_t(ol_lattice *q, const int *mask, const double *tem, const double *ux, const double *uy,
                  const double *uz) {
@@ -1147,6 +1185,46 @@ void ol_aad_collide_fc(ol_lattice *p, const double *rho, const double *ux, const
         if (issave) {
             /* the reference's 3D save writes no iuz (adjointadvection.h:1074-1084); kept */
             ip[idx] = a; iux[idx] = bx; iuy[idx] = by;
+            imx[idx] = mx; imy[idx] = my; put(imz, idx, mz);
+            item[idx] = it; iqx[idx] = hx; iqy[idx] = hy; put(iqz, idx, hz);
+            if (igsnap)
+                for (int c = 0; c < dq.nc; ++c) igsnap[(size_t)c * q->N + idx] = g[c];
+        }
+        ans_eq(dp, feq, vx, vy, vz, a, bx, by, bz);
+        bgk(dp, f, feq, omegaf, iomegaf);
+        aad_eq(dq, geq, it, hx, hy, hz, vx, vy, vz);
+        bgk(dq, g, geq, omegag, iomegag);
+        store(p, idx, f);
+        store(q, idx, g);
+    }
+}
+/* adjointadvection.h:1113-1230 (MacroBrinkmanCollideNaturalConvection) with ExternalForceNaturalConvection
+ * (adjointadvection.h:114-132): after the ANS re-macro every g_c (g_0 included) gains 3 (im.g), then
+ * the AAD macro is taken again.  Unlike the forced-convection step, the 3D save writes iuz. */
+void ol_aad_collide_nc(ol_lattice *p, const double *rho, const double *ux, const double *uy, const double *uz,
+                       double *ip, double *iux, double *iuy, double *iuz, double *imx, double *imy, double *imz,
+                       const double *alpha, double nu, ol_lattice *q, const double *tem, double *item, double *iqx,
+                       double *iqy, double *iqz, const double *diff, double gx, double gy, double gz, int issave,
+                       double *igsnap) {
+    const LatDesc &dp = *p->d, &dq = *q->d;
+    const bool d3 = dp.nd == 3;
+    double omegaf = 1.0 / (3.0 * nu + 0.5), iomegaf = 1.0 - omegaf;
+#pragma omp parallel for
+    for (int idx = 0; idx < p->N; ++idx) {
+        double omegag = 1.0 / (3.0 * diff[idx] + 0.5), iomegag = 1.0 - omegag;
+        double f[27], g[27], feq[27], geq[27], a, bx, by, bz, mx, my, mz, it, hx, hy, hz;
+        double vx = ux[idx], vy = uy[idx], vz = fld(uz, idx);
+        load(p, idx, f);
+        load(q, idx, g);
+        ans_macro(dp, f, vx, vy, vz, a, bx, by, bz, mx, my, mz);
+        aad_macro(dq, g, it, hx, hy, hz);
+        aad_brinkman(dp, f, rho[idx], vx, vy, vz, mx, my, mz, tem[idx], hx, hy, hz, omegag, alpha[idx]);
+        ans_macro(dp, f, vx, vy, vz, a, bx, by, bz, mx, my, mz);
+        double coef = d3 ? 3.0 * (mx * gx + my * gy + mz * gz) : 3.0 * (mx * gx + my * gy);
+        for (int c = 0; c < dq.nc; ++c) g[c] += coef;
+        aad_macro(dq, g, it, hx, hy, hz);
+        if (issave) {
+            ip[idx] = a; iux[idx] = bx; iuy[idx] = by; put(iuz, idx, bz);
             imx[idx] = mx; imy[idx] = my; put(imz, idx, mz);
             item[idx] = it; iqx[idx] = hx; iqy[idx] = hy; put(iqz, idx, hz);
             if (igsnap)

@@ -65,6 +65,9 @@ void ol_ad_init(ol_lattice *q, const double *tem, const double *ux, const double
 void ol_ad_collide_fc(ol_lattice *p, double *rho, double *ux, double *uy, double *uz, const double *alpha, double nu,
                       ol_lattice *q, double *tem, double *qx, double *qy, double *qz, const double *diff,
                       int issave, double *gsnap, const double *heat);
+void ol_ad_collide_nc(ol_lattice *p, double *rho, double *ux, double *uy, double *uz, const double *alpha, double nu,
+                      ol_lattice *q, double *tem, double *qx, double *qy, double *qz, const double *diff, double gx,
+                      double gy, double gz, double tem0, int issave, double *gsnap);
 void ol_ad_set_t(ol_lattice *q, const int *mask, const double *tem, const double *ux, const double *uy, const double *uz);
 /* diff_field == NULL selects the constant-diffusivity overload (advection.h:242 / :385) */
 void ol_ad_set_q(ol_lattice *q, const int *mask, const double *qn, const double *ux, const double *uy, const double *uz,
@@ -89,6 +92,11 @@ void ol_aad_collide_fc(ol_lattice *p, const double *rho, const double *ux, const
                        const double *alpha, double nu,
                        ol_lattice *q, const double *tem, double *item, double *iqx, double *iqy, double *iqz,
                        const double *diff, int issave, double *igsnap);
+void ol_aad_collide_nc(ol_lattice *p, const double *rho, const double *ux, const double *uy, const double *uz,
+                       double *ip, double *iux, double *iuy, double *iuz, double *imx, double *imy, double *imz,
+                       const double *alpha, double nu, ol_lattice *q, const double *tem, double *item, double *iqx,
+                       double *iqy, double *iqz, const double *diff, double gx, double gy, double gz, int issave,
+                       double *igsnap);
 void ol_aad_iset_t(ol_lattice *q, const int *mask, const double *ux, const double *uy, const double *uz);
 void ol_aad_iset_q(ol_lattice *q, const int *mask, const double *ux, const double *uy, const double *uz, double eps);
 /* type face array: 1 = SetT, 2 = SetQ (adjointadvection.h:15-18) */

@@ -169,6 +169,18 @@ void ref2_ad_collide_fc(int lx, int ly, int lz, double *f0, double *f, double *g
         save(p, f0, f); save(q, g0, g);
     }
 }
+void ref2_ad_collide_nc(int lx, int ly, int lz, double *f0, double *f, double *g0, double *g, double *rho,
+                       double *ux, double *uy, double *uz, const double *alpha, double nu, double *tem, double *qx,
+                       double *qy, double *qz, const double *diff, double gx, double gy, double gz, double tem0,
+                       int issave, double *gsnap) {
+    {
+        D2Q9<double> p(lx, ly), q(lx, ly);
+        load(p, f0, f); load(q, g0, g);
+        AD::MacroBrinkmanCollideNaturalConvection(p, rho, ux, uy, alpha, nu, q, tem, qx, qy, diff, gx, gy,
+                                                  tem0, issave != 0, gsnap);
+        save(p, f0, f); save(q, g0, g);
+    }
+}
 void ref2_ans_collide(int lx, int ly, int lz, double *f0, double *f, const double *rho, const double *ux,
                      const double *uy, const double *uz, double *ip, double *iux, double *iuy, double *iuz, double *imx,
                      double *imy, double *imz, double nu, const double *alpha, int issave) {
@@ -185,6 +197,20 @@ void ref2_aad_collide_fc(int lx, int ly, int lz, double *f0, double *f, double *
         load(p, f0, f); load(q, g0, g);
         AAD::MacroBrinkmanCollideForceConvection(p, rho, ux, uy, ip, iux, iuy, imx, imy, alpha, nu, q, tem, item, iqx,
                                                  iqy, diff, issave != 0, igsnap);
+        save(p, f0, f); save(q, g0, g);
+    }
+}
+
+void ref2_aad_collide_nc(int lx, int ly, int lz, double *f0, double *f, double *g0, double *g,
+                        const double *rho, const double *ux, const double *uy, const double *uz, double *ip, double *iux,
+                        double *iuy, double *iuz, double *imx, double *imy, double *imz, const double *alpha, double nu,
+                        const double *tem, double *item, double *iqx, double *iqy, double *iqz, const double *diff,
+                        double gx, double gy, double gz, int issave, double *igsnap) {
+    {
+        D2Q9<double> p(lx, ly), q(lx, ly);
+        load(p, f0, f); load(q, g0, g);
+        AAD::MacroBrinkmanCollideNaturalConvection(p, rho, ux, uy, ip, iux, iuy, imx, imy, alpha, nu, q,
+                                                   tem, item, iqx, iqy, diff, gx, gy, issave != 0, igsnap);
         save(p, f0, f); save(q, g0, g);
     }
 }

@@ -161,6 +161,18 @@ void ref3_ad_collide_fc(int lx, int ly, int lz, double *f0, double *f, double *g
         save(p, f0, f); save(q, g0, g);
     }
 }
+void ref3_ad_collide_nc(int lx, int ly, int lz, double *f0, double *f, double *g0, double *g, double *rho,
+                       double *ux, double *uy, double *uz, const double *alpha, double nu, double *tem, double *qx,
+                       double *qy, double *qz, const double *diff, double gx, double gy, double gz, double tem0,
+                       int issave, double *gsnap) {
+    {
+        D3Q15<double> p(lx, ly, lz), q(lx, ly, lz);
+        load(p, f0, f); load(q, g0, g);
+        AD::MacroBrinkmanCollideNaturalConvection(p, rho, ux, uy, uz, alpha, nu, q, tem, qx, qy, qz, diff, gx, gy, gz,
+                                                  tem0, issave != 0, gsnap);
+        save(p, f0, f); save(q, g0, g);
+    }
+}
 void ref3_ans_collide(int lx, int ly, int lz, double *f0, double *f, const double *rho, const double *ux,
                      const double *uy, const double *uz, double *ip, double *iux, double *iuy, double *iuz, double *imx,
                      double *imy, double *imz, double nu, const double *alpha, int issave) {
@@ -177,6 +189,20 @@ void ref3_aad_collide_fc(int lx, int ly, int lz, double *f0, double *f, double *
         load(p, f0, f); load(q, g0, g);
         AAD::MacroBrinkmanCollideForceConvection(p, rho, ux, uy, uz, ip, iux, iuy, iuz, imx, imy, imz, alpha, nu, q,
                                                  tem, item, iqx, iqy, iqz, diff, issave != 0, igsnap);
+        save(p, f0, f); save(q, g0, g);
+    }
+}
+
+void ref3_aad_collide_nc(int lx, int ly, int lz, double *f0, double *f, double *g0, double *g,
+                        const double *rho, const double *ux, const double *uy, const double *uz, double *ip, double *iux,
+                        double *iuy, double *iuz, double *imx, double *imy, double *imz, const double *alpha, double nu,
+                        const double *tem, double *item, double *iqx, double *iqy, double *iqz, const double *diff,
+                        double gx, double gy, double gz, int issave, double *igsnap) {
+    {
+        D3Q15<double> p(lx, ly, lz), q(lx, ly, lz);
+        load(p, f0, f); load(q, g0, g);
+        AAD::MacroBrinkmanCollideNaturalConvection(p, rho, ux, uy, uz, ip, iux, iuy, iuz, imx, imy, imz, alpha, nu, q,
+                                                   tem, item, iqx, iqy, iqz, diff, gx, gy, gz, issave != 0, igsnap);
         save(p, f0, f); save(q, g0, g);
     }
 }

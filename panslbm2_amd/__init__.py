@@ -72,6 +72,7 @@ def load():
         "lbm_ns_set_rho": [VP] * 5,
         "lbm_ad_initial_condition": [VP] * 5,
         "lbm_ad_macro_brinkman_collide_force_convection": [VP] * 6 + [D] + [VP] * 6 + [I, VP, VP],
+        "lbm_ad_macro_brinkman_collide_natural_convection": [VP] * 6 + [D] + [VP] * 6 + [D] * 4 + [I, VP],
         "lbm_ad_set_t": [VP] * 6,
         "lbm_ad_set_q": [VP] * 7 + [D],
         "lbm_ans_initial_condition": [VP] * 8,
@@ -80,6 +81,7 @@ def load():
         "lbm_ans_iset_rho": [VP, VP],
         "lbm_aad_initial_condition": [VP] * 8,
         "lbm_aad_macro_brinkman_collide_force_convection": [VP] * 13 + [D] + [VP] * 7 + [I, VP],
+        "lbm_aad_macro_brinkman_collide_natural_convection": [VP] * 13 + [D] + [VP] * 7 + [D] * 3 + [I, VP],
         "lbm_aad_iset_t": [VP] * 5,
         "lbm_aad_iset_q": [VP] * 5 + [D],
         "lbm_aad_iset_rho": [VP] * 8 + [D],
@@ -105,8 +107,10 @@ def load():
         "lbm_ns_macro_collide_p": [VP] * 6 + [D, I],
         "lbm_ns_macro_brinkman_collide_p": [VP] * 6 + [D, VP, I],
         "lbm_ad_macro_brinkman_collide_force_convection_p": [VP] * 7 + [D] + [VP] * 6 + [I, VP, VP],
+        "lbm_ad_macro_brinkman_collide_natural_convection_p": [VP] * 7 + [D] + [VP] * 6 + [D] * 4 + [I, VP],
         "lbm_ans_macro_brinkman_collide_p": [VP] * 13 + [D, VP, I],
         "lbm_aad_macro_brinkman_collide_force_convection_p": [VP] * 14 + [D] + [VP] * 7 + [I, VP],
+        "lbm_aad_macro_brinkman_collide_natural_convection_p": [VP] * 14 + [D] + [VP] * 7 + [D] * 3 + [I, VP],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

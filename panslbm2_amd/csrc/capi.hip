@@ -852,6 +852,32 @@ int lbm_ad_macro_brinkman_collide_force_convection(lbm_lattice *p, double *rho, 
     return lbm_ad_macro_brinkman_collide_force_convection_p(nullptr, p, rho, ux, uy, uz, alpha, nu, q, tem, qx, qy, qz,
                                                             diff, issave, g, heat);
 }
+int lbm_ad_macro_brinkman_collide_natural_convection_p(lbm_program *P, lbm_lattice *p, double *rho, double *ux,
+                                                       double *uy, double *uz, const double *alpha, double nu,
+                                                       lbm_lattice *q, double *tem, double *qx, double *qy,
+                                                       double *qz, const double *diff, double gx, double gy,
+                                                       double gz, double tem0, int issave, double *g) {
+    if (!ns_lattice(p) || !q || !alpha || !diff) return fail(LBM_ERR_ARG, "bad arguments");
+    if (issave && (!fields_ok({rho, ux, uy, tem, qx, qy}) || (p->nd == 3 && (!uz || !qz))))
+        return fail(LBM_ERR_ARG, "null field");
+    return collide_call(P, p, q, {rho, ux, uy, uz, tem, qx, qy, qz}, [&](const lbm::CollideLaunch &cl) {
+        lbm::ADArgs a{rho, ux, uy, uz, tem, qx, qy, qz, issave ? g : nullptr, alpha, diff, nullptr, nu, issave};
+        a.natural = 1;
+        a.gx = gx;
+        a.gy = gy;
+        a.gz = p->nd == 3 ? gz : 0.0;
+        a.tem0 = tem0;
+        return hip_result(lbm::launch_ad_collide(cl, a), "ad collide natural convection");
+    });
+}
+int lbm_ad_macro_brinkman_collide_natural_convection(lbm_lattice *p, double *rho, double *ux, double *uy, double *uz,
+                                                     const double *alpha, double nu, lbm_lattice *q, double *tem,
+                                                     double *qx, double *qy, double *qz, const double *diff,
+                                                     double gx, double gy, double gz, double tem0, int issave,
+                                                     double *g) {
+    return lbm_ad_macro_brinkman_collide_natural_convection_p(nullptr, p, rho, ux, uy, uz, alpha, nu, q, tem, qx, qy,
+                                                              qz, diff, gx, gy, gz, tem0, issave, g);
+}
 int lbm_ad_set_t(lbm_lattice *q, const int *mask, const double *tem, const double *ux, const double *uy,
                  const double *uz) {
     if (!q || !mask || !tem || !ux || !uy || (q->nd == 3 && !uz)) return fail(LBM_ERR_ARG, "bad arguments");
@@ -935,7 +961,8 @@ int lbm_aad_macro_brinkman_collide_force_convection_p(lbm_program *P, lbm_lattic
                                                       double *iqz, const double *diff, int issave, double *ig) {
     if (!ns_lattice(p) || !q || !fields_ok({rho, ux, uy, alpha, tem, diff}) || (p->nd == 3 && !uz))
         return fail(LBM_ERR_ARG, "bad arguments");
-    if (issave && (!fields_ok({ip, iux, iuy, imx, imy, item, iqx, iqy}) || (p->nd == 3 && (!iuz || !imz || !iqz))))
+    // the 3D forced-convection save writes no iuz (adjointadvection.h:1074-1084): iuz may be NULL
+    if (issave && (!fields_ok({ip, iux, iuy, imx, imy, item, iqx, iqy}) || (p->nd == 3 && (!imz || !iqz))))
         return fail(LBM_ERR_ARG, "null field");
     return collide_call(P, p, q, {ip, iux, iuy, iuz, imx, imy, imz, item, iqx, iqy, iqz}, [&](const lbm::CollideLaunch &cl) {
         lbm::AADArgs a{rho, ux, uy, uz, alpha, tem, diff, ip, iux, iuy, iuz, imx, imy, imz, item, iqx, iqy, iqz,
@@ -952,6 +979,39 @@ int lbm_aad_macro_brinkman_collide_force_convection(lbm_lattice *p, const double
     return lbm_aad_macro_brinkman_collide_force_convection_p(nullptr, p, rho, ux, uy, uz, ip, iux, iuy, iuz, imx, imy,
                                                              imz, alpha, nu, q, tem, item, iqx, iqy, iqz, diff, issave,
                                                              ig);
+}
+int lbm_aad_macro_brinkman_collide_natural_convection_p(lbm_program *P, lbm_lattice *p, const double *rho,
+                                                        const double *ux, const double *uy, const double *uz,
+                                                        double *ip, double *iux, double *iuy, double *iuz,
+                                                        double *imx, double *imy, double *imz, const double *alpha,
+                                                        double nu, lbm_lattice *q, const double *tem, double *item,
+                                                        double *iqx, double *iqy, double *iqz, const double *diff,
+                                                        double gx, double gy, double gz, int issave, double *ig) {
+    if (!ns_lattice(p) || !q || !fields_ok({rho, ux, uy, alpha, tem, diff}) || (p->nd == 3 && !uz))
+        return fail(LBM_ERR_ARG, "bad arguments");
+    if (issave &&
+        (!fields_ok({ip, iux, iuy, imx, imy, item, iqx, iqy}) || (p->nd == 3 && (!iuz || !imz || !iqz))))
+        return fail(LBM_ERR_ARG, "null field");
+    return collide_call(P, p, q, {ip, iux, iuy, iuz, imx, imy, imz, item, iqx, iqy, iqz}, [&](const lbm::CollideLaunch &cl) {
+        lbm::AADArgs a{rho, ux, uy, uz, alpha, tem, diff, ip, iux, iuy, iuz, imx, imy, imz, item, iqx, iqy, iqz,
+                       issave ? ig : nullptr, nu, issave};
+        a.natural = 1;
+        a.gx = gx;
+        a.gy = gy;
+        a.gz = p->nd == 3 ? gz : 0.0;
+        return hip_result(lbm::launch_aad_collide(cl, a), "aad collide natural convection");
+    });
+}
+int lbm_aad_macro_brinkman_collide_natural_convection(lbm_lattice *p, const double *rho, const double *ux,
+                                                      const double *uy, const double *uz, double *ip, double *iux,
+                                                      double *iuy, double *iuz, double *imx, double *imy,
+                                                      double *imz, const double *alpha, double nu, lbm_lattice *q,
+                                                      const double *tem, double *item, double *iqx, double *iqy,
+                                                      double *iqz, const double *diff, double gx, double gy,
+                                                      double gz, int issave, double *ig) {
+    return lbm_aad_macro_brinkman_collide_natural_convection_p(nullptr, p, rho, ux, uy, uz, ip, iux, iuy, iuz, imx,
+                                                               imy, imz, alpha, nu, q, tem, item, iqx, iqy, iqz,
+                                                               diff, gx, gy, gz, issave, ig);
 }
 int lbm_aad_iset_t(lbm_lattice *q, const int *mask, const double *ux, const double *uy, const double *uz) {
     if (!q || !mask || !ux || !uy || (q->nd == 3 && !uz)) return fail(LBM_ERR_ARG, "bad arguments");

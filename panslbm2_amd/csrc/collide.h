@@ -46,18 +46,27 @@ struct NSBody {
     }
 };
 
-// AD::MacroBrinkmanCollideForceConvection, advection.h:814-926 (+ optional volumetric source)
+// AD::MacroBrinkmanCollideForceConvection, advection.h:814-926 (+ optional volumetric source), and
+// AD::MacroBrinkmanCollideNaturalConvection, advection.h:930-1044 (natural = 1)
 template <class LA, class LB>
 struct ADBody {
     double *rho, *ux, *uy, *uz, *tem, *qx, *qy, *qz, *gsnap;
     const double *alpha, *diff, *heat;
     double omegaf, iomegaf;
-    int save, N;
+    int save, N, natural;
+    double gx, gy, gz, tem0;
     __device__ __forceinline__ void operator()(double *f, double *g, int n, bool bnd) const {
         double omegag = 1.0 / (3.0 * diff[n] + 0.5), iomegag = 1.0 - omegag;
         double r, vx, vy, vz, T, hx, hy, hz;
         ns_macro<LA>(f, r, vx, vy, vz);
         ad_macro<LB>(g, vx, vy, vz, omegag, T, hx, hy, hz);
+        if (natural) {  // ExternalForceNaturalConvection, advection.h:73-87
+            const double dT = T - tem0;
+            sfor<1, LA::nc>([&](auto c) {
+                constexpr int C = decltype(c)::value;
+                f[C] += 3.0 * LA::w[C] * cdot<LA, C>(gx, gy, gz) * dT;
+            });
+        }
         ns_brinkman<LA>(f, r, vx, vy, vz, alpha[n]);
         if (heat) {
             double h = heat[n];
@@ -123,13 +132,15 @@ struct ANSBody {
     }
 };
 
-// AAD::MacroBrinkmanCollideForceConvection, adjointadvection.h:997-1109
+// AAD::MacroBrinkmanCollideForceConvection, adjointadvection.h:997-1109, and
+// AAD::MacroBrinkmanCollideNaturalConvection, adjointadvection.h:1113-1230 (natural = 1)
 template <class LA, class LB>
 struct AADBody {
     const double *rho, *ux, *uy, *uz, *alpha, *tem, *diff;
     double *ip, *iux, *iuy, *iuz, *imx, *imy, *imz, *item, *iqx, *iqy, *iqz, *igsnap;
     double omegaf, iomegaf;
-    int save, N;
+    int save, N, natural;
+    double gx, gy, gz;
     __device__ __forceinline__ void operator()(double *f, double *g, int n, bool bnd) const {
         double omegag = 1.0 / (3.0 * diff[n] + 0.5), iomegag = 1.0 - omegag;
         double vx = ux[n], vy = uy[n], vz = LA::nd == 3 ? uz[n] : 0.0;
@@ -138,6 +149,13 @@ struct AADBody {
         aad_macro<LB>(g, it, hx, hy, hz);
         aad_brinkman<LA>(f, rho[n], vx, vy, vz, mx, my, mz, tem[n], hx, hy, hz, omegag, alpha[n]);
         ans_macro<LA>(f, vx, vy, vz, a, bx, by, bz, mx, my, mz);
+        if (natural) {  // ExternalForceNaturalConvection, adjointadvection.h:114-132, then re-macro
+            double coef;
+            if constexpr (LA::nd == 2) coef = 3.0 * (mx * gx + my * gy);
+            else coef = 3.0 * (mx * gx + my * gy + mz * gz);
+            sfor<0, LB::nc>([&](auto c) { g[decltype(c)::value] += coef; });
+            aad_macro<LB>(g, it, hx, hy, hz);
+        }
         if (save == 1 || (save == 2 && bnd)) {
             ip[n] = a;
             iux[n] = bx;
@@ -147,7 +165,8 @@ struct AADBody {
             item[n] = it;
             iqx[n] = hx;
             iqy[n] = hy;
-            if constexpr (LA::nd == 3) {  // the reference's 3D save writes no iuz (adjointadvection.h:1074-1084)
+            if constexpr (LA::nd == 3) {  // the 3D ForceConvection save writes no iuz (adjointadvection.h:1074-1084)
+                if (natural) iuz[n] = bz;
                 imz[n] = mz;
                 iqz[n] = hz;
             }
@@ -394,6 +413,7 @@ hipError_t ad_collide_nd(const CollideLaunch &cl, const ADArgs &a) {
         b.rho = a.rho; b.ux = a.ux; b.uy = a.uy; b.uz = a.uz; b.tem = a.tem; b.qx = a.qx; b.qy = a.qy; b.qz = a.qz;
         b.gsnap = a.gsnap; b.alpha = a.alpha; b.diff = a.diff; b.heat = a.heat;
         b.omegaf = omega; b.iomegaf = 1.0 - omega; b.save = a.save; b.N = cl.g.N;
+        b.natural = a.natural; b.gx = a.gx; b.gy = a.gy; b.gz = a.gz; b.tem0 = a.tem0;
         return b;
     });
 }
@@ -417,6 +437,7 @@ hipError_t aad_collide_nd(const CollideLaunch &cl, const AADArgs &a) {
         b.ip = a.ip; b.iux = a.iux; b.iuy = a.iuy; b.iuz = a.iuz; b.imx = a.imx; b.imy = a.imy; b.imz = a.imz;
         b.item = a.item; b.iqx = a.iqx; b.iqy = a.iqy; b.iqz = a.iqz; b.igsnap = a.igsnap;
         b.omegaf = omega; b.iomegaf = 1.0 - omega; b.save = a.save; b.N = cl.g.N;
+        b.natural = a.natural; b.gx = a.gx; b.gy = a.gy; b.gz = a.gz;
         return b;
     });
 }

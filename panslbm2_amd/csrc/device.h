@@ -30,6 +30,8 @@ struct ADArgs {
     const double *alpha, *diff, *heat;
     double nu;
     int save;
+    int natural;                  // 1: MacroBrinkmanCollideNaturalConvection (buoyancy g (T - tem0))
+    double gx, gy, gz, tem0;
 };
 struct ANSArgs {
     const double *rho, *ux, *uy, *uz, *alpha;
@@ -42,6 +44,8 @@ struct AADArgs {
     double *ip, *iux, *iuy, *iuz, *imx, *imy, *imz, *item, *iqx, *iqy, *iqz, *igsnap;
     double nu;
     int save;
+    int natural;                  // 1: MacroBrinkmanCollideNaturalConvection
+    double gx, gy, gz;
 };
 
 hipError_t launch_stream(int kind, const double *src, double *dst, Grid g, int pull, hipStream_t s);

@@ -26,7 +26,7 @@ from _oracle import ref  # noqa: E402
 
 
 def main(only=None):
-    """Write every fixture, or (only = list of case names) just those cases' fixtures."""
+    """Write every fixture, or (only = list of case and sequence names) just those fixtures."""
     if ref() is None:
         raise SystemExit("reference harness not built: make -C oracle ref")
     for case in refcases.CASES_2D + refcases.CASES_3D:
@@ -41,14 +41,15 @@ def main(only=None):
         path = os.path.join(HERE, case.name + ".npz")
         np.savez_compressed(path, **blob)
         print("wrote", path, os.path.getsize(path), "bytes")
-    if only:
-        return
     for seq in sequences.SEQUENCES:
+        if only and seq.name not in only:
+            continue
         out = seq.run(sequences.RefBackend())
         path = os.path.join(HERE, "seq_" + seq.name + ".npz")
         np.savez_compressed(path, **out)
         print("wrote", path, os.path.getsize(path), "bytes")
-    write_driver_golden()
+    if not only:
+        write_driver_golden()
 
 
 def write_driver_golden():

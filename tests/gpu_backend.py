@@ -136,6 +136,19 @@ class GpuBackend:
             self._prog(), p.h, *[self.f(n) for n in (rho, ux, uy, uz, alpha)], D(nu), q.h,
             *[self.f(n) for n in (tem, qx, qy, qz, diff)], int(save), self.f(gsnap), self.f(heat)))
 
+    def ad_collide_nc(self, p, rho, ux, uy, uz, alpha, nu, q, tem, qx, qy, qz, diff, gx, gy, gz, tem0, save=True,
+                      gsnap=None):
+        check(self.L.lbm_ad_macro_brinkman_collide_natural_convection_p(
+            self._prog(), p.h, *[self.f(n) for n in (rho, ux, uy, uz, alpha)], D(nu), q.h,
+            *[self.f(n) for n in (tem, qx, qy, qz, diff)], D(gx), D(gy), D(gz), D(tem0), int(save), self.f(gsnap)))
+
+    def aad_collide_nc(self, p, rho, ux, uy, uz, ip, iux, iuy, iuz, imx, imy, imz, alpha, nu, q, tem, item, iqx, iqy,
+                       iqz, diff, gx, gy, gz, save=True, igsnap=None):
+        check(self.L.lbm_aad_macro_brinkman_collide_natural_convection_p(
+            self._prog(), p.h, *[self.f(n) for n in (rho, ux, uy, uz, ip, iux, iuy, iuz, imx, imy, imz, alpha)],
+            D(nu), q.h, *[self.f(n) for n in (tem, item, iqx, iqy, iqz, diff)], D(gx), D(gy), D(gz), int(save),
+            self.f(igsnap)))
+
     def ans_collide(self, h, rho, ux, uy, uz, ip, iux, iuy, iuz, imx, imy, imz, nu, alpha, save=True):
         check(self.L.lbm_ans_macro_brinkman_collide_p(
             self._prog(), h.h, *[self.f(n) for n in (rho, ux, uy, uz, ip, iux, iuy, iuz, imx, imy, imz)], D(nu),

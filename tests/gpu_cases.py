@@ -7,6 +7,7 @@ import panslbm2_amd as pa
 from panslbm2_amd import D, check, hptr
 
 from _oracle import NC, ND
+from refcases import NC_G, NC_TEM0
 
 
 def run_gpu(ctx, case, op, inp):
@@ -93,6 +94,15 @@ def run_gpu(ctx, case, op, inp):
                                                                d("diff"), 1, snap, None))
         for n in names + ("gsnap",):
             out[n] = dev["o_" + n].download()
+    elif op == "ad_collide_nc":
+        names = ("rho", "ux", "uy", "uz", "tem", "qx", "qy", "qz")
+        o = outs(names)
+        snap = outs(["gsnap"], (NC[k], N))[0]
+        check(L.lbm_ad_macro_brinkman_collide_natural_convection(p.h, *o[:4], d("alpha"), D(0.1), q.h, *o[4:],
+                                                                 d("diff"), *[D(x) for x in NC_G], D(NC_TEM0), 1,
+                                                                 snap))
+        for n in names + ("gsnap",):
+            out[n] = dev["o_" + n].download()
     elif op == "ans_collide":
         names = ("ip", "iux", "iuy", "iuz", "imx", "imy", "imz")
         o = outs(names)
@@ -106,6 +116,15 @@ def run_gpu(ctx, case, op, inp):
         check(L.lbm_aad_macro_brinkman_collide_force_convection(p.h, d("rho"), d("ux"), d("uy"), uzp, *o[:7],
                                                                d("alpha"), D(0.1), q.h, d("tem"), *o[7:], d("diff"), 1,
                                                                snap))
+        for n in names + ("igsnap",):
+            out[n] = dev["o_" + n].download()
+    elif op == "aad_collide_nc":
+        names = ("ip", "iux", "iuy", "iuz", "imx", "imy", "imz", "item", "iqx", "iqy", "iqz")
+        o = outs(names)
+        snap = outs(["igsnap"], (NC[k], N))[0]
+        check(L.lbm_aad_macro_brinkman_collide_natural_convection(p.h, d("rho"), d("ux"), d("uy"), uzp, *o[:7],
+                                                                  d("alpha"), D(0.1), q.h, d("tem"), *o[7:],
+                                                                  d("diff"), *[D(x) for x in NC_G], 1, snap))
         for n in names + ("igsnap",):
             out[n] = dev["o_" + n].download()
     elif op == "sens_brinkman":

@@ -36,6 +36,11 @@ def _fields(rng, N):
     )
 
 
+# buoyancy parameters of the natural-convection cases: every component and T - tem0 nonzero
+NC_G = (1.3e-3, -2.1e-3, 1.7e-3)
+NC_TEM0 = 0.4
+
+
 def _i32(a):
     return np.ascontiguousarray(a, dtype=np.int32)
 
@@ -148,6 +153,14 @@ def run_ref(case, op, inp):
                             ptr(o["qz"]), ptr(inp["diff"]), 1, ptr(snap))
         out.update(o)
         out["gsnap"] = snap.reshape(case.N, nc).T.copy()
+    elif op == "ad_collide_nc":
+        o = {k: np.zeros(case.N) for k in ("rho", "ux", "uy", "uz", "tem", "qx", "qy", "qz")}
+        snap = np.zeros(nc * case.N)
+        fn("ad_collide_nc")(*L, ptr(f0), ptr(f), ptr(g0), ptr(g), ptr(o["rho"]), ptr(o["ux"]), ptr(o["uy"]),
+                            ptr(o["uz"]), ptr(inp["alpha"]), D(0.1), ptr(o["tem"]), ptr(o["qx"]), ptr(o["qy"]),
+                            ptr(o["qz"]), ptr(inp["diff"]), *[D(x) for x in NC_G], D(NC_TEM0), 1, ptr(snap))
+        out.update(o)
+        out["gsnap"] = snap.reshape(case.N, nc).T.copy()
     elif op == "ans_collide":
         o = {k: np.zeros(case.N) for k in ("ip", "iux", "iuy", "iuz", "imx", "imy", "imz")}
         fn("ans_collide")(*L, ptr(f0), ptr(f), ptr(inp["rho"]), ptr(inp["ux"]), ptr(inp["uy"]), ptr(uz),
@@ -161,6 +174,16 @@ def run_ref(case, op, inp):
                              ptr(uz), ptr(o["ip"]), ptr(o["iux"]), ptr(o["iuy"]), ptr(o["iuz"]), ptr(o["imx"]),
                              ptr(o["imy"]), ptr(o["imz"]), ptr(inp["alpha"]), D(0.1), ptr(inp["tem"]), ptr(o["item"]),
                              ptr(o["iqx"]), ptr(o["iqy"]), ptr(o["iqz"]), ptr(inp["diff"]), 1, ptr(snap))
+        out.update(o)
+        out["igsnap"] = snap.reshape(case.N, nc).T.copy()
+    elif op == "aad_collide_nc":
+        o = {k: np.zeros(case.N) for k in ("ip", "iux", "iuy", "iuz", "imx", "imy", "imz", "item", "iqx", "iqy", "iqz")}
+        snap = np.zeros(nc * case.N)
+        fn("aad_collide_nc")(*L, ptr(f0), ptr(f), ptr(g0), ptr(g), ptr(inp["rho"]), ptr(inp["ux"]), ptr(inp["uy"]),
+                             ptr(uz), ptr(o["ip"]), ptr(o["iux"]), ptr(o["iuy"]), ptr(o["iuz"]), ptr(o["imx"]),
+                             ptr(o["imy"]), ptr(o["imz"]), ptr(inp["alpha"]), D(0.1), ptr(inp["tem"]), ptr(o["item"]),
+                             ptr(o["iqx"]), ptr(o["iqy"]), ptr(o["iqz"]), ptr(inp["diff"]), *[D(x) for x in NC_G], 1,
+                             ptr(snap))
         out.update(o)
         out["igsnap"] = snap.reshape(case.N, nc).T.copy()
     elif op == "sens_brinkman":
@@ -277,6 +300,14 @@ def run_oracle(case, op, inp):
                            ptr(snap), None)
         out.update(o)
         out["gsnap"] = snap
+    elif op == "ad_collide_nc":
+        o = {kk: np.zeros(case.N) for kk in ("rho", "ux", "uy", "uz", "tem", "qx", "qy", "qz")}
+        snap = np.zeros((NC[k], case.N))
+        O.ol_ad_collide_nc(p.h, ptr(o["rho"]), ptr(o["ux"]), ptr(o["uy"]), ptr(o["uz"]), ptr(inp["alpha"]), D(0.1),
+                           q.h, ptr(o["tem"]), ptr(o["qx"]), ptr(o["qy"]), ptr(o["qz"]), ptr(inp["diff"]),
+                           *[D(x) for x in NC_G], D(NC_TEM0), 1, ptr(snap))
+        out.update(o)
+        out["gsnap"] = snap
     elif op == "ans_collide":
         o = {kk: np.zeros(case.N) for kk in ("ip", "iux", "iuy", "iuz", "imx", "imy", "imz")}
         O.ol_ans_collide(p.h, ptr(inp["rho"]), ptr(inp["ux"]), ptr(inp["uy"]), ptr(uz), ptr(o["ip"]), ptr(o["iux"]),
@@ -290,6 +321,15 @@ def run_oracle(case, op, inp):
                             ptr(o["iuy"]), ptr(o["iuz"]), ptr(o["imx"]), ptr(o["imy"]), ptr(o["imz"]),
                             ptr(inp["alpha"]), D(0.1), q.h, ptr(inp["tem"]), ptr(o["item"]), ptr(o["iqx"]),
                             ptr(o["iqy"]), ptr(o["iqz"]), ptr(inp["diff"]), 1, ptr(snap))
+        out.update(o)
+        out["igsnap"] = snap
+    elif op == "aad_collide_nc":
+        o = {kk: np.zeros(case.N) for kk in ("ip", "iux", "iuy", "iuz", "imx", "imy", "imz", "item", "iqx", "iqy", "iqz")}
+        snap = np.zeros((NC[k], case.N))
+        O.ol_aad_collide_nc(p.h, ptr(inp["rho"]), ptr(inp["ux"]), ptr(inp["uy"]), ptr(uz), ptr(o["ip"]), ptr(o["iux"]),
+                            ptr(o["iuy"]), ptr(o["iuz"]), ptr(o["imx"]), ptr(o["imy"]), ptr(o["imz"]),
+                            ptr(inp["alpha"]), D(0.1), q.h, ptr(inp["tem"]), ptr(o["item"]), ptr(o["iqx"]),
+                            ptr(o["iqy"]), ptr(o["iqz"]), ptr(inp["diff"]), *[D(x) for x in NC_G], 1, ptr(snap))
         out.update(o)
         out["igsnap"] = snap
     elif op == "sens_brinkman":
@@ -346,6 +386,7 @@ OPS_2D = {
     "ns_collide": "navierstokes.h:432-455", "ns_brinkman_collide": "navierstokes.h:487-514",
     "ad_collide_fc": "advection.h:814-867", "ans_collide": "adjointnavierstokes.h:398-431",
     "aad_collide_fc": "adjointadvection.h:997-1050",
+    "ad_collide_nc": "advection.h:930-984,73-79", "aad_collide_nc": "adjointadvection.h:1113-1168,114-122",
     "sens_brinkman": "adjointnavierstokes.h:542-546", "sens_brinkman_diffusivity": "adjointadvection.h:1461-1475",
     "sens_temp_heat_source": "adjointadvection.h:1497-1517,755-801",
     "residual": "residual.h:9-51", "normalize": "normalize.h:9-24", "density_filter": "densityfilter.h:390-420,512-515",
@@ -373,6 +414,7 @@ OPS_3D = {
     "ns_collide": "navierstokes.h:459-483", "ns_brinkman_collide": "navierstokes.h:518-546",
     "ad_collide_fc": "advection.h:871-926", "ans_collide": "adjointnavierstokes.h:435-470",
     "aad_collide_fc": "adjointadvection.h:1054-1109",
+    "ad_collide_nc": "advection.h:988-1044,81-87", "aad_collide_nc": "adjointadvection.h:1172-1230,124-132",
     "sens_brinkman": "adjointnavierstokes.h:550-554", "sens_brinkman_diffusivity": "adjointadvection.h:1478-1493",
     "residual": "residual.h:39-51", "normalize": "normalize.h:9-24", "density_filter": "densityfilter.h:390-420,512-515",
 }

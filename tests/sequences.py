@@ -125,6 +125,19 @@ class OracleBackend(_FieldStore):
                                *[ptr(self.f(n)) for n in (tem, qx, qy, qz, diff)], int(save), ptr(self.f(gsnap)),
                                ptr(self.f(heat)))
 
+    def ad_collide_nc(self, p, rho, ux, uy, uz, alpha, nu, q, tem, qx, qy, qz, diff, gx, gy, gz, tem0, save=True,
+                      gsnap=None):
+        lib().ol_ad_collide_nc(p.h, *[ptr(self.f(n)) for n in (rho, ux, uy, uz, alpha)], D(nu), q.h,
+                               *[ptr(self.f(n)) for n in (tem, qx, qy, qz, diff)], D(gx), D(gy), D(gz), D(tem0),
+                               int(save), ptr(self.f(gsnap)))
+
+    def aad_collide_nc(self, p, rho, ux, uy, uz, ip, iux, iuy, iuz, imx, imy, imz, alpha, nu, q, tem, item, iqx, iqy,
+                       iqz, diff, gx, gy, gz, save=True, igsnap=None):
+        lib().ol_aad_collide_nc(p.h, *[ptr(self.f(n)) for n in (rho, ux, uy, uz, ip, iux, iuy, iuz, imx, imy, imz,
+                                                                alpha)], D(nu), q.h,
+                                *[ptr(self.f(n)) for n in (tem, item, iqx, iqy, iqz, diff)], D(gx), D(gy), D(gz),
+                                int(save), ptr(self.f(igsnap)))
+
     def ans_collide(self, h, rho, ux, uy, uz, ip, iux, iuy, iuz, imx, imy, imz, nu, alpha, save=True):
         lib().ol_ans_collide(h.h, *[ptr(self.f(n)) for n in (rho, ux, uy, uz, ip, iux, iuy, iuz, imx, imy, imz)],
                              D(nu), ptr(self.f(alpha)), int(save))
@@ -282,6 +295,27 @@ class RefBackend(_FieldStore):
                                     ptr(buf) if gsnap else None)
         if save:
             self._snap_out(gsnap, q, buf)
+
+    def ad_collide_nc(self, p, rho, ux, uy, uz, alpha, nu, q, tem, qx, qy, qz, diff, gx, gy, gz, tem0, save=True,
+                      gsnap=None):
+        buf = np.zeros(NC[q.kind] * q.N)
+        self._r(p, "ad_collide_nc")(*p.L, ptr(p.f0), ptr(p.f), ptr(q.f0), ptr(q.f),
+                                    *[ptr(self.f(n)) for n in (rho, ux, uy, uz, alpha)], D(nu),
+                                    *[ptr(self.f(n)) for n in (tem, qx, qy, qz, diff)], D(gx), D(gy), D(gz), D(tem0),
+                                    int(save), ptr(buf) if gsnap else None)
+        if save:
+            self._snap_out(gsnap, q, buf)
+
+    def aad_collide_nc(self, p, rho, ux, uy, uz, ip, iux, iuy, iuz, imx, imy, imz, alpha, nu, q, tem, item, iqx, iqy,
+                       iqz, diff, gx, gy, gz, save=True, igsnap=None):
+        buf = np.zeros(NC[q.kind] * q.N)
+        self._r(p, "aad_collide_nc")(*p.L, ptr(p.f0), ptr(p.f), ptr(q.f0), ptr(q.f),
+                                     *[ptr(self.f(n)) for n in (rho, ux, uy, uz, ip, iux, iuy, iuz, imx, imy, imz,
+                                                                alpha)], D(nu),
+                                     *[ptr(self.f(n)) for n in (tem, item, iqx, iqy, iqz, diff)], D(gx), D(gy), D(gz),
+                                     int(save), ptr(buf) if igsnap else None)
+        if save:
+            self._snap_out(igsnap, q, buf)
 
     def ans_collide(self, h, rho, ux, uy, uz, ip, iux, iuy, iuz, imx, imy, imz, nu, alpha, save=True):
         self._r(h, "ans_collide")(*h.L, ptr(h.f0), ptr(h.f),
@@ -603,9 +637,118 @@ class ForcedConvection3D:
         return out
 
 
+class NaturalConvection2D:
+    """Coupled NS + AD natural convection, forward then adjoint, then the heat-source sensitivity: the
+    call order of test/nsadncsens.cpp:87-177 (MacroBrinkmanCollideNaturalConvection, advection.h:930
+    and adjointadvection.h:1113; mirror at x = 0, bounce-back elsewhere; T = tem0 on x = lx-1 and
+    y = ly-1; heat flux qn into y = 0 for x < L, adiabatic elsewhere on y = 0).  As in that driver the
+    velocity and flux arrays are swapped after every step and not swapped back, so the adjoint reads
+    the last-but-one forward velocity.  gx is nonzero here (0 in the driver) so that both buoyancy
+    components act; Ra is lowered so that the small cavity stays stable."""
+
+    def __init__(self, name, lx, ly, nt, nu=0.1, pr=6.0, ra=3.0e3, qn=1e-2, L=2.0, seed=7, check=None):
+        self.name, self.lx, self.ly, self.nt = name, lx, ly, nt
+        self.nu, self.pr, self.ra, self.qn, self.L = nu, pr, ra, qn, L
+        self.seed, self.check = seed, check
+
+    def setup(self):
+        lx, ly = self.lx, self.ly
+        N = lx * ly
+        i, j = np.meshgrid(np.arange(lx), np.arange(ly), indexing="xy")
+        i, j = i.reshape(-1), j.reshape(-1)
+        rng = np.random.default_rng(self.seed)
+        s = rng.uniform(0.01, 0.99, N)
+        qf, qg, amax = 1e-2, 1e0, 1.0e4
+        dfl = self.nu / self.pr
+        dso = 10.0 * dfl
+        u = self.nu * np.sqrt(self.ra / self.pr) / (ly - 1.0)
+        d = dict(
+            alpha=amax / (ly - 1.0) * qf * (1.0 - s) / (s + qf),
+            dads=-amax / (ly - 1.0) * qf * (1.0 + qf) / (s + qf) ** 2,
+            diff=dso + (dfl - dso) * s * (1.0 + qg) / (s + qg),
+            dkds=(dfl - dso) * qg * (1.0 + qg) / (s + qg) ** 2,
+        )
+        self.gy = u * u / (ly - 1.0)
+        self.gx = 0.3 * self.gy
+        self.tem0 = 0.0
+        d["type_f"] = np.where(i == 0, 2, 1).astype(np.int32)
+        d["type_g"] = np.where(i == 0, 2, 0).astype(np.int32)
+        d["set_t"] = ((i == lx - 1) | (j == ly - 1)).astype(np.int32)
+        d["bottom"] = (j == 0).astype(np.int32)
+        d["heat_edge"] = ((j == 0) & (i < self.L)).astype(np.int32)
+        d["qn"] = np.where((j == 0) & (i < self.L), self.qn, 0.0)
+        return d
+
+    def run(self, b):
+        lx, ly, N = self.lx, self.ly, self.lx * self.ly
+        d = self.setup()
+        pf = b.lattice(D2Q9, lx, ly)
+        pg = b.lattice(D2Q9, lx, ly)
+        for k in ("alpha", "dads", "diff", "dkds"):
+            b.put(k, d[k])
+        b.put("rho", np.ones(N))
+        for k in ("ux", "uy", "uxp", "uyp", "tem", "qx", "qy", "qxp", "qyp", "ip", "iux", "iuy", "iuxp", "iuyp",
+                  "imx", "imy", "item", "iqx", "iqy", "iqxp", "iqyp", "dfds"):
+            b.zeros(k, N)
+        b.zeros("g", (NC[D2Q9], N))
+        b.zeros("ig", (NC[D2Q9], N))
+        tem0 = np.full(N, self.tem0)
+        g = (self.gx, self.gy, 0.0)
+        b.ns_init(pf, "rho", "ux", "uy")
+        b.ad_init(pg, "tem", "ux", "uy")
+        res = []
+        for t in range(1, self.nt + 1):
+            b.ad_collide_nc(pf, "rho", "ux", "uy", None, "alpha", self.nu, pg, "tem", "qx", "qy", None, "diff", *g,
+                            self.tem0, True, "g")
+            if self.check and t % self.check == 0:
+                res.append((b.residual(["ux", "uy"], ["uxp", "uyp"]), b.residual(["qx", "qy"], ["qxp", "qyp"])))
+            b.stream(pf)
+            b.stream(pg)
+            b.bc(pf, d["type_f"])
+            b.ad_set_t(pg, d["set_t"], tem0, "ux", "uy")
+            b.ad_set_q(pg, d["bottom"], d["qn"], "ux", "uy", None, "diff")
+            b.bc(pg, d["type_g"])
+            b.smooth(pf)
+            b.smooth(pg)
+            for v in ("ux", "uy", "qx", "qy"):
+                b.swap(v, v + "p")
+        out = {"f": b.pops(pf), "g": b.pops(pg)}
+        for k in ("rho", "ux", "uy", "uxp", "uyp", "tem", "qx", "qy", "g"):
+            out["fwd_" + k] = b.get(k)
+        b.ans_init(pf, "ux", "uy", None, "ip", "iux", "iuy", None)
+        b.aad_init(pg, "ux", "uy", None, "item", "iqx", "iqy", None)
+        for t in range(1, self.nt + 1):
+            b.aad_collide_nc(pf, "rho", "ux", "uy", None, "ip", "iux", "iuy", None, "imx", "imy", None, "alpha",
+                             self.nu, pg, "tem", "item", "iqx", "iqy", None, "diff", *g, True, "ig")
+            if self.check and t % self.check == 0:
+                res.append((b.residual(["iux", "iuy"], ["iuxp", "iuyp"]), b.residual(["iqx", "iqy"], ["iqxp", "iqyp"])))
+            b.istream(pf)
+            b.istream(pg)
+            b.aad_iset_t(pg, d["set_t"], "ux", "uy")
+            b.aad_iset_q(pg, d["bottom"], "ux", "uy")
+            b.aad_iset_q(pg, d["heat_edge"], "ux", "uy", None, 1.0)
+            b.bc(pg, d["type_g"], adjoint=True)
+            b.bc(pf, d["type_f"], adjoint=True)
+            b.smooth(pf)
+            b.smooth(pg)
+            for v in ("iux", "iuy", "iqx", "iqy"):
+                b.swap(v, v + "p")
+        b.sens_temp_heat_source(pg, "dfds", "ux", "uy", None, "imx", "imy", None, "dads", "tem", "item", "iqx", "iqy",
+                                None, "g", "ig", "diff", "dkds", d["heat_edge"], d["qn"])
+        out.update({"adj_f": b.pops(pf), "adj_g": b.pops(pg), "dfds": b.get("dfds")})
+        for k in ("ip", "iux", "iuy", "imx", "imy", "item", "iqx", "iqy", "ig"):
+            out["adj_" + k] = b.get(k)
+        tem = out["fwd_tem"]
+        out["objective"] = np.array(sum(tem[i] for i in range(lx) if i < self.L) / self.L)
+        if res:
+            out["residuals"] = np.array(res)
+        return out
+
+
 SEQUENCES = [
     Cavity("cavity_21x21", 21, 21, 300),
     ForcedConvection2D("fc2d_24x16", 24, 16, 200, check=100),
     ForcedConvection3D("fc3d_7x5x5", 7, 5, 5, 60, check=30),
     ForcedConvection2D("fc2d_noswap_20x12", 20, 12, 120, swap=False),
+    NaturalConvection2D("nc2d_22x18", 22, 18, 200, check=100),
 ]
