@@ -48,11 +48,12 @@ def kernel_rate(hs, steps=100, reps=5):
     return ms / (reps * hs.check_every)
 
 
-def big_grid_roofline(ctx, lx=4096, ly=4096, use_graphs=True):
+def big_grid_roofline(ctx, lx=4096, ly=4096, use_graphs=True, prec=0):
     from panslbm2_amd import D2Q5
     from panslbm2_amd.heatsink2d import Heatsink2D
 
-    hs = Heatsink2D(ctx, lx=lx, ly=ly, ad_kind=D2Q5, max_steps=200, filter_radius=1.0, use_graphs=use_graphs)
+    hs = Heatsink2D(ctx, lx=lx, ly=ly, ad_kind=D2Q5, max_steps=200, filter_radius=1.0, use_graphs=use_graphs,
+                    prec=prec)
     L, d = hs.L, hs.d
     from panslbm2_amd import check
 

@@ -59,6 +59,13 @@ extern "C" {
 #define LBM_D2Q5 2
 #define LBM_D3Q7 3
 
+/* population storage (no reference counterpart: the reference is FP64 throughout).  LBM_F64 is
+ * value-identical to the reference.  LBM_F32 stores the populations in FP32, those of a lattice
+ * given an NS initial condition as deviations f - w_c from the rest state, and collides them in
+ * FP32 on deviations; boundary operators, fields, sensitivities and reductions stay FP64. */
+#define LBM_F64 0
+#define LBM_F32 1
+
 /* boundary types of lbm_bc: d2q9.h:19-22 */
 #define LBM_BARRIER 1
 #define LBM_MIRROR 2
@@ -89,6 +96,12 @@ int lbm_event_elapsed_ms(lbm_context *ctx, int event_start, int event_stop, floa
 
 /* ---- lattice: D2Q9<T>(lx, ly) / D3Q15<T>(lx, ly, lz), d2q9.h:28-58, d3q15.h:28-86 ---- */
 int lbm_lattice_create(lbm_context *ctx, int kind, int lx, int ly, int lz, lbm_lattice **out);
+/* the same with a storage precision LBM_F64 / LBM_F32; coupled lattices must share it */
+int lbm_lattice_create_prec(lbm_context *ctx, int kind, int lx, int ly, int lz, int prec, lbm_lattice **out);
+int lbm_lattice_precision(const lbm_lattice *lat);
+/* change the storage precision in place, converting the current populations (e.g. finish a solve
+ * in LBM_F64 after an LBM_F32 start); programs and graphs holding the old buffers are invalidated */
+int lbm_lattice_set_precision(lbm_lattice *lat, int prec);
 int lbm_lattice_destroy(lbm_lattice *lat);
 int lbm_lattice_nc(const lbm_lattice *lat);
 int lbm_lattice_nxyz(const lbm_lattice *lat);
@@ -97,7 +110,8 @@ int lbm_lattice_upload(lbm_lattice *lat, const double *soa_host);
 int lbm_lattice_download(lbm_lattice *lat, double *soa_host);
 int lbm_lattice_upload_aos(lbm_lattice *lat, const double *f0_host, const double *f_host);
 int lbm_lattice_download_aos(lbm_lattice *lat, double *f0_host, double *f_host);
-/* device pointer of the current population buffer (nc*N doubles, plane order) */
+/* device pointer of the current population buffer (nc*N doubles, plane order); NULL unless LBM_F64.
+ * upload / download convert to and from FP64 for the other storages. */
 double *lbm_lattice_populations(lbm_lattice *lat);
 
 /* ---- streaming and lattice boundaries ---- */

@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(HERE, "libpanslbm2_hip.so")
 D2Q9, D3Q15, D2Q5, D3Q7 = 0, 1, 2, 3
 NC = {D2Q9: 9, D3Q15: 15, D2Q5: 5, D3Q7: 7}
 ND = {D2Q9: 2, D3Q15: 3, D2Q5: 2, D3Q7: 3}
+F64, F32 = 0, 1  # population storage (lbm_lattice_create_prec)
 BARRIER, MIRROR = 1, 2
 SET_T, SET_Q = 1, 2
 
@@ -52,6 +53,9 @@ def load():
         "lbm_event_record": [VP, I],
         "lbm_event_elapsed_ms": [VP, I, I, ctypes.POINTER(ctypes.c_float)],
         "lbm_lattice_create": [VP, I, I, I, I, ctypes.POINTER(VP)],
+        "lbm_lattice_create_prec": [VP, I, I, I, I, I, ctypes.POINTER(VP)],
+        "lbm_lattice_precision": [VP],
+        "lbm_lattice_set_precision": [VP, I],
         "lbm_lattice_destroy": [VP],
         "lbm_lattice_nc": [VP],
         "lbm_lattice_nxyz": [VP],
@@ -220,13 +224,14 @@ class DeviceArray:
 class Lattice:
     """Device populations of one lattice (lbm_lattice)."""
 
-    def __init__(self, ctx, kind, lx, ly, lz=1):
+    def __init__(self, ctx, kind, lx, ly, lz=1, prec=None):
         self.ctx, self.kind, self.lx, self.ly = ctx, kind, lx, ly
         self.lz = lz if ND[kind] == 3 else 1
         self.nc = NC[kind]
         self.N = lx * ly * self.lz
+        self.prec = F64 if prec is None else prec
         h = VP()
-        check(ctx.lib.lbm_lattice_create(ctx.h, kind, lx, ly, self.lz, ctypes.byref(h)))
+        check(ctx.lib.lbm_lattice_create_prec(ctx.h, kind, lx, ly, self.lz, self.prec, ctypes.byref(h)))
         self.h = h
 
     def close(self):

@@ -15,8 +15,9 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpanslbm2_hip.so")
 BUILD = os.path.join(HERE, "_build")
-SOURCES = ["collide_ad3.hip", "collide_aad3.hip", "collide_ns3.hip", "collide_ans3.hip", "collide_ad2.hip",
-           "collide_aad2.hip", "collide_ns2.hip", "collide_ans2.hip", "device.hip", "capi.hip"]
+# heaviest first: the 3D coupled collides, then the rest, so the parallel build ends together
+SOURCES = ["collide_%s%s%s.hip" % (eq, nd, p) for nd in ("3", "2") for eq in ("ad", "aad", "ns", "ans")
+           for p in ("", "_f32")] + ["device.hip", "capi.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -ffp-contract=off: no FMA contraction, so FP64 kernels reproduce the reference's plain
 # x86-64 arithmetic.  Packed FP32 instructions are disabled for all device code.
@@ -44,7 +45,7 @@ def build_library(verbose=False):
         _run([HIPCC] + FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
         return obj
 
-    with ThreadPoolExecutor(len(SOURCES)) as ex:
+    with ThreadPoolExecutor(min(len(SOURCES), max(2, os.cpu_count() or 2))) as ex:
         objs = list(ex.map(compile_one, SOURCES))
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)

@@ -72,6 +72,10 @@ struct DevBuf {
         cap = 0;
     }
 };
+// a staging buffer freed at scope exit (the copy through it has completed by then)
+struct StageBuf : DevBuf {
+    ~StageBuf() { release(); }
+};
 
 }  // namespace
 
@@ -91,8 +95,13 @@ struct lbm_context {
 struct lbm_lattice {
     lbm_context *ctx;
     int kind, nc, nd, lx, ly, lz, N;
-    double *buf[2] = {nullptr, nullptr};
+    int prec = LBM_F64;    // population storage
+    // FP32 storage shift (lbm::Enc): 1 = the rest state w_c, set by the NS initial condition; 0 = none
+    int rest = 0;
+    void *buf[2] = {nullptr, nullptr};
     int cur = 0;
+    size_t esize() const { return prec == LBM_F64 ? sizeof(double) : sizeof(float); }
+    lbm::Enc enc() const { return lbm::Enc{prec == LBM_F64 ? 0.0 : double(rest)}; }
     // smoothing lists for the eager path (in-place kernels): edges (3D) then corners
     std::vector<int4> sm_edge, sm_corner;
     int4 *d_sm_edge = nullptr, *d_sm_corner = nullptr;
@@ -207,8 +216,10 @@ int smooth_eager(lbm_lattice *L) {
     if (L->lx < 3 || L->ly < 3 || (L->nd == 3 && L->lz < 3))
         return fail(LBM_ERR_UNSUPPORTED, "SmoothCorner needs at least 3 nodes per axis");
     hipStream_t s = L->ctx->stream;
-    HIPCHK(lbm::launch_smooth(L->kind, L->buf[L->cur], L->N, L->d_sm_edge, (int)L->sm_edge.size(), s));
-    HIPCHK(lbm::launch_smooth(L->kind, L->buf[L->cur], L->N, L->d_sm_corner, (int)L->sm_corner.size(), s));
+    HIPCHK(lbm::launch_smooth(L->kind, L->prec, L->buf[L->cur], L->N, L->d_sm_edge, (int)L->sm_edge.size(), L->enc(),
+                              s));
+    HIPCHK(lbm::launch_smooth(L->kind, L->prec, L->buf[L->cur], L->N, L->d_sm_corner, (int)L->sm_corner.size(),
+                              L->enc(), s));
     return LBM_OK;
 }
 
@@ -422,15 +433,17 @@ int prog_apply(lbm_program *P) {
     for (int k = 0; k < P->nlat; ++k) {
         lbm_lattice *L = P->lat[k];
         if (P->pull[k]) {
-            HIPCHK(lbm::launch_stream(L->kind, L->buf[L->cur], L->buf[L->cur ^ 1], L->grid(), P->pull[k], s));
+            HIPCHK(lbm::launch_stream(L->kind, L->prec, L->buf[L->cur], L->buf[L->cur ^ 1], L->grid(), P->pull[k],
+                                      L->enc(), s));
             swap_buf(L);
         }
     }
     lbm_lattice *A = P->lat[0], *B = P->nlat > 1 ? P->lat[1] : nullptr;
     ProgView v = prog_view(P);
     for (int seg = 0; seg <= P->seg; ++seg) {
-        HIPCHK(lbm::launch_program_apply(A->kind, B ? B->kind : -1, A->buf[A->cur], B ? B->buf[B->cur] : nullptr, A->N,
-                                         v, seg, s));
+        HIPCHK(lbm::launch_program_apply(A->kind, B ? B->kind : -1, A->prec, A->buf[A->cur],
+                                         B ? B->buf[B->cur] : nullptr, A->N, v, seg, A->enc(),
+                                         B ? B->enc() : lbm::Enc{}, s));
         if (seg < (int)P->seg_closer.size()) {
             rc = smooth_eager(P->lat[P->seg_closer[seg]]);
             if (rc) return rc;
@@ -475,11 +488,15 @@ int collide_call(lbm_program *P, lbm_lattice *p, lbm_lattice *q, std::initialize
     if (!p) return fail(LBM_ERR_ARG, "null lattice");
     if (q && (q->lx != p->lx || q->ly != p->ly || q->lz != p->lz))
         return fail(LBM_ERR_ARG, "coupled lattices must have the same grid");
+    if (q && q->prec != p->prec) return fail(LBM_ERR_ARG, "coupled lattices must have the same storage precision");
     lbm_context *C = p->ctx;
     if (C->recording) return fail(LBM_ERR_STATE, "collide called while recording a program");
     lbm::CollideLaunch cl{};
     cl.ka = p->kind;
     cl.kb = q ? q->kind : -1;
+    cl.prec = p->prec;
+    cl.ea = p->enc();
+    cl.eb = q ? q->enc() : lbm::Enc{};
     cl.g = p->grid();
     cl.stream = C->stream;
     bool fused = false;
@@ -508,6 +525,11 @@ int collide_call(lbm_program *P, lbm_lattice *p, lbm_lattice *q, std::initialize
     if ((rc = prog_upload(P))) return rc;
     ProgView v = prog_view(P);
     v.latswap = (P->lat[0] == q) ? 1 : 0;
+    if (P->nlat > 0 && P->lat[0]->prec != p->prec) return fail(LBM_ERR_ARG, "program and collide precisions differ");
+    // Measured on the D2Q9 + D2Q5 fused forward step: keeping a tile's row neighbours in one XCD's L2
+    // wins while the step's populations fit the last-level caches' reach (1448^2: 62 % vs 58 % of
+    // HBM peak), the plain order wins beyond (2048^2: 63 % vs 62 %, 4096^2: 65 % vs 62 %).
+    v.xcd = (size_t)p->N * 2u * p->esize() * (size_t)(p->nc + (q ? q->nc : 0)) <= lbm::kXcdRemapMaxBytes ? 1 : 0;
     v.pull[0] = v.pull[1] = 0;
     for (int k = 0; k < P->nlat; ++k) {
         if (P->lat[k] == p) v.pull[0] = P->pull[k];
@@ -643,8 +665,12 @@ int lbm_event_elapsed_ms(lbm_context *C, int a, int b, float *ms) {
 
 // ---- lattice ------------------------------------------------------------------------------
 int lbm_lattice_create(lbm_context *C, int kind, int lx, int ly, int lz, lbm_lattice **out) {
+    return lbm_lattice_create_prec(C, kind, lx, ly, lz, LBM_F64, out);
+}
+int lbm_lattice_create_prec(lbm_context *C, int kind, int lx, int ly, int lz, int prec, lbm_lattice **out) {
     if (!C || !out) return fail(LBM_ERR_ARG, "null argument");
     *out = nullptr;
+    if (prec != LBM_F64 && prec != LBM_F32) return fail(LBM_ERR_ARG, "unknown storage precision %d", prec);
     int nc = kind_nc(kind);
     if (!nc) return fail(LBM_ERR_ARG, "unknown lattice kind %d", kind);
     int nd = kind_nd(kind);
@@ -661,15 +687,20 @@ int lbm_lattice_create(lbm_context *C, int kind, int lx, int ly, int lz, lbm_lat
     L->ly = ly;
     L->lz = lz;
     L->N = (int)N;
+    L->prec = prec;
     for (int b = 0; b < 2; ++b) {
-        hipError_t e = hipMalloc(&L->buf[b], sizeof(double) * nc * N);
+        hipError_t e = hipMalloc(&L->buf[b], L->esize() * nc * N);
         if (e != hipSuccess) {
             for (int c = 0; c < b; ++c) (void)hipFree(L->buf[c]);
             delete L;
             return fail(LBM_ERR_HIP, "hipMalloc populations: %s", hipGetErrorString(e));
         }
     }
-    HIPCHK(lbm::launch_fill(L->buf[0], (size_t)nc * N, 0.0, C->stream));
+    if (prec == LBM_F64) {
+        HIPCHK(lbm::launch_fill((double *)L->buf[0], (size_t)nc * N, 0.0, C->stream));
+    } else {  // FP32 +0.0 is all-zero bits
+        HIPCHK(hipMemsetAsync(L->buf[0], 0, L->esize() * nc * N, C->stream));
+    }
     build_smooth_lists(L);
     if (!L->sm_edge.empty()) {
         HIPCHK(hipMalloc(&L->d_sm_edge, sizeof(int4) * L->sm_edge.size()));
@@ -695,14 +726,65 @@ int lbm_lattice_destroy(lbm_lattice *L) {
 int lbm_lattice_nc(const lbm_lattice *L) { return L ? L->nc : fail(LBM_ERR_ARG, "null lattice"); }
 int lbm_lattice_nxyz(const lbm_lattice *L) { return L ? L->N : fail(LBM_ERR_ARG, "null lattice"); }
 int lbm_lattice_nface(const lbm_lattice *L) { return L ? nface_total(L) : fail(LBM_ERR_ARG, "null lattice"); }
-double *lbm_lattice_populations(lbm_lattice *L) { return L ? L->buf[L->cur] : nullptr; }
+double *lbm_lattice_populations(lbm_lattice *L) {
+    if (!L) return nullptr;
+    if (L->prec != LBM_F64) {
+        fail(LBM_ERR_ARG, "populations are not stored in FP64");
+        return nullptr;
+    }
+    return (double *)L->buf[L->cur];
+}
+int lbm_lattice_precision(const lbm_lattice *L) { return L ? L->prec : fail(LBM_ERR_ARG, "null lattice"); }
+// Change the storage precision in place: both buffers are re-allocated and the current populations
+// converted (FP64 <- FP32 is exact; FP32 <- FP64 rounds each stored deviation once).  Programs and
+// graphs that hold the old buffers' addresses must be recorded again.
+int lbm_lattice_set_precision(lbm_lattice *L, int prec) {
+    if (!L) return fail(LBM_ERR_ARG, "null lattice");
+    if (prec != LBM_F64 && prec != LBM_F32) return fail(LBM_ERR_ARG, "unknown storage precision %d", prec);
+    if (L->ctx->recording) return fail(LBM_ERR_STATE, "precision change while recording a program");
+    if (prec == L->prec) return LBM_OK;
+    lbm_lattice T = *L;  // the new storage
+    T.prec = prec;
+    T.buf[0] = T.buf[1] = nullptr;
+    size_t bytes = T.esize() * L->nc * L->N;
+    for (int b = 0; b < 2; ++b) {
+        if (hipMalloc(&T.buf[b], bytes) != hipSuccess) {
+            if (T.buf[0]) (void)hipFree(T.buf[0]);
+            return fail(LBM_ERR_HIP, "hipMalloc populations failed");
+        }
+    }
+    hipStream_t s = L->ctx->stream;
+    HIPCHK(lbm::launch_convert(L->kind, L->prec, L->buf[L->cur], L->enc(), prec, T.buf[L->cur], T.enc(), L->N, s));
+    HIPCHK(hipStreamSynchronize(s));
+    (void)hipFree(L->buf[0]);
+    (void)hipFree(L->buf[1]);
+    L->prec = prec;
+    L->buf[0] = T.buf[0];
+    L->buf[1] = T.buf[1];
+    return LBM_OK;
+}
+// FP32 storage: the host's FP64 populations pass through a device staging buffer and the
+// encode / decode kernels (stored value = f - E_c, lbm::Enc).
 int lbm_lattice_upload(lbm_lattice *L, const double *soa) {
     if (!L || !soa) return fail(LBM_ERR_ARG, "null argument");
-    return lbm_copy_to_device(L->ctx, L->buf[L->cur], soa, sizeof(double) * L->nc * L->N);
+    size_t bytes = sizeof(double) * L->nc * L->N;
+    if (L->prec == LBM_F64) return lbm_copy_to_device(L->ctx, L->buf[L->cur], soa, bytes);
+    StageBuf stage;
+    if (stage.ensure(bytes)) return fail(LBM_ERR_HIP, "staging buffer allocation failed");
+    int rc = lbm_copy_to_device(L->ctx, stage.p, soa, bytes);
+    if (rc) return rc;
+    HIPCHK(lbm::launch_encode(L->kind, L->prec, (const double *)stage.p, L->buf[L->cur], L->N, L->enc(), L->ctx->stream));
+    HIPCHK(hipStreamSynchronize(L->ctx->stream));
+    return LBM_OK;
 }
 int lbm_lattice_download(lbm_lattice *L, double *soa) {
     if (!L || !soa) return fail(LBM_ERR_ARG, "null argument");
-    return lbm_copy_to_host(L->ctx, soa, L->buf[L->cur], sizeof(double) * L->nc * L->N);
+    size_t bytes = sizeof(double) * L->nc * L->N;
+    if (L->prec == LBM_F64) return lbm_copy_to_host(L->ctx, soa, L->buf[L->cur], bytes);
+    StageBuf stage;
+    if (stage.ensure(bytes)) return fail(LBM_ERR_HIP, "staging buffer allocation failed");
+    HIPCHK(lbm::launch_decode(L->kind, L->prec, L->buf[L->cur], (double *)stage.p, L->N, L->enc(), L->ctx->stream));
+    return lbm_copy_to_host(L->ctx, soa, stage.p, bytes);
 }
 int lbm_lattice_upload_aos(lbm_lattice *L, const double *f0, const double *f) {
     if (!L || !f0 || !f) return fail(LBM_ERR_ARG, "null argument");
@@ -739,7 +821,8 @@ static int stream_call(lbm_lattice *L, int pull) {
         P->pull[slot] = pull;
         return LBM_OK;
     }
-    HIPCHK(lbm::launch_stream(L->kind, L->buf[L->cur], L->buf[L->cur ^ 1], L->grid(), pull, C->stream));
+    HIPCHK(lbm::launch_stream(L->kind, L->prec, L->buf[L->cur], L->buf[L->cur ^ 1], L->grid(), pull, L->enc(),
+                              C->stream));
     swap_buf(L);
     return LBM_OK;
 }
@@ -779,7 +862,9 @@ static int unsupported_3d(const lbm_lattice *L, const char *what) {
 int lbm_ns_initial_condition(lbm_lattice *p, const double *rho, const double *ux, const double *uy, const double *uz) {
     if (!ns_lattice(p) || !fields_ok({rho, ux, uy}) || (p->nd == 3 && !uz)) return fail(LBM_ERR_ARG, "bad arguments");
     const double *a[7] = {rho, ux, uy, uz, nullptr, nullptr, nullptr};
-    return hip_result(lbm::launch_init(p->kind, p->buf[p->cur], p->N, 0, a, p->ctx->stream), "ns init");
+    p->rest = 1;  // NS populations sit near the rest state w_c
+    return hip_result(lbm::launch_init(p->kind, p->prec, p->buf[p->cur], p->N, 0, a, p->enc(), p->ctx->stream),
+                      "ns init");
 }
 int lbm_ns_macro_collide_p(lbm_program *P, lbm_lattice *p, double *rho, double *ux, double *uy, double *uz, double nu,
                            int issave) {
@@ -831,7 +916,9 @@ int lbm_ns_set_rho(lbm_lattice *p, const int *mask, const double *rho, const dou
 int lbm_ad_initial_condition(lbm_lattice *q, const double *tem, const double *ux, const double *uy, const double *uz) {
     if (!ad_lattice(q) || !fields_ok({tem, ux, uy}) || (q->nd == 3 && !uz)) return fail(LBM_ERR_ARG, "bad arguments");
     const double *a[7] = {tem, ux, uy, uz, nullptr, nullptr, nullptr};
-    return hip_result(lbm::launch_init(q->kind, q->buf[q->cur], q->N, 1, a, q->ctx->stream), "ad init");
+    q->rest = 0;
+    return hip_result(lbm::launch_init(q->kind, q->prec, q->buf[q->cur], q->N, 1, a, q->enc(), q->ctx->stream),
+                      "ad init");
 }
 int lbm_ad_macro_brinkman_collide_force_convection_p(lbm_program *P, lbm_lattice *p, double *rho, double *ux,
                                                      double *uy, double *uz, const double *alpha, double nu,
@@ -904,7 +991,9 @@ int lbm_ans_initial_condition(lbm_lattice *p, const double *ux, const double *uy
     if (!ns_lattice(p) || !fields_ok({ux, uy, ip, iux, iuy}) || (p->nd == 3 && (!uz || !iuz)))
         return fail(LBM_ERR_ARG, "bad arguments");
     const double *a[7] = {ux, uy, uz, ip, iux, iuy, iuz};
-    return hip_result(lbm::launch_init(p->kind, p->buf[p->cur], p->N, 2, a, p->ctx->stream), "ans init");
+    p->rest = 0;
+    return hip_result(lbm::launch_init(p->kind, p->prec, p->buf[p->cur], p->N, 2, a, p->enc(), p->ctx->stream),
+                      "ans init");
 }
 int lbm_ans_macro_brinkman_collide_p(lbm_program *P, lbm_lattice *p, const double *rho, const double *ux,
                                      const double *uy, const double *uz, double *ip, double *iux, double *iuy,
@@ -951,7 +1040,9 @@ int lbm_aad_initial_condition(lbm_lattice *q, const double *ux, const double *uy
     if (!q || !fields_ok({ux, uy, item, iqx, iqy}) || (q->nd == 3 && (!uz || !iqz)))
         return fail(LBM_ERR_ARG, "bad arguments");
     const double *a[7] = {ux, uy, uz, item, iqx, iqy, iqz};
-    return hip_result(lbm::launch_init(q->kind, q->buf[q->cur], q->N, 3, a, q->ctx->stream), "aad init");
+    q->rest = 0;
+    return hip_result(lbm::launch_init(q->kind, q->prec, q->buf[q->cur], q->N, 3, a, q->enc(), q->ctx->stream),
+                      "aad init");
 }
 int lbm_aad_macro_brinkman_collide_force_convection_p(lbm_program *P, lbm_lattice *p, const double *rho,
                                                       const double *ux, const double *uy, const double *uz, double *ip,

@@ -3,8 +3,9 @@
 //
 // Memory-bound stencil kernels: one thread per lattice node, populations in plane order so each
 // plane access is a coalesced 64-lane stream; no LDS and no MFMA (nothing here is GEMM-shaped).
-// Launch shapes: 256-thread workgroups; the fused kernel remaps workgroups so that consecutive
-// node tiles stay on one XCD (xcd_tile).
+// Launch shapes: 256-thread workgroups; on grids whose step moves up to kXcdRemapMaxBytes the fused
+// kernel remaps workgroups so that consecutive node tiles stay on one XCD (xcd_tile).  Storage S is
+// double or float (FP32 values are decoded to FP64 on load, Enc); the collide arithmetic is FP64.
 #pragma once
 #include "device.h"
 #include "kernels.h"
@@ -22,13 +23,16 @@ inline int nblocks(long n, int b = kBlock) { return (int)((n + b - 1) / b); }
 // in the reference signatures.
 // ------------------------------------------------------------------------------------------
 // NS::MacroCollide / NS::MacroBrinkmanCollide, navierstokes.h:432-546
+template <class LA, class LB, class R>
+struct NSBody;
 template <class LA, class LB>
-struct NSBody {
+struct NSBody<LA, LB, double> {
     double *rho, *ux, *uy, *uz;
     const double *alpha;
     double omega, iomega;
-    int save, brinkman;
-    __device__ __forceinline__ void operator()(double *f, double *, int n, bool bnd) const {
+    int save, brinkman, N;
+    Enc ea, eb;
+    __device__ __forceinline__ void operator()(double *f, double *, int n, bool bnd, const Sh4 &, const Sh4 &) const {
         double r, vx, vy, vz, feq[LA::nc];
         ns_macro<LA>(f, r, vx, vy, vz);
         if (brinkman) {
@@ -48,14 +52,17 @@ struct NSBody {
 
 // AD::MacroBrinkmanCollideForceConvection, advection.h:814-926 (+ optional volumetric source), and
 // AD::MacroBrinkmanCollideNaturalConvection, advection.h:930-1044 (natural = 1)
+template <class LA, class LB, class R>
+struct ADBody;
 template <class LA, class LB>
-struct ADBody {
+struct ADBody<LA, LB, double> {
     double *rho, *ux, *uy, *uz, *tem, *qx, *qy, *qz, *gsnap;
     const double *alpha, *diff, *heat;
     double omegaf, iomegaf;
     int save, N, natural;
     double gx, gy, gz, tem0;
-    __device__ __forceinline__ void operator()(double *f, double *g, int n, bool bnd) const {
+    Enc ea, eb;
+    __device__ __forceinline__ void operator()(double *f, double *g, int n, bool bnd, const Sh4 &, const Sh4 &) const {
         double omegag = 1.0 / (3.0 * diff[n] + 0.5), iomegag = 1.0 - omegag;
         double r, vx, vy, vz, T, hx, hy, hz;
         ns_macro<LA>(f, r, vx, vy, vz);
@@ -104,13 +111,16 @@ struct ADBody {
 };
 
 // ANS::MacroBrinkmanCollide, adjointnavierstokes.h:398-470
+template <class LA, class LB, class R>
+struct ANSBody;
 template <class LA, class LB>
-struct ANSBody {
+struct ANSBody<LA, LB, double> {
     const double *rho, *ux, *uy, *uz, *alpha;
     double *ip, *iux, *iuy, *iuz, *imx, *imy, *imz;
     double omega, iomega;
-    int save;
-    __device__ __forceinline__ void operator()(double *f, double *, int n, bool bnd) const {
+    int save, N;
+    Enc ea, eb;
+    __device__ __forceinline__ void operator()(double *f, double *, int n, bool bnd, const Sh4 &, const Sh4 &) const {
         double vx = ux[n], vy = uy[n], vz = LA::nd == 3 ? uz[n] : 0.0;
         double a, bx, by, bz, mx, my, mz, feq[LA::nc];
         ans_macro<LA>(f, vx, vy, vz, a, bx, by, bz, mx, my, mz);
@@ -134,14 +144,17 @@ struct ANSBody {
 
 // AAD::MacroBrinkmanCollideForceConvection, adjointadvection.h:997-1109, and
 // AAD::MacroBrinkmanCollideNaturalConvection, adjointadvection.h:1113-1230 (natural = 1)
+template <class LA, class LB, class R>
+struct AADBody;
 template <class LA, class LB>
-struct AADBody {
+struct AADBody<LA, LB, double> {
     const double *rho, *ux, *uy, *uz, *alpha, *tem, *diff;
     double *ip, *iux, *iuy, *iuz, *imx, *imy, *imz, *item, *iqx, *iqy, *iqz, *igsnap;
     double omegaf, iomegaf;
     int save, N, natural;
     double gx, gy, gz;
-    __device__ __forceinline__ void operator()(double *f, double *g, int n, bool bnd) const {
+    Enc ea, eb;
+    __device__ __forceinline__ void operator()(double *f, double *g, int n, bool bnd, const Sh4 &, const Sh4 &) const {
         double omegag = 1.0 / (3.0 * diff[n] + 0.5), iomegag = 1.0 - omegag;
         double vx = ux[n], vy = uy[n], vz = LA::nd == 3 ? uz[n] : 0.0;
         double a, bx, by, bz, mx, my, mz, it, hx, hy, hz;
@@ -186,30 +199,294 @@ struct AADBody {
 };
 
 // ------------------------------------------------------------------------------------------
-// Collide kernels.  In place: the reference's own collide (no pending stream / boundaries).
-// Fused: pull + boundary program + collide, reading `s*` and writing `d*`; nodes owned by the
-// smoothing fix-up kernels are skipped.  Fix-up: the smoothed nodes (2D corners; 3D edges and
-// corners), whose state averages neighbours' post-boundary states.
+// FP32 bodies.  Registers hold d_c = f_c - E_c, the deviations from the lattice's shift
+// E_c = w_c (a + 3 c.b) given as its moments o = (a, b) (the rest state (1, 0), or none), and the
+// forward sums run on deviations:
+//   NS: rho = a + sum d, j = b + sum c d, u = j / rho, and
+//     feq_c - E_c = w_c (drho + 3 c.dj + rho (4.5 (c.u)^2 - 1.5 u.u)), drho = sum d, dj = sum c d;
+//   AD: T = a + dT, J = b + dJ, q = (1 - omega/2) (dJ + (b - a u) - dT u) with b - a u formed by
+//     one fused multiply-add, and geq_c - E_c = w_c (dT + 3 c.(dT u - (b - a u)));
+//   the Brinkman, buoyancy and source terms are differences already.
+// The adjoints' populations carry no shift after their initial conditions; their bodies add any
+// shift back (exact when it is zero) and use the FP64 formulas in FP32.
 // ------------------------------------------------------------------------------------------
-template <class LA, class LB, class Body>
-__global__ void __launch_bounds__(kBlock) k_collide_inplace(double *a, double *b, Grid g, Body body) {
+template <class L>
+__device__ __forceinline__ void ns_macro_d(const float *d, const Sh4 &o, float &dr, float &djx, float &djy,
+                                           float &djz, float &rho, float &ux, float &uy, float &uz) {
+    dr = d[0];
+    djx = 0.0f;
+    djy = 0.0f;
+    djz = 0.0f;
+    sfor<1, L::nc>([&](auto c) {
+        constexpr int C = decltype(c)::value;
+        dr += d[C];
+        djx = acc<L::cx[C]>(djx, d[C]);
+        djy = acc<L::cy[C]>(djy, d[C]);
+        if constexpr (L::nd == 3) djz = acc<L::cz[C]>(djz, d[C]);
+    });
+    rho = o.a + dr;
+    float inv = 1.0f / rho;
+    ux = (o.bx + djx) * inv;
+    uy = (o.by + djy) * inv;
+    uz = L::nd == 3 ? (o.bz + djz) * inv : 0.0f;
+}
+template <class L>
+__device__ __forceinline__ void ns_eq_d(float *deq, float dr, float djx, float djy, float djz, float rho, float ux,
+                                        float uy, float uz) {
+    float uu = -1.5f * (ux * ux + uy * uy + (L::nd == 3 ? uz * uz : 0.0f));
+    sfor<0, L::nc>([&](auto c) {
+        constexpr int C = decltype(c)::value;
+        float ciu = cdot<L, C>(ux, uy, uz);
+        float cdj = cdot<L, C>(djx, djy, djz);
+        deq[C] = float(L::w[C]) * (dr + 3.0f * cdj + rho * (4.5f * ciu * ciu + uu));
+    });
+}
+// (b - a u) per axis and the AD macroscopic values
+template <class L>
+__device__ __forceinline__ void ad_macro_d(const float *d, const Sh4 &o, float ux, float uy, float uz, float omegag,
+                                           float &dT, float &T, float &qx, float &qy, float &qz, float &px, float &py,
+                                           float &pz) {
+    float jx = 0.0f, jy = 0.0f, jz = 0.0f;
+    dT = d[0];
+    sfor<1, L::nc>([&](auto c) {
+        constexpr int C = decltype(c)::value;
+        dT += d[C];
+        jx = acc<L::cx[C]>(jx, d[C]);
+        jy = acc<L::cy[C]>(jy, d[C]);
+        if constexpr (L::nd == 3) jz = acc<L::cz[C]>(jz, d[C]);
+    });
+    T = o.a + dT;
+    px = __builtin_fmaf(-o.a, ux, o.bx);
+    py = __builtin_fmaf(-o.a, uy, o.by);
+    pz = L::nd == 3 ? __builtin_fmaf(-o.a, uz, o.bz) : 0.0f;
+    float coef = 1.0f - 0.5f * omegag;
+    qx = coef * (jx + px - dT * ux);
+    qy = coef * (jy + py - dT * uy);
+    qz = L::nd == 3 ? coef * (jz + pz - dT * uz) : 0.0f;
+}
+template <class L>
+__device__ __forceinline__ void ad_eq_d(float *deq, float dT, float ux, float uy, float uz, float px, float py,
+                                        float pz) {
+    float ex = dT * ux - px, ey = dT * uy - py, ez = dT * uz - pz;
+    sfor<0, L::nc>([&](auto c) {
+        constexpr int C = decltype(c)::value;
+        deq[C] = float(L::w[C]) * (dT + 3.0f * cdot<L, C>(ex, ey, ez));
+    });
+}
+template <class L>
+__device__ __forceinline__ void add_shift_f(float *v, const Enc &e, const Sh4 &o, float sign) {
+    sfor<0, L::nc>([&](auto c) {
+        constexpr int C = decltype(c)::value;
+        v[C] += sign * float(shift_c<L, C>(e));
+    });
+}
+// FP64 snapshot of populations held as deviations
+template <class L>
+__device__ __forceinline__ void store_snapshot(double *dst, int N, int n, const float *d, const Enc &e, const Sh4 &o) {
+    sfor<0, L::nc>([&](auto c) {
+        constexpr int C = decltype(c)::value;
+        dst[(size_t)C * N + n] = double(d[C]) + shift_c<L, C>(e);
+    });
+}
+
+template <class LA, class LB>
+struct NSBody<LA, LB, float> {
+    double *rho, *ux, *uy, *uz;
+    const double *alpha;
+    float omega, iomega;
+    int save, brinkman, N;
+    Enc ea, eb;
+    __device__ __forceinline__ void operator()(float *f, float *, int n, bool bnd, const Sh4 &oa, const Sh4 &) const {
+        float dr, jx, jy, jz, r, vx, vy, vz, feq[LA::nc];
+        ns_macro_d<LA>(f, oa, dr, jx, jy, jz, r, vx, vy, vz);
+        if (brinkman) {
+            ns_brinkman<LA>(f, r, vx, vy, vz, float(alpha[n]));
+            ns_macro_d<LA>(f, oa, dr, jx, jy, jz, r, vx, vy, vz);
+        }
+        if (save == 1 || (save == 2 && bnd)) {
+            rho[n] = r;
+            ux[n] = vx;
+            uy[n] = vy;
+            if constexpr (LA::nd == 3) uz[n] = vz;
+        }
+        ns_eq_d<LA>(feq, dr, jx, jy, jz, r, vx, vy, vz);
+        bgk<LA>(f, feq, omega, iomega);
+    }
+};
+
+template <class LA, class LB>
+struct ADBody<LA, LB, float> {
+    double *rho, *ux, *uy, *uz, *tem, *qx, *qy, *qz, *gsnap;
+    const double *alpha, *diff, *heat;
+    float omegaf, iomegaf;
+    int save, N, natural;
+    float gx, gy, gz, tem0;
+    Enc ea, eb;
+    __device__ __forceinline__ void operator()(float *f, float *g, int n, bool bnd, const Sh4 &oa,
+                                               const Sh4 &ob) const {
+        float omegag = 1.0f / (3.0f * float(diff[n]) + 0.5f), iomegag = 1.0f - omegag;
+        float dr, jx, jy, jz, r, vx, vy, vz, dT, T, hx, hy, hz, px, py, pz;
+        ns_macro_d<LA>(f, oa, dr, jx, jy, jz, r, vx, vy, vz);
+        ad_macro_d<LB>(g, ob, vx, vy, vz, omegag, dT, T, hx, hy, hz, px, py, pz);
+        if (natural) {  // ExternalForceNaturalConvection, advection.h:73-87
+            const float dTb = T - tem0;
+            sfor<1, LA::nc>([&](auto c) {
+                constexpr int C = decltype(c)::value;
+                f[C] += 3.0f * float(LA::w[C]) * cdot<LA, C>(gx, gy, gz) * dTb;
+            });
+        }
+        ns_brinkman<LA>(f, r, vx, vy, vz, float(alpha[n]));
+        if (heat) {
+            float h = float(heat[n]);
+            sfor<0, LB::nc>([&](auto c) {
+                constexpr int C = decltype(c)::value;
+                g[C] += float(LB::w[C]) * h;
+            });
+        }
+        ns_macro_d<LA>(f, oa, dr, jx, jy, jz, r, vx, vy, vz);
+        ad_macro_d<LB>(g, ob, vx, vy, vz, omegag, dT, T, hx, hy, hz, px, py, pz);
+        if (save == 1 || (save == 2 && bnd)) {
+            rho[n] = r;
+            ux[n] = vx;
+            uy[n] = vy;
+            tem[n] = T;
+            qx[n] = hx;
+            qy[n] = hy;
+            if constexpr (LA::nd == 3) {
+                uz[n] = vz;
+                qz[n] = hz;
+            }
+            if (gsnap && save == 1) store_snapshot<LB>(gsnap, N, n, g, eb, ob);
+        }
+        {
+            float feq[LA::nc];
+            ns_eq_d<LA>(feq, dr, jx, jy, jz, r, vx, vy, vz);
+            bgk<LA>(f, feq, omegaf, iomegaf);
+        }
+        {
+            float geq[LB::nc];
+            ad_eq_d<LB>(geq, dT, vx, vy, vz, px, py, pz);
+            bgk<LB>(g, geq, omegag, iomegag);
+        }
+    }
+};
+
+template <class LA, class LB>
+struct ANSBody<LA, LB, float> {
+    const double *rho, *ux, *uy, *uz, *alpha;
+    double *ip, *iux, *iuy, *iuz, *imx, *imy, *imz;
+    float omega, iomega;
+    int save, N;
+    Enc ea, eb;
+    __device__ __forceinline__ void operator()(float *f, float *, int n, bool bnd, const Sh4 &oa, const Sh4 &) const {
+        float vx = float(ux[n]), vy = float(uy[n]), vz = LA::nd == 3 ? float(uz[n]) : 0.0f;
+        float a, bx, by, bz, mx, my, mz, feq[LA::nc];
+        add_shift_f<LA>(f, ea, oa, 1.0f);
+        ans_macro<LA>(f, vx, vy, vz, a, bx, by, bz, mx, my, mz);
+        ans_brinkman<LA>(f, float(rho[n]), vx, vy, vz, mx, my, mz, float(alpha[n]));
+        ans_macro<LA>(f, vx, vy, vz, a, bx, by, bz, mx, my, mz);
+        if (save == 1 || (save == 2 && bnd)) {
+            ip[n] = a;
+            iux[n] = bx;
+            iuy[n] = by;
+            imx[n] = mx;
+            imy[n] = my;
+            if constexpr (LA::nd == 3) {
+                iuz[n] = bz;
+                imz[n] = mz;
+            }
+        }
+        ans_eq<LA>(feq, vx, vy, vz, a, bx, by, bz);
+        bgk<LA>(f, feq, omega, iomega);
+        add_shift_f<LA>(f, ea, oa, -1.0f);
+    }
+};
+
+template <class LA, class LB>
+struct AADBody<LA, LB, float> {
+    const double *rho, *ux, *uy, *uz, *alpha, *tem, *diff;
+    double *ip, *iux, *iuy, *iuz, *imx, *imy, *imz, *item, *iqx, *iqy, *iqz, *igsnap;
+    float omegaf, iomegaf;
+    int save, N, natural;
+    float gx, gy, gz;
+    Enc ea, eb;
+    __device__ __forceinline__ void operator()(float *f, float *g, int n, bool bnd, const Sh4 &oa,
+                                               const Sh4 &ob) const {
+        float omegag = 1.0f / (3.0f * float(diff[n]) + 0.5f), iomegag = 1.0f - omegag;
+        float vx = float(ux[n]), vy = float(uy[n]), vz = LA::nd == 3 ? float(uz[n]) : 0.0f;
+        float a, bx, by, bz, mx, my, mz, it, hx, hy, hz;
+        add_shift_f<LA>(f, ea, oa, 1.0f);
+        add_shift_f<LB>(g, eb, ob, 1.0f);
+        ans_macro<LA>(f, vx, vy, vz, a, bx, by, bz, mx, my, mz);
+        aad_macro<LB>(g, it, hx, hy, hz);
+        aad_brinkman<LA>(f, float(rho[n]), vx, vy, vz, mx, my, mz, float(tem[n]), hx, hy, hz, omegag, float(alpha[n]));
+        ans_macro<LA>(f, vx, vy, vz, a, bx, by, bz, mx, my, mz);
+        if (natural) {
+            float coef;
+            if constexpr (LA::nd == 2) coef = 3.0f * (mx * gx + my * gy);
+            else coef = 3.0f * (mx * gx + my * gy + mz * gz);
+            sfor<0, LB::nc>([&](auto c) { g[decltype(c)::value] += coef; });
+            aad_macro<LB>(g, it, hx, hy, hz);
+        }
+        if (save == 1 || (save == 2 && bnd)) {
+            ip[n] = a;
+            iux[n] = bx;
+            iuy[n] = by;
+            imx[n] = mx;
+            imy[n] = my;
+            item[n] = it;
+            iqx[n] = hx;
+            iqy[n] = hy;
+            if constexpr (LA::nd == 3) {
+                if (natural) iuz[n] = bz;
+                imz[n] = mz;
+                iqz[n] = hz;
+            }
+            if (igsnap && save == 1) store_snapshot<LB>(igsnap, N, n, g, Enc{0.0}, Sh4{});
+        }
+        {
+            float feq[LA::nc];
+            ans_eq<LA>(feq, vx, vy, vz, a, bx, by, bz);
+            bgk<LA>(f, feq, omegaf, iomegaf);
+        }
+        float ge = aad_eq<LB>(it, hx, hy, hz, vx, vy, vz);
+        sfor<0, LB::nc>([&](auto c) {
+            constexpr int C = decltype(c)::value;
+            g[C] = iomegag * g[C] + omegag * ge;
+        });
+        add_shift_f<LA>(f, ea, oa, -1.0f);
+        add_shift_f<LB>(g, eb, ob, -1.0f);
+    }
+};
+
+// ------------------------------------------------------------------------------------------
+// Collide kernels.  In place: the reference's own collide (no pending stream / boundaries).
+// Fused: pull + boundary program + collide, reading `s*` and writing `d*`.  2D smoothed corners:
+// by the fused grid's last workgroup, or after the pass by k_collide_corner2d from the neighbour
+// states the pass saved.
+// ------------------------------------------------------------------------------------------
+template <class LA, class LB, class Body, class S>
+__global__ void __launch_bounds__(kBlock) k_collide_inplace(S *a, S *b, Grid g, Body body) {
+    using R = RegOf<S>;
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     int N = g.N;
     if (n >= N) return;
     int i, j, k;
     coords(n, g, i, j, k);
-    double fa[LA::nc], fb[LB::nc > 0 ? LB::nc : 1];
-    load_node<LA>(a, N, n, fa);
-    load_node<LB>(b, N, n, fb);
-    body(fa, fb, n, nboundary(i, j, k, g, LA::nd) > 0);
-    store_node<LA>(a, N, n, fa);
-    store_node<LB>(b, N, n, fb);
+    R fa[LA::nc], fb[LB::nc > 0 ? LB::nc : 1];
+    const Sh4 oa = own_shift<S>(body.ea), ob = own_shift<S>(body.eb);
+    load_raw<LA>(a, N, n, fa);
+    load_raw<LB>(b, N, n, fb);
+    body(fa, fb, n, nboundary(i, j, k, g, LA::nd) > 0, oa, ob);
+    store_raw<LA>(a, N, n, fa);
+    store_raw<LB>(b, N, n, fb);
 }
 
-template <class LA, class LB, class Body>
-__global__ void __launch_bounds__(kBlock) k_collide_fused(const double *__restrict__ sa, const double *__restrict__ sb,
-                                                          double *__restrict__ da, double *__restrict__ db, Grid g,
-                                                          ProgView pv, int any_smooth, Body body) {
+template <class LA, class LB, class Body, class S>
+__global__ void __launch_bounds__(kBlock) k_collide_fused(const S *__restrict__ sa, const S *__restrict__ sb,
+                                                          S *__restrict__ da, S *__restrict__ db, Grid g, ProgView pv,
+                                                          int any_smooth, Body body) {
+    using R = RegOf<S>;
     constexpr int NB = LB::nc > 0 ? LB::nc : 1;
     // the corner workgroup (last in the grid when pv.corner_block): lanes [0, nside) build the
     // corners' inward neighbours' states, lanes [nside, nside + ncorner) the corners
@@ -220,7 +497,7 @@ __global__ void __launch_bounds__(kBlock) k_collide_fused(const double *__restri
         n = t < pv.nside ? pv.side_nodes[t] : (t < pv.nside + pv.ncorner ? pv.corners[t - pv.nside].x : -1);
         nb = 2;
     } else {
-        int tile = xcd_tile(blockIdx.x, gridDim.x - pv.corner_block);
+        int tile = pv.xcd ? xcd_tile(blockIdx.x, gridDim.x - pv.corner_block) : (int)blockIdx.x;
         n = tile * blockDim.x + threadIdx.x;
         if (n >= g.N) return;
         int i, j, k;
@@ -230,61 +507,68 @@ __global__ void __launch_bounds__(kBlock) k_collide_fused(const double *__restri
         if (pv.side_of_slot != nullptr && nb == 1)
             side = pv.side_of_slot[boundary_slot(i, j, k, g.nx, g.ny, g.nz, LA::nd)];
     }
-    double fa[LA::nc], fb[NB];
-    if (n >= 0) node_state<LA, LB>(sa, sb, g, pv, n, fa, fb);
+    R fa[LA::nc], fb[NB];
+    Sh4 oa{}, ob{};
+    if (n >= 0) {
+        oa = own_shift<S>(body.ea);
+        ob = own_shift<S>(body.eb);
+        node_state<LA, LB>(sa, sb, g, pv, n, fa, fb, body.ea, body.eb, oa, ob);
+    }
     if (cblk) {
-        __shared__ double sst[kMaxSide * kSideStride];
+        __shared__ double sst[kMaxSide * kSideStride];  // FP64 populations
         const int t = threadIdx.x;
         if (t < pv.nside) {
-            sfor<0, LA::nc>([&](auto c) { sst[t * kSideStride + decltype(c)::value] = fa[decltype(c)::value]; });
-            if constexpr (LB::nc > 0)
-                sfor<0, LB::nc>([&](auto c) { sst[t * kSideStride + LA::nc + decltype(c)::value] = fb[decltype(c)::value]; });
+            put_state<LA>(sst + t * kSideStride, fa, body.ea, oa);
+            if constexpr (LB::nc > 0) put_state<LB>(sst + t * kSideStride + LA::nc, fb, body.eb, ob);
         }
         __syncthreads();
         if (t < pv.nside || t >= pv.nside + pv.ncorner) return;
         // d2q9.h:577-587: a smoothed lattice's corner state is the mean of its two neighbours' states
         const int4 e = pv.corners[t - pv.nside];
         const bool smA = (e.w >> (0 ^ pv.latswap)) & 1, smB = (e.w >> (1 ^ pv.latswap)) & 1;
-        if (smA)
-            sfor<0, LA::nc>([&](auto c) {
-                constexpr int C = decltype(c)::value;
-                fa[C] = 0.5 * (sst[e.y * kSideStride + C] + sst[e.z * kSideStride + C]);
-            });
+        if (smA) mean_state<LA>(fa, sst + e.y * kSideStride, sst + e.z * kSideStride, body.ea, oa);
         if constexpr (LB::nc > 0) {
-            if (smB)
-                sfor<0, LB::nc>([&](auto c) {
-                    constexpr int C = decltype(c)::value;
-                    fb[C] = 0.5 * (sst[e.y * kSideStride + LA::nc + C] + sst[e.z * kSideStride + LA::nc + C]);
-                });
+            if (smB) mean_state<LB>(fb, sst + e.y * kSideStride + LA::nc, sst + e.z * kSideStride + LA::nc, body.eb, ob);
         }
     } else if (side >= 0) {  // a smoothed corner's neighbour: keep its post-boundary state for the corner
         double *sp = pv.side + (size_t)side * kSideStride;
-        sfor<0, LA::nc>([&](auto c) { sp[decltype(c)::value] = fa[decltype(c)::value]; });
-        if constexpr (LB::nc > 0) sfor<0, LB::nc>([&](auto c) { sp[LA::nc + decltype(c)::value] = fb[decltype(c)::value]; });
+        put_state<LA>(sp, fa, body.ea, oa);
+        if constexpr (LB::nc > 0) put_state<LB>(sp + LA::nc, fb, body.eb, ob);
     }
-    body(fa, fb, n, nb > 0);
-    store_node<LA>(da, g.N, n, fa);
-    store_node<LB>(db, g.N, n, fb);
+    body(fa, fb, n, nb > 0, oa, ob);
+    store_raw<LA>(da, g.N, n, fa);
+    store_raw<LB>(db, g.N, n, fb);
 }
 
 // 2D smoothed corners after the fused pass: pull, the corner's own operators in record order, a
 // SMOOTH operator averaging the two inward neighbours' post-boundary states that the fused pass
 // saved (d2q9.h:577-587), then the collide.  The corner's fields are untouched by the fused pass,
-// which skips corner nodes.
-template <class LA, class LB, class Body>
-__global__ void __launch_bounds__(kBlock) k_collide_corner2d(const double *__restrict__ sa,
-                                                             const double *__restrict__ sb, double *__restrict__ da,
-                                                             double *__restrict__ db, Grid g, ProgView pv,
-                                                             const int4 *__restrict__ corners, int count, Body body) {
+// which skips corner nodes.  The operators run on FP64 populations.
+template <class LA, class LB, class Body, class S>
+__global__ void __launch_bounds__(kBlock) k_collide_corner2d(const S *__restrict__ sa, const S *__restrict__ sb,
+                                                             S *__restrict__ da, S *__restrict__ db, Grid g,
+                                                             ProgView pv, const int4 *__restrict__ corners, int count,
+                                                             Body body) {
+    using R = RegOf<S>;
+    constexpr int NB = LB::nc > 0 ? LB::nc : 1;
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= count) return;
     int4 e = corners[t];
     int n = e.x;
     int i, j, k;
     coords(n, g, i, j, k);
-    double fa[LA::nc], fb[LB::nc > 0 ? LB::nc : 1];
-    load_pull<LA>(sa, g, i, j, k, n, pv.pull[0], fa);
-    load_pull<LB>(sb, g, i, j, k, n, pv.pull[1], fb);
+    const Sh4 oa = own_shift<S>(body.ea), ob = own_shift<S>(body.eb);
+    R fa[LA::nc], fb[NB];
+    double xa[LA::nc], xb[NB];
+    if constexpr (std::is_same_v<S, double>) {
+        load_pull<LA>(sa, g, i, j, k, n, pv.pull[0], xa);
+        load_pull<LB>(sb, g, i, j, k, n, pv.pull[1], xb);
+    } else {
+        load_pull<LA>(sa, g, i, j, k, n, pv.pull[0], fa);
+        load_pull<LB>(sb, g, i, j, k, n, pv.pull[1], fb);
+        put_state<LA>(xa, fa, body.ea, oa);
+        if constexpr (LB::nc > 0) put_state<LB>(xb, fb, body.eb, ob);
+    }
     const double *s0 = pv.side + (size_t)e.y * kSideStride, *s1 = pv.side + (size_t)e.z * kSideStride;
     int slot = boundary_slot(i, j, k, g.nx, g.ny, g.nz, LA::nd);
     const Op *p = pv.sops + (size_t)slot * pv.maxops;
@@ -296,23 +580,25 @@ __global__ void __launch_bounds__(kBlock) k_collide_corner2d(const double *__res
             const Op &op = o[decltype(q)::value];
             if (op.code == OP_NONE) return;
             if (op.code != OP_SMOOTH) {
-                apply_op<LA, LB>(op, fa, fb, pv, n, fc);
+                apply_op<LA, LB>(op, xa, xb, pv, n, fc);
             } else if ((op.lat ^ pv.latswap) == 0) {
                 sfor<0, LA::nc>([&](auto c) {
                     constexpr int C = decltype(c)::value;
-                    fa[C] = 0.5 * (s0[C] + s1[C]);
+                    xa[C] = 0.5 * (s0[C] + s1[C]);
                 });
             } else if constexpr (LB::nc > 0) {
                 sfor<0, LB::nc>([&](auto c) {
                     constexpr int C = decltype(c)::value;
-                    fb[C] = 0.5 * (s0[LA::nc + C] + s1[LA::nc + C]);
+                    xb[C] = 0.5 * (s0[LA::nc + C] + s1[LA::nc + C]);
                 });
             }
         });
     }
-    body(fa, fb, n, true);
-    store_node<LA>(da, g.N, n, fa);
-    store_node<LB>(db, g.N, n, fb);
+    get_state<LA>(fa, xa, body.ea, oa);
+    if constexpr (LB::nc > 0) get_state<LB>(fb, xb, body.eb, ob);
+    body(fa, fb, n, true, oa, ob);
+    store_raw<LA>(da, g.N, n, fa);
+    store_raw<LB>(db, g.N, n, fb);
 }
 
 // Dispatch a functor over the supported lattices (single) or (A, B) lattice pairs (coupled).
@@ -368,23 +654,32 @@ static hipError_t dispatch_coupled_nd(int ka, int kb, F &&f) {
     return hipErrorInvalidValue;
 }
 
-template <template <class, class> class Body, bool COUPLED, int ND, class Make>
+template <template <class, class, class> class Body, bool COUPLED, int ND, int P, class Make>
 static hipError_t launch_collide(const CollideLaunch &cl, Make &&make) {
+    using S = typename PrecT<P>::S;
+    using R = typename PrecT<P>::R;
+    if (cl.prec != P) return hipErrorInvalidValue;
     auto go = [&](auto la, auto lb) -> hipError_t {
         using LA = decltype(la);
         using LB = decltype(lb);
-        Body<LA, LB> body = make(la, lb);
+        using B = Body<LA, LB, R>;
+        B body = make(la, lb, R(0));
+        body.N = cl.g.N;
+        body.ea = cl.ea;
+        body.eb = cl.eb;
         const Grid &g = cl.g;
+        S *da = static_cast<S *>(cl.da), *db = static_cast<S *>(cl.db);
         if (!cl.pv) {
-            LBM_LAUNCH((k_collide_inplace<LA, LB, Body<LA, LB>>), nblocks(g.N), cl.stream, cl.da, cl.db, g, body);
+            LBM_LAUNCH((k_collide_inplace<LA, LB, B, S>), nblocks(g.N), cl.stream, da, db, g, body);
             return hipGetLastError();
         }
+        const S *sa = static_cast<const S *>(cl.sa), *sb = static_cast<const S *>(cl.sb);
         const ProgView &pv = *cl.pv;
-        LBM_LAUNCH((k_collide_fused<LA, LB, Body<LA, LB>>), nblocks(g.N) + pv.corner_block, cl.stream, cl.sa, cl.sb,
-                   cl.da, cl.db, g, pv, cl.any_smooth, body);
+        LBM_LAUNCH((k_collide_fused<LA, LB, B, S>), nblocks(g.N) + pv.corner_block, cl.stream, sa, sb, da, db, g, pv,
+                   cl.any_smooth, body);
         if (cl.n_corner2d > 0) {
-            LBM_LAUNCH((k_collide_corner2d<LA, LB, Body<LA, LB>>), nblocks(cl.n_corner2d), cl.stream, cl.sa, cl.sb,
-                       cl.da, cl.db, g, pv, cl.corner2d, cl.n_corner2d, body);
+            LBM_LAUNCH((k_collide_corner2d<LA, LB, B, S>), nblocks(cl.n_corner2d), cl.stream, sa, sb, da, db, g, pv,
+                       cl.corner2d, cl.n_corner2d, body);
         }
         return hipGetLastError();
     };
@@ -393,51 +688,55 @@ static hipError_t launch_collide(const CollideLaunch &cl, Make &&make) {
 }
 
 
-// The collides of one equation and one dimension; each is instantiated in its own translation
-// unit (collide_<eq><nd>.hip) and dispatched by launch_<eq>_collide (device.hip).
-template <int ND>
+// The collides of one equation, dimension and storage; each is instantiated in its own
+// translation unit (collide_<eq><nd>[_f32].hip) and dispatched by launch_<eq>_collide (device.hip).
+template <int ND, int P>
 hipError_t ns_collide_nd(const CollideLaunch &cl, const NSArgs &a) {
     double omega = 1.0 / (3.0 * a.nu + 0.5);
-    return launch_collide<NSBody, false, ND>(cl, [&](auto la, auto lb) {
-        NSBody<decltype(la), decltype(lb)> b;
+    return launch_collide<NSBody, false, ND, P>(cl, [&](auto la, auto lb, auto r) {
+        using R = decltype(r);
+        NSBody<decltype(la), decltype(lb), R> b;
         b.rho = a.rho; b.ux = a.ux; b.uy = a.uy; b.uz = a.uz; b.alpha = a.alpha;
-        b.omega = omega; b.iomega = 1.0 - omega; b.save = a.save; b.brinkman = a.alpha != nullptr;
+        b.omega = R(omega); b.iomega = R(1.0 - omega); b.save = a.save; b.brinkman = a.alpha != nullptr;
         return b;
     });
 }
-template <int ND>
+template <int ND, int P>
 hipError_t ad_collide_nd(const CollideLaunch &cl, const ADArgs &a) {
     double omega = 1.0 / (3.0 * a.nu + 0.5);
-    return launch_collide<ADBody, true, ND>(cl, [&](auto la, auto lb) {
-        ADBody<decltype(la), decltype(lb)> b;
+    return launch_collide<ADBody, true, ND, P>(cl, [&](auto la, auto lb, auto r) {
+        using R = decltype(r);
+        ADBody<decltype(la), decltype(lb), R> b;
         b.rho = a.rho; b.ux = a.ux; b.uy = a.uy; b.uz = a.uz; b.tem = a.tem; b.qx = a.qx; b.qy = a.qy; b.qz = a.qz;
         b.gsnap = a.gsnap; b.alpha = a.alpha; b.diff = a.diff; b.heat = a.heat;
-        b.omegaf = omega; b.iomegaf = 1.0 - omega; b.save = a.save; b.N = cl.g.N;
-        b.natural = a.natural; b.gx = a.gx; b.gy = a.gy; b.gz = a.gz; b.tem0 = a.tem0;
+        b.omegaf = R(omega); b.iomegaf = R(1.0 - omega); b.save = a.save;
+        b.natural = a.natural; b.gx = R(a.gx); b.gy = R(a.gy); b.gz = R(a.gz); b.tem0 = R(a.tem0);
         return b;
     });
 }
-template <int ND>
+template <int ND, int P>
 hipError_t ans_collide_nd(const CollideLaunch &cl, const ANSArgs &a) {
     double omega = 1.0 / (3.0 * a.nu + 0.5);
-    return launch_collide<ANSBody, false, ND>(cl, [&](auto la, auto lb) {
-        ANSBody<decltype(la), decltype(lb)> b;
+    return launch_collide<ANSBody, false, ND, P>(cl, [&](auto la, auto lb, auto r) {
+        using R = decltype(r);
+        ANSBody<decltype(la), decltype(lb), R> b;
         b.rho = a.rho; b.ux = a.ux; b.uy = a.uy; b.uz = a.uz; b.alpha = a.alpha;
         b.ip = a.ip; b.iux = a.iux; b.iuy = a.iuy; b.iuz = a.iuz; b.imx = a.imx; b.imy = a.imy; b.imz = a.imz;
-        b.omega = omega; b.iomega = 1.0 - omega; b.save = a.save;
+        b.omega = R(omega); b.iomega = R(1.0 - omega); b.save = a.save;
         return b;
     });
 }
-template <int ND>
+template <int ND, int P>
 hipError_t aad_collide_nd(const CollideLaunch &cl, const AADArgs &a) {
     double omega = 1.0 / (3.0 * a.nu + 0.5);
-    return launch_collide<AADBody, true, ND>(cl, [&](auto la, auto lb) {
-        AADBody<decltype(la), decltype(lb)> b;
+    return launch_collide<AADBody, true, ND, P>(cl, [&](auto la, auto lb, auto r) {
+        using R = decltype(r);
+        AADBody<decltype(la), decltype(lb), R> b;
         b.rho = a.rho; b.ux = a.ux; b.uy = a.uy; b.uz = a.uz; b.alpha = a.alpha; b.tem = a.tem; b.diff = a.diff;
         b.ip = a.ip; b.iux = a.iux; b.iuy = a.iuy; b.iuz = a.iuz; b.imx = a.imx; b.imy = a.imy; b.imz = a.imz;
         b.item = a.item; b.iqx = a.iqx; b.iqy = a.iqy; b.iqz = a.iqz; b.igsnap = a.igsnap;
-        b.omegaf = omega; b.iomegaf = 1.0 - omega; b.save = a.save; b.N = cl.g.N;
-        b.natural = a.natural; b.gx = a.gx; b.gy = a.gy; b.gz = a.gz;
+        b.omegaf = R(omega); b.iomegaf = R(1.0 - omega); b.save = a.save;
+        b.natural = a.natural; b.gx = R(a.gx); b.gy = R(a.gy); b.gz = R(a.gz);
         return b;
     });
 }

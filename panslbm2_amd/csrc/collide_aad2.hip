@@ -1,7 +1,8 @@
-// panslbm2_amd/csrc/collide_aad2.hip -- the AAD collides on the 2D lattices (one translation unit
-// per equation and dimension so the kernel instantiations compile in parallel).
+// panslbm2_amd/csrc/collide_aad2.hip -- the AAD collides on the 2D lattices, FP64 storage (one
+// translation unit per equation, dimension and precision so the kernel instantiations compile in
+// parallel).
 #include "collide.h"
 
 namespace lbm {
-template hipError_t aad_collide_nd<2>(const CollideLaunch &cl, const AADArgs &a);
+template hipError_t aad_collide_nd<2, kF64>(const CollideLaunch &cl, const AADArgs &a);
 }  // namespace lbm

@@ -1,7 +1,8 @@
-// panslbm2_amd/csrc/collide_ns3.hip -- the NS collides on the 3D lattices (one translation unit
-// per equation and dimension so the kernel instantiations compile in parallel).
+// panslbm2_amd/csrc/collide_ns3.hip -- the NS collides on the 3D lattices, FP64 storage (one
+// translation unit per equation, dimension and precision so the kernel instantiations compile in
+// parallel).
 #include "collide.h"
 
 namespace lbm {
-template hipError_t ns_collide_nd<3>(const CollideLaunch &cl, const NSArgs &a);
+template hipError_t ns_collide_nd<3, kF64>(const CollideLaunch &cl, const NSArgs &a);
 }  // namespace lbm

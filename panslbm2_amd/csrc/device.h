@@ -9,10 +9,12 @@ namespace lbm {
 
 struct CollideLaunch {
     int ka, kb;                // lattice kinds of the kernel's lattices (kb < 0: single lattice)
+    int prec;                  // population storage (Prec), the same for both lattices
     Grid g;
     hipStream_t stream;
-    const double *sa, *sb;     // sources (fused) -- unused in place
-    double *da, *db;           // destinations (fused) or the in-place buffers
+    const void *sa, *sb;       // sources (fused) -- unused in place
+    void *da, *db;             // destinations (fused) or the in-place buffers
+    Enc ea, eb;                // FP32 storage shift of each lattice (no shift in FP64)
     const ProgView *pv;        // nullptr: plain in-place collide
     const int4 *corner2d;      // 2D smoothed corners (node, side index of x-neighbour, of y-neighbour, -1),
     int n_corner2d;            // done after the fused pass from the side states it saved
@@ -48,27 +50,35 @@ struct AADArgs {
     double gx, gy, gz;
 };
 
-hipError_t launch_stream(int kind, const double *src, double *dst, Grid g, int pull, hipStream_t s);
-hipError_t launch_smooth(int kind, double *f, int N, const int4 *list, int count, hipStream_t s);
-hipError_t launch_program_apply(int ka, int kb, double *a, double *b, int N, const ProgView &pv, int seg,
-                                hipStream_t s);
-template <int ND> hipError_t ns_collide_nd(const CollideLaunch &cl, const NSArgs &a);
-template <int ND> hipError_t ad_collide_nd(const CollideLaunch &cl, const ADArgs &a);
-template <int ND> hipError_t ans_collide_nd(const CollideLaunch &cl, const ANSArgs &a);
-template <int ND> hipError_t aad_collide_nd(const CollideLaunch &cl, const AADArgs &a);
-extern template hipError_t ns_collide_nd<2>(const CollideLaunch &, const NSArgs &);
-extern template hipError_t ns_collide_nd<3>(const CollideLaunch &, const NSArgs &);
-extern template hipError_t ad_collide_nd<2>(const CollideLaunch &, const ADArgs &);
-extern template hipError_t ad_collide_nd<3>(const CollideLaunch &, const ADArgs &);
-extern template hipError_t ans_collide_nd<2>(const CollideLaunch &, const ANSArgs &);
-extern template hipError_t ans_collide_nd<3>(const CollideLaunch &, const ANSArgs &);
-extern template hipError_t aad_collide_nd<2>(const CollideLaunch &, const AADArgs &);
-extern template hipError_t aad_collide_nd<3>(const CollideLaunch &, const AADArgs &);
+hipError_t launch_stream(int kind, int prec, const void *src, void *dst, Grid g, int pull, const Enc &e, hipStream_t s);
+hipError_t launch_smooth(int kind, int prec, void *f, int N, const int4 *list, int count, const Enc &e, hipStream_t s);
+hipError_t launch_program_apply(int ka, int kb, int prec, void *a, void *b, int N, const ProgView &pv, int seg,
+                                const Enc &ea, const Enc &eb, hipStream_t s);
+template <int ND, int P> hipError_t ns_collide_nd(const CollideLaunch &cl, const NSArgs &a);
+template <int ND, int P> hipError_t ad_collide_nd(const CollideLaunch &cl, const ADArgs &a);
+template <int ND, int P> hipError_t ans_collide_nd(const CollideLaunch &cl, const ANSArgs &a);
+template <int ND, int P> hipError_t aad_collide_nd(const CollideLaunch &cl, const AADArgs &a);
+#define LBM_EXTERN_COLLIDES(eq, Args)                                                  \
+    extern template hipError_t eq##_collide_nd<2, kF64>(const CollideLaunch &, const Args &); \
+    extern template hipError_t eq##_collide_nd<3, kF64>(const CollideLaunch &, const Args &); \
+    extern template hipError_t eq##_collide_nd<2, kF32>(const CollideLaunch &, const Args &); \
+    extern template hipError_t eq##_collide_nd<3, kF32>(const CollideLaunch &, const Args &);
+LBM_EXTERN_COLLIDES(ns, NSArgs)
+LBM_EXTERN_COLLIDES(ad, ADArgs)
+LBM_EXTERN_COLLIDES(ans, ANSArgs)
+LBM_EXTERN_COLLIDES(aad, AADArgs)
+#undef LBM_EXTERN_COLLIDES
 hipError_t launch_ns_collide(const CollideLaunch &cl, const NSArgs &a);
 hipError_t launch_ad_collide(const CollideLaunch &cl, const ADArgs &a);
 hipError_t launch_ans_collide(const CollideLaunch &cl, const ANSArgs &a);
 hipError_t launch_aad_collide(const CollideLaunch &cl, const AADArgs &a);
-hipError_t launch_init(int kind, double *f, int N, int which, const double *const *p, hipStream_t s);
+// e: the shift the FP32 values are stored with
+hipError_t launch_init(int kind, int prec, void *f, int N, int which, const double *const *p, const Enc &e,
+                       hipStream_t s);
+hipError_t launch_convert(int kind, int from, const void *src, const Enc &es, int to, void *dst, const Enc &et, int N,
+                          hipStream_t s);
+hipError_t launch_encode(int kind, int prec, const double *src, void *dst, int N, const Enc &e, hipStream_t s);
+hipError_t launch_decode(int kind, int prec, const void *src, double *dst, int N, const Enc &e, hipStream_t s);
 hipError_t launch_sens_brinkman(int nd, double *dfds, int N, const double *const *p, hipStream_t s);
 hipError_t launch_sens_bd(int kind, double *dfds, int N, const double *const *p, hipStream_t s);
 hipError_t launch_sens_hs_boundary(double *dfds, int N, const ProgView &pv, const double *ux, const double *uy,

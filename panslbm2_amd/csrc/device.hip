@@ -10,52 +10,56 @@ namespace lbm {
 // Streaming: d2q9.h:282-291 / d3q15.h:602-613 (pull from x - c), iStream :414-423 / :965-976
 // (pull from x + c).  The rest population is copied through.
 // ------------------------------------------------------------------------------------------
-template <class L>
-__global__ void __launch_bounds__(kBlock) k_stream(const double *__restrict__ src, double *__restrict__ dst, Grid g,
-                                                   int pull) {
+// FP32 storage re-encodes each moved value with its new node's shift.
+template <class L, class S>
+__global__ void __launch_bounds__(kBlock) k_stream(const S *__restrict__ src, S *__restrict__ dst, Grid g, int pull,
+                                                   Enc e) {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= g.N) return;
     int i, j, k;
     coords(n, g, i, j, k);
     double f[L::nc];
-    load_pull<L>(src, g, i, j, k, n, pull, f);
-    store_node<L>(dst, g.N, n, f);
+    load_pull<L>(src, g, i, j, k, n, pull, f, e);
+    store_node<L>(dst, g.N, n, f, e);
 }
 
 // ------------------------------------------------------------------------------------------
 // Boundary program, applied in place: one thread per boundary node runs that node's operators of
 // segment `seg` in call order (segments are separated by SmoothCorner calls).
 // ------------------------------------------------------------------------------------------
-template <class LA, class LB>
-__global__ void __launch_bounds__(kBlock) k_program_apply(double *a, double *b, int N, ProgView pv, int seg) {
+template <class LA, class LB, class S>
+__global__ void __launch_bounds__(kBlock) k_program_apply(S *a, S *b, int N, ProgView pv, int seg, Enc ea, Enc eb) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= pv.nentry) return;
     int n = pv.entry_node[e];
     int2 r = pv.entry_ops[e];
     double fa[LA::nc], fb[LB::nc > 0 ? LB::nc : 1];
-    load_node<LA>(a, N, n, fa);
-    load_node<LB>(b, N, n, fb);
+    load_node<LA>(a, N, n, fa, ea);
+    load_node<LB>(b, N, n, fb, eb);
     NodeFieldCache fc = prefetch_fields(pv, n);
     for (int t = r.x; t < r.x + r.y; ++t) {
         Op op = pv.ops[t];
         if (op.code != OP_SMOOTH && (int)op.seg == seg) apply_op<LA, LB>(op, fa, fb, pv, n, fc);
     }
-    store_node<LA>(a, N, n, fa);
-    store_node<LB>(b, N, n, fb);
+    store_node<LA>(a, N, n, fa, ea);
+    store_node<LB>(b, N, n, fb, eb);
 }
 
 // Corner / edge smoothing in place, d2q9.h:577-587, d3q15.h:1241-1303.  Entry = (node, nb0,
 // nb1, nb2 or -1); neighbours are listed in axis order.
-template <class L>
-__global__ void __launch_bounds__(kBlock) k_smooth(double *f, int N, const int4 *__restrict__ list, int count) {
+template <class L, class S>
+__global__ void __launch_bounds__(kBlock) k_smooth(S *f, int N, const int4 *__restrict__ list, int count, Enc en) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= count) return;
     int4 s = list[e];
     sfor<0, L::nc>([&](auto c) {
         constexpr int C = decltype(c)::value;
-        double *p = f + (size_t)C * N;
-        if (s.w < 0) p[s.x] = 0.5 * (p[s.y] + p[s.z]);
-        else p[s.x] = (p[s.y] + p[s.z] + p[s.w]) / 3.0;
+        S *p = f + (size_t)C * N;
+        double a = dec<L, C>(p[s.y], en), b = dec<L, C>(p[s.z], en);
+        double v;
+        if (s.w < 0) v = 0.5 * (a + b);
+        else v = (a + b + dec<L, C>(p[s.w], en)) / 3.0;
+        p[s.x] = enc<L, C, S>(v, en);
     });
 }
 
@@ -63,10 +67,11 @@ __global__ void __launch_bounds__(kBlock) k_smooth(double *f, int N, const int4 
 // Initial conditions (equilibria of given fields): navierstokes.h:550-572, advection.h:1048-1070,
 // adjointnavierstokes.h:474-498, adjointadvection.h:1359-1381
 // ------------------------------------------------------------------------------------------
-template <class L>
-__global__ void __launch_bounds__(kBlock) k_init(double *f, int N, int which, const double *a0, const double *a1,
+// Equilibria formed in FP64, stored with the lattice's shift.
+template <class L, class S>
+__global__ void __launch_bounds__(kBlock) k_init(S *f, int N, int which, const double *a0, const double *a1,
                                                  const double *a2, const double *a3, const double *a4, const double *a5,
-                                                 const double *a6) {
+                                                 const double *a6, Enc e) {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
     double feq[L::nc];
@@ -81,7 +86,34 @@ __global__ void __launch_bounds__(kBlock) k_init(double *f, int N, int which, co
         double ge = aad_eq<L>(a3[n], a4[n], a5[n], z(a6), a0[n], a1[n], z(a2));
         sfor<0, L::nc>([&](auto c) { feq[decltype(c)::value] = ge; });
     }
-    store_node<L>(f, N, n, feq);
+    store_node<L>(f, N, n, feq, e);
+}
+
+// Host FP64 populations (plane order) <-> storage; storage -> storage of another precision.
+template <class L, class S>
+__global__ void __launch_bounds__(kBlock) k_encode(const double *__restrict__ src, S *__restrict__ dst, int N, Enc e) {
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    double v[L::nc];
+    load_node<L>(src, N, n, v);
+    store_node<L>(dst, N, n, v, e);
+}
+template <class L, class S>
+__global__ void __launch_bounds__(kBlock) k_decode(const S *__restrict__ src, double *__restrict__ dst, int N, Enc e) {
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    double v[L::nc];
+    load_node<L>(src, N, n, v, e);
+    store_node<L>(dst, N, n, v);
+}
+template <class L, class S, class T>
+__global__ void __launch_bounds__(kBlock) k_convert(const S *__restrict__ src, Enc es, T *__restrict__ dst, Enc et,
+                                                    int N) {
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    double v[L::nc];
+    load_node<L>(src, N, n, v, es);
+    store_node<L>(dst, N, n, v, et);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -279,75 +311,114 @@ __global__ void __launch_bounds__(kBlock) k_density_filter(const double *v, doub
 // typed launchers
 // ==========================================================================================
 
-template <class L>
-static hipError_t launch_stream_t(const double *src, double *dst, Grid g, int pull, hipStream_t s) {
-    LBM_LAUNCH((k_stream<L>), nblocks(g.N), s, src, dst, g, pull);
-    return hipGetLastError();
-}
-hipError_t launch_stream(int kind, const double *src, double *dst, Grid g, int pull, hipStream_t s) {
+// Dispatch over one lattice kind and the storage type.
+template <class F>
+static hipError_t dispatch_kind_prec(int kind, int prec, F &&f) {
+    auto by_prec = [&](auto l) -> hipError_t {
+        if (prec == kF64) return f(l, double{});
+        if (prec == kF32) return f(l, float{});
+        return hipErrorInvalidValue;
+    };
     switch (kind) {
-    case kD2Q9: return launch_stream_t<D2Q9>(src, dst, g, pull, s);
-    case kD3Q15: return launch_stream_t<D3Q15>(src, dst, g, pull, s);
-    case kD2Q5: return launch_stream_t<D2Q5>(src, dst, g, pull, s);
-    case kD3Q7: return launch_stream_t<D3Q7>(src, dst, g, pull, s);
+    case kD2Q9: return by_prec(D2Q9{});
+    case kD3Q15: return by_prec(D3Q15{});
+    case kD2Q5: return by_prec(D2Q5{});
+    case kD3Q7: return by_prec(D3Q7{});
     }
     return hipErrorInvalidValue;
 }
 
-template <class L>
-static hipError_t launch_smooth_t(double *f, int N, const int4 *list, int count, hipStream_t s) {
-    if (count > 0) LBM_LAUNCH((k_smooth<L>), nblocks(count), s, f, N, list, count);
-    return hipGetLastError();
-}
-hipError_t launch_smooth(int kind, double *f, int N, const int4 *list, int count, hipStream_t s) {
-    switch (kind) {
-    case kD2Q9: return launch_smooth_t<D2Q9>(f, N, list, count, s);
-    case kD3Q15: return launch_smooth_t<D3Q15>(f, N, list, count, s);
-    case kD2Q5: return launch_smooth_t<D2Q5>(f, N, list, count, s);
-    case kD3Q7: return launch_smooth_t<D3Q7>(f, N, list, count, s);
-    }
-    return hipErrorInvalidValue;
-}
-
-hipError_t launch_program_apply(int ka, int kb, double *a, double *b, int N, const ProgView &pv, int seg,
-                                hipStream_t s) {
-    if (pv.nentry == 0) return hipSuccess;
-    return dispatch_pair(ka, kb, [&](auto la, auto lb) {
-        using LA = decltype(la);
-        using LB = decltype(lb);
-        LBM_LAUNCH((k_program_apply<LA, LB>), nblocks(pv.nentry), s, a, b, N, pv, seg);
+hipError_t launch_stream(int kind, int prec, const void *src, void *dst, Grid g, int pull, const Enc &e, hipStream_t s) {
+    return dispatch_kind_prec(kind, prec, [&](auto l, auto st) {
+        using L = decltype(l);
+        using S = decltype(st);
+        LBM_LAUNCH((k_stream<L, S>), nblocks(g.N), s, static_cast<const S *>(src), static_cast<S *>(dst), g, pull, e);
         return hipGetLastError();
     });
 }
 
-// collides: by the dimension of the first lattice (kinds D2Q9/D2Q5 are 2D, D3Q15/D3Q7 3D)
-static bool is_3d_kind(int k) { return k == kD3Q15 || k == kD3Q7; }
-hipError_t launch_ns_collide(const CollideLaunch &cl, const NSArgs &a) {
-    return is_3d_kind(cl.ka) ? ns_collide_nd<3>(cl, a) : ns_collide_nd<2>(cl, a);
-}
-hipError_t launch_ad_collide(const CollideLaunch &cl, const ADArgs &a) {
-    return is_3d_kind(cl.ka) ? ad_collide_nd<3>(cl, a) : ad_collide_nd<2>(cl, a);
-}
-hipError_t launch_ans_collide(const CollideLaunch &cl, const ANSArgs &a) {
-    return is_3d_kind(cl.ka) ? ans_collide_nd<3>(cl, a) : ans_collide_nd<2>(cl, a);
-}
-hipError_t launch_aad_collide(const CollideLaunch &cl, const AADArgs &a) {
-    return is_3d_kind(cl.ka) ? aad_collide_nd<3>(cl, a) : aad_collide_nd<2>(cl, a);
+hipError_t launch_smooth(int kind, int prec, void *f, int N, const int4 *list, int count, const Enc &e, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    return dispatch_kind_prec(kind, prec, [&](auto l, auto st) {
+        using L = decltype(l);
+        using S = decltype(st);
+        LBM_LAUNCH((k_smooth<L, S>), nblocks(count), s, static_cast<S *>(f), N, list, count, e);
+        return hipGetLastError();
+    });
 }
 
-template <class L>
-static hipError_t launch_init_t(double *f, int N, int which, const double *const *p, hipStream_t s) {
-    LBM_LAUNCH((k_init<L>), nblocks(N), s, f, N, which, p[0], p[1], p[2], p[3], p[4], p[5], p[6]);
-    return hipGetLastError();
+hipError_t launch_program_apply(int ka, int kb, int prec, void *a, void *b, int N, const ProgView &pv, int seg,
+                                const Enc &ea, const Enc &eb, hipStream_t s) {
+    if (pv.nentry == 0) return hipSuccess;
+    return dispatch_pair(ka, kb, [&](auto la, auto lb) {
+        using LA = decltype(la);
+        using LB = decltype(lb);
+        if (prec == kF64)
+            LBM_LAUNCH((k_program_apply<LA, LB, double>), nblocks(pv.nentry), s, static_cast<double *>(a),
+                       static_cast<double *>(b), N, pv, seg, ea, eb);
+        else if (prec == kF32)
+            LBM_LAUNCH((k_program_apply<LA, LB, float>), nblocks(pv.nentry), s, static_cast<float *>(a),
+                       static_cast<float *>(b), N, pv, seg, ea, eb);
+        else
+            return hipErrorInvalidValue;
+        return hipGetLastError();
+    });
 }
-hipError_t launch_init(int kind, double *f, int N, int which, const double *const *p, hipStream_t s) {
-    switch (kind) {
-    case kD2Q9: return launch_init_t<D2Q9>(f, N, which, p, s);
-    case kD3Q15: return launch_init_t<D3Q15>(f, N, which, p, s);
-    case kD2Q5: return launch_init_t<D2Q5>(f, N, which, p, s);
-    case kD3Q7: return launch_init_t<D3Q7>(f, N, which, p, s);
-    }
-    return hipErrorInvalidValue;
+
+// collides: by the dimension of the first lattice (kinds D2Q9/D2Q5 are 2D, D3Q15/D3Q7 3D) and the
+// storage precision
+static bool is_3d_kind(int k) { return k == kD3Q15 || k == kD3Q7; }
+#define LBM_COLLIDE_DISPATCH(eq, cl, a)                                                            \
+    (cl.prec == kF64 ? (is_3d_kind(cl.ka) ? eq##_collide_nd<3, kF64>(cl, a) : eq##_collide_nd<2, kF64>(cl, a)) \
+                     : (is_3d_kind(cl.ka) ? eq##_collide_nd<3, kF32>(cl, a) : eq##_collide_nd<2, kF32>(cl, a)))
+hipError_t launch_ns_collide(const CollideLaunch &cl, const NSArgs &a) { return LBM_COLLIDE_DISPATCH(ns, cl, a); }
+hipError_t launch_ad_collide(const CollideLaunch &cl, const ADArgs &a) { return LBM_COLLIDE_DISPATCH(ad, cl, a); }
+hipError_t launch_ans_collide(const CollideLaunch &cl, const ANSArgs &a) { return LBM_COLLIDE_DISPATCH(ans, cl, a); }
+hipError_t launch_aad_collide(const CollideLaunch &cl, const AADArgs &a) { return LBM_COLLIDE_DISPATCH(aad, cl, a); }
+#undef LBM_COLLIDE_DISPATCH
+
+hipError_t launch_init(int kind, int prec, void *f, int N, int which, const double *const *p, const Enc &e,
+                       hipStream_t s) {
+    return dispatch_kind_prec(kind, prec, [&](auto l, auto st) {
+        using L = decltype(l);
+        using S = decltype(st);
+        LBM_LAUNCH((k_init<L, S>), nblocks(N), s, static_cast<S *>(f), N, which, p[0], p[1], p[2], p[3], p[4], p[5],
+                   p[6], e);
+        return hipGetLastError();
+    });
+}
+hipError_t launch_convert(int kind, int from, const void *src, const Enc &es, int to, void *dst, const Enc &et, int N,
+                          hipStream_t s) {
+    return dispatch_kind_prec(kind, from, [&](auto l, auto sf) {
+        using L = decltype(l);
+        using S = decltype(sf);
+        if (to == kF64)
+            LBM_LAUNCH((k_convert<L, S, double>), nblocks(N), s, static_cast<const S *>(src), es,
+                       static_cast<double *>(dst), et, N);
+        else if (to == kF32)
+            LBM_LAUNCH((k_convert<L, S, float>), nblocks(N), s, static_cast<const S *>(src), es,
+                       static_cast<float *>(dst), et, N);
+        else
+            return hipErrorInvalidValue;
+        return hipGetLastError();
+    });
+}
+
+hipError_t launch_encode(int kind, int prec, const double *src, void *dst, int N, const Enc &e, hipStream_t s) {
+    return dispatch_kind_prec(kind, prec, [&](auto l, auto st) {
+        using L = decltype(l);
+        using S = decltype(st);
+        LBM_LAUNCH((k_encode<L, S>), nblocks(N), s, src, static_cast<S *>(dst), N, e);
+        return hipGetLastError();
+    });
+}
+hipError_t launch_decode(int kind, int prec, const void *src, double *dst, int N, const Enc &e, hipStream_t s) {
+    return dispatch_kind_prec(kind, prec, [&](auto l, auto st) {
+        using L = decltype(l);
+        using S = decltype(st);
+        LBM_LAUNCH((k_decode<L, S>), nblocks(N), s, static_cast<const S *>(src), dst, N, e);
+        return hipGetLastError();
+    });
 }
 
 hipError_t launch_sens_brinkman(int nd, double *dfds, int N, const double *const *p, hipStream_t s) {

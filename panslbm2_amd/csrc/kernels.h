@@ -182,12 +182,30 @@ __device__ __forceinline__ void run_slot_ops(const ProgView &pv, int slot, int n
     }
 }
 
+// E_c of a shift, in FP64
+template <class L, int C>
+__device__ __forceinline__ double shift_c(const Enc &e) {
+    return e.rest * L::w[C];
+}
+// decode a stored value of node n (FP64 storage: the value itself)
+template <class L, int C, class S>
+__device__ __forceinline__ double dec(S x, const Enc &e) {
+    if constexpr (std::is_same_v<S, double>) return x;
+    else return double(x) + shift_c<L, C>(e);
+}
+template <class L, int C, class S>
+__device__ __forceinline__ S enc(double v, const Enc &e) {
+    if constexpr (std::is_same_v<S, double>) return v;
+    else return S(v - shift_c<L, C>(e));
+}
+
 // Pull node n's populations: pull = +1 from x - c (Stream), -1 from x + c (iStream), 0 in place.
-template <class L>
-__device__ __forceinline__ void load_pull(const double *__restrict__ src, const Grid &g, int i, int j, int k, int n,
-                                          int pull, double *f) {
+// S: storage type; values are decoded with the source node's shift.
+template <class L, class S, class R>
+__device__ __forceinline__ void load_pull(const S *__restrict__ src, const Grid &g, int i, int j, int k, int n,
+                                          int pull, R *f, const Enc &e = Enc{}) {
     if constexpr (L::nc > 0) {
-        f[0] = src[n];
+        f[0] = R(dec<L, 0>(src[n], e));
         sfor<1, L::nc>([&](auto c) {
             constexpr int C = decltype(c)::value;
             int s = n;
@@ -197,23 +215,93 @@ __device__ __forceinline__ void load_pull(const double *__restrict__ src, const 
                 int kk = L::nd == 3 ? wrap(k, -pull * L::cz[C], g.nz) : 0;
                 s = ii + g.nx * (jj + g.ny * kk);
             }
-            f[C] = src[(size_t)C * g.N + s];
+            f[C] = R(dec<L, C>(src[(size_t)C * g.N + s], e));
         });
     }
 }
-template <class L>
-__device__ __forceinline__ void load_node(const double *__restrict__ src, int N, int n, double *f) {
+template <class L, class S>
+__device__ __forceinline__ void load_node(const S *__restrict__ src, int N, int n, double *f, const Enc &e = Enc{}) {
     sfor<0, L::nc>([&](auto c) {
         constexpr int C = decltype(c)::value;
-        f[C] = src[(size_t)C * N + n];
+        f[C] = dec<L, C>(src[(size_t)C * N + n], e);
+    });
+}
+template <class L, class S>
+__device__ __forceinline__ void store_node(S *__restrict__ dst, int N, int n, const double *f, const Enc &e = Enc{}) {
+    sfor<0, L::nc>([&](auto c) {
+        constexpr int C = decltype(c)::value;
+        dst[(size_t)C * N + n] = enc<L, C, S>(f[C], e);
+    });
+}
+
+// register type of a storage type, and a lattice's shift as moments (FP64 storage: none)
+template <class S>
+using RegOf = std::conditional_t<std::is_same_v<S, double>, double, float>;
+template <class S>
+__device__ __forceinline__ Sh4 own_shift(const Enc &e) {
+    if constexpr (std::is_same_v<S, double>) return Sh4{};
+    else return Sh4{float(e.rest), 0.0f, 0.0f, 0.0f};
+}
+// Values as stored, without decoding (the hot path keeps FP32 deviations of the node's own shift).
+template <class L, class S, class R>
+__device__ __forceinline__ void load_raw(const S *__restrict__ src, int N, int n, R *f) {
+    sfor<0, L::nc>([&](auto c) { f[decltype(c)::value] = R(src[(size_t)decltype(c)::value * N + n]); });
+}
+template <class L, class S, class R>
+__device__ __forceinline__ void store_raw(S *__restrict__ dst, int N, int n, const R *f) {
+    sfor<0, L::nc>([&](auto c) { dst[(size_t)decltype(c)::value * N + n] = S(f[decltype(c)::value]); });
+}
+// Register state <-> FP64 populations (identity in FP64; deviations of shift o in FP32)
+template <class L>
+__device__ __forceinline__ void put_state(double *dst, const double *v, const Enc &, const Sh4 &) {
+    sfor<0, L::nc>([&](auto c) { dst[decltype(c)::value] = v[decltype(c)::value]; });
+}
+template <class L>
+__device__ __forceinline__ void put_state(double *dst, const float *v, const Enc &e, const Sh4 &o);
+template <class L>
+__device__ __forceinline__ void get_state(double *v, const double *src, const Enc &, const Sh4 &) {
+    sfor<0, L::nc>([&](auto c) { v[decltype(c)::value] = src[decltype(c)::value]; });
+}
+template <class L>
+__device__ __forceinline__ void get_state(float *v, const double *src, const Enc &e, const Sh4 &o);
+// d2q9.h:577-587: the mean of two FP64 states
+template <class L>
+__device__ __forceinline__ void mean_state(double *v, const double *a, const double *b, const Enc &, const Sh4 &) {
+    sfor<0, L::nc>([&](auto c) {
+        constexpr int C = decltype(c)::value;
+        v[C] = 0.5 * (a[C] + b[C]);
     });
 }
 template <class L>
-__device__ __forceinline__ void store_node(double *__restrict__ dst, int N, int n, const double *f) {
+__device__ __forceinline__ void mean_state(float *v, const double *a, const double *b, const Enc &e, const Sh4 &) {
     sfor<0, L::nc>([&](auto c) {
         constexpr int C = decltype(c)::value;
-        dst[(size_t)C * N + n] = f[C];
+        v[C] = float(0.5 * (a[C] + b[C]) - shift_c<L, C>(e));
     });
+}
+
+// FP32 deviations <-> FP64 populations, for the boundary operators and smoothing
+template <class L>
+__device__ __forceinline__ void dev_to_f64(const float *d, double *f, const Enc &e, const Sh4 &) {
+    sfor<0, L::nc>([&](auto c) {
+        constexpr int C = decltype(c)::value;
+        f[C] = double(d[C]) + shift_c<L, C>(e);
+    });
+}
+template <class L>
+__device__ __forceinline__ void f64_to_dev(const double *f, float *d, const Enc &e, const Sh4 &) {
+    sfor<0, L::nc>([&](auto c) {
+        constexpr int C = decltype(c)::value;
+        d[C] = float(f[C] - shift_c<L, C>(e));
+    });
+}
+template <class L>
+__device__ __forceinline__ void put_state(double *dst, const float *v, const Enc &e, const Sh4 &o) {
+    dev_to_f64<L>(v, dst, e, o);
+}
+template <class L>
+__device__ __forceinline__ void get_state(float *v, const double *src, const Enc &e, const Sh4 &o) {
+    f64_to_dev<L>(src, v, e, o);
 }
 
 __device__ __forceinline__ void coords(int n, const Grid &g, int &i, int &j, int &k) {
@@ -229,16 +317,35 @@ __device__ __forceinline__ int nboundary(int i, int j, int k, const Grid &g, int
 }
 
 // Post-stream, post-boundary state of node n; SMOOTH operators are skipped (callers that own
-// smoothed nodes apply them).  Returns nothing for interior nodes beyond the pull.
+// smoothed nodes apply them).  FP64: the populations.  FP32: their deviations from node n's shift
+// (oa, ob); the boundary operators run on FP64 populations.
 template <class LA, class LB>
 __device__ __forceinline__ void node_state(const double *sa, const double *sb, const Grid &g, const ProgView &pv, int n,
-                                           double *a, double *b) {
+                                           double *a, double *b, const Enc &, const Enc &, const Sh4 &, const Sh4 &) {
     int i, j, k;
     coords(n, g, i, j, k);
     load_pull<LA>(sa, g, i, j, k, n, pv.pull[0], a);
     load_pull<LB>(sb, g, i, j, k, n, pv.pull[1], b);
     int slot = boundary_slot(i, j, k, g.nx, g.ny, g.nz, LA::nd);
     if (slot >= 0 && slot < pv.nslot) run_slot_ops<LA, LB>(pv, slot, n, a, b);
+}
+template <class LA, class LB>
+__device__ __forceinline__ void node_state(const float *sa, const float *sb, const Grid &g, const ProgView &pv, int n,
+                                           float *a, float *b, const Enc &ea, const Enc &eb, const Sh4 &oa,
+                                           const Sh4 &ob) {
+    int i, j, k;
+    coords(n, g, i, j, k);
+    load_pull<LA>(sa, g, i, j, k, n, pv.pull[0], a);
+    load_pull<LB>(sb, g, i, j, k, n, pv.pull[1], b);
+    int slot = boundary_slot(i, j, k, g.nx, g.ny, g.nz, LA::nd);
+    if (slot >= 0 && slot < pv.nslot) {
+        double xa[LA::nc], xb[LB::nc > 0 ? LB::nc : 1];
+        dev_to_f64<LA>(a, xa, ea, oa);
+        if constexpr (LB::nc > 0) dev_to_f64<LB>(b, xb, eb, ob);
+        run_slot_ops<LA, LB>(pv, slot, n, xa, xb);
+        f64_to_dev<LA>(xa, a, ea, oa);
+        if constexpr (LB::nc > 0) f64_to_dev<LB>(xb, b, eb, ob);
+    }
 }
 
 }  // namespace lbm

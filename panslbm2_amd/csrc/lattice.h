@@ -8,11 +8,42 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 #include <utility>
 
 namespace lbm {
 
 enum Kind : int { kD2Q9 = 0, kD3Q15 = 1, kD2Q5 = 2, kD3Q7 = 3 };
+
+// Population storage.  F64: the reference's arithmetic in FP64, value-identical to it.  F32:
+// populations stored in FP32 as deviations from a shift E_c (Enc below) and collided in FP32 on
+// those deviations (the *_d forms in collide.h), so no FP32 sum carries the rest-state part of a
+// population.
+enum Prec : int { kF64 = 0, kF32 = 1 };
+template <int P>
+struct PrecT;
+template <>
+struct PrecT<kF64> {
+    using S = double;  // storage
+    using R = double;  // arithmetic
+};
+template <>
+struct PrecT<kF32> {
+    using S = float;
+    using R = float;
+};
+template <class R>
+inline constexpr bool is_f64 = std::is_same_v<R, double>;
+
+// Shift of FP32-stored populations: population c is stored as f_c - E_c, E_c = rest * w_c: the
+// rest state (rest = 1, set by the NS initial condition) or none (rest = 0).  FP64 storage: none.
+// Sh4 holds a shift as the moments (a, b) of E_c = w_c (a + 3 c.b); here (rest, 0).
+struct Sh4 {
+    float a, bx, by, bz;
+};
+struct Enc {
+    double rest;
+};
 
 struct D2Q9 {
     static constexpr int kind = kD2Q9, nc = 9, nd = 2;
@@ -60,18 +91,18 @@ __host__ __device__ __forceinline__ void sfor(F &&f) {
 
 // s + C*v for a lattice component C in {-1, 0, 1}: the same value the reference's
 // `s += C*v` produces (adding C*v = +-0 leaves s unchanged up to the sign of zero).
-template <int C>
-__device__ __forceinline__ double acc(double s, double v) {
+template <int C, class R>
+__device__ __forceinline__ R acc(R s, R v) {
     if constexpr (C == 1) return s + v;
     else if constexpr (C == -1) return s - v;
     else return s;
 }
 // C*v for C in {-1, 0, 1} (C == 0 gives +0.0, the reference's 0*v up to the sign of zero)
-template <int C>
-__device__ __forceinline__ double cmul(double v) {
+template <int C, class R>
+__device__ __forceinline__ R cmul(R v) {
     if constexpr (C == 1) return v;
     else if constexpr (C == -1) return -v;
-    else return 0.0;
+    else return R(0);
 }
 
 // ---- grid geometry ------------------------------------------------------------------------

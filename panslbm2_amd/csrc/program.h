@@ -48,6 +48,7 @@ struct FieldSet {
     const double *rho, *ux, *uy, *uz, *tem, *diff;
 };
 constexpr int kMaxFieldSets = 4;
+constexpr size_t kXcdRemapMaxBytes = size_t(640) << 20;  // population bytes read + written per step
 constexpr int kOpBatch = 4;  // operators fetched per batch by the fused kernels
 
 // Device view of a program.
@@ -63,6 +64,7 @@ struct ProgView {
     FieldSet fs[kMaxFieldSets];
     int pull[2];            // per kernel lattice: +1 stream (pull from x - c), -1 iStream (x + c), 0 none
     int latswap;            // 1 when the program's lattice 0 is the kernel's second lattice
+    int xcd;                // 1: workgroup -> tile remap that keeps consecutive tiles on one XCD
     // 2D corner smoothing: the fused pass saves the post-boundary state of each corner's inward
     // neighbours here (kSideStride doubles each: the kernel's first lattice, then its second) and a
     // corner kernel launched after it averages them.  nullptr when unused.

@@ -27,7 +27,7 @@ import time
 
 import numpy as np
 
-from . import D2Q5, D2Q9, NC, Context, Lattice, check, hptr, to_face
+from . import D2Q5, D2Q9, F32, F64, NC, Context, Lattice, check, hptr, to_face
 
 D = ctypes.c_double
 VP = ctypes.c_void_p
@@ -36,7 +36,7 @@ VP = ctypes.c_void_p
 class Heatsink2D:
     def __init__(self, ctx=None, lx=200, ly=400, ad_kind=D2Q5, seed=0, nu=0.1, pr=6.0, umax=0.05, heat=1e-3,
                  alpha_max=1e4, qf=0.01, qg=1.0, filter_radius=3.0, eps=1e-6, check_every=100, max_steps=400000,
-                 use_graphs=True):
+                 use_graphs=True, prec=F64, fp32_floor=1e-5):
         self.ctx = ctx or Context(0)
         self.L = self.ctx.lib
         self.lx, self.ly, self.N = lx, ly, lx * ly
@@ -78,8 +78,12 @@ class Heatsink2D:
         self.d["one"] = c.array((N,), fill=1.0)
         self.d["zero"] = c.array((N,), fill=0.0)
 
-        self.pf = Lattice(c, D2Q9, lx, ly)
-        self.pg = Lattice(c, ad_kind, lx, ly)
+        # prec: population storage of both lattices.  F32: each solve runs in FP32 until its residual
+        # stops falling, then converts the lattices to FP64 and finishes at eps (mixed precision).
+        self.prec = prec
+        self.fp32_floor = fp32_floor
+        self.pf = Lattice(c, D2Q9, lx, ly, prec=prec)
+        self.pg = Lattice(c, ad_kind, lx, ly, prec=prec)
 
         # boundary lambdas, tabulated
         H = ly - 1.0
@@ -198,34 +202,65 @@ class Heatsink2D:
         check(self.L.lbm_residual(self.ctx.h, 2, a0.ptr, a1.ptr, None, b0.ptr, b1.ptr, None, self.N, ctypes.byref(r)))
         return r.value
 
+    def _set_prec(self, prec):
+        if self.pf.prec == prec:
+            return
+        for lat in (self.pf, self.pg):
+            check(self.L.lbm_lattice_set_precision(lat.h, prec))
+            lat.prec = prec
+        for g in self._graphs.values():  # captured with the old buffers
+            self.L.lbm_graph_destroy(g)
+        self._graphs = {}
+
+    def _solve(self, key, step, res):
+        """Chunks of check_every steps until both residuals are below eps.  In FP32 the residual is
+        watched: once it is below fp32_floor, or below 10 fp32_floor without falling by 5 % over
+        ten checks, the lattices convert to FP64 and the solve continues there.  Returns (steps,
+        steps in FP32, final residuals)."""
+        ce = self.check_every
+        t, t32, best, stall = 0, 0, np.inf, 0
+        r = (np.inf, np.inf)
+        while t < self.max_steps:
+            if t == 0:
+                for s in range(1, ce + 1):
+                    step(s)
+            else:
+                self._chunk(key, t, step)
+            t += ce
+            r = res(t)
+            worst = max(r)
+            if worst < self.eps:
+                break
+            if self.pf.prec == F32:
+                t32 = t
+                stall = 0 if worst < 0.95 * best else stall + 1
+                best = min(best, worst)
+                if worst < self.fp32_floor or (worst < 10.0 * self.fp32_floor and stall >= 10):
+                    self._set_prec(F64)
+        return t, (t32 if self.prec == F32 else 0), r
+
     def forward(self):
         L, d = self.L, self.d
+        self._set_prec(self.prec)
         for k in ("ux0", "uy0"):
             d[k].fill(0.0)
         d["tem"].fill(0.0)
         check(L.lbm_ns_initial_condition(self.pf.h, d["one"].ptr, d["ux0"].ptr, d["uy0"].ptr, None))
         check(L.lbm_ad_initial_condition(self.pg.h, d["tem"].ptr, d["ux0"].ptr, d["uy0"].ptr, None))
-        ce = self.check_every
-        t = 0
-        while t < self.max_steps:
-            if t == 0:
-                for s in range(1, ce + 1):
-                    self._fwd_step(s)
-            else:
-                self._chunk("fwd", t, self._fwd_step)
-            t += ce
+
+        def res(t):
             b, pb = t % 2, (t - 1) % 2
-            ru = self._residual(d["ux%d" % b], d["uy%d" % b], d["ux%d" % pb], d["uy%d" % pb])
-            rq = self._residual(d["qx%d" % b], d["qy%d" % b], d["qx%d" % pb], d["qy%d" % pb])
-            if ru < self.eps and rq < self.eps:
-                break
+            return (self._residual(d["ux%d" % b], d["uy%d" % b], d["ux%d" % pb], d["uy%d" % pb]),
+                    self._residual(d["qx%d" % b], d["qy%d" % b], d["qx%d" % pb], d["qy%d" % pb]))
+
+        t, self.fwd_steps32, self.fwd_res = self._solve("fwd", self._fwd_step, res)
         self.fwd_steps = t
         self.fwd_buf = t % 2
-        self.fwd_res = (ru, rq)
         return t
 
     def adjoint(self):
         L, d = self.L, self.d
+        self._set_prec(self.prec)
         fb = self.fwd_buf
         if self.adj_prog is None or getattr(self, "_adj_fb", None) != fb:
             if self.adj_prog is not None:
@@ -241,24 +276,15 @@ class Heatsink2D:
                                           d["iux0"].ptr, d["iuy0"].ptr, None))
         check(L.lbm_aad_initial_condition(self.pg.h, d["ux%d" % fb].ptr, d["uy%d" % fb].ptr, None, d["item"].ptr,
                                           d["iqx0"].ptr, d["iqy0"].ptr, None))
-        ce = self.check_every
-        step = lambda s: self._adj_step(s, fb)
-        t = 0
-        while t < self.max_steps:
-            if t == 0:
-                for s in range(1, ce + 1):
-                    step(s)
-            else:
-                self._chunk("adj", t, step)
-            t += ce
+
+        def res(t):
             b, pb = t % 2, (t - 1) % 2
-            ru = self._residual(d["iux%d" % b], d["iuy%d" % b], d["iux%d" % pb], d["iuy%d" % pb])
-            rq = self._residual(d["iqx%d" % b], d["iqy%d" % b], d["iqx%d" % pb], d["iqy%d" % pb])
-            if ru < self.eps and rq < self.eps:
-                break
+            return (self._residual(d["iux%d" % b], d["iuy%d" % b], d["iux%d" % pb], d["iuy%d" % pb]),
+                    self._residual(d["iqx%d" % b], d["iqy%d" % b], d["iqx%d" % pb], d["iqy%d" % pb]))
+
+        t, self.adj_steps32, self.adj_res = self._solve("adj", lambda s: self._adj_step(s, fb), res)
         self.adj_steps = t
         self.adj_buf = t % 2
-        self.adj_res = (ru, rq)
         return t
 
     def sensitivity(self):
@@ -284,7 +310,8 @@ class Heatsink2D:
         self.ctx.sync()
         dt = time.perf_counter() - t0
         return {"seconds": dt, "fwd_steps": self.fwd_steps, "adj_steps": self.adj_steps,
-                "objective": self.objective, "fwd_residual": self.fwd_res, "adj_residual": self.adj_res}
+                "fp32_steps": (self.fwd_steps32, self.adj_steps32), "objective": self.objective,
+                "fwd_residual": self.fwd_res, "adj_residual": self.adj_res}
 
     def close(self):
         for g in self._graphs.values():

@@ -17,10 +17,11 @@ VP = ctypes.c_void_p
 class GpuBackend:
     name = "gpu"
 
-    def __init__(self, ctx=None):
+    def __init__(self, ctx=None, prec=pa.F64):
         self.ctx = ctx or pa.Context(0)
         self.L = self.ctx.lib
         self.fields = {}
+        self.prec = prec
 
     # ---- fields -------------------------------------------------------------------------
     def put(self, name, arr):
@@ -42,7 +43,7 @@ class GpuBackend:
 
     # ---- lattices -----------------------------------------------------------------------
     def lattice(self, kind, lx, ly, lz=1):
-        return pa.Lattice(self.ctx, kind, lx, ly, lz)
+        return pa.Lattice(self.ctx, kind, lx, ly, lz, prec=self.prec)
 
     def pops(self, h):
         return h.download()
@@ -187,8 +188,8 @@ class FusedGpuBackend(GpuBackend):
 
     name = "gpu-fused"
 
-    def __init__(self, ctx=None):
-        super().__init__(ctx)
+    def __init__(self, ctx=None, prec=pa.F64):
+        super().__init__(ctx, prec)
         self._recording = False
         self._pending = None
         self.fused_calls = 0

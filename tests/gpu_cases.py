@@ -10,12 +10,12 @@ from _oracle import NC, ND
 from refcases import NC_G, NC_TEM0
 
 
-def run_gpu(ctx, case, op, inp):
+def run_gpu(ctx, case, op, inp, prec=pa.F64):
     L = ctx.lib
     k = case.kind
     N = case.N
-    p = pa.Lattice(ctx, k, case.lx, case.ly, case.lz)
-    q = pa.Lattice(ctx, k, case.lx, case.ly, case.lz)
+    p = pa.Lattice(ctx, k, case.lx, case.ly, case.lz, prec=prec)
+    q = pa.Lattice(ctx, k, case.lx, case.ly, case.lz, prec=prec)
     p.upload(inp["f"])
     q.upload(inp["g"])
     F = lambda a, dt=np.float64: pa.to_face(k, case.lx, case.ly, case.lz, a, dt)
