@@ -266,6 +266,33 @@ int lbm_aad_macro_brinkman_collide_natural_convection_p(
     double viscosity, lbm_lattice *q, const double *tem, double *item, double *iqx, double *iqy, double *iqz,
     const double *diffusivity, double gx, double gy, double gz, int issave, double *ig);
 
+/* ---- batches (no reference counterpart; the reference runs one design per process): the fused
+ * steps of many independent lattice instances in one kernel launch.  Between lbm_batch_begin and
+ * lbm_batch_end, *_collide_p calls with fusable programs are collected, not run (all of one
+ * equation, lattice pair and storage; each instance keeps its own grid, program, buffers and
+ * fields).  A collected call swaps its lattices' buffers as a run call would, so collect a cycle of
+ * steps that leaves every binding as it found it (e.g. an even and an odd step, two batches) and
+ * launch them in that order.  The programs and field arrays must outlive the batch. ---- */
+typedef struct lbm_batch lbm_batch;
+int lbm_batch_begin(lbm_context *ctx);
+int lbm_batch_end(lbm_context *ctx, lbm_batch **out);
+int lbm_batch_size(const lbm_batch *batch);
+int lbm_batch_launch(lbm_batch *batch);
+/* per instance (collection order) 1 = step, 0 = frozen: its state and buffers stay as they are, so
+ * freeze at a point of the cycle where the instance's buffer bookkeeping matches (e.g. after an even
+ * number of steps); not during graph capture (graphs replay with the current flags) */
+int lbm_batch_set_active(lbm_batch *batch, const int *active);
+/* workgroup size 64 / 128 / 256 and the XCD-contiguous tile remap (0 / 1); not during graph capture */
+int lbm_batch_configure(lbm_batch *batch, int block, int xcd);
+/* autotuner: times `reps` rounds of the batches (launched in order) for each configuration, keeps
+ * the fastest on all of them.  The timed rounds are real steps: the cycle advances 2 * 3 * reps + 6
+ * rounds. */
+int lbm_batch_tune(lbm_batch *const *batches, int nbatch, int reps, int *block_out, int *xcd_out);
+int lbm_batch_destroy(lbm_batch *batch);
+/* Residual (residual.h:24-51) of n array sets at once: a[3*i + k], b[3*i + k] for k < ncomp */
+int lbm_residual_batch(lbm_context *ctx, int n, int ncomp, const double *const *a, const double *const *b,
+                       const int *len, double *out);
+
 /* ---- HIP graphs of the context stream (no reference counterpart; the reference loops on the
  * host).  Calls between begin and end are captured, not run; they must not return host data.
  * Trap: a captured sequence must leave every buffer binding as it found it (e.g. an even number

@@ -56,6 +56,16 @@ def load():
         "lbm_lattice_create_prec": [VP, I, I, I, I, I, ctypes.POINTER(VP)],
         "lbm_lattice_precision": [VP],
         "lbm_lattice_set_precision": [VP, I],
+        "lbm_batch_begin": [VP],
+        "lbm_batch_end": [VP, ctypes.POINTER(VP)],
+        "lbm_batch_size": [VP],
+        "lbm_batch_launch": [VP],
+        "lbm_batch_set_active": [VP, VP],
+        "lbm_batch_configure": [VP, I, I],
+        "lbm_batch_tune": [ctypes.POINTER(VP), I, I, ctypes.POINTER(I), ctypes.POINTER(I)],
+        "lbm_batch_destroy": [VP],
+        "lbm_residual_batch": [VP, I, I, ctypes.POINTER(VP), ctypes.POINTER(VP), ctypes.POINTER(I),
+                               ctypes.POINTER(D)],
         "lbm_lattice_destroy": [VP],
         "lbm_lattice_nc": [VP],
         "lbm_lattice_nxyz": [VP],

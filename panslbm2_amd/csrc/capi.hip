@@ -81,16 +81,29 @@ struct StageBuf : DevBuf {
 
 struct lbm_program;
 
+struct lbm_batch;
+
 struct lbm_context {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[8] = {};
     lbm_program *recording = nullptr;
+    lbm_batch *batching = nullptr;  // between lbm_batch_begin and lbm_batch_end
     double *d_part = nullptr;  // reduction partials
     double *d_out = nullptr;   // reduction results
     double *h_out = nullptr;   // pinned
     DevBuf scratch;            // index lists for gather / filter weights
+    DevBuf multi;              // batched residual items and partials
 };
+
+// Collected fused steps of many instances, launched as one kernel (lbm_batch_*).
+struct lbm_batch {
+    lbm_context *ctx = nullptr;
+    lbm::BatchSink sink;
+    DevBuf d_items, d_tiles, d_active;
+    int block = 256, xcd = 0, total = 0;  // workgroup size, XCD remap, workgroups in the launch
+};
+static lbm::BatchSink *batch_sink(lbm_context *C) { return C->batching ? &C->batching->sink : nullptr; }
 
 struct lbm_lattice {
     lbm_context *ctx;
@@ -491,6 +504,7 @@ int collide_call(lbm_program *P, lbm_lattice *p, lbm_lattice *q, std::initialize
     if (q && q->prec != p->prec) return fail(LBM_ERR_ARG, "coupled lattices must have the same storage precision");
     lbm_context *C = p->ctx;
     if (C->recording) return fail(LBM_ERR_STATE, "collide called while recording a program");
+    lbm::BatchSink *sink = batch_sink(C);
     lbm::CollideLaunch cl{};
     cl.ka = p->kind;
     cl.kb = q ? q->kind : -1;
@@ -515,6 +529,7 @@ int collide_call(lbm_program *P, lbm_lattice *p, lbm_lattice *q, std::initialize
         }
     }
     int rc;
+    if (sink && !fused) return fail(LBM_ERR_UNSUPPORTED, "a batched collide needs a fusable program");
     if (!fused) {
         cl.da = p->buf[p->cur];
         cl.db = q ? q->buf[q->cur] : nullptr;
@@ -559,9 +574,15 @@ int collide_call(lbm_program *P, lbm_lattice *p, lbm_lattice *q, std::initialize
         }
     }
     cl.any_smooth = (P->smoothed[0] || P->smoothed[1]) ? 1 : 0;
+    if (sink) {
+        if (cl.n_corner2d > 0)
+            return fail(LBM_ERR_UNSUPPORTED, "a batched collide cannot smooth corners from fields it saves");
+        v.xcd = 0;  // the batch decides
+        cl.batch = sink;
+    }
     rc = launch(cl);
     if (rc) return rc;
-    swap_buf(p);
+    swap_buf(p);  // a collected step swaps too: the next collected step reads what this one writes
     if (q) swap_buf(q);
     return LBM_OK;
 }
@@ -1310,6 +1331,157 @@ int lbm_graph_destroy(lbm_graph *G) {
     delete G;
     return LBM_OK;
 }
+// ---- batches ----------------------------------------------------------------------------------
+static int batch_retile(lbm_batch *B) {
+    const lbm::BatchSink &k = B->sink;
+    int n = (int)k.nodes.size();
+    std::vector<int2> t(n);
+    int start = 0;
+    for (int i = 0; i < n; ++i) {
+        int nb = (k.nodes[i] + B->block - 1) / B->block + k.corner[i];
+        t[i] = make_int2(start, nb);
+        start += (nb + 7) & ~7;  // instances start on multiples of 8 workgroups (one per XCD)
+    }
+    B->total = start;
+    if (B->d_tiles.ensure(sizeof(int2) * n)) return fail(LBM_ERR_HIP, "batch tile table allocation failed");
+    HIPCHK(hipStreamSynchronize(B->ctx->stream));
+    HIPCHK(hipMemcpy(B->d_tiles.p, t.data(), sizeof(int2) * n, hipMemcpyHostToDevice));
+    return LBM_OK;
+}
+int lbm_batch_begin(lbm_context *C) {
+    if (!C) return fail(LBM_ERR_ARG, "null context");
+    if (C->recording || C->batching) return fail(LBM_ERR_STATE, "already recording or collecting a batch");
+    C->batching = new lbm_batch;
+    C->batching->ctx = C;
+    return LBM_OK;
+}
+int lbm_batch_end(lbm_context *C, lbm_batch **out) {
+    if (!C || !out) return fail(LBM_ERR_ARG, "null argument");
+    *out = nullptr;
+    lbm_batch *B = C->batching;
+    if (!B) return fail(LBM_ERR_STATE, "no batch begun");
+    C->batching = nullptr;
+    if (B->sink.nodes.empty()) {
+        delete B;
+        return fail(LBM_ERR_STATE, "empty batch");
+    }
+    int rc = LBM_OK;
+    if (B->d_items.ensure(B->sink.bytes.size())) rc = fail(LBM_ERR_HIP, "batch item allocation failed");
+    if (!rc && hipMemcpy(B->d_items.p, B->sink.bytes.data(), B->sink.bytes.size(), hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(LBM_ERR_HIP, "batch item upload failed");
+    if (!rc) rc = batch_retile(B);
+    if (!rc) {
+        std::vector<int> on(B->sink.nodes.size(), 1);
+        rc = lbm_batch_set_active(B, on.data());
+    }
+    if (rc) {
+        lbm_batch_destroy(B);
+        return rc;
+    }
+    *out = B;
+    return LBM_OK;
+}
+int lbm_batch_size(const lbm_batch *B) { return B ? (int)B->sink.nodes.size() : fail(LBM_ERR_ARG, "null batch"); }
+int lbm_batch_launch(lbm_batch *B) {
+    if (!B) return fail(LBM_ERR_ARG, "null batch");
+    if (B->ctx->recording || B->ctx->batching) return fail(LBM_ERR_STATE, "batch launched while recording");
+    return hip_result(B->sink.launch(B->d_items.p, (const int2 *)B->d_tiles.p, (const int *)B->d_active.p,
+                                     (int)B->sink.nodes.size(), B->total, B->block, B->xcd, B->ctx->stream),
+                      "batch launch");
+}
+int lbm_batch_set_active(lbm_batch *B, const int *active) {
+    if (!B || !active) return fail(LBM_ERR_ARG, "null argument");
+    size_t n = B->sink.nodes.size();
+    if (B->d_active.ensure(sizeof(int) * n)) return fail(LBM_ERR_HIP, "batch flag allocation failed");
+    HIPCHK(hipStreamSynchronize(B->ctx->stream));
+    HIPCHK(hipMemcpy(B->d_active.p, active, sizeof(int) * n, hipMemcpyHostToDevice));
+    return LBM_OK;
+}
+int lbm_batch_configure(lbm_batch *B, int block, int xcd) {
+    if (!B || (block != 64 && block != 128 && block != 256) || (xcd != 0 && xcd != 1))
+        return fail(LBM_ERR_ARG, "bad batch configuration");
+    B->block = block;
+    B->xcd = xcd;
+    return batch_retile(B);
+}
+int lbm_batch_tune(lbm_batch *const *batches, int nbatch, int reps, int *block_out, int *xcd_out) {
+    if (!batches || nbatch < 1 || reps < 1) return fail(LBM_ERR_ARG, "bad arguments");
+    lbm_context *C = batches[0]->ctx;
+    const int blocks[3] = {64, 128, 256};
+    float best = 0.0f;
+    int bb = 256, bx = 0;
+    for (int xcd = 0; xcd <= 1; ++xcd)
+        for (int blk : blocks) {
+            for (int i = 0; i < nbatch; ++i) {
+                int rc = lbm_batch_configure(batches[i], blk, xcd);
+                if (rc) return rc;
+            }
+            for (int i = 0; i < nbatch; ++i) {  // warm-up round
+                int rc = lbm_batch_launch(batches[i]);
+                if (rc) return rc;
+            }
+            HIPCHK(hipEventRecord(C->ev[6], C->stream));
+            for (int r = 0; r < reps; ++r)
+                for (int i = 0; i < nbatch; ++i) {
+                    int rc = lbm_batch_launch(batches[i]);
+                    if (rc) return rc;
+                }
+            HIPCHK(hipEventRecord(C->ev[7], C->stream));
+            HIPCHK(hipEventSynchronize(C->ev[7]));
+            float ms = 0.0f;
+            HIPCHK(hipEventElapsedTime(&ms, C->ev[6], C->ev[7]));
+            if (best == 0.0f || ms < best) {
+                best = ms;
+                bb = blk;
+                bx = xcd;
+            }
+        }
+    for (int i = 0; i < nbatch; ++i) {
+        int rc = lbm_batch_configure(batches[i], bb, bx);
+        if (rc) return rc;
+    }
+    if (block_out) *block_out = bb;
+    if (xcd_out) *xcd_out = bx;
+    return LBM_OK;
+}
+int lbm_batch_destroy(lbm_batch *B) {
+    if (!B) return LBM_OK;
+    (void)hipStreamSynchronize(B->ctx->stream);
+    B->d_items.release();
+    B->d_tiles.release();
+    B->d_active.release();
+    delete B;
+    return LBM_OK;
+}
+int lbm_residual_batch(lbm_context *C, int n, int ncomp, const double *const *a, const double *const *b,
+                       const int *len, double *out) {
+    if (!C || n < 1 || ncomp < 1 || ncomp > 3 || !a || !b || !len || !out) return fail(LBM_ERR_ARG, "bad arguments");
+    std::vector<lbm::ResItem> items(n);
+    for (int i = 0; i < n; ++i) {
+        for (int k = 0; k < 3; ++k) {
+            items[i].a[k] = k < ncomp ? a[3 * i + k] : nullptr;
+            items[i].b[k] = k < ncomp ? b[3 * i + k] : nullptr;
+            if (k < ncomp && (!items[i].a[k] || !items[i].b[k])) return fail(LBM_ERR_ARG, "null field");
+        }
+        items[i].n = len[i];
+        items[i].ncomp = ncomp;
+    }
+    const int rb = lbm::kResidualBlocksPerItem;
+    size_t item_bytes = sizeof(lbm::ResItem) * n, part_bytes = sizeof(double) * 2 * (size_t)n * rb;
+    if (C->multi.ensure(item_bytes + part_bytes + sizeof(double) * 2 * n))
+        return fail(LBM_ERR_HIP, "residual scratch allocation failed");
+    char *base = (char *)C->multi.p;
+    HIPCHK(hipStreamSynchronize(C->stream));
+    HIPCHK(hipMemcpy(base, items.data(), item_bytes, hipMemcpyHostToDevice));
+    double *part = (double *)(base + item_bytes), *res = part + 2 * (size_t)n * rb;
+    HIPCHK(lbm::launch_residual_multi((const lbm::ResItem *)base, n, part, res, C->stream));
+    std::vector<double> h(2 * n);
+    HIPCHK(hipMemcpyAsync(h.data(), res, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, C->stream));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    for (int i = 0; i < n; ++i) out[i] = std::sqrt(h[2 * i] / h[2 * i + 1]);
+    return LBM_OK;
+}
+
 int lbm_program_num_field_sets(const lbm_program *P) { return P ? (int)P->fsets.size() : fail(LBM_ERR_ARG, "null"); }
 int lbm_program_equal(const lbm_program *a, const lbm_program *b) {
     if (!a || !b) return fail(LBM_ERR_ARG, "null program");

@@ -482,22 +482,24 @@ __global__ void __launch_bounds__(kBlock) k_collide_inplace(S *a, S *b, Grid g, 
     store_raw<LB>(b, N, n, fb);
 }
 
+// One workgroup's share of a fused step: workgroup blk of the nblk that cover the grid (the last
+// is the corner workgroup when pv.corner_block); xcd: remap workgroups to XCD-contiguous tiles.
 template <class LA, class LB, class Body, class S>
-__global__ void __launch_bounds__(kBlock) k_collide_fused(const S *__restrict__ sa, const S *__restrict__ sb,
-                                                          S *__restrict__ da, S *__restrict__ db, Grid g, ProgView pv,
-                                                          int any_smooth, Body body) {
+__device__ __forceinline__ void fused_block(const S *__restrict__ sa, const S *__restrict__ sb, S *__restrict__ da,
+                                            S *__restrict__ db, const Grid &g, const ProgView &pv, int any_smooth,
+                                            const Body &body, int blk, int nblk, int xcd) {
     using R = RegOf<S>;
     constexpr int NB = LB::nc > 0 ? LB::nc : 1;
     // the corner workgroup (last in the grid when pv.corner_block): lanes [0, nside) build the
     // corners' inward neighbours' states, lanes [nside, nside + ncorner) the corners
-    const bool cblk = pv.corner_block && blockIdx.x == gridDim.x - 1;
+    const bool cblk = pv.corner_block && blk == nblk - 1;
     int n, nb, side = -1;
     if (cblk) {
         int t = threadIdx.x;
         n = t < pv.nside ? pv.side_nodes[t] : (t < pv.nside + pv.ncorner ? pv.corners[t - pv.nside].x : -1);
         nb = 2;
     } else {
-        int tile = pv.xcd ? xcd_tile(blockIdx.x, gridDim.x - pv.corner_block) : (int)blockIdx.x;
+        int tile = xcd ? xcd_tile(blk, nblk - pv.corner_block) : blk;
         n = tile * blockDim.x + threadIdx.x;
         if (n >= g.N) return;
         int i, j, k;
@@ -538,6 +540,51 @@ __global__ void __launch_bounds__(kBlock) k_collide_fused(const S *__restrict__ 
     body(fa, fb, n, nb > 0, oa, ob);
     store_raw<LA>(da, g.N, n, fa);
     store_raw<LB>(db, g.N, n, fb);
+}
+template <class LA, class LB, class Body, class S>
+__global__ void __launch_bounds__(kBlock) k_collide_fused(const S *__restrict__ sa, const S *__restrict__ sb,
+                                                          S *__restrict__ da, S *__restrict__ db, Grid g, ProgView pv,
+                                                          int any_smooth, Body body) {
+    fused_block<LA, LB, Body, S>(sa, sb, da, db, g, pv, any_smooth, body, blockIdx.x, gridDim.x, pv.xcd);
+}
+
+// Batched fused steps: the workgroups of many lattice instances (each its own grid, program,
+// buffers and collide arguments, all of one equation, lattice pair and storage) in one launch.
+// Instance i owns workgroups [tiles[i].x, tiles[i].x + tiles[i].y); starts are multiples of 8 so
+// that a workgroup's XCD (its index mod 8) is the same counted from the instance's start.  An
+// instance with active[i] == 0 is frozen: its workgroups return at once and its state stays.
+template <class Body, class S>
+struct BatchItem {
+    const S *sa, *sb;
+    S *da, *db;
+    Grid g;
+    ProgView pv;
+    Body body;
+    int any_smooth;
+};
+template <class LA, class LB, class Body, class S>
+__global__ void __launch_bounds__(kBlock) k_collide_batch(const BatchItem<Body, S> *__restrict__ items,
+                                                          const int2 *__restrict__ tiles,
+                                                          const int *__restrict__ active, int nitem, int xcd) {
+    const int b = blockIdx.x;
+    int lo = 0, hi = nitem - 1;  // last instance whose first workgroup is <= b
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (tiles[mid].x <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const int2 t = tiles[lo];
+    const int blk = b - t.x;
+    if (blk >= t.y || !active[lo]) return;  // alignment padding between instances, or frozen
+    const BatchItem<Body, S> &it = items[lo];
+    fused_block<LA, LB, Body, S>(it.sa, it.sb, it.da, it.db, it.g, it.pv, it.any_smooth, it.body, blk, t.y, xcd);
+}
+template <class LA, class LB, class Body, class S>
+static hipError_t launch_batch_t(const void *items, const int2 *tiles, const int *active, int nitem, int nblocks_total,
+                                 int block, int xcd, hipStream_t s) {
+    hipLaunchKernelGGL((k_collide_batch<LA, LB, Body, S>), dim3(nblocks_total), dim3(block), 0, s,
+                       static_cast<const BatchItem<Body, S> *>(items), tiles, active, nitem, xcd);
+    return hipGetLastError();
 }
 
 // 2D smoothed corners after the fused pass: pull, the corner's own operators in record order, a
@@ -675,6 +722,20 @@ static hipError_t launch_collide(const CollideLaunch &cl, Make &&make) {
         }
         const S *sa = static_cast<const S *>(cl.sa), *sb = static_cast<const S *>(cl.sb);
         const ProgView &pv = *cl.pv;
+        if (cl.batch) {  // collect the step instead of launching it
+            if (cl.n_corner2d > 0) return hipErrorNotSupported;
+            BatchSink &k = *cl.batch;
+            BatchItem<B, S> it{sa, sb, da, db, g, pv, body, cl.any_smooth};
+            BatchSink::LaunchFn fn = &launch_batch_t<LA, LB, B, S>;
+            if (k.launch && k.launch != fn) return hipErrorInvalidValue;  // one kernel per batch
+            k.launch = fn;
+            k.item_size = sizeof(it);
+            const unsigned char *p = reinterpret_cast<const unsigned char *>(&it);
+            k.bytes.insert(k.bytes.end(), p, p + sizeof(it));
+            k.nodes.push_back(g.N);
+            k.corner.push_back(pv.corner_block);
+            return hipSuccess;
+        }
         LBM_LAUNCH((k_collide_fused<LA, LB, B, S>), nblocks(g.N) + pv.corner_block, cl.stream, sa, sb, da, db, g, pv,
                    cl.any_smooth, body);
         if (cl.n_corner2d > 0) {

@@ -2,10 +2,23 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "lattice.h"
 #include "program.h"
 
 namespace lbm {
+
+// Fused steps collected for one batched launch (k_collide_batch): the items' bytes, per item its
+// node count and corner workgroup, and the launcher of the one kernel instantiation they share.
+struct BatchSink {
+    using LaunchFn = hipError_t (*)(const void *items, const int2 *tiles, const int *active, int nitem,
+                                    int nblocks_total, int block, int xcd, hipStream_t s);
+    std::vector<unsigned char> bytes;
+    size_t item_size = 0;
+    std::vector<int> nodes, corner;
+    LaunchFn launch = nullptr;
+};
 
 struct CollideLaunch {
     int ka, kb;                // lattice kinds of the kernel's lattices (kb < 0: single lattice)
@@ -19,6 +32,7 @@ struct CollideLaunch {
     const int4 *corner2d;      // 2D smoothed corners (node, side index of x-neighbour, of y-neighbour, -1),
     int n_corner2d;            // done after the fused pass from the side states it saved
     int any_smooth;
+    BatchSink *batch;          // non-null: collect the fused step instead of launching it
 };
 
 struct NSArgs {
@@ -85,6 +99,13 @@ hipError_t launch_sens_hs_boundary(double *dfds, int N, const ProgView &pv, cons
                                    const double *ig, const double *diff, const double *dkds, hipStream_t s);
 hipError_t launch_residual(int ncomp, const double *const *a, const double *const *b, int n, double *part,
                            double *out, hipStream_t s);
+// Residual sums of many arrays sets in one launch pair (lbm_residual_batch)
+struct ResItem {
+    const double *a[3], *b[3];
+    int n, ncomp;
+};
+constexpr int kResidualBlocksPerItem = 16;
+hipError_t launch_residual_multi(const ResItem *items, int nitem, double *part, double *out, hipStream_t s);
 hipError_t launch_normalize(double *v, int n, double *part, double *out, hipStream_t s);
 hipError_t launch_gather_sum(const double *v, const int *idx, int count, double *out, hipStream_t s);
 hipError_t launch_fill(double *p, size_t n, double v, hipStream_t s);

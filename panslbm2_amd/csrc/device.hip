@@ -221,6 +221,47 @@ __global__ void __launch_bounds__(kBlock) k_residual_partial(int ncomp, const do
         part[2 * blockIdx.x + 1] = s1[0];
     }
 }
+// Per item: kResidualBlocksPerItem workgroups, each summing a strided share of the item's arrays
+// (sum (a - b)^2, sum a^2), then one workgroup per item adding the partials in a fixed order.
+__global__ void __launch_bounds__(kBlock) k_residual_multi(const ResItem *__restrict__ items, double *part) {
+    __shared__ double s0[kBlock], s1[kBlock];
+    const int rb = kResidualBlocksPerItem;
+    const ResItem it = items[blockIdx.x / rb];
+    const int blk = blockIdx.x % rb;
+    double x0 = 0.0, x1 = 0.0;
+    for (int t = blk * blockDim.x + threadIdx.x; t < it.n; t += rb * blockDim.x) {
+        for (int k = 0; k < it.ncomp; ++k) {
+            double a = it.a[k][t], d = a - it.b[k][t];
+            x0 += d * d;
+            x1 += a * a;
+        }
+    }
+    s0[threadIdx.x] = x0;
+    s1[threadIdx.x] = x1;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            s0[threadIdx.x] += s0[threadIdx.x + w];
+            s1[threadIdx.x] += s1[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = s0[0];
+        part[2 * blockIdx.x + 1] = s1[0];
+    }
+}
+__global__ void k_residual_multi_final(const double *part, double *out) {
+    if (threadIdx.x != 0) return;
+    const int rb = kResidualBlocksPerItem;
+    double x0 = 0.0, x1 = 0.0;
+    for (int j = 0; j < rb; ++j) {
+        x0 += part[2 * (blockIdx.x * rb + j)];
+        x1 += part[2 * (blockIdx.x * rb + j) + 1];
+    }
+    out[2 * blockIdx.x] = x0;
+    out[2 * blockIdx.x + 1] = x1;
+}
 __global__ void __launch_bounds__(kBlock) k_pair_final(const double *part, int nb, double *out) {
     __shared__ double s0[kBlock], s1[kBlock];
     double x0 = 0.0, x1 = 0.0;
@@ -451,6 +492,11 @@ hipError_t launch_residual(int ncomp, const double *const *a, const double *cons
                            double *out, hipStream_t s) {
     LBM_LAUNCH(k_residual_partial, kRedBlocks, s, ncomp, a[0], a[1], a[2], b[0], b[1], b[2], n, part);
     LBM_LAUNCH(k_pair_final, 1, s, part, kRedBlocks, out);
+    return hipGetLastError();
+}
+hipError_t launch_residual_multi(const ResItem *items, int nitem, double *part, double *out, hipStream_t s) {
+    LBM_LAUNCH(k_residual_multi, nitem * kResidualBlocksPerItem, s, items, part);
+    hipLaunchKernelGGL(k_residual_multi_final, dim3(nitem), dim3(64), 0, s, part, out);
     return hipGetLastError();
 }
 hipError_t launch_normalize(double *v, int n, double *part, double *out, hipStream_t s) {

@@ -214,9 +214,9 @@ class Heatsink2D:
 
     def _solve(self, key, step, res):
         """Chunks of check_every steps until both residuals are below eps.  In FP32 the residual is
-        watched: once it is below fp32_floor, or below 10 fp32_floor without falling by 5 % over
-        ten checks, the lattices convert to FP64 and the solve continues there.  Returns (steps,
-        steps in FP32, final residuals)."""
+        watched: once it is below fp32_floor or has not fallen by 5 % in ten checks, the lattices
+        convert to FP64 and the solve continues there.  Returns (steps, steps in FP32, final
+        residuals)."""
         ce = self.check_every
         t, t32, best, stall = 0, 0, np.inf, 0
         r = (np.inf, np.inf)
@@ -229,17 +229,18 @@ class Heatsink2D:
             t += ce
             r = res(t)
             worst = max(r)
-            if worst < self.eps:
+            if worst < self.eps or not np.isfinite(worst):  # converged, or diverged: stop either way
                 break
             if self.pf.prec == F32:
                 t32 = t
                 stall = 0 if worst < 0.95 * best else stall + 1
                 best = min(best, worst)
-                if worst < self.fp32_floor or (worst < 10.0 * self.fp32_floor and stall >= 10):
+                if worst < self.fp32_floor or stall >= 10:
                     self._set_prec(F64)
         return t, (t32 if self.prec == F32 else 0), r
 
-    def forward(self):
+    def init_forward(self):
+        """Cold start of the forward solve: rest state, T = 0."""
         L, d = self.L, self.d
         self._set_prec(self.prec)
         for k in ("ux0", "uy0"):
@@ -248,17 +249,24 @@ class Heatsink2D:
         check(L.lbm_ns_initial_condition(self.pf.h, d["one"].ptr, d["ux0"].ptr, d["uy0"].ptr, None))
         check(L.lbm_ad_initial_condition(self.pg.h, d["tem"].ptr, d["ux0"].ptr, d["uy0"].ptr, None))
 
-        def res(t):
-            b, pb = t % 2, (t - 1) % 2
-            return (self._residual(d["ux%d" % b], d["uy%d" % b], d["ux%d" % pb], d["uy%d" % pb]),
-                    self._residual(d["qx%d" % b], d["qy%d" % b], d["qx%d" % pb], d["qy%d" % pb]))
+    def residual_fields(self, adjoint, t):
+        """(a, b) field pairs of the two residuals after step t: u and q (or their adjoints)."""
+        d = self.d
+        b, pb = t % 2, (t - 1) % 2
+        u, q = (("iux", "iuy"), ("iqx", "iqy")) if adjoint else (("ux", "uy"), ("qx", "qy"))
+        return ((d["%s%d" % (u[0], b)], d["%s%d" % (u[1], b)], d["%s%d" % (u[0], pb)], d["%s%d" % (u[1], pb)]),
+                (d["%s%d" % (q[0], b)], d["%s%d" % (q[1], b)], d["%s%d" % (q[0], pb)], d["%s%d" % (q[1], pb)]))
 
+    def forward(self):
+        self.init_forward()
+        res = lambda t: tuple(self._residual(*f) for f in self.residual_fields(False, t))
         t, self.fwd_steps32, self.fwd_res = self._solve("fwd", self._fwd_step, res)
         self.fwd_steps = t
         self.fwd_buf = t % 2
         return t
 
-    def adjoint(self):
+    def init_adjoint(self):
+        """Adjoint program for the converged forward velocity, and the adjoint cold start."""
         L, d = self.L, self.d
         self._set_prec(self.prec)
         fb = self.fwd_buf
@@ -267,7 +275,9 @@ class Heatsink2D:
                 L.lbm_program_destroy(self.adj_prog)
             self.adj_prog = self._record_adjoint(fb)
             self._adj_fb = fb
-            self._graphs.pop("adj", None)
+            g = self._graphs.pop("adj", None)
+            if g is not None:
+                L.lbm_graph_destroy(g)
         for k in ("ip", "item"):
             d[k].fill(0.0)
         for k in ("iux0", "iuy0", "iqx0", "iqy0"):
@@ -277,11 +287,10 @@ class Heatsink2D:
         check(L.lbm_aad_initial_condition(self.pg.h, d["ux%d" % fb].ptr, d["uy%d" % fb].ptr, None, d["item"].ptr,
                                           d["iqx0"].ptr, d["iqy0"].ptr, None))
 
-        def res(t):
-            b, pb = t % 2, (t - 1) % 2
-            return (self._residual(d["iux%d" % b], d["iuy%d" % b], d["iux%d" % pb], d["iuy%d" % pb]),
-                    self._residual(d["iqx%d" % b], d["iqy%d" % b], d["iqx%d" % pb], d["iqy%d" % pb]))
-
+    def adjoint(self):
+        self.init_adjoint()
+        fb = self.fwd_buf
+        res = lambda t: tuple(self._residual(*f) for f in self.residual_fields(True, t))
         t, self.adj_steps32, self.adj_res = self._solve("adj", lambda s: self._adj_step(s, fb), res)
         self.adj_steps = t
         self.adj_buf = t % 2
@@ -309,9 +318,10 @@ class Heatsink2D:
         self.sensitivity()
         self.ctx.sync()
         dt = time.perf_counter() - t0
+        diverged = not (np.all(np.isfinite(self.fwd_res)) and np.all(np.isfinite(self.adj_res)))
         return {"seconds": dt, "fwd_steps": self.fwd_steps, "adj_steps": self.adj_steps,
                 "fp32_steps": (self.fwd_steps32, self.adj_steps32), "objective": self.objective,
-                "fwd_residual": self.fwd_res, "adj_residual": self.adj_res}
+                "fwd_residual": self.fwd_res, "adj_residual": self.adj_res, "diverged": diverged}
 
     def close(self):
         for g in self._graphs.values():
