@@ -40,6 +40,12 @@ HBM_PEAK_GBS = 8000.0
 BYTES_FWD_Q9Q5 = 2 * (9 + 5) * 8 + 3 * 8
 BYTES_FWD_Q9Q9 = 2 * (9 + 9) * 8 + 3 * 8
 BYTES_FWD_Q9Q5_F32 = 2 * (9 + 5) * 4 + 3 * 8  # FP32 populations, FP64 coefficient fields
+# Memory-side traffic of one forward step of the 200x400 workload, from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE (KiB per dispatch; profiles/k_collide_fused_heatsink_200x400.txt), FETCH_SIZE scaled by
+# 1.920, the factor that makes it read the known bytes of the pull-stream copy kernel
+# (profiles/pmc_scale_k_stream_4096x4096.txt).  The counters see traffic below the L2, which on this
+# cache-sized step includes Infinity Cache hits.
+TRAFFIC_FWD_200x400 = round((5767.5 * 1.920 + 8920.5) * 1024)
 
 
 def kernel_rate(hs, steps=100, reps=5):
@@ -274,7 +280,8 @@ def main():
     ms_step = kernel_rate(hs)
     achieved = BYTES_FWD_Q9Q5 * hs.N / (ms_step * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": TRAFFIC_FWD_200x400 if (a.lx, a.ly) == (200, 400) else None,
                 "kernel": "k_collide_fused<D2Q9,D2Q5,ADBody> (pull + boundary program + corner smoothing + "
                           "collide in one pass), per fused forward step",
                 "bytes_per_launch": BYTES_FWD_Q9Q5 * hs.N, "us_per_launch": round(ms_step * 1e3, 3),

@@ -1,0 +1,241 @@
+"""3D heat-sink design evaluation on the HIP path: D3Q15 Navier-Stokes with Brinkman drag coupled to
+D3Q7 advection-diffusion, forward steady solve, adjoint steady solve, design sensitivity.
+
+The step order is the 2D driver's (panslbm2_amd/heatsink2d.py, after production/heatsink.cpp:
+136-232) on the 3D overloads (advection.h:871, adjointadvection.h:1054, :1478), with the boundary
+set of a duct heat sink:
+  * inlet x = 0 (face interior): uniform velocity u_in (Zou-He), T = 0;
+  * outlet x = lx-1 (face interior): pressure rho = 1 (Zou-He), zero heat flux;
+  * walls y = 0, ly-1, z = 0, lz-1: bounce-back, zero heat flux;
+  * base plate z < plate: solid (gamma = 0) with a volumetric heat source Q;
+  * design gamma elsewhere: seeded uniform[0,1] smoothed by the density filter (radius 2);
+  * objective: mean temperature on the bottom face z = 0, injected into the adjoint through the flux
+    condition with eps = 1 (heatsink.cpp:202); the adjoint outlet is ANS::iBoundaryConditionSetRho3D
+    (the reference's 3D AAD::iBoundaryConditionSetRho does not compile).
+No SmoothCorner: the corner and edge nodes lie on bounce-back walls, and the fused 3D path runs
+every operator but SmoothCorner, so each time step is one kernel pass.
+"""
+import ctypes
+
+import numpy as np
+
+from . import D3Q7, D3Q15, F32, F64, NC, Context, Lattice, check, hptr, to_face
+from .heatsink2d import Heatsink2D
+
+D = ctypes.c_double
+VP = ctypes.c_void_p
+
+
+class Heatsink3D(Heatsink2D):
+    def __init__(self, ctx=None, lx=128, ly=128, lz=64, ad_kind=D3Q7, seed=0, nu=0.1, pr=6.0, uin=0.03, heat=1e-3,
+                 plate=8, alpha_max=1e4, qf=0.01, qg=1.0, filter_radius=2.0, eps=1e-6, check_every=100,
+                 max_steps=400000, use_graphs=True, prec=F64, fp32_floor=1e-5, fused=True):
+        self.ctx = ctx or Context(0)
+        self.L = self.ctx.lib
+        self.lx, self.ly, self.lz = lx, ly, lz
+        self.N = N = lx * ly * lz
+        self.ad_kind = ad_kind
+        self.nu, self.eps, self.check_every, self.max_steps = nu, eps, check_every, max_steps
+        assert check_every % 2 == 0
+        self.use_graphs, self.fused = use_graphs and fused, fused
+        self.prec, self.fp32_floor = prec, fp32_floor
+        c = self.ctx
+        n = np.arange(N)
+        i, j, k = n % lx, (n // lx) % ly, n // (lx * ly)
+        self.i, self.j, self.k = i, j, k
+
+        rng = np.random.default_rng(seed)
+        raw = c.array((N,), data=rng.uniform(0.0, 1.0, N))
+        gam = c.array((N,), fill=0.0)
+        check(self.L.lbm_density_filter(c.h, lx, ly, lz, D(filter_radius), raw.ptr, gam.ptr))
+        s = gam.download()
+        s[k < plate] = 0.0  # the base plate is solid
+        self.gamma = s
+        dfl = nu / pr
+        dso = 10.0 * dfl
+        coef = {
+            "alpha": alpha_max / (ly - 1.0) * qf * (1.0 - s) / (s + qf),
+            "dads": -alpha_max / (ly - 1.0) * qf * (1.0 + qf) / (s + qf) ** 2,
+            "diff": dso + (dfl - dso) * s * (1.0 + qg) / (s + qg),
+            "dkds": (dfl - dso) * qg * (1.0 + qg) / (s + qg) ** 2,
+            "heat": np.where(k < plate, heat, 0.0),
+        }
+        self.d = {name: c.array((N,), data=v) for name, v in coef.items()}
+        for name in ("rho", "tem", "ip", "imx", "imy", "imz", "item", "dfds"):
+            self.d[name] = c.array((N,), fill=0.0)
+        for name in ("ux", "uy", "uz", "qx", "qy", "qz", "iux", "iuy", "iuz", "iqx", "iqy", "iqz"):
+            self.d[name + "0"] = c.array((N,), fill=0.0)
+            self.d[name + "1"] = c.array((N,), fill=0.0)
+        self.d["g"] = c.array((NC[ad_kind], N), fill=0.0)
+        self.d["ig"] = c.array((NC[ad_kind], N), fill=0.0)
+        self.d["one"] = c.array((N,), fill=1.0)
+
+        self.pf = Lattice(c, D3Q15, lx, ly, lz, prec=prec)
+        self.pg = Lattice(c, ad_kind, lx, ly, lz, prec=prec)
+
+        wall = (j == 0) | (j == ly - 1) | (k == 0) | (k == lz - 1)
+        inner = ~wall
+        inlet = ((i == 0) & inner).astype(np.int32)
+        outlet = ((i == lx - 1) & inner).astype(np.int32)
+        flux = (((i > 0) | wall)).astype(np.int32)  # every boundary node but the inlet's interior
+        bottom = (k == 0).astype(np.int32)
+        self.obj_nodes = np.nonzero(k == 0)[0].astype(np.int32)
+        F = lambda kind, a, dt=np.float64: to_face(kind, lx, ly, lz, a, dt)
+        zero = np.zeros(N)
+        self.faces = dict(
+            walls_f=F(D3Q15, wall.astype(np.int32), np.int32), inlet_f=F(D3Q15, inlet, np.int32),
+            outlet_f=F(D3Q15, outlet, np.int32), uin_f=F(D3Q15, np.full(N, uin)), zero_f=F(D3Q15, zero),
+            one_f=F(D3Q15, np.ones(N)), inlet_g=F(ad_kind, inlet, np.int32), flux_g=F(ad_kind, flux, np.int32),
+            obj_g=F(ad_kind, bottom, np.int32), zero_g=F(ad_kind, zero),
+        )
+        self._graphs = {}
+        self.adj_prog = None
+        self.fwd_prog = self._record_forward() if fused else None
+
+    # ---- boundary calls of one step (recorded into a program, or run eagerly) ----------------------
+    def _fwd_boundaries(self, ux, uy, uz):
+        L, d, fc = self.L, self.d, self.faces
+        check(L.lbm_stream(self.pf.h))
+        check(L.lbm_stream(self.pg.h))
+        check(L.lbm_bc(self.pf.h, hptr(fc["walls_f"])))
+        check(L.lbm_ns_set_u(self.pf.h, hptr(fc["inlet_f"]), hptr(fc["uin_f"]), hptr(fc["zero_f"]), hptr(fc["zero_f"])))
+        check(L.lbm_ns_set_rho(self.pf.h, hptr(fc["outlet_f"]), hptr(fc["one_f"]), hptr(fc["zero_f"]),
+                               hptr(fc["zero_f"])))
+        check(L.lbm_ad_set_t(self.pg.h, hptr(fc["inlet_g"]), hptr(fc["zero_g"]), ux, uy, uz))
+        check(L.lbm_ad_set_q(self.pg.h, hptr(fc["flux_g"]), hptr(fc["zero_g"]), ux, uy, uz, d["diff"].ptr, D(0.0)))
+
+    def _adj_boundaries(self, ux, uy, uz):
+        L, fc = self.L, self.faces
+        check(L.lbm_istream(self.pf.h))
+        check(L.lbm_istream(self.pg.h))
+        check(L.lbm_aad_iset_t(self.pg.h, hptr(fc["inlet_g"]), ux, uy, uz))
+        check(L.lbm_aad_iset_q(self.pg.h, hptr(fc["flux_g"]), ux, uy, uz, D(0.0)))
+        check(L.lbm_aad_iset_q(self.pg.h, hptr(fc["obj_g"]), ux, uy, uz, D(1.0)))
+        check(L.lbm_ibc(self.pf.h, hptr(fc["walls_f"])))
+        check(L.lbm_ans_iset_u(self.pf.h, hptr(fc["inlet_f"]), hptr(fc["uin_f"]), hptr(fc["zero_f"]),
+                               hptr(fc["zero_f"]), D(0.0), 1))
+        check(L.lbm_ans_iset_rho(self.pf.h, hptr(fc["outlet_f"])))
+
+    def _record(self, calls):
+        check(self.L.lbm_record_begin(self.ctx.h))
+        calls()
+        prog = VP()
+        check(self.L.lbm_record_end(self.ctx.h, ctypes.byref(prog)))
+        assert self.L.lbm_program_fusable(prog) == 1
+        return prog
+
+    def _record_forward(self):
+        d = self.d
+        return self._record(lambda: self._fwd_boundaries(d["ux0"].ptr, d["uy0"].ptr, d["uz0"].ptr))
+
+    def _record_adjoint(self, b):
+        d = self.d
+        return self._record(lambda: self._adj_boundaries(d["ux%d" % b].ptr, d["uy%d" % b].ptr, d["uz%d" % b].ptr))
+
+    # ---- steps ------------------------------------------------------------------------------------
+    def _fwd_step(self, t):
+        L, d = self.L, self.d
+        b, pb = t % 2, (t - 1) % 2
+        prog = None
+        if t > 1:
+            if self.fused:
+                prog = self.fwd_prog
+                for s in range(L.lbm_program_num_field_sets(prog)):
+                    check(L.lbm_program_bind_fields(prog, s, None, d["ux%d" % pb].ptr, d["uy%d" % pb].ptr,
+                                                    d["uz%d" % pb].ptr, None, None))
+            else:
+                self._fwd_boundaries(d["ux%d" % pb].ptr, d["uy%d" % pb].ptr, d["uz%d" % pb].ptr)
+        mode = self._save_mode(t, False)
+        check(L.lbm_ad_macro_brinkman_collide_force_convection_p(
+            prog, self.pf.h, d["rho"].ptr, d["ux%d" % b].ptr, d["uy%d" % b].ptr, d["uz%d" % b].ptr, d["alpha"].ptr,
+            D(self.nu), self.pg.h, d["tem"].ptr, d["qx%d" % b].ptr, d["qy%d" % b].ptr, d["qz%d" % b].ptr,
+            d["diff"].ptr, mode, d["g"].ptr, d["heat"].ptr))
+
+    def _adj_step(self, t, fb):
+        L, d = self.L, self.d
+        b = t % 2
+        prog = None
+        if t > 1:
+            if self.fused:
+                prog = self.adj_prog
+            else:
+                self._adj_boundaries(d["ux%d" % fb].ptr, d["uy%d" % fb].ptr, d["uz%d" % fb].ptr)
+        mode = self._save_mode(t, True)
+        check(L.lbm_aad_macro_brinkman_collide_force_convection_p(
+            prog, self.pf.h, d["rho"].ptr, d["ux%d" % fb].ptr, d["uy%d" % fb].ptr, d["uz%d" % fb].ptr, d["ip"].ptr,
+            d["iux%d" % b].ptr, d["iuy%d" % b].ptr, d["iuz%d" % b].ptr, d["imx"].ptr, d["imy"].ptr, d["imz"].ptr,
+            d["alpha"].ptr, D(self.nu), self.pg.h, d["tem"].ptr, d["item"].ptr, d["iqx%d" % b].ptr,
+            d["iqy%d" % b].ptr, d["iqz%d" % b].ptr, d["diff"].ptr, mode, d["ig"].ptr))
+
+    def _residual3(self, a, b):
+        r = ctypes.c_double()
+        check(self.L.lbm_residual(self.ctx.h, 3, a[0].ptr, a[1].ptr, a[2].ptr, b[0].ptr, b[1].ptr, b[2].ptr, self.N,
+                                  ctypes.byref(r)))
+        return r.value
+
+    def residual_fields(self, adjoint, t):
+        d = self.d
+        b, pb = t % 2, (t - 1) % 2
+        u, q = (("iux", "iuy", "iuz"), ("iqx", "iqy", "iqz")) if adjoint else (("ux", "uy", "uz"), ("qx", "qy", "qz"))
+        return tuple(([d["%s%d" % (v, b)] for v in f], [d["%s%d" % (v, pb)] for v in f]) for f in (u, q))
+
+    def init_forward(self):
+        L, d = self.L, self.d
+        self._set_prec(self.prec)
+        for name in ("ux0", "uy0", "uz0"):
+            d[name].fill(0.0)
+        d["tem"].fill(0.0)
+        check(L.lbm_ns_initial_condition(self.pf.h, d["one"].ptr, d["ux0"].ptr, d["uy0"].ptr, d["uz0"].ptr))
+        check(L.lbm_ad_initial_condition(self.pg.h, d["tem"].ptr, d["ux0"].ptr, d["uy0"].ptr, d["uz0"].ptr))
+
+    def forward(self):
+        self.init_forward()
+        res = lambda t: tuple(self._residual3(*f) for f in self.residual_fields(False, t))
+        t, self.fwd_steps32, self.fwd_res = self._solve("fwd", self._fwd_step, res)
+        self.fwd_steps = t
+        self.fwd_buf = t % 2
+        return t
+
+    def init_adjoint(self):
+        L, d = self.L, self.d
+        self._set_prec(self.prec)
+        fb = self.fwd_buf
+        if self.fused and (self.adj_prog is None or getattr(self, "_adj_fb", None) != fb):
+            if self.adj_prog is not None:
+                L.lbm_program_destroy(self.adj_prog)
+            self.adj_prog = self._record_adjoint(fb)
+            self._adj_fb = fb
+            g = self._graphs.pop("adj", None)
+            if g is not None:
+                L.lbm_graph_destroy(g)
+        for name in ("ip", "item"):
+            d[name].fill(0.0)
+        for name in ("iux0", "iuy0", "iuz0", "iqx0", "iqy0", "iqz0"):
+            d[name].fill(0.0)
+        check(L.lbm_ans_initial_condition(self.pf.h, d["ux%d" % fb].ptr, d["uy%d" % fb].ptr, d["uz%d" % fb].ptr,
+                                          d["ip"].ptr, d["iux0"].ptr, d["iuy0"].ptr, d["iuz0"].ptr))
+        check(L.lbm_aad_initial_condition(self.pg.h, d["ux%d" % fb].ptr, d["uy%d" % fb].ptr, d["uz%d" % fb].ptr,
+                                          d["item"].ptr, d["iqx0"].ptr, d["iqy0"].ptr, d["iqz0"].ptr))
+
+    def adjoint(self):
+        self.init_adjoint()
+        fb = self.fwd_buf
+        res = lambda t: tuple(self._residual3(*f) for f in self.residual_fields(True, t))
+        t, self.adj_steps32, self.adj_res = self._solve("adj", lambda s: self._adj_step(s, fb), res)
+        self.adj_steps = t
+        self.adj_buf = t % 2
+        return t
+
+    def sensitivity(self):
+        """dF/dgamma: Brinkman and diffusivity terms (adjointadvection.h:1478-1493)."""
+        L, d = self.L, self.d
+        fb, ab = self.fwd_buf, self.adj_buf
+        d["dfds"].fill(0.0)
+        check(L.lbm_sensitivity_brinkman_diffusivity(
+            self.pg.h, d["dfds"].ptr, d["ux%d" % fb].ptr, d["uy%d" % fb].ptr, d["uz%d" % fb].ptr, d["imx"].ptr,
+            d["imy"].ptr, d["imz"].ptr, d["dads"].ptr, d["tem"].ptr, d["item"].ptr, d["iqx%d" % ab].ptr,
+            d["iqy%d" % ab].ptr, d["iqz%d" % ab].ptr, d["g"].ptr, d["ig"].ptr, d["diff"].ptr, d["dkds"].ptr))
+        out = ctypes.c_double()
+        check(L.lbm_gather_sum(self.ctx.h, d["tem"].ptr, hptr(self.obj_nodes), len(self.obj_nodes), ctypes.byref(out)))
+        self.objective = out.value / len(self.obj_nodes)
+        return self.objective
