@@ -149,6 +149,12 @@ int lbm_ad_macro_brinkman_collide_natural_convection(lbm_lattice *p, double *rho
                                                      double *tem, double *qx, double *qy, double *qz,
                                                      const double *diffusivity, double gx, double gy, double gz,
                                                      double tem0, int issave, double *g);
+/* MacroBrinkmanCollideHeatExchange :716-810: constant diffusivity, per-node beta; after the Brinkman
+ * force g_c += w_c beta (1 - T)/(1 + beta) (ExternalForceHeatExchange :89-95); no snapshot */
+int lbm_ad_macro_brinkman_collide_heat_exchange(lbm_lattice *p, double *rho, double *ux, double *uy, double *uz,
+                                                const double *alpha, double viscosity, lbm_lattice *q, double *tem,
+                                                double *qx, double *qy, double *qz, const double *beta,
+                                                double diffusivity, int issave);
 int lbm_ad_set_t(lbm_lattice *q, const int *mask_face, const double *tem_face, const double *ux, const double *uy,
                  const double *uz);                                  /* BoundaryConditionSetT :1074-1090 */
 /* BoundaryConditionSetQ :1094-1130; diffusivity NULL selects the constant-D overload */
@@ -184,6 +190,13 @@ int lbm_aad_macro_brinkman_collide_natural_convection(
     double *iuy, double *iuz, double *imx, double *imy, double *imz, const double *alpha, double viscosity,
     lbm_lattice *q, const double *tem, double *item, double *iqx, double *iqy, double *iqz, const double *diffusivity,
     double gx, double gy, double gz, int issave, double *ig);
+/* MacroBrinkmanCollideHeatExchange :894-993: g_c -= beta (1 + iT)/(1 + beta) after the ANS re-macro
+ * (ExternalForceHeatExchange :106-112), then the AAD macro again; no snapshot */
+int lbm_aad_macro_brinkman_collide_heat_exchange(
+    lbm_lattice *p, const double *rho, const double *ux, const double *uy, const double *uz, double *ip, double *iux,
+    double *iuy, double *iuz, double *imx, double *imy, double *imz, const double *alpha, double viscosity,
+    lbm_lattice *q, const double *tem, double *item, double *iqx, double *iqy, double *iqz, const double *beta,
+    double diffusivity, int issave);
 int lbm_aad_iset_t(lbm_lattice *q, const int *mask_face, const double *ux, const double *uy,
                    const double *uz);                                /* iBoundaryConditionSetT :1385-1401 */
 int lbm_aad_iset_q(lbm_lattice *q, const int *mask_face, const double *ux, const double *uy, const double *uz,
@@ -196,6 +209,10 @@ int lbm_aad_iset_rho(lbm_lattice *p, lbm_lattice *q, const int *type_face, const
 int lbm_sensitivity_brinkman(lbm_lattice *p, double *dfds, const double *ux, const double *uy, const double *uz,
                              const double *imx, const double *imy, const double *imz,
                              const double *dads);                    /* adjointnavierstokes.h:542-554 */
+/* SensitivityHeatExchange adjointadvection.h:1445-1457: dfds += 3 dads (u.im) - dbds (1 - T)(1 + iT) */
+int lbm_sensitivity_heat_exchange(lbm_lattice *q, double *dfds, const double *ux, const double *uy, const double *uz,
+                                  const double *imx, const double *imy, const double *imz, const double *dads,
+                                  const double *tem, const double *item, const double *dbds);
 int lbm_sensitivity_brinkman_diffusivity(lbm_lattice *q, double *dfds, const double *ux, const double *uy,
                                          const double *uz, const double *imx, const double *imy, const double *imz,
                                          const double *dads, const double *tem, const double *item, const double *iqx,
@@ -251,6 +268,10 @@ int lbm_ad_macro_brinkman_collide_natural_convection_p(lbm_program *prog, lbm_la
                                                        lbm_lattice *q, double *tem, double *qx, double *qy,
                                                        double *qz, const double *diffusivity, double gx, double gy,
                                                        double gz, double tem0, int issave, double *g);
+int lbm_ad_macro_brinkman_collide_heat_exchange_p(lbm_program *prog, lbm_lattice *p, double *rho, double *ux,
+                                                  double *uy, double *uz, const double *alpha, double viscosity,
+                                                  lbm_lattice *q, double *tem, double *qx, double *qy, double *qz,
+                                                  const double *beta, double diffusivity, int issave);
 int lbm_ans_macro_brinkman_collide_p(lbm_program *prog, lbm_lattice *p, const double *rho, const double *ux,
                                      const double *uy, const double *uz, double *ip, double *iux, double *iuy,
                                      double *iuz, double *imx, double *imy, double *imz, double viscosity,
@@ -292,6 +313,12 @@ int lbm_batch_destroy(lbm_batch *batch);
 /* Residual (residual.h:24-51) of n array sets at once: a[3*i + k], b[3*i + k] for k < ncomp */
 int lbm_residual_batch(lbm_context *ctx, int n, int ncomp, const double *const *a, const double *const *b,
                        const int *len, double *out);
+
+int lbm_aad_macro_brinkman_collide_heat_exchange_p(
+    lbm_program *prog, lbm_lattice *p, const double *rho, const double *ux, const double *uy, const double *uz,
+    double *ip, double *iux, double *iuy, double *iuz, double *imx, double *imy, double *imz, const double *alpha,
+    double viscosity, lbm_lattice *q, const double *tem, double *item, double *iqx, double *iqy, double *iqz,
+    const double *beta, double diffusivity, int issave);
 
 /* ---- HIP graphs of the context stream (no reference counterpart; the reference loops on the
  * host).  Calls between begin and end are captured, not run; they must not return host data.

@@ -720,6 +720,39 @@ void MacroBrinkmanCollideNaturalConvection(P<T> &_p, T *_rho, T *_ux, T *_uy, T 
                "AD::MacroBrinkmanCollideNaturalConvection");
     if (_issave) rt.wrote({_rho, _ux, _uy, _uz, _tem, _qx, _qy, _qz, _g});
 }
+// :716-760
+template <class T, template <class> class P, template <class> class Q>
+void MacroBrinkmanCollideHeatExchange(P<T> &_p, T *_rho, T *_ux, T *_uy, const T *_alpha, T _viscosity, Q<T> &_q,
+                                      T *_tem, T *_qx, T *_qy, const T *_beta, T _diffusivity, bool _issave = false) {
+    static_assert(P<T>::nd == 2 && Q<T>::nd == 2, "2D overload");
+    gpu::Runtime &rt = gpu::Runtime::get();
+    size_t n = gpu::N(_p);
+    const double *alpha = rt.in(_alpha, n), *beta = rt.in(_beta, n);
+    double *rho = rt.out(_rho, n), *ux = rt.out(_ux, n), *uy = rt.out(_uy, n);
+    double *tem = rt.out(_tem, n), *qx = rt.out(_qx, n), *qy = rt.out(_qy, n);
+    gpu::check(lbm_ad_macro_brinkman_collide_heat_exchange_p(rt.take(), _p.handle(), rho, ux, uy, nullptr, alpha,
+                                                             _viscosity, _q.handle(), tem, qx, qy, nullptr, beta,
+                                                             _diffusivity, _issave ? 1 : 0),
+               "AD::MacroBrinkmanCollideHeatExchange");
+    if (_issave) rt.wrote({_rho, _ux, _uy, _tem, _qx, _qy});
+}
+// :764-810
+template <class T, template <class> class P, template <class> class Q>
+void MacroBrinkmanCollideHeatExchange(P<T> &_p, T *_rho, T *_ux, T *_uy, T *_uz, const T *_alpha, T _viscosity,
+                                      Q<T> &_q, T *_tem, T *_qx, T *_qy, T *_qz, const T *_beta, T _diffusivity,
+                                      bool _issave = false) {
+    static_assert(P<T>::nd == 3 && Q<T>::nd == 3, "3D overload");
+    gpu::Runtime &rt = gpu::Runtime::get();
+    size_t n = gpu::N(_p);
+    const double *alpha = rt.in(_alpha, n), *beta = rt.in(_beta, n);
+    double *rho = rt.out(_rho, n), *ux = rt.out(_ux, n), *uy = rt.out(_uy, n), *uz = rt.out(_uz, n);
+    double *tem = rt.out(_tem, n), *qx = rt.out(_qx, n), *qy = rt.out(_qy, n), *qz = rt.out(_qz, n);
+    gpu::check(lbm_ad_macro_brinkman_collide_heat_exchange_p(rt.take(), _p.handle(), rho, ux, uy, uz, alpha,
+                                                             _viscosity, _q.handle(), tem, qx, qy, qz, beta,
+                                                             _diffusivity, _issave ? 1 : 0),
+               "AD::MacroBrinkmanCollideHeatExchange");
+    if (_issave) rt.wrote({_rho, _ux, _uy, _uz, _tem, _qx, _qy, _qz});
+}
 template <class T, template <class> class Q>
 void InitialCondition(Q<T> &_q, const T *_tem, const T *_ux, const T *_uy) {  // :1048-1057
     static_assert(Q<T>::nd == 2, "2D overload");
@@ -1104,6 +1137,76 @@ void iBoundaryConditionSetRho(P<T> &_p, Q<T> &_q, const T *_rho, const T *_ux, c
         _p.handle(), "AAD::iBoundaryConditionSetRho",
         [&] { return lbm_aad_iset_rho(_p.handle(), _q.handle(), m.data(), rho, ux, uy, nullptr, tem, _eps); },
         _q.handle());
+}
+// :1445-1450
+template <class T, template <class> class Q>
+void SensitivityHeatExchange(Q<T> &_q, T *_dfds, const T *_ux, const T *_uy, const T *_imx, const T *_imy,
+                             const T *_dads, const T *_tem, const T *_item, const T *_dbds) {
+    static_assert(Q<T>::nd == 2, "2D overload");
+    gpu::Runtime &rt = gpu::Runtime::get();
+    size_t n = gpu::N(_q);
+    double *dfds = rt.inout(_dfds, n);
+    gpu::check(lbm_sensitivity_heat_exchange(_q.handle(), dfds, rt.in(_ux, n), rt.in(_uy, n), nullptr, rt.in(_imx, n),
+                                             rt.in(_imy, n), nullptr, rt.in(_dads, n), rt.in(_tem, n),
+                                             rt.in(_item, n), rt.in(_dbds, n)),
+               "AAD::SensitivityHeatExchange");
+    rt.wrote({_dfds}, true);
+}
+// :1452-1457
+template <class T, template <class> class Q>
+void SensitivityHeatExchange(Q<T> &_q, T *_dfds, const T *_ux, const T *_uy, const T *_uz, const T *_imx,
+                             const T *_imy, const T *_imz, const T *_dads, const T *_tem, const T *_item,
+                             const T *_dbds) {
+    static_assert(Q<T>::nd == 3, "3D overload");
+    gpu::Runtime &rt = gpu::Runtime::get();
+    size_t n = gpu::N(_q);
+    double *dfds = rt.inout(_dfds, n);
+    gpu::check(lbm_sensitivity_heat_exchange(_q.handle(), dfds, rt.in(_ux, n), rt.in(_uy, n), rt.in(_uz, n),
+                                             rt.in(_imx, n), rt.in(_imy, n), rt.in(_imz, n), rt.in(_dads, n),
+                                             rt.in(_tem, n), rt.in(_item, n), rt.in(_dbds, n)),
+               "AAD::SensitivityHeatExchange");
+    rt.wrote({_dfds}, true);
+}
+// :894-940
+template <class T, template <class> class P, template <class> class Q>
+void MacroBrinkmanCollideHeatExchange(P<T> &_p, const T *_rho, const T *_ux, const T *_uy, T *_ip, T *_iux, T *_iuy,
+                                      T *_imx, T *_imy, const T *_alpha, T _viscosity, Q<T> &_q, const T *_tem,
+                                      T *_item, T *_iqx, T *_iqy, const T *_beta, T _diffusivity,
+                                      bool _issave = false) {
+    static_assert(P<T>::nd == 2 && Q<T>::nd == 2, "2D overload");
+    gpu::Runtime &rt = gpu::Runtime::get();
+    size_t n = gpu::N(_p);
+    const double *rho = rt.in(_rho, n), *ux = rt.in(_ux, n), *uy = rt.in(_uy, n), *alpha = rt.in(_alpha, n);
+    const double *tem = rt.in(_tem, n), *beta = rt.in(_beta, n);
+    double *ip = rt.out(_ip, n), *iux = rt.out(_iux, n), *iuy = rt.out(_iuy, n);
+    double *imx = rt.out(_imx, n), *imy = rt.out(_imy, n);
+    double *item = rt.out(_item, n), *iqx = rt.out(_iqx, n), *iqy = rt.out(_iqy, n);
+    gpu::check(lbm_aad_macro_brinkman_collide_heat_exchange_p(
+                   rt.take(), _p.handle(), rho, ux, uy, nullptr, ip, iux, iuy, nullptr, imx, imy, nullptr, alpha,
+                   _viscosity, _q.handle(), tem, item, iqx, iqy, nullptr, beta, _diffusivity, _issave ? 1 : 0),
+               "AAD::MacroBrinkmanCollideHeatExchange");
+    if (_issave) rt.wrote({_ip, _iux, _iuy, _imx, _imy, _item, _iqx, _iqy});
+}
+// :944-993
+template <class T, template <class> class P, template <class> class Q>
+void MacroBrinkmanCollideHeatExchange(P<T> &_p, const T *_rho, const T *_ux, const T *_uy, const T *_uz, T *_ip,
+                                      T *_iux, T *_iuy, T *_iuz, T *_imx, T *_imy, T *_imz, const T *_alpha,
+                                      T _viscosity, Q<T> &_q, const T *_tem, T *_item, T *_iqx, T *_iqy, T *_iqz,
+                                      const T *_beta, T _diffusivity, bool _issave = false) {
+    static_assert(P<T>::nd == 3 && Q<T>::nd == 3, "3D overload");
+    gpu::Runtime &rt = gpu::Runtime::get();
+    size_t n = gpu::N(_p);
+    const double *rho = rt.in(_rho, n), *ux = rt.in(_ux, n), *uy = rt.in(_uy, n), *uz = rt.in(_uz, n);
+    const double *alpha = rt.in(_alpha, n), *tem = rt.in(_tem, n), *beta = rt.in(_beta, n);
+    double *ip = rt.out(_ip, n), *iux = rt.out(_iux, n), *iuy = rt.out(_iuy, n), *iuz = rt.out(_iuz, n);
+    double *imx = rt.out(_imx, n), *imy = rt.out(_imy, n), *imz = rt.out(_imz, n);
+    double *item = rt.out(_item, n), *iqx = rt.out(_iqx, n), *iqy = rt.out(_iqy, n), *iqz = rt.out(_iqz, n);
+    gpu::check(lbm_aad_macro_brinkman_collide_heat_exchange_p(rt.take(), _p.handle(), rho, ux, uy, uz, ip, iux, iuy,
+                                                              iuz, imx, imy, imz, alpha, _viscosity, _q.handle(), tem,
+                                                              item, iqx, iqy, iqz, beta, _diffusivity,
+                                                              _issave ? 1 : 0),
+               "AAD::MacroBrinkmanCollideHeatExchange");
+    if (_issave) rt.wrote({_ip, _iux, _iuy, _iuz, _imx, _imy, _imz, _item, _iqx, _iqy, _iqz});
 }
 template <class T, template <class> class Q>
 void SensitivityBrinkmanDiffusivity(Q<T> &_q, T *_dfds, const T *_ux, const T *_uy, const T *_imx, const T *_imy,

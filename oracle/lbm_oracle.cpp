@@ -1065,6 +1065,40 @@ void ol_ad_collide_nc(ol_lattice *p, double *rho, double *ux, double *uy, double
         store(q, idx, g);
     }
 }
+/* advection.h:716-810 (MacroBrinkmanCollideHeatExchange) with ExternalForceHeatExchange
+ * (advection.h:89-95): constant diffusivity, per-node beta; after the Brinkman force and the NS
+ * re-macro, g_c += w_c beta (1 - T) / (1 + beta) with T from the first macro, then the AD re-macro.
+ * No snapshot. */
+void ol_ad_collide_he(ol_lattice *p, double *rho, double *ux, double *uy, double *uz, const double *alpha, double nu,
+                      ol_lattice *q, double *tem, double *qx, double *qy, double *qz, const double *beta, double diff,
+                      int issave) {
+    const LatDesc &dp = *p->d, &dq = *q->d;
+    double omegaf = 1.0 / (3.0 * nu + 0.5), iomegaf = 1.0 - omegaf;
+    double omegag = 1.0 / (3.0 * diff + 0.5), iomegag = 1.0 - omegag;
+#pragma omp parallel for
+    for (int idx = 0; idx < p->N; ++idx) {
+        double f[27], g[27], feq[27], geq[27], r, vx, vy, vz, T, hx, hy, hz;
+        load(p, idx, f);
+        load(q, idx, g);
+        ns_macro(dp, f, r, vx, vy, vz);
+        ad_macro(dq, g, vx, vy, vz, omegag, T, hx, hy, hz);
+        ns_brinkman(dp, f, r, vx, vy, vz, alpha[idx]);
+        ns_macro(dp, f, r, vx, vy, vz);
+        double coef = beta[idx] * (1.0 - T) / (1.0 + beta[idx]);
+        for (int c = 0; c < dq.nc; ++c) g[c] += dq.w[c] * coef;
+        ad_macro(dq, g, vx, vy, vz, omegag, T, hx, hy, hz);
+        if (issave) {
+            rho[idx] = r; ux[idx] = vx; uy[idx] = vy; put(uz, idx, vz);
+            tem[idx] = T; qx[idx] = hx; qy[idx] = hy; put(qz, idx, hz);
+        }
+        ns_eq(dp, feq, r, vx, vy, vz);
+        bgk(dp, f, feq, omegaf, iomegaf);
+        ad_eq(dq, geq, T, vx, vy, vz);
+        bgk(dq, g, geq, omegag, iomegag);
+        store(p, idx, f);
+        store(q, idx, g);
+    }
+}
 /* advection.h:1074-1090 with :99-238 */
 void ol_ad_set_t(ol_lattice *q, const int *mask, const double *tem, const double *ux, const double *uy,
                  const double *uz) {
@@ -1238,6 +1272,43 @@ void ol_aad_collide_nc(ol_lattice *p, const double *rho, const double *ux, const
         store(q, idx, g);
     }
 }
+/* adjointadvection.h:894-993 (MacroBrinkmanCollideHeatExchange) with ExternalForceHeatExchange
+ * (adjointadvection.h:106-112): constant diffusivity; after the ANS re-macro every g_c (g_0 included)
+ * loses beta (1 + iT) / (1 + beta) with iT from the first AAD macro, then the AAD macro is taken
+ * again.  No snapshot. */
+void ol_aad_collide_he(ol_lattice *p, const double *rho, const double *ux, const double *uy, const double *uz,
+                       double *ip, double *iux, double *iuy, double *iuz, double *imx, double *imy, double *imz,
+                       const double *alpha, double nu, ol_lattice *q, const double *tem, double *item, double *iqx,
+                       double *iqy, double *iqz, const double *beta, double diff, int issave) {
+    const LatDesc &dp = *p->d, &dq = *q->d;
+    double omegaf = 1.0 / (3.0 * nu + 0.5), iomegaf = 1.0 - omegaf;
+    double omegag = 1.0 / (3.0 * diff + 0.5), iomegag = 1.0 - omegag;
+#pragma omp parallel for
+    for (int idx = 0; idx < p->N; ++idx) {
+        double f[27], g[27], feq[27], geq[27], a, bx, by, bz, mx, my, mz, it, hx, hy, hz;
+        double vx = ux[idx], vy = uy[idx], vz = fld(uz, idx);
+        load(p, idx, f);
+        load(q, idx, g);
+        ans_macro(dp, f, vx, vy, vz, a, bx, by, bz, mx, my, mz);
+        aad_macro(dq, g, it, hx, hy, hz);
+        aad_brinkman(dp, f, rho[idx], vx, vy, vz, mx, my, mz, tem[idx], hx, hy, hz, omegag, alpha[idx]);
+        ans_macro(dp, f, vx, vy, vz, a, bx, by, bz, mx, my, mz);
+        double coef = beta[idx] * (1.0 + it) / (1.0 + beta[idx]);
+        for (int c = 0; c < dq.nc; ++c) g[c] -= coef;
+        aad_macro(dq, g, it, hx, hy, hz);
+        if (issave) {
+            ip[idx] = a; iux[idx] = bx; iuy[idx] = by; put(iuz, idx, bz);
+            imx[idx] = mx; imy[idx] = my; put(imz, idx, mz);
+            item[idx] = it; iqx[idx] = hx; iqy[idx] = hy; put(iqz, idx, hz);
+        }
+        ans_eq(dp, feq, vx, vy, vz, a, bx, by, bz);
+        bgk(dp, f, feq, omegaf, iomegaf);
+        aad_eq(dq, geq, it, hx, hy, hz, vx, vy, vz);
+        bgk(dq, g, geq, omegag, iomegag);
+        store(p, idx, f);
+        store(q, idx, g);
+    }
+}
 /* adjointadvection.h:1385-1401 */
 void ol_aad_iset_t(ol_lattice *q, const int *mask, const double *ux, const double *uy, const double *uz) {
     const bool d3 = q->d->nd == 3;
@@ -1284,6 +1355,16 @@ void ol_sens_brinkman(const ol_lattice *p, double *dfds, const double *ux, const
     for (int idx = 0; idx < p->N; ++idx) {
         if (p->d->nd == 2) dfds[idx] += 3.0 * dads[idx] * (ux[idx] * imx[idx] + uy[idx] * imy[idx]);
         else dfds[idx] += 3.0 * dads[idx] * (ux[idx] * imx[idx] + uy[idx] * imy[idx] + uz[idx] * imz[idx]);
+    }
+}
+/* adjointadvection.h:1445-1457 (SensitivityHeatExchange) */
+void ol_sens_heat_exchange(const ol_lattice *q, double *dfds, const double *ux, const double *uy, const double *uz,
+                           const double *imx, const double *imy, const double *imz, const double *dads,
+                           const double *tem, const double *item, const double *dbds) {
+    for (int idx = 0; idx < q->N; ++idx) {
+        double um = q->d->nd == 2 ? ux[idx] * imx[idx] + uy[idx] * imy[idx]
+                                  : ux[idx] * imx[idx] + uy[idx] * imy[idx] + uz[idx] * imz[idx];
+        dfds[idx] += 3.0 * dads[idx] * um - dbds[idx] * (1.0 - tem[idx]) * (1.0 + item[idx]);
     }
 }
 /* adjointadvection.h:1461-1493 */

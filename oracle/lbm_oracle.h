@@ -68,6 +68,9 @@ void ol_ad_collide_fc(ol_lattice *p, double *rho, double *ux, double *uy, double
 void ol_ad_collide_nc(ol_lattice *p, double *rho, double *ux, double *uy, double *uz, const double *alpha, double nu,
                       ol_lattice *q, double *tem, double *qx, double *qy, double *qz, const double *diff, double gx,
                       double gy, double gz, double tem0, int issave, double *gsnap);
+void ol_ad_collide_he(ol_lattice *p, double *rho, double *ux, double *uy, double *uz, const double *alpha, double nu,
+                      ol_lattice *q, double *tem, double *qx, double *qy, double *qz, const double *beta, double diff,
+                      int issave);
 void ol_ad_set_t(ol_lattice *q, const int *mask, const double *tem, const double *ux, const double *uy, const double *uz);
 /* diff_field == NULL selects the constant-diffusivity overload (advection.h:242 / :385) */
 void ol_ad_set_q(ol_lattice *q, const int *mask, const double *qn, const double *ux, const double *uy, const double *uz,
@@ -97,6 +100,10 @@ void ol_aad_collide_nc(ol_lattice *p, const double *rho, const double *ux, const
                        const double *alpha, double nu, ol_lattice *q, const double *tem, double *item, double *iqx,
                        double *iqy, double *iqz, const double *diff, double gx, double gy, double gz, int issave,
                        double *igsnap);
+void ol_aad_collide_he(ol_lattice *p, const double *rho, const double *ux, const double *uy, const double *uz,
+                       double *ip, double *iux, double *iuy, double *iuz, double *imx, double *imy, double *imz,
+                       const double *alpha, double nu, ol_lattice *q, const double *tem, double *item, double *iqx,
+                       double *iqy, double *iqz, const double *beta, double diff, int issave);
 void ol_aad_iset_t(ol_lattice *q, const int *mask, const double *ux, const double *uy, const double *uz);
 void ol_aad_iset_q(ol_lattice *q, const int *mask, const double *ux, const double *uy, const double *uz, double eps);
 /* type face array: 1 = SetT, 2 = SetQ (adjointadvection.h:15-18) */
@@ -106,6 +113,9 @@ void ol_aad_iset_rho(ol_lattice *p, ol_lattice *q, const int *type, const double
 /* sensitivities: accumulate (+=) into dfds.  Snapshots g/ig are SoA nc*N. */
 void ol_sens_brinkman(const ol_lattice *p, double *dfds, const double *ux, const double *uy, const double *uz,
                       const double *imx, const double *imy, const double *imz, const double *dads);
+void ol_sens_heat_exchange(const ol_lattice *q, double *dfds, const double *ux, const double *uy, const double *uz,
+                           const double *imx, const double *imy, const double *imz, const double *dads,
+                           const double *tem, const double *item, const double *dbds);
 void ol_sens_brinkman_diffusivity(const ol_lattice *q, double *dfds, const double *ux, const double *uy,
                                   const double *uz, const double *imx, const double *imy, const double *imz,
                                   const double *dads, const double *tem, const double *item, const double *iqx,

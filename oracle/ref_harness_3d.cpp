@@ -173,6 +173,17 @@ void ref3_ad_collide_nc(int lx, int ly, int lz, double *f0, double *f, double *g
         save(p, f0, f); save(q, g0, g);
     }
 }
+void ref3_ad_collide_he(int lx, int ly, int lz, double *f0, double *f, double *g0, double *g, double *rho,
+                       double *ux, double *uy, double *uz, const double *alpha, double nu, double *tem, double *qx,
+                       double *qy, double *qz, const double *beta, double diff, int issave) {
+    {
+        D3Q15<double> p(lx, ly, lz), q(lx, ly, lz);
+        load(p, f0, f); load(q, g0, g);
+        AD::MacroBrinkmanCollideHeatExchange(p, rho, ux, uy, uz, alpha, nu, q, tem, qx, qy, qz, beta, diff,
+                                             issave != 0);
+        save(p, f0, f); save(q, g0, g);
+    }
+}
 void ref3_ans_collide(int lx, int ly, int lz, double *f0, double *f, const double *rho, const double *ux,
                      const double *uy, const double *uz, double *ip, double *iux, double *iuy, double *iuz, double *imx,
                      double *imy, double *imz, double nu, const double *alpha, int issave) {
@@ -205,6 +216,26 @@ void ref3_aad_collide_nc(int lx, int ly, int lz, double *f0, double *f, double *
                                                    tem, item, iqx, iqy, iqz, diff, gx, gy, gz, issave != 0, igsnap);
         save(p, f0, f); save(q, g0, g);
     }
+}
+
+void ref3_aad_collide_he(int lx, int ly, int lz, double *f0, double *f, double *g0, double *g,
+                        const double *rho, const double *ux, const double *uy, const double *uz, double *ip, double *iux,
+                        double *iuy, double *iuz, double *imx, double *imy, double *imz, const double *alpha, double nu,
+                        const double *tem, double *item, double *iqx, double *iqy, double *iqz, const double *beta,
+                        double diff, int issave) {
+    {
+        D3Q15<double> p(lx, ly, lz), q(lx, ly, lz);
+        load(p, f0, f); load(q, g0, g);
+        AAD::MacroBrinkmanCollideHeatExchange(p, rho, ux, uy, uz, ip, iux, iuy, iuz, imx, imy, imz, alpha, nu, q,
+                                              tem, item, iqx, iqy, iqz, beta, diff, issave != 0);
+        save(p, f0, f); save(q, g0, g);
+    }
+}
+void ref3_sens_heat_exchange(int lx, int ly, int lz, double *dfds, const double *ux, const double *uy,
+                               const double *uz, const double *imx, const double *imy, const double *imz,
+                               const double *dads, const double *tem, const double *item, const double *dbds) {
+    D3Q15<double> p(lx, ly, lz);
+    { AAD::SensitivityHeatExchange(p, dfds, ux, uy, uz, imx, imy, imz, dads, tem, item, dbds); }
 }
 
 /* snapshots g / ig in the reference layout nc*idx + c */

@@ -87,6 +87,11 @@ def load():
         "lbm_ad_initial_condition": [VP] * 5,
         "lbm_ad_macro_brinkman_collide_force_convection": [VP] * 6 + [D] + [VP] * 6 + [I, VP, VP],
         "lbm_ad_macro_brinkman_collide_natural_convection": [VP] * 6 + [D] + [VP] * 6 + [D] * 4 + [I, VP],
+        "lbm_ad_macro_brinkman_collide_heat_exchange": [VP] * 6 + [D] + [VP] * 6 + [D, I],
+        "lbm_ad_macro_brinkman_collide_heat_exchange_p": [VP] * 7 + [D] + [VP] * 6 + [D, I],
+        "lbm_aad_macro_brinkman_collide_heat_exchange": [VP] * 13 + [D] + [VP] * 7 + [D, I],
+        "lbm_aad_macro_brinkman_collide_heat_exchange_p": [VP] * 14 + [D] + [VP] * 7 + [D, I],
+        "lbm_sensitivity_heat_exchange": [VP] * 12,
         "lbm_ad_set_t": [VP] * 6,
         "lbm_ad_set_q": [VP] * 7 + [D],
         "lbm_ans_initial_condition": [VP] * 8,

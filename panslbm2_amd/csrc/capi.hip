@@ -986,6 +986,27 @@ int lbm_ad_macro_brinkman_collide_natural_convection(lbm_lattice *p, double *rho
     return lbm_ad_macro_brinkman_collide_natural_convection_p(nullptr, p, rho, ux, uy, uz, alpha, nu, q, tem, qx, qy,
                                                               qz, diff, gx, gy, gz, tem0, issave, g);
 }
+int lbm_ad_macro_brinkman_collide_heat_exchange_p(lbm_program *P, lbm_lattice *p, double *rho, double *ux,
+                                                  double *uy, double *uz, const double *alpha, double nu,
+                                                  lbm_lattice *q, double *tem, double *qx, double *qy, double *qz,
+                                                  const double *beta, double diffusivity, int issave) {
+    if (!ns_lattice(p) || !q || !alpha || !beta) return fail(LBM_ERR_ARG, "bad arguments");
+    if (issave && (!fields_ok({rho, ux, uy, tem, qx, qy}) || (p->nd == 3 && (!uz || !qz))))
+        return fail(LBM_ERR_ARG, "null field");
+    return collide_call(P, p, q, {rho, ux, uy, uz, tem, qx, qy, qz}, [&](const lbm::CollideLaunch &cl) {
+        lbm::ADArgs a{rho, ux, uy, uz, tem, qx, qy, qz, nullptr, alpha, nullptr, nullptr, nu, issave};
+        a.beta = beta;
+        a.diffc = diffusivity;
+        return hip_result(lbm::launch_ad_collide(cl, a), "ad collide heat exchange");
+    });
+}
+int lbm_ad_macro_brinkman_collide_heat_exchange(lbm_lattice *p, double *rho, double *ux, double *uy, double *uz,
+                                                const double *alpha, double nu, lbm_lattice *q, double *tem,
+                                                double *qx, double *qy, double *qz, const double *beta,
+                                                double diffusivity, int issave) {
+    return lbm_ad_macro_brinkman_collide_heat_exchange_p(nullptr, p, rho, ux, uy, uz, alpha, nu, q, tem, qx, qy, qz,
+                                                         beta, diffusivity, issave);
+}
 int lbm_ad_set_t(lbm_lattice *q, const int *mask, const double *tem, const double *ux, const double *uy,
                  const double *uz) {
     if (!q || !mask || !tem || !ux || !uy || (q->nd == 3 && !uz)) return fail(LBM_ERR_ARG, "bad arguments");
@@ -1125,6 +1146,35 @@ int lbm_aad_macro_brinkman_collide_natural_convection(lbm_lattice *p, const doub
                                                                imy, imz, alpha, nu, q, tem, item, iqx, iqy, iqz,
                                                                diff, gx, gy, gz, issave, ig);
 }
+int lbm_aad_macro_brinkman_collide_heat_exchange_p(lbm_program *P, lbm_lattice *p, const double *rho,
+                                                   const double *ux, const double *uy, const double *uz, double *ip,
+                                                   double *iux, double *iuy, double *iuz, double *imx, double *imy,
+                                                   double *imz, const double *alpha, double nu, lbm_lattice *q,
+                                                   const double *tem, double *item, double *iqx, double *iqy,
+                                                   double *iqz, const double *beta, double diffusivity, int issave) {
+    if (!ns_lattice(p) || !q || !fields_ok({rho, ux, uy, alpha, tem, beta}) || (p->nd == 3 && !uz))
+        return fail(LBM_ERR_ARG, "bad arguments");
+    if (issave &&
+        (!fields_ok({ip, iux, iuy, imx, imy, item, iqx, iqy}) || (p->nd == 3 && (!iuz || !imz || !iqz))))
+        return fail(LBM_ERR_ARG, "null field");
+    return collide_call(P, p, q, {ip, iux, iuy, iuz, imx, imy, imz, item, iqx, iqy, iqz}, [&](const lbm::CollideLaunch &cl) {
+        lbm::AADArgs a{rho, ux, uy, uz, alpha, tem, nullptr, ip, iux, iuy, iuz, imx, imy, imz, item, iqx, iqy, iqz,
+                       nullptr, nu, issave};
+        a.beta = beta;
+        a.diffc = diffusivity;
+        return hip_result(lbm::launch_aad_collide(cl, a), "aad collide heat exchange");
+    });
+}
+int lbm_aad_macro_brinkman_collide_heat_exchange(lbm_lattice *p, const double *rho, const double *ux,
+                                                 const double *uy, const double *uz, double *ip, double *iux,
+                                                 double *iuy, double *iuz, double *imx, double *imy, double *imz,
+                                                 const double *alpha, double nu, lbm_lattice *q, const double *tem,
+                                                 double *item, double *iqx, double *iqy, double *iqz,
+                                                 const double *beta, double diffusivity, int issave) {
+    return lbm_aad_macro_brinkman_collide_heat_exchange_p(nullptr, p, rho, ux, uy, uz, ip, iux, iuy, iuz, imx, imy,
+                                                          imz, alpha, nu, q, tem, item, iqx, iqy, iqz, beta,
+                                                          diffusivity, issave);
+}
 int lbm_aad_iset_t(lbm_lattice *q, const int *mask, const double *ux, const double *uy, const double *uz) {
     if (!q || !mask || !ux || !uy || (q->nd == 3 && !uz)) return fail(LBM_ERR_ARG, "bad arguments");
     FieldSet fs{nullptr, ux, uy, uz, nullptr, nullptr};
@@ -1162,6 +1212,14 @@ int lbm_sensitivity_brinkman(lbm_lattice *p, double *dfds, const double *ux, con
         return fail(LBM_ERR_ARG, "bad arguments");
     const double *a[7] = {ux, uy, uz, imx, imy, imz, dads};
     return hip_result(lbm::launch_sens_brinkman(p->nd, dfds, p->N, a, p->ctx->stream), "sensitivity brinkman");
+}
+int lbm_sensitivity_heat_exchange(lbm_lattice *q, double *dfds, const double *ux, const double *uy, const double *uz,
+                                  const double *imx, const double *imy, const double *imz, const double *dads,
+                                  const double *tem, const double *item, const double *dbds) {
+    if (!q || !fields_ok({dfds, ux, uy, imx, imy, dads, tem, item, dbds}) || (q->nd == 3 && (!uz || !imz)))
+        return fail(LBM_ERR_ARG, "bad arguments");
+    const double *a[10] = {ux, uy, uz, imx, imy, imz, dads, tem, item, dbds};
+    return hip_result(lbm::launch_sens_he(q->nd, dfds, q->N, a, q->ctx->stream), "sensitivity heat exchange");
 }
 int lbm_sensitivity_brinkman_diffusivity(lbm_lattice *q, double *dfds, const double *ux, const double *uy,
                                          const double *uz, const double *imx, const double *imy, const double *imz,

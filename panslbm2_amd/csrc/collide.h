@@ -50,20 +50,22 @@ struct NSBody<LA, LB, double> {
     }
 };
 
-// AD::MacroBrinkmanCollideForceConvection, advection.h:814-926 (+ optional volumetric source), and
-// AD::MacroBrinkmanCollideNaturalConvection, advection.h:930-1044 (natural = 1)
+// AD::MacroBrinkmanCollideForceConvection, advection.h:814-926 (+ optional volumetric source),
+// AD::MacroBrinkmanCollideNaturalConvection, advection.h:930-1044 (natural = 1), and
+// AD::MacroBrinkmanCollideHeatExchange, advection.h:716-810 (beta set, constant diffusivity diffc).
+// The exchange term changes only g and uses T of the first macro, so it sits with the source term.
 template <class LA, class LB, class R>
 struct ADBody;
 template <class LA, class LB>
 struct ADBody<LA, LB, double> {
     double *rho, *ux, *uy, *uz, *tem, *qx, *qy, *qz, *gsnap;
-    const double *alpha, *diff, *heat;
-    double omegaf, iomegaf;
+    const double *alpha, *diff, *heat, *beta;
+    double omegaf, iomegaf, diffc;
     int save, N, natural;
     double gx, gy, gz, tem0;
     Enc ea, eb;
     __device__ __forceinline__ void operator()(double *f, double *g, int n, bool bnd, const Sh4 &, const Sh4 &) const {
-        double omegag = 1.0 / (3.0 * diff[n] + 0.5), iomegag = 1.0 - omegag;
+        double omegag = 1.0 / (3.0 * (diff ? diff[n] : diffc) + 0.5), iomegag = 1.0 - omegag;
         double r, vx, vy, vz, T, hx, hy, hz;
         ns_macro<LA>(f, r, vx, vy, vz);
         ad_macro<LB>(g, vx, vy, vz, omegag, T, hx, hy, hz);
@@ -80,6 +82,13 @@ struct ADBody<LA, LB, double> {
             sfor<0, LB::nc>([&](auto c) {
                 constexpr int C = decltype(c)::value;
                 g[C] += LB::w[C] * h;
+            });
+        }
+        if (beta) {  // ExternalForceHeatExchange, advection.h:89-95
+            double b = beta[n], coef = b * (1.0 - T) / (1.0 + b);
+            sfor<0, LB::nc>([&](auto c) {
+                constexpr int C = decltype(c)::value;
+                g[C] += LB::w[C] * coef;
             });
         }
         ns_macro<LA>(f, r, vx, vy, vz);
@@ -142,20 +151,22 @@ struct ANSBody<LA, LB, double> {
     }
 };
 
-// AAD::MacroBrinkmanCollideForceConvection, adjointadvection.h:997-1109, and
-// AAD::MacroBrinkmanCollideNaturalConvection, adjointadvection.h:1113-1230 (natural = 1)
+// AAD::MacroBrinkmanCollideForceConvection, adjointadvection.h:997-1109,
+// AAD::MacroBrinkmanCollideNaturalConvection, adjointadvection.h:1113-1230 (natural = 1), and
+// AAD::MacroBrinkmanCollideHeatExchange, adjointadvection.h:894-993 (beta set, constant diffusivity)
 template <class LA, class LB, class R>
 struct AADBody;
 template <class LA, class LB>
 struct AADBody<LA, LB, double> {
     const double *rho, *ux, *uy, *uz, *alpha, *tem, *diff;
     double *ip, *iux, *iuy, *iuz, *imx, *imy, *imz, *item, *iqx, *iqy, *iqz, *igsnap;
-    double omegaf, iomegaf;
+    const double *beta;
+    double omegaf, iomegaf, diffc;
     int save, N, natural;
     double gx, gy, gz;
     Enc ea, eb;
     __device__ __forceinline__ void operator()(double *f, double *g, int n, bool bnd, const Sh4 &, const Sh4 &) const {
-        double omegag = 1.0 / (3.0 * diff[n] + 0.5), iomegag = 1.0 - omegag;
+        double omegag = 1.0 / (3.0 * (diff ? diff[n] : diffc) + 0.5), iomegag = 1.0 - omegag;
         double vx = ux[n], vy = uy[n], vz = LA::nd == 3 ? uz[n] : 0.0;
         double a, bx, by, bz, mx, my, mz, it, hx, hy, hz;
         ans_macro<LA>(f, vx, vy, vz, a, bx, by, bz, mx, my, mz);
@@ -169,6 +180,11 @@ struct AADBody<LA, LB, double> {
             sfor<0, LB::nc>([&](auto c) { g[decltype(c)::value] += coef; });
             aad_macro<LB>(g, it, hx, hy, hz);
         }
+        if (beta) {  // ExternalForceHeatExchange, adjointadvection.h:106-112, then re-macro
+            double b = beta[n], coef = b * (1.0 + it) / (1.0 + b);
+            sfor<0, LB::nc>([&](auto c) { g[decltype(c)::value] -= coef; });
+            aad_macro<LB>(g, it, hx, hy, hz);
+        }
         if (save == 1 || (save == 2 && bnd)) {
             ip[n] = a;
             iux[n] = bx;
@@ -179,7 +195,7 @@ struct AADBody<LA, LB, double> {
             iqx[n] = hx;
             iqy[n] = hy;
             if constexpr (LA::nd == 3) {  // the 3D ForceConvection save writes no iuz (adjointadvection.h:1074-1084)
-                if (natural) iuz[n] = bz;
+                if (natural || beta) iuz[n] = bz;
                 imz[n] = mz;
                 iqz[n] = hz;
             }
@@ -317,14 +333,14 @@ struct NSBody<LA, LB, float> {
 template <class LA, class LB>
 struct ADBody<LA, LB, float> {
     double *rho, *ux, *uy, *uz, *tem, *qx, *qy, *qz, *gsnap;
-    const double *alpha, *diff, *heat;
-    float omegaf, iomegaf;
+    const double *alpha, *diff, *heat, *beta;
+    float omegaf, iomegaf, diffc;
     int save, N, natural;
     float gx, gy, gz, tem0;
     Enc ea, eb;
     __device__ __forceinline__ void operator()(float *f, float *g, int n, bool bnd, const Sh4 &oa,
                                                const Sh4 &ob) const {
-        float omegag = 1.0f / (3.0f * float(diff[n]) + 0.5f), iomegag = 1.0f - omegag;
+        float omegag = 1.0f / (3.0f * (diff ? float(diff[n]) : diffc) + 0.5f), iomegag = 1.0f - omegag;
         float dr, jx, jy, jz, r, vx, vy, vz, dT, T, hx, hy, hz, px, py, pz;
         ns_macro_d<LA>(f, oa, dr, jx, jy, jz, r, vx, vy, vz);
         ad_macro_d<LB>(g, ob, vx, vy, vz, omegag, dT, T, hx, hy, hz, px, py, pz);
@@ -341,6 +357,13 @@ struct ADBody<LA, LB, float> {
             sfor<0, LB::nc>([&](auto c) {
                 constexpr int C = decltype(c)::value;
                 g[C] += float(LB::w[C]) * h;
+            });
+        }
+        if (beta) {
+            float b = float(beta[n]), coef = b * (1.0f - T) / (1.0f + b);
+            sfor<0, LB::nc>([&](auto c) {
+                constexpr int C = decltype(c)::value;
+                g[C] += float(LB::w[C]) * coef;
             });
         }
         ns_macro_d<LA>(f, oa, dr, jx, jy, jz, r, vx, vy, vz);
@@ -406,13 +429,14 @@ template <class LA, class LB>
 struct AADBody<LA, LB, float> {
     const double *rho, *ux, *uy, *uz, *alpha, *tem, *diff;
     double *ip, *iux, *iuy, *iuz, *imx, *imy, *imz, *item, *iqx, *iqy, *iqz, *igsnap;
-    float omegaf, iomegaf;
+    const double *beta;
+    float omegaf, iomegaf, diffc;
     int save, N, natural;
     float gx, gy, gz;
     Enc ea, eb;
     __device__ __forceinline__ void operator()(float *f, float *g, int n, bool bnd, const Sh4 &oa,
                                                const Sh4 &ob) const {
-        float omegag = 1.0f / (3.0f * float(diff[n]) + 0.5f), iomegag = 1.0f - omegag;
+        float omegag = 1.0f / (3.0f * (diff ? float(diff[n]) : diffc) + 0.5f), iomegag = 1.0f - omegag;
         float vx = float(ux[n]), vy = float(uy[n]), vz = LA::nd == 3 ? float(uz[n]) : 0.0f;
         float a, bx, by, bz, mx, my, mz, it, hx, hy, hz;
         add_shift_f<LA>(f, ea, oa, 1.0f);
@@ -428,6 +452,11 @@ struct AADBody<LA, LB, float> {
             sfor<0, LB::nc>([&](auto c) { g[decltype(c)::value] += coef; });
             aad_macro<LB>(g, it, hx, hy, hz);
         }
+        if (beta) {
+            float b = float(beta[n]), coef = b * (1.0f + it) / (1.0f + b);
+            sfor<0, LB::nc>([&](auto c) { g[decltype(c)::value] -= coef; });
+            aad_macro<LB>(g, it, hx, hy, hz);
+        }
         if (save == 1 || (save == 2 && bnd)) {
             ip[n] = a;
             iux[n] = bx;
@@ -438,7 +467,7 @@ struct AADBody<LA, LB, float> {
             iqx[n] = hx;
             iqy[n] = hy;
             if constexpr (LA::nd == 3) {
-                if (natural) iuz[n] = bz;
+                if (natural || beta) iuz[n] = bz;
                 imz[n] = mz;
                 iqz[n] = hz;
             }
@@ -772,6 +801,7 @@ hipError_t ad_collide_nd(const CollideLaunch &cl, const ADArgs &a) {
         b.gsnap = a.gsnap; b.alpha = a.alpha; b.diff = a.diff; b.heat = a.heat;
         b.omegaf = R(omega); b.iomegaf = R(1.0 - omega); b.save = a.save;
         b.natural = a.natural; b.gx = R(a.gx); b.gy = R(a.gy); b.gz = R(a.gz); b.tem0 = R(a.tem0);
+        b.beta = a.beta; b.diffc = R(a.diffc);
         return b;
     });
 }
@@ -798,6 +828,7 @@ hipError_t aad_collide_nd(const CollideLaunch &cl, const AADArgs &a) {
         b.item = a.item; b.iqx = a.iqx; b.iqy = a.iqy; b.iqz = a.iqz; b.igsnap = a.igsnap;
         b.omegaf = R(omega); b.iomegaf = R(1.0 - omega); b.save = a.save;
         b.natural = a.natural; b.gx = R(a.gx); b.gy = R(a.gy); b.gz = R(a.gz);
+        b.beta = a.beta; b.diffc = R(a.diffc);
         return b;
     });
 }

@@ -48,6 +48,8 @@ struct ADArgs {
     int save;
     int natural;                  // 1: MacroBrinkmanCollideNaturalConvection (buoyancy g (T - tem0))
     double gx, gy, gz, tem0;
+    const double *beta;           // MacroBrinkmanCollideHeatExchange: per-node exchange coefficient
+    double diffc;                 // diffusivity when diff is null (heat exchange: a constant)
 };
 struct ANSArgs {
     const double *rho, *ux, *uy, *uz, *alpha;
@@ -62,6 +64,8 @@ struct AADArgs {
     int save;
     int natural;                  // 1: MacroBrinkmanCollideNaturalConvection
     double gx, gy, gz;
+    const double *beta;           // MacroBrinkmanCollideHeatExchange
+    double diffc;                 // diffusivity when diff is null
 };
 
 hipError_t launch_stream(int kind, int prec, const void *src, void *dst, Grid g, int pull, const Enc &e, hipStream_t s);
@@ -94,6 +98,7 @@ hipError_t launch_convert(int kind, int from, const void *src, const Enc &es, in
 hipError_t launch_encode(int kind, int prec, const double *src, void *dst, int N, const Enc &e, hipStream_t s);
 hipError_t launch_decode(int kind, int prec, const void *src, double *dst, int N, const Enc &e, hipStream_t s);
 hipError_t launch_sens_brinkman(int nd, double *dfds, int N, const double *const *p, hipStream_t s);
+hipError_t launch_sens_he(int nd, double *dfds, int N, const double *const *p, hipStream_t s);
 hipError_t launch_sens_bd(int kind, double *dfds, int N, const double *const *p, hipStream_t s);
 hipError_t launch_sens_hs_boundary(double *dfds, int N, const ProgView &pv, const double *ux, const double *uy,
                                    const double *ig, const double *diff, const double *dkds, hipStream_t s);

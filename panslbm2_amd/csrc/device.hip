@@ -130,6 +130,20 @@ __global__ void __launch_bounds__(kBlock) k_sens_brinkman(double *dfds, int N, c
     else dfds[n] += 3.0 * dads[n] * (ux[n] * imx[n] + uy[n] * imy[n] + uz[n] * imz[n]);
 }
 
+// adjointadvection.h:1445-1457 (SensitivityHeatExchange)
+template <int ND>
+__global__ void __launch_bounds__(kBlock) k_sens_he(double *dfds, int N, const double *ux, const double *uy,
+                                                    const double *uz, const double *imx, const double *imy,
+                                                    const double *imz, const double *dads, const double *tem,
+                                                    const double *item, const double *dbds) {
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    double um;
+    if constexpr (ND == 2) um = ux[n] * imx[n] + uy[n] * imy[n];
+    else um = ux[n] * imx[n] + uy[n] * imy[n] + uz[n] * imz[n];
+    dfds[n] += 3.0 * dads[n] * um - dbds[n] * (1.0 - tem[n]) * (1.0 + item[n]);
+}
+
 template <class L>
 __global__ void __launch_bounds__(kBlock) k_sens_bd(double *dfds, int N, const double *ux, const double *uy,
                                                     const double *uz, const double *imx, const double *imy,
@@ -465,6 +479,13 @@ hipError_t launch_decode(int kind, int prec, const void *src, double *dst, int N
 hipError_t launch_sens_brinkman(int nd, double *dfds, int N, const double *const *p, hipStream_t s) {
     if (nd == 2) LBM_LAUNCH((k_sens_brinkman<2>), nblocks(N), s, dfds, N, p[0], p[1], p[2], p[3], p[4], p[5], p[6]);
     else LBM_LAUNCH((k_sens_brinkman<3>), nblocks(N), s, dfds, N, p[0], p[1], p[2], p[3], p[4], p[5], p[6]);
+    return hipGetLastError();
+}
+hipError_t launch_sens_he(int nd, double *dfds, int N, const double *const *p, hipStream_t s) {
+    if (nd == 2)
+        LBM_LAUNCH((k_sens_he<2>), nblocks(N), s, dfds, N, p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], p[9]);
+    else
+        LBM_LAUNCH((k_sens_he<3>), nblocks(N), s, dfds, N, p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], p[9]);
     return hipGetLastError();
 }
 hipError_t launch_sens_bd(int kind, double *dfds, int N, const double *const *p, hipStream_t s) {

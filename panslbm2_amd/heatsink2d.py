@@ -214,9 +214,9 @@ class Heatsink2D:
 
     def _solve(self, key, step, res):
         """Chunks of check_every steps until both residuals are below eps.  In FP32 the residual is
-        watched: once it is below fp32_floor or has not fallen by 5 % in ten checks, the lattices
-        convert to FP64 and the solve continues there.  Returns (steps, steps in FP32, final
-        residuals)."""
+        watched: once it is below fp32_floor, or below 100 fp32_floor and not 5 % lower in ten
+        checks (plateaus of the transient above that do not count), the lattices convert to FP64 and
+        the solve continues there.  Returns (steps, steps in FP32, final residuals)."""
         ce = self.check_every
         t, t32, best, stall = 0, 0, np.inf, 0
         r = (np.inf, np.inf)
@@ -235,7 +235,7 @@ class Heatsink2D:
                 t32 = t
                 stall = 0 if worst < 0.95 * best else stall + 1
                 best = min(best, worst)
-                if worst < self.fp32_floor or stall >= 10:
+                if worst < self.fp32_floor or (stall >= 10 and worst < 100.0 * self.fp32_floor):
                     self._set_prec(F64)
         return t, (t32 if self.prec == F32 else 0), r
 

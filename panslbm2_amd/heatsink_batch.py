@@ -8,8 +8,8 @@ The step order per instance is exactly the single-instance driver's, so with FP6
 instance's result is value-identical to solving it alone.
 
 Precision as in Heatsink2D: with prec=F32 all instances run in FP32 until every active one's
-residual has reached the FP32 floor or stopped falling (not 5 % lower in ten checks), then every
-lattice converts to FP64 and the solves finish at eps.  An instance that has converged is frozen (its workgroups return at once)
+residual has reached the FP32 floor, or is within 100x of it and has stopped falling (not 5 %
+lower in ten checks), then every lattice converts to FP64 and the solves finish at eps.  An instance that has converged is frozen (its workgroups return at once)
 at that check step, so each instance stops where it would have stopped alone.
 """
 import ctypes
@@ -151,7 +151,8 @@ class HeatsinkBatch2D:
             if self.hs[0].pf.prec == F32:
                 stall = np.where(r < 0.95 * best, 0, stall + 1)
                 best = np.minimum(best, r)
-                if np.all((r[act] < self.fp32_floor) | (stall[act] >= 10)):
+                ready = (r < self.fp32_floor) | ((stall >= 10) & (r < 100 * self.fp32_floor))
+                if np.all(ready[act]):
                     self.t32 = t
                     if graph is not None:
                         self.L.lbm_graph_destroy(graph)

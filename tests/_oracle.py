@@ -66,6 +66,7 @@ def lib():
         L.ol_ad_init.argtypes = [VP] * 5
         L.ol_ad_collide_fc.argtypes = [VP] + [VP] * 5 + [D] + [VP] * 6 + [I, VP, VP]
         L.ol_ad_collide_nc.argtypes = [VP] + [VP] * 5 + [D] + [VP] * 6 + [D] * 4 + [I, VP]
+        L.ol_ad_collide_he.argtypes = [VP] + [VP] * 5 + [D] + [VP] * 6 + [D, I]
         L.ol_ad_set_t.argtypes = [VP] * 6
         L.ol_ad_set_q.argtypes = [VP] * 7 + [D]
         L.ol_ans_init.argtypes = [VP] * 8
@@ -75,10 +76,12 @@ def lib():
         L.ol_aad_init.argtypes = [VP] * 8
         L.ol_aad_collide_fc.argtypes = [VP] + [VP] * 11 + [VP, D, VP] + [VP] * 5 + [VP, I, VP]
         L.ol_aad_collide_nc.argtypes = [VP] + [VP] * 11 + [VP, D, VP] + [VP] * 5 + [VP] + [D] * 3 + [I, VP]
+        L.ol_aad_collide_he.argtypes = [VP] + [VP] * 11 + [VP, D, VP] + [VP] * 5 + [VP, D, I]
         L.ol_aad_iset_t.argtypes = [VP] * 5
         L.ol_aad_iset_q.argtypes = [VP] * 5 + [D]
         L.ol_aad_iset_rho.argtypes = [VP] * 8 + [D]
         L.ol_sens_brinkman.argtypes = [VP] * 9
+        L.ol_sens_heat_exchange.argtypes = [VP] * 12
         L.ol_sens_brinkman_diffusivity.argtypes = [VP] * 18
         L.ol_sens_temp_heat_source.argtypes = [VP] * 20
         L.ol_residual.argtypes = [I] + [VP] * 6 + [I]

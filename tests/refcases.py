@@ -41,6 +41,12 @@ NC_G = (1.3e-3, -2.1e-3, 1.7e-3)
 NC_TEM0 = 0.4
 
 
+def he_inputs(inp):
+    """Heat-exchange coefficient beta (per node), its design derivative and the constant
+    diffusivity, derived from the seeded inputs."""
+    return 0.1 * inp["alpha"], inp["dkds"], float(inp["dconst"])
+
+
 def _i32(a):
     return np.ascontiguousarray(a, dtype=np.int32)
 
@@ -161,6 +167,13 @@ def run_ref(case, op, inp):
                             ptr(o["qz"]), ptr(inp["diff"]), *[D(x) for x in NC_G], D(NC_TEM0), 1, ptr(snap))
         out.update(o)
         out["gsnap"] = snap.reshape(case.N, nc).T.copy()
+    elif op == "ad_collide_he":
+        beta, _, dc = he_inputs(inp)
+        o = {k: np.zeros(case.N) for k in ("rho", "ux", "uy", "uz", "tem", "qx", "qy", "qz")}
+        fn("ad_collide_he")(*L, ptr(f0), ptr(f), ptr(g0), ptr(g), ptr(o["rho"]), ptr(o["ux"]), ptr(o["uy"]),
+                            ptr(o["uz"]), ptr(inp["alpha"]), D(0.1), ptr(o["tem"]), ptr(o["qx"]), ptr(o["qy"]),
+                            ptr(o["qz"]), ptr(beta), D(dc), 1)
+        out.update(o)
     elif op == "ans_collide":
         o = {k: np.zeros(case.N) for k in ("ip", "iux", "iuy", "iuz", "imx", "imy", "imz")}
         fn("ans_collide")(*L, ptr(f0), ptr(f), ptr(inp["rho"]), ptr(inp["ux"]), ptr(inp["uy"]), ptr(uz),
@@ -186,6 +199,20 @@ def run_ref(case, op, inp):
                              ptr(snap))
         out.update(o)
         out["igsnap"] = snap.reshape(case.N, nc).T.copy()
+    elif op == "aad_collide_he":
+        beta, _, dc = he_inputs(inp)
+        o = {k: np.zeros(case.N) for k in ("ip", "iux", "iuy", "iuz", "imx", "imy", "imz", "item", "iqx", "iqy", "iqz")}
+        fn("aad_collide_he")(*L, ptr(f0), ptr(f), ptr(g0), ptr(g), ptr(inp["rho"]), ptr(inp["ux"]), ptr(inp["uy"]),
+                             ptr(uz), ptr(o["ip"]), ptr(o["iux"]), ptr(o["iuy"]), ptr(o["iuz"]), ptr(o["imx"]),
+                             ptr(o["imy"]), ptr(o["imz"]), ptr(inp["alpha"]), D(0.1), ptr(inp["tem"]), ptr(o["item"]),
+                             ptr(o["iqx"]), ptr(o["iqy"]), ptr(o["iqz"]), ptr(beta), D(dc), 1)
+        out.update(o)
+    elif op == "sens_heat_exchange":
+        _, dbds, _ = he_inputs(inp)
+        d = inp["dfds0"].copy()
+        fn("sens_heat_exchange")(*L, ptr(d), ptr(inp["ux"]), ptr(inp["uy"]), ptr(uz), ptr(inp["v0"]), ptr(inp["v1"]),
+                                 ptr(inp["v2"]), ptr(inp["dads"]), ptr(inp["tem"]), ptr(inp["rho"]), ptr(dbds))
+        out["dfds"] = d
     elif op == "sens_brinkman":
         d = inp["dfds0"].copy()
         fn("sens_brinkman")(*L, ptr(d), ptr(inp["ux"]), ptr(inp["uy"]), ptr(uz), ptr(inp["v0"]), ptr(inp["v1"]),
@@ -308,6 +335,12 @@ def run_oracle(case, op, inp):
                            *[D(x) for x in NC_G], D(NC_TEM0), 1, ptr(snap))
         out.update(o)
         out["gsnap"] = snap
+    elif op == "ad_collide_he":
+        beta, _, dc = he_inputs(inp)
+        o = {kk: np.zeros(case.N) for kk in ("rho", "ux", "uy", "uz", "tem", "qx", "qy", "qz")}
+        O.ol_ad_collide_he(p.h, ptr(o["rho"]), ptr(o["ux"]), ptr(o["uy"]), ptr(o["uz"]), ptr(inp["alpha"]), D(0.1),
+                           q.h, ptr(o["tem"]), ptr(o["qx"]), ptr(o["qy"]), ptr(o["qz"]), ptr(beta), D(dc), 1)
+        out.update(o)
     elif op == "ans_collide":
         o = {kk: np.zeros(case.N) for kk in ("ip", "iux", "iuy", "iuz", "imx", "imy", "imz")}
         O.ol_ans_collide(p.h, ptr(inp["rho"]), ptr(inp["ux"]), ptr(inp["uy"]), ptr(uz), ptr(o["ip"]), ptr(o["iux"]),
@@ -332,6 +365,20 @@ def run_oracle(case, op, inp):
                             ptr(o["iqy"]), ptr(o["iqz"]), ptr(inp["diff"]), *[D(x) for x in NC_G], 1, ptr(snap))
         out.update(o)
         out["igsnap"] = snap
+    elif op == "aad_collide_he":
+        beta, _, dc = he_inputs(inp)
+        o = {kk: np.zeros(case.N) for kk in ("ip", "iux", "iuy", "iuz", "imx", "imy", "imz", "item", "iqx", "iqy", "iqz")}
+        O.ol_aad_collide_he(p.h, ptr(inp["rho"]), ptr(inp["ux"]), ptr(inp["uy"]), ptr(uz), ptr(o["ip"]), ptr(o["iux"]),
+                            ptr(o["iuy"]), ptr(o["iuz"]), ptr(o["imx"]), ptr(o["imy"]), ptr(o["imz"]),
+                            ptr(inp["alpha"]), D(0.1), q.h, ptr(inp["tem"]), ptr(o["item"]), ptr(o["iqx"]),
+                            ptr(o["iqy"]), ptr(o["iqz"]), ptr(beta), D(dc), 1)
+        out.update(o)
+    elif op == "sens_heat_exchange":
+        _, dbds, _ = he_inputs(inp)
+        d = inp["dfds0"].copy()
+        O.ol_sens_heat_exchange(p.h, ptr(d), ptr(inp["ux"]), ptr(inp["uy"]), ptr(uz), ptr(inp["v0"]), ptr(inp["v1"]),
+                                ptr(inp["v2"]), ptr(inp["dads"]), ptr(inp["tem"]), ptr(inp["rho"]), ptr(dbds))
+        out["dfds"] = d
     elif op == "sens_brinkman":
         d = inp["dfds0"].copy()
         O.ol_sens_brinkman(p.h, ptr(d), ptr(inp["ux"]), ptr(inp["uy"]), ptr(uz), ptr(inp["v0"]), ptr(inp["v1"]),
@@ -387,6 +434,8 @@ OPS_2D = {
     "ad_collide_fc": "advection.h:814-867", "ans_collide": "adjointnavierstokes.h:398-431",
     "aad_collide_fc": "adjointadvection.h:997-1050",
     "ad_collide_nc": "advection.h:930-984,73-79", "aad_collide_nc": "adjointadvection.h:1113-1168,114-122",
+    "ad_collide_he": "advection.h:716-760,89-95", "aad_collide_he": "adjointadvection.h:894-940,106-112",
+    "sens_heat_exchange": "adjointadvection.h:1445-1450",
     "sens_brinkman": "adjointnavierstokes.h:542-546", "sens_brinkman_diffusivity": "adjointadvection.h:1461-1475",
     "sens_temp_heat_source": "adjointadvection.h:1497-1517,755-801",
     "residual": "residual.h:9-51", "normalize": "normalize.h:9-24", "density_filter": "densityfilter.h:390-420,512-515",
@@ -415,6 +464,8 @@ OPS_3D = {
     "ad_collide_fc": "advection.h:871-926", "ans_collide": "adjointnavierstokes.h:435-470",
     "aad_collide_fc": "adjointadvection.h:1054-1109",
     "ad_collide_nc": "advection.h:988-1044,81-87", "aad_collide_nc": "adjointadvection.h:1172-1230,124-132",
+    "ad_collide_he": "advection.h:764-810,89-95", "aad_collide_he": "adjointadvection.h:944-993,106-112",
+    "sens_heat_exchange": "adjointadvection.h:1452-1457",
     "sens_brinkman": "adjointnavierstokes.h:550-554", "sens_brinkman_diffusivity": "adjointadvection.h:1478-1493",
     "residual": "residual.h:39-51", "normalize": "normalize.h:9-24", "density_filter": "densityfilter.h:390-420,512-515",
 }

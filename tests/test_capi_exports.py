@@ -40,3 +40,49 @@ def test_library_reports_errors_without_gpu():
     h = ctypes.c_void_p()
     rc = L.lbm_context_create(10 ** 6, ctypes.byref(h))
     assert rc != 0 and L.lbm_last_error()
+
+
+def declared_params():
+    """name -> list of parameter declarations, for every function the header declares."""
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    out = {}
+    for m in re.finditer(r"\b(lbm_[a-z0-9_]+)\s*\(([^;{]*?)\)\s*;", text):
+        params = [p.strip() for p in m.group(2).split(",") if p.strip() and p.strip() != "void"]
+        out[m.group(1)] = params
+    return out
+
+
+def _ctype_of(param):
+    import panslbm2_amd as pa
+
+    if "*" in param:
+        return pa.VP
+    base = param.rsplit(None, 1)[0].replace("const ", "").strip()
+    return {"double": pa.D, "int": pa.I, "size_t": pa.SZ, "float": ctypes.c_float}.get(base, pa.VP)
+
+
+def _kind(t):
+    if t in (ctypes.c_void_p, ctypes.c_char_p) or issubclass(t, ctypes._Pointer):
+        return "pointer"
+    return t.__name__
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
+def test_python_bindings_match_header():
+    """Every ctypes signature the Python package sets has the header's parameter count and kinds
+    (pointer / double / int / size_t), so a call through it cannot shift its arguments."""
+    import panslbm2_amd as pa
+
+    L = pa.load()
+    decl = declared_params()
+    checked = 0
+    for name, params in decl.items():
+        fn = getattr(L, name)
+        if fn.argtypes is None:
+            continue
+        want = [_kind(_ctype_of(p)) for p in params]
+        got = [_kind(t) for t in fn.argtypes]
+        assert got == want, "%s: bound %s, header %s" % (name, fn.argtypes, params)
+        checked += 1
+    assert checked > 50

@@ -61,6 +61,40 @@ def test_header_compiles_standalone(tmp_path):
     assert r.returncode == 0, r.stdout
 
 
+THERMAL_USE = r"""
+#include "panslbm2_amd.hpp"
+using namespace PANSLBM2;
+void use2(D2Q9<double> &p, D2Q5<double> &q, double *a) {
+    AD::MacroBrinkmanCollideNaturalConvection(p, a, a, a, a, 0.1, q, a, a, a, a, 0.0, -1e-4, 0.5, true, a);
+    AD::MacroBrinkmanCollideHeatExchange(p, a, a, a, a, 0.1, q, a, a, a, a, 0.05, true);
+    AAD::MacroBrinkmanCollideNaturalConvection(p, a, a, a, a, a, a, a, a, a, 0.1, q, a, a, a, a, a, 0.0, -1e-4, true, a);
+    AAD::MacroBrinkmanCollideHeatExchange(p, a, a, a, a, a, a, a, a, a, 0.1, q, a, a, a, a, a, 0.05, true);
+    AAD::SensitivityHeatExchange(q, a, a, a, a, a, a, a, a, a);
+}
+void use3(D3Q15<double> &p, D3Q7<double> &q, double *a) {
+    AD::MacroBrinkmanCollideNaturalConvection(p, a, a, a, a, a, 0.1, q, a, a, a, a, a, 0.0, 0.0, -1e-4, 0.5, true, a);
+    AD::MacroBrinkmanCollideHeatExchange(p, a, a, a, a, a, 0.1, q, a, a, a, a, a, 0.05, true);
+    AAD::MacroBrinkmanCollideNaturalConvection(p, a, a, a, a, a, a, a, a, a, a, a, a, 0.1, q, a, a, a, a, a, a, 0.0,
+                                               0.0, -1e-4, true, a);
+    AAD::MacroBrinkmanCollideHeatExchange(p, a, a, a, a, a, a, a, a, a, a, a, a, 0.1, q, a, a, a, a, a, a, 0.05, true);
+    AAD::SensitivityHeatExchange(q, a, a, a, a, a, a, a, a, a, a, a);
+}
+int main() { return 0; }
+"""
+
+
+def test_thermal_variants_instantiate(tmp_path):
+    """The natural-convection and heat-exchange overloads take the reference's argument lists
+    (advection.h:716-810 and :930-1044, adjointadvection.h:894-993, :1113-1230 and :1445-1457):
+    calls written in those orders compile against the header."""
+    src = tmp_path / "thermal.cpp"
+    src.write_text(THERMAL_USE)
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Wextra", "-Werror", "-Wno-unused-parameter", "-fsyntax-only",
+                        "-I", os.path.join(ROOT, "include"), str(src)],
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    assert r.returncode == 0, r.stdout
+
+
 def test_driver_built():
     assert os.path.exists(DRIVER), "build the C++ drivers first (python -c 'import __graft_entry__ as g; g.build()')"
 
