@@ -330,6 +330,43 @@ int lbm_graph_end(lbm_context *ctx, lbm_graph **out);
 int lbm_graph_launch(lbm_graph *graph);
 int lbm_graph_destroy(lbm_graph *graph);
 
+/* ---- Learned warm start (no reference counterpart: the reference starts every steady solve
+ * from the InitialCondition of uniform fields, e.g. production/heatsink3D.cpp:97-300).  A
+ * 3-level 3D U-Net (channels 16/32/64, 3x3x3 convolutions, bfloat16 weights and activations,
+ * FP32 accumulation on the matrix cores) maps input fields (the design and masks) to the fields
+ * the InitialCondition calls take; it is trained online with Adam on converged fields.  Fields
+ * are device FP64 arrays of nx*ny*nz values (x fastest), normalised as (f - mean) / scale on the
+ * way in and f = mean + scale * y on the way out (null mean/scale: 0 and 1).  Each side of the
+ * grid must be a multiple of 4.  The initial weights depend only on the seed, and gradients are
+ * reduced in a fixed order, so training runs repeat bit for bit. ---- */
+typedef struct lbm_unet lbm_unet;
+/* cin 1..8 input fields, cout 1..16 output fields, base must be 16 */
+int lbm_unet_create(lbm_context *ctx, int nx, int ny, int nz, int cin, int cout, int base, unsigned long long seed,
+                    lbm_unet **out);
+int lbm_unet_destroy(lbm_unet *net);
+int lbm_unet_num_params(lbm_unet *net, size_t *n);
+/* parameter vector: per convolution (e1a e1b e2a e2b e3a e3b d2a d2b d1a d1b head) W[cout][taps][cin]
+ * then b[cout]; host FP64 copies of the FP32 master values */
+int lbm_unet_get_params(lbm_unet *net, double *host, size_t n);
+int lbm_unet_set_params(lbm_unet *net, const double *host, size_t n);
+int lbm_unet_get_grads(lbm_unet *net, double *host, size_t n);
+int lbm_unet_reset_optimizer(lbm_unet *net);
+/* fields: host array of nfields device pointers */
+int lbm_unet_set_input(lbm_unet *net, const double *const *fields, const double *mean, const double *scale,
+                       int nfields);
+int lbm_unet_set_target(lbm_unet *net, const double *const *fields, const double *mean, const double *scale,
+                        int nfields);
+int lbm_unet_forward(lbm_unet *net);
+int lbm_unet_output(lbm_unet *net, double *const *fields, const double *mean, const double *scale, int nfields);
+/* loss = mean over voxels and output fields of (y - target)^2 for the last forward pass; fills the
+ * gradient vector.  loss may be null (then nothing waits for the device). */
+int lbm_unet_backward(lbm_unet *net, double *loss);
+int lbm_unet_adam(lbm_unet *net, double lr, double beta1, double beta2, double eps);
+/* steps x (forward, backward, Adam with beta 0.9/0.999, eps 1e-8); loss of the last step's forward pass */
+int lbm_unet_train(lbm_unet *net, int steps, double lr, double *loss);
+/* host copy of a named tensor ("x0", "e1a", ..., "y", "target"), [voxel][channel] real channels */
+int lbm_unet_read(lbm_unet *net, const char *name, double *host, size_t n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -41,6 +41,21 @@ def load():
     L = ctypes.CDLL(LIB_PATH)
     L.lbm_last_error.restype = ctypes.c_char_p
     sig = {
+        "lbm_unet_create": [VP, I, I, I, I, I, I, ctypes.c_ulonglong, ctypes.POINTER(VP)],
+        "lbm_unet_destroy": [VP],
+        "lbm_unet_num_params": [VP, ctypes.POINTER(SZ)],
+        "lbm_unet_get_params": [VP, VP, SZ],
+        "lbm_unet_set_params": [VP, VP, SZ],
+        "lbm_unet_get_grads": [VP, VP, SZ],
+        "lbm_unet_reset_optimizer": [VP],
+        "lbm_unet_set_input": [VP, VP, VP, VP, I],
+        "lbm_unet_set_target": [VP, VP, VP, VP, I],
+        "lbm_unet_forward": [VP],
+        "lbm_unet_output": [VP, VP, VP, VP, I],
+        "lbm_unet_backward": [VP, VP],
+        "lbm_unet_adam": [VP, D, D, D, D],
+        "lbm_unet_train": [VP, I, D, VP],
+        "lbm_unet_read": [VP, ctypes.c_char_p, VP, SZ],
         "lbm_context_create": [I, ctypes.POINTER(VP)],
         "lbm_context_destroy": [VP],
         "lbm_context_synchronize": [VP],

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/panslbm2_amd.h"
+#include "context.h"
 #include "device.h"
 
 using lbm::FieldSet;
@@ -595,9 +596,24 @@ int hip_result(hipError_t e, const char *what) {
 }  // namespace
 
 // =============================================================================================
+namespace lbm {
+int context_device(const lbm_context *C) { return C->device; }
+hipStream_t context_stream(const lbm_context *C) { return C->stream; }
+int set_error(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+}  // namespace lbm
+
 extern "C" {
 
 const char *lbm_last_error(void) { return g_err.c_str(); }
+
 int lbm_version(void) { return 1; }
 
 int lbm_context_create(int device, lbm_context **out) {

@@ -59,7 +59,8 @@ def _ctype_of(param):
     if "*" in param:
         return pa.VP
     base = param.rsplit(None, 1)[0].replace("const ", "").strip()
-    return {"double": pa.D, "int": pa.I, "size_t": pa.SZ, "float": ctypes.c_float}.get(base, pa.VP)
+    return {"double": pa.D, "int": pa.I, "size_t": pa.SZ, "float": ctypes.c_float,
+            "unsigned long long": ctypes.c_ulonglong}.get(base, pa.VP)
 
 
 def _kind(t):
