@@ -337,8 +337,10 @@ int lbm_graph_destroy(lbm_graph *graph);
  * the InitialCondition calls take; it is trained online with Adam on converged fields.  Fields
  * are device FP64 arrays of nx*ny*nz values (x fastest), normalised as (f - mean) / scale on the
  * way in and f = mean + scale * y on the way out (null mean/scale: 0 and 1).  Each side of the
- * grid must be a multiple of 4.  The initial weights depend only on the seed, and gradients are
- * reduced in a fixed order, so training runs repeat bit for bit. ---- */
+ * grid must be a multiple of 4.  The initial weights depend only on the seed (the linear head
+ * starts at zero), and gradients are reduced in a fixed order, so training runs repeat bit for
+ * bit.  The network runs on its own stream: set_input, set_target and output wait for the
+ * context stream's queued work, and the context stream's later work waits for output. ---- */
 typedef struct lbm_unet lbm_unet;
 /* cin 1..8 input fields, cout 1..16 output fields, base must be 16 */
 int lbm_unet_create(lbm_context *ctx, int nx, int ny, int nz, int cin, int cout, int base, unsigned long long seed,
@@ -354,16 +356,21 @@ int lbm_unet_reset_optimizer(lbm_unet *net);
 /* fields: host array of nfields device pointers */
 int lbm_unet_set_input(lbm_unet *net, const double *const *fields, const double *mean, const double *scale,
                        int nfields);
-int lbm_unet_set_target(lbm_unet *net, const double *const *fields, const double *mean, const double *scale,
-                        int nfields);
+/* target = (fields - base) normalised (base: host array of device pointers, or null) */
+int lbm_unet_set_target(lbm_unet *net, const double *const *fields, const double *const *base, const double *mean,
+                        const double *scale, int nfields);
 int lbm_unet_forward(lbm_unet *net);
-int lbm_unet_output(lbm_unet *net, double *const *fields, const double *mean, const double *scale, int nfields);
+/* f = base + mean + scale * y, clamped to [lo, hi] per field (null base / lo / hi: none) */
+int lbm_unet_output(lbm_unet *net, double *const *fields, const double *const *base, const double *mean,
+                    const double *scale, const double *lo, const double *hi, int nfields);
 /* loss = mean over voxels and output fields of (y - target)^2 for the last forward pass; fills the
  * gradient vector.  loss may be null (then nothing waits for the device). */
 int lbm_unet_backward(lbm_unet *net, double *loss);
 int lbm_unet_adam(lbm_unet *net, double lr, double beta1, double beta2, double eps);
 /* steps x (forward, backward, Adam with beta 0.9/0.999, eps 1e-8); loss of the last step's forward pass */
 int lbm_unet_train(lbm_unet *net, int steps, double lr, double *loss);
+/* waits for the network's stream (training is queued there and runs beside the solver) */
+int lbm_unet_synchronize(lbm_unet *net);
 /* host copy of a named tensor ("x0", "e1a", ..., "y", "target"), [voxel][channel] real channels */
 int lbm_unet_read(lbm_unet *net, const char *name, double *host, size_t n);
 

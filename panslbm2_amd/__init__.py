@@ -49,9 +49,10 @@ def load():
         "lbm_unet_get_grads": [VP, VP, SZ],
         "lbm_unet_reset_optimizer": [VP],
         "lbm_unet_set_input": [VP, VP, VP, VP, I],
-        "lbm_unet_set_target": [VP, VP, VP, VP, I],
+        "lbm_unet_set_target": [VP, VP, VP, VP, VP, I],
+        "lbm_unet_synchronize": [VP],
         "lbm_unet_forward": [VP],
-        "lbm_unet_output": [VP, VP, VP, VP, I],
+        "lbm_unet_output": [VP, VP, VP, VP, VP, VP, VP, I],
         "lbm_unet_backward": [VP, VP],
         "lbm_unet_adam": [VP, D, D, D, D],
         "lbm_unet_train": [VP, I, D, VP],

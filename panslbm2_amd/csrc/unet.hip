@@ -18,8 +18,12 @@
 //   * Adam keeps FP32 master weights and repacks the bfloat16 copies after each update.
 //
 // The fields the solver uses are FP64 arrays; the network reads and writes them directly
-// (normalised by a per-channel mean and scale).  The CPU restatement of the same network is
-// oracle/unet_oracle.py.  No reference counterpart: the reference starts every solve from rest.
+// (normalised by a per-channel mean and scale, optionally relative to base fields, so that it can
+// learn the change from a previous solution).  The network has its own HIP stream: its training
+// kernels (matrix-core bound) overlap the solver's memory-bound steps on the context stream; the
+// calls that read or write solver fields are ordered against the context stream with events.
+// The CPU restatement of the same network is oracle/unet_oracle.py.  No reference counterpart:
+// the reference starts every solve from rest.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -264,29 +268,36 @@ __global__ void k_pack_bias(const float *__restrict__ b, int co, int rows, float
 // ---- elementwise pieces ---------------------------------------------------------------------------
 struct FieldPtrs {
     const double *f[kMaxFields];
+    const double *base[kMaxFields];  // null: no base
     double mean[kMaxFields], scale[kMaxFields];
 };
 struct FieldOut {
     double *f[kMaxFields];
-    double mean[kMaxFields], scale[kMaxFields];
+    const double *base[kMaxFields];
+    double mean[kMaxFields], scale[kMaxFields], lo[kMaxFields], hi[kMaxFields];
 };
 
-// xs[v][c] = (f_c[v] - mean_c) / scale_c for c < nf, zero up to cs; bf16 or FP32
+// xs[v][c] = (f_c[v] - base_c[v] - mean_c) / scale_c for c < nf, zero up to cs; bf16 or FP32
 __global__ void k_gather(FieldPtrs F, int nf, int cs, int V, bf16_t *__restrict__ xb, float *__restrict__ xf) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= V * cs) return;
     const int v = n / cs, c = n - v * cs;
     float val = 0.f;
-    if (c < nf) val = (float)((F.f[c][v] - F.mean[c]) / F.scale[c]);
+    if (c < nf) {
+        const double b = F.base[c] ? F.base[c][v] : 0.0;
+        val = (float)((F.f[c][v] - b - F.mean[c]) / F.scale[c]);
+    }
     if (xb) xb[n] = f2bf(val);
     if (xf) xf[n] = val;
 }
-// f_c[v] = mean_c + scale_c y[v][c]
+// f_c[v] = base_c[v] + mean_c + scale_c y[v][c], clamped to [lo_c, hi_c]
 __global__ void k_scatter(const float *__restrict__ y, int cs, int nf, int V, FieldOut F) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= V * nf) return;
     const int c = n / V, v = n - c * V;
-    F.f[c][v] = F.mean[c] + F.scale[c] * (double)y[(size_t)v * cs + c];
+    const double b = F.base[c] ? F.base[c][v] : 0.0;
+    const double f = b + F.mean[c] + F.scale[c] * (double)y[(size_t)v * cs + c];
+    F.f[c][v] = fmin(fmax(f, F.lo[c]), F.hi[c]);
 }
 
 // 2x2x2 average pooling, fine grid (nx, ny, nz)
@@ -418,7 +429,8 @@ inline int r32(int c) { return (c + 31) & ~31; }
 // ==================================================================================================
 struct lbm_unet {
     lbm_context *ctx = nullptr;
-    hipStream_t s = nullptr;
+    hipStream_t s = nullptr;                          // the network's own stream
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;     // ordering against the context stream
     int dims[3][3] = {};  // per level: nx, ny, nz
     int V[3] = {};
     int cin = 0, cout = 0, base = 16;
@@ -625,11 +637,13 @@ int adam(lbm_unet *N, double lr, double b1, double b2, double eps) {
     return repack(N);
 }
 
-int fields_in(const double *const *f, const double *mean, const double *scale, int nf, FieldPtrs *F) {
+int fields_in(const double *const *f, const double *const *base, const double *mean, const double *scale, int nf,
+              FieldPtrs *F) {
     if (nf < 1 || nf > kMaxFields || !f) return lbm::set_error(LBM_ERR_ARG, "1..%d fields expected", kMaxFields);
     for (int c = 0; c < nf; ++c) {
         if (!f[c]) return lbm::set_error(LBM_ERR_ARG, "field %d is null", c);
         F->f[c] = f[c];
+        F->base[c] = base ? base[c] : nullptr;
         F->mean[c] = mean ? mean[c] : 0.0;
         F->scale[c] = scale ? scale[c] : 1.0;
         if (F->scale[c] == 0.0) return lbm::set_error(LBM_ERR_ARG, "field %d has scale 0", c);
@@ -640,10 +654,26 @@ int fields_in(const double *const *f, const double *mean, const double *scale, i
 void destroy(lbm_unet *N) {
     if (!N) return;
     (void)hipSetDevice(lbm::context_device(N->ctx));
-    (void)hipStreamSynchronize(N->s);
+    if (N->s) (void)hipStreamSynchronize(N->s);
     for (void *q : N->allocs) (void)hipFree(q);
     if (N->hloss) (void)hipHostFree(N->hloss);
+    if (N->ev_in) (void)hipEventDestroy(N->ev_in);
+    if (N->ev_out) (void)hipEventDestroy(N->ev_out);
+    if (N->s) (void)hipStreamDestroy(N->s);
     delete N;
+}
+
+// the network stream waits for what the context stream has queued so far
+int after_ctx(lbm_unet *N) {
+    UCHK(hipEventRecord(N->ev_in, lbm::context_stream(N->ctx)));
+    UCHK(hipStreamWaitEvent(N->s, N->ev_in, 0));
+    return LBM_OK;
+}
+// the context stream waits for what the network stream has queued so far
+int before_ctx(lbm_unet *N) {
+    UCHK(hipEventRecord(N->ev_out, N->s));
+    UCHK(hipStreamWaitEvent(lbm::context_stream(N->ctx), N->ev_out, 0));
+    return LBM_OK;
 }
 
 // xorshift64* stream for the initial weights
@@ -674,7 +704,12 @@ int lbm_unet_create(lbm_context *ctx, int nx, int ny, int nz, int cin, int cout,
     UCHK(hipSetDevice(lbm::context_device(ctx)));
     lbm_unet *N = new lbm_unet;
     N->ctx = ctx;
-    N->s = lbm::context_stream(ctx);
+    if (hipStreamCreateWithFlags(&N->s, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&N->ev_in, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&N->ev_out, hipEventDisableTiming) != hipSuccess) {
+        destroy(N);
+        return lbm::set_error(LBM_ERR_HIP, "stream or event creation failed");
+    }
     N->cin = cin;
     N->cout = cout;
     N->base = base;
@@ -768,12 +803,13 @@ int lbm_unet_create(lbm_context *ctx, int nx, int ny, int nz, int cin, int cout,
         destroy(N);
         return rc;
     }
-    // He-uniform initial weights (bound sqrt(6 / fan_in)), linear head bound sqrt(3 / fan_in), zero biases
+    // He-uniform initial weights (bound sqrt(6 / fan_in)), zero biases, and a zero linear head:
+    // the untrained network outputs zero (the base fields, when it predicts a change)
     std::vector<float> h(N->np, 0.f);
     Rng rng{seed * 0x9E3779B97F4A7C15ull + 0x2545F4914F6CDD1Dull};
     for (auto &c : N->conv) {
-        const double fan = (double)c.taps * c.ci;
-        const double b = c.name == "head" ? std::sqrt(3.0 / fan) : std::sqrt(6.0 / fan);
+        if (c.name == "head") continue;
+        const double b = std::sqrt(6.0 / ((double)c.taps * c.ci));
         for (size_t i = c.wo; i < c.bo; ++i) h[i] = (float)((2.0 * rng.uniform() - 1.0) * b);
     }
     hipError_t e = hipMemcpyAsync(N->p, h.data(), N->np * sizeof(float), hipMemcpyHostToDevice, N->s);
@@ -841,22 +877,24 @@ int lbm_unet_set_input(lbm_unet *N, const double *const *fields, const double *m
     if (!N) return lbm::set_error(LBM_ERR_ARG, "null argument");
     if (nfields != N->cin) return lbm::set_error(LBM_ERR_ARG, "%d input fields, network takes %d", nfields, N->cin);
     FieldPtrs F;
-    int rc = fields_in(fields, mean, scale, nfields, &F);
+    int rc = fields_in(fields, nullptr, mean, scale, nfields, &F);
     if (rc != LBM_OK) return rc;
     UCHK(hipSetDevice(lbm::context_device(N->ctx)));
+    if ((rc = after_ctx(N)) != LBM_OK) return rc;
     k_gather<<<blocks((size_t)N->V[0] * 8), 256, 0, N->s>>>(F, nfields, 8, N->V[0], N->x0, nullptr);
     UCHK(hipGetLastError());
     return LBM_OK;
 }
 
-int lbm_unet_set_target(lbm_unet *N, const double *const *fields, const double *mean, const double *scale,
-                        int nfields) {
+int lbm_unet_set_target(lbm_unet *N, const double *const *fields, const double *const *base, const double *mean,
+                        const double *scale, int nfields) {
     if (!N) return lbm::set_error(LBM_ERR_ARG, "null argument");
     if (nfields != N->cout) return lbm::set_error(LBM_ERR_ARG, "%d target fields, network gives %d", nfields, N->cout);
     FieldPtrs F;
-    int rc = fields_in(fields, mean, scale, nfields, &F);
+    int rc = fields_in(fields, base, mean, scale, nfields, &F);
     if (rc != LBM_OK) return rc;
     UCHK(hipSetDevice(lbm::context_device(N->ctx)));
+    if ((rc = after_ctx(N)) != LBM_OK) return rc;
     const int cs = r16(N->cout);
     k_gather<<<blocks((size_t)N->V[0] * cs), 256, 0, N->s>>>(F, nfields, cs, N->V[0], nullptr, N->tgt);
     UCHK(hipGetLastError());
@@ -870,20 +908,27 @@ int lbm_unet_forward(lbm_unet *N) {
     return forward(N);
 }
 
-int lbm_unet_output(lbm_unet *N, double *const *fields, const double *mean, const double *scale, int nfields) {
+int lbm_unet_output(lbm_unet *N, double *const *fields, const double *const *base, const double *mean,
+                    const double *scale, const double *lo, const double *hi, int nfields) {
     if (!N || !fields) return lbm::set_error(LBM_ERR_ARG, "null argument");
     if (nfields != N->cout) return lbm::set_error(LBM_ERR_ARG, "%d output fields, network gives %d", nfields, N->cout);
     FieldOut F;
     for (int c = 0; c < nfields; ++c) {
         if (!fields[c]) return lbm::set_error(LBM_ERR_ARG, "field %d is null", c);
         F.f[c] = fields[c];
+        F.base[c] = base ? base[c] : nullptr;
         F.mean[c] = mean ? mean[c] : 0.0;
         F.scale[c] = scale ? scale[c] : 1.0;
+        F.lo[c] = lo ? lo[c] : -HUGE_VAL;
+        F.hi[c] = hi ? hi[c] : HUGE_VAL;
+        if (!(F.lo[c] <= F.hi[c])) return lbm::set_error(LBM_ERR_ARG, "field %d: empty clamp range", c);
     }
     UCHK(hipSetDevice(lbm::context_device(N->ctx)));
+    int rc = after_ctx(N);
+    if (rc != LBM_OK) return rc;
     k_scatter<<<blocks((size_t)N->V[0] * nfields), 256, 0, N->s>>>(N->y32, r16(N->cout), nfields, N->V[0], F);
     UCHK(hipGetLastError());
-    return LBM_OK;
+    return before_ctx(N);
 }
 
 int lbm_unet_backward(lbm_unet *N, double *loss) {
@@ -925,6 +970,13 @@ int lbm_unet_train(lbm_unet *N, int steps, double lr, double *loss) {
         UCHK(hipStreamSynchronize(N->s));
         *loss = steps ? *N->hloss : NAN;
     }
+    return LBM_OK;
+}
+
+int lbm_unet_synchronize(lbm_unet *N) {
+    if (!N) return lbm::set_error(LBM_ERR_ARG, "null argument");
+    UCHK(hipSetDevice(lbm::context_device(N->ctx)));
+    UCHK(hipStreamSynchronize(N->s));
     return LBM_OK;
 }
 

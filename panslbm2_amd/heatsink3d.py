@@ -16,6 +16,7 @@ No SmoothCorner: the corner and edge nodes lie on bounce-back walls, and the fus
 every operator but SmoothCorner, so each time step is one kernel pass.
 """
 import ctypes
+import time
 
 import numpy as np
 
@@ -44,23 +45,15 @@ class Heatsink3D(Heatsink2D):
         i, j, k = n % lx, (n // lx) % ly, n // (lx * ly)
         self.i, self.j, self.k = i, j, k
 
+        self.plate, self.filter_radius = plate, filter_radius
+        self.alpha_max, self.qf, self.qg = alpha_max, qf, qg
+        self.dfl = nu / pr
+        self.dso = 10.0 * self.dfl
         rng = np.random.default_rng(seed)
-        raw = c.array((N,), data=rng.uniform(0.0, 1.0, N))
-        gam = c.array((N,), fill=0.0)
-        check(self.L.lbm_density_filter(c.h, lx, ly, lz, D(filter_radius), raw.ptr, gam.ptr))
-        s = gam.download()
-        s[k < plate] = 0.0  # the base plate is solid
-        self.gamma = s
-        dfl = nu / pr
-        dso = 10.0 * dfl
-        coef = {
-            "alpha": alpha_max / (ly - 1.0) * qf * (1.0 - s) / (s + qf),
-            "dads": -alpha_max / (ly - 1.0) * qf * (1.0 + qf) / (s + qf) ** 2,
-            "diff": dso + (dfl - dso) * s * (1.0 + qg) / (s + qg),
-            "dkds": (dfl - dso) * qg * (1.0 + qg) / (s + qg) ** 2,
-            "heat": np.where(k < plate, heat, 0.0),
-        }
-        self.d = {name: c.array((N,), data=v) for name, v in coef.items()}
+        self.d = {name: c.array((N,), fill=0.0)
+                  for name in ("alpha", "dads", "diff", "dkds", "raw", "design", "dgamma")}
+        self.d["heat"] = c.array((N,), data=np.where(k < plate, heat, 0.0))
+        self.set_design(rng.uniform(0.0, 1.0, N))
         for name in ("rho", "tem", "ip", "imx", "imy", "imz", "item", "dfds"):
             self.d[name] = c.array((N,), fill=0.0)
         for name in ("ux", "uy", "uz", "qx", "qy", "qz", "iux", "iuy", "iuz", "iqx", "iqy", "iqz"):
@@ -91,6 +84,50 @@ class Heatsink3D(Heatsink2D):
         self._graphs = {}
         self.adj_prog = None
         self.fwd_prog = self._record_forward() if fused else None
+
+    # ---- design -----------------------------------------------------------------------------------
+    def set_design(self, x):
+        """Design variables x (fluid fraction before filtering, per node) -> gamma = density-filtered
+        x with the base plate solid -> Brinkman and diffusivity coefficients (heatsink.cpp:104-117
+        interpolation; 3D duct geometry)."""
+        c, lx, ly, lz, d = self.ctx, self.lx, self.ly, self.lz, self.d
+        self.x = np.clip(np.asarray(x, dtype=np.float64), 0.0, 1.0).copy()
+        d["raw"].upload(self.x)
+        check(self.L.lbm_density_filter(c.h, lx, ly, lz, D(self.filter_radius), d["raw"].ptr, d["design"].ptr))
+        s = d["design"].download()
+        s[self.k < self.plate] = 0.0  # the base plate is solid
+        d["design"].upload(s)
+        prev = getattr(self, "gamma", None)
+        d["dgamma"].upload(s - prev if prev is not None else np.zeros_like(s))
+        self.gamma = s
+        am, qf, qg, dfl, dso = self.alpha_max, self.qf, self.qg, self.dfl, self.dso
+        d["alpha"].upload(am / (ly - 1.0) * qf * (1.0 - s) / (s + qf))
+        d["dads"].upload(-am / (ly - 1.0) * qf * (1.0 + qf) / (s + qf) ** 2)
+        d["diff"].upload(dso + (dfl - dso) * s * (1.0 + qg) / (s + qg))
+        d["dkds"].upload((dfl - dso) * qg * (1.0 + qg) / (s + qg) ** 2)
+
+    def design_update(self, vf=0.4, move=0.05):
+        """One projected-gradient design step (a stand-in for the reference's MMA update, mma.h:196,
+        which is not built): x <- clip(x - move dF/ds / max|dF/ds| + lam, x - move, x + move, 0, 1)
+        over the nodes above the plate, with lam found by bisection so that the solid fraction
+        mean(1 - x) there equals vf."""
+        dfds = self.d["dfds"].download()
+        region = self.k >= self.plate
+        g = dfds[region]
+        g = g / max(np.abs(g).max(), 1e-300)
+        x = self.x[region]
+        lo_b, hi_b = np.maximum(0.0, x - move), np.minimum(1.0, x + move)
+        trial = lambda lam: np.clip(x - move * g + lam, lo_b, hi_b)
+        lo, hi = -1.0, 1.0
+        for _ in range(60):
+            mid = 0.5 * (lo + hi)
+            if trial(mid).mean() > 1.0 - vf:
+                hi = mid
+            else:
+                lo = mid
+        xn = self.x.copy()
+        xn[region] = trial(0.5 * (lo + hi))
+        self.set_design(xn)
 
     # ---- boundary calls of one step (recorded into a program, or run eagerly) ----------------------
     def _fwd_boundaries(self, ux, uy, uz):
@@ -179,24 +216,30 @@ class Heatsink3D(Heatsink2D):
         u, q = (("iux", "iuy", "iuz"), ("iqx", "iqy", "iqz")) if adjoint else (("ux", "uy", "uz"), ("qx", "qy", "qz"))
         return tuple(([d["%s%d" % (v, b)] for v in f], [d["%s%d" % (v, pb)] for v in f]) for f in (u, q))
 
-    def init_forward(self):
+    def init_forward(self, warm=False):
+        """Cold start (rest, T = 0), or with warm=True the equilibrium of the predicted fields."""
         L, d = self.L, self.d
         self._set_prec(self.prec)
+        if warm:
+            w = self.w
+            check(L.lbm_ns_initial_condition(self.pf.h, w["rho"].ptr, w["ux"].ptr, w["uy"].ptr, w["uz"].ptr))
+            check(L.lbm_ad_initial_condition(self.pg.h, w["tem"].ptr, w["ux"].ptr, w["uy"].ptr, w["uz"].ptr))
+            return
         for name in ("ux0", "uy0", "uz0"):
             d[name].fill(0.0)
         d["tem"].fill(0.0)
         check(L.lbm_ns_initial_condition(self.pf.h, d["one"].ptr, d["ux0"].ptr, d["uy0"].ptr, d["uz0"].ptr))
         check(L.lbm_ad_initial_condition(self.pg.h, d["tem"].ptr, d["ux0"].ptr, d["uy0"].ptr, d["uz0"].ptr))
 
-    def forward(self):
-        self.init_forward()
+    def forward(self, warm=False):
+        self.init_forward(warm)
         res = lambda t: tuple(self._residual3(*f) for f in self.residual_fields(False, t))
         t, self.fwd_steps32, self.fwd_res = self._solve("fwd", self._fwd_step, res)
         self.fwd_steps = t
         self.fwd_buf = t % 2
         return t
 
-    def init_adjoint(self):
+    def init_adjoint(self, warm=False):
         L, d = self.L, self.d
         self._set_prec(self.prec)
         fb = self.fwd_buf
@@ -208,17 +251,23 @@ class Heatsink3D(Heatsink2D):
             g = self._graphs.pop("adj", None)
             if g is not None:
                 L.lbm_graph_destroy(g)
+        uf = (d["ux%d" % fb].ptr, d["uy%d" % fb].ptr, d["uz%d" % fb].ptr)
+        if warm:
+            w = self.w
+            check(L.lbm_ans_initial_condition(self.pf.h, *uf, w["ip"].ptr, w["iux"].ptr, w["iuy"].ptr, w["iuz"].ptr))
+            check(L.lbm_aad_initial_condition(self.pg.h, *uf, w["item"].ptr, w["iqx"].ptr, w["iqy"].ptr,
+                                              w["iqz"].ptr))
+            return
         for name in ("ip", "item"):
             d[name].fill(0.0)
         for name in ("iux0", "iuy0", "iuz0", "iqx0", "iqy0", "iqz0"):
             d[name].fill(0.0)
-        check(L.lbm_ans_initial_condition(self.pf.h, d["ux%d" % fb].ptr, d["uy%d" % fb].ptr, d["uz%d" % fb].ptr,
-                                          d["ip"].ptr, d["iux0"].ptr, d["iuy0"].ptr, d["iuz0"].ptr))
-        check(L.lbm_aad_initial_condition(self.pg.h, d["ux%d" % fb].ptr, d["uy%d" % fb].ptr, d["uz%d" % fb].ptr,
-                                          d["item"].ptr, d["iqx0"].ptr, d["iqy0"].ptr, d["iqz0"].ptr))
+        check(L.lbm_ans_initial_condition(self.pf.h, *uf, d["ip"].ptr, d["iux0"].ptr, d["iuy0"].ptr, d["iuz0"].ptr))
+        check(L.lbm_aad_initial_condition(self.pg.h, *uf, d["item"].ptr, d["iqx0"].ptr, d["iqy0"].ptr,
+                                          d["iqz0"].ptr))
 
-    def adjoint(self):
-        self.init_adjoint()
+    def adjoint(self, warm=False):
+        self.init_adjoint(warm)
         fb = self.fwd_buf
         res = lambda t: tuple(self._residual3(*f) for f in self.residual_fields(True, t))
         t, self.adj_steps32, self.adj_res = self._solve("adj", lambda s: self._adj_step(s, fb), res)
@@ -239,3 +288,96 @@ class Heatsink3D(Heatsink2D):
         check(L.lbm_gather_sum(self.ctx.h, d["tem"].ptr, hptr(self.obj_nodes), len(self.obj_nodes), ctypes.byref(out)))
         self.objective = out.value / len(self.obj_nodes)
         return self.objective
+
+    # ---- learned warm start -----------------------------------------------------------------------
+    WARM_OUT = ("rho", "ux", "uy", "uz", "tem", "ip", "iux", "iuy", "iuz", "item", "iqx", "iqy", "iqz")
+
+    def attach_warm_start(self, seed=0, lr=1e-3, steps=40, replay=4):
+        """A WarmStart network for this grid.  Inputs: gamma, its change since the previous design,
+        x/(lx-1), z/(lz-1) (the coordinates tell inlet from outlet and plate from lid); outputs:
+        the change of the fields of WARM_OUT, which the forward (NS, AD) and adjoint (ANS, AAD)
+        InitialCondition calls take, since the previous converged design."""
+        from .warmstart import WarmStart
+
+        c = self.ctx
+        self.d["xpos"] = c.array((self.N,), data=self.i / (self.lx - 1.0))
+        self.d["zpos"] = c.array((self.N,), data=self.k / (self.lz - 1.0))
+        self.w = {n: c.array((self.N,), fill=0.0) for n in self.WARM_OUT}
+        self.warm = WarmStart(c, (self.lx, self.ly, self.lz), 4, len(self.WARM_OUT), seed=seed, lr=lr, steps=steps,
+                              replay=replay)
+        return self.warm
+
+    def _warm_inputs(self):
+        return [self.d["design"], self.d["dgamma"], self.d["xpos"], self.d["zpos"]]
+
+    def _warm_targets(self):
+        d, fb, ab = self.d, self.fwd_buf, self.adj_buf
+        return [d["rho"], d["ux%d" % fb], d["uy%d" % fb], d["uz%d" % fb], d["tem"], d["ip"], d["iux%d" % ab],
+                d["iuy%d" % ab], d["iuz%d" % ab], d["item"], d["iqx%d" % ab], d["iqy%d" % ab], d["iqz%d" % ab]]
+
+    def evaluate(self, warm=None, train=True):
+        """One design evaluation: forward solve, adjoint solve, sensitivity.
+
+        warm=None starts both solves from the attached warm start's prediction once it has seen a
+        converged solution (warm=False forces the cold start); a warm-started solve whose residual
+        turns non-finite is repeated from the cold start.  With train=True the warm start's
+        training on the stored solutions is queued right after its prediction and runs beside the
+        solves, and a converged evaluation is then stored as a new solution.  'seconds' is the
+        wall time of the solves (training beside them included), 'train_seconds' the time to
+        store the solution afterwards."""
+        ws = getattr(self, "warm", None)
+        use = ws is not None and ws.ready and warm is not False
+        t0 = time.perf_counter()
+        fallback = [False, False]
+        if use:
+            ws.predict(self._warm_inputs(), [self.w[n] for n in self.WARM_OUT])
+        if ws is not None and train:
+            ws.train()
+        self.forward(use)
+        if use and not np.all(np.isfinite(self.fwd_res)):
+            fallback[0] = True
+            self.forward(False)
+        self.adjoint(use)
+        if use and not np.all(np.isfinite(self.adj_res)):
+            fallback[1] = True
+            self.adjoint(False)
+        self.sensitivity()
+        self.ctx.sync()
+        t1 = time.perf_counter()
+        res = np.concatenate([np.ravel(self.fwd_res), np.ravel(self.adj_res)])
+        converged = bool(np.all(np.isfinite(res)) and res.max() < self.eps)
+        if ws is not None and train and converged:
+            ws.observe(self._warm_inputs(), self._warm_targets())
+            self.ctx.sync()
+        t2 = time.perf_counter()
+        return {"seconds": t1 - t0, "train_seconds": t2 - t1, "fwd_steps": self.fwd_steps,
+                "adj_steps": self.adj_steps, "fp32_steps": (self.fwd_steps32, self.adj_steps32),
+                "objective": self.objective, "fwd_residual": self.fwd_res, "adj_residual": self.adj_res,
+                "diverged": not np.all(np.isfinite(res)), "converged": converged, "warm": use,
+                "fallback": fallback}
+
+    def close(self):
+        super().close()
+        ws = getattr(self, "warm", None)
+        if ws is not None:
+            ws.close()
+            self.warm = None
+
+    def optimize(self, iterations, vf=0.4, move=0.05, warm=None, x0=None, callback=None):
+        """Design loop from a uniform solid fraction vf above the plate (or x0): evaluate, then
+        design_update.  Returns the per-iteration evaluate() records."""
+        x = np.where(self.k >= self.plate, 1.0 - vf, 0.0) if x0 is None else x0
+        self.set_design(x)
+        hist = []
+        for it in range(iterations):
+            r = self.evaluate(warm=warm)
+            r["iteration"] = it
+            hist.append(r)
+            if callback is not None:
+                callback(r)
+            if it + 1 < iterations:
+                self.design_update(vf, move)
+        ws = getattr(self, "warm", None)
+        if ws is not None:
+            ws.net.synchronize()
+        return hist

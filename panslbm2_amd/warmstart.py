@@ -2,11 +2,13 @@
 
 UNet wraps one network on the device: a 3-level 3D U-Net (16/32/64 channels, 3x3x3 convolutions,
 bfloat16 weights and activations, FP32 accumulation on the matrix cores) that maps input fields
-(the design and boundary masks) to output fields (the macroscopic fields the InitialCondition
-calls take).  WarmStart is the online predictor a design loop uses: it keeps the last few
-converged solutions, trains the network on them with Adam after every solve, and writes its
-prediction for the next design into the solver's arrays.  The solver's residual still decides
-convergence; a warm start that diverges is replaced by the cold start (Heatsink3D.evaluate).
+(the design, its change, and coordinates) to output fields (the macroscopic fields the
+InitialCondition calls take).  WarmStart is the online predictor a design loop uses: the network
+learns the change of the converged fields from one design to the next, trained with Adam on the
+solutions the loop produces; its prediction for a new design is the previous converged fields plus
+that learned change.  Training runs on the network's own stream, beside the solver.  The solver's
+residual still decides convergence; a warm start that diverges is replaced by the cold start
+(Heatsink3D.evaluate).
 
 No reference counterpart: the reference starts every solve from uniform fields
 (production/heatsink3D.cpp:97-300).
@@ -71,25 +73,39 @@ class UNet:
     def reset_optimizer(self):
         check(self.L.lbm_unet_reset_optimizer(self.h))
 
-    def _fields(self, fn, arrays, mean, scale):
-        n = len(arrays)
-        m, s = _dbl(mean, n), _dbl(scale, n)
-        check(fn(self.h, _ptrs(arrays), None if m is None else m.ctypes.data_as(VP),
-                 None if s is None else s.ctypes.data_as(VP), n))
+    def synchronize(self):
+        check(self.L.lbm_unet_synchronize(self.h))
+
+    @staticmethod
+    def _vec(v, n):
+        a = _dbl(v, n)
+        return None if a is None else a.ctypes.data_as(VP), a
 
     def set_input(self, arrays, mean=None, scale=None):
-        self._fields(self.L.lbm_unet_set_input, arrays, mean, scale)
+        n = len(arrays)
+        (m, _m), (s, _s) = self._vec(mean, n), self._vec(scale, n)
+        check(self.L.lbm_unet_set_input(self.h, _ptrs(arrays), m, s, n))
 
-    def set_target(self, arrays, mean=None, scale=None):
-        self._fields(self.L.lbm_unet_set_target, arrays, mean, scale)
+    def set_target(self, arrays, base=None, mean=None, scale=None):
+        n = len(arrays)
+        (m, _m), (s, _s) = self._vec(mean, n), self._vec(scale, n)
+        check(self.L.lbm_unet_set_target(self.h, _ptrs(arrays), None if base is None else _ptrs(base), m, s, n))
 
     def forward(self):
         check(self.L.lbm_unet_forward(self.h))
 
-    def output(self, arrays, mean=None, scale=None):
-        self._fields(self.L.lbm_unet_output, arrays, mean, scale)
+    def output(self, arrays, base=None, mean=None, scale=None, lo=None, hi=None):
+        n = len(arrays)
+        v = [self._vec(a, n) for a in (mean, scale, lo, hi)]
+        check(self.L.lbm_unet_output(self.h, _ptrs(arrays), None if base is None else _ptrs(base),
+                                     *[p for p, _ in v], n))
 
-    def backward(self):
+    def backward(self, want_loss=True):
+        """Loss of the last forward pass (waits for the network stream), or None with
+        want_loss=False (nothing waits)."""
+        if not want_loss:
+            check(self.L.lbm_unet_backward(self.h, None))
+            return None
         loss = ctypes.c_double()
         check(self.L.lbm_unet_backward(self.h, ctypes.byref(loss)))
         return loss.value
@@ -112,59 +128,90 @@ class UNet:
 
 
 class WarmStart:
-    """Online predictor of converged fields.
+    """Online predictor of converged fields, as the change from the previous converged solution.
 
-    observe(inputs, targets) stores a copy of one converged solution (device arrays) and trains
-    `steps` Adam updates over the stored solutions (the newest `replay` of them, round robin).
-    predict(inputs, outputs) writes the prediction for the given inputs into `outputs`.
-    The per-field normalisation (mean, scale) is fixed from the first observed solution.
+    observe(inputs, targets): after a converged solve, stores (inputs, targets, previous targets)
+        as a training sample and keeps a copy of targets as the base of the next prediction.
+    train(): queues `steps` Adam updates over the newest `replay` samples (round robin) on the
+        network's stream and returns at once; the solve that follows runs beside it.
+    predict(inputs, outputs): outputs = previous targets + predicted change, each field clamped
+        to the range its observed values spanned (widened by 5 % on either side).  Before any
+        training the network outputs zero, so the first warm start is the previous solution.
+    The normalisation of the change (per-field mean and scale) is fixed from the first sample.
     """
 
-    def __init__(self, ctx, dims, n_in, n_out, seed=0, lr=1e-3, steps=40, replay=4, min_samples=1):
+    def __init__(self, ctx, dims, n_in, n_out, seed=0, lr=1e-3, steps=40, replay=4):
         self.ctx = ctx
         self.net = UNet(ctx, dims, n_in, n_out, seed=seed)
-        self.lr, self.steps, self.replay, self.min_samples = lr, steps, replay, min_samples
-        self.samples = []  # [(inputs copies, targets copies)]
-        self.mean = self.scale = None
-        self.last_loss = None
+        self.lr, self.steps, self.replay = lr, steps, replay
+        self.samples = []  # [(inputs, targets, base)] device copies
+        self.prev = None   # copies of the last observed targets
+        self.mean = np.zeros(n_out)
+        self.scale = np.ones(n_out)
+        self.have_stats = False
+        self.lo = self.hi = None
         self.n_observed = 0
+        self.trained_steps = 0
 
     @property
     def ready(self):
-        return self.n_observed >= self.min_samples
+        return self.prev is not None
+
+    def _copies(self, src, dst=None):
+        if dst is None:
+            dst = [self.ctx.array(a.shape) for a in src]
+        for d, a in zip(dst, src):
+            check(self.ctx.lib.lbm_device_copy(self.ctx.h, d.ptr, a.ptr, 8 * a.size))
+        return dst
 
     def observe(self, inputs, targets):
-        if self.mean is None:
-            host = [t.download().ravel() for t in targets]
-            self.mean = np.array([h.mean() for h in host])
-            self.scale = np.array([max(h.std(), 1e-12) for h in host])
-        if len(self.samples) >= self.replay:
-            ins, outs = self.samples.pop(0)
-        else:
-            ins = [self.ctx.array(a.shape) for a in inputs]
-            outs = [self.ctx.array(a.shape) for a in targets]
-        for dst, src in zip(ins + outs, list(inputs) + list(targets)):
-            check(self.ctx.lib.lbm_device_copy(self.ctx.h, dst.ptr, src.ptr, 8 * src.size))
-        self.samples.append((ins, outs))
+        host = [t.download().ravel() for t in targets]
+        lo = np.array([h.min() for h in host])
+        hi = np.array([h.max() for h in host])
+        self.lo = lo if self.lo is None else np.minimum(self.lo, lo)
+        self.hi = hi if self.hi is None else np.maximum(self.hi, hi)
+        if self.prev is not None:
+            if not self.have_stats:
+                base = [b.download().ravel() for b in self.prev]
+                self.mean = np.array([(h - b).mean() for h, b in zip(host, base)])
+                self.scale = np.array([max((h - b).std(), 1e-30) for h, b in zip(host, base)])
+                self.have_stats = True
+            old = self.samples.pop(0) if len(self.samples) >= self.replay else (None, None, None)
+            ins = self._copies(inputs, old[0])
+            outs = self._copies(targets, old[1])
+            base = self._copies(self.prev, old[2])
+            self.samples.append((ins, outs, base))
+        self.prev = self._copies(targets, self.prev)
         self.n_observed += 1
-        loss = None
-        for s in range(self.steps):
-            ins, outs = self.samples[(len(self.samples) - 1 - s) % len(self.samples)]
+
+    def train(self, steps=None):
+        steps = self.steps if steps is None else steps
+        if not self.samples:
+            return
+        for s in range(steps):
+            ins, outs, base = self.samples[(len(self.samples) - 1 - s) % len(self.samples)]
             self.net.set_input(ins)
-            self.net.set_target(outs, self.mean, self.scale)
+            self.net.set_target(outs, base, self.mean, self.scale)
             self.net.forward()
-            if s == self.steps - 1:
-                loss = self.net.backward()  # waits for the device once per observation
-            else:
-                check(self.net.L.lbm_unet_backward(self.net.h, None))
+            self.net.backward(want_loss=False)
             self.net.adam(self.lr)
-        self.last_loss = loss
-        return loss
+        self.trained_steps += steps
+
+    def sample_loss(self, k=-1):
+        """Loss on stored sample k (waits for the device); None without samples."""
+        if not self.samples:
+            return None
+        ins, outs, base = self.samples[k]
+        self.net.set_input(ins)
+        self.net.set_target(outs, base, self.mean, self.scale)
+        self.net.forward()
+        return self.net.backward()
 
     def predict(self, inputs, outputs):
         self.net.set_input(inputs)
         self.net.forward()
-        self.net.output(outputs, self.mean, self.scale)
+        pad = 0.05 * (self.hi - self.lo)
+        self.net.output(outputs, self.prev, self.mean, self.scale, self.lo - pad, self.hi + pad)
 
     def close(self):
         self.net.close()

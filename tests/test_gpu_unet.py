@@ -41,8 +41,10 @@ def _setup(ctx, dims, cin, cout, seed):
     net = UNet(ctx, dims, cin, cout, seed=seed)
     ref = U.UNet(dims, cin, cout)
     assert net.nparams == ref.nparams
-    p = net.params()  # the device's own He-uniform start, as FP32 values
-    # nonzero biases so that the bias paths are exercised
+    p = net.params()  # the device's own He-uniform start, as FP32 values (the head starts at zero)
+    # a nonzero head and nonzero biases, so that every gradient path is exercised
+    name, ci, co, taps, wo, bo = ref.layers[-1]
+    p[wo:bo] = np.float32(rng.uniform(-1, 1, bo - wo) * np.sqrt(3.0 / ci))
     for name, ci, co, taps, wo, bo in ref.layers:
         p[bo:bo + co] = np.float32(rng.uniform(-0.05, 0.05, co))
     net.set_params(p)
