@@ -44,7 +44,7 @@ typedef __attribute__((ext_vector_type(4))) float f4;
 constexpr int kMaxFields = 16;
 constexpr int kMaxCg = 64;   // widest output gradient a weight-gradient launch stages
 constexpr int kMaxCx = 96;   // widest input it stages
-constexpr int kTileV = 32;   // voxels per LDS tile (one matrix-core K step)
+constexpr int kTileV = 64;   // voxels per LDS tile (two matrix-core K steps of 32)
 
 __host__ __device__ inline bf16_t f2bf(float f) {
     uint32_t u;
@@ -140,10 +140,12 @@ __global__ void __launch_bounds__(256) k_conv(const bf16_t *__restrict__ x, int 
 
 // ---- weight gradient ----------------------------------------------------------------------------
 // part[split][tap][o][i] = sum over the split's voxels v of g[v][o] x[v + off(tap)][i].
-// Matrix core: A[m = o][k = v], B[k = v][n = i], K = 32 voxels per LDS tile.
+// Matrix core: A[m = o][k = v], B[k = v][n = i], two K steps of 32 voxels per LDS tile.
+// The tap-0 workgroups also sum the staged g tiles: bpart[split][o] = sum over the split of g[v][o].
 __global__ void __launch_bounds__(256) k_conv_wgrad(const bf16_t *__restrict__ g, int cg,
                                                     const bf16_t *__restrict__ x, int cx, int taps, int nx, int ny,
-                                                    int nz, int chunk, float *__restrict__ part) {
+                                                    int nz, int chunk, float *__restrict__ part,
+                                                    float *__restrict__ bpart) {
     __shared__ __align__(16) bf16_t sg[kMaxCg][kTileV + 8];
     __shared__ __align__(16) bf16_t sx[kMaxCx][kTileV + 8];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, r = lane & 15, q = lane >> 4;
@@ -157,6 +159,10 @@ __global__ void __launch_bounds__(256) k_conv_wgrad(const bf16_t *__restrict__ g
 #pragma unroll
     for (int s = 0; s < 6; ++s) acc[s] = f4{0.f, 0.f, 0.f, 0.f};
     const int gu = cg / 8, xu = cxp / 8;
+    // bias: in tap-0 workgroups thread (o, quarter) = (tid / 4, tid % 4) sums a quarter of row o
+    const bool bias = tap == 0 && tid < 4 * cg;
+    const int bo = tid >> 2, bq = tid & 3;
+    float bsum = 0.f;
     for (int vb = vbeg; vb < vend; vb += kTileV) {
         for (int u = tid; u < kTileV * gu; u += 256) {
             const int vl = u / gu, c8 = (u - vl * gu) * 8, v = vb + vl;
@@ -179,14 +185,20 @@ __global__ void __launch_bounds__(256) k_conv_wgrad(const bf16_t *__restrict__ g
             for (int j = 0; j < 8; ++j) sx[c8 + j][vl] = e[j];
         }
         __syncthreads();
+        if (bias)
+#pragma unroll
+            for (int vl = 0; vl < kTileV / 4; ++vl) bsum += bf2f(sg[bo][bq * (kTileV / 4) + vl]);
 #pragma unroll
         for (int s = 0; s < 6; ++s) {
             const int t = wv + 4 * s;
             if (t < tiles) {
                 const int mt = t / nnt, nt = t - mt * nnt;
-                const bf16x8 a = __builtin_bit_cast(bf16x8, *(const uint4 *)&sg[mt * 16 + r][8 * q]);
-                const bf16x8 b = __builtin_bit_cast(bf16x8, *(const uint4 *)&sx[nt * 16 + r][8 * q]);
-                acc[s] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[s], 0, 0, 0);
+#pragma unroll
+                for (int kh = 0; kh < kTileV / 32; ++kh) {
+                    const bf16x8 a = __builtin_bit_cast(bf16x8, *(const uint4 *)&sg[mt * 16 + r][32 * kh + 8 * q]);
+                    const bf16x8 b = __builtin_bit_cast(bf16x8, *(const uint4 *)&sx[nt * 16 + r][32 * kh + 8 * q]);
+                    acc[s] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[s], 0, 0, 0);
+                }
             }
         }
         __syncthreads();
@@ -201,42 +213,29 @@ __global__ void __launch_bounds__(256) k_conv_wgrad(const bf16_t *__restrict__ g
             for (int j = 0; j < 4; ++j) out[(size_t)(mt * 16 + 4 * q + j) * cxp + nt * 16 + r] = acc[s][j];
         }
     }
-}
-
-// grad[(o taps + tap) ci + i] = sum over splits (in order) of part[split][tap][o][i]
-__global__ void k_wgrad_reduce(const float *__restrict__ part, int nsplit, int taps, int cg, int cxp, int co, int ci,
-                               float *__restrict__ grad) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= co * taps * ci) return;
-    const int i = n % ci, t = (n / ci) % taps, o = n / (ci * taps);
-    float s = 0.f;
-    for (int p = 0; p < nsplit; ++p) s += part[(((size_t)p * taps + t) * cg + o) * cxp + i];
-    grad[n] = s;
-}
-
-// bias gradient: bpart[block][o] = sum over the block's voxels of g[v][o]
-__global__ void __launch_bounds__(256) k_bgrad(const bf16_t *__restrict__ g, int cg, int V, int chunk,
-                                               float *__restrict__ bpart) {
-    __shared__ float red[256];
-    const int tid = threadIdx.x, ngrp = 256 / cg, grp = tid / cg, o = tid - grp * cg;
-    const int vbeg = blockIdx.x * chunk, vend = min(V, vbeg + chunk);
-    float s = 0.f;
-    if (grp < ngrp)
-        for (int v = vbeg + grp; v < vend; v += ngrp) s += bf2f(g[(size_t)v * cg + o]);
-    red[tid] = s;
-    __syncthreads();
-    if (tid < cg) {
-        float t = 0.f;
-        for (int k = 0; k < ngrp; ++k) t += red[k * cg + tid];
-        bpart[(size_t)blockIdx.x * cg + tid] = t;
+    if (tap == 0) {  // uniform per workgroup
+        __shared__ float bred[4 * kMaxCg];
+        if (bias) bred[tid] = bsum;
+        __syncthreads();
+        if (tid < cg) bpart[(size_t)split * cg + tid] = (bred[4 * tid] + bred[4 * tid + 1]) + (bred[4 * tid + 2] + bred[4 * tid + 3]);
     }
 }
-__global__ void k_bgrad_reduce(const float *__restrict__ bpart, int nblk, int cg, int co, float *__restrict__ grad) {
-    const int o = threadIdx.x;
-    if (o >= co) return;
+
+// grad[(o taps + tap) ci + i] = sum over splits (in order) of part[split][tap][o][i]; the bias
+// gradient follows the weights: grad[co taps ci + o] = sum over splits of bpart[split][o]
+__global__ void k_wgrad_reduce(const float *__restrict__ part, const float *__restrict__ bpart, int nsplit, int taps,
+                               int cg, int cxp, int co, int ci, float *__restrict__ grad) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nw = co * taps * ci;
+    if (n >= nw + co) return;
     float s = 0.f;
-    for (int b = 0; b < nblk; ++b) s += bpart[(size_t)b * cg + o];
-    grad[o] = s;
+    if (n < nw) {
+        const int i = n % ci, t = (n / ci) % taps, o = n / (ci * taps);
+        for (int p = 0; p < nsplit; ++p) s += part[(((size_t)p * taps + t) * cg + o) * cxp + i];
+    } else {
+        for (int p = 0; p < nsplit; ++p) s += bpart[(size_t)p * cg + (n - nw)];
+    }
+    grad[n] = s;
 }
 
 // ---- weight packing -----------------------------------------------------------------------------
@@ -348,11 +347,19 @@ __global__ void __launch_bounds__(256) k_loss(const float *__restrict__ y, const
     }
     if (threadIdx.x == 0) lpart[blockIdx.x] = red[0];
 }
-__global__ void k_loss_final(const float *__restrict__ lpart, int n, double inv, double *__restrict__ out) {
-    if (threadIdx.x != 0) return;
+// one workgroup of 256: strided partial sums, then a tree (fixed order, deterministic)
+__global__ void __launch_bounds__(256) k_loss_final(const float *__restrict__ lpart, int n, double inv,
+                                                    double *__restrict__ out) {
+    __shared__ double red[256];
     double s = 0.0;
-    for (int i = 0; i < n; ++i) s += lpart[i];
-    out[0] = s * inv;
+    for (int i = threadIdx.x; i < n; i += 256) s += lpart[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0] * inv;
 }
 // out[v][c] = act[v][c] > 0 ? bf16(d[v][c]) : 0   (d has stride cd)
 __global__ void k_relu_bwd(const float *__restrict__ d, int cd, const bf16_t *__restrict__ act, int C, int V,
@@ -398,14 +405,6 @@ __global__ void k_adam(float *__restrict__ p, const float *__restrict__ g, float
     m[i] = mi;
     v[i] = vi;
     p[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
-}
-__global__ void k_f32_to_f64(const float *__restrict__ a, double *__restrict__ b, size_t n) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) b[i] = a[i];
-}
-__global__ void k_f64_to_f32(const double *__restrict__ a, float *__restrict__ b, size_t n) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) b[i] = (float)a[i];
 }
 __global__ void k_bf_to_f64(const bf16_t *__restrict__ a, int cs, int cr, int V, double *__restrict__ b) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
@@ -495,13 +494,13 @@ int launch_conv(lbm_unet *N, const bf16_t *x, int cin, int taps, const bf16_t *w
     return LBM_OK;
 }
 
-// splits of a voxel range for the weight gradient: about 2048 workgroups over all taps
+// splits of a voxel range for the weight gradient: about 1024 workgroups over all taps
 void wsplit(int V, int taps, int *nsplit, int *chunk) {
-    int want = (2048 + taps - 1) / taps;
+    int want = (1024 + taps - 1) / taps;
     int maxs = (V + kTileV - 1) / kTileV;
     int ns = want < maxs ? want : maxs;
     if (ns < 1) ns = 1;
-    int ch = r32((V + ns - 1) / ns);
+    int ch = ((V + ns - 1) / ns + kTileV - 1) / kTileV * kTileV;
     *chunk = ch;
     *nsplit = (V + ch - 1) / ch;
 }
@@ -514,15 +513,13 @@ int conv_grads(lbm_unet *N, const lbm_unet::Conv &c, const bf16_t *g, const bf16
     int nsplit, chunk;
     wsplit(V, c.taps, &nsplit, &chunk);
     if ((size_t)nsplit * c.taps * cg * cxp > N->wpart_n) return lbm::set_error(LBM_ERR_ARG, "weight partials overflow");
-    k_conv_wgrad<<<dim3(nsplit, c.taps), 256, 0, N->s>>>(g, cg, x, c.cis, c.taps, d[0], d[1], d[2], chunk, N->wpart);
+    if ((size_t)nsplit * cg > N->bpart_n) return lbm::set_error(LBM_ERR_ARG, "bias partials overflow");
+    k_conv_wgrad<<<dim3(nsplit, c.taps), 256, 0, N->s>>>(g, cg, x, c.cis, c.taps, d[0], d[1], d[2], chunk, N->wpart,
+                                                         N->bpart);
     UCHK(hipGetLastError());
-    const int nw = c.co * c.taps * c.ci;
-    k_wgrad_reduce<<<blocks(nw), 256, 0, N->s>>>(N->wpart, nsplit, c.taps, cg, cxp, c.co, c.ci, N->g + c.wo);
-    UCHK(hipGetLastError());
-    int nb = (V + 1023) / 1024;
-    if ((size_t)nb * cg > N->bpart_n) return lbm::set_error(LBM_ERR_ARG, "bias partials overflow");
-    k_bgrad<<<nb, 256, 0, N->s>>>(g, cg, V, 1024, N->bpart);
-    k_bgrad_reduce<<<1, 64, 0, N->s>>>(N->bpart, nb, cg, c.co, N->g + c.bo);
+    const int nw = c.co * c.taps * c.ci;  // the bias follows the weights in the parameter vector
+    k_wgrad_reduce<<<blocks(nw + c.co), 256, 0, N->s>>>(N->wpart, N->bpart, nsplit, c.taps, cg, cxp, c.co, c.ci,
+                                                         N->g + c.wo);
     UCHK(hipGetLastError());
     return LBM_OK;
 }
@@ -584,7 +581,7 @@ int backward(lbm_unet *N) {
     int rc;
     const int nl = (int)blocks((size_t)V1 * co);
     k_loss<<<nl, 256, 0, N->s>>>(N->y32, N->tgt, co, N->cout, V1, N->gb0, N->lpart);
-    k_loss_final<<<1, 64, 0, N->s>>>(N->lpart, nl, 1.0 / ((double)V1 * N->cout), N->dloss);
+    k_loss_final<<<1, 256, 0, N->s>>>(N->lpart, nl, 1.0 / ((double)V1 * N->cout), N->dloss);
     // head and level-1 decoder
     F(conv_grads(N, N->conv[HEAD], N->gb0, N->d1b));
     F(conv_dx(N, HEAD, N->gb0, N->dxt));
@@ -792,7 +789,13 @@ int lbm_unet_create(lbm_context *ctx, int nx, int ny, int nz, int cin, int cout,
     }
     N->wpart_n = wpn;
     A(N->wpart, wpn);
-    N->bpart_n = ((V1 + 1023) / 1024) * kMaxCg;
+    size_t bpn = 0;
+    for (auto &c : N->conv) {
+        int ns, ch;
+        wsplit(N->V[c.level], c.taps, &ns, &ch);
+        bpn = std::max(bpn, (size_t)ns * r16(c.co));
+    }
+    N->bpart_n = bpn;
     A(N->bpart, N->bpart_n);
     A(N->lpart, blocks(V1 * r16(cout)));
     A(N->dloss, 1);

@@ -292,7 +292,7 @@ class Heatsink3D(Heatsink2D):
     # ---- learned warm start -----------------------------------------------------------------------
     WARM_OUT = ("rho", "ux", "uy", "uz", "tem", "ip", "iux", "iuy", "iuz", "item", "iqx", "iqy", "iqz")
 
-    def attach_warm_start(self, seed=0, lr=1e-3, steps=40, replay=4):
+    def attach_warm_start(self, seed=0, lr=1e-3, steps=40, replay=4, learn=True):
         """A WarmStart network for this grid.  Inputs: gamma, its change since the previous design,
         x/(lx-1), z/(lz-1) (the coordinates tell inlet from outlet and plate from lid); outputs:
         the change of the fields of WARM_OUT, which the forward (NS, AD) and adjoint (ANS, AAD)
@@ -304,7 +304,7 @@ class Heatsink3D(Heatsink2D):
         self.d["zpos"] = c.array((self.N,), data=self.k / (self.lz - 1.0))
         self.w = {n: c.array((self.N,), fill=0.0) for n in self.WARM_OUT}
         self.warm = WarmStart(c, (self.lx, self.ly, self.lz), 4, len(self.WARM_OUT), seed=seed, lr=lr, steps=steps,
-                              replay=replay)
+                              replay=replay, learn=learn)
         return self.warm
 
     def _warm_inputs(self):

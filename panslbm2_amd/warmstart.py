@@ -138,12 +138,14 @@ class WarmStart:
         to the range its observed values spanned (widened by 5 % on either side).  Before any
         training the network outputs zero, so the first warm start is the previous solution.
     The normalisation of the change (per-field mean and scale) is fixed from the first sample.
+    learn=False never trains: every prediction is then the previous solution (the plain
+    continuation warm start the learned one is measured against).
     """
 
-    def __init__(self, ctx, dims, n_in, n_out, seed=0, lr=1e-3, steps=40, replay=4):
+    def __init__(self, ctx, dims, n_in, n_out, seed=0, lr=1e-3, steps=40, replay=4, learn=True):
         self.ctx = ctx
         self.net = UNet(ctx, dims, n_in, n_out, seed=seed)
-        self.lr, self.steps, self.replay = lr, steps, replay
+        self.lr, self.steps, self.replay, self.learn = lr, steps, replay, learn
         self.samples = []  # [(inputs, targets, base)] device copies
         self.prev = None   # copies of the last observed targets
         self.mean = np.zeros(n_out)
@@ -170,7 +172,7 @@ class WarmStart:
         hi = np.array([h.max() for h in host])
         self.lo = lo if self.lo is None else np.minimum(self.lo, lo)
         self.hi = hi if self.hi is None else np.maximum(self.hi, hi)
-        if self.prev is not None:
+        if self.prev is not None and self.learn:
             if not self.have_stats:
                 base = [b.download().ravel() for b in self.prev]
                 self.mean = np.array([(h - b).mean() for h, b in zip(host, base)])
@@ -186,7 +188,7 @@ class WarmStart:
 
     def train(self, steps=None):
         steps = self.steps if steps is None else steps
-        if not self.samples:
+        if not self.samples or not self.learn:
             return
         for s in range(steps):
             ins, outs, base = self.samples[(len(self.samples) - 1 - s) % len(self.samples)]

@@ -97,3 +97,22 @@ def test_diverging_warm_start_falls_back_to_cold(ctx):
     assert _max_rel(hs.d["dfds"].download(), dfds_cold) <= 1e-4
     assert r["objective"] == pytest.approx(ref["objective"], rel=1e-6)
     hs.close()
+
+
+def test_learned_change_beats_previous_solution(ctx):
+    """What the trained network gains over the method without it: the same design loop started
+    from the previous converged solution alone (learn=False: the network is never trained, so its
+    prediction is exactly the previous solution) needs more LBM steps than with the learned change."""
+    from panslbm2_amd.heatsink3d import Heatsink3D
+
+    steps = {}
+    for learn in (False, True):
+        hs = Heatsink3D(ctx, **GRID)
+        hs.attach_warm_start(seed=1, steps=60, learn=learn)
+        hist = hs.optimize(8, vf=0.4, move=0.05)
+        assert all(r["converged"] for r in hist) and not any(any(r["fallback"]) for r in hist)
+        steps[learn] = [r["fwd_steps"] + r["adj_steps"] for r in hist]
+        hs.close()
+    print("previous-solution steps", steps[False], "learned steps", steps[True], flush=True)
+    assert steps[False][0] == steps[True][0]  # both cold
+    assert sum(steps[True][2:]) < sum(steps[False][2:])
