@@ -23,6 +23,12 @@ Also reported:
                    heat source and inlet velocity drawn per design) evaluated together, one kernel
                    launch per time step for all of them (FP32 mixed, autotuned workgroup shape),
                    against evaluating them one after another in plain FP64.
+  warm_start    -- a 3D design loop (D3Q15 + D3Q7 heat sink, 96 x 96 x 48, solid fraction 0.4,
+                   projected-gradient updates) run four ways: cold start in FP64 (the plain GPU
+                   form), previous-solution warm start, learned warm start (U-Net on the matrix
+                   cores predicting the change of the fields, trained beside the solves), and the
+                   learned warm start with FP32 mixed-precision storage; wall seconds and LBM steps
+                   per design iteration after the first, every solve to the 1e-6 residual.
 """
 import argparse
 import ctypes
@@ -186,6 +192,40 @@ def batch_extra(ctx, n, seed=2024):
                     "spread a 1e-6 step-to-step residual leaves (~1e-4 of the slowest mode)"}
 
 
+def warm_extra(ctx, iters, train_steps=20):
+    """The 96x96x48 3D design loop, cold vs warm-started (see the module docstring)."""
+    import numpy as np
+
+    from panslbm2_amd import F32, F64
+    from panslbm2_amd.heatsink3d import Heatsink3D
+
+    out = {"grid": [96, 96, 48], "lattices": "D3Q15 NS + D3Q7 AD", "iterations": iters, "solid_fraction": 0.4,
+           "train_steps_per_iteration": train_steps}
+    for mode, prec, learn in (("cold_fp64", F64, None), ("previous_solution_fp64", F64, False),
+                              ("learned_fp64", F64, True), ("learned_fp32", F32, True)):
+        hs = Heatsink3D(ctx, lx=96, ly=96, lz=48, plate=6, prec=prec)
+        if learn is not None:
+            hs.attach_warm_start(seed=0, steps=train_steps, learn=learn)
+        t0 = time.perf_counter()
+        hist = hs.optimize(iters, warm=(False if learn is None else None))
+        total = time.perf_counter() - t0
+        later = hist[1:]
+        out[mode] = {"s_per_iteration": round(float(np.mean([r["seconds"] + r["train_seconds"] for r in later])), 4),
+                     "steps_per_iteration": int(np.mean([r["fwd_steps"] + r["adj_steps"] for r in later])),
+                     "total_s": round(total, 2), "objective_last": hist[-1]["objective"],
+                     "fallbacks": int(sum(any(r["fallback"]) for r in hist)),
+                     "all_converged": bool(all(r["converged"] for r in hist))}
+        hs.close()
+        print("bench: warm start %s %.3f s/iteration" % (mode, out[mode]["s_per_iteration"]), file=sys.stderr,
+              flush=True)
+    base = out["cold_fp64"]["s_per_iteration"]
+    for mode in ("previous_solution_fp64", "learned_fp64", "learned_fp32"):
+        out[mode]["speedup_vs_cold_fp64"] = round(base / out[mode]["s_per_iteration"], 2)
+    out["note"] = ("per iteration after the first (the first is a cold start in every mode); the designs of the "
+                   "four loops drift apart slightly because each solve stops at the 1e-6 residual")
+    return out
+
+
 def kernel_only(ctx, a):
     """Launch one kernel kind a.steps times (eager, no graphs) and print its event-timed average."""
     from panslbm2_amd import D2Q5, D2Q9, Lattice, check
@@ -241,6 +281,7 @@ def main():
     ap.add_argument("--no-graphs", action="store_true", help="launch every step (for profilers)")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--batch", type=int, default=64, help="designs in the batched evaluation (0: skip it)")
+    ap.add_argument("--warm-iters", type=int, default=10, help="3D design-loop iterations per warm-start mode (0: skip)")
     ap.add_argument("--kernel-only", choices=["fwd", "adj", "stream"],
                     help="launch only the fused forward step, the fused adjoint step or the pull-stream copy "
                          "--steps times on --lx x --ly (profiling and counter calibration)")
@@ -298,6 +339,8 @@ def main():
         extra["fp32"] = fp32_extra(ctx, a, runs[-1])
     if a.batch > 0:
         extra["batch"] = batch_extra(ctx, a.batch)
+    if a.warm_iters > 0:
+        extra["warm_start"] = warm_extra(ctx, a.warm_iters)
     # like-for-like with the reference's lattice pairing (D2Q9 advection-diffusion)
     hs9 = Heatsink2D(ctx, lx=a.lx, ly=a.ly, ad_kind=D2Q9, max_steps=a.max_steps, use_graphs=not a.no_graphs)
     r9 = hs9.evaluate()
