@@ -226,11 +226,49 @@ def warm_extra(ctx, iters, train_steps=20):
     return out
 
 
+def unet_only(ctx, a):
+    """The warm-start network at 96x96x48 (4 input, 13 output fields): a.steps forward passes,
+    backward passes and Adam updates, host-timed around its own stream, with the forward FLOPs."""
+    import numpy as np
+
+    from panslbm2_amd.warmstart import UNet
+
+    dims = (96, 96, 48)
+    V = dims[0] * dims[1] * dims[2]
+    net = UNet(ctx, dims, 4, 13, seed=0)
+    rng = np.random.default_rng(0)
+    din = [ctx.array((V,), data=rng.uniform(0, 1, V)) for _ in range(4)]
+    dout = [ctx.array((V,), data=rng.standard_normal(V)) for _ in range(13)]
+    net.set_input(din)
+    net.set_target(dout)
+    c1, c2, c3 = 16, 32, 64
+    convs = [(0, 4, c1, 27), (0, c1, c1, 27), (1, c1, c2, 27), (1, c2, c2, 27), (2, c2, c3, 27), (2, c3, c3, 27),
+             (1, c3 + c2, c2, 27), (1, c2, c2, 27), (0, c2 + c1, c1, 27), (0, c1, c1, 27), (0, c1, 13, 1)]
+    flop = sum(2.0 * (V >> (3 * lvl)) * taps * ci * co for lvl, ci, co, taps in convs)
+    res = {}
+    for name, fn in (("forward", net.forward), ("backward", lambda: net.backward(want_loss=False)),
+                     ("adam", lambda: net.adam(1e-4))):
+        fn()
+        net.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            fn()
+        net.synchronize()
+        res[name + "_ms"] = round((time.perf_counter() - t0) * 1e3 / a.steps, 3)
+    res["forward_gflop"] = round(flop / 1e9, 2)
+    res["forward_tflops"] = round(flop / (res["forward_ms"] * 1e-3) / 1e12, 1)
+    res["backward_tflops"] = round(2 * flop / (res["backward_ms"] * 1e-3) / 1e12, 1)
+    net.close()
+    print(json.dumps({"kernel_only": "unet", "grid": list(dims), "passes": a.steps, **res}))
+
+
 def kernel_only(ctx, a):
     """Launch one kernel kind a.steps times (eager, no graphs) and print its event-timed average."""
     from panslbm2_amd import D2Q5, D2Q9, Lattice, check
     from panslbm2_amd.heatsink2d import Heatsink2D
 
+    if a.kernel_only == "unet":
+        return unet_only(ctx, a)
     if a.kernel_only == "stream":
         lat = Lattice(ctx, D2Q9, a.lx, a.ly)
         check(ctx.lib.lbm_stream(lat.h))
@@ -282,9 +320,10 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--batch", type=int, default=64, help="designs in the batched evaluation (0: skip it)")
     ap.add_argument("--warm-iters", type=int, default=10, help="3D design-loop iterations per warm-start mode (0: skip)")
-    ap.add_argument("--kernel-only", choices=["fwd", "adj", "stream"],
+    ap.add_argument("--kernel-only", choices=["fwd", "adj", "stream", "unet"],
                     help="launch only the fused forward step, the fused adjoint step or the pull-stream copy "
-                         "--steps times on --lx x --ly (profiling and counter calibration)")
+                         "--steps times on --lx x --ly (profiling and counter calibration), or time the "
+                         "warm-start network's passes at 96x96x48")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (requested --gpus %d)" % a.gpus,
