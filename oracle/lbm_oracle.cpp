@@ -48,12 +48,24 @@ const LatDesc kD3Q7 = {7, 3,
                        {0, 0, 0, 1, 0, 0, -1},
                        {0.0, 1.0 / 6.0, 1.0 / 6.0, 1.0 / 6.0, 1.0 / 6.0, 1.0 / 6.0, 1.0 / 6.0}};
 
+/* No reference counterpart: D3Q27 (rest; 6 axis in the D3Q15 order; 12 edges: xy, yz, xz; 8
+ * corners in the D3Q15 order), weights 8/27, 2/27, 1/54, 1/216. */
+const LatDesc kD3Q27 = {27, 3,
+                        {0, 1, 0, 0, -1, 0, 0, 1, -1, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, -1, 1, 1, -1, 1, -1, -1},
+                        {0, 0, 1, 0, 0, -1, 0, 1, 1, -1, -1, 1, -1, 1, -1, 0, 0, 0, 0, 1, 1, -1, 1, -1, -1, 1, -1},
+                        {0, 0, 0, 1, 0, 0, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, -1, 1, -1, 1, 1, 1, -1, -1, -1, -1, 1},
+                        {8.0 / 27.0, 2.0 / 27.0, 2.0 / 27.0, 2.0 / 27.0, 2.0 / 27.0, 2.0 / 27.0, 2.0 / 27.0,
+                         1.0 / 54.0, 1.0 / 54.0, 1.0 / 54.0, 1.0 / 54.0, 1.0 / 54.0, 1.0 / 54.0, 1.0 / 54.0,
+                         1.0 / 54.0, 1.0 / 54.0, 1.0 / 54.0, 1.0 / 54.0, 1.0 / 54.0, 1.0 / 216.0, 1.0 / 216.0,
+                         1.0 / 216.0, 1.0 / 216.0, 1.0 / 216.0, 1.0 / 216.0, 1.0 / 216.0, 1.0 / 216.0}};
+
 const LatDesc *desc_of(int kind) {
     switch (kind) {
     case OL_D2Q9: return &kD2Q9;
     case OL_D3Q15: return &kD3Q15;
     case OL_D2Q5: return &kD2Q5;
     case OL_D3Q7: return &kD3Q7;
+    case OL_D3Q27: return &kD3Q27;
     default: return nullptr;
     }
 }
@@ -658,6 +670,108 @@ void ad_unknowns_3d(const LatDesc &d, double *g, int a, int dd, double tem0, con
         g[c] = tem0 * e / inv_w(d, c);
     }
 }
+/* ---- non-equilibrium bounce-back on any 3D NS lattice (no reference counterpart; used for
+ * D3Q27).  The reference's D3Q15 Zou-He operators above are this construction written out for
+ * D3Q15: unknowns (c_a == -dd) f_c = f_opp + 6 w_c rho (c.u) + sum_t c_t Q_t with
+ * Q_t = (2/3 rho u_t - sum_{c_a == 0} c_t f_c) / beta, beta = sum over the unknowns of c_t^2
+ * (4 on D3Q15, 6 on D3Q27), which fixes the momentum to rho u.  The adjoint operators are the
+ * reference's D3Q15 adjoint operators in the same form: the outgoing populations (c_a == dd)
+ * become f_opp + rho0.  tests/test_oracle_d3q27.py checks all four against the reference-pinned
+ * D3Q15 operators. */
+double nebb_beta(const LatDesc &d, int a) {
+    int t = (a + 1) % 3, n = 0;
+    for (int c = 1; c < d.nc; ++c)
+        if (comp(d, c, a) == -1 && comp(d, c, t) != 0) ++n;
+    return (double)n;
+}
+/* c.u in axis order, zero components skipped */
+double cdot3(const LatDesc &d, int c, const double u[3]) {
+    double s = d.cx[c] > 0 ? u[0] : (d.cx[c] < 0 ? -u[0] : 0.0);
+    if (d.cy[c] > 0) s = s + u[1];
+    else if (d.cy[c] < 0) s = s - u[1];
+    if (d.cz[c] > 0) s = s + u[2];
+    else if (d.cz[c] < 0) s = s - u[2];
+    return s;
+}
+void nebb_unknowns(const LatDesc &d, double *f, int a, int dd, double rho, const double u[3]) {
+    const double be = nebb_beta(d, a);
+    double q[3] = {0.0, 0.0, 0.0};
+    for (int t = 0; t < 3; ++t) {
+        if (t == a) continue;
+        double sz = 0.0;
+        bool first = true;
+        for (int c = 1; c < d.nc; ++c) {
+            if (comp(d, c, a) != 0 || comp(d, c, t) == 0) continue;
+            if (first) { sz = comp(d, c, t) > 0 ? f[c] : -f[c]; first = false; }
+            else sz = comp(d, c, t) > 0 ? sz + f[c] : sz - f[c];
+        }
+        q[t] = (2.0 * rho * u[t] / 3.0 - sz) / be;
+    }
+    for (int c = 1; c < d.nc; ++c) {
+        if (comp(d, c, a) != -dd) continue;
+        double r = f[opposite(d, c)] + 6.0 * d.w[c] * rho * cdot3(d, c, u);
+        for (int t = 0; t < 3; ++t) {
+            if (t == a) continue;
+            if (comp(d, c, t) > 0) r = r + q[t];
+            else if (comp(d, c, t) < 0) r = r - q[t];
+        }
+        f[c] = r;
+    }
+}
+void nebb_set_u(const LatDesc &d, double *f, int face, double ux, double uy, double uz) {
+    int a = face / 2, dd = face % 2 ? 1 : -1;
+    double u[3] = {ux, uy, uz};
+    double rho = zou_he_known_3d(d, f, a, dd) / (dd < 0 ? 1.0 - u[a] : 1.0 + u[a]);
+    nebb_unknowns(d, f, a, dd, rho, u);
+}
+/* tangential velocities in the cyclic order of ns_set_rho_3d */
+void nebb_set_rho(const LatDesc &d, double *f, int face, double rho, double us, double ut) {
+    int a = face / 2, dd = face % 2 ? 1 : -1;
+    double u[3];
+    u[(a + 1) % 3] = us;
+    u[(a + 2) % 3] = ut;
+    double known = zou_he_known_3d(d, f, a, dd);
+    u[a] = dd < 0 ? 1.0 - known / rho : -1.0 + known / rho;
+    nebb_unknowns(d, f, a, dd, rho, u);
+}
+/* rho0 = (sum_{c_a == -dd} f_c (12 w_c c.u + 4/(3 beta) sum_t c_t u_t) - 2/3 eps) / (1 + dd u_a) */
+void nebb_iset_u(const LatDesc &d, double *f, int face, double ux, double uy, double uz, double eps) {
+    int a = face / 2, dd = face % 2 ? 1 : -1;
+    double u[3] = {ux, uy, uz};
+    const double be = nebb_beta(d, a);
+    double s = 0.0;
+    bool first = true;
+    for (int c = 1; c < d.nc; ++c) {
+        if (comp(d, c, a) != -dd) continue;
+        double ct = 0.0;
+        for (int t = 0; t < 3; ++t) {
+            if (t == a) continue;
+            if (comp(d, c, t) > 0) ct = ct + u[t];
+            else if (comp(d, c, t) < 0) ct = ct - u[t];
+        }
+        double term = f[c] * (12.0 * d.w[c] * cdot3(d, c, u) + 4.0 * ct / (3.0 * be));
+        if (first) { s = term; first = false; }
+        else s = s + term;
+    }
+    double rho0 = (s - 2.0 * eps / 3.0) / (dd < 0 ? 1.0 - u[a] : 1.0 + u[a]);
+    for (int c = 1; c < d.nc; ++c)
+        if (comp(d, c, a) == dd) f[c] = f[opposite(d, c)] + rho0;
+}
+/* rho0 = 12 sum_{c_a == -dd} w_c f_c */
+void nebb_iset_rho(const LatDesc &d, double *f, int face) {
+    int a = face / 2, dd = face % 2 ? 1 : -1;
+    double s = 0.0;
+    bool first = true;
+    for (int c = 1; c < d.nc; ++c) {
+        if (comp(d, c, a) != -dd) continue;
+        if (first) { s = d.w[c] * f[c]; first = false; }
+        else s = s + d.w[c] * f[c];
+    }
+    double rho0 = 12.0 * s;
+    for (int c = 1; c < d.nc; ++c)
+        if (comp(d, c, a) == dd) f[c] = f[opposite(d, c)] - rho0;
+}
+
 /* advection.h:149-238 (D3Q15 Dirichlet temperature); D3Q7: the same construction */
 void ad_set_t_3d(const LatDesc &d, double *g, int face, double tem, double ux, double uy, double uz) {
     int a = face / 2, dd = face % 2 ? 1 : -1;
@@ -964,7 +1078,8 @@ void ol_ns_set_u(ol_lattice *p, const int *mask, const double *ux, const double 
         if (!mask[t]) return;
         double f[27];
         load(p, idx, f);
-        if (d3) ns_set_u_3d(*p->d, f, face, ux[t], uy[t], uz[t]);
+        if (d3 && p->d->nc == 27) nebb_set_u(*p->d, f, face, ux[t], uy[t], uz[t]);
+        else if (d3) ns_set_u_3d(*p->d, f, face, ux[t], uy[t], uz[t]);
         else ns_set_u_2d(f, face, ux[t], uy[t]);
         store(p, idx, f);
     });
@@ -976,7 +1091,8 @@ void ol_ns_set_rho(ol_lattice *p, const int *mask, const double *rho, const doub
         if (!mask[t]) return;
         double f[27];
         load(p, idx, f);
-        if (d3) ns_set_rho_3d(*p->d, f, face, rho[t], us[t], ut[t]);
+        if (d3 && p->d->nc == 27) nebb_set_rho(*p->d, f, face, rho[t], us[t], ut[t]);
+        else if (d3) ns_set_rho_3d(*p->d, f, face, rho[t], us[t], ut[t]);
         else ns_set_rho_2d(f, face, rho[t], us[t]);
         store(p, idx, f);
     });
@@ -1169,8 +1285,28 @@ void ol_ans_iset_u(ol_lattice *p, const int *mask, const double *ux, const doubl
         if (!mask[t]) return;
         double f[27];
         load(p, idx, f);
-        if (d3) ans_iset_u_3d(*p->d, f, face, ux[t], uy[t], uz[t], eps);
+        if (d3 && p->d->nc == 27) nebb_iset_u(*p->d, f, face, ux[t], uy[t], uz[t], eps);
+        else if (d3) ans_iset_u_3d(*p->d, f, face, ux[t], uy[t], uz[t], eps);
         else ans_iset_u_2d(f, face, ux[t], uy[t], eps, compat);
+        store(p, idx, f);
+    });
+}
+/* the non-equilibrium bounce-back operators on any 3D NS lattice (D3Q15 included), for the
+ * tests that hold them against the reference's D3Q15 operators: op 0 SetU (v0, v1, v2 = u),
+ * 1 SetRho (v0 = rho, v1, v2 = tangential u), 2 adjoint SetU (u, eps), 3 adjoint SetRho */
+void ol_nebb_3d(ol_lattice *p, int op, const int *mask, const double *v0, const double *v1, const double *v2,
+                double eps) {
+    if (p->d->nd != 3) return;
+    for_faces(p, [&](int face, int t, int idx) {
+        if (!mask[t]) return;
+        double f[27];
+        load(p, idx, f);
+        switch (op) {
+        case 0: nebb_set_u(*p->d, f, face, v0[t], v1[t], v2[t]); break;
+        case 1: nebb_set_rho(*p->d, f, face, v0[t], v1[t], v2[t]); break;
+        case 2: nebb_iset_u(*p->d, f, face, v0[t], v1[t], v2[t], eps); break;
+        default: nebb_iset_rho(*p->d, f, face); break;
+        }
         store(p, idx, f);
     });
 }
@@ -1181,7 +1317,8 @@ void ol_ans_iset_rho(ol_lattice *p, const int *mask) {
         if (!mask[t]) return;
         double f[27];
         load(p, idx, f);
-        if (d3) ans_iset_rho_3d(*p->d, f, face);
+        if (d3 && p->d->nc == 27) nebb_iset_rho(*p->d, f, face);
+        else if (d3) ans_iset_rho_3d(*p->d, f, face);
         else ans_iset_rho_2d(f, face);
         store(p, idx, f);
     });

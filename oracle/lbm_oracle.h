@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-enum { OL_D2Q9 = 0, OL_D3Q15 = 1, OL_D2Q5 = 2, OL_D3Q7 = 3 };
+enum { OL_D2Q9 = 0, OL_D3Q15 = 1, OL_D2Q5 = 2, OL_D3Q7 = 3, OL_D3Q27 = 4 };
 
 typedef struct ol_lattice ol_lattice;
 
@@ -86,6 +86,10 @@ void ol_ans_collide(ol_lattice *p, const double *rho, const double *ux, const do
 void ol_ans_iset_u(ol_lattice *p, const int *mask, const double *ux, const double *uy, const double *uz,
                    double eps, int compat);
 void ol_ans_iset_rho(ol_lattice *p, const int *mask);
+/* non-equilibrium bounce-back on any 3D NS lattice (op 0 SetU, 1 SetRho, 2 iSetU, 3 iSetRho);
+ * the D3Q27 operators, and on D3Q15 the construction checked against the reference's operators */
+void ol_nebb_3d(ol_lattice *p, int op, const int *mask, const double *v0, const double *v1, const double *v2,
+                double eps);
 
 /* AAD (adjointadvection.h) */
 void ol_aad_init(ol_lattice *q, const double *ux, const double *uy, const double *uz,

@@ -12,9 +12,9 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(ROOT, "oracle", "liblbm_oracle.so")
 REF_SO = os.path.join(ROOT, "oracle", "_ref", "libref_panslbm2.so")
 
-D2Q9, D3Q15, D2Q5, D3Q7 = 0, 1, 2, 3
-NC = {D2Q9: 9, D3Q15: 15, D2Q5: 5, D3Q7: 7}
-ND = {D2Q9: 2, D3Q15: 3, D2Q5: 2, D3Q7: 3}
+D2Q9, D3Q15, D2Q5, D3Q7, D3Q27 = 0, 1, 2, 3, 4
+NC = {D2Q9: 9, D3Q15: 15, D2Q5: 5, D3Q7: 7, D3Q27: 27}
+ND = {D2Q9: 2, D3Q15: 3, D2Q5: 2, D3Q7: 3, D3Q27: 3}
 
 VP = ctypes.c_void_p
 I = ctypes.c_int
@@ -73,6 +73,7 @@ def lib():
         L.ol_ans_collide.argtypes = [VP] + [VP] * 11 + [D, VP, I]
         L.ol_ans_iset_u.argtypes = [VP] * 5 + [D, I]
         L.ol_ans_iset_rho.argtypes = [VP, VP]
+        L.ol_nebb_3d.argtypes = [VP, I, VP, VP, VP, VP, D]
         L.ol_aad_init.argtypes = [VP] * 8
         L.ol_aad_collide_fc.argtypes = [VP] + [VP] * 11 + [VP, D, VP] + [VP] * 5 + [VP, I, VP]
         L.ol_aad_collide_nc.argtypes = [VP] + [VP] * 11 + [VP, D, VP] + [VP] * 5 + [VP] + [D] * 3 + [I, VP]
