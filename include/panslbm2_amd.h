@@ -58,6 +58,7 @@ extern "C" {
 #define LBM_D3Q15 1
 #define LBM_D2Q5 2
 #define LBM_D3Q7 3
+#define LBM_D3Q27 4 /* no reference counterpart: NS lattice, coupled with D3Q7 advection-diffusion */
 
 /* population storage (no reference counterpart: the reference is FP64 throughout).  LBM_F64 is
  * value-identical to the reference.  LBM_F32 stores the populations in FP32, those of a lattice

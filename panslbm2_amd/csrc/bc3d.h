@@ -10,6 +10,7 @@
 // constructions with its single unknown population.
 #pragma once
 #include "lattice.h"
+#include "physics.h"
 
 namespace lbm {
 namespace b3 {
@@ -136,10 +137,119 @@ __device__ __forceinline__ void ad_unknowns(double *g, double tem0, const double
     });
 }
 
+// ---- non-equilibrium bounce-back (D3Q27; no reference counterpart) ------------------------
+// The reference's D3Q15 Zou-He operators below are this construction written out for D3Q15:
+// unknowns (c_A == -DD) f_c = f_opp + 6 w_c rho (c.u) + sum_t c_t Q_t with
+// Q_t = (2/3 rho u_t - sum_{c_A == 0} c_t f_c) / beta, beta = sum over the unknowns of c_t^2, which
+// fixes the momentum to rho u; the adjoints set the outgoing populations to f_opp + rho0 as the
+// reference's D3Q15 adjoints do.  Same evaluation order as the oracle's nebb_* functions.
+template <class L, int A>
+constexpr double nebb_beta() {
+    int n = 0;
+    for (int c = 1; c < L::nc; ++c)
+        if (comp<L>(c, A) == -1 && comp<L>(c, (A + 1) % 3) != 0) ++n;
+    return (double)n;
+}
+template <class L, int A, int DD>
+__device__ __forceinline__ void nebb_unknowns(double *f, double rho, const double (&u)[3]) {
+    constexpr double BE = nebb_beta<L, A>();
+    double q[3] = {0.0, 0.0, 0.0};
+    sfor<0, 3>([&](auto t) {
+        constexpr int T = decltype(t)::value;
+        if constexpr (T != A) {
+            double sz = 0.0;
+            bool first = true;
+            sfor<1, L::nc>([&](auto c) {
+                constexpr int C = decltype(c)::value;
+                if constexpr (comp<L>(C, A) == 0 && comp<L>(C, T) != 0) {
+                    if (first) {
+                        sz = comp<L>(C, T) > 0 ? f[C] : -f[C];
+                        first = false;
+                    } else {
+                        sz = comp<L>(C, T) > 0 ? sz + f[C] : sz - f[C];
+                    }
+                }
+            });
+            q[T] = (2.0 * rho * u[T] / 3.0 - sz) / BE;
+        }
+    });
+    sfor<1, L::nc>([&](auto c) {
+        constexpr int C = decltype(c)::value;
+        if constexpr (comp<L>(C, A) == -DD) {
+            double r = f[opp<L>(C)] + 6.0 * L::w[C] * rho * cdot<L, C>(u[0], u[1], u[2]);
+            sfor<0, 3>([&](auto t) {
+                constexpr int T = decltype(t)::value;
+                if constexpr (T != A) {
+                    if constexpr (comp<L>(C, T) > 0) r = r + q[T];
+                    else if constexpr (comp<L>(C, T) < 0) r = r - q[T];
+                }
+            });
+            f[C] = r;
+        }
+    });
+}
+template <class L, int A, int DD>
+__device__ __forceinline__ void nebb_iset_u(double *f, const double (&u)[3], double eps) {
+    constexpr double BE = nebb_beta<L, A>();
+    double s = 0.0;
+    bool first = true;
+    sfor<1, L::nc>([&](auto c) {
+        constexpr int C = decltype(c)::value;
+        if constexpr (comp<L>(C, A) == -DD) {
+            double ct = 0.0;
+            sfor<0, 3>([&](auto t) {
+                constexpr int T = decltype(t)::value;
+                if constexpr (T != A) {
+                    if constexpr (comp<L>(C, T) > 0) ct = ct + u[T];
+                    else if constexpr (comp<L>(C, T) < 0) ct = ct - u[T];
+                }
+            });
+            double term = f[C] * (12.0 * L::w[C] * cdot<L, C>(u[0], u[1], u[2]) + 4.0 * ct / (3.0 * BE));
+            if (first) {
+                s = term;
+                first = false;
+            } else {
+                s = s + term;
+            }
+        }
+    });
+    double rho0 = (s - 2.0 * eps / 3.0) / (DD < 0 ? 1.0 - u[A] : 1.0 + u[A]);
+    sfor<1, L::nc>([&](auto c) {
+        constexpr int C = decltype(c)::value;
+        if constexpr (comp<L>(C, A) == DD) f[C] = f[opp<L>(C)] + rho0;
+    });
+}
+template <class L, int A, int DD>
+__device__ __forceinline__ void nebb_iset_rho(double *f) {
+    double s = 0.0;
+    bool first = true;
+    sfor<1, L::nc>([&](auto c) {
+        constexpr int C = decltype(c)::value;
+        if constexpr (comp<L>(C, A) == -DD) {
+            if (first) {
+                s = L::w[C] * f[C];
+                first = false;
+            } else {
+                s = s + L::w[C] * f[C];
+            }
+        }
+    });
+    double rho0 = 12.0 * s;
+    sfor<1, L::nc>([&](auto c) {
+        constexpr int C = decltype(c)::value;
+        if constexpr (comp<L>(C, A) == DD) f[C] = f[opp<L>(C)] - rho0;
+    });
+}
+
 // ---- per-face operators -------------------------------------------------------------------
 template <class L, int A, int DD>
 __device__ __forceinline__ void ns_set_u(double *f, double ux, double uy, double uz) {
     const double u[3] = {ux, uy, uz};
+    if constexpr (L::nc == 27) {
+        double rho = zou_he_known<L, A, DD>(f) / (DD < 0 ? 1.0 - u[A] : 1.0 + u[A]);
+        nebb_unknowns<L, A, DD>(f, rho, u);
+        return;
+    }
     double rho0 = zou_he_known<L, A, DD>(f) / (DD < 0 ? 1.0 - u[A] : 1.0 + u[A]);
     double m[3];
     sfor<0, 3>([&](auto x) {
@@ -157,6 +267,10 @@ __device__ __forceinline__ void ns_set_rho(double *f, double rho, double us, dou
     u[(A + 2) % 3] = ut;
     double known = zou_he_known<L, A, DD>(f);
     u[A] = DD < 0 ? 1.0 - known / rho : -1.0 + known / rho;
+    if constexpr (L::nc == 27) {
+        nebb_unknowns<L, A, DD>(f, rho, u);
+        return;
+    }
     double m[3];
     sfor<0, 3>([&](auto x) {
         constexpr int AX = decltype(x)::value;
@@ -190,6 +304,10 @@ __device__ __forceinline__ void ad_set_q(double *g, double qn, double ux, double
 template <class L, int A, int DD>
 __device__ __forceinline__ void ans_iset_u(double *f, double ux, double uy, double uz, double eps) {
     const double u[3] = {ux, uy, uz};
+    if constexpr (L::nc == 27) {
+        nebb_iset_u<L, A, DD>(f, u, eps);
+        return;
+    }
     double r = -4.0 * eps;
     sfor<0, 3>([&](auto x) {
         constexpr int AX = decltype(x)::value;
@@ -209,6 +327,10 @@ __device__ __forceinline__ void ans_iset_u(double *f, double ux, double uy, doub
 }
 template <class L, int A, int DD>
 __device__ __forceinline__ void ans_iset_rho(double *f) {
+    if constexpr (L::nc == 27) {
+        nebb_iset_rho<L, A, DD>(f);
+        return;
+    }
     double rho0 = normal_sum8<L, A, -DD>(f) / 6.0;
     sfor<1, L::nc>([&](auto c) {
         constexpr int C = decltype(c)::value;

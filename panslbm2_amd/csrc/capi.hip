@@ -48,11 +48,12 @@ int kind_nc(int kind) {
     case LBM_D3Q15: return 15;
     case LBM_D2Q5: return 5;
     case LBM_D3Q7: return 7;
+    case LBM_D3Q27: return 27;
     }
     return 0;
 }
-int kind_nd(int kind) { return (kind == LBM_D3Q15 || kind == LBM_D3Q7) ? 3 : 2; }
-bool is_ns_kind(int kind) { return kind == LBM_D2Q9 || kind == LBM_D3Q15; }
+int kind_nd(int kind) { return (kind == LBM_D3Q15 || kind == LBM_D3Q7 || kind == LBM_D3Q27) ? 3 : 2; }
+bool is_ns_kind(int kind) { return kind == LBM_D2Q9 || kind == LBM_D3Q15 || kind == LBM_D3Q27; }
 
 // grow-only device buffer
 struct DevBuf {
@@ -1026,6 +1027,7 @@ int lbm_ad_macro_brinkman_collide_heat_exchange(lbm_lattice *p, double *rho, dou
 int lbm_ad_set_t(lbm_lattice *q, const int *mask, const double *tem, const double *ux, const double *uy,
                  const double *uz) {
     if (!q || !mask || !tem || !ux || !uy || (q->nd == 3 && !uz)) return fail(LBM_ERR_ARG, "bad arguments");
+    if (q->kind == LBM_D3Q27) return fail(LBM_ERR_UNSUPPORTED, "advection-diffusion operators run on D2Q5/D2Q9/D3Q7/D3Q15, not D3Q27");
     FieldSet fs{nullptr, ux, uy, uz, nullptr, nullptr};
     return boundary_call(q, [&](lbm_program *P) {
         return prog_record(P, q, lbm::OP_AD_SET_T, mask, &fs, [&](int t, Op &o) { o.v[0] = tem[t]; });
@@ -1034,6 +1036,7 @@ int lbm_ad_set_t(lbm_lattice *q, const int *mask, const double *tem, const doubl
 int lbm_ad_set_q(lbm_lattice *q, const int *mask, const double *qn, const double *ux, const double *uy,
                  const double *uz, const double *diff, double diff_const) {
     if (!q || !mask || !qn || !ux || !uy || (q->nd == 3 && !uz)) return fail(LBM_ERR_ARG, "bad arguments");
+    if (q->kind == LBM_D3Q27) return fail(LBM_ERR_UNSUPPORTED, "advection-diffusion operators run on D2Q5/D2Q9/D3Q7/D3Q15, not D3Q27");
     FieldSet fs{nullptr, ux, uy, uz, nullptr, diff};
     return boundary_call(q, [&](lbm_program *P) {
         return prog_record(P, q, lbm::OP_AD_SET_Q, mask, &fs, [&](int t, Op &o) {
@@ -1193,6 +1196,7 @@ int lbm_aad_macro_brinkman_collide_heat_exchange(lbm_lattice *p, const double *r
 }
 int lbm_aad_iset_t(lbm_lattice *q, const int *mask, const double *ux, const double *uy, const double *uz) {
     if (!q || !mask || !ux || !uy || (q->nd == 3 && !uz)) return fail(LBM_ERR_ARG, "bad arguments");
+    if (q->kind == LBM_D3Q27) return fail(LBM_ERR_UNSUPPORTED, "advection-diffusion operators run on D2Q5/D2Q9/D3Q7/D3Q15, not D3Q27");
     FieldSet fs{nullptr, ux, uy, uz, nullptr, nullptr};
     return boundary_call(q, [&](lbm_program *P) {
         return prog_record(P, q, lbm::OP_AAD_ISET_T, mask, &fs, [](int, Op &) {});
@@ -1200,6 +1204,7 @@ int lbm_aad_iset_t(lbm_lattice *q, const int *mask, const double *ux, const doub
 }
 int lbm_aad_iset_q(lbm_lattice *q, const int *mask, const double *ux, const double *uy, const double *uz, double eps) {
     if (!q || !mask || !ux || !uy || (q->nd == 3 && !uz)) return fail(LBM_ERR_ARG, "bad arguments");
+    if (q->kind == LBM_D3Q27) return fail(LBM_ERR_UNSUPPORTED, "advection-diffusion operators run on D2Q5/D2Q9/D3Q7/D3Q15, not D3Q27");
     FieldSet fs{nullptr, ux, uy, uz, nullptr, nullptr};
     return boundary_call(q, [&](lbm_program *P) {
         return prog_record(P, q, lbm::OP_AAD_ISET_Q, mask, &fs, [&](int, Op &o) { o.v3 = eps; });

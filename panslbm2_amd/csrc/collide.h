@@ -685,6 +685,7 @@ static hipError_t dispatch_single(int ka, F &&f) {
     case kD3Q15: return f(D3Q15{}, NoLat{});
     case kD2Q5: return f(D2Q5{}, NoLat{});
     case kD3Q7: return f(D3Q7{}, NoLat{});
+    case kD3Q27: return f(D3Q27{}, NoLat{});
     }
     return hipErrorInvalidValue;
 }
@@ -696,6 +697,8 @@ static hipError_t dispatch_coupled(int ka, int kb, F &&f) {
     } else if (ka == kD3Q15) {
         if (kb == kD3Q15) return f(D3Q15{}, D3Q15{});
         if (kb == kD3Q7) return f(D3Q15{}, D3Q7{});
+    } else if (ka == kD3Q27) {
+        if (kb == kD3Q7) return f(D3Q27{}, D3Q7{});
     }
     return hipErrorInvalidValue;
 }
@@ -715,6 +718,7 @@ static hipError_t dispatch_single_nd(int ka, F &&f) {
     } else {
         if (ka == kD3Q15) return f(D3Q15{}, NoLat{});
         if (ka == kD3Q7) return f(D3Q7{}, NoLat{});
+        if (ka == kD3Q27) return f(D3Q27{}, NoLat{});
     }
     return hipErrorInvalidValue;
 }
@@ -726,6 +730,7 @@ static hipError_t dispatch_coupled_nd(int ka, int kb, F &&f) {
     } else {
         if (ka == kD3Q15 && kb == kD3Q15) return f(D3Q15{}, D3Q15{});
         if (ka == kD3Q15 && kb == kD3Q7) return f(D3Q15{}, D3Q7{});
+        if (ka == kD3Q27 && kb == kD3Q7) return f(D3Q27{}, D3Q7{});
     }
     return hipErrorInvalidValue;
 }

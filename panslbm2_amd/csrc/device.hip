@@ -379,6 +379,7 @@ static hipError_t dispatch_kind_prec(int kind, int prec, F &&f) {
     case kD3Q15: return by_prec(D3Q15{});
     case kD2Q5: return by_prec(D2Q5{});
     case kD3Q7: return by_prec(D3Q7{});
+    case kD3Q27: return by_prec(D3Q27{});
     }
     return hipErrorInvalidValue;
 }
@@ -420,9 +421,9 @@ hipError_t launch_program_apply(int ka, int kb, int prec, void *a, void *b, int 
     });
 }
 
-// collides: by the dimension of the first lattice (kinds D2Q9/D2Q5 are 2D, D3Q15/D3Q7 3D) and the
-// storage precision
-static bool is_3d_kind(int k) { return k == kD3Q15 || k == kD3Q7; }
+// collides: by the dimension of the first lattice (kinds D2Q9/D2Q5 are 2D, D3Q15/D3Q7/D3Q27 3D)
+// and the storage precision
+static bool is_3d_kind(int k) { return k == kD3Q15 || k == kD3Q7 || k == kD3Q27; }
 #define LBM_COLLIDE_DISPATCH(eq, cl, a)                                                            \
     (cl.prec == kF64 ? (is_3d_kind(cl.ka) ? eq##_collide_nd<3, kF64>(cl, a) : eq##_collide_nd<2, kF64>(cl, a)) \
                      : (is_3d_kind(cl.ka) ? eq##_collide_nd<3, kF32>(cl, a) : eq##_collide_nd<2, kF32>(cl, a)))

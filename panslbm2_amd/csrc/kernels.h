@@ -100,36 +100,39 @@ __device__ __forceinline__ void apply_on(const Op &op, double *f, const double *
     case OP_IBOUNCE: bounce<L, true>(f, op.face, op.arg); break;
     case OP_NS_SET_U:
         if constexpr (L::kind == kD2Q9) ns_set_u_2d(f, op.face, op.v[0], op.v[1]);
-        else if constexpr (L::kind == kD3Q15) ns_set_u_3d<L>(f, op.face, op.v[0], op.v[1], op.v[2]);
+        else if constexpr (L::kind == kD3Q15 || L::kind == kD3Q27)
+            ns_set_u_3d<L>(f, op.face, op.v[0], op.v[1], op.v[2]);
         break;
     case OP_NS_SET_RHO:
         if constexpr (L::kind == kD2Q9) ns_set_rho_2d(f, op.face, op.v[0], op.v[1]);
-        else if constexpr (L::kind == kD3Q15) ns_set_rho_3d<L>(f, op.face, op.v[0], op.v[1], op.v[2]);
+        else if constexpr (L::kind == kD3Q15 || L::kind == kD3Q27)
+            ns_set_rho_3d<L>(f, op.face, op.v[0], op.v[1], op.v[2]);
         break;
-    case OP_AD_SET_T:
+    case OP_AD_SET_T:  // (the host rejects advection-diffusion operators on D3Q27)
         if constexpr (L::nd == 2) ad_set_t_2d<L>(f, op.face, op.v[0], s.ux, s.uy);
-        else if constexpr (L::nd == 3) ad_set_t_3d<L>(f, op.face, op.v[0], s.ux, s.uy, s.uz);
+        else if constexpr (L::nd == 3 && L::kind != kD3Q27) ad_set_t_3d<L>(f, op.face, op.v[0], s.ux, s.uy, s.uz);
         break;
     case OP_AD_SET_Q: {
         double d = s.has_diff ? s.diff : op.v[1];
         if constexpr (L::nd == 2) ad_set_q_2d<L>(f, op.face, op.v[0], s.ux, s.uy, d);
-        else if constexpr (L::nd == 3) ad_set_q_3d<L>(f, op.face, op.v[0], s.ux, s.uy, s.uz, d);
+        else if constexpr (L::nd == 3 && L::kind != kD3Q27) ad_set_q_3d<L>(f, op.face, op.v[0], s.ux, s.uy, s.uz, d);
     } break;
     case OP_ANS_ISET_U:
         if constexpr (L::kind == kD2Q9) ans_iset_u_2d(f, op.face, op.v[0], op.v[1], op.v3, op.arg);
-        else if constexpr (L::kind == kD3Q15) ans_iset_u_3d<L>(f, op.face, op.v[0], op.v[1], op.v[2], op.v3);
+        else if constexpr (L::kind == kD3Q15 || L::kind == kD3Q27)
+            ans_iset_u_3d<L>(f, op.face, op.v[0], op.v[1], op.v[2], op.v3);
         break;
     case OP_ANS_ISET_RHO:
         if constexpr (L::kind == kD2Q9) ans_iset_rho_2d(f, op.face);
-        else if constexpr (L::kind == kD3Q15) ans_iset_rho_3d<L>(f, op.face);
+        else if constexpr (L::kind == kD3Q15 || L::kind == kD3Q27) ans_iset_rho_3d<L>(f, op.face);
         break;
     case OP_AAD_ISET_T:
         if constexpr (L::nd == 2) aad_iset_t_2d<L>(f, op.face, s.ux, s.uy);
-        else if constexpr (L::nd == 3) aad_iset_t_3d<L>(f, op.face, s.ux, s.uy, s.uz);
+        else if constexpr (L::nd == 3 && L::kind != kD3Q27) aad_iset_t_3d<L>(f, op.face, s.ux, s.uy, s.uz);
         break;
     case OP_AAD_ISET_Q:
         if constexpr (L::nd == 2) aad_iset_q_2d<L>(f, op.face, s.ux, s.uy, op.v3);
-        else if constexpr (L::nd == 3) aad_iset_q_3d<L>(f, op.face, s.ux, s.uy, s.uz, op.v3);
+        else if constexpr (L::nd == 3 && L::kind != kD3Q27) aad_iset_q_3d<L>(f, op.face, s.ux, s.uy, s.uz, op.v3);
         break;
     case OP_AAD_ISET_RHO:
         if constexpr (L::kind == kD2Q9 && LO::nd == 2)

@@ -1,8 +1,9 @@
 // panslbm2_amd/csrc/lattice.h -- compile-time lattice descriptors and small device helpers.
 //
 // Velocity sets and weights of the reference lattices (d2q9.h:160-163, d3q15.h:251-254) plus
-// the advection-diffusion lattices D2Q5 and D3Q7 the reference lacks (c_s^2 = 1/3, so the AD
-// constants 1/c_s^2 = 3 and omega = 1/(3D + 1/2) of advection.h carry over).  Populations are
+// the advection-diffusion lattices D2Q5 and D3Q7 and the Navier-Stokes lattice D3Q27 the
+// reference lacks (c_s^2 = 1/3 throughout, so the constants of navierstokes.h and advection.h
+// carry over).  Populations are
 // stored structure-of-arrays: population c of node n at buf[c*N + n].
 #pragma once
 #include <hip/hip_runtime.h>
@@ -13,7 +14,7 @@
 
 namespace lbm {
 
-enum Kind : int { kD2Q9 = 0, kD3Q15 = 1, kD2Q5 = 2, kD3Q7 = 3 };
+enum Kind : int { kD2Q9 = 0, kD3Q15 = 1, kD2Q5 = 2, kD3Q7 = 3, kD3Q27 = 4 };
 
 // Population storage.  F64: the reference's arithmetic in FP64, value-identical to it.  F32:
 // populations stored in FP32 as deviations from a shift E_c (Enc below) and collided in FP32 on
@@ -61,6 +62,21 @@ struct D3Q15 {
     static constexpr double w[15] = {2.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0,
                                      1.0 / 9.0, 1.0 / 9.0, 1.0 / 72.0, 1.0 / 72.0, 1.0 / 72.0,
                                      1.0 / 72.0, 1.0 / 72.0, 1.0 / 72.0, 1.0 / 72.0, 1.0 / 72.0};
+};
+// rest; 6 axis in the D3Q15 order; 12 edges (xy, yz, xz); 8 corners in the D3Q15 order
+struct D3Q27 {
+    static constexpr int kind = kD3Q27, nc = 27, nd = 3;
+    static constexpr int cx[27] = {0, 1, 0, 0, -1, 0, 0, 1, -1, 1, -1, 0, 0, 0, 0,
+                                   1, 1, -1, -1, 1, -1, 1, 1, -1, 1, -1, -1};
+    static constexpr int cy[27] = {0, 0, 1, 0, 0, -1, 0, 1, 1, -1, -1, 1, -1, 1, -1,
+                                   0, 0, 0, 0, 1, 1, -1, 1, -1, -1, 1, -1};
+    static constexpr int cz[27] = {0, 0, 0, 1, 0, 0, -1, 0, 0, 0, 0, 1, 1, -1, -1,
+                                   1, -1, 1, -1, 1, 1, 1, -1, -1, -1, -1, 1};
+    static constexpr double w[27] = {8.0 / 27.0,  2.0 / 27.0,  2.0 / 27.0,  2.0 / 27.0,  2.0 / 27.0,  2.0 / 27.0,
+                                     2.0 / 27.0,  1.0 / 54.0,  1.0 / 54.0,  1.0 / 54.0,  1.0 / 54.0,  1.0 / 54.0,
+                                     1.0 / 54.0,  1.0 / 54.0,  1.0 / 54.0,  1.0 / 54.0,  1.0 / 54.0,  1.0 / 54.0,
+                                     1.0 / 54.0,  1.0 / 216.0, 1.0 / 216.0, 1.0 / 216.0, 1.0 / 216.0, 1.0 / 216.0,
+                                     1.0 / 216.0, 1.0 / 216.0, 1.0 / 216.0};
 };
 struct D2Q5 {
     static constexpr int kind = kD2Q5, nc = 5, nd = 2;

@@ -15,7 +15,7 @@ def run_gpu(ctx, case, op, inp, prec=pa.F64):
     k = case.kind
     N = case.N
     p = pa.Lattice(ctx, k, case.lx, case.ly, case.lz, prec=prec)
-    q = pa.Lattice(ctx, k, case.lx, case.ly, case.lz, prec=prec)
+    q = pa.Lattice(ctx, case.kind_b, case.lx, case.ly, case.lz, prec=prec)
     p.upload(inp["f"])
     q.upload(inp["g"])
     F = lambda a, dt=np.float64: pa.to_face(k, case.lx, case.ly, case.lz, a, dt)
@@ -89,7 +89,7 @@ def run_gpu(ctx, case, op, inp, prec=pa.F64):
     elif op == "ad_collide_fc":
         names = ("rho", "ux", "uy", "uz", "tem", "qx", "qy", "qz")
         o = outs(names)
-        snap = outs(["gsnap"], (NC[k], N))[0]
+        snap = outs(["gsnap"], (NC[case.kind_b], N))[0]
         check(L.lbm_ad_macro_brinkman_collide_force_convection(p.h, *o[:4], d("alpha"), D(0.1), q.h, *o[4:],
                                                                d("diff"), 1, snap, None))
         for n in names + ("gsnap",):
@@ -97,7 +97,7 @@ def run_gpu(ctx, case, op, inp, prec=pa.F64):
     elif op == "ad_collide_nc":
         names = ("rho", "ux", "uy", "uz", "tem", "qx", "qy", "qz")
         o = outs(names)
-        snap = outs(["gsnap"], (NC[k], N))[0]
+        snap = outs(["gsnap"], (NC[case.kind_b], N))[0]
         check(L.lbm_ad_macro_brinkman_collide_natural_convection(p.h, *o[:4], d("alpha"), D(0.1), q.h, *o[4:],
                                                                  d("diff"), *[D(x) for x in NC_G], D(NC_TEM0), 1,
                                                                  snap))
@@ -120,7 +120,7 @@ def run_gpu(ctx, case, op, inp, prec=pa.F64):
     elif op == "aad_collide_fc":
         names = ("ip", "iux", "iuy", "iuz", "imx", "imy", "imz", "item", "iqx", "iqy", "iqz")
         o = outs(names)
-        snap = outs(["igsnap"], (NC[k], N))[0]
+        snap = outs(["igsnap"], (NC[case.kind_b], N))[0]
         check(L.lbm_aad_macro_brinkman_collide_force_convection(p.h, d("rho"), d("ux"), d("uy"), uzp, *o[:7],
                                                                d("alpha"), D(0.1), q.h, d("tem"), *o[7:], d("diff"), 1,
                                                                snap))
@@ -129,7 +129,7 @@ def run_gpu(ctx, case, op, inp, prec=pa.F64):
     elif op == "aad_collide_nc":
         names = ("ip", "iux", "iuy", "iuz", "imx", "imy", "imz", "item", "iqx", "iqy", "iqz")
         o = outs(names)
-        snap = outs(["igsnap"], (NC[k], N))[0]
+        snap = outs(["igsnap"], (NC[case.kind_b], N))[0]
         check(L.lbm_aad_macro_brinkman_collide_natural_convection(p.h, d("rho"), d("ux"), d("uy"), uzp, *o[:7],
                                                                   d("alpha"), D(0.1), q.h, d("tem"), *o[7:],
                                                                   d("diff"), *[D(x) for x in NC_G], 1, snap))

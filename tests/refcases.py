@@ -9,7 +9,8 @@ import ctypes
 
 import numpy as np
 
-from _oracle import (D2Q9, D3Q15, NC, ND, Lattice, aos_to_soa, lib, ptr, ref, soa_to_aos, to_face)
+from _oracle import (D2Q9, D3Q7, D3Q15, D3Q27, NC, ND, Lattice, aos_to_soa, lib, ptr, ref, soa_to_aos,
+                     to_face)
 
 D = ctypes.c_double
 
@@ -18,7 +19,9 @@ def _pops(rng, kind, N, scale=0.05):
     """Populations near a rest equilibrium with rho ~ 1 (all positive)."""
     nc = NC[kind]
     w = {9: [4 / 9] + [1 / 9] * 4 + [1 / 36] * 4,
-         15: [2 / 9] + [1 / 9] * 6 + [1 / 72] * 8}[nc]
+         15: [2 / 9] + [1 / 9] * 6 + [1 / 72] * 8,
+         27: [8 / 27] + [2 / 27] * 6 + [1 / 54] * 12 + [1 / 216] * 8,
+         7: [0.05] + [1 / 6] * 6}[nc]  # D3Q7's rest weight is 0; give its population a value anyway
     return np.array(w)[:, None] * (1.0 + scale * rng.uniform(-1, 1, (nc, N)))
 
 
@@ -52,8 +55,9 @@ def _i32(a):
 
 
 class Case:
-    def __init__(self, name, kind, shape, seed, cite):
+    def __init__(self, name, kind, shape, seed, cite, kind_b=None):
         self.name, self.kind, self.seed, self.cite = name, kind, seed, cite
+        self.kind_b = kind if kind_b is None else kind_b  # the advection-diffusion lattice
         self.lx, self.ly, self.lz = shape if len(shape) == 3 else (shape[0], shape[1], 1)
         self.N = self.lx * self.ly * self.lz
 
@@ -74,7 +78,7 @@ class Case:
 # ----------------------------------------------------------------------------------------------
 def make_inputs(case):
     rng = np.random.default_rng(case.seed)
-    inp = {"f": _pops(rng, case.kind, case.N), "g": _pops(rng, case.kind, case.N)}
+    inp = {"f": _pops(rng, case.kind, case.N), "g": _pops(rng, case.kind_b, case.N)}
     inp.update(_fields(rng, case.N))
     inp["type"] = rng.integers(0, 3, case.N).astype(np.int32)
     inp["mask"] = (rng.uniform(size=case.N) < 0.7).astype(np.int32)
@@ -262,7 +266,7 @@ def run_oracle(case, op, inp):
     O = lib()
     k = case.kind
     p = Lattice(k, case.lx, case.ly, case.lz)
-    q = Lattice(k, case.lx, case.ly, case.lz)
+    q = Lattice(case.kind_b, case.lx, case.ly, case.lz)
     p.set(inp["f"])
     q.set(inp["g"])
     F = lambda a, dt=None: case.face(a, dt)
@@ -321,7 +325,7 @@ def run_oracle(case, op, inp):
         out.update(o)
     elif op == "ad_collide_fc":
         o = {kk: np.zeros(case.N) for kk in ("rho", "ux", "uy", "uz", "tem", "qx", "qy", "qz")}
-        snap = np.zeros((NC[k], case.N))
+        snap = np.zeros((NC[case.kind_b], case.N))
         O.ol_ad_collide_fc(p.h, ptr(o["rho"]), ptr(o["ux"]), ptr(o["uy"]), ptr(o["uz"]), ptr(inp["alpha"]), D(0.1),
                            q.h, ptr(o["tem"]), ptr(o["qx"]), ptr(o["qy"]), ptr(o["qz"]), ptr(inp["diff"]), 1,
                            ptr(snap), None)
@@ -329,7 +333,7 @@ def run_oracle(case, op, inp):
         out["gsnap"] = snap
     elif op == "ad_collide_nc":
         o = {kk: np.zeros(case.N) for kk in ("rho", "ux", "uy", "uz", "tem", "qx", "qy", "qz")}
-        snap = np.zeros((NC[k], case.N))
+        snap = np.zeros((NC[case.kind_b], case.N))
         O.ol_ad_collide_nc(p.h, ptr(o["rho"]), ptr(o["ux"]), ptr(o["uy"]), ptr(o["uz"]), ptr(inp["alpha"]), D(0.1),
                            q.h, ptr(o["tem"]), ptr(o["qx"]), ptr(o["qy"]), ptr(o["qz"]), ptr(inp["diff"]),
                            *[D(x) for x in NC_G], D(NC_TEM0), 1, ptr(snap))
@@ -349,7 +353,7 @@ def run_oracle(case, op, inp):
         out.update(o)
     elif op == "aad_collide_fc":
         o = {kk: np.zeros(case.N) for kk in ("ip", "iux", "iuy", "iuz", "imx", "imy", "imz", "item", "iqx", "iqy", "iqz")}
-        snap = np.zeros((NC[k], case.N))
+        snap = np.zeros((NC[case.kind_b], case.N))
         O.ol_aad_collide_fc(p.h, ptr(inp["rho"]), ptr(inp["ux"]), ptr(inp["uy"]), ptr(uz), ptr(o["ip"]), ptr(o["iux"]),
                             ptr(o["iuy"]), ptr(o["iuz"]), ptr(o["imx"]), ptr(o["imy"]), ptr(o["imz"]),
                             ptr(inp["alpha"]), D(0.1), q.h, ptr(inp["tem"]), ptr(o["item"]), ptr(o["iqx"]),
@@ -358,7 +362,7 @@ def run_oracle(case, op, inp):
         out["igsnap"] = snap
     elif op == "aad_collide_nc":
         o = {kk: np.zeros(case.N) for kk in ("ip", "iux", "iuy", "iuz", "imx", "imy", "imz", "item", "iqx", "iqy", "iqz")}
-        snap = np.zeros((NC[k], case.N))
+        snap = np.zeros((NC[case.kind_b], case.N))
         O.ol_aad_collide_nc(p.h, ptr(inp["rho"]), ptr(inp["ux"]), ptr(inp["uy"]), ptr(uz), ptr(o["ip"]), ptr(o["iux"]),
                             ptr(o["iuy"]), ptr(o["iuz"]), ptr(o["imx"]), ptr(o["imy"]), ptr(o["imz"]),
                             ptr(inp["alpha"]), D(0.1), q.h, ptr(inp["tem"]), ptr(o["item"]), ptr(o["iqx"]),
@@ -469,6 +473,12 @@ OPS_3D = {
     "sens_brinkman": "adjointnavierstokes.h:550-554", "sens_brinkman_diffusivity": "adjointadvection.h:1478-1493",
     "residual": "residual.h:39-51", "normalize": "normalize.h:9-24", "density_filter": "densityfilter.h:390-420,512-515",
 }
+
+
+# D3Q27 (no reference counterpart) coupled with D3Q7 advection-diffusion: every 3D function, the
+# expected values from the oracle (tests/test_gpu_d3q27.py), which tests/test_oracle_d3q27.py pins.
+CASES_D3Q27 = [Case("d3q27_5x4x6", D3Q27, (5, 4, 6), 31, "D3Q27", kind_b=D3Q7),
+               Case("d3q27_4x6x5", D3Q27, (4, 6, 5), 32, "D3Q27", kind_b=D3Q7)]
 
 
 def ops_for(case):
