@@ -327,6 +327,20 @@ struct TabD3Q15 {
                                       1.0 / 9.0,  1.0 / 9.0,  1.0 / 72.0, 1.0 / 72.0, 1.0 / 72.0,
                                       1.0 / 72.0, 1.0 / 72.0, 1.0 / 72.0, 1.0 / 72.0, 1.0 / 72.0};
 };
+struct TabD3Q27 {  // no reference counterpart: rest, 6 axis, 12 edges (xy, yz, xz), 8 corners
+    static constexpr int kind = LBM_D3Q27, nc = 27, nd = 3;
+    static constexpr int cx[nc] = {0, 1, 0, 0, -1, 0, 0, 1, -1, 1, -1, 0, 0, 0, 0,
+                                   1, 1, -1, -1, 1, -1, 1, 1, -1, 1, -1, -1};
+    static constexpr int cy[nc] = {0, 0, 1, 0, 0, -1, 0, 1, 1, -1, -1, 1, -1, 1, -1,
+                                   0, 0, 0, 0, 1, 1, -1, 1, -1, -1, 1, -1};
+    static constexpr int cz[nc] = {0, 0, 0, 1, 0, 0, -1, 0, 0, 0, 0, 1, 1, -1, -1,
+                                   1, -1, 1, -1, 1, 1, 1, -1, -1, -1, -1, 1};
+    static constexpr double ei[nc] = {8.0 / 27.0,  2.0 / 27.0,  2.0 / 27.0,  2.0 / 27.0,  2.0 / 27.0,  2.0 / 27.0,
+                                      2.0 / 27.0,  1.0 / 54.0,  1.0 / 54.0,  1.0 / 54.0,  1.0 / 54.0,  1.0 / 54.0,
+                                      1.0 / 54.0,  1.0 / 54.0,  1.0 / 54.0,  1.0 / 54.0,  1.0 / 54.0,  1.0 / 54.0,
+                                      1.0 / 54.0,  1.0 / 216.0, 1.0 / 216.0, 1.0 / 216.0, 1.0 / 216.0, 1.0 / 216.0,
+                                      1.0 / 216.0, 1.0 / 216.0, 1.0 / 216.0};
+};
 struct TabD3Q7 {
     static constexpr int kind = LBM_D3Q7, nc = 7, nd = 3;
     static constexpr int cx[nc] = {0, 1, 0, 0, -1, 0, 0};
@@ -485,6 +499,12 @@ class D3Q15 : public gpu::Lattice<T, gpu::TabD3Q15> {
 public:
     D3Q15(int _lx, int _ly, int _lz, int _PEid = 0, int _mx = 1, int _my = 1, int _mz = 1)
         : gpu::Lattice<T, gpu::TabD3Q15>(_lx, _ly, _lz, _PEid, _mx, _my, _mz) {}
+};
+template <class T>
+class D3Q27 : public gpu::Lattice<T, gpu::TabD3Q27> {  // no reference counterpart: NS with D3Q7 AD
+public:
+    D3Q27(int _lx, int _ly, int _lz, int _PEid = 0, int _mx = 1, int _my = 1, int _mz = 1)
+        : gpu::Lattice<T, gpu::TabD3Q27>(_lx, _ly, _lz, _PEid, _mx, _my, _mz) {}
 };
 template <class T>
 class D3Q7 : public gpu::Lattice<T, gpu::TabD3Q7> {  // no reference counterpart

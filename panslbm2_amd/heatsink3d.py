@@ -14,13 +14,15 @@ set of a duct heat sink:
     (the reference's 3D AAD::iBoundaryConditionSetRho does not compile).
 No SmoothCorner: the corner and edge nodes lie on bounce-back walls, and the fused 3D path runs
 every operator but SmoothCorner, so each time step is one kernel pass.
+The Navier-Stokes lattice is D3Q15 (the reference's) or D3Q27 (ns_kind=D3Q27, no reference
+counterpart; its velocity and pressure boundaries are non-equilibrium bounce-back).
 """
 import ctypes
 import time
 
 import numpy as np
 
-from . import D3Q7, D3Q15, F32, F64, NC, Context, Lattice, check, hptr, to_face
+from . import D3Q7, D3Q15, D3Q27, F32, F64, NC, Context, Lattice, check, hptr, to_face
 from .heatsink2d import Heatsink2D
 
 D = ctypes.c_double
@@ -30,7 +32,8 @@ VP = ctypes.c_void_p
 class Heatsink3D(Heatsink2D):
     def __init__(self, ctx=None, lx=128, ly=128, lz=64, ad_kind=D3Q7, seed=0, nu=0.1, pr=6.0, uin=0.03, heat=1e-3,
                  plate=8, alpha_max=1e4, qf=0.01, qg=1.0, filter_radius=2.0, eps=1e-6, check_every=100,
-                 max_steps=400000, use_graphs=True, prec=F64, fp32_floor=1e-5, fused=True):
+                 max_steps=400000, use_graphs=True, prec=F64, fp32_floor=1e-5, fused=True, ns_kind=D3Q15):
+        assert ns_kind in (D3Q15, D3Q27)
         self.ctx = ctx or Context(0)
         self.L = self.ctx.lib
         self.lx, self.ly, self.lz = lx, ly, lz
@@ -63,7 +66,8 @@ class Heatsink3D(Heatsink2D):
         self.d["ig"] = c.array((NC[ad_kind], N), fill=0.0)
         self.d["one"] = c.array((N,), fill=1.0)
 
-        self.pf = Lattice(c, D3Q15, lx, ly, lz, prec=prec)
+        self.ns_kind = ns_kind
+        self.pf = Lattice(c, ns_kind, lx, ly, lz, prec=prec)
         self.pg = Lattice(c, ad_kind, lx, ly, lz, prec=prec)
 
         wall = (j == 0) | (j == ly - 1) | (k == 0) | (k == lz - 1)

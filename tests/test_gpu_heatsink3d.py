@@ -24,11 +24,14 @@ def _state(h):
     return [h.pf.download(), h.pg.download()] + [d[k].download() for k in names]
 
 
-def test_fused_steps_equal_eager_steps(ctx):
+@pytest.mark.parametrize("ns_kind", ["D3Q15", "D3Q27"])
+def test_fused_steps_equal_eager_steps(ctx, ns_kind):
     import panslbm2_amd as pa
     from panslbm2_amd.heatsink3d import Heatsink3D
 
-    hs = {f: Heatsink3D(ctx, **DIMS, seed=4, use_graphs=False, fused=f, max_steps=100) for f in (True, False)}
+    k = getattr(pa, ns_kind)
+    hs = {f: Heatsink3D(ctx, **DIMS, seed=4, use_graphs=False, fused=f, max_steps=100, ns_kind=k)
+          for f in (True, False)}
     for h in hs.values():
         h.init_forward()
         for t in range(1, 121):
@@ -43,7 +46,8 @@ def test_fused_steps_equal_eager_steps(ctx):
         h.close()
 
 
-def test_mixed_precision_evaluation(ctx):
+@pytest.mark.parametrize("ns_kind", ["D3Q15", "D3Q27"])
+def test_mixed_precision_evaluation(ctx, ns_kind):
     """FP32 storage (switching to FP64 near the end) against FP64, both solved to 1e-9 so that only
     precision separates them: objective 1e-5, dfds 1e-4 of its largest entry."""
     import panslbm2_amd as pa
@@ -51,7 +55,7 @@ def test_mixed_precision_evaluation(ctx):
 
     out = {}
     for prec in (pa.F64, pa.F32):
-        h = Heatsink3D(ctx, **DIMS, seed=4, prec=prec, eps=1e-9, max_steps=200000)
+        h = Heatsink3D(ctx, **DIMS, seed=4, prec=prec, eps=1e-9, max_steps=200000, ns_kind=getattr(pa, ns_kind))
         r = h.evaluate()
         print("3d prec", prec, r["fwd_steps"], r["adj_steps"], r["fp32_steps"], r["objective"], flush=True)
         assert not r["diverged"] and max(r["fwd_residual"]) < 1e-9 and max(r["adj_residual"]) < 1e-9

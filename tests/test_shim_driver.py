@@ -79,6 +79,13 @@ void use3(D3Q15<double> &p, D3Q7<double> &q, double *a) {
     AAD::MacroBrinkmanCollideHeatExchange(p, a, a, a, a, a, a, a, a, a, a, a, a, 0.1, q, a, a, a, a, a, a, 0.05, true);
     AAD::SensitivityHeatExchange(q, a, a, a, a, a, a, a, a, a, a, a);
 }
+void use27(D3Q27<double> &p, D3Q7<double> &q, double *a) {
+    NS::MacroBrinkmanCollide(p, a, a, a, a, 0.1, a, true);
+    AD::MacroBrinkmanCollideForceConvection(p, a, a, a, a, a, 0.1, q, a, a, a, a, a, true);
+    p.Stream();
+    p.iStream();
+    p.SmoothCorner();
+}
 int main() { return 0; }
 """
 
@@ -86,7 +93,7 @@ int main() { return 0; }
 def test_thermal_variants_instantiate(tmp_path):
     """The natural-convection and heat-exchange overloads take the reference's argument lists
     (advection.h:716-810 and :930-1044, adjointadvection.h:894-993, :1113-1230 and :1445-1457):
-    calls written in those orders compile against the header."""
+    calls written in those orders compile against the header; so do the 3D calls on D3Q27."""
     src = tmp_path / "thermal.cpp"
     src.write_text(THERMAL_USE)
     r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Wextra", "-Werror", "-Wno-unused-parameter", "-fsyntax-only",
