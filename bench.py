@@ -17,8 +17,9 @@ Also reported:
   cpu_baseline  -- the reference's own plain C++ (oracle/_ref, OpenMP) running the same forward
                    and adjoint time steps on D2Q9 + D2Q9 (the reference has no D2Q5) on the host
                    cores, a bounded number of steps.
-  fp32          -- the same evaluation with FP32 population storage (mixed precision: FP32 until
-                   the residual stops falling, FP64 to 1e-6), and the FP32 fused step on 4096^2.
+  fp32, fp16    -- the same evaluation with FP32 / FP16 population storage (mixed precision: the
+                   reduced precision until the residual reaches its floor or stops falling, FP64 to
+                   1e-6), and the reduced-precision fused step on 4096^2.
   batch         -- 64 independent designs (grid sizes from {64x128, 128x256, 200x400, 256x512},
                    heat source and inlet velocity drawn per design) evaluated together, one kernel
                    launch per time step for all of them (FP32 mixed, autotuned workgroup shape),
@@ -46,6 +47,7 @@ HBM_PEAK_GBS = 8000.0
 BYTES_FWD_Q9Q5 = 2 * (9 + 5) * 8 + 3 * 8
 BYTES_FWD_Q9Q9 = 2 * (9 + 9) * 8 + 3 * 8
 BYTES_FWD_Q9Q5_F32 = 2 * (9 + 5) * 4 + 3 * 8  # FP32 populations, FP64 coefficient fields
+BYTES_FWD_Q9Q5_F16 = 2 * (9 + 5) * 2 + 3 * 8  # FP16 populations
 # Memory-side traffic of one forward step of the 200x400 workload, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE (KiB per dispatch; profiles/k_collide_fused_heatsink_200x400.txt), FETCH_SIZE scaled by
 # 1.920, the factor that makes it read the known bytes of the pull-stream copy kernel
@@ -124,25 +126,26 @@ def cpu_baseline(hs, target_seconds=15.0):
                       % (nt, nt, secs[0], secs[1])}
 
 
-def fp32_extra(ctx, a, r64):
-    """The benchmark evaluation with FP32 storage (mixed precision), and the FP32 fused step's
-    bandwidth on 4096^2."""
-    from panslbm2_amd import D2Q5, F32
+def fp32_extra(ctx, a, r64, prec=None):
+    """The benchmark evaluation with FP32 (or FP16) storage (mixed precision), and the reduced-
+    precision fused step's bandwidth on 4096^2."""
+    from panslbm2_amd import D2Q5, F16, F32
     from panslbm2_amd.heatsink2d import Heatsink2D
 
+    prec = F32 if prec is None else prec
     hs = Heatsink2D(ctx, lx=a.lx, ly=a.ly, ad_kind=D2Q5, max_steps=a.max_steps, use_graphs=not a.no_graphs,
-                    prec=F32)
+                    prec=prec)
     hs.evaluate()  # captures the graphs
     r = hs.evaluate()
     out = {"mlups": round(hs.N * (r["fwd_steps"] + r["adj_steps"]) / r["seconds"] / 1e6, 1),
            "seconds": round(r["seconds"], 4), "fwd_steps": r["fwd_steps"], "adj_steps": r["adj_steps"],
-           "steps_in_fp32": list(r["fp32_steps"]),
+           "steps_in_reduced_precision": list(r["fp32_steps"]),
            "objective_rel_err_vs_fp64": abs(r["objective"] - r64["objective"]) / abs(r64["objective"]),
            "residuals": [max(r["fwd_residual"]), max(r["adj_residual"])]}
     hs.close()
     if not a.no_big:
-        ms, n = big_grid_roofline(ctx, use_graphs=not a.no_graphs, prec=F32)
-        ach = BYTES_FWD_Q9Q5_F32 * n / (ms * 1e-3) / 1e9
+        ms, n = big_grid_roofline(ctx, use_graphs=not a.no_graphs, prec=prec)
+        ach = (BYTES_FWD_Q9Q5_F32 if prec == F32 else BYTES_FWD_Q9Q5_F16) * n / (ms * 1e-3) / 1e9
         out["roofline_hbm_4096x4096"] = {"achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                          "frac": round(ach / HBM_PEAK_GBS, 4), "us_per_launch": round(ms * 1e3, 1),
                                          "mlups": round(n / (ms * 1e-3) / 1e6, 1)}
@@ -375,7 +378,10 @@ def main():
                                            "us_per_launch": round(ms_big * 1e3, 1),
                                            "mlups": round(nbig / (ms_big * 1e-3) / 1e6, 1)}
     if not a.no_fp32:
+        from panslbm2_amd import F16
+
         extra["fp32"] = fp32_extra(ctx, a, runs[-1])
+        extra["fp16"] = fp32_extra(ctx, a, runs[-1], prec=F16)
     if a.batch > 0:
         extra["batch"] = batch_extra(ctx, a.batch)
     if a.warm_iters > 0:

@@ -63,9 +63,11 @@ extern "C" {
 /* population storage (no reference counterpart: the reference is FP64 throughout).  LBM_F64 is
  * value-identical to the reference.  LBM_F32 stores the populations in FP32, those of a lattice
  * given an NS initial condition as deviations f - w_c from the rest state, and collides them in
- * FP32 on deviations; boundary operators, fields, sensitivities and reductions stay FP64. */
+ * FP32 on deviations; boundary operators, fields, sensitivities and reductions stay FP64.
+ * LBM_F16 stores the same deviations in IEEE binary16 and collides them in FP32. */
 #define LBM_F64 0
 #define LBM_F32 1
+#define LBM_F16 2
 
 /* boundary types of lbm_bc: d2q9.h:19-22 */
 #define LBM_BARRIER 1
@@ -97,7 +99,7 @@ int lbm_event_elapsed_ms(lbm_context *ctx, int event_start, int event_stop, floa
 
 /* ---- lattice: D2Q9<T>(lx, ly) / D3Q15<T>(lx, ly, lz), d2q9.h:28-58, d3q15.h:28-86 ---- */
 int lbm_lattice_create(lbm_context *ctx, int kind, int lx, int ly, int lz, lbm_lattice **out);
-/* the same with a storage precision LBM_F64 / LBM_F32; coupled lattices must share it */
+/* the same with a storage precision LBM_F64 / LBM_F32 / LBM_F16; coupled lattices must share it */
 int lbm_lattice_create_prec(lbm_context *ctx, int kind, int lx, int ly, int lz, int prec, lbm_lattice **out);
 int lbm_lattice_precision(const lbm_lattice *lat);
 /* change the storage precision in place, converting the current populations (e.g. finish a solve

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "libpanslbm2_hip.so")
 D2Q9, D3Q15, D2Q5, D3Q7, D3Q27 = 0, 1, 2, 3, 4
 NC = {D2Q9: 9, D3Q15: 15, D2Q5: 5, D3Q7: 7, D3Q27: 27}
 ND = {D2Q9: 2, D3Q15: 3, D2Q5: 2, D3Q7: 3, D3Q27: 3}
-F64, F32 = 0, 1  # population storage (lbm_lattice_create_prec)
+F64, F32, F16 = 0, 1, 2  # population storage (lbm_lattice_create_prec)
 BARRIER, MIRROR = 1, 2
 SET_T, SET_Q = 1, 2
 

@@ -115,7 +115,7 @@ struct lbm_lattice {
     int rest = 0;
     void *buf[2] = {nullptr, nullptr};
     int cur = 0;
-    size_t esize() const { return prec == LBM_F64 ? sizeof(double) : sizeof(float); }
+    size_t esize() const { return prec == LBM_F64 ? sizeof(double) : (prec == LBM_F32 ? sizeof(float) : 2); }
     lbm::Enc enc() const { return lbm::Enc{prec == LBM_F64 ? 0.0 : double(rest)}; }
     // smoothing lists for the eager path (in-place kernels): edges (3D) then corners
     std::vector<int4> sm_edge, sm_corner;
@@ -708,7 +708,8 @@ int lbm_lattice_create(lbm_context *C, int kind, int lx, int ly, int lz, lbm_lat
 int lbm_lattice_create_prec(lbm_context *C, int kind, int lx, int ly, int lz, int prec, lbm_lattice **out) {
     if (!C || !out) return fail(LBM_ERR_ARG, "null argument");
     *out = nullptr;
-    if (prec != LBM_F64 && prec != LBM_F32) return fail(LBM_ERR_ARG, "unknown storage precision %d", prec);
+    if (prec != LBM_F64 && prec != LBM_F32 && prec != LBM_F16)
+        return fail(LBM_ERR_ARG, "unknown storage precision %d", prec);
     int nc = kind_nc(kind);
     if (!nc) return fail(LBM_ERR_ARG, "unknown lattice kind %d", kind);
     int nd = kind_nd(kind);
@@ -736,7 +737,7 @@ int lbm_lattice_create_prec(lbm_context *C, int kind, int lx, int ly, int lz, in
     }
     if (prec == LBM_F64) {
         HIPCHK(lbm::launch_fill((double *)L->buf[0], (size_t)nc * N, 0.0, C->stream));
-    } else {  // FP32 +0.0 is all-zero bits
+    } else {  // FP32 and FP16 +0.0 are all-zero bits
         HIPCHK(hipMemsetAsync(L->buf[0], 0, L->esize() * nc * N, C->stream));
     }
     build_smooth_lists(L);
@@ -778,7 +779,8 @@ int lbm_lattice_precision(const lbm_lattice *L) { return L ? L->prec : fail(LBM_
 // graphs that hold the old buffers' addresses must be recorded again.
 int lbm_lattice_set_precision(lbm_lattice *L, int prec) {
     if (!L) return fail(LBM_ERR_ARG, "null lattice");
-    if (prec != LBM_F64 && prec != LBM_F32) return fail(LBM_ERR_ARG, "unknown storage precision %d", prec);
+    if (prec != LBM_F64 && prec != LBM_F32 && prec != LBM_F16)
+        return fail(LBM_ERR_ARG, "unknown storage precision %d", prec);
     if (L->ctx->recording) return fail(LBM_ERR_STATE, "precision change while recording a program");
     if (prec == L->prec) return LBM_OK;
     lbm_lattice T = *L;  // the new storage

@@ -372,6 +372,7 @@ static hipError_t dispatch_kind_prec(int kind, int prec, F &&f) {
     auto by_prec = [&](auto l) -> hipError_t {
         if (prec == kF64) return f(l, double{});
         if (prec == kF32) return f(l, float{});
+        if (prec == kF16) return f(l, _Float16{});
         return hipErrorInvalidValue;
     };
     switch (kind) {
@@ -415,6 +416,9 @@ hipError_t launch_program_apply(int ka, int kb, int prec, void *a, void *b, int 
         else if (prec == kF32)
             LBM_LAUNCH((k_program_apply<LA, LB, float>), nblocks(pv.nentry), s, static_cast<float *>(a),
                        static_cast<float *>(b), N, pv, seg, ea, eb);
+        else if (prec == kF16)
+            LBM_LAUNCH((k_program_apply<LA, LB, _Float16>), nblocks(pv.nentry), s, static_cast<_Float16 *>(a),
+                       static_cast<_Float16 *>(b), N, pv, seg, ea, eb);
         else
             return hipErrorInvalidValue;
         return hipGetLastError();
@@ -424,9 +428,10 @@ hipError_t launch_program_apply(int ka, int kb, int prec, void *a, void *b, int 
 // collides: by the dimension of the first lattice (kinds D2Q9/D2Q5 are 2D, D3Q15/D3Q7/D3Q27 3D)
 // and the storage precision
 static bool is_3d_kind(int k) { return k == kD3Q15 || k == kD3Q7 || k == kD3Q27; }
-#define LBM_COLLIDE_DISPATCH(eq, cl, a)                                                            \
-    (cl.prec == kF64 ? (is_3d_kind(cl.ka) ? eq##_collide_nd<3, kF64>(cl, a) : eq##_collide_nd<2, kF64>(cl, a)) \
-                     : (is_3d_kind(cl.ka) ? eq##_collide_nd<3, kF32>(cl, a) : eq##_collide_nd<2, kF32>(cl, a)))
+#define LBM_COLLIDE_DISPATCH(eq, cl, a)                                                                          \
+    (cl.prec == kF64   ? (is_3d_kind(cl.ka) ? eq##_collide_nd<3, kF64>(cl, a) : eq##_collide_nd<2, kF64>(cl, a)) \
+     : cl.prec == kF32 ? (is_3d_kind(cl.ka) ? eq##_collide_nd<3, kF32>(cl, a) : eq##_collide_nd<2, kF32>(cl, a)) \
+                       : (is_3d_kind(cl.ka) ? eq##_collide_nd<3, kF16>(cl, a) : eq##_collide_nd<2, kF16>(cl, a)))
 hipError_t launch_ns_collide(const CollideLaunch &cl, const NSArgs &a) { return LBM_COLLIDE_DISPATCH(ns, cl, a); }
 hipError_t launch_ad_collide(const CollideLaunch &cl, const ADArgs &a) { return LBM_COLLIDE_DISPATCH(ad, cl, a); }
 hipError_t launch_ans_collide(const CollideLaunch &cl, const ANSArgs &a) { return LBM_COLLIDE_DISPATCH(ans, cl, a); }
@@ -454,6 +459,9 @@ hipError_t launch_convert(int kind, int from, const void *src, const Enc &es, in
         else if (to == kF32)
             LBM_LAUNCH((k_convert<L, S, float>), nblocks(N), s, static_cast<const S *>(src), es,
                        static_cast<float *>(dst), et, N);
+        else if (to == kF16)
+            LBM_LAUNCH((k_convert<L, S, _Float16>), nblocks(N), s, static_cast<const S *>(src), es,
+                       static_cast<_Float16 *>(dst), et, N);
         else
             return hipErrorInvalidValue;
         return hipGetLastError();

@@ -332,8 +332,8 @@ __device__ __forceinline__ void node_state(const double *sa, const double *sb, c
     int slot = boundary_slot(i, j, k, g.nx, g.ny, g.nz, LA::nd);
     if (slot >= 0 && slot < pv.nslot) run_slot_ops<LA, LB>(pv, slot, n, a, b);
 }
-template <class LA, class LB>
-__device__ __forceinline__ void node_state(const float *sa, const float *sb, const Grid &g, const ProgView &pv, int n,
+template <class LA, class LB, class S, std::enable_if_t<!std::is_same_v<S, double>, int> = 0>
+__device__ __forceinline__ void node_state(const S *sa, const S *sb, const Grid &g, const ProgView &pv, int n,
                                            float *a, float *b, const Enc &ea, const Enc &eb, const Sh4 &oa,
                                            const Sh4 &ob) {
     int i, j, k;

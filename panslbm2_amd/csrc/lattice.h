@@ -19,8 +19,9 @@ enum Kind : int { kD2Q9 = 0, kD3Q15 = 1, kD2Q5 = 2, kD3Q7 = 3, kD3Q27 = 4 };
 // Population storage.  F64: the reference's arithmetic in FP64, value-identical to it.  F32:
 // populations stored in FP32 as deviations from a shift E_c (Enc below) and collided in FP32 on
 // those deviations (the *_d forms in collide.h), so no FP32 sum carries the rest-state part of a
-// population.
-enum Prec : int { kF64 = 0, kF32 = 1 };
+// population.  F16: the same deviations stored in IEEE binary16 (rounded once per step, to
+// nearest), collided in FP32.
+enum Prec : int { kF64 = 0, kF32 = 1, kF16 = 2 };
 template <int P>
 struct PrecT;
 template <>
@@ -31,6 +32,11 @@ struct PrecT<kF64> {
 template <>
 struct PrecT<kF32> {
     using S = float;
+    using R = float;
+};
+template <>
+struct PrecT<kF16> {
+    using S = _Float16;
     using R = float;
 };
 template <class R>

@@ -27,7 +27,7 @@ import time
 
 import numpy as np
 
-from . import D2Q5, D2Q9, F32, F64, NC, Context, Lattice, check, hptr, to_face
+from . import D2Q5, D2Q9, F16, F32, F64, NC, Context, Lattice, check, hptr, to_face
 
 D = ctypes.c_double
 VP = ctypes.c_void_p
@@ -36,7 +36,7 @@ VP = ctypes.c_void_p
 class Heatsink2D:
     def __init__(self, ctx=None, lx=200, ly=400, ad_kind=D2Q5, seed=0, nu=0.1, pr=6.0, umax=0.05, heat=1e-3,
                  alpha_max=1e4, qf=0.01, qg=1.0, filter_radius=3.0, eps=1e-6, check_every=100, max_steps=400000,
-                 use_graphs=True, prec=F64, fp32_floor=1e-5):
+                 use_graphs=True, prec=F64, fp32_floor=None):
         self.ctx = ctx or Context(0)
         self.L = self.ctx.lib
         self.lx, self.ly, self.N = lx, ly, lx * ly
@@ -78,10 +78,12 @@ class Heatsink2D:
         self.d["one"] = c.array((N,), fill=1.0)
         self.d["zero"] = c.array((N,), fill=0.0)
 
-        # prec: population storage of both lattices.  F32: each solve runs in FP32 until its residual
-        # stops falling, then converts the lattices to FP64 and finishes at eps (mixed precision).
+        # prec: population storage of both lattices.  F32 / F16: each solve runs in the reduced
+        # precision until its residual reaches fp32_floor or stops falling, then converts the
+        # lattices to FP64 and finishes at eps (mixed precision).  Default floors: 1e-5 (F32),
+        # 1e-3 (F16).
         self.prec = prec
-        self.fp32_floor = fp32_floor
+        self.fp32_floor = fp32_floor if fp32_floor is not None else (1e-3 if prec == F16 else 1e-5)
         self.pf = Lattice(c, D2Q9, lx, ly, prec=prec)
         self.pg = Lattice(c, ad_kind, lx, ly, prec=prec)
 
@@ -231,13 +233,13 @@ class Heatsink2D:
             worst = max(r)
             if worst < self.eps or not np.isfinite(worst):  # converged, or diverged: stop either way
                 break
-            if self.pf.prec == F32:
+            if self.pf.prec != F64:
                 t32 = t
                 stall = 0 if worst < 0.95 * best else stall + 1
                 best = min(best, worst)
                 if worst < self.fp32_floor or (stall >= 10 and worst < 100.0 * self.fp32_floor):
                     self._set_prec(F64)
-        return t, (t32 if self.prec == F32 else 0), r
+        return t, (t32 if self.prec != F64 else 0), r
 
     def init_forward(self):
         """Cold start of the forward solve: rest state, T = 0."""

@@ -22,7 +22,7 @@ import time
 
 import numpy as np
 
-from . import D3Q7, D3Q15, D3Q27, F32, F64, NC, Context, Lattice, check, hptr, to_face
+from . import D3Q7, D3Q15, D3Q27, F16, F32, F64, NC, Context, Lattice, check, hptr, to_face
 from .heatsink2d import Heatsink2D
 
 D = ctypes.c_double
@@ -32,7 +32,7 @@ VP = ctypes.c_void_p
 class Heatsink3D(Heatsink2D):
     def __init__(self, ctx=None, lx=128, ly=128, lz=64, ad_kind=D3Q7, seed=0, nu=0.1, pr=6.0, uin=0.03, heat=1e-3,
                  plate=8, alpha_max=1e4, qf=0.01, qg=1.0, filter_radius=2.0, eps=1e-6, check_every=100,
-                 max_steps=400000, use_graphs=True, prec=F64, fp32_floor=1e-5, fused=True, ns_kind=D3Q15):
+                 max_steps=400000, use_graphs=True, prec=F64, fp32_floor=None, fused=True, ns_kind=D3Q15):
         assert ns_kind in (D3Q15, D3Q27)
         self.ctx = ctx or Context(0)
         self.L = self.ctx.lib
@@ -42,7 +42,8 @@ class Heatsink3D(Heatsink2D):
         self.nu, self.eps, self.check_every, self.max_steps = nu, eps, check_every, max_steps
         assert check_every % 2 == 0
         self.use_graphs, self.fused = use_graphs and fused, fused
-        self.prec, self.fp32_floor = prec, fp32_floor
+        self.prec = prec
+        self.fp32_floor = fp32_floor if fp32_floor is not None else (1e-3 if prec == F16 else 1e-5)
         c = self.ctx
         n = np.arange(N)
         i, j, k = n % lx, (n // lx) % ly, n // (lx * ly)
