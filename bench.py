@@ -25,7 +25,8 @@ Also reported:
                    launch per time step for all of them (FP32 mixed, autotuned workgroup shape),
                    against evaluating them one after another in plain FP64.
   warm_start    -- a 3D design loop (D3Q15 + D3Q7 heat sink, 96 x 96 x 48, solid fraction 0.4,
-                   projected-gradient updates) run four ways: cold start in FP64 (the plain GPU
+                   projected-gradient updates: the MMA loop's designs make the adjoint diverge,
+                   from the cold start too, after ~6 iterations at this alpha_max) run four ways: cold start in FP64 (the plain GPU
                    form), previous-solution warm start, learned warm start (U-Net on the matrix
                    cores predicting the change of the fields, trained beside the solves), and the
                    learned warm start with FP32 mixed-precision storage; wall seconds and LBM steps
@@ -210,7 +211,7 @@ def warm_extra(ctx, iters, train_steps=20):
         if learn is not None:
             hs.attach_warm_start(seed=0, steps=train_steps, learn=learn)
         t0 = time.perf_counter()
-        hist = hs.optimize(iters, warm=(False if learn is None else None))
+        hist = hs.optimize(iters, warm=(False if learn is None else None), optimizer="pg")
         total = time.perf_counter() - t0
         later = hist[1:]
         out[mode] = {"s_per_iteration": round(float(np.mean([r["seconds"] + r["train_seconds"] for r in later])), 4),

@@ -333,6 +333,23 @@ int lbm_graph_end(lbm_context *ctx, lbm_graph **out);
 int lbm_graph_launch(lbm_graph *graph);
 int lbm_graph_destroy(lbm_graph *graph);
 
+/* ---- MMA optimiser (src/utility/mma.h) with the design vectors in device memory.  n-long
+ * arrays (x, dfdx, dgdx, xmin, xmax) are device FP64 (xmin / xmax may also be host arrays);
+ * a, c, d and g are host arrays of m values; dgdx is m rows of n.  1 <= m <= 4, n > m. ---- */
+typedef struct lbm_mma lbm_mma;
+/* MMA<T>::MMA, mma.h:63-85 (move 0.5, albefa 0.1, asyinit 0.5, asydecr 0.7, asyincr 1.2, raa0 1e-5) */
+int lbm_mma_create(lbm_context *ctx, int n, int m, double a0, const double *a, const double *c, const double *d,
+                   const double *xmin, const double *xmax, lbm_mma **out);
+int lbm_mma_destroy(lbm_mma *opt);
+/* the public parameters of mma.h:94-105 by name: move, albefa, asyinit, asydecr, asyincr, raa0, epsvalue */
+int lbm_mma_set(lbm_mma *opt, const char *name, double value);
+/* UpdateVariables, mma.h:196-548: x is replaced by the new design; iterations (may be null) gets
+ * the interior-point iterations used (at most 500; the reference has no cap) */
+int lbm_mma_update(lbm_mma *opt, double *x, double f, const double *dfdx, const double *g, const double *dgdx,
+                   int *iterations);
+/* IsConvergence, mma.h:90-92 */
+int lbm_mma_converged(const lbm_mma *opt, double f, int *converged);
+
 /* ---- Learned warm start (no reference counterpart: the reference starts every steady solve
  * from the InitialCondition of uniform fields, e.g. production/heatsink3D.cpp:97-300).  A
  * 3-level 3D U-Net (channels 16/32/64, 3x3x3 convolutions, bfloat16 weights and activations,

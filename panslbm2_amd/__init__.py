@@ -41,6 +41,11 @@ def load():
     L = ctypes.CDLL(LIB_PATH)
     L.lbm_last_error.restype = ctypes.c_char_p
     sig = {
+        "lbm_mma_create": [VP, I, I, D, VP, VP, VP, VP, VP, ctypes.POINTER(VP)],
+        "lbm_mma_destroy": [VP],
+        "lbm_mma_set": [VP, ctypes.c_char_p, D],
+        "lbm_mma_update": [VP, VP, D, VP, VP, VP, ctypes.POINTER(I)],
+        "lbm_mma_converged": [VP, D, ctypes.POINTER(I)],
         "lbm_unet_create": [VP, I, I, I, I, I, I, ctypes.c_ulonglong, ctypes.POINTER(VP)],
         "lbm_unet_destroy": [VP],
         "lbm_unet_num_params": [VP, ctypes.POINTER(SZ)],

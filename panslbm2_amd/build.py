@@ -17,7 +17,7 @@ LIB = os.path.join(HERE, "libpanslbm2_hip.so")
 BUILD = os.path.join(HERE, "_build")
 # heaviest first: the 3D coupled collides, then the rest, so the parallel build ends together
 SOURCES = ["collide_%s%s%s.hip" % (eq, nd, p) for nd in ("3", "2") for eq in ("ad", "aad", "ns", "ans")
-           for p in ("", "_f32", "_f16")] + ["device.hip", "capi.hip", "unet.hip"]
+           for p in ("", "_f32", "_f16")] + ["device.hip", "capi.hip", "unet.hip", "mma.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -ffp-contract=off: no FMA contraction, so FP64 kernels reproduce the reference's plain
 # x86-64 arithmetic.  Packed FP32 instructions are disabled for all device code.

@@ -111,11 +111,16 @@ class Heatsink3D(Heatsink2D):
         d["diff"].upload(dso + (dfl - dso) * s * (1.0 + qg) / (s + qg))
         d["dkds"].upload((dfl - dso) * qg * (1.0 + qg) / (s + qg) ** 2)
 
-    def design_update(self, vf=0.4, move=0.05):
-        """One projected-gradient design step (a stand-in for the reference's MMA update, mma.h:196,
-        which is not built): x <- clip(x - move dF/ds / max|dF/ds| + lam, x - move, x + move, 0, 1)
-        over the nodes above the plate, with lam found by bisection so that the solid fraction
-        mean(1 - x) there equals vf."""
+    def design_update(self, vf=0.4, move=0.05, optimizer="mma"):
+        """One design step over the nodes above the plate.  optimizer="mma": the device MMA
+        optimiser (panslbm2_amd/optimizer.py) as the reference's driver uses it (heatsink3D.cpp:78-85,
+        :120-139, :282): one constraint g = mean(1 - x) / vf - 1 <= 0 (solid fraction at most vf),
+        a0 = 1, a = 0, c = 1e4, d = 0, the move limit, and the sensitivity normalised to a largest
+        magnitude of 1 (normalize.h:9-24).  optimizer="pg": a projected-gradient step
+        x <- clip(x - move dF/ds / max|dF/ds| + lam, x - move, x + move, 0, 1) with lam found by
+        bisection so that the solid fraction equals vf."""
+        if optimizer == "mma":
+            return self._mma_update(vf, move)
         dfds = self.d["dfds"].download()
         region = self.k >= self.plate
         g = dfds[region]
@@ -132,6 +137,28 @@ class Heatsink3D(Heatsink2D):
                 lo = mid
         xn = self.x.copy()
         xn[region] = trial(0.5 * (lo + hi))
+        self.set_design(xn)
+
+    def _mma_update(self, vf, move):
+        from .optimizer import MMA
+
+        region = np.nonzero(self.k >= self.plate)[0]
+        n = len(region)
+        c = self.ctx
+        if getattr(self, "_mma", None) is None:
+            self._mma = MMA(c, n, 1, 1.0, [0.0], [1e4], [0.0], np.zeros(n), np.ones(n), move=move)
+            self._mx = c.array((n,))
+            self._mdf = c.array((n,))
+            self._mdg = c.array((1, n), data=np.full(n, -1.0 / (vf * n)))
+        dfds = self.d["dfds"].download()[region]
+        dfds = dfds / max(np.abs(dfds).max(), 1e-300)
+        x = self.x[region]
+        g = (1.0 - x).sum() / (vf * n) - 1.0
+        self._mx.upload(x)
+        self._mdf.upload(dfds)
+        self.mma_iterations = self._mma.update(self._mx, self.objective, self._mdf, [g], self._mdg)
+        xn = self.x.copy()
+        xn[region] = self._mx.download()
         self.set_design(xn)
 
     # ---- boundary calls of one step (recorded into a program, or run eagerly) ----------------------
@@ -363,16 +390,23 @@ class Heatsink3D(Heatsink2D):
 
     def close(self):
         super().close()
+        if getattr(self, "_mma", None) is not None:
+            self._mma.close()
+            self._mma = None
         ws = getattr(self, "warm", None)
         if ws is not None:
             ws.close()
             self.warm = None
 
-    def optimize(self, iterations, vf=0.4, move=0.05, warm=None, x0=None, callback=None):
+    def optimize(self, iterations, vf=0.4, move=0.05, warm=None, x0=None, callback=None, optimizer="mma"):
         """Design loop from a uniform solid fraction vf above the plate (or x0): evaluate, then
         design_update.  Returns the per-iteration evaluate() records."""
         x = np.where(self.k >= self.plate, 1.0 - vf, 0.0) if x0 is None else x0
         self.set_design(x)
+        old = getattr(self, "_mma", None)
+        if old is not None:
+            old.close()
+        self._mma = None
         hist = []
         for it in range(iterations):
             r = self.evaluate(warm=warm)
@@ -381,7 +415,7 @@ class Heatsink3D(Heatsink2D):
             if callback is not None:
                 callback(r)
             if it + 1 < iterations:
-                self.design_update(vf, move)
+                self.design_update(vf, move, optimizer)
         ws = getattr(self, "warm", None)
         if ws is not None:
             ws.net.synchronize()

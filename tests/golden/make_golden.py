@@ -50,6 +50,44 @@ def main(only=None):
         print("wrote", path, os.path.getsize(path), "bytes")
     if not only:
         write_driver_golden()
+        write_mma_golden()
+
+
+def write_mma_golden():
+    """Fixtures of the reference optimiser (src/utility/mma.h) on the problems of tests/mmacases.py."""
+    import ctypes
+
+    import mmacases
+
+    R = ref()
+    VP, D, I = ctypes.c_void_p, ctypes.c_double, ctypes.c_int
+    R.refmma_create.restype = VP
+    R.refmma_create.argtypes = [I, I, D, VP, VP, VP, VP, VP, D]
+    R.refmma_update.argtypes = [VP, I, I, VP, D, VP, VP, VP]
+    R.refmma_converged.argtypes = [VP, D]
+    R.refmma_destroy.argtypes = [VP]
+    p = lambda a: np.ascontiguousarray(a, dtype=np.float64).ctypes.data_as(VP)
+    for pr in mmacases.PROBLEMS:
+        n, m = pr["n"], pr["m"]
+        a, c, d = (np.array(pr[k], dtype=np.float64) for k in ("a", "c", "d"))
+        lo, hi = mmacases.xbounds(pr)
+        h = R.refmma_create(n, m, D(pr["a0"]), p(a), p(c), p(d), p(lo), p(hi), D(pr["move"]))
+        x = mmacases.x0(pr)
+        blob = {}
+        for k in range(pr["steps"]):
+            f, dfdx, g, dgdx = mmacases.step_inputs(pr, k, x)
+            blob.update({"x_in/%d" % k: x.copy(), "f/%d" % k: np.array(f), "dfdx/%d" % k: dfdx, "g/%d" % k: g,
+                         "dgdx/%d" % k: dgdx})
+            conv = R.refmma_converged(h, D(f))
+            xs = np.ascontiguousarray(x, dtype=np.float64)
+            R.refmma_update(h, n, m, xs.ctypes.data_as(VP), D(f), p(dfdx), p(g), p(dgdx))
+            blob["x_out/%d" % k] = xs.copy()
+            blob["converged/%d" % k] = np.array(conv)
+            x = xs
+        R.refmma_destroy(h)
+        path = os.path.join(HERE, pr["name"] + ".npz")
+        np.savez_compressed(path, **blob)
+        print("wrote", path, os.path.getsize(path), "bytes")
 
 
 def write_driver_golden():
@@ -66,5 +104,7 @@ def write_driver_golden():
 if __name__ == "__main__":
     if sys.argv[1:] == ["driver"]:
         write_driver_golden()
+    elif sys.argv[1:] == ["mma"]:
+        write_mma_golden()
     else:
         main(sys.argv[1:] or None)

