@@ -239,6 +239,13 @@ int lbm_normalize(lbm_context *ctx, double *v, int n);              /* Normalize
  * (production/heatsink.cpp:215-226) */
 int lbm_gather_sum(lbm_context *ctx, const double *v, const int *idx_host, int count, double *out_host);
 /* DensityFilter::GetFilteredValue with the default hat weight, densityfilter.h:390-420,512-515 */
+/* HeavisideFilter (heavisidefilter.h, default hat weights): the density-filtered design projected
+ * as 0.5 (tanh(beta/2) + tanh(beta (s~ - 1/2))) / tanh(beta/2) (GetFilteredVariable :405-585), and
+ * the sensitivity through it (GetFilteredSensitivity :587-899); device arrays of lx*ly*lz */
+int lbm_heaviside_filter(lbm_context *ctx, int lx, int ly, int lz, double R, double beta, const double *v,
+                         double *out);
+int lbm_heaviside_sensitivity(lbm_context *ctx, int lx, int ly, int lz, double R, double beta, const double *v,
+                              const double *dfdrho, double *dfds);
 int lbm_density_filter(lbm_context *ctx, int lx, int ly, int lz, double R, const double *v, double *out);
 
 /* ---- recording and fused steps ---- */

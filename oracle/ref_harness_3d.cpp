@@ -23,6 +23,7 @@
 #include "src/utility/residual.h"
 #include "src/utility/normalize.h"
 #include "src/utility/densityfilter.h"
+#include "src/utility/heavisidefilter.h"
 
 using namespace PANSLBM2;
 
@@ -267,6 +268,21 @@ void ref3_density_filter(int lx, int ly, int lz, double R, const double *v, doub
                  std::vector<double> r = DensityFilter::GetFilteredValue(p, R, in);
                  std::memcpy(out, r.data(), sizeof(double) * p.nxyz);
              }
+}
+
+/* HeavisideFilter with the default hat weights (heavisidefilter.h:582-585, :897-899) */
+void ref3_heaviside_value(int lx, int ly, int lz, double R, double beta, const double *v, double *out) {
+    D3Q15<double> p(lx, ly, lz);
+    std::vector<double> in(v, v + p.nxyz);
+    std::vector<double> r = HeavisideFilter::GetFilteredVariable(p, R, beta, in);
+    std::memcpy(out, r.data(), sizeof(double) * p.nxyz);
+}
+void ref3_heaviside_sensitivity(int lx, int ly, int lz, double R, double beta, const double *v, const double *dfdrho,
+                                double *out) {
+    D3Q15<double> p(lx, ly, lz);
+    std::vector<double> in(v, v + p.nxyz), d(dfdrho, dfdrho + p.nxyz);
+    std::vector<double> r = HeavisideFilter::GetFilteredSensitivity(p, R, beta, in, d);
+    std::memcpy(out, r.data(), sizeof(double) * p.nxyz);
 }
 
 }  // extern "C"

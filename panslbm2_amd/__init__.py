@@ -41,6 +41,8 @@ def load():
     L = ctypes.CDLL(LIB_PATH)
     L.lbm_last_error.restype = ctypes.c_char_p
     sig = {
+        "lbm_heaviside_filter": [VP, I, I, I, D, D, VP, VP],
+        "lbm_heaviside_sensitivity": [VP, I, I, I, D, D, VP, VP, VP],
         "lbm_mma_create": [VP, I, I, D, VP, VP, VP, VP, VP, ctypes.POINTER(VP)],
         "lbm_mma_destroy": [VP],
         "lbm_mma_set": [VP, ctypes.c_char_p, D],

@@ -96,6 +96,7 @@ struct lbm_context {
     double *h_out = nullptr;   // pinned
     DevBuf scratch;            // index lists for gather / filter weights
     DevBuf multi;              // batched residual items and partials
+    DevBuf filt;               // filter intermediates
 };
 
 // Collected fused steps of many instances, launched as one kernel (lbm_batch_*).
@@ -647,6 +648,8 @@ int lbm_context_destroy(lbm_context *C) {
     (void)hipFree(C->d_out);
     (void)hipHostFree(C->h_out);
     C->scratch.release();
+    C->multi.release();
+    C->filt.release();
     (void)hipStreamDestroy(C->stream);
     delete C;
     return LBM_OK;
@@ -1307,11 +1310,9 @@ int lbm_gather_sum(lbm_context *C, const double *v, const int *idx, int count, d
     *out = C->h_out[0];
     return LBM_OK;
 }
-int lbm_density_filter(lbm_context *C, int lx, int ly, int lz, double R, const double *v, double *out) {
-    if (!C || !v || !out || R <= 0.0 || lx < 1 || ly < 1 || lz < 1) return fail(LBM_ERR_ARG, "bad arguments");
-    int nR = (int)R, W = 2 * nR + 1, nd = lz > 1 ? 3 : 2;
-    // hat weights, densityfilter.h:403-406,512-515 (same libm calls as the reference); -1 marks
-    // offsets outside the radius
+// hat weights (R - d) / R within the radius, -1 outside, on the device (C->scratch)
+static int filter_weights(lbm_context *C, double R, int *nR_out) {
+    int nR = (int)R, W = 2 * nR + 1;
     std::vector<double> w((size_t)W * W * W);
     for (int dk = -nR; dk <= nR; ++dk)
         for (int dj = -nR; dj <= nR; ++dj)
@@ -1323,6 +1324,37 @@ int lbm_density_filter(lbm_context *C, int lx, int ly, int lz, double R, const d
     if (C->scratch.ensure(sizeof(double) * w.size())) return fail(LBM_ERR_HIP, "scratch allocation failed");
     HIPCHK(hipStreamSynchronize(C->stream));
     HIPCHK(hipMemcpy(C->scratch.p, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice));
+    *nR_out = nR;
+    return LBM_OK;
+}
+// HeavisideFilter::GetFilteredVariable with the default weights (heavisidefilter.h:405-585)
+int lbm_heaviside_filter(lbm_context *C, int lx, int ly, int lz, double R, double beta, const double *v, double *out) {
+    if (!C || !v || !out || R <= 0.0 || !(beta > 0.0) || lx < 1 || ly < 1 || lz < 1) return fail(LBM_ERR_ARG, "bad arguments");
+    int nR, rc;
+    if ((rc = filter_weights(C, R, &nR)) != LBM_OK) return rc;
+    HIPCHK(lbm::launch_heaviside(v, out, lx, ly, lz, nR, (const double *)C->scratch.p, beta, std::tanh(0.5 * beta),
+                                 C->stream));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    return LBM_OK;
+}
+// HeavisideFilter::GetFilteredSensitivity with the default weights (heavisidefilter.h:587-899)
+int lbm_heaviside_sensitivity(lbm_context *C, int lx, int ly, int lz, double R, double beta, const double *v,
+                              const double *dfdrho, double *dfds) {
+    if (!C || !v || !dfdrho || !dfds || R <= 0.0 || !(beta > 0.0) || lx < 1 || ly < 1 || lz < 1)
+        return fail(LBM_ERR_ARG, "bad arguments");
+    int nR, rc;
+    if ((rc = filter_weights(C, R, &nR)) != LBM_OK) return rc;
+    if (C->filt.ensure(sizeof(double) * (size_t)lx * ly * lz)) return fail(LBM_ERR_HIP, "scratch allocation failed");
+    HIPCHK(lbm::launch_heaviside_sens(v, dfdrho, (double *)C->filt.p, dfds, lx, ly, lz, nR,
+                                      (const double *)C->scratch.p, beta, std::tanh(0.5 * beta), C->stream));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    return LBM_OK;
+}
+int lbm_density_filter(lbm_context *C, int lx, int ly, int lz, double R, const double *v, double *out) {
+    if (!C || !v || !out || R <= 0.0 || lx < 1 || ly < 1 || lz < 1) return fail(LBM_ERR_ARG, "bad arguments");
+    // hat weights, densityfilter.h:403-406,512-515 (same libm calls as the reference)
+    int nR, rc, nd = lz > 1 ? 3 : 2;
+    if ((rc = filter_weights(C, R, &nR)) != LBM_OK) return rc;
     HIPCHK(lbm::launch_density_filter(v, out, lx, ly, lz, nR, nd, (const double *)C->scratch.p, C->stream));
     HIPCHK(hipStreamSynchronize(C->stream));
     return LBM_OK;

@@ -32,7 +32,8 @@ VP = ctypes.c_void_p
 class Heatsink3D(Heatsink2D):
     def __init__(self, ctx=None, lx=128, ly=128, lz=64, ad_kind=D3Q7, seed=0, nu=0.1, pr=6.0, uin=0.03, heat=1e-3,
                  plate=8, alpha_max=1e4, qf=0.01, qg=1.0, filter_radius=2.0, eps=1e-6, check_every=100,
-                 max_steps=400000, use_graphs=True, prec=F64, fp32_floor=None, fused=True, ns_kind=D3Q15):
+                 max_steps=400000, use_graphs=True, prec=F64, fp32_floor=None, fused=True, ns_kind=D3Q15,
+                 heaviside_beta=None):
         assert ns_kind in (D3Q15, D3Q27)
         self.ctx = ctx or Context(0)
         self.L = self.ctx.lib
@@ -50,6 +51,7 @@ class Heatsink3D(Heatsink2D):
         self.i, self.j, self.k = i, j, k
 
         self.plate, self.filter_radius = plate, filter_radius
+        self.heaviside_beta = heaviside_beta
         self.alpha_max, self.qf, self.qg = alpha_max, qf, qg
         self.dfl = nu / pr
         self.dso = 10.0 * self.dfl
@@ -93,12 +95,18 @@ class Heatsink3D(Heatsink2D):
     # ---- design -----------------------------------------------------------------------------------
     def set_design(self, x):
         """Design variables x (fluid fraction before filtering, per node) -> gamma = density-filtered
-        x with the base plate solid -> Brinkman and diffusivity coefficients (heatsink.cpp:104-117
-        interpolation; 3D duct geometry)."""
+        x (or, with heaviside_beta, its Heaviside projection, heatsink3D.cpp:103) with the base plate
+        solid -> Brinkman and diffusivity coefficients (heatsink.cpp:104-117 interpolation; 3D duct
+        geometry)."""
         c, lx, ly, lz, d = self.ctx, self.lx, self.ly, self.lz, self.d
         self.x = np.clip(np.asarray(x, dtype=np.float64), 0.0, 1.0).copy()
         d["raw"].upload(self.x)
-        check(self.L.lbm_density_filter(c.h, lx, ly, lz, D(self.filter_radius), d["raw"].ptr, d["design"].ptr))
+        if self.heaviside_beta:
+            check(self.L.lbm_heaviside_filter(c.h, lx, ly, lz, D(self.filter_radius), D(self.heaviside_beta),
+                                              d["raw"].ptr, d["design"].ptr))
+        else:
+            check(self.L.lbm_density_filter(c.h, lx, ly, lz, D(self.filter_radius), d["raw"].ptr,
+                                            d["design"].ptr))
         s = d["design"].download()
         s[self.k < self.plate] = 0.0  # the base plate is solid
         d["design"].upload(s)
@@ -150,10 +158,26 @@ class Heatsink3D(Heatsink2D):
             self._mx = c.array((n,))
             self._mdf = c.array((n,))
             self._mdg = c.array((1, n), data=np.full(n, -1.0 / (vf * n)))
-        dfds = self.d["dfds"].download()[region]
-        dfds = dfds / max(np.abs(dfds).max(), 1e-300)
         x = self.x[region]
-        g = (1.0 - x).sum() / (vf * n) - 1.0
+        if self.heaviside_beta:
+            # through the filter (heatsink3D.cpp:249-250): the constraint on the projected design
+            d, lx, ly, lz = self.d, self.lx, self.ly, self.lz
+            R, beta = D(self.filter_radius), D(self.heaviside_beta)
+            if "dfdx" not in d:
+                d["dfdx"] = c.array((self.N,))
+                d["dgdg"] = c.array((self.N,), data=np.where(self.k >= self.plate, -1.0 / (vf * n), 0.0))
+                d["dgdx"] = c.array((self.N,))
+            check(self.L.lbm_heaviside_sensitivity(c.h, lx, ly, lz, R, beta, d["raw"].ptr, d["dfds"].ptr,
+                                                   d["dfdx"].ptr))
+            check(self.L.lbm_heaviside_sensitivity(c.h, lx, ly, lz, R, beta, d["raw"].ptr, d["dgdg"].ptr,
+                                                   d["dgdx"].ptr))
+            dfds = d["dfdx"].download()[region]
+            self._mdg.upload(d["dgdx"].download()[region][None])
+            g = (1.0 - self.gamma[region]).sum() / (vf * n) - 1.0
+        else:
+            dfds = self.d["dfds"].download()[region]
+            g = (1.0 - x).sum() / (vf * n) - 1.0
+        dfds = dfds / max(np.abs(dfds).max(), 1e-300)
         self._mx.upload(x)
         self._mdf.upload(dfds)
         self.mma_iterations = self._mma.update(self._mx, self.objective, self._mdf, [g], self._mdg)

@@ -51,6 +51,7 @@ def main(only=None):
     if not only:
         write_driver_golden()
         write_mma_golden()
+        write_heaviside_golden()
 
 
 def write_mma_golden():
@@ -90,6 +91,38 @@ def write_mma_golden():
         print("wrote", path, os.path.getsize(path), "bytes")
 
 
+HEAVISIDE_CASES = [  # (name, lx, ly, lz, R, beta, seed)
+    ("h2d_12x10", 12, 10, 1, 1.5, 4.0, 41), ("h3d_7x6x5", 7, 6, 5, 2.4, 8.0, 42), ("h3d_6x5x4_r1", 6, 5, 4, 1.0, 16.0, 43),
+]
+
+
+def heaviside_inputs(lx, ly, lz, seed):
+    rng = np.random.default_rng(seed)
+    n = lx * ly * lz
+    return rng.uniform(0.0, 1.0, n), rng.standard_normal(n)
+
+
+def write_heaviside_golden():
+    """HeavisideFilter (src/utility/heavisidefilter.h, default weights) on the cases above."""
+    import ctypes
+
+    R_ = ref()
+    VP, D, I = ctypes.c_void_p, ctypes.c_double, ctypes.c_int
+    R_.ref3_heaviside_value.argtypes = [I, I, I, D, D, VP, VP]
+    R_.ref3_heaviside_sensitivity.argtypes = [I, I, I, D, D, VP, VP, VP]
+    p = lambda a: a.ctypes.data_as(VP)
+    blob = {}
+    for name, lx, ly, lz, R, beta, seed in HEAVISIDE_CASES:
+        s, dfdrho = heaviside_inputs(lx, ly, lz, seed)
+        rho, dfds = np.empty_like(s), np.empty_like(s)
+        R_.ref3_heaviside_value(lx, ly, lz, D(R), D(beta), p(s), p(rho))
+        R_.ref3_heaviside_sensitivity(lx, ly, lz, D(R), D(beta), p(s), p(dfdrho), p(dfds))
+        blob.update({name + "/s": s, name + "/dfdrho": dfdrho, name + "/rho": rho, name + "/dfds": dfds})
+    path = os.path.join(HERE, "heaviside.npz")
+    np.savez_compressed(path, **blob)
+    print("wrote", path, os.path.getsize(path), "bytes")
+
+
 def write_driver_golden():
     drv = os.path.join(os.path.dirname(os.path.dirname(HERE)), "oracle", "_ref", "heatsink_driver_ref")
     if not os.path.exists(drv):
@@ -106,5 +139,7 @@ if __name__ == "__main__":
         write_driver_golden()
     elif sys.argv[1:] == ["mma"]:
         write_mma_golden()
+    elif sys.argv[1:] == ["heaviside"]:
+        write_heaviside_golden()
     else:
         main(sys.argv[1:] or None)
