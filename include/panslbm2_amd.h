@@ -152,6 +152,15 @@ int lbm_ad_macro_brinkman_collide_natural_convection(lbm_lattice *p, double *rho
                                                      double *tem, double *qx, double *qy, double *qz,
                                                      const double *diffusivity, double gx, double gy, double gz,
                                                      double tem0, int issave, double *g);
+/* MacroCollideForceConvection :528-614 and MacroCollideNaturalConvection :616-712: the BGK-only
+ * steps (no Brinkman term, one constant diffusivity, no snapshot); gz is ignored on 2D lattices */
+int lbm_ad_macro_collide_force_convection(lbm_lattice *p, double *rho, double *ux, double *uy, double *uz,
+                                          double viscosity, lbm_lattice *q, double *tem, double *qx, double *qy,
+                                          double *qz, double diffusivity, int issave);
+int lbm_ad_macro_collide_natural_convection(lbm_lattice *p, double *rho, double *ux, double *uy, double *uz,
+                                            double viscosity, lbm_lattice *q, double *tem, double *qx, double *qy,
+                                            double *qz, double diffusivity, double gx, double gy, double gz,
+                                            double tem0, int issave);
 /* MacroBrinkmanCollideHeatExchange :716-810: constant diffusivity, per-node beta; after the Brinkman
  * force g_c += w_c beta (1 - T)/(1 + beta) (ExternalForceHeatExchange :89-95); no snapshot */
 int lbm_ad_macro_brinkman_collide_heat_exchange(lbm_lattice *p, double *rho, double *ux, double *uy, double *uz,
@@ -193,6 +202,16 @@ int lbm_aad_macro_brinkman_collide_natural_convection(
     double *iuy, double *iuz, double *imx, double *imy, double *imz, const double *alpha, double viscosity,
     lbm_lattice *q, const double *tem, double *item, double *iqx, double *iqy, double *iqz, const double *diffusivity,
     double gx, double gy, double gz, int issave, double *ig);
+/* MacroBrinkmanCollideNaturalConvectionMassFlow :1232-1358: the natural-convection step with the
+ * mass-flow objective's source on f before the Brinkman force (ExternalForceMassFlow :134-150:
+ * f_0 += (u.d)/rho, f_c -= ((c - u).d)/rho), d = (dirx, diry, dirz) per node; dirz is ignored on 2D
+ * lattices.  The 3D save writes iuz. */
+int lbm_aad_macro_brinkman_collide_natural_convection_mass_flow(
+    lbm_lattice *p, const double *rho, const double *ux, const double *uy, const double *uz, double *ip, double *iux,
+    double *iuy, double *iuz, double *imx, double *imy, double *imz, const double *alpha, double viscosity,
+    lbm_lattice *q, const double *tem, double *item, double *iqx, double *iqy, double *iqz, const double *diffusivity,
+    double gx, double gy, double gz, const double *dirx, const double *diry, const double *dirz, int issave,
+    double *ig);
 /* MacroBrinkmanCollideHeatExchange :894-993: g_c -= beta (1 + iT)/(1 + beta) after the ANS re-macro
  * (ExternalForceHeatExchange :106-112), then the AAD macro again; no snapshot */
 int lbm_aad_macro_brinkman_collide_heat_exchange(
@@ -278,6 +297,13 @@ int lbm_ad_macro_brinkman_collide_natural_convection_p(lbm_program *prog, lbm_la
                                                        lbm_lattice *q, double *tem, double *qx, double *qy,
                                                        double *qz, const double *diffusivity, double gx, double gy,
                                                        double gz, double tem0, int issave, double *g);
+int lbm_ad_macro_collide_force_convection_p(lbm_program *prog, lbm_lattice *p, double *rho, double *ux, double *uy,
+                                            double *uz, double viscosity, lbm_lattice *q, double *tem, double *qx,
+                                            double *qy, double *qz, double diffusivity, int issave);
+int lbm_ad_macro_collide_natural_convection_p(lbm_program *prog, lbm_lattice *p, double *rho, double *ux,
+                                              double *uy, double *uz, double viscosity, lbm_lattice *q, double *tem,
+                                              double *qx, double *qy, double *qz, double diffusivity, double gx,
+                                              double gy, double gz, double tem0, int issave);
 int lbm_ad_macro_brinkman_collide_heat_exchange_p(lbm_program *prog, lbm_lattice *p, double *rho, double *ux,
                                                   double *uy, double *uz, const double *alpha, double viscosity,
                                                   lbm_lattice *q, double *tem, double *qx, double *qy, double *qz,
@@ -296,6 +322,12 @@ int lbm_aad_macro_brinkman_collide_natural_convection_p(
     double *ip, double *iux, double *iuy, double *iuz, double *imx, double *imy, double *imz, const double *alpha,
     double viscosity, lbm_lattice *q, const double *tem, double *item, double *iqx, double *iqy, double *iqz,
     const double *diffusivity, double gx, double gy, double gz, int issave, double *ig);
+int lbm_aad_macro_brinkman_collide_natural_convection_mass_flow_p(
+    lbm_program *prog, lbm_lattice *p, const double *rho, const double *ux, const double *uy, const double *uz,
+    double *ip, double *iux, double *iuy, double *iuz, double *imx, double *imy, double *imz, const double *alpha,
+    double viscosity, lbm_lattice *q, const double *tem, double *item, double *iqx, double *iqy, double *iqz,
+    const double *diffusivity, double gx, double gy, double gz, const double *dirx, const double *diry,
+    const double *dirz, int issave, double *ig);
 
 /* ---- batches (no reference counterpart; the reference runs one design per process): the fused
  * steps of many independent lattice instances in one kernel launch.  Between lbm_batch_begin and

@@ -704,6 +704,66 @@ void MacroBrinkmanCollideForceConvection(P<T> &_p, T *_rho, T *_ux, T *_uy, T *_
                "AD::MacroBrinkmanCollideForceConvection");
     if (_issave) rt.wrote({_rho, _ux, _uy, _uz, _tem, _qx, _qy, _qz, _g});
 }
+// :528-569
+template <class T, template <class> class P, template <class> class Q>
+void MacroCollideForceConvection(P<T> &_p, T *_rho, T *_ux, T *_uy, T _viscosity, Q<T> &_q, T *_tem, T *_qx,
+                                 T *_qy, T _diffusivity, bool _issave = false) {
+    static_assert(P<T>::nd == 2 && Q<T>::nd == 2, "2D overload");
+    gpu::Runtime &rt = gpu::Runtime::get();
+    size_t n = gpu::N(_p);
+    double *rho = rt.out(_rho, n), *ux = rt.out(_ux, n), *uy = rt.out(_uy, n);
+    double *tem = rt.out(_tem, n), *qx = rt.out(_qx, n), *qy = rt.out(_qy, n);
+    gpu::check(lbm_ad_macro_collide_force_convection_p(rt.take(), _p.handle(), rho, ux, uy, nullptr, _viscosity,
+                                                       _q.handle(), tem, qx, qy, nullptr, _diffusivity,
+                                                       _issave ? 1 : 0),
+               "AD::MacroCollideForceConvection");
+    if (_issave) rt.wrote({_rho, _ux, _uy, _tem, _qx, _qy});
+}
+// :571-614
+template <class T, template <class> class P, template <class> class Q>
+void MacroCollideForceConvection(P<T> &_p, T *_rho, T *_ux, T *_uy, T *_uz, T _viscosity, Q<T> &_q, T *_tem, T *_qx,
+                                 T *_qy, T *_qz, T _diffusivity, bool _issave = false) {
+    static_assert(P<T>::nd == 3 && Q<T>::nd == 3, "3D overload");
+    gpu::Runtime &rt = gpu::Runtime::get();
+    size_t n = gpu::N(_p);
+    double *rho = rt.out(_rho, n), *ux = rt.out(_ux, n), *uy = rt.out(_uy, n), *uz = rt.out(_uz, n);
+    double *tem = rt.out(_tem, n), *qx = rt.out(_qx, n), *qy = rt.out(_qy, n), *qz = rt.out(_qz, n);
+    gpu::check(lbm_ad_macro_collide_force_convection_p(rt.take(), _p.handle(), rho, ux, uy, uz, _viscosity,
+                                                       _q.handle(), tem, qx, qy, qz, _diffusivity, _issave ? 1 : 0),
+               "AD::MacroCollideForceConvection");
+    if (_issave) rt.wrote({_rho, _ux, _uy, _uz, _tem, _qx, _qy, _qz});
+}
+// :616-662
+template <class T, template <class> class P, template <class> class Q>
+void MacroCollideNaturalConvection(P<T> &_p, T *_rho, T *_ux, T *_uy, T _viscosity, Q<T> &_q, T *_tem, T *_qx,
+                                   T *_qy, T _diffusivity, T _gx, T _gy, T _tem0, bool _issave = false) {
+    static_assert(P<T>::nd == 2 && Q<T>::nd == 2, "2D overload");
+    gpu::Runtime &rt = gpu::Runtime::get();
+    size_t n = gpu::N(_p);
+    double *rho = rt.out(_rho, n), *ux = rt.out(_ux, n), *uy = rt.out(_uy, n);
+    double *tem = rt.out(_tem, n), *qx = rt.out(_qx, n), *qy = rt.out(_qy, n);
+    gpu::check(lbm_ad_macro_collide_natural_convection_p(rt.take(), _p.handle(), rho, ux, uy, nullptr, _viscosity,
+                                                         _q.handle(), tem, qx, qy, nullptr, _diffusivity, _gx, _gy,
+                                                         0.0, _tem0, _issave ? 1 : 0),
+               "AD::MacroCollideNaturalConvection");
+    if (_issave) rt.wrote({_rho, _ux, _uy, _tem, _qx, _qy});
+}
+// :664-712
+template <class T, template <class> class P, template <class> class Q>
+void MacroCollideNaturalConvection(P<T> &_p, T *_rho, T *_ux, T *_uy, T *_uz, T _viscosity, Q<T> &_q, T *_tem,
+                                   T *_qx, T *_qy, T *_qz, T _diffusivity, T _gx, T _gy, T _gz, T _tem0,
+                                   bool _issave = false) {
+    static_assert(P<T>::nd == 3 && Q<T>::nd == 3, "3D overload");
+    gpu::Runtime &rt = gpu::Runtime::get();
+    size_t n = gpu::N(_p);
+    double *rho = rt.out(_rho, n), *ux = rt.out(_ux, n), *uy = rt.out(_uy, n), *uz = rt.out(_uz, n);
+    double *tem = rt.out(_tem, n), *qx = rt.out(_qx, n), *qy = rt.out(_qy, n), *qz = rt.out(_qz, n);
+    gpu::check(lbm_ad_macro_collide_natural_convection_p(rt.take(), _p.handle(), rho, ux, uy, uz, _viscosity,
+                                                         _q.handle(), tem, qx, qy, qz, _diffusivity, _gx, _gy, _gz,
+                                                         _tem0, _issave ? 1 : 0),
+               "AD::MacroCollideNaturalConvection");
+    if (_issave) rt.wrote({_rho, _ux, _uy, _uz, _tem, _qx, _qy, _qz});
+}
 // :930-984
 template <class T, template <class> class P, template <class> class Q>
 void MacroBrinkmanCollideNaturalConvection(P<T> &_p, T *_rho, T *_ux, T *_uy, const T *_alpha, T _viscosity,
@@ -1080,6 +1140,54 @@ void MacroBrinkmanCollideNaturalConvection(P<T> &_p, const T *_rho, const T *_ux
                "AAD::MacroBrinkmanCollideNaturalConvection");
     if (_issave) rt.wrote({_ip, _iux, _iuy, _iuz, _imx, _imy, _imz, _item, _iqx, _iqy, _iqz, _ig});
 }
+// :1232-1291
+template <class T, template <class> class P, template <class> class Q>
+void MacroBrinkmanCollideNaturalConvectionMassFlow(P<T> &_p, const T *_rho, const T *_ux, const T *_uy, T *_ip,
+                                                   T *_iux, T *_iuy, T *_imx, T *_imy, const T *_alpha, T _viscosity,
+                                                   Q<T> &_q, const T *_tem, T *_item, T *_iqx, T *_iqy,
+                                                   const T *_diffusivity, T _gx, T _gy, const T *_directionx,
+                                                   const T *_directiony, bool _issave = false, T *_ig = nullptr) {
+    static_assert(P<T>::nd == 2 && Q<T>::nd == 2, "2D overload");
+    gpu::Runtime &rt = gpu::Runtime::get();
+    size_t n = gpu::N(_p);
+    const double *rho = rt.in(_rho, n), *ux = rt.in(_ux, n), *uy = rt.in(_uy, n), *alpha = rt.in(_alpha, n);
+    const double *tem = rt.in(_tem, n), *diff = rt.in(_diffusivity, n);
+    const double *dx = rt.in(_directionx, n), *dy = rt.in(_directiony, n);
+    double *ip = rt.out(_ip, n), *iux = rt.out(_iux, n), *iuy = rt.out(_iuy, n);
+    double *imx = rt.out(_imx, n), *imy = rt.out(_imy, n);
+    double *item = rt.out(_item, n), *iqx = rt.out(_iqx, n), *iqy = rt.out(_iqy, n);
+    double *ig = _issave ? rt.out(_ig, n * Q<T>::nc, Q<T>::nc) : nullptr;
+    gpu::check(lbm_aad_macro_brinkman_collide_natural_convection_mass_flow_p(
+                   rt.take(), _p.handle(), rho, ux, uy, nullptr, ip, iux, iuy, nullptr, imx, imy, nullptr, alpha,
+                   _viscosity, _q.handle(), tem, item, iqx, iqy, nullptr, diff, _gx, _gy, 0.0, dx, dy, nullptr,
+                   _issave ? 1 : 0, ig),
+               "AAD::MacroBrinkmanCollideNaturalConvectionMassFlow");
+    if (_issave) rt.wrote({_ip, _iux, _iuy, _imx, _imy, _item, _iqx, _iqy, _ig});
+}
+// :1293-1358
+template <class T, template <class> class P, template <class> class Q>
+void MacroBrinkmanCollideNaturalConvectionMassFlow(P<T> &_p, const T *_rho, const T *_ux, const T *_uy,
+                                                   const T *_uz, T *_ip, T *_iux, T *_iuy, T *_iuz, T *_imx, T *_imy,
+                                                   T *_imz, const T *_alpha, T _viscosity, Q<T> &_q, const T *_tem,
+                                                   T *_item, T *_iqx, T *_iqy, T *_iqz, const T *_diffusivity, T _gx,
+                                                   T _gy, T _gz, const T *_directionx, const T *_directiony,
+                                                   const T *_directionz, bool _issave = false, T *_ig = nullptr) {
+    static_assert(P<T>::nd == 3 && Q<T>::nd == 3, "3D overload");
+    gpu::Runtime &rt = gpu::Runtime::get();
+    size_t n = gpu::N(_p);
+    const double *rho = rt.in(_rho, n), *ux = rt.in(_ux, n), *uy = rt.in(_uy, n), *uz = rt.in(_uz, n);
+    const double *alpha = rt.in(_alpha, n), *tem = rt.in(_tem, n), *diff = rt.in(_diffusivity, n);
+    const double *dx = rt.in(_directionx, n), *dy = rt.in(_directiony, n), *dz = rt.in(_directionz, n);
+    double *ip = rt.out(_ip, n), *iux = rt.out(_iux, n), *iuy = rt.out(_iuy, n), *iuz = rt.out(_iuz, n);
+    double *imx = rt.out(_imx, n), *imy = rt.out(_imy, n), *imz = rt.out(_imz, n);
+    double *item = rt.out(_item, n), *iqx = rt.out(_iqx, n), *iqy = rt.out(_iqy, n), *iqz = rt.out(_iqz, n);
+    double *ig = _issave ? rt.out(_ig, n * Q<T>::nc, Q<T>::nc) : nullptr;
+    gpu::check(lbm_aad_macro_brinkman_collide_natural_convection_mass_flow_p(
+                   rt.take(), _p.handle(), rho, ux, uy, uz, ip, iux, iuy, iuz, imx, imy, imz, alpha, _viscosity,
+                   _q.handle(), tem, item, iqx, iqy, iqz, diff, _gx, _gy, _gz, dx, dy, dz, _issave ? 1 : 0, ig),
+               "AAD::MacroBrinkmanCollideNaturalConvectionMassFlow");
+    if (_issave) rt.wrote({_ip, _iux, _iuy, _iuz, _imx, _imy, _imz, _item, _iqx, _iqy, _iqz, _ig});
+}
 template <class T, template <class> class Q>
 void InitialCondition(Q<T> &_q, const T *_ux, const T *_uy, const T *_item, const T *_iqx,
                       const T *_iqy) {  // :1359-1368
@@ -1295,3 +1403,5 @@ void SensitivityTemperatureAtHeatSource(Q<T> &_q, T *_dfds, const T *_ux, const 
 }  // namespace AAD
 
 }  // namespace PANSLBM2
+
+#include "panslbm2_amd_vtk.hpp"  // VTKExport, VTKXMLExport, VTKImport (host I/O)

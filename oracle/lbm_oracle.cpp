@@ -1181,6 +1181,44 @@ void ol_ad_collide_nc(ol_lattice *p, double *rho, double *ux, double *uy, double
         store(q, idx, g);
     }
 }
+/* advection.h:528-614 (MacroCollideForceConvection, natural = 0) and advection.h:616-712
+ * (MacroCollideNaturalConvection, natural = 1): no Brinkman term and one constant diffusivity.  The
+ * forced step collides the first macro's moments; the natural step adds the buoyancy force of
+ * ol_ad_collide_nc to f and takes both macros again.  No snapshot. */
+void ol_ad_collide_bgk(ol_lattice *p, double *rho, double *ux, double *uy, double *uz, double nu, ol_lattice *q,
+                       double *tem, double *qx, double *qy, double *qz, double diff, int natural, double gx, double gy,
+                       double gz, double tem0, int issave) {
+    const LatDesc &dp = *p->d, &dq = *q->d;
+    const bool d3 = dp.nd == 3;
+    double omegaf = 1.0 / (3.0 * nu + 0.5), iomegaf = 1.0 - omegaf;
+    double omegag = 1.0 / (3.0 * diff + 0.5), iomegag = 1.0 - omegag;
+#pragma omp parallel for
+    for (int idx = 0; idx < p->N; ++idx) {
+        double f[27], g[27], feq[27], geq[27], r, vx, vy, vz, T, hx, hy, hz;
+        load(p, idx, f);
+        load(q, idx, g);
+        ns_macro(dp, f, r, vx, vy, vz);
+        ad_macro(dq, g, vx, vy, vz, omegag, T, hx, hy, hz);
+        if (natural) {
+            for (int c = 1; c < dp.nc; ++c) {
+                double cg = d3 ? dp.cx[c] * gx + dp.cy[c] * gy + dp.cz[c] * gz : dp.cx[c] * gx + dp.cy[c] * gy;
+                f[c] += 3.0 * dp.w[c] * cg * (T - tem0);
+            }
+            ns_macro(dp, f, r, vx, vy, vz);
+            ad_macro(dq, g, vx, vy, vz, omegag, T, hx, hy, hz);
+        }
+        if (issave) {
+            rho[idx] = r; ux[idx] = vx; uy[idx] = vy; put(uz, idx, vz);
+            tem[idx] = T; qx[idx] = hx; qy[idx] = hy; put(qz, idx, hz);
+        }
+        ns_eq(dp, feq, r, vx, vy, vz);
+        bgk(dp, f, feq, omegaf, iomegaf);
+        ad_eq(dq, geq, T, vx, vy, vz);
+        bgk(dq, g, geq, omegag, iomegag);
+        store(p, idx, f);
+        store(q, idx, g);
+    }
+}
 /* advection.h:716-810 (MacroBrinkmanCollideHeatExchange) with ExternalForceHeatExchange
  * (advection.h:89-95): constant diffusivity, per-node beta; after the Brinkman force and the NS
  * re-macro, g_c += w_c beta (1 - T) / (1 + beta) with T from the first macro, then the AD re-macro.
@@ -1390,6 +1428,54 @@ void ol_aad_collide_nc(ol_lattice *p, const double *rho, const double *ux, const
         ans_macro(dp, f, vx, vy, vz, a, bx, by, bz, mx, my, mz);
         aad_macro(dq, g, it, hx, hy, hz);
         aad_brinkman(dp, f, rho[idx], vx, vy, vz, mx, my, mz, tem[idx], hx, hy, hz, omegag, alpha[idx]);
+        ans_macro(dp, f, vx, vy, vz, a, bx, by, bz, mx, my, mz);
+        double coef = d3 ? 3.0 * (mx * gx + my * gy + mz * gz) : 3.0 * (mx * gx + my * gy);
+        for (int c = 0; c < dq.nc; ++c) g[c] += coef;
+        aad_macro(dq, g, it, hx, hy, hz);
+        if (issave) {
+            ip[idx] = a; iux[idx] = bx; iuy[idx] = by; put(iuz, idx, bz);
+            imx[idx] = mx; imy[idx] = my; put(imz, idx, mz);
+            item[idx] = it; iqx[idx] = hx; iqy[idx] = hy; put(iqz, idx, hz);
+            if (igsnap)
+                for (int c = 0; c < dq.nc; ++c) igsnap[(size_t)c * q->N + idx] = g[c];
+        }
+        ans_eq(dp, feq, vx, vy, vz, a, bx, by, bz);
+        bgk(dp, f, feq, omegaf, iomegaf);
+        aad_eq(dq, geq, it, hx, hy, hz, vx, vy, vz);
+        bgk(dq, g, geq, omegag, iomegag);
+        store(p, idx, f);
+        store(q, idx, g);
+    }
+}
+/* adjointadvection.h:1232-1358 (MacroBrinkmanCollideNaturalConvectionMassFlow): the natural-convection
+ * step of ol_aad_collide_nc with, before the Brinkman force, the mass-flow objective's source
+ * (ExternalForceMassFlow, adjointadvection.h:134-150): with s = (u.d) / rho,
+ * f_0 += s and f_c -= ((c - u).d) / rho for c >= 1, d the per-node direction field. */
+void ol_aad_collide_mf(ol_lattice *p, const double *rho, const double *ux, const double *uy, const double *uz,
+                       double *ip, double *iux, double *iuy, double *iuz, double *imx, double *imy, double *imz,
+                       const double *alpha, double nu, ol_lattice *q, const double *tem, double *item, double *iqx,
+                       double *iqy, double *iqz, const double *diff, double gx, double gy, double gz,
+                       const double *dirx, const double *diry, const double *dirz, int issave, double *igsnap) {
+    const LatDesc &dp = *p->d, &dq = *q->d;
+    const bool d3 = dp.nd == 3;
+    double omegaf = 1.0 / (3.0 * nu + 0.5), iomegaf = 1.0 - omegaf;
+#pragma omp parallel for
+    for (int idx = 0; idx < p->N; ++idx) {
+        double omegag = 1.0 / (3.0 * diff[idx] + 0.5), iomegag = 1.0 - omegag;
+        double f[27], g[27], feq[27], geq[27], a, bx, by, bz, mx, my, mz, it, hx, hy, hz;
+        double vx = ux[idx], vy = uy[idx], vz = fld(uz, idx), r = rho[idx];
+        double ex = dirx[idx], ey = diry[idx], ez = fld(dirz, idx);
+        load(p, idx, f);
+        load(q, idx, g);
+        ans_macro(dp, f, vx, vy, vz, a, bx, by, bz, mx, my, mz);
+        aad_macro(dq, g, it, hx, hy, hz);
+        f[0] += (d3 ? vx * ex + vy * ey + vz * ez : vx * ex + vy * ey) / r;
+        for (int c = 1; c < dp.nc; ++c) {
+            double s = d3 ? (dp.cx[c] - vx) * ex + (dp.cy[c] - vy) * ey + (dp.cz[c] - vz) * ez
+                          : (dp.cx[c] - vx) * ex + (dp.cy[c] - vy) * ey;
+            f[c] -= s / r;
+        }
+        aad_brinkman(dp, f, r, vx, vy, vz, mx, my, mz, tem[idx], hx, hy, hz, omegag, alpha[idx]);
         ans_macro(dp, f, vx, vy, vz, a, bx, by, bz, mx, my, mz);
         double coef = d3 ? 3.0 * (mx * gx + my * gy + mz * gz) : 3.0 * (mx * gx + my * gy);
         for (int c = 0; c < dq.nc; ++c) g[c] += coef;

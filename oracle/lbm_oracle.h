@@ -68,6 +68,9 @@ void ol_ad_collide_fc(ol_lattice *p, double *rho, double *ux, double *uy, double
 void ol_ad_collide_nc(ol_lattice *p, double *rho, double *ux, double *uy, double *uz, const double *alpha, double nu,
                       ol_lattice *q, double *tem, double *qx, double *qy, double *qz, const double *diff, double gx,
                       double gy, double gz, double tem0, int issave, double *gsnap);
+void ol_ad_collide_bgk(ol_lattice *p, double *rho, double *ux, double *uy, double *uz, double nu, ol_lattice *q,
+                       double *tem, double *qx, double *qy, double *qz, double diff, int natural, double gx, double gy,
+                       double gz, double tem0, int issave);
 void ol_ad_collide_he(ol_lattice *p, double *rho, double *ux, double *uy, double *uz, const double *alpha, double nu,
                       ol_lattice *q, double *tem, double *qx, double *qy, double *qz, const double *beta, double diff,
                       int issave);
@@ -104,6 +107,11 @@ void ol_aad_collide_nc(ol_lattice *p, const double *rho, const double *ux, const
                        const double *alpha, double nu, ol_lattice *q, const double *tem, double *item, double *iqx,
                        double *iqy, double *iqz, const double *diff, double gx, double gy, double gz, int issave,
                        double *igsnap);
+void ol_aad_collide_mf(ol_lattice *p, const double *rho, const double *ux, const double *uy, const double *uz,
+                       double *ip, double *iux, double *iuy, double *iuz, double *imx, double *imy, double *imz,
+                       const double *alpha, double nu, ol_lattice *q, const double *tem, double *item, double *iqx,
+                       double *iqy, double *iqz, const double *diff, double gx, double gy, double gz,
+                       const double *dirx, const double *diry, const double *dirz, int issave, double *igsnap);
 void ol_aad_collide_he(ol_lattice *p, const double *rho, const double *ux, const double *uy, const double *uz,
                        double *ip, double *iux, double *iuy, double *iuz, double *imx, double *imy, double *imz,
                        const double *alpha, double nu, ol_lattice *q, const double *tem, double *item, double *iqx,

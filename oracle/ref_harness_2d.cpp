@@ -181,6 +181,27 @@ void ref2_ad_collide_nc(int lx, int ly, int lz, double *f0, double *f, double *g
         save(p, f0, f); save(q, g0, g);
     }
 }
+void ref2_ad_collide_bgk_fc(int lx, int ly, int lz, double *f0, double *f, double *g0, double *g, double *rho,
+                           double *ux, double *uy, double *uz, double nu, double *tem, double *qx, double *qy,
+                           double *qz, double diff, int issave) {
+    {
+        D2Q9<double> p(lx, ly), q(lx, ly);
+        load(p, f0, f); load(q, g0, g);
+        AD::MacroCollideForceConvection(p, rho, ux, uy, nu, q, tem, qx, qy, diff, issave != 0);
+        save(p, f0, f); save(q, g0, g);
+    }
+}
+void ref2_ad_collide_bgk_nc(int lx, int ly, int lz, double *f0, double *f, double *g0, double *g, double *rho,
+                           double *ux, double *uy, double *uz, double nu, double *tem, double *qx, double *qy,
+                           double *qz, double diff, double gx, double gy, double gz, double tem0, int issave) {
+    {
+        D2Q9<double> p(lx, ly), q(lx, ly);
+        load(p, f0, f); load(q, g0, g);
+        AD::MacroCollideNaturalConvection(p, rho, ux, uy, nu, q, tem, qx, qy, diff, gx, gy, tem0,
+                                          issave != 0);
+        save(p, f0, f); save(q, g0, g);
+    }
+}
 void ref2_ad_collide_he(int lx, int ly, int lz, double *f0, double *f, double *g0, double *g, double *rho,
                        double *ux, double *uy, double *uz, const double *alpha, double nu, double *tem, double *qx,
                        double *qy, double *qz, const double *beta, double diff, int issave) {
@@ -222,6 +243,22 @@ void ref2_aad_collide_nc(int lx, int ly, int lz, double *f0, double *f, double *
         load(p, f0, f); load(q, g0, g);
         AAD::MacroBrinkmanCollideNaturalConvection(p, rho, ux, uy, ip, iux, iuy, imx, imy, alpha, nu, q,
                                                    tem, item, iqx, iqy, diff, gx, gy, issave != 0, igsnap);
+        save(p, f0, f); save(q, g0, g);
+    }
+}
+
+void ref2_aad_collide_mf(int lx, int ly, int lz, double *f0, double *f, double *g0, double *g,
+                        const double *rho, const double *ux, const double *uy, const double *uz, double *ip, double *iux,
+                        double *iuy, double *iuz, double *imx, double *imy, double *imz, const double *alpha, double nu,
+                        const double *tem, double *item, double *iqx, double *iqy, double *iqz, const double *diff,
+                        double gx, double gy, double gz, const double *dirx, const double *diry, const double *dirz,
+                        int issave, double *igsnap) {
+    {
+        D2Q9<double> p(lx, ly), q(lx, ly);
+        load(p, f0, f); load(q, g0, g);
+        AAD::MacroBrinkmanCollideNaturalConvectionMassFlow(p, rho, ux, uy, ip, iux, iuy, imx, imy, alpha, nu,
+                                                           q, tem, item, iqx, iqy, diff, gx, gy, dirx, diry, 
+                                                           issave != 0, igsnap);
         save(p, f0, f); save(q, g0, g);
     }
 }

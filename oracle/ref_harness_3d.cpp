@@ -174,6 +174,27 @@ void ref3_ad_collide_nc(int lx, int ly, int lz, double *f0, double *f, double *g
         save(p, f0, f); save(q, g0, g);
     }
 }
+void ref3_ad_collide_bgk_fc(int lx, int ly, int lz, double *f0, double *f, double *g0, double *g, double *rho,
+                           double *ux, double *uy, double *uz, double nu, double *tem, double *qx, double *qy,
+                           double *qz, double diff, int issave) {
+    {
+        D3Q15<double> p(lx, ly, lz), q(lx, ly, lz);
+        load(p, f0, f); load(q, g0, g);
+        AD::MacroCollideForceConvection(p, rho, ux, uy, uz, nu, q, tem, qx, qy, qz, diff, issave != 0);
+        save(p, f0, f); save(q, g0, g);
+    }
+}
+void ref3_ad_collide_bgk_nc(int lx, int ly, int lz, double *f0, double *f, double *g0, double *g, double *rho,
+                           double *ux, double *uy, double *uz, double nu, double *tem, double *qx, double *qy,
+                           double *qz, double diff, double gx, double gy, double gz, double tem0, int issave) {
+    {
+        D3Q15<double> p(lx, ly, lz), q(lx, ly, lz);
+        load(p, f0, f); load(q, g0, g);
+        AD::MacroCollideNaturalConvection(p, rho, ux, uy, uz, nu, q, tem, qx, qy, qz, diff, gx, gy, gz, tem0,
+                                          issave != 0);
+        save(p, f0, f); save(q, g0, g);
+    }
+}
 void ref3_ad_collide_he(int lx, int ly, int lz, double *f0, double *f, double *g0, double *g, double *rho,
                        double *ux, double *uy, double *uz, const double *alpha, double nu, double *tem, double *qx,
                        double *qy, double *qz, const double *beta, double diff, int issave) {
@@ -215,6 +236,22 @@ void ref3_aad_collide_nc(int lx, int ly, int lz, double *f0, double *f, double *
         load(p, f0, f); load(q, g0, g);
         AAD::MacroBrinkmanCollideNaturalConvection(p, rho, ux, uy, uz, ip, iux, iuy, iuz, imx, imy, imz, alpha, nu, q,
                                                    tem, item, iqx, iqy, iqz, diff, gx, gy, gz, issave != 0, igsnap);
+        save(p, f0, f); save(q, g0, g);
+    }
+}
+
+void ref3_aad_collide_mf(int lx, int ly, int lz, double *f0, double *f, double *g0, double *g,
+                        const double *rho, const double *ux, const double *uy, const double *uz, double *ip, double *iux,
+                        double *iuy, double *iuz, double *imx, double *imy, double *imz, const double *alpha, double nu,
+                        const double *tem, double *item, double *iqx, double *iqy, double *iqz, const double *diff,
+                        double gx, double gy, double gz, const double *dirx, const double *diry, const double *dirz,
+                        int issave, double *igsnap) {
+    {
+        D3Q15<double> p(lx, ly, lz), q(lx, ly, lz);
+        load(p, f0, f); load(q, g0, g);
+        AAD::MacroBrinkmanCollideNaturalConvectionMassFlow(p, rho, ux, uy, uz, ip, iux, iuy, iuz, imx, imy, imz, alpha, nu,
+                                                           q, tem, item, iqx, iqy, iqz, diff, gx, gy, gz, dirx, diry, dirz, 
+                                                           issave != 0, igsnap);
         save(p, f0, f); save(q, g0, g);
     }
 }

@@ -111,6 +111,14 @@ def load():
         "lbm_ad_macro_brinkman_collide_force_convection": [VP] * 6 + [D] + [VP] * 6 + [I, VP, VP],
         "lbm_ad_macro_brinkman_collide_natural_convection": [VP] * 6 + [D] + [VP] * 6 + [D] * 4 + [I, VP],
         "lbm_ad_macro_brinkman_collide_heat_exchange": [VP] * 6 + [D] + [VP] * 6 + [D, I],
+        "lbm_ad_macro_collide_force_convection": [VP] * 5 + [D] + [VP] * 5 + [D, I],
+        "lbm_ad_macro_collide_force_convection_p": [VP] * 6 + [D] + [VP] * 5 + [D, I],
+        "lbm_ad_macro_collide_natural_convection": [VP] * 5 + [D] + [VP] * 5 + [D] * 5 + [I],
+        "lbm_ad_macro_collide_natural_convection_p": [VP] * 6 + [D] + [VP] * 5 + [D] * 5 + [I],
+        "lbm_aad_macro_brinkman_collide_natural_convection_mass_flow":
+            [VP] * 13 + [D] + [VP] * 7 + [D] * 3 + [VP] * 3 + [I, VP],
+        "lbm_aad_macro_brinkman_collide_natural_convection_mass_flow_p":
+            [VP] * 14 + [D] + [VP] * 7 + [D] * 3 + [VP] * 3 + [I, VP],
         "lbm_ad_macro_brinkman_collide_heat_exchange_p": [VP] * 7 + [D] + [VP] * 6 + [D, I],
         "lbm_aad_macro_brinkman_collide_heat_exchange": [VP] * 13 + [D] + [VP] * 7 + [D, I],
         "lbm_aad_macro_brinkman_collide_heat_exchange_p": [VP] * 14 + [D] + [VP] * 7 + [D, I],

@@ -1008,6 +1008,49 @@ int lbm_ad_macro_brinkman_collide_natural_convection(lbm_lattice *p, double *rho
     return lbm_ad_macro_brinkman_collide_natural_convection_p(nullptr, p, rho, ux, uy, uz, alpha, nu, q, tem, qx, qy,
                                                               qz, diff, gx, gy, gz, tem0, issave, g);
 }
+int lbm_ad_macro_collide_force_convection_p(lbm_program *P, lbm_lattice *p, double *rho, double *ux, double *uy,
+                                            double *uz, double nu, lbm_lattice *q, double *tem, double *qx,
+                                            double *qy, double *qz, double diffusivity, int issave) {
+    if (!ns_lattice(p) || !q) return fail(LBM_ERR_ARG, "bad arguments");
+    if (issave && (!fields_ok({rho, ux, uy, tem, qx, qy}) || (p->nd == 3 && (!uz || !qz))))
+        return fail(LBM_ERR_ARG, "null field");
+    return collide_call(P, p, q, {rho, ux, uy, uz, tem, qx, qy, qz}, [&](const lbm::CollideLaunch &cl) {
+        lbm::ADArgs a{rho, ux, uy, uz, tem, qx, qy, qz, nullptr, nullptr, nullptr, nullptr, nu, issave};
+        a.diffc = diffusivity;
+        return hip_result(lbm::launch_ad_collide(cl, a), "ad collide force convection (BGK)");
+    });
+}
+int lbm_ad_macro_collide_force_convection(lbm_lattice *p, double *rho, double *ux, double *uy, double *uz, double nu,
+                                          lbm_lattice *q, double *tem, double *qx, double *qy, double *qz,
+                                          double diffusivity, int issave) {
+    return lbm_ad_macro_collide_force_convection_p(nullptr, p, rho, ux, uy, uz, nu, q, tem, qx, qy, qz, diffusivity,
+                                                   issave);
+}
+int lbm_ad_macro_collide_natural_convection_p(lbm_program *P, lbm_lattice *p, double *rho, double *ux, double *uy,
+                                              double *uz, double nu, lbm_lattice *q, double *tem, double *qx,
+                                              double *qy, double *qz, double diffusivity, double gx, double gy,
+                                              double gz, double tem0, int issave) {
+    if (!ns_lattice(p) || !q) return fail(LBM_ERR_ARG, "bad arguments");
+    if (issave && (!fields_ok({rho, ux, uy, tem, qx, qy}) || (p->nd == 3 && (!uz || !qz))))
+        return fail(LBM_ERR_ARG, "null field");
+    return collide_call(P, p, q, {rho, ux, uy, uz, tem, qx, qy, qz}, [&](const lbm::CollideLaunch &cl) {
+        lbm::ADArgs a{rho, ux, uy, uz, tem, qx, qy, qz, nullptr, nullptr, nullptr, nullptr, nu, issave};
+        a.diffc = diffusivity;
+        a.natural = 1;
+        a.gx = gx;
+        a.gy = gy;
+        a.gz = p->nd == 3 ? gz : 0.0;
+        a.tem0 = tem0;
+        return hip_result(lbm::launch_ad_collide(cl, a), "ad collide natural convection (BGK)");
+    });
+}
+int lbm_ad_macro_collide_natural_convection(lbm_lattice *p, double *rho, double *ux, double *uy, double *uz,
+                                            double nu, lbm_lattice *q, double *tem, double *qx, double *qy,
+                                            double *qz, double diffusivity, double gx, double gy, double gz,
+                                            double tem0, int issave) {
+    return lbm_ad_macro_collide_natural_convection_p(nullptr, p, rho, ux, uy, uz, nu, q, tem, qx, qy, qz, diffusivity,
+                                                     gx, gy, gz, tem0, issave);
+}
 int lbm_ad_macro_brinkman_collide_heat_exchange_p(lbm_program *P, lbm_lattice *p, double *rho, double *ux,
                                                   double *uy, double *uz, const double *alpha, double nu,
                                                   lbm_lattice *q, double *tem, double *qx, double *qy, double *qz,
@@ -1169,6 +1212,40 @@ int lbm_aad_macro_brinkman_collide_natural_convection(lbm_lattice *p, const doub
     return lbm_aad_macro_brinkman_collide_natural_convection_p(nullptr, p, rho, ux, uy, uz, ip, iux, iuy, iuz, imx,
                                                                imy, imz, alpha, nu, q, tem, item, iqx, iqy, iqz,
                                                                diff, gx, gy, gz, issave, ig);
+}
+int lbm_aad_macro_brinkman_collide_natural_convection_mass_flow_p(
+    lbm_program *P, lbm_lattice *p, const double *rho, const double *ux, const double *uy, const double *uz,
+    double *ip, double *iux, double *iuy, double *iuz, double *imx, double *imy, double *imz, const double *alpha,
+    double nu, lbm_lattice *q, const double *tem, double *item, double *iqx, double *iqy, double *iqz,
+    const double *diff, double gx, double gy, double gz, const double *dirx, const double *diry, const double *dirz,
+    int issave, double *ig) {
+    if (!ns_lattice(p) || !q || !fields_ok({rho, ux, uy, alpha, tem, diff, dirx, diry}) ||
+        (p->nd == 3 && (!uz || !dirz)))
+        return fail(LBM_ERR_ARG, "bad arguments");
+    if (issave &&
+        (!fields_ok({ip, iux, iuy, imx, imy, item, iqx, iqy}) || (p->nd == 3 && (!iuz || !imz || !iqz))))
+        return fail(LBM_ERR_ARG, "null field");
+    return collide_call(P, p, q, {ip, iux, iuy, iuz, imx, imy, imz, item, iqx, iqy, iqz}, [&](const lbm::CollideLaunch &cl) {
+        lbm::AADArgs a{rho, ux, uy, uz, alpha, tem, diff, ip, iux, iuy, iuz, imx, imy, imz, item, iqx, iqy, iqz,
+                       issave ? ig : nullptr, nu, issave};
+        a.natural = 1;
+        a.gx = gx;
+        a.gy = gy;
+        a.gz = p->nd == 3 ? gz : 0.0;
+        a.dirx = dirx;
+        a.diry = diry;
+        a.dirz = p->nd == 3 ? dirz : nullptr;
+        return hip_result(lbm::launch_aad_collide(cl, a), "aad collide natural convection mass flow");
+    });
+}
+int lbm_aad_macro_brinkman_collide_natural_convection_mass_flow(
+    lbm_lattice *p, const double *rho, const double *ux, const double *uy, const double *uz, double *ip, double *iux,
+    double *iuy, double *iuz, double *imx, double *imy, double *imz, const double *alpha, double nu, lbm_lattice *q,
+    const double *tem, double *item, double *iqx, double *iqy, double *iqz, const double *diff, double gx, double gy,
+    double gz, const double *dirx, const double *diry, const double *dirz, int issave, double *ig) {
+    return lbm_aad_macro_brinkman_collide_natural_convection_mass_flow_p(
+        nullptr, p, rho, ux, uy, uz, ip, iux, iuy, iuz, imx, imy, imz, alpha, nu, q, tem, item, iqx, iqy, iqz, diff,
+        gx, gy, gz, dirx, diry, dirz, issave, ig);
 }
 int lbm_aad_macro_brinkman_collide_heat_exchange_p(lbm_program *P, lbm_lattice *p, const double *rho,
                                                    const double *ux, const double *uy, const double *uz, double *ip,

@@ -52,7 +52,9 @@ struct NSBody<LA, LB, double> {
 
 // AD::MacroBrinkmanCollideForceConvection, advection.h:814-926 (+ optional volumetric source),
 // AD::MacroBrinkmanCollideNaturalConvection, advection.h:930-1044 (natural = 1), and
-// AD::MacroBrinkmanCollideHeatExchange, advection.h:716-810 (beta set, constant diffusivity diffc).
+// AD::MacroBrinkmanCollideHeatExchange, advection.h:716-810 (beta set, constant diffusivity diffc), and
+// without alpha the BGK-only AD::MacroCollideForceConvection / MacroCollideNaturalConvection,
+// advection.h:528-712 (constant diffusivity; the forced step collides the first macro's moments).
 // The exchange term changes only g and uses T of the first macro, so it sits with the source term.
 template <class LA, class LB, class R>
 struct ADBody;
@@ -76,7 +78,7 @@ struct ADBody<LA, LB, double> {
                 f[C] += 3.0 * LA::w[C] * cdot<LA, C>(gx, gy, gz) * dT;
             });
         }
-        ns_brinkman<LA>(f, r, vx, vy, vz, alpha[n]);
+        if (alpha) ns_brinkman<LA>(f, r, vx, vy, vz, alpha[n]);
         if (heat) {
             double h = heat[n];
             sfor<0, LB::nc>([&](auto c) {
@@ -91,8 +93,10 @@ struct ADBody<LA, LB, double> {
                 g[C] += LB::w[C] * coef;
             });
         }
-        ns_macro<LA>(f, r, vx, vy, vz);
-        ad_macro<LB>(g, vx, vy, vz, omegag, T, hx, hy, hz);
+        if (alpha || natural || heat || beta) {  // BGK forced convection: nothing to take again
+            ns_macro<LA>(f, r, vx, vy, vz);
+            ad_macro<LB>(g, vx, vy, vz, omegag, T, hx, hy, hz);
+        }
         if (save == 1 || (save == 2 && bnd)) {
             rho[n] = r;
             ux[n] = vx;
@@ -153,14 +157,16 @@ struct ANSBody<LA, LB, double> {
 
 // AAD::MacroBrinkmanCollideForceConvection, adjointadvection.h:997-1109,
 // AAD::MacroBrinkmanCollideNaturalConvection, adjointadvection.h:1113-1230 (natural = 1), and
-// AAD::MacroBrinkmanCollideHeatExchange, adjointadvection.h:894-993 (beta set, constant diffusivity)
+// AAD::MacroBrinkmanCollideHeatExchange, adjointadvection.h:894-993 (beta set, constant diffusivity),
+// and AAD::MacroBrinkmanCollideNaturalConvectionMassFlow, adjointadvection.h:1232-1358 (natural = 1
+// with the direction fields dir*: the mass-flow source on f before the Brinkman force)
 template <class LA, class LB, class R>
 struct AADBody;
 template <class LA, class LB>
 struct AADBody<LA, LB, double> {
     const double *rho, *ux, *uy, *uz, *alpha, *tem, *diff;
     double *ip, *iux, *iuy, *iuz, *imx, *imy, *imz, *item, *iqx, *iqy, *iqz, *igsnap;
-    const double *beta;
+    const double *beta, *dirx, *diry, *dirz;
     double omegaf, iomegaf, diffc;
     int save, N, natural;
     double gx, gy, gz;
@@ -171,6 +177,7 @@ struct AADBody<LA, LB, double> {
         double a, bx, by, bz, mx, my, mz, it, hx, hy, hz;
         ans_macro<LA>(f, vx, vy, vz, a, bx, by, bz, mx, my, mz);
         aad_macro<LB>(g, it, hx, hy, hz);
+        if (dirx) aad_mass_flow<LA>(f, rho[n], vx, vy, vz, dirx[n], diry[n], LA::nd == 3 ? dirz[n] : 0.0);
         aad_brinkman<LA>(f, rho[n], vx, vy, vz, mx, my, mz, tem[n], hx, hy, hz, omegag, alpha[n]);
         ans_macro<LA>(f, vx, vy, vz, a, bx, by, bz, mx, my, mz);
         if (natural) {  // ExternalForceNaturalConvection, adjointadvection.h:114-132, then re-macro
@@ -351,7 +358,7 @@ struct ADBody<LA, LB, float> {
                 f[C] += 3.0f * float(LA::w[C]) * cdot<LA, C>(gx, gy, gz) * dTb;
             });
         }
-        ns_brinkman<LA>(f, r, vx, vy, vz, float(alpha[n]));
+        if (alpha) ns_brinkman<LA>(f, r, vx, vy, vz, float(alpha[n]));
         if (heat) {
             float h = float(heat[n]);
             sfor<0, LB::nc>([&](auto c) {
@@ -366,8 +373,10 @@ struct ADBody<LA, LB, float> {
                 g[C] += float(LB::w[C]) * coef;
             });
         }
-        ns_macro_d<LA>(f, oa, dr, jx, jy, jz, r, vx, vy, vz);
-        ad_macro_d<LB>(g, ob, vx, vy, vz, omegag, dT, T, hx, hy, hz, px, py, pz);
+        if (alpha || natural || heat || beta) {
+            ns_macro_d<LA>(f, oa, dr, jx, jy, jz, r, vx, vy, vz);
+            ad_macro_d<LB>(g, ob, vx, vy, vz, omegag, dT, T, hx, hy, hz, px, py, pz);
+        }
         if (save == 1 || (save == 2 && bnd)) {
             rho[n] = r;
             ux[n] = vx;
@@ -429,7 +438,7 @@ template <class LA, class LB>
 struct AADBody<LA, LB, float> {
     const double *rho, *ux, *uy, *uz, *alpha, *tem, *diff;
     double *ip, *iux, *iuy, *iuz, *imx, *imy, *imz, *item, *iqx, *iqy, *iqz, *igsnap;
-    const double *beta;
+    const double *beta, *dirx, *diry, *dirz;
     float omegaf, iomegaf, diffc;
     int save, N, natural;
     float gx, gy, gz;
@@ -443,6 +452,9 @@ struct AADBody<LA, LB, float> {
         add_shift_f<LB>(g, eb, ob, 1.0f);
         ans_macro<LA>(f, vx, vy, vz, a, bx, by, bz, mx, my, mz);
         aad_macro<LB>(g, it, hx, hy, hz);
+        if (dirx)
+            aad_mass_flow<LA>(f, float(rho[n]), vx, vy, vz, float(dirx[n]), float(diry[n]),
+                              LA::nd == 3 ? float(dirz[n]) : 0.0f);
         aad_brinkman<LA>(f, float(rho[n]), vx, vy, vz, mx, my, mz, float(tem[n]), hx, hy, hz, omegag, float(alpha[n]));
         ans_macro<LA>(f, vx, vy, vz, a, bx, by, bz, mx, my, mz);
         if (natural) {
@@ -833,7 +845,7 @@ hipError_t aad_collide_nd(const CollideLaunch &cl, const AADArgs &a) {
         b.item = a.item; b.iqx = a.iqx; b.iqy = a.iqy; b.iqz = a.iqz; b.igsnap = a.igsnap;
         b.omegaf = R(omega); b.iomegaf = R(1.0 - omega); b.save = a.save;
         b.natural = a.natural; b.gx = R(a.gx); b.gy = R(a.gy); b.gz = R(a.gz);
-        b.beta = a.beta; b.diffc = R(a.diffc);
+        b.beta = a.beta; b.diffc = R(a.diffc); b.dirx = a.dirx; b.diry = a.diry; b.dirz = a.dirz;
         return b;
     });
 }

@@ -43,7 +43,7 @@ struct NSArgs {
 };
 struct ADArgs {
     double *rho, *ux, *uy, *uz, *tem, *qx, *qy, *qz, *gsnap;
-    const double *alpha, *diff, *heat;
+    const double *alpha, *diff, *heat;  // alpha null: MacroCollide{Force,Natural}Convection (no Brinkman)
     double nu;
     int save;
     int natural;                  // 1: MacroBrinkmanCollideNaturalConvection (buoyancy g (T - tem0))
@@ -66,6 +66,7 @@ struct AADArgs {
     double gx, gy, gz;
     const double *beta;           // MacroBrinkmanCollideHeatExchange
     double diffc;                 // diffusivity when diff is null
+    const double *dirx, *diry, *dirz;  // MacroBrinkmanCollideNaturalConvectionMassFlow directions
 };
 
 hipError_t launch_stream(int kind, int prec, const void *src, void *dst, Grid g, int pull, const Enc &e, hipStream_t s);

@@ -196,5 +196,18 @@ __device__ __forceinline__ void aad_brinkman(R *f, R rho, R ux, R uy, R uz, R im
             f[C] += coef * ((R(L::cx[C]) - ux) * ax + (R(L::cy[C]) - uy) * ay + (R(L::cz[C]) - uz) * az);
     });
 }
+// adjointadvection.h:134-150: the mass-flow objective's source, d the per-node direction
+template <class L, class R>
+__device__ __forceinline__ void aad_mass_flow(R *f, R rho, R ux, R uy, R uz, R dx, R dy, R dz) {
+    if constexpr (L::nd == 2) f[0] += (ux * dx + uy * dy) / rho;
+    else f[0] += (ux * dx + uy * dy + uz * dz) / rho;
+    sfor<1, L::nc>([&](auto c) {
+        constexpr int C = decltype(c)::value;
+        if constexpr (L::nd == 2)
+            f[C] -= ((R(L::cx[C]) - ux) * dx + (R(L::cy[C]) - uy) * dy) / rho;
+        else
+            f[C] -= ((R(L::cx[C]) - ux) * dx + (R(L::cy[C]) - uy) * dy + (R(L::cz[C]) - uz) * dz) / rho;
+    });
+}
 
 }  // namespace lbm

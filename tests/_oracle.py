@@ -66,6 +66,7 @@ def lib():
         L.ol_ad_init.argtypes = [VP] * 5
         L.ol_ad_collide_fc.argtypes = [VP] + [VP] * 5 + [D] + [VP] * 6 + [I, VP, VP]
         L.ol_ad_collide_nc.argtypes = [VP] + [VP] * 5 + [D] + [VP] * 6 + [D] * 4 + [I, VP]
+        L.ol_ad_collide_bgk.argtypes = [VP] * 5 + [D, VP] + [VP] * 4 + [D, I] + [D] * 4 + [I]
         L.ol_ad_collide_he.argtypes = [VP] + [VP] * 5 + [D] + [VP] * 6 + [D, I]
         L.ol_ad_set_t.argtypes = [VP] * 6
         L.ol_ad_set_q.argtypes = [VP] * 7 + [D]
@@ -77,6 +78,7 @@ def lib():
         L.ol_aad_init.argtypes = [VP] * 8
         L.ol_aad_collide_fc.argtypes = [VP] + [VP] * 11 + [VP, D, VP] + [VP] * 5 + [VP, I, VP]
         L.ol_aad_collide_nc.argtypes = [VP] + [VP] * 11 + [VP, D, VP] + [VP] * 5 + [VP] + [D] * 3 + [I, VP]
+        L.ol_aad_collide_mf.argtypes = [VP] + [VP] * 11 + [VP, D, VP] + [VP] * 5 + [VP] + [D] * 3 + [VP] * 3 + [I, VP]
         L.ol_aad_collide_he.argtypes = [VP] + [VP] * 11 + [VP, D, VP] + [VP] * 5 + [VP, D, I]
         L.ol_aad_iset_t.argtypes = [VP] * 5
         L.ol_aad_iset_q.argtypes = [VP] * 5 + [D]

@@ -52,6 +52,7 @@ def main(only=None):
         write_driver_golden()
         write_mma_golden()
         write_heaviside_golden()
+        write_vtk_golden()
 
 
 def write_mma_golden():
@@ -123,6 +124,18 @@ def write_heaviside_golden():
     print("wrote", path, os.path.getsize(path), "bytes")
 
 
+def write_vtk_golden():
+    """The files the reference's VTK writers (and the reader's read-back) produce for
+    tests/cpp/vtk_driver.cpp."""
+    drv = os.path.join(os.path.dirname(os.path.dirname(HERE)), "oracle", "_ref", "vtk_driver_ref")
+    if not os.path.exists(drv):
+        raise SystemExit("reference VTK driver not built: make -C oracle ref")
+    out = os.path.join(HERE, "vtk")
+    os.makedirs(out, exist_ok=True)
+    subprocess.run([drv, out], check=True, stdout=subprocess.DEVNULL)
+    print("wrote", out, sorted(os.listdir(out)))
+
+
 def write_driver_golden():
     drv = os.path.join(os.path.dirname(os.path.dirname(HERE)), "oracle", "_ref", "heatsink_driver_ref")
     if not os.path.exists(drv):
@@ -141,5 +154,7 @@ if __name__ == "__main__":
         write_mma_golden()
     elif sys.argv[1:] == ["heaviside"]:
         write_heaviside_golden()
+    elif sys.argv[1:] == ["vtk"]:
+        write_vtk_golden()
     else:
         main(sys.argv[1:] or None)

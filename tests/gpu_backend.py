@@ -143,6 +143,15 @@ class GpuBackend:
             self._prog(), p.h, *[self.f(n) for n in (rho, ux, uy, uz, alpha)], D(nu), q.h,
             *[self.f(n) for n in (tem, qx, qy, qz, diff)], D(gx), D(gy), D(gz), D(tem0), int(save), self.f(gsnap)))
 
+    def ad_collide_bgk(self, p, rho, ux, uy, uz, nu, q, tem, qx, qy, qz, diff, natural, gx=0.0, gy=0.0, gz=0.0,
+                       tem0=0.0, save=True):
+        a = [self._prog(), p.h, *[self.f(n) for n in (rho, ux, uy, uz)], D(nu), q.h,
+             *[self.f(n) for n in (tem, qx, qy, qz)], D(diff)]
+        if natural:
+            check(self.L.lbm_ad_macro_collide_natural_convection_p(*a, D(gx), D(gy), D(gz), D(tem0), int(save)))
+        else:
+            check(self.L.lbm_ad_macro_collide_force_convection_p(*a, int(save)))
+
     def aad_collide_nc(self, p, rho, ux, uy, uz, ip, iux, iuy, iuz, imx, imy, imz, alpha, nu, q, tem, item, iqx, iqy,
                        iqz, diff, gx, gy, gz, save=True, igsnap=None):
         check(self.L.lbm_aad_macro_brinkman_collide_natural_convection_p(

@@ -7,7 +7,7 @@ import panslbm2_amd as pa
 from panslbm2_amd import D, check, hptr
 
 from _oracle import NC, ND
-from refcases import NC_G, NC_TEM0, he_inputs
+from refcases import NC_G, NC_TEM0, he_inputs, mf_dirs
 
 
 def run_gpu(ctx, case, op, inp, prec=pa.F64):
@@ -103,6 +103,18 @@ def run_gpu(ctx, case, op, inp, prec=pa.F64):
                                                                  snap))
         for n in names + ("gsnap",):
             out[n] = dev["o_" + n].download()
+    elif op in ("ad_collide_bgk_fc", "ad_collide_bgk_nc"):
+        names = ("rho", "ux", "uy", "uz", "tem", "qx", "qy", "qz")
+        o = outs(names)
+        if op == "ad_collide_bgk_fc":
+            check(L.lbm_ad_macro_collide_force_convection(p.h, *o[:4], D(0.1), q.h, *o[4:], D(float(inp["dconst"])),
+                                                          1))
+        else:
+            check(L.lbm_ad_macro_collide_natural_convection(p.h, *o[:4], D(0.1), q.h, *o[4:],
+                                                            D(float(inp["dconst"])), *[D(x) for x in NC_G],
+                                                            D(NC_TEM0), 1))
+        for n in names:
+            out[n] = dev["o_" + n].download()
     elif op == "ad_collide_he":
         beta, _, dc = he_inputs(inp)
         names = ("rho", "ux", "uy", "uz", "tem", "qx", "qy", "qz")
@@ -133,6 +145,16 @@ def run_gpu(ctx, case, op, inp, prec=pa.F64):
         check(L.lbm_aad_macro_brinkman_collide_natural_convection(p.h, d("rho"), d("ux"), d("uy"), uzp, *o[:7],
                                                                   d("alpha"), D(0.1), q.h, d("tem"), *o[7:],
                                                                   d("diff"), *[D(x) for x in NC_G], 1, snap))
+        for n in names + ("igsnap",):
+            out[n] = dev["o_" + n].download()
+    elif op == "aad_collide_mf":
+        dx, dy, dz = mf_dirs(inp)
+        names = ("ip", "iux", "iuy", "iuz", "imx", "imy", "imz", "item", "iqx", "iqy", "iqz")
+        o = outs(names)
+        snap = outs(["igsnap"], (NC[case.kind_b], N))[0]
+        check(L.lbm_aad_macro_brinkman_collide_natural_convection_mass_flow(
+            p.h, d("rho"), d("ux"), d("uy"), uzp, *o[:7], d("alpha"), D(0.1), q.h, d("tem"), *o[7:], d("diff"),
+            *[D(x) for x in NC_G], d("dirx", dx), d("diry", dy), d("dirz", dz) if uzp else None, 1, snap))
         for n in names + ("igsnap",):
             out[n] = dev["o_" + n].download()
     elif op == "aad_collide_he":

@@ -50,6 +50,11 @@ def he_inputs(inp):
     return 0.1 * inp["alpha"], inp["dkds"], float(inp["dconst"])
 
 
+def mf_dirs(inp):
+    """Direction fields of the mass-flow objective, O(1) and derived from the seeded inputs."""
+    return 10.0 * inp["v0"], 10.0 * inp["v1"], 10.0 * inp["v2"]
+
+
 def _i32(a):
     return np.ascontiguousarray(a, dtype=np.int32)
 
@@ -171,6 +176,14 @@ def run_ref(case, op, inp):
                             ptr(o["qz"]), ptr(inp["diff"]), *[D(x) for x in NC_G], D(NC_TEM0), 1, ptr(snap))
         out.update(o)
         out["gsnap"] = snap.reshape(case.N, nc).T.copy()
+    elif op in ("ad_collide_bgk_fc", "ad_collide_bgk_nc"):
+        o = {k: np.zeros(case.N) for k in ("rho", "ux", "uy", "uz", "tem", "qx", "qy", "qz")}
+        args = [ptr(f0), ptr(f), ptr(g0), ptr(g), ptr(o["rho"]), ptr(o["ux"]), ptr(o["uy"]), ptr(o["uz"]), D(0.1),
+                ptr(o["tem"]), ptr(o["qx"]), ptr(o["qy"]), ptr(o["qz"]), D(float(inp["dconst"]))]
+        if op == "ad_collide_bgk_nc":
+            args += [D(x) for x in NC_G] + [D(NC_TEM0)]
+        fn(op)(*L, *args, 1)
+        out.update(o)
     elif op == "ad_collide_he":
         beta, _, dc = he_inputs(inp)
         o = {k: np.zeros(case.N) for k in ("rho", "ux", "uy", "uz", "tem", "qx", "qy", "qz")}
@@ -201,6 +214,17 @@ def run_ref(case, op, inp):
                              ptr(o["imy"]), ptr(o["imz"]), ptr(inp["alpha"]), D(0.1), ptr(inp["tem"]), ptr(o["item"]),
                              ptr(o["iqx"]), ptr(o["iqy"]), ptr(o["iqz"]), ptr(inp["diff"]), *[D(x) for x in NC_G], 1,
                              ptr(snap))
+        out.update(o)
+        out["igsnap"] = snap.reshape(case.N, nc).T.copy()
+    elif op == "aad_collide_mf":
+        dx, dy, dz = mf_dirs(inp)
+        o = {k: np.zeros(case.N) for k in ("ip", "iux", "iuy", "iuz", "imx", "imy", "imz", "item", "iqx", "iqy", "iqz")}
+        snap = np.zeros(nc * case.N)
+        fn("aad_collide_mf")(*L, ptr(f0), ptr(f), ptr(g0), ptr(g), ptr(inp["rho"]), ptr(inp["ux"]), ptr(inp["uy"]),
+                             ptr(uz), ptr(o["ip"]), ptr(o["iux"]), ptr(o["iuy"]), ptr(o["iuz"]), ptr(o["imx"]),
+                             ptr(o["imy"]), ptr(o["imz"]), ptr(inp["alpha"]), D(0.1), ptr(inp["tem"]), ptr(o["item"]),
+                             ptr(o["iqx"]), ptr(o["iqy"]), ptr(o["iqz"]), ptr(inp["diff"]), *[D(x) for x in NC_G],
+                             ptr(dx), ptr(dy), ptr(dz), 1, ptr(snap))
         out.update(o)
         out["igsnap"] = snap.reshape(case.N, nc).T.copy()
     elif op == "aad_collide_he":
@@ -339,6 +363,12 @@ def run_oracle(case, op, inp):
                            *[D(x) for x in NC_G], D(NC_TEM0), 1, ptr(snap))
         out.update(o)
         out["gsnap"] = snap
+    elif op in ("ad_collide_bgk_fc", "ad_collide_bgk_nc"):
+        o = {kk: np.zeros(case.N) for kk in ("rho", "ux", "uy", "uz", "tem", "qx", "qy", "qz")}
+        O.ol_ad_collide_bgk(p.h, ptr(o["rho"]), ptr(o["ux"]), ptr(o["uy"]), ptr(o["uz"]), D(0.1), q.h, ptr(o["tem"]),
+                            ptr(o["qx"]), ptr(o["qy"]), ptr(o["qz"]), D(float(inp["dconst"])),
+                            int(op == "ad_collide_bgk_nc"), *[D(x) for x in NC_G], D(NC_TEM0), 1)
+        out.update(o)
     elif op == "ad_collide_he":
         beta, _, dc = he_inputs(inp)
         o = {kk: np.zeros(case.N) for kk in ("rho", "ux", "uy", "uz", "tem", "qx", "qy", "qz")}
@@ -367,6 +397,17 @@ def run_oracle(case, op, inp):
                             ptr(o["iuy"]), ptr(o["iuz"]), ptr(o["imx"]), ptr(o["imy"]), ptr(o["imz"]),
                             ptr(inp["alpha"]), D(0.1), q.h, ptr(inp["tem"]), ptr(o["item"]), ptr(o["iqx"]),
                             ptr(o["iqy"]), ptr(o["iqz"]), ptr(inp["diff"]), *[D(x) for x in NC_G], 1, ptr(snap))
+        out.update(o)
+        out["igsnap"] = snap
+    elif op == "aad_collide_mf":
+        dx, dy, dz = mf_dirs(inp)
+        o = {kk: np.zeros(case.N) for kk in ("ip", "iux", "iuy", "iuz", "imx", "imy", "imz", "item", "iqx", "iqy", "iqz")}
+        snap = np.zeros((NC[case.kind_b], case.N))
+        O.ol_aad_collide_mf(p.h, ptr(inp["rho"]), ptr(inp["ux"]), ptr(inp["uy"]), ptr(uz), ptr(o["ip"]), ptr(o["iux"]),
+                            ptr(o["iuy"]), ptr(o["iuz"]), ptr(o["imx"]), ptr(o["imy"]), ptr(o["imz"]),
+                            ptr(inp["alpha"]), D(0.1), q.h, ptr(inp["tem"]), ptr(o["item"]), ptr(o["iqx"]),
+                            ptr(o["iqy"]), ptr(o["iqz"]), ptr(inp["diff"]), *[D(x) for x in NC_G], ptr(dx), ptr(dy),
+                            ptr(dz) if ND[case.kind] == 3 else None, 1, ptr(snap))
         out.update(o)
         out["igsnap"] = snap
     elif op == "aad_collide_he":
@@ -439,6 +480,8 @@ OPS_2D = {
     "aad_collide_fc": "adjointadvection.h:997-1050",
     "ad_collide_nc": "advection.h:930-984,73-79", "aad_collide_nc": "adjointadvection.h:1113-1168,114-122",
     "ad_collide_he": "advection.h:716-760,89-95", "aad_collide_he": "adjointadvection.h:894-940,106-112",
+    "ad_collide_bgk_fc": "advection.h:528-569", "ad_collide_bgk_nc": "advection.h:616-662,73-79",
+    "aad_collide_mf": "adjointadvection.h:1232-1291,134-141",
     "sens_heat_exchange": "adjointadvection.h:1445-1450",
     "sens_brinkman": "adjointnavierstokes.h:542-546", "sens_brinkman_diffusivity": "adjointadvection.h:1461-1475",
     "sens_temp_heat_source": "adjointadvection.h:1497-1517,755-801",
@@ -469,6 +512,8 @@ OPS_3D = {
     "aad_collide_fc": "adjointadvection.h:1054-1109",
     "ad_collide_nc": "advection.h:988-1044,81-87", "aad_collide_nc": "adjointadvection.h:1172-1230,124-132",
     "ad_collide_he": "advection.h:764-810,89-95", "aad_collide_he": "adjointadvection.h:944-993,106-112",
+    "ad_collide_bgk_fc": "advection.h:571-614", "ad_collide_bgk_nc": "advection.h:664-712,81-87",
+    "aad_collide_mf": "adjointadvection.h:1293-1358,143-150",
     "sens_heat_exchange": "adjointadvection.h:1452-1457",
     "sens_brinkman": "adjointnavierstokes.h:550-554", "sens_brinkman_diffusivity": "adjointadvection.h:1478-1493",
     "residual": "residual.h:39-51", "normalize": "normalize.h:9-24", "density_filter": "densityfilter.h:390-420,512-515",

@@ -131,6 +131,12 @@ class OracleBackend(_FieldStore):
                                *[ptr(self.f(n)) for n in (tem, qx, qy, qz, diff)], D(gx), D(gy), D(gz), D(tem0),
                                int(save), ptr(self.f(gsnap)))
 
+    def ad_collide_bgk(self, p, rho, ux, uy, uz, nu, q, tem, qx, qy, qz, diff, natural, gx=0.0, gy=0.0, gz=0.0,
+                       tem0=0.0, save=True):
+        lib().ol_ad_collide_bgk(p.h, *[ptr(self.f(n)) for n in (rho, ux, uy, uz)], D(nu), q.h,
+                                *[ptr(self.f(n)) for n in (tem, qx, qy, qz)], D(diff), int(natural), D(gx), D(gy),
+                                D(gz), D(tem0), int(save))
+
     def aad_collide_nc(self, p, rho, ux, uy, uz, ip, iux, iuy, iuz, imx, imy, imz, alpha, nu, q, tem, item, iqx, iqy,
                        iqz, diff, gx, gy, gz, save=True, igsnap=None):
         lib().ol_aad_collide_nc(p.h, *[ptr(self.f(n)) for n in (rho, ux, uy, uz, ip, iux, iuy, iuz, imx, imy, imz,
@@ -295,6 +301,15 @@ class RefBackend(_FieldStore):
                                     ptr(buf) if gsnap else None)
         if save:
             self._snap_out(gsnap, q, buf)
+
+    def ad_collide_bgk(self, p, rho, ux, uy, uz, nu, q, tem, qx, qy, qz, diff, natural, gx=0.0, gy=0.0, gz=0.0,
+                       tem0=0.0, save=True):
+        args = [ptr(p.f0), ptr(p.f), ptr(q.f0), ptr(q.f), *[ptr(self.f(n)) for n in (rho, ux, uy, uz)], D(nu),
+                *[ptr(self.f(n)) for n in (tem, qx, qy, qz)], D(diff)]
+        if natural:
+            self._r(p, "ad_collide_bgk_nc")(*p.L, *args, D(gx), D(gy), D(gz), D(tem0), int(save))
+        else:
+            self._r(p, "ad_collide_bgk_fc")(*p.L, *args, int(save))
 
     def ad_collide_nc(self, p, rho, ux, uy, uz, alpha, nu, q, tem, qx, qy, qz, diff, gx, gy, gz, tem0, save=True,
                       gsnap=None):
@@ -745,10 +760,59 @@ class NaturalConvection2D:
         return out
 
 
+class NaturalConvectionBGK2D:
+    """test/naturalconvection.cpp:30-63: D2Q9 flow and D2Q9 temperature, the BGK-only
+    AD::MacroCollideNaturalConvection (advection.h:616-662) saving every step; mirror walls at y = 0
+    and y = ly-1 and periodic in x for the flow, periodic for the temperature, T = Th at y = 0 and
+    Tl at y = ly-1 (AD::BoundaryConditionSetT), SmoothCorner on both.  The program's 101x51 box and
+    1e5 steps shrink to a small box; gravity is raised so that the layer (Ra ~ 2e4) overturns within
+    the steps run, and a small seeded perturbation of the initial temperature breaks the symmetry
+    (the program starts from T = (Th + Tl)/2 exactly)."""
+
+    def __init__(self, name, lx, ly, nt, nu=0.02, alpha=0.02, gy=1.0e-3, th=2.0, tl=1.0, seed=9):
+        self.name, self.lx, self.ly, self.nt = name, lx, ly, nt
+        self.nu, self.alpha, self.gy, self.th, self.tl = nu, alpha, gy, th, tl
+        self.seed = seed
+
+    def run(self, b):
+        lx, ly, N = self.lx, self.ly, self.lx * self.ly
+        i, j = np.meshgrid(np.arange(lx), np.arange(ly), indexing="xy")
+        i, j = i.reshape(-1), j.reshape(-1)
+        wall = (j == 0) | (j == ly - 1)
+        type_f = np.where(wall, 2, 0).astype(np.int32)
+        type_g = np.zeros(N, dtype=np.int32)
+        tem_b = np.where(j == 0, self.th, np.where(j == ly - 1, self.tl, 0.0))
+        tm = 0.5 * (self.th + self.tl)
+        rng = np.random.default_rng(self.seed)
+        pf = b.lattice(D2Q9, lx, ly)
+        pg = b.lattice(D2Q9, lx, ly)
+        b.put("rho", np.ones(N))
+        b.put("tem", tm + 0.01 * rng.uniform(-1, 1, N))
+        for k in ("ux", "uy", "qx", "qy"):
+            b.zeros(k, N)
+        b.ns_init(pf, "rho", "ux", "uy")
+        b.ad_init(pg, "tem", "ux", "uy")
+        for _ in range(self.nt):
+            b.ad_collide_bgk(pf, "rho", "ux", "uy", None, self.nu, pg, "tem", "qx", "qy", None, self.alpha, 1,
+                             0.0, self.gy, 0.0, tm, True)
+            b.stream(pf)
+            b.stream(pg)
+            b.bc(pf, type_f)
+            b.bc(pg, type_g)
+            b.ad_set_t(pg, wall.astype(np.int32), tem_b, "ux", "uy")
+            b.smooth(pf)
+            b.smooth(pg)
+        out = {"f": b.pops(pf), "g": b.pops(pg)}
+        for k in ("rho", "ux", "uy", "tem", "qx", "qy"):
+            out[k] = b.get(k)
+        return out
+
+
 SEQUENCES = [
     Cavity("cavity_21x21", 21, 21, 300),
     ForcedConvection2D("fc2d_24x16", 24, 16, 200, check=100),
     ForcedConvection3D("fc3d_7x5x5", 7, 5, 5, 60, check=30),
     ForcedConvection2D("fc2d_noswap_20x12", 20, 12, 120, swap=False),
     NaturalConvection2D("nc2d_22x18", 22, 18, 200, check=100),
+    NaturalConvectionBGK2D("ncbgk2d_32x17", 32, 17, 600),
 ]

@@ -70,6 +70,10 @@ void use2(D2Q9<double> &p, D2Q5<double> &q, double *a) {
     AAD::MacroBrinkmanCollideNaturalConvection(p, a, a, a, a, a, a, a, a, a, 0.1, q, a, a, a, a, a, 0.0, -1e-4, true, a);
     AAD::MacroBrinkmanCollideHeatExchange(p, a, a, a, a, a, a, a, a, a, 0.1, q, a, a, a, a, a, 0.05, true);
     AAD::SensitivityHeatExchange(q, a, a, a, a, a, a, a, a, a);
+    AD::MacroCollideForceConvection(p, a, a, a, 0.02, q, a, a, a, 0.02, true);
+    AD::MacroCollideNaturalConvection(p, a, a, a, 0.02, q, a, a, a, 0.02, 0.0, 1.6e-5, 1.5, true);
+    AAD::MacroBrinkmanCollideNaturalConvectionMassFlow(p, a, a, a, a, a, a, a, a, a, 0.1, q, a, a, a, a, a, 0.0, -1e-4,
+                                                       a, a, true, a);
 }
 void use3(D3Q15<double> &p, D3Q7<double> &q, double *a) {
     AD::MacroBrinkmanCollideNaturalConvection(p, a, a, a, a, a, 0.1, q, a, a, a, a, a, 0.0, 0.0, -1e-4, 0.5, true, a);
@@ -78,6 +82,10 @@ void use3(D3Q15<double> &p, D3Q7<double> &q, double *a) {
                                                0.0, -1e-4, true, a);
     AAD::MacroBrinkmanCollideHeatExchange(p, a, a, a, a, a, a, a, a, a, a, a, a, 0.1, q, a, a, a, a, a, a, 0.05, true);
     AAD::SensitivityHeatExchange(q, a, a, a, a, a, a, a, a, a, a, a);
+    AD::MacroCollideForceConvection(p, a, a, a, a, 0.02, q, a, a, a, a, 0.02, true);
+    AD::MacroCollideNaturalConvection(p, a, a, a, a, 0.02, q, a, a, a, a, 0.02, 0.0, 0.0, 1.6e-5, 1.5, true);
+    AAD::MacroBrinkmanCollideNaturalConvectionMassFlow(p, a, a, a, a, a, a, a, a, a, a, a, a, 0.1, q, a, a, a, a, a, a,
+                                                       0.0, 0.0, -1e-4, a, a, a, true, a);
 }
 void use27(D3Q27<double> &p, D3Q7<double> &q, double *a) {
     NS::MacroBrinkmanCollide(p, a, a, a, a, 0.1, a, true);
@@ -92,7 +100,8 @@ int main() { return 0; }
 
 def test_thermal_variants_instantiate(tmp_path):
     """The natural-convection and heat-exchange overloads take the reference's argument lists
-    (advection.h:716-810 and :930-1044, adjointadvection.h:894-993, :1113-1230 and :1445-1457):
+    (advection.h:528-712, :716-810 and :930-1044, adjointadvection.h:894-993, :1113-1358 and
+    :1445-1457; the callers test/naturalconvection.cpp:48 and production/ncpump.cpp:193):
     calls written in those orders compile against the header; so do the 3D calls on D3Q27."""
     src = tmp_path / "thermal.cpp"
     src.write_text(THERMAL_USE)
