@@ -760,6 +760,81 @@ class NaturalConvection2D:
         return out
 
 
+class TransientNaturalConvection2D(NaturalConvection2D):
+    """production/heatsink_transient.cpp:138-222: the transient natural-convection heat sink, its
+    objective (the heat-source edge temperature averaged over every step) and the adjoint run
+    backward in time over the stored forward trajectory, accumulating the heat-source sensitivity
+    at every step.  Geometry, boundaries and coefficients as NaturalConvection2D (the same driver
+    family); D2Q9 for both lattices as in that program; T starts at tem0 = 0.  The program leaves the
+    step-0 populations snapshot gi[0] unset before the adjoint's last step reads it; here it holds
+    the initial populations (AD::InitialCondition's equilibrium)."""
+
+    def run(self, b):
+        lx, ly, N, nt = self.lx, self.ly, self.lx * self.ly, self.nt
+        d = self.setup()
+        pf = b.lattice(D2Q9, lx, ly)
+        pg = b.lattice(D2Q9, lx, ly)
+        for k in ("alpha", "dads", "diff", "dkds"):
+            b.put(k, d[k])
+        b.put("rho_0", np.ones(N))
+        for k in ("ux_0", "uy_0", "tem_0", "qx", "qy"):
+            b.zeros(k, N)
+        tem0 = np.full(N, self.tem0)
+        g = (self.gx, self.gy, 0.0)
+        b.ns_init(pf, "rho_0", "ux_0", "uy_0")
+        b.ad_init(pg, "tem_0", "ux_0", "uy_0")
+        b.put("gi_0", b.pops(pg))
+        for t in range(1, nt):
+            for k in ("rho", "ux", "uy", "tem"):
+                b.zeros("%s_%d" % (k, t), N)
+            b.zeros("gi_%d" % t, (NC[D2Q9], N))
+            ux, uy = "ux_%d" % t, "uy_%d" % t
+            b.ad_collide_nc(pf, "rho_%d" % t, ux, uy, None, "alpha", self.nu, pg, "tem_%d" % t, "qx", "qy", None, "diff",
+                            *g, self.tem0, True, "gi_%d" % t)
+            b.stream(pf)
+            b.stream(pg)
+            b.bc(pf, d["type_f"])
+            b.ad_set_t(pg, d["set_t"], tem0, ux, uy)
+            b.ad_set_q(pg, d["bottom"], d["qn"], ux, uy, None, "diff")
+            b.bc(pg, d["type_g"])
+            b.smooth(pf)
+            b.smooth(pg)
+        out = {"f": b.pops(pf), "g": b.pops(pg)}
+        for k in ("ip", "iux", "iuy", "imx", "imy", "item", "iqx", "iqy", "dfds"):
+            b.zeros(k, N)
+        b.zeros("ig", (NC[D2Q9], N))
+        last = nt - 1
+        b.ans_init(pf, "ux_%d" % last, "uy_%d" % last, None, "ip", "iux", "iuy", None)
+        b.aad_init(pg, "ux_%d" % last, "uy_%d" % last, None, "item", "iqx", "iqy", None)
+        for t in range(nt - 2, -1, -1):
+            ux, uy = "ux_%d" % t, "uy_%d" % t
+            b.aad_collide_nc(pf, "rho_%d" % t, ux, uy, None, "ip", "iux", "iuy", None, "imx", "imy", None, "alpha",
+                             self.nu, pg, "tem_%d" % t, "item", "iqx", "iqy", None, "diff", *g, True, "ig")
+            b.sens_temp_heat_source(pg, "dfds", ux, uy, None, "imx", "imy", None, "dads", "tem_%d" % t, "item", "iqx",
+                                    "iqy", None, "gi_%d" % t, "ig", "diff", "dkds", d["heat_edge"], d["qn"])
+            b.istream(pf)
+            b.istream(pg)
+            b.aad_iset_t(pg, d["set_t"], ux, uy)
+            b.aad_iset_q(pg, d["bottom"], ux, uy)
+            b.aad_iset_q(pg, d["heat_edge"], ux, uy, None, 1.0)
+            b.bc(pg, d["type_g"], adjoint=True)
+            b.bc(pf, d["type_f"], adjoint=True)
+            b.smooth(pf)
+            b.smooth(pg)
+        out.update({"adj_f": b.pops(pf), "adj_g": b.pops(pg), "dfds": b.get("dfds")})
+        for k in ("ip", "iux", "iuy", "imx", "imy", "item", "iqx", "iqy", "ig"):
+            out["adj_" + k] = b.get(k)
+        edge = [i for i in range(lx) if i < self.L]
+        obj = 0.0
+        for t in range(nt):
+            tem = b.get("tem_%d" % t)
+            for i in edge:
+                obj += tem[i]
+        out["objective"] = np.array(obj / (self.L * nt))
+        out["tem_last"] = b.get("tem_%d" % last)
+        return out
+
+
 class NaturalConvectionBGK2D:
     """test/naturalconvection.cpp:30-63: D2Q9 flow and D2Q9 temperature, the BGK-only
     AD::MacroCollideNaturalConvection (advection.h:616-662) saving every step; mirror walls at y = 0
@@ -815,4 +890,5 @@ SEQUENCES = [
     ForcedConvection2D("fc2d_noswap_20x12", 20, 12, 120, swap=False),
     NaturalConvection2D("nc2d_22x18", 22, 18, 200, check=100),
     NaturalConvectionBGK2D("ncbgk2d_32x17", 32, 17, 600),
+    TransientNaturalConvection2D("transient_nc2d_16x12", 16, 12, 60, L=3.0),
 ]
