@@ -196,7 +196,7 @@ def batch_extra(ctx, n, seed=2024):
                     "spread a 1e-6 step-to-step residual leaves (~1e-4 of the slowest mode)"}
 
 
-def warm_extra(ctx, iters, train_steps=20):
+def warm_extra(ctx, iters, train_steps=20, use_graphs=True):
     """The 96x96x48 3D design loop, cold vs warm-started (see the module docstring)."""
     import numpy as np
 
@@ -207,7 +207,7 @@ def warm_extra(ctx, iters, train_steps=20):
            "train_steps_per_iteration": train_steps}
     for mode, prec, learn in (("cold_fp64", F64, None), ("previous_solution_fp64", F64, False),
                               ("learned_fp64", F64, True), ("learned_fp32", F32, True)):
-        hs = Heatsink3D(ctx, lx=96, ly=96, lz=48, plate=6, prec=prec)
+        hs = Heatsink3D(ctx, lx=96, ly=96, lz=48, plate=6, prec=prec, use_graphs=use_graphs)
         if learn is not None:
             hs.attach_warm_start(seed=0, steps=train_steps, learn=learn)
         t0 = time.perf_counter()
@@ -386,7 +386,7 @@ def main():
     if a.batch > 0:
         extra["batch"] = batch_extra(ctx, a.batch)
     if a.warm_iters > 0:
-        extra["warm_start"] = warm_extra(ctx, a.warm_iters)
+        extra["warm_start"] = warm_extra(ctx, a.warm_iters, use_graphs=not a.no_graphs)
     # like-for-like with the reference's lattice pairing (D2Q9 advection-diffusion)
     hs9 = Heatsink2D(ctx, lx=a.lx, ly=a.ly, ad_kind=D2Q9, max_steps=a.max_steps, use_graphs=not a.no_graphs)
     r9 = hs9.evaluate()

@@ -265,6 +265,13 @@ int lbm_heaviside_filter(lbm_context *ctx, int lx, int ly, int lz, double R, dou
                          double *out);
 int lbm_heaviside_sensitivity(lbm_context *ctx, int lx, int ly, int lz, double R, double beta, const double *v,
                               const double *dfdrho, double *dfds);
+/* the same with the design-region weight lambda of production/heatsink.cpp:83-89 and
+ * heatsink3D.cpp:87-93: the hat weight between two nodes of the box [0,mx) x [0,my) x [0,mz), else 1
+ * for a node with itself and 0 otherwise (nodes outside the box keep their own value) */
+int lbm_heaviside_filter_region(lbm_context *ctx, int lx, int ly, int lz, double R, double beta, int mx, int my,
+                                int mz, const double *v, double *out);
+int lbm_heaviside_sensitivity_region(lbm_context *ctx, int lx, int ly, int lz, double R, double beta, int mx, int my,
+                                     int mz, const double *v, const double *dfdrho, double *dfds);
 int lbm_density_filter(lbm_context *ctx, int lx, int ly, int lz, double R, const double *v, double *out);
 
 /* ---- recording and fused steps ---- */

@@ -322,4 +322,29 @@ void ref3_heaviside_sensitivity(int lx, int ly, int lz, double R, double beta, c
     std::memcpy(out, r.data(), sizeof(double) * p.nxyz);
 }
 
+/* HeavisideFilter with the design-region weight lambda of production/heatsink3D.cpp:87-93 */
+static auto region_weight(double R, int mx, int my, int mz) {
+    return [=](int i1, int j1, int k1, int i2, int j2, int k2) {
+        if (i1 < mx && j1 < my && k1 < mz && i2 < mx && j2 < my && k2 < mz) {
+            return (R - sqrt(pow(i1 - i2, 2.0) + pow(j1 - j2, 2.0) + pow(k1 - k2, 2.0))) / R;
+        } else {
+            return (i1 == i2 && j1 == j2 && k1 == k2) ? 1.0 : 0.0;
+        }
+    };
+}
+void ref3_heaviside_value_region(int lx, int ly, int lz, double R, double beta, int mx, int my, int mz,
+                                 const double *v, double *out) {
+    D3Q15<double> p(lx, ly, lz);
+    std::vector<double> in(v, v + p.nxyz);
+    std::vector<double> r = HeavisideFilter::GetFilteredVariable(p, R, beta, in, region_weight(R, mx, my, mz));
+    std::memcpy(out, r.data(), sizeof(double) * p.nxyz);
+}
+void ref3_heaviside_sensitivity_region(int lx, int ly, int lz, double R, double beta, int mx, int my, int mz,
+                                       const double *v, const double *dfdrho, double *out) {
+    D3Q15<double> p(lx, ly, lz);
+    std::vector<double> in(v, v + p.nxyz), d(dfdrho, dfdrho + p.nxyz);
+    std::vector<double> r = HeavisideFilter::GetFilteredSensitivity(p, R, beta, in, d, region_weight(R, mx, my, mz));
+    std::memcpy(out, r.data(), sizeof(double) * p.nxyz);
+}
+
 }  // extern "C"

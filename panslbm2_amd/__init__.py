@@ -43,6 +43,8 @@ def load():
     sig = {
         "lbm_heaviside_filter": [VP, I, I, I, D, D, VP, VP],
         "lbm_heaviside_sensitivity": [VP, I, I, I, D, D, VP, VP, VP],
+        "lbm_heaviside_filter_region": [VP, I, I, I, D, D, I, I, I, VP, VP],
+        "lbm_heaviside_sensitivity_region": [VP, I, I, I, D, D, I, I, I, VP, VP, VP],
         "lbm_mma_create": [VP, I, I, D, VP, VP, VP, VP, VP, ctypes.POINTER(VP)],
         "lbm_mma_destroy": [VP],
         "lbm_mma_set": [VP, ctypes.c_char_p, D],

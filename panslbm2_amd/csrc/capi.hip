@@ -1404,28 +1404,42 @@ static int filter_weights(lbm_context *C, double R, int *nR_out) {
     *nR_out = nR;
     return LBM_OK;
 }
-// HeavisideFilter::GetFilteredVariable with the default weights (heavisidefilter.h:405-585)
-int lbm_heaviside_filter(lbm_context *C, int lx, int ly, int lz, double R, double beta, const double *v, double *out) {
+// HeavisideFilter::GetFilteredVariable (heavisidefilter.h:405-585) with the design-region weights of
+// the drivers (production/heatsink.cpp:83-89, heatsink3D.cpp:87-93); the box [0,mx)x[0,my)x[0,mz)
+int lbm_heaviside_filter_region(lbm_context *C, int lx, int ly, int lz, double R, double beta, int mx, int my, int mz,
+                                const double *v, double *out) {
     if (!C || !v || !out || R <= 0.0 || !(beta > 0.0) || lx < 1 || ly < 1 || lz < 1) return fail(LBM_ERR_ARG, "bad arguments");
+    if (mx < 0 || my < 0 || mz < 0) return fail(LBM_ERR_ARG, "negative design region");
     int nR, rc;
     if ((rc = filter_weights(C, R, &nR)) != LBM_OK) return rc;
-    HIPCHK(lbm::launch_heaviside(v, out, lx, ly, lz, nR, (const double *)C->scratch.p, beta, std::tanh(0.5 * beta),
-                                 C->stream));
+    HIPCHK(lbm::launch_heaviside(v, out, lx, ly, lz, nR, (const double *)C->scratch.p, make_int3(mx, my, mz), beta,
+                                 std::tanh(0.5 * beta), C->stream));
     HIPCHK(hipStreamSynchronize(C->stream));
     return LBM_OK;
 }
-// HeavisideFilter::GetFilteredSensitivity with the default weights (heavisidefilter.h:587-899)
-int lbm_heaviside_sensitivity(lbm_context *C, int lx, int ly, int lz, double R, double beta, const double *v,
-                              const double *dfdrho, double *dfds) {
+// ... with the default weights (heavisidefilter.h:582-585): the whole grid as the region
+int lbm_heaviside_filter(lbm_context *C, int lx, int ly, int lz, double R, double beta, const double *v, double *out) {
+    return lbm_heaviside_filter_region(C, lx, ly, lz, R, beta, lx, ly, lz, v, out);
+}
+// HeavisideFilter::GetFilteredSensitivity (heavisidefilter.h:587-899), design-region weights as above
+int lbm_heaviside_sensitivity_region(lbm_context *C, int lx, int ly, int lz, double R, double beta, int mx, int my,
+                                     int mz, const double *v, const double *dfdrho, double *dfds) {
     if (!C || !v || !dfdrho || !dfds || R <= 0.0 || !(beta > 0.0) || lx < 1 || ly < 1 || lz < 1)
         return fail(LBM_ERR_ARG, "bad arguments");
+    if (mx < 0 || my < 0 || mz < 0) return fail(LBM_ERR_ARG, "negative design region");
     int nR, rc;
     if ((rc = filter_weights(C, R, &nR)) != LBM_OK) return rc;
     if (C->filt.ensure(sizeof(double) * (size_t)lx * ly * lz)) return fail(LBM_ERR_HIP, "scratch allocation failed");
     HIPCHK(lbm::launch_heaviside_sens(v, dfdrho, (double *)C->filt.p, dfds, lx, ly, lz, nR,
-                                      (const double *)C->scratch.p, beta, std::tanh(0.5 * beta), C->stream));
+                                      (const double *)C->scratch.p, make_int3(mx, my, mz), beta,
+                                      std::tanh(0.5 * beta), C->stream));
     HIPCHK(hipStreamSynchronize(C->stream));
     return LBM_OK;
+}
+// ... with the default weights (heavisidefilter.h:897-899)
+int lbm_heaviside_sensitivity(lbm_context *C, int lx, int ly, int lz, double R, double beta, const double *v,
+                              const double *dfdrho, double *dfds) {
+    return lbm_heaviside_sensitivity_region(C, lx, ly, lz, R, beta, lx, ly, lz, v, dfdrho, dfds);
 }
 int lbm_density_filter(lbm_context *C, int lx, int ly, int lz, double R, const double *v, double *out) {
     if (!C || !v || !out || R <= 0.0 || lx < 1 || ly < 1 || lz < 1) return fail(LBM_ERR_ARG, "bad arguments");

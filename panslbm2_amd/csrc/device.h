@@ -115,10 +115,10 @@ hipError_t launch_residual_multi(const ResItem *items, int nitem, double *part, 
 hipError_t launch_normalize(double *v, int n, double *part, double *out, hipStream_t s);
 hipError_t launch_gather_sum(const double *v, const int *idx, int count, double *out, hipStream_t s);
 hipError_t launch_fill(double *p, size_t n, double v, hipStream_t s);
-hipError_t launch_heaviside(const double *v, double *out, int lx, int ly, int lz, int nR, const double *w, double beta,
-                            double tb, hipStream_t s);
+hipError_t launch_heaviside(const double *v, double *out, int lx, int ly, int lz, int nR, const double *w, int3 box,
+                            double beta, double tb, hipStream_t s);
 hipError_t launch_heaviside_sens(const double *v, const double *dfdrho, double *tmp, double *out, int lx, int ly,
-                                 int lz, int nR, const double *w, double beta, double tb, hipStream_t s);
+                                 int lz, int nR, const double *w, int3 box, double beta, double tb, hipStream_t s);
 hipError_t launch_density_filter(const double *v, double *out, int lx, int ly, int lz, int nR, int nd, const double *w,
                                  hipStream_t s);
 int reduction_blocks();

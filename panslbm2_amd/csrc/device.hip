@@ -364,17 +364,22 @@ __global__ void __launch_bounds__(kBlock) k_density_filter(const double *v, doub
 
 // HeavisideFilter (heavisidefilter.h:455-560): the hat-filtered design s~ (the loop order and
 // sums of k_density_filter) projected as 0.5 (tanh(beta/2) + tanh(beta (s~ - 1/2))) / tanh(beta/2);
-// tb = tanh(beta/2), evaluated once on the host.
-__device__ __forceinline__ double hat_mean(const double *v, int lx, int ly, int lz, int nR, const double *w, int n,
-                                           double *wsum) {
+// tb = tanh(beta/2), evaluated once on the host.  box: the design region [0,mx) x [0,my) x [0,mz)
+// of the drivers' weight lambdas (production/heatsink.cpp:83-89, heatsink3D.cpp:87-93): the hat
+// weight when both nodes lie in it, else 1 for the node itself and 0 for the others (the whole
+// grid as the box gives the default weights).
+__device__ __forceinline__ double hat_mean(const double *v, int lx, int ly, int lz, int nR, const double *w, int3 box,
+                                           int n, double *wsum) {
     int i1 = n % lx, j1 = (n / lx) % ly, k1 = n / (lx * ly);
     int W = 2 * nR + 1;
+    const bool in1 = i1 < box.x && j1 < box.y && k1 < box.z;
     double ws = 0.0, acc_ = 0.0;
     for (int i2 = i1 - nR; i2 <= i1 + nR; ++i2)
         for (int j2 = j1 - nR; j2 <= j1 + nR; ++j2)
             for (int k2 = k1 - nR; k2 <= k1 + nR; ++k2) {
                 double wt = w[(i2 - i1 + nR) + W * ((j2 - j1 + nR) + W * (k2 - k1 + nR))];
                 if (wt == -1.0) continue;
+                if (!(in1 && i2 < box.x && j2 < box.y && k2 < box.z)) wt = (i2 == i1 && j2 == j1 && k2 == k1) ? 1.0 : 0.0;
                 if (0 <= i2 && i2 < lx && 0 <= j2 && j2 < ly && 0 <= k2 && k2 < lz) {
                     acc_ += wt * v[i2 + lx * (j2 + ly * k2)];
                     ws += wt;
@@ -384,28 +389,28 @@ __device__ __forceinline__ double hat_mean(const double *v, int lx, int ly, int 
     return acc_;
 }
 __global__ void __launch_bounds__(kBlock) k_heaviside_value(const double *v, double *out, int lx, int ly, int lz,
-                                                            int nR, const double *w, double beta, double tb) {
+                                                            int nR, const double *w, int3 box, double beta, double tb) {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= lx * ly * lz) return;
-    double ws, s = hat_mean(v, lx, ly, lz, nR, w, n, &ws);
+    double ws, s = hat_mean(v, lx, ly, lz, nR, w, box, n, &ws);
     out[n] = 0.5 * (tb + tanh(beta * (s / ws - 0.5))) / tb;
 }
 // GetFilteredSensitivity (heavisidefilter.h:636-875), pass 1: dfds~ = dfdrho * drho/ds~
 __global__ void __launch_bounds__(kBlock) k_heaviside_sens1(const double *v, const double *dfdrho, double *tmp, int lx,
-                                                            int ly, int lz, int nR, const double *w, double beta,
-                                                            double tb) {
+                                                            int ly, int lz, int nR, const double *w, int3 box,
+                                                            double beta, double tb) {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= lx * ly * lz) return;
-    double ws, s = hat_mean(v, lx, ly, lz, nR, w, n, &ws);
+    double ws, s = hat_mean(v, lx, ly, lz, nR, w, box, n, &ws);
     double t = tanh(beta * (s / ws - 0.5));
     tmp[n] = dfdrho[n] * (0.5 * beta * (1.0 - t * t) / tb);
 }
 // pass 2: the hat filter of dfds~, divided by the node's own weight sum (heavisidefilter.h:780-862)
 __global__ void __launch_bounds__(kBlock) k_heaviside_sens2(const double *tmp, double *out, int lx, int ly, int lz,
-                                                            int nR, const double *w) {
+                                                            int nR, const double *w, int3 box) {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= lx * ly * lz) return;
-    double ws, s = hat_mean(tmp, lx, ly, lz, nR, w, n, &ws);
+    double ws, s = hat_mean(tmp, lx, ly, lz, nR, w, box, n, &ws);
     out[n] = s / ws;
 }
 
@@ -595,15 +600,15 @@ hipError_t launch_density_filter(const double *v, double *out, int lx, int ly, i
     LBM_LAUNCH(k_density_filter, nblocks((long)lx * ly * lz), s, v, out, lx, ly, lz, nR, nd, w);
     return hipGetLastError();
 }
-hipError_t launch_heaviside(const double *v, double *out, int lx, int ly, int lz, int nR, const double *w, double beta,
-                            double tb, hipStream_t s) {
-    LBM_LAUNCH(k_heaviside_value, nblocks((long)lx * ly * lz), s, v, out, lx, ly, lz, nR, w, beta, tb);
+hipError_t launch_heaviside(const double *v, double *out, int lx, int ly, int lz, int nR, const double *w, int3 box,
+                            double beta, double tb, hipStream_t s) {
+    LBM_LAUNCH(k_heaviside_value, nblocks((long)lx * ly * lz), s, v, out, lx, ly, lz, nR, w, box, beta, tb);
     return hipGetLastError();
 }
 hipError_t launch_heaviside_sens(const double *v, const double *dfdrho, double *tmp, double *out, int lx, int ly,
-                                 int lz, int nR, const double *w, double beta, double tb, hipStream_t s) {
-    LBM_LAUNCH(k_heaviside_sens1, nblocks((long)lx * ly * lz), s, v, dfdrho, tmp, lx, ly, lz, nR, w, beta, tb);
-    LBM_LAUNCH(k_heaviside_sens2, nblocks((long)lx * ly * lz), s, tmp, out, lx, ly, lz, nR, w);
+                                 int lz, int nR, const double *w, int3 box, double beta, double tb, hipStream_t s) {
+    LBM_LAUNCH(k_heaviside_sens1, nblocks((long)lx * ly * lz), s, v, dfdrho, tmp, lx, ly, lz, nR, w, box, beta, tb);
+    LBM_LAUNCH(k_heaviside_sens2, nblocks((long)lx * ly * lz), s, tmp, out, lx, ly, lz, nR, w, box);
     return hipGetLastError();
 }
 int reduction_blocks() { return kRedBlocks; }

@@ -33,7 +33,7 @@ class Heatsink3D(Heatsink2D):
     def __init__(self, ctx=None, lx=128, ly=128, lz=64, ad_kind=D3Q7, seed=0, nu=0.1, pr=6.0, uin=0.03, heat=1e-3,
                  plate=8, alpha_max=1e4, qf=0.01, qg=1.0, filter_radius=2.0, eps=1e-6, check_every=100,
                  max_steps=400000, use_graphs=True, prec=F64, fp32_floor=None, fused=True, ns_kind=D3Q15,
-                 heaviside_beta=None):
+                 heaviside_beta=None, design_box=None):
         assert ns_kind in (D3Q15, D3Q27)
         self.ctx = ctx or Context(0)
         self.L = self.ctx.lib
@@ -52,6 +52,9 @@ class Heatsink3D(Heatsink2D):
 
         self.plate, self.filter_radius = plate, filter_radius
         self.heaviside_beta = heaviside_beta
+        # the Heaviside filter's design region [0,mx) x [0,my) x [0,mz) (heatsink3D.cpp:87-93); the
+        # whole grid by default (the filter's default weights)
+        self.design_box = tuple(design_box) if design_box is not None else (lx, ly, lz)
         self.alpha_max, self.qf, self.qg = alpha_max, qf, qg
         self.dfl = nu / pr
         self.dso = 10.0 * self.dfl
@@ -102,8 +105,8 @@ class Heatsink3D(Heatsink2D):
         self.x = np.clip(np.asarray(x, dtype=np.float64), 0.0, 1.0).copy()
         d["raw"].upload(self.x)
         if self.heaviside_beta:
-            check(self.L.lbm_heaviside_filter(c.h, lx, ly, lz, D(self.filter_radius), D(self.heaviside_beta),
-                                              d["raw"].ptr, d["design"].ptr))
+            check(self.L.lbm_heaviside_filter_region(c.h, lx, ly, lz, D(self.filter_radius), D(self.heaviside_beta),
+                                                     *self.design_box, d["raw"].ptr, d["design"].ptr))
         else:
             check(self.L.lbm_density_filter(c.h, lx, ly, lz, D(self.filter_radius), d["raw"].ptr,
                                             d["design"].ptr))
@@ -167,10 +170,11 @@ class Heatsink3D(Heatsink2D):
                 d["dfdx"] = c.array((self.N,))
                 d["dgdg"] = c.array((self.N,), data=np.where(self.k >= self.plate, -1.0 / (vf * n), 0.0))
                 d["dgdx"] = c.array((self.N,))
-            check(self.L.lbm_heaviside_sensitivity(c.h, lx, ly, lz, R, beta, d["raw"].ptr, d["dfds"].ptr,
-                                                   d["dfdx"].ptr))
-            check(self.L.lbm_heaviside_sensitivity(c.h, lx, ly, lz, R, beta, d["raw"].ptr, d["dgdg"].ptr,
-                                                   d["dgdx"].ptr))
+            box = self.design_box
+            check(self.L.lbm_heaviside_sensitivity_region(c.h, lx, ly, lz, R, beta, *box, d["raw"].ptr,
+                                                          d["dfds"].ptr, d["dfdx"].ptr))
+            check(self.L.lbm_heaviside_sensitivity_region(c.h, lx, ly, lz, R, beta, *box, d["raw"].ptr,
+                                                          d["dgdg"].ptr, d["dgdx"].ptr))
             dfds = d["dfdx"].download()[region]
             self._mdg.upload(d["dgdx"].download()[region][None])
             g = (1.0 - self.gamma[region]).sum() / (vf * n) - 1.0

@@ -97,6 +97,12 @@ HEAVISIDE_CASES = [  # (name, lx, ly, lz, R, beta, seed)
 ]
 
 
+# (name, lx, ly, lz, R, beta, mx, my, mz, seed): the design-region weights of the drivers
+HEAVISIDE_REGION_CASES = [
+    ("hr2d_14x11", 14, 11, 1, 2.4, 4.0, 9, 7, 1, 44), ("hr3d_8x7x6", 8, 7, 6, 2.4, 8.0, 5, 6, 4, 45),
+]
+
+
 def heaviside_inputs(lx, ly, lz, seed):
     rng = np.random.default_rng(seed)
     n = lx * ly * lz
@@ -118,6 +124,14 @@ def write_heaviside_golden():
         rho, dfds = np.empty_like(s), np.empty_like(s)
         R_.ref3_heaviside_value(lx, ly, lz, D(R), D(beta), p(s), p(rho))
         R_.ref3_heaviside_sensitivity(lx, ly, lz, D(R), D(beta), p(s), p(dfdrho), p(dfds))
+        blob.update({name + "/s": s, name + "/dfdrho": dfdrho, name + "/rho": rho, name + "/dfds": dfds})
+    R_.ref3_heaviside_value_region.argtypes = [I, I, I, D, D, I, I, I, VP, VP]
+    R_.ref3_heaviside_sensitivity_region.argtypes = [I, I, I, D, D, I, I, I, VP, VP, VP]
+    for name, lx, ly, lz, R, beta, mx, my, mz, seed in HEAVISIDE_REGION_CASES:
+        s, dfdrho = heaviside_inputs(lx, ly, lz, seed)
+        rho, dfds = np.empty_like(s), np.empty_like(s)
+        R_.ref3_heaviside_value_region(lx, ly, lz, D(R), D(beta), mx, my, mz, p(s), p(rho))
+        R_.ref3_heaviside_sensitivity_region(lx, ly, lz, D(R), D(beta), mx, my, mz, p(s), p(dfdrho), p(dfds))
         blob.update({name + "/s": s, name + "/dfdrho": dfdrho, name + "/rho": rho, name + "/dfds": dfds})
     path = os.path.join(HERE, "heaviside.npz")
     np.savez_compressed(path, **blob)

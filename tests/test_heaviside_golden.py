@@ -78,3 +78,42 @@ def test_reference_sensitivity_is_the_chain_rule_in_the_interior():
         sm[n] -= h
         fd = (F(sp) - F(sm)) / (2 * h)
         assert abs(fd - dfds[n]) <= 1e-6 * max(1.0, abs(fd)), (i, j, fd, dfds[n])
+
+
+def test_region_fixture_closed_forms():
+    """Design-region weights (production/heatsink3D.cpp:87-93): a node outside the box keeps its own
+    value, projected; its sensitivity is its own chain-rule factor (only it weighs itself)."""
+    b = np.load(BLOB)
+    for name, lx, ly, lz, R, beta, mx, my, mz, seed in mg.HEAVISIDE_REGION_CASES:
+        s, d = b[name + "/s"], b[name + "/dfdrho"]
+        rho, dfds = b[name + "/rho"], b[name + "/dfds"]
+        n = np.arange(lx * ly * lz)
+        i, j, k = n % lx, (n // lx) % ly, n // (lx * ly)
+        out = ~((i < mx) & (j < my) & (k < mz))
+        assert out.any() and (~out).any()
+        tb = np.tanh(0.5 * beta)
+        t = np.tanh(beta * (s - 0.5))
+        np.testing.assert_allclose(rho[out], 0.5 * (tb + t[out]) / tb, rtol=1e-14, atol=1e-15)
+        np.testing.assert_allclose(dfds[out], (d * 0.5 * beta * (1.0 - t * t) / tb)[out], rtol=1e-12, atol=1e-14)
+        assert not np.allclose(rho[~out], (0.5 * (tb + t) / tb)[~out])  # inside, the neighbours count
+
+
+@pytest.mark.skipif(ref() is None, reason="reference harness not built")
+def test_reference_region_reproduces_fixture_and_whole_grid_is_default():
+    R = _bind()
+    VP, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    R.ref3_heaviside_value_region.argtypes = [I, I, I, D, D, I, I, I, VP, VP]
+    R.ref3_heaviside_sensitivity_region.argtypes = [I, I, I, D, D, I, I, I, VP, VP, VP]
+    b = np.load(BLOB)
+    p = lambda a: a.ctypes.data_as(VP)
+    for name, lx, ly, lz, Rr, beta, mx, my, mz, seed in mg.HEAVISIDE_REGION_CASES:
+        s, d = b[name + "/s"].copy(), b[name + "/dfdrho"].copy()
+        rho, dfds = np.empty_like(s), np.empty_like(s)
+        R.ref3_heaviside_value_region(lx, ly, lz, Rr, beta, mx, my, mz, p(s), p(rho))
+        R.ref3_heaviside_sensitivity_region(lx, ly, lz, Rr, beta, mx, my, mz, p(s), p(d), p(dfds))
+        np.testing.assert_array_equal(rho, b[name + "/rho"])
+        np.testing.assert_array_equal(dfds, b[name + "/dfds"])
+        r0, r1 = np.empty_like(s), np.empty_like(s)
+        R.ref3_heaviside_value_region(lx, ly, lz, Rr, beta, lx, ly, lz, p(s), p(r0))
+        R.ref3_heaviside_value(lx, ly, lz, Rr, beta, p(s), p(r1))
+        np.testing.assert_array_equal(r0, r1)
