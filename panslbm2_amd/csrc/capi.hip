@@ -121,6 +121,11 @@ struct lbm_lattice {
     // smoothing lists for the eager path (in-place kernels): edges (3D) then corners
     std::vector<int4> sm_edge, sm_corner;
     int4 *d_sm_edge = nullptr, *d_sm_corner = nullptr;
+    // the heat-source edge operators of lbm_sensitivity_temperature_at_heat_source, kept on the
+    // device while the face arrays they were built from repeat
+    lbm_program *hs_prog = nullptr;
+    std::vector<int> hs_mask;
+    std::vector<double> hs_qn;
     Grid grid() const { return Grid{lx, ly, lz, N}; }
 };
 
@@ -762,6 +767,7 @@ int lbm_lattice_destroy(lbm_lattice *L) {
     (void)hipFree(L->buf[1]);
     if (L->d_sm_edge) (void)hipFree(L->d_sm_edge);
     if (L->d_sm_corner) (void)hipFree(L->d_sm_corner);
+    delete L->hs_prog;
     delete L;
     return LBM_OK;
 }
@@ -1348,15 +1354,31 @@ int lbm_sensitivity_temperature_at_heat_source(lbm_lattice *q, double *dfds, con
     int rc = lbm_sensitivity_brinkman_diffusivity(q, dfds, ux, uy, uz, imx, imy, imz, dads, tem, item, iqx, iqy, iqz, g,
                                                   ig, diff, dkds);
     if (rc) return rc;
-    lbm_program P;
-    P.ctx = q->ctx;
-    rc = prog_record(&P, q, lbm::OP_NONE, mask, nullptr, [&](int t, Op &o) { o.v[0] = qn[t]; });
-    if (rc) return rc;
-    if (P.ops.empty()) return LBM_OK;
-    if ((rc = prog_upload(&P))) return rc;
-    ProgView v = prog_view(&P);
+    // the edge's operator list is built and uploaded once per distinct (mask, qn) face pair and
+    // kept with the lattice, so repeated calls (a transient adjoint calls this every step) launch
+    // without host work or a wait
+    const size_t nf = (size_t)nface_total(q);
+    const bool same = q->hs_prog && q->hs_mask.size() == nf &&
+                      std::memcmp(q->hs_mask.data(), mask, nf * sizeof(int)) == 0 &&
+                      std::memcmp(q->hs_qn.data(), qn, nf * sizeof(double)) == 0;
+    if (!same) {
+        lbm_program *P = new lbm_program;
+        P->ctx = q->ctx;
+        rc = prog_record(P, q, lbm::OP_NONE, mask, nullptr, [&](int t, Op &o) { o.v[0] = qn[t]; });
+        if (!rc && !P->ops.empty()) rc = prog_upload(P);
+        if (rc) {
+            delete P;
+            return rc;
+        }
+        HIPCHK(hipStreamSynchronize(q->ctx->stream));  // launches may still read the previous list
+        delete q->hs_prog;
+        q->hs_prog = P;
+        q->hs_mask.assign(mask, mask + nf);
+        q->hs_qn.assign(qn, qn + nf);
+    }
+    if (q->hs_prog->ops.empty()) return LBM_OK;
+    ProgView v = prog_view(q->hs_prog);
     HIPCHK(lbm::launch_sens_hs_boundary(dfds, q->N, v, ux, uy, ig, diff, dkds, q->ctx->stream));
-    HIPCHK(hipStreamSynchronize(q->ctx->stream));
     return LBM_OK;
 }
 
