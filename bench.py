@@ -31,6 +31,10 @@ Also reported:
                    cores predicting the change of the fields, trained beside the solves), and the
                    learned warm start with FP32 mixed-precision storage; wall seconds and LBM steps
                    per design iteration after the first, every solve to the 1e-6 residual.
+  transient     -- the transient natural-convection heat sink of the reference's transient
+                   program (141 x 161, 2000 steps forward and backward, the adjoint over the stored
+                   trajectory): seconds per evaluation with the trajectory resident in HBM and with
+                   checkpoints + recomputed windows.
 """
 import argparse
 import ctypes
@@ -196,6 +200,23 @@ def batch_extra(ctx, n, seed=2024):
                     "spread a 1e-6 step-to-step residual leaves (~1e-4 of the slowest mode)"}
 
 
+def transient_extra(ctx, nt=2000):
+    """panslbm2_amd/transient2d.py on the reference program's grid (second evaluation timed)."""
+    from panslbm2_amd.transient2d import TransientHeatsink2D
+
+    out = {"grid": [141, 161], "steps": nt, "lattices": "D2Q9 NS + D2Q9 AD", "precision": "fp64"}
+    for storage in ("full", "checkpoint"):
+        drv = TransientHeatsink2D(ctx, 141, 161, nt, seed=1, storage=storage)
+        drv.evaluate()
+        t0 = time.perf_counter()
+        f, _ = drv.evaluate()
+        dt = time.perf_counter() - t0
+        out[storage] = {"s_per_evaluation": round(dt, 4), "window": drv.K,
+                        "mlups": round(drv.N * (2 * nt - 3) / dt / 1e6, 1), "objective": f}
+        drv.close()
+    return out
+
+
 def warm_extra(ctx, iters, train_steps=20, use_graphs=True):
     """The 96x96x48 3D design loop, cold vs warm-started (see the module docstring)."""
     import numpy as np
@@ -322,6 +343,7 @@ def main():
     ap.add_argument("--max-steps", type=int, default=400000, help="cap on forward and on adjoint steps")
     ap.add_argument("--no-graphs", action="store_true", help="launch every step (for profilers)")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-transient", action="store_true")
     ap.add_argument("--batch", type=int, default=64, help="designs in the batched evaluation (0: skip it)")
     ap.add_argument("--warm-iters", type=int, default=10, help="3D design-loop iterations per warm-start mode (0: skip)")
     ap.add_argument("--kernel-only", choices=["fwd", "adj", "stream", "unet"],
@@ -387,6 +409,8 @@ def main():
         extra["batch"] = batch_extra(ctx, a.batch)
     if a.warm_iters > 0:
         extra["warm_start"] = warm_extra(ctx, a.warm_iters, use_graphs=not a.no_graphs)
+    if not a.no_transient:
+        extra["transient"] = transient_extra(ctx)
     # like-for-like with the reference's lattice pairing (D2Q9 advection-diffusion)
     hs9 = Heatsink2D(ctx, lx=a.lx, ly=a.ly, ad_kind=D2Q9, max_steps=a.max_steps, use_graphs=not a.no_graphs)
     r9 = hs9.evaluate()
