@@ -20,7 +20,12 @@ Both give the same values: the recomputed steps repeat the same launches on the 
 
 Each step is one fused launch (stream, boundary program, corner smoothing and collide; the
 boundary calls read the previous step's velocity through the program's field binding), and the
-adjoint step adds the sensitivity launch.  FP64 storage (the reference's transient is FP64).
+adjoint step adds the sensitivity launches.  use_graphs=True (trajectory resident, nt even)
+captures chunks of steps once as HIP graphs and replays them in every later evaluation (the slots'
+addresses do not change; a new design changes only the coefficient arrays' contents); on the
+program's 141 x 161 grid it measured the same time as plain launches, the step being device time
+(a latency-bound fused launch plus the sensitivity kernels), so it is off by default.  FP64
+storage (the reference's transient is FP64).
 
 Differences from the reference program: the design s is taken as given (the program builds it with
 its Heaviside filter, available as lbm_heaviside_filter); the snapshot of step 0, which the
@@ -38,7 +43,8 @@ VP = ctypes.c_void_p
 
 class TransientHeatsink2D:
     def __init__(self, ctx=None, lx=141, ly=161, nt=1000, s=None, seed=0, nu=0.1, pr=6.0, ra=2e5, qn=1e-2, L=4.0,
-                 alpha_max=1e4, qf=1e-2, qg=1.0, tem0=0.0, gx=0.0, gy=None, storage="full", window=None):
+                 alpha_max=1e4, qf=1e-2, qg=1.0, tem0=0.0, gx=0.0, gy=None, storage="full", window=None,
+                 use_graphs=False, chunk=100):
         if storage not in ("full", "checkpoint"):
             raise ValueError("storage is 'full' or 'checkpoint'")
         self.ctx = ctx or Context(0)
@@ -95,6 +101,11 @@ class TransientHeatsink2D:
         self.fwd_prog = self._record_forward()
         self.adj_prog = self._record_adjoint()
         self.objective = None
+        # a replayed graph expects each lattice's current buffer to be the one it was captured with:
+        # each pass makes nt - 2 buffer swaps (its first step collides in place), so an evaluation
+        # leaves the buffers where it found them when nt is even
+        self.use_graphs, self.chunk = use_graphs and storage == "full" and nt % 2 == 0, chunk
+        self._graphs = {}
 
     # ---- addresses --------------------------------------------------------------------------
     def _at(self, arr, row, width=None):
@@ -172,10 +183,30 @@ class TransientHeatsink2D:
             prog, self.pa.h, self._slot("rho", t), ux, uy, None, d["ip"].ptr, d["iux"].ptr, d["iuy"].ptr, None,
             d["imx"].ptr, d["imy"].ptr, None, d["alpha"].ptr, D(self.nu), self.pb.h, tem, d["item"].ptr,
             d["iqx"].ptr, d["iqy"].ptr, None, d["diff"].ptr, D(self.gx), D(self.gy), D(0.0), 1, d["ig"].ptr))
+        self._sens(t)
+
+    def _sens(self, t):
+        L, d = self.Lb, self.d
         check(L.lbm_sensitivity_temperature_at_heat_source(
-            self.pb.h, d["dfds"].ptr, ux, uy, None, d["imx"].ptr, d["imy"].ptr, None, d["dads"].ptr, tem,
-            d["item"].ptr, d["iqx"].ptr, d["iqy"].ptr, None, self._slot("gi", t), d["ig"].ptr, d["diff"].ptr,
-            d["dkds"].ptr, hptr(self.faces["heat_edge"]), hptr(self.faces["qn"])))
+            self.pb.h, d["dfds"].ptr, self._slot("ux", t), self._slot("uy", t), None, d["imx"].ptr, d["imy"].ptr, None,
+            d["dads"].ptr, self._slot("tem", t), d["item"].ptr, d["iqx"].ptr, d["iqy"].ptr, None, self._slot("gi", t),
+            d["ig"].ptr, d["diff"].ptr, d["dkds"].ptr, hptr(self.faces["heat_edge"]), hptr(self.faces["qn"])))
+
+    def _run(self, key, steps):
+        """Run the callables in steps; with graphs, as one graph captured the first time."""
+        if not self.use_graphs:
+            for s in steps:
+                s()
+            return
+        g = self._graphs.get(key)
+        if g is None:
+            check(self.Lb.lbm_graph_begin(self.ctx.h))
+            for s in steps:
+                s()
+            g = VP()
+            check(self.Lb.lbm_graph_end(self.ctx.h, ctypes.byref(g)))
+            self._graphs[key] = g
+        check(self.Lb.lbm_graph_launch(g))
 
     # ---- checkpoints ------------------------------------------------------------------------
     def _fields(self):
@@ -222,11 +253,17 @@ class TransientHeatsink2D:
         if self.nck:
             self._save(0, 0)
         self._copy(self._at(self.edge_t, 0, ne), self._slot("tem", 0), ne)
-        for t in range(1, self.nt):
-            self._fwd_step(t)
-            self._copy(self._at(self.edge_t, t, ne), self._slot("tem", t), ne)
-            if self.nck and t % self.K == 0:
-                self._save(t // self.K, t)
+
+        def step(t):
+            def run():
+                self._fwd_step(t)
+                self._copy(self._at(self.edge_t, t, ne), self._slot("tem", t), ne)
+                if self.nck and t % self.K == 0:
+                    self._save(t // self.K, t)
+            return run
+
+        for t0 in range(1, self.nt, self.chunk):
+            self._run(("f", t0), [step(t) for t in range(t0, min(t0 + self.chunk, self.nt))])
         edge = self.edge_t.download()
         f = 0.0
         for t in range(self.nt):  # heatsink_transient.cpp:196-207, in its order
@@ -238,14 +275,19 @@ class TransientHeatsink2D:
     def adjoint(self):
         """The adjoint pass backward in time, accumulating dfds (the unfiltered sensitivity)."""
         L, d, nt = self.Lb, self.d, self.nt
+        if self.use_graphs and nt >= 2:
+            # the heat-source edge operators are built on the first call: make it here, outside any
+            # capture (dfds is cleared just below)
+            self._sens(0)
         for k in ("ip", "iux", "iuy", "item", "iqx", "iqy", "dfds"):
             d[k].fill(0.0)
         ux, uy = self._slot("ux", nt - 1), self._slot("uy", nt - 1)
         check(L.lbm_ans_initial_condition(self.pa.h, ux, uy, None, d["ip"].ptr, d["iux"].ptr, d["iuy"].ptr, None))
         check(L.lbm_aad_initial_condition(self.pb.h, ux, uy, None, d["item"].ptr, d["iqx"].ptr, d["iqy"].ptr, None))
         if not self.nck:
-            for t in range(nt - 2, -1, -1):
-                self._adj_step(t)
+            for t0 in range(nt - 2, -1, -self.chunk):
+                self._run(("a", t0), [(lambda t: (lambda: self._adj_step(t)))(t)
+                                      for t in range(t0, max(t0 - self.chunk, -1), -1)])
             return d["dfds"]
         for c in range(self.nck - 1, -1, -1):
             t0 = c * self.K
@@ -266,6 +308,9 @@ class TransientHeatsink2D:
         return self.objective, self.d["dfds"].download()
 
     def close(self):
+        for g in self._graphs.values():
+            self.Lb.lbm_graph_destroy(g)
+        self._graphs = {}
         for p in (self.fwd_prog, self.adj_prog):
             if p:
                 self.Lb.lbm_program_destroy(p)

@@ -50,17 +50,21 @@ def _check(drv, tem_last):
     np.testing.assert_array_equal(tem_last, blob["tem_last"])
 
 
-@pytest.mark.parametrize("storage,window", [("full", None), ("checkpoint", None), ("checkpoint", 2),
-                                            ("checkpoint", 13), ("checkpoint", 59)])
-def test_transient_adjoint_matches_reference(ctx, storage, window):
-    print("transient", storage, window, flush=True)
-    drv = _driver(ctx, storage=storage, window=window)
-    drv.forward()
-    ctx.sync()
-    tem_last = drv.slots["tem"].download()[(drv.nt - 1) % (drv.nt if storage == "full" else drv.K + 1)]
-    drv.adjoint()
-    ctx.sync()
-    _check(drv, tem_last)
+@pytest.mark.parametrize("storage,window,graphs", [("full", None, True), ("full", None, False),
+                                                   ("checkpoint", None, False), ("checkpoint", 2, False),
+                                                   ("checkpoint", 13, False), ("checkpoint", 59, False)])
+def test_transient_adjoint_matches_reference(ctx, storage, window, graphs):
+    """Twice in a row (the second evaluation replays the graphs captured by the first)."""
+    print("transient", storage, window, "graphs" if graphs else "", flush=True)
+    drv = _driver(ctx, storage=storage, window=window, use_graphs=graphs, chunk=16)
+    assert drv.use_graphs == graphs
+    for _ in range(2):
+        drv.forward()
+        ctx.sync()
+        tem_last = drv.slots["tem"].download()[(drv.nt - 1) % (drv.nt if storage == "full" else drv.K + 1)]
+        drv.adjoint()
+        ctx.sync()
+        _check(drv, tem_last)
     drv.close()
 
 
