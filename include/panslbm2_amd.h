@@ -240,6 +240,16 @@ int lbm_sensitivity_brinkman_diffusivity(lbm_lattice *q, double *dfds, const dou
                                          const double *dads, const double *tem, const double *item, const double *iqx,
                                          const double *iqy, const double *iqz, const double *g, const double *ig,
                                          const double *diffusivity, const double *dkds); /* adjointadvection.h:1461-1493 */
+/* No reference counterpart: the design derivative of the volumetric heat source Q(gamma) that the
+ * forced-convection collide's `heat` argument adds (w_c Q into g_c before the re-macro).  The
+ * collide's output moves with Q as w_c (1 + 3 omega_g c.u); contracted with the adjoint populations
+ * in the sign convention of the diffusivity term of SensitivityBrinkmanDiffusivity
+ * (adjointadvection.h:1461-1493), which is also how SensitivityHeatExchange (:1445-1457) treats its
+ * design-dependent source:  dfds += -dhds (item + 3 omega_g u.iq), omega_g = 1/(3 D + 1/2).
+ * item/iq are the adjoint collide's saved fields; dhds = dQ/dgamma per node. */
+int lbm_sensitivity_heat_source(lbm_lattice *q, double *dfds, const double *ux, const double *uy, const double *uz,
+                                const double *item, const double *iqx, const double *iqy, const double *iqz,
+                                const double *diffusivity, const double *dhds);
 /* adjointadvection.h:1497-1517 (2D); mask_face / qn_face tabulate the reference's bctype / qnbc lambdas */
 int lbm_sensitivity_temperature_at_heat_source(lbm_lattice *q, double *dfds, const double *ux, const double *uy,
                                                const double *uz, const double *imx, const double *imy,

@@ -1295,6 +1295,35 @@ void SensitivityHeatExchange(Q<T> &_q, T *_dfds, const T *_ux, const T *_uy, con
                "AAD::SensitivityHeatExchange");
     rt.wrote({_dfds}, true);
 }
+// No reference counterpart: the design derivative of the volumetric source that the _heat argument
+// of AD::MacroBrinkmanCollideForceConvection adds (lbm_sensitivity_heat_source):
+// dfds += -dhds (item + 3 omega_g u.iq), omega_g = 1/(3 D + 1/2).
+template <class T, template <class> class Q>
+void SensitivityHeatSource(Q<T> &_q, T *_dfds, const T *_ux, const T *_uy, const T *_item, const T *_iqx,
+                           const T *_iqy, const T *_diffusivity, const T *_dhds) {
+    static_assert(Q<T>::nd == 2, "2D overload");
+    gpu::Runtime &rt = gpu::Runtime::get();
+    size_t n = gpu::N(_q);
+    double *dfds = rt.inout(_dfds, n);
+    gpu::check(lbm_sensitivity_heat_source(_q.handle(), dfds, rt.in(_ux, n), rt.in(_uy, n), nullptr, rt.in(_item, n),
+                                           rt.in(_iqx, n), rt.in(_iqy, n), nullptr, rt.in(_diffusivity, n),
+                                           rt.in(_dhds, n)),
+               "AAD::SensitivityHeatSource");
+    rt.wrote({_dfds}, true);
+}
+template <class T, template <class> class Q>
+void SensitivityHeatSource(Q<T> &_q, T *_dfds, const T *_ux, const T *_uy, const T *_uz, const T *_item,
+                           const T *_iqx, const T *_iqy, const T *_iqz, const T *_diffusivity, const T *_dhds) {
+    static_assert(Q<T>::nd == 3, "3D overload");
+    gpu::Runtime &rt = gpu::Runtime::get();
+    size_t n = gpu::N(_q);
+    double *dfds = rt.inout(_dfds, n);
+    gpu::check(lbm_sensitivity_heat_source(_q.handle(), dfds, rt.in(_ux, n), rt.in(_uy, n), rt.in(_uz, n),
+                                           rt.in(_item, n), rt.in(_iqx, n), rt.in(_iqy, n), rt.in(_iqz, n),
+                                           rt.in(_diffusivity, n), rt.in(_dhds, n)),
+               "AAD::SensitivityHeatSource");
+    rt.wrote({_dfds}, true);
+}
 // :894-940
 template <class T, template <class> class P, template <class> class Q>
 void MacroBrinkmanCollideHeatExchange(P<T> &_p, const T *_rho, const T *_ux, const T *_uy, T *_ip, T *_iux, T *_iuy,

@@ -1626,6 +1626,19 @@ void ol_sens_temp_heat_source(const ol_lattice *q, double *dfds, const double *u
     });
 }
 
+/* volumetric heat-source design derivative (no reference counterpart; see the header) */
+void ol_sens_heat_source(const ol_lattice *q, double *dfds, const double *ux, const double *uy, const double *uz,
+                         const double *item, const double *iqx, const double *iqy, const double *iqz,
+                         const double *diff, const double *dhds) {
+    bool d3 = q->d->nd == 3;
+    for (int idx = 0; idx < q->N; ++idx) {
+        double omegag = 1.0 / (3.0 * diff[idx] + 0.5);
+        double uq = !d3 ? ux[idx] * iqx[idx] + uy[idx] * iqy[idx]
+                        : ux[idx] * iqx[idx] + uy[idx] * iqy[idx] + uz[idx] * iqz[idx];
+        dfds[idx] += -dhds[idx] * (item[idx] + 3.0 * omegag * uq);
+    }
+}
+
 /* residual.h:9-51 */
 double ol_residual(int ncomp, const double *a0, const double *a1, const double *a2, const double *b0, const double *b1,
                    const double *b2, int n) {

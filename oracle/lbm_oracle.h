@@ -139,6 +139,16 @@ void ol_sens_temp_heat_source(const ol_lattice *q, double *dfds, const double *u
                               const double *iqz, const double *g, const double *ig, const double *diff,
                               const double *dkds, const int *mask, const double *qn);
 
+/* No reference counterpart: design derivative of the volumetric heat source that ol_ad_collide_fc's
+ * `heat` argument adds (g_c += w_c Q before the re-macro).  The collide's output depends on Q
+ * through d(g_c^out)/dQ = w_c (1 + 3 omega_g c.u); contracted with the adjoint populations in the
+ * reference's sign convention (the one SensitivityBrinkmanDiffusivity, adjointadvection.h:1461-1493,
+ * uses for its diffusivity term) this is
+ *     dfds += -dhds (item + 3 omega_g (u.iq)),   omega_g = 1/(3 D + 1/2). */
+void ol_sens_heat_source(const ol_lattice *q, double *dfds, const double *ux, const double *uy, const double *uz,
+                         const double *item, const double *iqx, const double *iqy, const double *iqz,
+                         const double *diff, const double *dhds);
+
 /* residual.h:9-51 (ncomp = 1, 2, 3), normalize.h:9-24, densityfilter.h:390-420,512-515 */
 double ol_residual(int ncomp, const double *a0, const double *a1, const double *a2,
                    const double *b0, const double *b1, const double *b2, int n);

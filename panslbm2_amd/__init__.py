@@ -125,6 +125,7 @@ def load():
         "lbm_aad_macro_brinkman_collide_heat_exchange": [VP] * 13 + [D] + [VP] * 7 + [D, I],
         "lbm_aad_macro_brinkman_collide_heat_exchange_p": [VP] * 14 + [D] + [VP] * 7 + [D, I],
         "lbm_sensitivity_heat_exchange": [VP] * 12,
+        "lbm_sensitivity_heat_source": [VP] * 11,
         "lbm_ad_set_t": [VP] * 6,
         "lbm_ad_set_q": [VP] * 7 + [D],
         "lbm_ans_initial_condition": [VP] * 8,

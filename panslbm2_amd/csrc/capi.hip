@@ -1341,6 +1341,14 @@ int lbm_sensitivity_brinkman_diffusivity(lbm_lattice *q, double *dfds, const dou
     const double *a[16] = {ux, uy, uz, imx, imy, imz, dads, tem, item, iqx, iqy, iqz, g, ig, diff, dkds};
     return hip_result(lbm::launch_sens_bd(q->kind, dfds, q->N, a, q->ctx->stream), "sensitivity brinkman diffusivity");
 }
+int lbm_sensitivity_heat_source(lbm_lattice *q, double *dfds, const double *ux, const double *uy, const double *uz,
+                                const double *item, const double *iqx, const double *iqy, const double *iqz,
+                                const double *diff, const double *dhds) {
+    if (!q || !fields_ok({dfds, ux, uy, item, iqx, iqy, diff, dhds}) || (q->nd == 3 && (!uz || !iqz)))
+        return fail(LBM_ERR_ARG, "bad arguments");
+    const double *a[9] = {ux, uy, uz, item, iqx, iqy, iqz, diff, dhds};
+    return hip_result(lbm::launch_sens_hsrc(q->nd, dfds, q->N, a, q->ctx->stream), "sensitivity heat source");
+}
 int lbm_sensitivity_temperature_at_heat_source(lbm_lattice *q, double *dfds, const double *ux, const double *uy,
                                                const double *uz, const double *imx, const double *imy,
                                                const double *imz, const double *dads, const double *tem,

@@ -100,6 +100,7 @@ hipError_t launch_encode(int kind, int prec, const double *src, void *dst, int N
 hipError_t launch_decode(int kind, int prec, const void *src, double *dst, int N, const Enc &e, hipStream_t s);
 hipError_t launch_sens_brinkman(int nd, double *dfds, int N, const double *const *p, hipStream_t s);
 hipError_t launch_sens_he(int nd, double *dfds, int N, const double *const *p, hipStream_t s);
+hipError_t launch_sens_hsrc(int nd, double *dfds, int N, const double *const *p, hipStream_t s);
 hipError_t launch_sens_bd(int kind, double *dfds, int N, const double *const *p, hipStream_t s);
 hipError_t launch_sens_hs_boundary(double *dfds, int N, const ProgView &pv, const double *ux, const double *uy,
                                    const double *ig, const double *diff, const double *dkds, hipStream_t s);

@@ -144,6 +144,23 @@ __global__ void __launch_bounds__(kBlock) k_sens_he(double *dfds, int N, const d
     dfds[n] += 3.0 * dads[n] * um - dbds[n] * (1.0 - tem[n]) * (1.0 + item[n]);
 }
 
+// Design derivative of the volumetric heat source Q(gamma) that the forced-convection collide adds
+// as g_c += w_c Q (no reference counterpart): dfds += -dhds (item + 3 omega_g u.iq),
+// omega_g = 1/(3D + 1/2), in the sign convention of the diffusivity term below.
+template <int ND>
+__global__ void __launch_bounds__(kBlock) k_sens_hsrc(double *dfds, int N, const double *ux, const double *uy,
+                                                      const double *uz, const double *item, const double *iqx,
+                                                      const double *iqy, const double *iqz, const double *diff,
+                                                      const double *dhds) {
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    double uq;
+    if constexpr (ND == 2) uq = ux[n] * iqx[n] + uy[n] * iqy[n];
+    else uq = ux[n] * iqx[n] + uy[n] * iqy[n] + uz[n] * iqz[n];
+    double omegag = 1.0 / (3.0 * diff[n] + 0.5);
+    dfds[n] += -dhds[n] * (item[n] + 3.0 * omegag * uq);
+}
+
 template <class L>
 __global__ void __launch_bounds__(kBlock) k_sens_bd(double *dfds, int N, const double *ux, const double *uy,
                                                     const double *uz, const double *imx, const double *imy,
@@ -547,6 +564,13 @@ hipError_t launch_sens_he(int nd, double *dfds, int N, const double *const *p, h
         LBM_LAUNCH((k_sens_he<2>), nblocks(N), s, dfds, N, p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], p[9]);
     else
         LBM_LAUNCH((k_sens_he<3>), nblocks(N), s, dfds, N, p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], p[9]);
+    return hipGetLastError();
+}
+hipError_t launch_sens_hsrc(int nd, double *dfds, int N, const double *const *p, hipStream_t s) {
+    if (nd == 2)
+        LBM_LAUNCH((k_sens_hsrc<2>), nblocks(N), s, dfds, N, p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8]);
+    else
+        LBM_LAUNCH((k_sens_hsrc<3>), nblocks(N), s, dfds, N, p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8]);
     return hipGetLastError();
 }
 hipError_t launch_sens_bd(int kind, double *dfds, int N, const double *const *p, hipStream_t s) {

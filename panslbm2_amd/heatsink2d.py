@@ -65,6 +65,7 @@ class Heatsink2D:
             "diff": dso + (dfl - dso) * s * (1.0 + qg) / (s + qg),
             "dkds": (dfl - dso) * qg * (1.0 + qg) / (s + qg) ** 2,
             "heat": heat * (1.0 - s),
+            "dhds": np.full(N, -heat),
         }
         self.d = {k: c.array((N,), data=v) for k, v in coef.items()}
         # fields: saved values alternate between two buffers per quantity (u at t and t-1)
@@ -299,7 +300,8 @@ class Heatsink2D:
         return t
 
     def sensitivity(self):
-        """dF/dgamma: Brinkman and diffusivity terms (adjointadvection.h:1461-1475)."""
+        """dF/dgamma: Brinkman and diffusivity terms (adjointadvection.h:1461-1475) plus the design
+        derivative of the volumetric source Q(1 - gamma) (lbm_sensitivity_heat_source)."""
         L, d = self.L, self.d
         fb, ab = self.fwd_buf, self.adj_buf
         d["dfds"].fill(0.0)
@@ -307,6 +309,9 @@ class Heatsink2D:
             self.pg.h, d["dfds"].ptr, d["ux%d" % fb].ptr, d["uy%d" % fb].ptr, None, d["imx"].ptr, d["imy"].ptr, None,
             d["dads"].ptr, d["tem"].ptr, d["item"].ptr, d["iqx%d" % ab].ptr, d["iqy%d" % ab].ptr, None, d["g"].ptr,
             d["ig"].ptr, d["diff"].ptr, d["dkds"].ptr))
+        check(L.lbm_sensitivity_heat_source(
+            self.pg.h, d["dfds"].ptr, d["ux%d" % fb].ptr, d["uy%d" % fb].ptr, None, d["item"].ptr,
+            d["iqx%d" % ab].ptr, d["iqy%d" % ab].ptr, None, d["diff"].ptr, d["dhds"].ptr))
         out = ctypes.c_double()
         check(L.lbm_gather_sum(self.ctx.h, d["tem"].ptr, hptr(self.obj_nodes), len(self.obj_nodes), ctypes.byref(out)))
         self.objective = out.value / len(self.obj_nodes)

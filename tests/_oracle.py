@@ -87,6 +87,7 @@ def lib():
         L.ol_sens_heat_exchange.argtypes = [VP] * 12
         L.ol_sens_brinkman_diffusivity.argtypes = [VP] * 18
         L.ol_sens_temp_heat_source.argtypes = [VP] * 20
+        L.ol_sens_heat_source.argtypes = [VP] * 11
         L.ol_residual.argtypes = [I] + [VP] * 6 + [I]
         L.ol_residual.restype = D
         L.ol_normalize.argtypes = [VP, I]

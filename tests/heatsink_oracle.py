@@ -104,6 +104,9 @@ def run(lx, ly, ad_kind, seed=0, nu=0.1, pr=6.0, umax=0.05, heat=1e-3, alpha_max
     O.ol_sens_brinkman_diffusivity(pg.h, ptr(dfds), ptr(ux), ptr(uy), None, ptr(imx), ptr(imy), None, ptr(dads),
                                    ptr(tem), ptr(item), ptr(IQ[ab][0]), ptr(IQ[ab][1]), None, ptr(g), ptr(ig),
                                    ptr(diff), ptr(dkds))
+    dhds = np.full(N, -heat)
+    O.ol_sens_heat_source(pg.h, ptr(dfds), ptr(ux), ptr(uy), None, ptr(item), ptr(IQ[ab][0]), ptr(IQ[ab][1]), None,
+                          ptr(diff), ptr(dhds))
     objsum = 0.0
     for n in np.nonzero(objedge)[0]:
         objsum += tem[n]
