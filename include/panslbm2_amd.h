@@ -250,7 +250,17 @@ int lbm_sensitivity_brinkman_diffusivity(lbm_lattice *q, double *dfds, const dou
 int lbm_sensitivity_heat_source(lbm_lattice *q, double *dfds, const double *ux, const double *uy, const double *uz,
                                 const double *item, const double *iqx, const double *iqy, const double *iqz,
                                 const double *diffusivity, const double *dhds);
-/* adjointadvection.h:1497-1517 (2D); mask_face / qn_face tabulate the reference's bctype / qnbc lambdas */
+/* AAD::SensitivityTemperatureAtHeatSource, adjointadvection.h:1497-1543 (D2Q9 :1497-1517 with the
+ * edge terms :755-801; D3Q15 :1519-1543 with the face terms :805-888), on D2Q9, D2Q5, D3Q15 and
+ * D3Q7; mask_face / qn_face tabulate the reference's bctype / qnbc lambdas.  The plain 3D
+ * reference passes (ux, uy, uz, ig) to helpers declared (ux, uy, ig, uz) (:1537-1542), reading ig
+ * out of bounds; this follows the declared order, as its AVX overloads do
+ * (equation_avx/adjointadvection_avx.h:1507-1512).  On D2Q5 / D3Q7 (no reference counterpart) the
+ * face term is the same construction: -dkds sum_c (ig_c - 1) d(SetQ)_c/dD.  This entry point
+ * reproduces the reference's D3Q15 arithmetic including its missing factor 3 on the tangential
+ * terms of ymax, zmin and zmax (:851-852, 874-875, 880-881); the _c form with compat = 0 uses the
+ * factor on every face.  The per-face operator table is uploaded once per distinct mask/qn pair;
+ * changing the pair while the context stream is being captured returns LBM_ERR_STATE. */
 int lbm_sensitivity_temperature_at_heat_source(lbm_lattice *q, double *dfds, const double *ux, const double *uy,
                                                const double *uz, const double *imx, const double *imy,
                                                const double *imz, const double *dads, const double *tem,
@@ -258,6 +268,13 @@ int lbm_sensitivity_temperature_at_heat_source(lbm_lattice *q, double *dfds, con
                                                const double *iqz, const double *g, const double *ig,
                                                const double *diffusivity, const double *dkds, const int *mask_face,
                                                const double *qn_face);
+int lbm_sensitivity_temperature_at_heat_source_c(lbm_lattice *q, double *dfds, const double *ux, const double *uy,
+                                                 const double *uz, const double *imx, const double *imy,
+                                                 const double *imz, const double *dads, const double *tem,
+                                                 const double *item, const double *iqx, const double *iqy,
+                                                 const double *iqz, const double *g, const double *ig,
+                                                 const double *diffusivity, const double *dkds,
+                                                 const int *mask_face, const double *qn_face, int compat);
 
 /* ---- reductions (FP64, deterministic fixed-order trees) ---- */
 /* Residual, residual.h:9-51: sqrt(sum|a-b|^2 / sum|a|^2) over ncomp (1..3) components */

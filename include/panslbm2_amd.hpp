@@ -1429,6 +1429,32 @@ void SensitivityTemperatureAtHeatSource(Q<T> &_q, T *_dfds, const T *_ux, const 
                "AAD::SensitivityTemperatureAtHeatSource");
     rt.wrote({_dfds}, true);
 }
+// :1519-1543 with the face terms of :805-888 taken in their declared argument order (the plain
+// reference passes them out of order; its AVX overloads do not)
+template <class T, template <class> class Q, class Fv, class Ff>
+void SensitivityTemperatureAtHeatSource(Q<T> &_q, T *_dfds, const T *_ux, const T *_uy, const T *_uz,
+                                        const T *_imx, const T *_imy, const T *_imz, const T *_dads, const T *_tem,
+                                        const T *_item, const T *_iqx, const T *_iqy, const T *_iqz, const T *_g,
+                                        const T *_ig, const T *_diffusivity, const T *_dkds, Fv _qnbc,
+                                        Ff _bctype) {
+    static_assert(Q<T>::nd == 3, "3D overload");
+    gpu::Runtime &rt = gpu::Runtime::get();
+    rt.flush();
+    size_t n = gpu::N(_q), ng = n * Q<T>::nc;
+    std::vector<int> m = _q.faces_int(_bctype);
+    std::vector<double> qn = _q.faces_val(_qnbc, m);
+    const double *ux = rt.in(_ux, n), *uy = rt.in(_uy, n), *uz = rt.in(_uz, n);
+    const double *imx = rt.in(_imx, n), *imy = rt.in(_imy, n), *imz = rt.in(_imz, n);
+    const double *dads = rt.in(_dads, n), *tem = rt.in(_tem, n), *item = rt.in(_item, n);
+    const double *iqx = rt.in(_iqx, n), *iqy = rt.in(_iqy, n), *iqz = rt.in(_iqz, n);
+    const double *g = rt.in(_g, ng, Q<T>::nc), *ig = rt.in(_ig, ng, Q<T>::nc);
+    const double *diff = rt.in(_diffusivity, n), *dkds = rt.in(_dkds, n);
+    double *dfds = rt.inout(_dfds, n);
+    gpu::check(lbm_sensitivity_temperature_at_heat_source(_q.handle(), dfds, ux, uy, uz, imx, imy, imz, dads, tem, item,
+                                                          iqx, iqy, iqz, g, ig, diff, dkds, m.data(), qn.data()),
+               "AAD::SensitivityTemperatureAtHeatSource");
+    rt.wrote({_dfds}, true);
+}
 }  // namespace AAD
 
 }  // namespace PANSLBM2

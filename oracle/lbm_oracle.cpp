@@ -873,6 +873,53 @@ double sens_hs_boundary_2d(int face, const double *ig /* ig[c] at node */, doubl
     }
     return 0.0;
 }
+/* Heat-source boundary term on any AD lattice except D2Q9 (which keeps the reference's D2Q9 order
+ * above).  The forward flux operator (ad_set_q_*) sets each unknown population (c_a == -dd) to
+ * tem0 w_c (1 + 3 c.u) with tem0 = 6 ((1 + 1/(6D)) qn + ...)/(1 + dd 3 u_a); the term is
+ * -dkds sum_unknown (ig_c - 1) d(g_c)/dD, written with the weights scaled by S:
+ *   qn dkds [ (1 - dd 3u_a)(-S/6 + sum_c S w_c ig_c) + sum_t k_t u_t sum_c S w_c sign(c_t) ig_c ]
+ *        / (S (1 + dd 3u_a) D^2),
+ * tangential axes t = a+1, a+2 (mod nd), populations in ascending index order, S = 72 in 3D and
+ * 36 in 2D.  On D3Q15 that is adjointadvection.h:805-888 term by term (same order); k_t = 3
+ * except that the reference drops the factor 3 on the ymax face and both z faces
+ * (adjointadvection.h:851-852,874-875,880-881, quirk Q2, kept when compat != 0).  The reference's
+ * plain 3D entry point passes (ux, uy, uz, ig) to helpers declared (ux, uy, ig, uz)
+ * (:1537-1542 vs :805), so it reads ig out of bounds there; the argument order used here is the
+ * declared one, as in its AVX overloads (equation_avx/adjointadvection_avx.h:82). */
+double sens_hs_boundary_nd(const LatDesc &d, int face, const double *ig, double qn, double dkds, const double u[3],
+                           double diff, int compat) {
+    const int a = face / 2, dd = face % 2 ? 1 : -1, nd = d.nd;
+    const double S = nd == 3 ? 72.0 : 36.0;
+    double sum = -S / 6.0, tang[2] = {0.0, 0.0};
+    bool first[2] = {true, true};
+    int axis_c = -1;
+    for (int c = 1; c < d.nc; ++c)
+        if (comp(d, c, a) == -dd) {
+            bool ax = true;
+            for (int t = 0; t < nd; ++t)
+                if (t != a && comp(d, c, t) != 0) ax = false;
+            if (ax) axis_c = c;
+        }
+    sum = sum + S / inv_w(d, axis_c) * ig[axis_c];
+    for (int c = 1; c < d.nc; ++c) {
+        if (comp(d, c, a) != -dd || c == axis_c) continue;
+        double v = S / inv_w(d, c) * ig[c];
+        sum = sum + v;
+        for (int k = 0; k < nd - 1; ++k) {
+            int t = (a + 1 + k) % nd;
+            double sv = comp(d, c, t) > 0 ? v : -v;
+            if (first[k]) { tang[k] = sv; first[k] = false; }
+            else tang[k] = tang[k] + sv;
+        }
+    }
+    double num = (dd < 0 ? 1.0 + 3.0 * u[a] : 1.0 - 3.0 * u[a]) * sum;
+    for (int k = 0; k < nd - 1; ++k) {
+        int t = (a + 1 + k) % nd;
+        bool q2 = compat && d.nc == 15 && (face == 3 || face == 4 || face == 5);
+        num = num + (q2 ? u[t] : 3.0 * u[t]) * tang[k];
+    }
+    return qn * dkds * num / (S * (dd < 0 ? 1.0 - 3.0 * u[a] : 1.0 + 3.0 * u[a]) * std::pow(diff, 2.0));
+}
 
 /* Run `op(face, t, idx)` over every face node in the reference order (xmin, xmax, ymin, ymax,
  * zmin, zmax), passing the running face-array offset. */
@@ -1609,21 +1656,35 @@ void ol_sens_brinkman_diffusivity(const ol_lattice *q, double *dfds, const doubl
         dfds[idx] += -3.0 / std::pow(3.0 * diff[idx] + 0.5, 2.0) * dkds[idx] * (sumg - tem[idx] * (item[idx] + 3.0 * uq));
     }
 }
-/* adjointadvection.h:1497-1517 (2D); the 3D boundary term is in lbm_oracle3d.cpp */
+/* adjointadvection.h:1497-1543: the Brinkman and diffusivity terms, then the heat-source boundary
+ * term face by face in the reference's order (xmin, xmax, ymin, ymax[, zmin, zmax]).  compat
+ * selects quirk Q2 on D3Q15 (see sens_hs_boundary_nd). */
+void ol_sens_temp_heat_source_c(const ol_lattice *q, double *dfds, const double *ux, const double *uy,
+                                const double *uz, const double *imx, const double *imy, const double *imz,
+                                const double *dads, const double *tem, const double *item, const double *iqx,
+                                const double *iqy, const double *iqz, const double *g, const double *ig,
+                                const double *diff, const double *dkds, const int *mask, const double *qn, int compat) {
+    ol_sens_brinkman_diffusivity(q, dfds, ux, uy, uz, imx, imy, imz, dads, tem, item, iqx, iqy, iqz, g, ig, diff, dkds);
+    int N = q->N, nc = q->d->nc;
+    for_faces(q, [&](int face, int t, int idx) {
+        if (!mask[t]) return;
+        double igl[27];
+        for (int c = 0; c < nc; ++c) igl[c] = ig[(size_t)c * N + idx];
+        if (nc == 9) {
+            dfds[idx] += sens_hs_boundary_2d(face, igl, qn[t], dkds[idx], ux[idx], uy[idx], diff[idx]);
+        } else {
+            double u[3] = {ux[idx], uy[idx], q->d->nd == 3 ? uz[idx] : 0.0};
+            dfds[idx] += sens_hs_boundary_nd(*q->d, face, igl, qn[t], dkds[idx], u, diff[idx], compat);
+        }
+    });
+}
 void ol_sens_temp_heat_source(const ol_lattice *q, double *dfds, const double *ux, const double *uy, const double *uz,
                               const double *imx, const double *imy, const double *imz, const double *dads,
                               const double *tem, const double *item, const double *iqx, const double *iqy,
                               const double *iqz, const double *g, const double *ig, const double *diff,
                               const double *dkds, const int *mask, const double *qn) {
-    ol_sens_brinkman_diffusivity(q, dfds, ux, uy, uz, imx, imy, imz, dads, tem, item, iqx, iqy, iqz, g, ig, diff, dkds);
-    if (q->d->nd != 2 || q->d->nc != 9) return;
-    int N = q->N;
-    for_faces(q, [&](int face, int t, int idx) {
-        if (!mask[t]) return;
-        double igl[9];
-        for (int c = 0; c < 9; ++c) igl[c] = ig[(size_t)c * N + idx];
-        dfds[idx] += sens_hs_boundary_2d(face, igl, qn[t], dkds[idx], ux[idx], uy[idx], diff[idx]);
-    });
+    ol_sens_temp_heat_source_c(q, dfds, ux, uy, uz, imx, imy, imz, dads, tem, item, iqx, iqy, iqz, g, ig, diff, dkds,
+                               mask, qn, 1);
 }
 
 /* volumetric heat-source design derivative (no reference counterpart; see the header) */

@@ -138,6 +138,13 @@ void ol_sens_temp_heat_source(const ol_lattice *q, double *dfds, const double *u
                               const double *tem, const double *item, const double *iqx, const double *iqy,
                               const double *iqz, const double *g, const double *ig, const double *diff,
                               const double *dkds, const int *mask, const double *qn);
+/* the same on every AD lattice; compat != 0 keeps quirk Q2 of the D3Q15 term (adjointadvection.h:851-881),
+ * ol_sens_temp_heat_source is compat = 1 */
+void ol_sens_temp_heat_source_c(const ol_lattice *q, double *dfds, const double *ux, const double *uy,
+                                const double *uz, const double *imx, const double *imy, const double *imz,
+                                const double *dads, const double *tem, const double *item, const double *iqx,
+                                const double *iqy, const double *iqz, const double *g, const double *ig,
+                                const double *diff, const double *dkds, const int *mask, const double *qn, int compat);
 
 /* No reference counterpart: design derivative of the volumetric heat source that ol_ad_collide_fc's
  * `heat` argument adds (g_c += w_c Q before the re-macro).  The collide's output depends on Q

@@ -141,6 +141,7 @@ def load():
         "lbm_sensitivity_brinkman": [VP] * 9,
         "lbm_sensitivity_brinkman_diffusivity": [VP] * 18,
         "lbm_sensitivity_temperature_at_heat_source": [VP] * 20,
+        "lbm_sensitivity_temperature_at_heat_source_c": [VP] * 20 + [I],
         "lbm_residual": [VP, I] + [VP] * 6 + [I, ctypes.POINTER(D)],
         "lbm_normalize": [VP, VP, I],
         "lbm_gather_sum": [VP, VP, VP, I, ctypes.POINTER(D)],

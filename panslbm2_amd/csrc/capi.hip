@@ -124,6 +124,7 @@ struct lbm_lattice {
     // the heat-source edge operators of lbm_sensitivity_temperature_at_heat_source, kept on the
     // device while the face arrays they were built from repeat
     lbm_program *hs_prog = nullptr;
+    std::vector<lbm_program *> hs_retired;  // earlier heat-source tables: graphs may still read them
     std::vector<int> hs_mask;
     std::vector<double> hs_qn;
     Grid grid() const { return Grid{lx, ly, lz, N}; }
@@ -768,6 +769,7 @@ int lbm_lattice_destroy(lbm_lattice *L) {
     if (L->d_sm_edge) (void)hipFree(L->d_sm_edge);
     if (L->d_sm_corner) (void)hipFree(L->d_sm_corner);
     delete L->hs_prog;
+    for (lbm_program *P : L->hs_retired) delete P;
     delete L;
     return LBM_OK;
 }
@@ -1349,26 +1351,35 @@ int lbm_sensitivity_heat_source(lbm_lattice *q, double *dfds, const double *ux, 
     const double *a[9] = {ux, uy, uz, item, iqx, iqy, iqz, diff, dhds};
     return hip_result(lbm::launch_sens_hsrc(q->nd, dfds, q->N, a, q->ctx->stream), "sensitivity heat source");
 }
-int lbm_sensitivity_temperature_at_heat_source(lbm_lattice *q, double *dfds, const double *ux, const double *uy,
-                                               const double *uz, const double *imx, const double *imy,
-                                               const double *imz, const double *dads, const double *tem,
-                                               const double *item, const double *iqx, const double *iqy,
-                                               const double *iqz, const double *g, const double *ig,
-                                               const double *diff, const double *dkds, const int *mask,
-                                               const double *qn) {
+int lbm_sensitivity_temperature_at_heat_source_c(lbm_lattice *q, double *dfds, const double *ux, const double *uy,
+                                                 const double *uz, const double *imx, const double *imy,
+                                                 const double *imz, const double *dads, const double *tem,
+                                                 const double *item, const double *iqx, const double *iqy,
+                                                 const double *iqz, const double *g, const double *ig,
+                                                 const double *diff, const double *dkds, const int *mask,
+                                                 const double *qn, int compat) {
     if (!q || !mask || !qn) return fail(LBM_ERR_ARG, "bad arguments");
-    if (q->kind != LBM_D2Q9)
-        return fail(LBM_ERR_UNSUPPORTED, "the heat-source boundary term exists for D2Q9 only in this build");
-    int rc = lbm_sensitivity_brinkman_diffusivity(q, dfds, ux, uy, uz, imx, imy, imz, dads, tem, item, iqx, iqy, iqz, g,
-                                                  ig, diff, dkds);
-    if (rc) return rc;
+    if (q->kind == LBM_D3Q27)
+        return fail(LBM_ERR_UNSUPPORTED, "the heat-source term exists on the advection-diffusion lattices");
+    if (q->nd == 3 && !uz) return fail(LBM_ERR_ARG, "bad arguments");
     // the edge's operator list is built and uploaded once per distinct (mask, qn) face pair and
     // kept with the lattice, so repeated calls (a transient adjoint calls this every step) launch
-    // without host work or a wait
+    // without host work or a wait.  A replaced list is retired, not freed: graphs captured with
+    // it may still be replayed.  Under capture the list cannot be rebuilt (the upload is a
+    // host-synchronous copy), so a changed face pair is an error there.
     const size_t nf = (size_t)nface_total(q);
     const bool same = q->hs_prog && q->hs_mask.size() == nf &&
                       std::memcmp(q->hs_mask.data(), mask, nf * sizeof(int)) == 0 &&
                       std::memcmp(q->hs_qn.data(), qn, nf * sizeof(double)) == 0;
+    if (!same) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        HIPCHK(hipStreamIsCapturing(q->ctx->stream, &cs));
+        if (cs != hipStreamCaptureStatusNone)
+            return fail(LBM_ERR_STATE, "heat-source faces changed during graph capture: call once outside the capture");
+    }
+    int rc = lbm_sensitivity_brinkman_diffusivity(q, dfds, ux, uy, uz, imx, imy, imz, dads, tem, item, iqx, iqy, iqz, g,
+                                                  ig, diff, dkds);
+    if (rc) return rc;
     if (!same) {
         lbm_program *P = new lbm_program;
         P->ctx = q->ctx;
@@ -1378,16 +1389,25 @@ int lbm_sensitivity_temperature_at_heat_source(lbm_lattice *q, double *dfds, con
             delete P;
             return rc;
         }
-        HIPCHK(hipStreamSynchronize(q->ctx->stream));  // launches may still read the previous list
-        delete q->hs_prog;
+        if (q->hs_prog) q->hs_retired.push_back(q->hs_prog);
         q->hs_prog = P;
         q->hs_mask.assign(mask, mask + nf);
         q->hs_qn.assign(qn, qn + nf);
     }
     if (q->hs_prog->ops.empty()) return LBM_OK;
     ProgView v = prog_view(q->hs_prog);
-    HIPCHK(lbm::launch_sens_hs_boundary(dfds, q->N, v, ux, uy, ig, diff, dkds, q->ctx->stream));
+    HIPCHK(lbm::launch_sens_hs_boundary(q->kind, dfds, q->N, v, ux, uy, uz, ig, diff, dkds, compat, q->ctx->stream));
     return LBM_OK;
+}
+int lbm_sensitivity_temperature_at_heat_source(lbm_lattice *q, double *dfds, const double *ux, const double *uy,
+                                               const double *uz, const double *imx, const double *imy,
+                                               const double *imz, const double *dads, const double *tem,
+                                               const double *item, const double *iqx, const double *iqy,
+                                               const double *iqz, const double *g, const double *ig,
+                                               const double *diff, const double *dkds, const int *mask,
+                                               const double *qn) {
+    return lbm_sensitivity_temperature_at_heat_source_c(q, dfds, ux, uy, uz, imx, imy, imz, dads, tem, item, iqx, iqy,
+                                                        iqz, g, ig, diff, dkds, mask, qn, 1);
 }
 
 // ---- reductions -------------------------------------------------------------------------------

@@ -102,8 +102,9 @@ hipError_t launch_sens_brinkman(int nd, double *dfds, int N, const double *const
 hipError_t launch_sens_he(int nd, double *dfds, int N, const double *const *p, hipStream_t s);
 hipError_t launch_sens_hsrc(int nd, double *dfds, int N, const double *const *p, hipStream_t s);
 hipError_t launch_sens_bd(int kind, double *dfds, int N, const double *const *p, hipStream_t s);
-hipError_t launch_sens_hs_boundary(double *dfds, int N, const ProgView &pv, const double *ux, const double *uy,
-                                   const double *ig, const double *diff, const double *dkds, hipStream_t s);
+hipError_t launch_sens_hs_boundary(int kind, double *dfds, int N, const ProgView &pv, const double *ux,
+                                   const double *uy, const double *uz, const double *ig, const double *diff,
+                                   const double *dkds, int compat, hipStream_t s);
 hipError_t launch_residual(int ncomp, const double *const *a, const double *const *b, int n, double *part,
                            double *out, hipStream_t s);
 // Residual sums of many arrays sets in one launch pair (lbm_residual_batch)

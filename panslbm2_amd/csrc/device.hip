@@ -186,26 +186,74 @@ __global__ void __launch_bounds__(kBlock) k_sens_bd(double *dfds, int N, const d
     dfds[n] = d;
 }
 
-// adjointadvection.h:755-801: boundary term of the heat-source sensitivity (D2Q9), one thread per
-// boundary node; the node's face terms are added in the reference's face order.
+// Heat-source boundary term of AAD::SensitivityTemperatureAtHeatSource, one thread per boundary
+// node; the node's face terms are added in the reference's face order.  D2Q9: adjointadvection.h:
+// 755-801 as written.  Other AD lattices: -dkds sum_unknown (ig_c - 1) d(SetQ(g; D))_c/dD with the
+// weights scaled by S (72 in 3D, 36 in 2D), tangential axes a+1, a+2, populations in ascending
+// order; on D3Q15 that is :805-888 term by term, with the factor 3 of the tangential terms dropped
+// on ymax, zmin and zmax when compat is set (the reference's arithmetic there).
+template <class L>
+__device__ __forceinline__ double hs_face_term(int face, const double *igl, double qn, double dk, const double *u,
+                                               double D2, int compat) {
+    const int a = face / 2, dd = (face & 1) ? 1 : -1;
+    constexpr double S = L::nd == 3 ? 72.0 : 36.0;
+    double sum = -S / 6.0, tang[2] = {0.0, 0.0};
+    bool first[2] = {true, true};
+    auto cmp = [](int c, int ax) { return ax == 0 ? L::cx[c] : (ax == 1 ? L::cy[c] : L::cz[c]); };
+    sfor<1, L::nc>([&](auto cc) {  // the axis population first
+        constexpr int C = decltype(cc)::value;
+        constexpr int nz = (L::cx[C] != 0) + (L::cy[C] != 0) + (L::cz[C] != 0);
+        // S w_c of an axis population: 72/9, 72/6, 36/9, 36/6
+        constexpr double WA = L::nc == 15 ? 8.0 : (L::nc == 7 ? 12.0 : (L::nc == 9 ? 4.0 : 6.0));
+        if (nz == 1 && cmp(C, a) == -dd) sum = sum + WA * igl[C];
+    });
+    sfor<1, L::nc>([&](auto cc) {
+        constexpr int C = decltype(cc)::value;
+        constexpr int nz = (L::cx[C] != 0) + (L::cy[C] != 0) + (L::cz[C] != 0);
+        if (nz > 1 && cmp(C, a) == -dd) {
+            double v = igl[C];  // weight S w_c = 1 for every diagonal population (D2Q9, D3Q15)
+            sum = sum + v;
+            for (int k = 0; k < L::nd - 1; ++k) {
+                int t = (a + 1 + k) % L::nd;
+                double sv = cmp(C, t) > 0 ? v : -v;
+                tang[k] = first[k] ? sv : tang[k] + sv;
+                first[k] = false;
+            }
+        }
+    });
+    double num = (dd < 0 ? 1.0 + 3.0 * u[a] : 1.0 - 3.0 * u[a]) * sum;
+    for (int k = 0; k < L::nd - 1; ++k) {
+        int t = (a + 1 + k) % L::nd;
+        bool q2 = compat && L::nc == 15 && face >= 3;
+        num = num + (q2 ? u[t] : 3.0 * u[t]) * tang[k];
+    }
+    return qn * dk * num / (S * (dd < 0 ? 1.0 - 3.0 * u[a] : 1.0 + 3.0 * u[a]) * D2);
+}
+
+template <class L>
 __global__ void __launch_bounds__(kBlock) k_sens_hs_boundary(double *dfds, int N, ProgView pv, const double *ux,
-                                                             const double *uy, const double *ig, const double *diff,
-                                                             const double *dkds) {
+                                                             const double *uy, const double *uz, const double *ig,
+                                                             const double *diff, const double *dkds, int compat) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= pv.nentry) return;
     int n = pv.entry_node[e];
     int2 r = pv.entry_ops[e];
-    double igl[9];
-    sfor<0, 9>([&](auto c) { igl[decltype(c)::value] = ig[(size_t)decltype(c)::value * N + n]; });
+    double igl[L::nc];
+    sfor<0, L::nc>([&](auto c) { igl[decltype(c)::value] = ig[(size_t)decltype(c)::value * N + n]; });
     double vx = ux[n], vy = uy[n], D2 = diff[n] * diff[n], dk = dkds[n], acc_ = dfds[n];
+    double u[3] = {vx, vy, L::nd == 3 ? uz[n] : 0.0};
     for (int t = r.x; t < r.x + r.y; ++t) {
         Op op = pv.ops[t];
         double qn = op.v[0], term;
-        switch (op.face) {
-        case 0: term = qn * dk * ((1.0 + 3.0 * vx) * (-6.0 + 4.0 * igl[1] + igl[5] + igl[8]) + 3.0 * vy * (igl[5] - igl[8])) / (36.0 * (1.0 - 3.0 * vx) * D2); break;
-        case 1: term = qn * dk * ((1.0 - 3.0 * vx) * (-6.0 + 4.0 * igl[3] + igl[6] + igl[7]) + 3.0 * vy * (igl[6] - igl[7])) / (36.0 * (1.0 + 3.0 * vx) * D2); break;
-        case 2: term = qn * dk * ((1.0 + 3.0 * vy) * (-6.0 + 4.0 * igl[2] + igl[5] + igl[6]) + 3.0 * vx * (igl[5] - igl[6])) / (36.0 * (1.0 - 3.0 * vy) * D2); break;
-        default: term = qn * dk * ((1.0 - 3.0 * vy) * (-6.0 + 4.0 * igl[4] + igl[7] + igl[8]) + 3.0 * vx * (igl[8] - igl[7])) / (36.0 * (1.0 + 3.0 * vy) * D2); break;
+        if constexpr (L::nc == 9) {
+            switch (op.face) {
+            case 0: term = qn * dk * ((1.0 + 3.0 * vx) * (-6.0 + 4.0 * igl[1] + igl[5] + igl[8]) + 3.0 * vy * (igl[5] - igl[8])) / (36.0 * (1.0 - 3.0 * vx) * D2); break;
+            case 1: term = qn * dk * ((1.0 - 3.0 * vx) * (-6.0 + 4.0 * igl[3] + igl[6] + igl[7]) + 3.0 * vy * (igl[6] - igl[7])) / (36.0 * (1.0 + 3.0 * vx) * D2); break;
+            case 2: term = qn * dk * ((1.0 + 3.0 * vy) * (-6.0 + 4.0 * igl[2] + igl[5] + igl[6]) + 3.0 * vx * (igl[5] - igl[6])) / (36.0 * (1.0 - 3.0 * vy) * D2); break;
+            default: term = qn * dk * ((1.0 - 3.0 * vy) * (-6.0 + 4.0 * igl[4] + igl[7] + igl[8]) + 3.0 * vx * (igl[8] - igl[7])) / (36.0 * (1.0 + 3.0 * vy) * D2); break;
+            }
+        } else {
+            term = hs_face_term<L>(op.face, igl, qn, dk, u, D2, compat);
         }
         acc_ += term;
     }
@@ -588,10 +636,22 @@ hipError_t launch_sens_bd(int kind, double *dfds, int N, const double *const *p,
     }
     return hipErrorInvalidValue;
 }
-hipError_t launch_sens_hs_boundary(double *dfds, int N, const ProgView &pv, const double *ux, const double *uy,
-                                   const double *ig, const double *diff, const double *dkds, hipStream_t s) {
-    if (pv.nentry > 0) LBM_LAUNCH(k_sens_hs_boundary, nblocks(pv.nentry), s, dfds, N, pv, ux, uy, ig, diff, dkds);
-    return hipGetLastError();
+hipError_t launch_sens_hs_boundary(int kind, double *dfds, int N, const ProgView &pv, const double *ux,
+                                   const double *uy, const double *uz, const double *ig, const double *diff,
+                                   const double *dkds, int compat, hipStream_t s) {
+    if (pv.nentry <= 0) return hipSuccess;
+    auto go = [&](auto l) {
+        using L = decltype(l);
+        LBM_LAUNCH((k_sens_hs_boundary<L>), nblocks(pv.nentry), s, dfds, N, pv, ux, uy, uz, ig, diff, dkds, compat);
+        return hipGetLastError();
+    };
+    switch (kind) {
+    case kD2Q9: return go(D2Q9{});
+    case kD3Q15: return go(D3Q15{});
+    case kD2Q5: return go(D2Q5{});
+    case kD3Q7: return go(D3Q7{});
+    }
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_residual(int ncomp, const double *const *a, const double *const *b, int n, double *part,

@@ -11,6 +11,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(ROOT, "oracle", "liblbm_oracle.so")
 REF_SO = os.path.join(ROOT, "oracle", "_ref", "libref_panslbm2.so")
+REF_AVX_SO = os.path.join(ROOT, "oracle", "_ref", "libref_avx.so")
 
 D2Q9, D3Q15, D2Q5, D3Q7, D3Q27 = 0, 1, 2, 3, 4
 NC = {D2Q9: 9, D3Q15: 15, D2Q5: 5, D3Q7: 7, D3Q27: 27}
@@ -88,6 +89,7 @@ def lib():
         L.ol_sens_brinkman_diffusivity.argtypes = [VP] * 18
         L.ol_sens_temp_heat_source.argtypes = [VP] * 20
         L.ol_sens_heat_source.argtypes = [VP] * 11
+        L.ol_sens_temp_heat_source_c.argtypes = [VP] * 20 + [I]
         L.ol_residual.argtypes = [I] + [VP] * 6 + [I]
         L.ol_residual.restype = D
         L.ol_normalize.argtypes = [VP, I]
@@ -105,6 +107,17 @@ def ref():
     if _ref is None:
         _ref = _load(REF_SO)
     return _ref
+
+
+_ref_avx = None
+
+
+def ref_avx():
+    """The D3Q15 heat-source boundary sensitivity of the reference's AVX build, or None."""
+    global _ref_avx
+    if _ref_avx is None:
+        _ref_avx = _load(REF_AVX_SO)
+    return _ref_avx
 
 
 class Lattice:

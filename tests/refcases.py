@@ -9,7 +9,7 @@ import ctypes
 
 import numpy as np
 
-from _oracle import (D2Q9, D3Q7, D3Q15, D3Q27, NC, ND, Lattice, aos_to_soa, lib, ptr, ref, soa_to_aos,
+from _oracle import (D2Q9, D3Q7, D3Q15, D3Q27, NC, ND, Lattice, aos_to_soa, lib, ptr, ref, ref_avx, soa_to_aos,
                      to_face)
 
 D = ctypes.c_double
@@ -255,10 +255,20 @@ def run_ref(case, op, inp):
             fn(op)(*L, *args, ptr(uz), ptr(inp["v0"]), ptr(inp["v1"]), ptr(inp["v2"]), ptr(inp["dads"]),
                    ptr(inp["tem"]), ptr(inp["rho"]), ptr(inp["alpha"]), ptr(inp["uz"]), ptr(inp["uz"]), ptr(gs),
                    ptr(igs), ptr(inp["diff"]), ptr(inp["dkds"]))
-        else:
+        elif ND[case.kind] == 2:
             fn(op)(case.lx, case.ly, *args, ptr(inp["v0"]), ptr(inp["v1"]), ptr(inp["dads"]), ptr(inp["tem"]),
                    ptr(inp["rho"]), ptr(inp["alpha"]), ptr(inp["uz"]), ptr(gs), ptr(igs), ptr(inp["diff"]),
                    ptr(inp["dkds"]), ptr(inp["mask"]), ptr(inp["v2"]))
+        else:
+            # the plain Brinkman and diffusivity terms, then the face terms as the AVX build calls
+            # them (the plain 3D call passes its arguments out of order; see ref_harness_avx.cpp)
+            fn("sens_brinkman_diffusivity")(*L, *args, ptr(uz), ptr(inp["v0"]), ptr(inp["v1"]), ptr(inp["v2"]),
+                                            ptr(inp["dads"]), ptr(inp["tem"]), ptr(inp["rho"]), ptr(inp["alpha"]),
+                                            ptr(inp["uz"]), ptr(inp["uz"]), ptr(gs), ptr(igs), ptr(inp["diff"]),
+                                            ptr(inp["dkds"]))
+            ref_avx().ref3avx_sens_hs_boundary(*L, ptr(d), ptr(inp["ux"]), ptr(inp["uy"]), ptr(uz),
+                                               ptr(np.ascontiguousarray(inp["g"])), ptr(inp["diff"]),
+                                               ptr(inp["dkds"]), ptr(inp["mask"]), ptr(inp["v2"]))
         out["dfds"] = d
     elif op == "residual":
         fr = getattr(R, case.rk + "residual")
@@ -496,8 +506,8 @@ CASES_2D = [Case("d2q9_12x10", D2Q9, (12, 10), 11, "D2Q9"), Case("d2q9_7x9", D2Q
 CASES_3D = [Case("d3q15_5x4x6", D3Q15, (5, 4, 6), 21, "D3Q15"), Case("d3q15_4x6x5", D3Q15, (4, 6, 5), 22, "D3Q15")]
 
 # D3Q15.  AAD::iBoundaryConditionSetRho has no 3D case: the reference's D3Q15 overload does not
-# compile (adjointadvection.h:583 names an undeclared `_bctype`); the 3D boundary term of
-# SensitivityTemperatureAtHeatSource is not restated.
+# compile (adjointadvection.h:583 names an undeclared `_bctype`).  The 3D boundary term of
+# SensitivityTemperatureAtHeatSource is taken from the reference's AVX build (oracle/_ref/libref_avx.so).
 OPS_3D = {
     "stream": "d3q15.h:602-613", "istream": "d3q15.h:965-976", "bc": "d3q15.h:982-1110",
     "ibc": "d3q15.h:1112-1239", "smooth": "d3q15.h:199-220,1241-1303",
@@ -516,6 +526,7 @@ OPS_3D = {
     "aad_collide_mf": "adjointadvection.h:1293-1358,143-150",
     "sens_heat_exchange": "adjointadvection.h:1452-1457",
     "sens_brinkman": "adjointnavierstokes.h:550-554", "sens_brinkman_diffusivity": "adjointadvection.h:1478-1493",
+    "sens_temp_heat_source": "adjointadvection.h:1519-1543,805-888 (argument order of equation_avx/adjointadvection_avx.h:1507-1512)",
     "residual": "residual.h:39-51", "normalize": "normalize.h:9-24", "density_filter": "densityfilter.h:390-420,512-515",
 }
 

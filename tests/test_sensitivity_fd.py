@@ -350,3 +350,69 @@ def test_reference_gradient_deviation_is_recorded(conduction):
     # measured on 10x8: median about -0.064 (D2Q9), spread ~0.7 of the median; D2Q5 similar
     assert -0.2 < med < -0.02
     assert 0.1 < spread < 2.0
+
+
+@pytest.mark.parametrize("kind", AD_KINDS)
+def test_heat_source_boundary_term_is_flux_operator_derivative(kind):
+    """The boundary term of SensitivityTemperatureAtHeatSource on face nodes equals
+    -dkds sum_c (ig_c - 1) d(SetQ(g; D))_c / dD, central differences of the forward flux operator
+    (AD::BoundaryConditionSetQ with a diffusivity field) in D: the '-1' is the objective's own
+    dependence on D through the prescribed flux.  On D3Q15 the reference's term (compat = 1) drops the
+    factor 3 of the tangential terms on ymax, zmin and zmax (quirk Q2) and fails the identity there;
+    compat = 0 satisfies it on every face."""
+    O = lib()
+    lx, ly, lz = (6, 5, 1) if ND[kind] == 2 else (5, 4, 4)
+    rng = np.random.default_rng(50 + kind)
+    q = Lattice(kind, lx, ly, lz)
+    N, nc = q.N, NC[kind]
+    d3 = ND[kind] == 3
+    P = lambda a: ptr(a) if d3 else None
+    ux, uy, uz = (0.05 * rng.uniform(-1, 1, N) for _ in range(3))
+    diff = rng.uniform(0.02, 0.2, N)
+    dkds = rng.uniform(-0.2, -0.05, N)
+    ig = rng.standard_normal((nc, N))
+    g = np.abs(rng.standard_normal((nc, N)))
+    z = np.zeros(N)
+    qn_node = rng.uniform(0.5, 1.5, N) * 1e-2
+    nfaces = 2 * ND[kind]
+    from _oracle import face_nodes
+    fn = face_nodes(kind, lx, ly, lz)
+    # face of each face-array slot, from the face lengths (xmin, xmax, ymin, ymax[, zmin, zmax])
+    dims = (lx, ly, lz)
+    flen = []
+    for f in range(nfaces):
+        a = f // 2
+        flen.append(int(np.prod([dims[t] for t in range(ND[kind]) if t != a])))
+    slot_face = np.repeat(np.arange(nfaces), flen)
+    qn_face = np.ascontiguousarray(qn_node[fn])
+    for face in range(nfaces):
+        mask = (slot_face == face).astype(np.int32)
+        nodes = np.unique(fn[mask == 1])
+        # sensitivity term alone: g = 0, tem = 0 and dads = 0 switch the other terms off
+        terms = {}
+        for compat in (0, 1):
+            d = np.zeros(N)
+            O.ol_sens_temp_heat_source_c(q.h, ptr(d), ptr(ux), ptr(uy), P(uz), ptr(z), ptr(z), P(z), ptr(z), ptr(z),
+                                         ptr(z), ptr(z), ptr(z), P(z), ptr(np.zeros((nc, N))), ptr(ig), ptr(diff),
+                                         ptr(dkds), ptr(mask), ptr(qn_face), compat)
+            terms[compat] = d
+        # central differences of the forward flux operator in D, one node of the face at a time
+        expect = np.zeros(N)
+        h = 1e-7
+        for n in nodes:
+            xs = []
+            for sgn in (1, -1):
+                dd = diff.copy()
+                dd[n] += sgn * h
+                q.set(g)
+                O.ol_ad_set_q(q.h, ptr(mask), ptr(qn_face), ptr(ux), ptr(uy), P(uz), ptr(dd), D(0.0))
+                xs.append(q.get()[:, n])
+            dx = (xs[0] - xs[1]) / (2 * h)
+            expect[n] = -dkds[n] * np.sum((ig[:, n] - 1.0) * dx)
+        scale = np.abs(expect).max()
+        np.testing.assert_allclose(terms[0][nodes], expect[nodes], rtol=1e-6, atol=1e-7 * scale)
+        assert np.all(terms[0][np.setdiff1d(np.arange(N), nodes)] == 0.0)
+        if kind == D3Q15 and face >= 3:
+            assert np.abs(terms[1][nodes] - expect[nodes]).max() > 1e-3 * scale
+        else:
+            assert np.array_equal(terms[1], terms[0])
