@@ -389,6 +389,16 @@ int lbm_batch_destroy(lbm_batch *batch);
 /* Residual (residual.h:24-51) of n array sets at once: a[3*i + k], b[3*i + k] for k < ncomp */
 int lbm_residual_batch(lbm_context *ctx, int n, int ncomp, const double *const *a, const double *const *b,
                        const int *len, double *out);
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): lbm_residual_batch with the
+ * same arguments and identical results, whose item table and partial sums live in a fixed
+ * file-scope device array of the library (82 KB per GPU, held until the process ends)
+ * instead of a buffer the context grows.  At most LBM_RESIDUAL_STATIC_MAX residuals per call
+ * (LBM_ERR_ARG beyond).  Every call on one GPU shares that array: calls that overlap on one GPU,
+ * from any context or thread, race. */
+#define LBM_RESIDUAL_STATIC_MAX 256
+int lbm_residual_batch_static(lbm_context *ctx, int n, int ncomp, const double *const *a, const double *const *b,
+                              const int *len, double *out);
 
 int lbm_aad_macro_brinkman_collide_heat_exchange_p(
     lbm_program *prog, lbm_lattice *p, const double *rho, const double *ux, const double *uy, const double *uz,

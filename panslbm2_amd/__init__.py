@@ -91,6 +91,8 @@ def load():
         "lbm_batch_destroy": [VP],
         "lbm_residual_batch": [VP, I, I, ctypes.POINTER(VP), ctypes.POINTER(VP), ctypes.POINTER(I),
                                ctypes.POINTER(D)],
+        "lbm_residual_batch_static": [VP, I, I, ctypes.POINTER(VP), ctypes.POINTER(VP), ctypes.POINTER(I),
+                                      ctypes.POINTER(D)],
         "lbm_lattice_destroy": [VP],
         "lbm_lattice_nc": [VP],
         "lbm_lattice_nxyz": [VP],

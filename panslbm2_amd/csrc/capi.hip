@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -97,6 +98,7 @@ struct lbm_context {
     DevBuf scratch;            // index lists for gather / filter weights
     DevBuf multi;              // batched residual items and partials
     DevBuf filt;               // filter intermediates
+    void *static_scratch = nullptr;  // this device's copy of `scratch` (lbm_residual_batch_static)
 };
 
 // Collected fused steps of many instances, launched as one kernel (lbm_batch_*).
@@ -624,6 +626,40 @@ const char *lbm_last_error(void) { return g_err.c_str(); }
 
 int lbm_version(void) { return 1; }
 
+}  // extern "C"
+
+// ---- file-scope scratch of lbm_residual_batch_static ---------------------------------------------
+// The entry point allocates nothing: its item table, per-block partial sums and per-item sums live
+// in this array, one copy per GPU for the life of the process.  Host code never uses the array's
+// name (a host-side stand-in): each device's address comes from hipGetSymbolAddress, taken once per
+// device with that device current, at context creation, and is what the kernels receive.
+namespace {
+constexpr size_t kStaticItemBytes = (sizeof(lbm::ResItem) * LBM_RESIDUAL_STATIC_MAX + 255) / 256 * 256;
+constexpr size_t kStaticPartBytes = sizeof(double) * 2 * LBM_RESIDUAL_STATIC_MAX * lbm::kResidualBlocksPerItem;
+constexpr size_t kStaticSumBytes = sizeof(double) * 2 * LBM_RESIDUAL_STATIC_MAX;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+static __device__ __align__(256) unsigned char scratch[kStaticItemBytes + kStaticPartBytes + kStaticSumBytes];
+
+// the device address of `scratch` on `device` (current), looked up on the first call per device
+static int static_scratch_address(int device, void **out) {
+    if (device < 0 || device >= kMaxDevices) return fail(LBM_ERR_ARG, "device %d beyond the scratch table", device);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(LBM_ERR_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        g_scratch_ptr[device] = p;
+    }
+    *out = g_scratch_ptr[device];
+    return LBM_OK;
+}
+
+extern "C" {
+
 int lbm_context_create(int device, lbm_context **out) {
     if (!out) return fail(LBM_ERR_ARG, "null out");
     *out = nullptr;
@@ -642,6 +678,11 @@ int lbm_context_create(int device, lbm_context **out) {
     HIPCHK(hipMalloc(&C->d_part, sizeof(double) * 2 * lbm::reduction_blocks()));
     HIPCHK(hipMalloc(&C->d_out, sizeof(double) * 4));
     HIPCHK(hipHostMalloc(&C->h_out, sizeof(double) * 4, hipHostMallocDefault));
+    int rc = static_scratch_address(device, &C->static_scratch);
+    if (rc) {
+        (void)lbm_context_destroy(C);
+        return rc;
+    }
     *out = C;
     return LBM_OK;
 }
@@ -1731,6 +1772,38 @@ int lbm_residual_batch(lbm_context *C, int n, int ncomp, const double *const *a,
     HIPCHK(lbm::launch_residual_multi((const lbm::ResItem *)base, n, part, res, C->stream));
     std::vector<double> h(2 * n);
     HIPCHK(hipMemcpyAsync(h.data(), res, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, C->stream));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    for (int i = 0; i < n; ++i) out[i] = std::sqrt(h[2 * i] / h[2 * i + 1]);
+    return LBM_OK;
+}
+
+// lbm_residual_batch without allocation: the same kernels and summation order on the file-scope
+// scratch above, so results are identical.  Calls that overlap on one GPU share that scratch.
+int lbm_residual_batch_static(lbm_context *C, int n, int ncomp, const double *const *a, const double *const *b,
+                              const int *len, double *out) {
+    if (!C || n < 1 || ncomp < 1 || ncomp > 3 || !a || !b || !len || !out) return fail(LBM_ERR_ARG, "bad arguments");
+    if (n > LBM_RESIDUAL_STATIC_MAX)
+        return fail(LBM_ERR_ARG, "%d residuals requested: the no-allocation form takes at most %d", n,
+                    LBM_RESIDUAL_STATIC_MAX);
+    if (!C->static_scratch) return fail(LBM_ERR_STATE, "context has no static scratch");
+    lbm::ResItem items[LBM_RESIDUAL_STATIC_MAX];
+    for (int i = 0; i < n; ++i) {
+        for (int k = 0; k < 3; ++k) {
+            items[i].a[k] = k < ncomp ? a[3 * i + k] : nullptr;
+            items[i].b[k] = k < ncomp ? b[3 * i + k] : nullptr;
+            if (k < ncomp && (!items[i].a[k] || !items[i].b[k])) return fail(LBM_ERR_ARG, "null field");
+        }
+        if (len[i] < 0) return fail(LBM_ERR_ARG, "negative length");
+        items[i].n = len[i];
+        items[i].ncomp = ncomp;
+    }
+    char *base = static_cast<char *>(C->static_scratch);
+    double *part = reinterpret_cast<double *>(base + kStaticItemBytes);
+    double *sums = reinterpret_cast<double *>(base + kStaticItemBytes + kStaticPartBytes);
+    double h[2 * LBM_RESIDUAL_STATIC_MAX];
+    HIPCHK(hipMemcpyAsync(base, items, sizeof(lbm::ResItem) * n, hipMemcpyHostToDevice, C->stream));
+    HIPCHK(lbm::launch_residual_multi(reinterpret_cast<const lbm::ResItem *>(base), n, part, sums, C->stream));
+    HIPCHK(hipMemcpyAsync(h, sums, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, C->stream));
     HIPCHK(hipStreamSynchronize(C->stream));
     for (int i = 0; i < n; ++i) out[i] = std::sqrt(h[2 * i] / h[2 * i + 1]);
     return LBM_OK;
