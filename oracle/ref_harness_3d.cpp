@@ -13,7 +13,10 @@
  * exactly what a driver's lambda would.  Used to write the golden fixtures under tests/golden/
  * and, when present, as a live cross-check of the oracle.
  */
+#include <chrono>
 #include <cstring>
+#include <utility>
+#include <vector>
 
 #include "src/particle/d3q15.h"
 #include "src/equation/navierstokes.h"
@@ -345,6 +348,69 @@ void ref3_heaviside_sensitivity_region(int lx, int ly, int lz, double R, double 
     std::vector<double> in(v, v + p.nxyz), d(dfdrho, dfdrho + p.nxyz);
     std::vector<double> r = HeavisideFilter::GetFilteredSensitivity(p, R, beta, in, d, region_weight(R, mx, my, mz));
     std::memcpy(out, r.data(), sizeof(double) * p.nxyz);
+}
+
+/* The 3D heat-sink duct's time steps with the reference's own functions on D3Q15 + D3Q15 (the
+ * reference has no D3Q7), in the driver order of panslbm2_amd/heatsink3d.py (after
+ * production/heatsink.cpp:140-212 on the 3D overloads): nt_fwd forward steps from rest, then nt_adj
+ * adjoint steps; secs[0], secs[1] = wall seconds of each loop.  Used as the CPU baseline. */
+void ref3_heatsink_steps(int lx, int ly, int lz, int nt_fwd, int nt_adj, double nu, double uin, const double *alpha,
+                         const double *diff, const int *walls, const int *inlet, const int *outlet, const int *flux,
+                         const int *bottom, double *secs) {
+    D3Q15<double> pf(lx, ly, lz), pg(lx, ly, lz);
+    const int N = pf.nxyz;
+    std::vector<double> rho(N, 1.0), tem(N, 0.0), g(15 * (size_t)N, 0.0), ig(15 * (size_t)N, 0.0);
+    std::vector<double> u[2][3], q[2][3], iu[2][3], iq[2][3];
+    for (int b = 0; b < 2; ++b)
+        for (int a = 0; a < 3; ++a) {
+            u[b][a].assign(N, 0.0); q[b][a].assign(N, 0.0); iu[b][a].assign(N, 0.0); iq[b][a].assign(N, 0.0);
+        }
+    std::vector<double> ip(N, 0.0), imx(N, 0.0), imy(N, 0.0), imz(N, 0.0), item(N, 0.0);
+    N3 bw{lx, ly, walls}, bi{lx, ly, inlet}, bo{lx, ly, outlet}, bq{lx, ly, flux}, be{lx, ly, bottom};
+    auto zero = [](int, int, int) { return 0.0; };
+    auto one = [](int, int, int) { return 1.0; };
+    auto vin = [=](int, int, int) { return uin; };
+    auto t0 = std::chrono::steady_clock::now();
+    NS::InitialCondition(pf, rho.data(), u[0][0].data(), u[0][1].data(), u[0][2].data());
+    AD::InitialCondition(pg, tem.data(), u[0][0].data(), u[0][1].data(), u[0][2].data());
+    for (int t = 1; t <= nt_fwd; ++t) {
+        auto &v = u[t % 2];
+        AD::MacroBrinkmanCollideForceConvection(pf, rho.data(), v[0].data(), v[1].data(), v[2].data(), alpha, nu, pg,
+                                                tem.data(), q[t % 2][0].data(), q[t % 2][1].data(), q[t % 2][2].data(),
+                                                diff, true, g.data());
+        pf.Stream();
+        pg.Stream();
+        pf.BoundaryCondition(bw);
+        NS::BoundaryConditionSetU(pf, vin, zero, zero, bi);
+        NS::BoundaryConditionSetRho(pf, one, zero, zero, bo);
+        AD::BoundaryConditionSetT(pg, zero, v[0].data(), v[1].data(), v[2].data(), bi);
+        AD::BoundaryConditionSetQ(pg, zero, v[0].data(), v[1].data(), v[2].data(), diff, bq);
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    auto &uf = u[nt_fwd % 2];
+    ANS::InitialCondition(pf, uf[0].data(), uf[1].data(), uf[2].data(), ip.data(), iu[0][0].data(), iu[0][1].data(),
+                          iu[0][2].data());
+    AAD::InitialCondition(pg, uf[0].data(), uf[1].data(), uf[2].data(), item.data(), iq[0][0].data(), iq[0][1].data(),
+                          iq[0][2].data());
+    for (int t = 1; t <= nt_adj; ++t) {
+        auto &w = iu[t % 2];
+        auto &h = iq[t % 2];
+        AAD::MacroBrinkmanCollideForceConvection(pf, rho.data(), uf[0].data(), uf[1].data(), uf[2].data(), ip.data(),
+                                                 w[0].data(), w[1].data(), w[2].data(), imx.data(), imy.data(),
+                                                 imz.data(), alpha, nu, pg, tem.data(), item.data(), h[0].data(),
+                                                 h[1].data(), h[2].data(), diff, true, ig.data());
+        pf.iStream();
+        pg.iStream();
+        AAD::iBoundaryConditionSetT(pg, uf[0].data(), uf[1].data(), uf[2].data(), bi);
+        AAD::iBoundaryConditionSetQ(pg, uf[0].data(), uf[1].data(), uf[2].data(), bq);
+        AAD::iBoundaryConditionSetQ(pg, uf[0].data(), uf[1].data(), uf[2].data(), be, 1.0);
+        pf.iBoundaryCondition(bw);
+        ANS::iBoundaryConditionSetU(pf, vin, zero, zero, bi);
+        ANS::iBoundaryConditionSetRho3D(pf, bo);
+    }
+    auto t2 = std::chrono::steady_clock::now();
+    secs[0] = std::chrono::duration<double>(t1 - t0).count();
+    secs[1] = std::chrono::duration<double>(t2 - t1).count();
 }
 
 }  // extern "C"
