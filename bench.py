@@ -1,32 +1,35 @@
 #!/usr/bin/env python3
-"""bench.py -- design evaluations of the 2D forced-convection heatsink on one MI355X.
+"""bench.py -- design evaluations of the 3D heat sink on one MI355X.
 
 One step = one design evaluation: forward solve to a 1e-6 step-to-step residual (checked every
-100 steps), adjoint solve to the same residual, design sensitivity (see
-panslbm2_amd/heatsink2d.py).  Workload: 200 x 400 nodes, D2Q9 Navier-Stokes with Brinkman drag
-coupled to D2Q5 advection-diffusion with a volumetric heat source, FP64, seeded synthetic design.
+100 steps), adjoint solve to the same residual, design sensitivity (see panslbm2_amd/heatsink3d.py).
+Workload: the 3D heat sink at 128 x 128 x 64 nodes (1,048,576), D3Q15 Navier-Stokes with Brinkman
+drag coupled to D3Q7 advection-diffusion, inlet u = 0.03 on the x = 0 face, heat source in the
+z < 8 base plate, seeded design filtered with radius 2, FP64, cold start.
 
 value = MLUPS over the job: nodes x (forward + adjoint time steps) / wall seconds, a coupled
 NS+AD node update counting once.  Inputs are resident on the device before timing starts.
 
 Also reported:
-  roofline      -- the fused forward step kernel (pull + boundary program + collide) timed with
-                   HIP events on the library's stream over 100-step graph launches, against the
-                   8 TB/s HBM peak, for this workload (cache resident: ~20 MB per step) and for a
-                   4096 x 4096 grid (4.2 GB per step, HBM bound).
-  cpu_baseline  -- the reference's own plain C++ (oracle/_ref, OpenMP) running the same forward
-                   and adjoint time steps on D2Q9 + D2Q9 (the reference has no D2Q5) on the host
-                   cores, a bounded number of steps.
-  fp32, fp16    -- the same evaluation with FP32 / FP16 population storage (mixed precision: the
-                   reduced precision until the residual reaches its floor or stops falling, FP64 to
-                   1e-6), and the reduced-precision fused step on 4096^2.
-  batch         -- 64 independent designs (grid sizes from {64x128, 128x256, 200x400, 256x512},
+  roofline      -- the fused 3D forward step kernel (pull + boundary program + collide in one pass)
+                   timed with HIP events on the library's stream over 100-step graph launches,
+                   against the 8 TB/s HBM peak; `traffic` from rocprofv3 FETCH_SIZE / WRITE_SIZE.
+  cpu_baseline  -- the reference's own plain C++ (oracle/_ref, OpenMP) running the same forward and
+                   adjoint time steps on D3Q15 + D3Q15 (the reference has no D3Q7) at the same grid
+                   on the host cores, a bounded number of steps.
+  c3_adjoint    -- the fused 3D adjoint step kernel, timed the same way.
+  c3_fp16, c3_fp32 -- the same evaluation with FP16 / FP32 population storage (collided in FP32,
+                   FP64 once the residual reaches its floor), with the objective's and the
+                   sensitivity's error against the FP64 evaluation.
+  heatsink2d    -- the 2D heat sink (200 x 400, D2Q9 + D2Q5, FP64): MLUPS, its fused step's roofline,
+                   the same kernel on 4096^2 (HBM bound), FP32 / FP16 variants, the reference pairing
+                   D2Q9 + D2Q9 and the reference CPU path on it.
+  batch         -- 64 independent 2D designs (grid sizes from {64x128, 128x256, 200x400, 256x512},
                    heat source and inlet velocity drawn per design) evaluated together, one kernel
                    launch per time step for all of them (FP32 mixed, autotuned workgroup shape),
                    against evaluating them one after another in plain FP64.
   warm_start    -- a 3D design loop (D3Q15 + D3Q7 heat sink, 96 x 96 x 48, solid fraction 0.4,
-                   projected-gradient updates: the MMA loop's designs make the adjoint diverge,
-                   from the cold start too, after ~6 iterations at this alpha_max) run four ways: cold start in FP64 (the plain GPU
+                   projected-gradient updates) run four ways: cold start in FP64 (the plain GPU
                    form), previous-solution warm start, learned warm start (U-Net on the matrix
                    cores predicting the change of the fields, trained beside the solves), and the
                    learned warm start with FP32 mixed-precision storage; wall seconds and LBM steps
@@ -53,12 +56,19 @@ BYTES_FWD_Q9Q5 = 2 * (9 + 5) * 8 + 3 * 8
 BYTES_FWD_Q9Q9 = 2 * (9 + 9) * 8 + 3 * 8
 BYTES_FWD_Q9Q5_F32 = 2 * (9 + 5) * 4 + 3 * 8  # FP32 populations, FP64 coefficient fields
 BYTES_FWD_Q9Q5_F16 = 2 * (9 + 5) * 2 + 3 * 8  # FP16 populations
+# 3D: D3Q15 + D3Q7 populations once each way, alpha, diffusivity and heat source read (forward);
+# alpha, diffusivity and the forward rho, ux, uy, uz, T read (adjoint)
+BYTES_FWD_Q15Q7 = 2 * (15 + 7) * 8 + 3 * 8
+BYTES_ADJ_Q15Q7 = 2 * (15 + 7) * 8 + 2 * 8 + 5 * 8
 # Memory-side traffic of one forward step of the 200x400 workload, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE (KiB per dispatch; profiles/k_collide_fused_heatsink_200x400.txt), FETCH_SIZE scaled by
 # 1.920, the factor that makes it read the known bytes of the pull-stream copy kernel
 # (profiles/pmc_scale_k_stream_4096x4096.txt).  The counters see traffic below the L2, which on this
 # cache-sized step includes Infinity Cache hits.
 TRAFFIC_FWD_200x400 = round((5767.5 * 1.920 + 8920.5) * 1024)
+# The same for the fused 3D forward and adjoint steps at 128x128x64 (profiles/k_collide_fused_3d_128x128x64.txt)
+TRAFFIC_FWD_128x128x64 = round((118115.5 * 1.920 + 187170.0) * 1024)
+TRAFFIC_ADJ_128x128x64 = round((137131.1 * 1.920 + 180224.0) * 1024)
 
 
 def kernel_rate(hs, steps=100, reps=5):
@@ -72,6 +82,115 @@ def kernel_rate(hs, steps=100, reps=5):
     hs.ctx.record(1)
     ms = hs.ctx.elapsed_ms(0, 1)
     return ms / (reps * hs.check_every)
+
+
+def kernel_rate_adj(hs, reps=5):
+    """Average duration (ms) of one fused adjoint step (after an evaluation: the adjoint graph exists)."""
+    fb = hs.fwd_buf
+    step = lambda t: hs._adj_step(t, fb)
+    hs._chunk("adj", 100, step)
+    hs.ctx.sync()
+    hs.ctx.record(0)
+    for _ in range(reps):
+        hs._chunk("adj", 100, step)
+    hs.ctx.record(1)
+    return hs.ctx.elapsed_ms(0, 1) / (reps * hs.check_every)
+
+
+def host_info():
+    """The host the CPU baseline runs on: logical CPUs, model, sockets, cores, OpenMP binding."""
+    import subprocess
+
+    info = {"nproc": os.cpu_count()}
+    try:
+        out = subprocess.run(["lscpu"], stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True,
+                             timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "Model name":
+                info["model"] = v
+            elif k == "Socket(s)":
+                info["sockets"] = int(v)
+            elif k == "Core(s) per socket":
+                info["cores_per_socket"] = int(v)
+            elif k == "Thread(s) per core":
+                info["threads_per_core"] = int(v)
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES"):
+        info[k] = os.environ.get(k)
+    return info
+
+
+def _omp_env():
+    """Thread count = physical cores available to this process (capped by OMP_NUM_THREADS when it is
+    set), bound close to cores; must run before the OpenMP runtime starts."""
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
+
+
+def cpu_baseline_3d(hs, target_seconds=15.0):
+    """The reference's plain OpenMP path on the host cores (oracle/_ref), D3Q15 + D3Q15, the same
+    128 x 128 x 64 duct: forward then adjoint steps, a bounded number."""
+    _omp_env()
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
+
+    from _oracle import lib as oracle_lib
+    from _oracle import ref
+
+    R = ref()
+    if R is None:
+        return None
+    lx, ly, lz, N = hs.lx, hs.ly, hs.lz, hs.N
+    i, j, k = hs.i, hs.j, hs.k
+    wall = (j == 0) | (j == ly - 1) | (k == 0) | (k == lz - 1)
+    inlet = ((i == 0) & ~wall).astype(np.int32)
+    outlet = ((i == lx - 1) & ~wall).astype(np.int32)
+    flux = ((i > 0) | wall).astype(np.int32)
+    bottom = (k == 0).astype(np.int32)
+    walls = wall.astype(np.int32)
+    alpha = np.ascontiguousarray(hs.d["alpha"].download())
+    diff = np.ascontiguousarray(hs.d["diff"].download())
+    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    secs = np.zeros(2)
+    f = R.ref3_heatsink_steps
+    f.argtypes = [ctypes.c_int] * 5 + [ctypes.c_double] * 2 + [ctypes.c_void_p] * 8
+    nt = 4
+    f(lx, ly, lz, nt, nt, 0.1, 0.03, P(alpha), P(diff), P(walls), P(inlet), P(outlet), P(flux), P(bottom), P(secs))
+    per = secs.sum() / (2 * nt)
+    nt = max(4, int(target_seconds / max(per, 1e-9) / 2))
+    f(lx, ly, lz, nt, nt, 0.1, 0.03, P(alpha), P(diff), P(walls), P(inlet), P(outlet), P(flux), P(bottom), P(secs))
+    mlups = N * 2 * nt / secs.sum() / 1e6
+    return {"value": round(mlups, 3), "unit": "MLUPS", "cores": int(oracle_lib().ol_num_threads()),
+            "kind": "reference",
+            "sample": "%d forward + %d adjoint steps of the same 128x128x64 duct from rest with the reference's "
+                      "plain OpenMP functions on D3Q15+D3Q15 (the reference has no D3Q7): fwd %.2f s, adj %.2f s"
+                      % (nt, nt, secs[0], secs[1]),
+            "host": host_info()}
+
+
+def c3_precision_extra(ctx, a, r64, dfds64, prec):
+    """The 3D evaluation with FP16 / FP32 population storage; error against the FP64 evaluation."""
+    import numpy as np
+
+    from panslbm2_amd import D3Q7
+    from panslbm2_amd.heatsink3d import Heatsink3D
+
+    hs = Heatsink3D(ctx, a.lx3, a.ly3, a.lz3, ad_kind=D3Q7, max_steps=a.max_steps, use_graphs=not a.no_graphs,
+                    prec=prec)
+    hs.evaluate()  # captures the graphs
+    r = hs.evaluate()
+    dfds = hs.d["dfds"].download()
+    out = {"mlups": round(hs.N * (r["fwd_steps"] + r["adj_steps"]) / r["seconds"] / 1e6, 1),
+           "seconds": round(r["seconds"], 4), "fwd_steps": r["fwd_steps"], "adj_steps": r["adj_steps"],
+           "steps_in_reduced_precision": list(r["fp32_steps"]),
+           "objective_rel_err_vs_fp64": abs(r["objective"] - r64["objective"]) / abs(r64["objective"]),
+           "dfds_max_err_rel_to_max_vs_fp64": float(np.abs(dfds - dfds64).max() / np.abs(dfds64).max()),
+           "residuals": [max(r["fwd_residual"]), max(r["adj_residual"])]}
+    hs.close()
+    return out
 
 
 def big_grid_roofline(ctx, lx=4096, ly=4096, use_graphs=True, prec=0):
@@ -294,6 +413,34 @@ def kernel_only(ctx, a):
 
     if a.kernel_only == "unet":
         return unet_only(ctx, a)
+    if a.kernel_only in ("fwd3", "adj3"):
+        from panslbm2_amd import D3Q7
+        from panslbm2_amd.heatsink3d import Heatsink3D
+
+        hs = Heatsink3D(ctx, a.lx3, a.ly3, a.lz3, ad_kind=D3Q7, max_steps=a.steps, use_graphs=False,
+                        filter_radius=1.0)
+        hs.init_forward()
+        if a.kernel_only == "adj3":
+            hs.fwd_buf = 0
+            hs.init_adjoint()
+            step = lambda t: hs._adj_step(t, 0)
+            nbytes = BYTES_ADJ_Q15Q7 * hs.N
+        else:
+            step = hs._fwd_step
+            nbytes = BYTES_FWD_Q15Q7 * hs.N
+        for t in range(1, 3):
+            step(t)
+        ctx.sync()
+        ctx.record(0)
+        for t in range(3, 3 + a.steps):
+            step(t)
+        ctx.record(1)
+        ms = ctx.elapsed_ms(0, 1) / a.steps
+        hs.close()
+        print(json.dumps({"kernel_only": a.kernel_only, "grid": [a.lx3, a.ly3, a.lz3], "launches": a.steps,
+                          "us_per_launch": round(ms * 1e3, 3), "algorithmic_bytes": nbytes,
+                          "achieved_GBs": round(nbytes / (ms * 1e-3) / 1e9, 1)}))
+        return
     if a.kernel_only == "stream":
         lat = Lattice(ctx, D2Q9, a.lx, a.ly)
         check(ctx.lib.lbm_stream(lat.h))
@@ -331,40 +478,89 @@ def kernel_only(ctx, a):
                       "achieved_GBs": round(nbytes / (ms * 1e-3) / 1e9, 1)}))
 
 
+def heatsink2d_extra(ctx, a):
+    """The 2D heat sink at 200 x 400 (D2Q9 + D2Q5, FP64): evaluations, its fused forward step's
+    roofline, the same kernel on 4096^2, FP32 / FP16 storage, the reference pairing D2Q9 + D2Q9 and
+    the reference CPU path on it."""
+    from panslbm2_amd import D2Q5, D2Q9, F16
+    from panslbm2_amd.heatsink2d import Heatsink2D
+
+    hs = Heatsink2D(ctx, lx=a.lx, ly=a.ly, ad_kind=D2Q5, max_steps=a.max_steps, use_graphs=not a.no_graphs)
+    hs.evaluate()
+    runs = [hs.evaluate() for _ in range(2)]
+    r = runs[-1]
+    secs = sum(x["seconds"] for x in runs)
+    ups = sum(hs.N * (x["fwd_steps"] + x["adj_steps"]) for x in runs)
+    ms_step = kernel_rate(hs)
+    ach = BYTES_FWD_Q9Q5 * hs.N / (ms_step * 1e-3) / 1e9
+    out = {"grid": [a.lx, a.ly], "lattices": "D2Q9 NS + D2Q5 AD", "precision": "fp64",
+           "mlups": round(ups / secs / 1e6, 1), "s_per_evaluation": round(secs / len(runs), 4),
+           "fwd_steps": r["fwd_steps"], "adj_steps": r["adj_steps"], "objective": r["objective"],
+           "roofline": {"achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(ach / HBM_PEAK_GBS, 4), "us_per_launch": round(ms_step * 1e3, 3),
+                        "traffic": TRAFFIC_FWD_200x400 if (a.lx, a.ly) == (200, 400) else None,
+                        "note": "working set ~20 MB per step: resident in the 256 MB Infinity Cache"}}
+    hs.close()
+    if not a.no_big:
+        ms_big, nbig = big_grid_roofline(ctx, use_graphs=not a.no_graphs)
+        ach_big = BYTES_FWD_Q9Q5 * nbig / (ms_big * 1e-3) / 1e9
+        out["roofline_hbm_4096x4096"] = {"achieved": round(ach_big, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                         "frac": round(ach_big / HBM_PEAK_GBS, 4),
+                                         "us_per_launch": round(ms_big * 1e3, 1),
+                                         "mlups": round(nbig / (ms_big * 1e-3) / 1e6, 1)}
+    if not a.no_fp32:
+        out["fp32"] = fp32_extra(ctx, a, r)
+        out["fp16"] = fp32_extra(ctx, a, r, prec=F16)
+    # like-for-like with the reference's lattice pairing (D2Q9 advection-diffusion)
+    hs9 = Heatsink2D(ctx, lx=a.lx, ly=a.ly, ad_kind=D2Q9, max_steps=a.max_steps, use_graphs=not a.no_graphs)
+    r9 = hs9.evaluate()
+    out["mlups_d2q9_d2q9"] = round(hs9.N * (r9["fwd_steps"] + r9["adj_steps"]) / r9["seconds"] / 1e6, 1)
+    if not a.no_cpu:
+        out["cpu_baseline_d2q9_d2q9"] = cpu_baseline(hs9, target_seconds=8.0)
+    hs9.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--lx", type=int, default=200)
+    ap.add_argument("--lx3", type=int, default=128)
+    ap.add_argument("--ly3", type=int, default=128)
+    ap.add_argument("--lz3", type=int, default=64)
+    ap.add_argument("--lx", type=int, default=200, help="2D grid (and the 2D --kernel-only modes)")
     ap.add_argument("--ly", type=int, default=400)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-big", action="store_true")
+    ap.add_argument("--no-2d", action="store_true", help="skip the 2D heat-sink block")
     ap.add_argument("--max-steps", type=int, default=400000, help="cap on forward and on adjoint steps")
     ap.add_argument("--no-graphs", action="store_true", help="launch every step (for profilers)")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-transient", action="store_true")
     ap.add_argument("--batch", type=int, default=64, help="designs in the batched evaluation (0: skip it)")
     ap.add_argument("--warm-iters", type=int, default=10, help="3D design-loop iterations per warm-start mode (0: skip)")
-    ap.add_argument("--kernel-only", choices=["fwd", "adj", "stream", "unet"],
-                    help="launch only the fused forward step, the fused adjoint step or the pull-stream copy "
-                         "--steps times on --lx x --ly (profiling and counter calibration), or time the "
-                         "warm-start network's passes at 96x96x48")
+    ap.add_argument("--kernel-only", choices=["fwd", "adj", "stream", "unet", "fwd3", "adj3"],
+                    help="launch only the fused 2D forward / adjoint step (--lx x --ly), the pull-stream copy, "
+                         "or the fused 3D forward / adjoint step (--lx3 x --ly3 x --lz3) --steps times "
+                         "(profiling and counter calibration), or time the warm-start network's passes at 96x96x48")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (requested --gpus %d)" % a.gpus,
               file=sys.stderr)
         sys.exit(2)
 
-    from panslbm2_amd import D2Q5, D2Q9, Context
-    from panslbm2_amd.heatsink2d import Heatsink2D
+    from panslbm2_amd import D3Q7, F16, F32, Context
+    from panslbm2_amd.heatsink3d import Heatsink3D
 
     ctx = Context(0)
     if a.kernel_only:
-        return kernel_only(ctx, a)
-    hs = Heatsink2D(ctx, lx=a.lx, ly=a.ly, ad_kind=D2Q5, max_steps=a.max_steps, use_graphs=not a.no_graphs)
-    print("bench: heatsink %dx%d D2Q9+D2Q5 FP64, warmup %d, steps %d" % (a.lx, a.ly, a.warmup, a.steps),
-          file=sys.stderr, flush=True)
+        kernel_only(ctx, a)
+        ctx.close()
+        return
+    hs = Heatsink3D(ctx, a.lx3, a.ly3, a.lz3, ad_kind=D3Q7, max_steps=a.max_steps, use_graphs=not a.no_graphs)
+    print("bench: 3D heatsink %dx%dx%d D3Q15+D3Q7 FP64, warmup %d, steps %d" % (a.lx3, a.ly3, a.lz3, a.warmup,
+                                                                                  a.steps), file=sys.stderr, flush=True)
     for w in range(a.warmup):
         r = hs.evaluate()
         print("warmup %d: %.3f s fwd %d adj %d" % (w, r["seconds"], r["fwd_steps"], r["adj_steps"]), file=sys.stderr,
@@ -377,46 +573,44 @@ def main():
         r = hs.evaluate()
         runs.append(r)
         total_updates += hs.N * (r["fwd_steps"] + r["adj_steps"])
-        print("step %d: %.3f s fwd %d adj %d obj %.6e" % (s, r["seconds"], r["fwd_steps"], r["adj_steps"],
+        print("step %d: %.3f s fwd %d adj %d obj %.9e" % (s, r["seconds"], r["fwd_steps"], r["adj_steps"],
                                                            r["objective"]), file=sys.stderr, flush=True)
     ctx.sync()
     wall = time.perf_counter() - t0
     mlups = total_updates / wall / 1e6
+    dfds64 = hs.d["dfds"].download()
 
     ms_step = kernel_rate(hs)
-    achieved = BYTES_FWD_Q9Q5 * hs.N / (ms_step * 1e-3) / 1e9
+    achieved = BYTES_FWD_Q15Q7 * hs.N / (ms_step * 1e-3) / 1e9
+    big = (a.lx3, a.ly3, a.lz3) == (128, 128, 64)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": TRAFFIC_FWD_200x400 if (a.lx, a.ly) == (200, 400) else None,
-                "kernel": "k_collide_fused<D2Q9,D2Q5,ADBody> (pull + boundary program + corner smoothing + "
-                          "collide in one pass), per fused forward step",
-                "bytes_per_launch": BYTES_FWD_Q9Q5 * hs.N, "us_per_launch": round(ms_step * 1e3, 3),
-                "note": "working set ~20 MB per step: resident in the 256 MB Infinity Cache"}
-    extra = {}
-    if not a.no_big:
-        ms_big, nbig = big_grid_roofline(ctx, use_graphs=not a.no_graphs)
-        ach_big = BYTES_FWD_Q9Q5 * nbig / (ms_big * 1e-3) / 1e9
-        extra["roofline_hbm_4096x4096"] = {"achieved": round(ach_big, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                           "frac": round(ach_big / HBM_PEAK_GBS, 4),
-                                           "us_per_launch": round(ms_big * 1e3, 1),
-                                           "mlups": round(nbig / (ms_big * 1e-3) / 1e6, 1)}
+                "traffic": TRAFFIC_FWD_128x128x64 if big else None,
+                "kernel": "k_collide_fused<D3Q15,D3Q7,ADBody> (pull + boundary program + collide in one pass), "
+                          "per fused forward step",
+                "bytes_per_launch": BYTES_FWD_Q15Q7 * hs.N, "us_per_launch": round(ms_step * 1e3, 3),
+                "note": "algorithmic bytes: D3Q15 + D3Q7 populations read and written once (352 B) + alpha, "
+                        "diffusivity, heat source (24 B) per node; working set ~0.8 GB per step (HBM)"}
+    ms_adj = kernel_rate_adj(hs)
+    ach_adj = BYTES_ADJ_Q15Q7 * hs.N / (ms_adj * 1e-3) / 1e9
+    extra = {"c3_adjoint": {"kernel": "k_collide_fused<D3Q15,D3Q7,AADBody>, per fused adjoint step",
+                            "bytes_per_launch": BYTES_ADJ_Q15Q7 * hs.N, "us_per_launch": round(ms_adj * 1e3, 3),
+                            "achieved": round(ach_adj, 1), "frac": round(ach_adj / HBM_PEAK_GBS, 4),
+                            "traffic": TRAFFIC_ADJ_128x128x64 if big else None}}
+    cpu = None if a.no_cpu else cpu_baseline_3d(hs)
+    r64 = runs[-1]
+    hs.close()
     if not a.no_fp32:
-        from panslbm2_amd import F16
-
-        extra["fp32"] = fp32_extra(ctx, a, runs[-1])
-        extra["fp16"] = fp32_extra(ctx, a, runs[-1], prec=F16)
+        extra["c3_fp16"] = c3_precision_extra(ctx, a, r64, dfds64, F16)
+        extra["c3_fp32"] = c3_precision_extra(ctx, a, r64, dfds64, F32)
+    if not a.no_2d:
+        extra["heatsink2d"] = heatsink2d_extra(ctx, a)
     if a.batch > 0:
         extra["batch"] = batch_extra(ctx, a.batch)
     if a.warm_iters > 0:
         extra["warm_start"] = warm_extra(ctx, a.warm_iters, use_graphs=not a.no_graphs)
     if not a.no_transient:
         extra["transient"] = transient_extra(ctx)
-    # like-for-like with the reference's lattice pairing (D2Q9 advection-diffusion)
-    hs9 = Heatsink2D(ctx, lx=a.lx, ly=a.ly, ad_kind=D2Q9, max_steps=a.max_steps, use_graphs=not a.no_graphs)
-    r9 = hs9.evaluate()
-    extra["mlups_d2q9_d2q9"] = round(hs9.N * (r9["fwd_steps"] + r9["adj_steps"]) / r9["seconds"] / 1e6, 1)
-    cpu = None if a.no_cpu else cpu_baseline(hs9)
-    hs9.close()
 
     out = {
         "metric": "MLUPS (fwd+adjoint) & % HBM peak; wall-time to steady, warm vs cold",
@@ -431,15 +625,16 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded uniform design field, density-filtered)",
-        "config": {"workload": "2D heatsink design evaluation (forward+adjoint to 1e-6, sensitivity), cold start",
-                   "grid": [a.lx, a.ly], "lattices": "D2Q9 NS + D2Q5 AD", "precision": "fp64",
-                   "fwd_steps": runs[-1]["fwd_steps"], "adj_steps": runs[-1]["adj_steps"],
-                   "wall_s_to_steady_cold": round(runs[-1]["seconds"], 3)},
+        "config": {"workload": "3D heatsink design evaluation, 128x128x64 D3Q15 NS + D3Q7 AD (forward + adjoint "
+                               "to 1e-6, sensitivity), cold start",
+                   "grid": [a.lx3, a.ly3, a.lz3], "lattices": "D3Q15 NS + D3Q7 AD", "precision": "fp64",
+                   "fwd_steps": r64["fwd_steps"], "adj_steps": r64["adj_steps"],
+                   "wall_s_to_steady_cold": round(r64["seconds"], 3), "objective": r64["objective"]},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
     out.update(extra)
-    hs.close()
+    ctx.close()
     print(json.dumps(out))
 
 
