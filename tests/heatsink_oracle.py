@@ -114,3 +114,108 @@ def run(lx, ly, ad_kind, seed=0, nu=0.1, pr=6.0, umax=0.05, heat=1e-3, alpha_max
                 "iqx": IQ[ab][0].copy(), "iqy": IQ[ab][1].copy(), "ig": ig, "dfds": dfds,
                 "objective": objsum / int(objedge.sum()), "adj_steps": t, "adj_res": (ru, rq), "gamma": s})
     return out
+
+
+def run3d(lx, ly, lz, ad_kind, seed=0, nu=0.1, pr=6.0, uin=0.03, heat=1e-3, plate=8, alpha_max=1e4, qf=0.01,
+          qg=1.0, filter_radius=2.0, eps=1e-6, check_every=100, max_steps=400000):
+    """CPU-oracle replica of panslbm2_amd/heatsink3d.py (Heatsink3D, D3Q15 NS): the duct's design,
+    coefficients, forward and adjoint step order, residual test and sensitivity."""
+    from _oracle import D3Q15
+
+    O = lib()
+    N = lx * ly * lz
+    n = np.arange(N)
+    i, j, k = n % lx, (n // lx) % ly, n // (lx * ly)
+    rng = np.random.default_rng(seed)
+    raw = np.clip(rng.uniform(0.0, 1.0, N), 0.0, 1.0)
+    s = np.zeros(N)
+    O.ol_density_filter(lx, ly, lz, D(filter_radius), ptr(raw), ptr(s))
+    s[k < plate] = 0.0
+    dfl = nu / pr
+    dso = 10.0 * dfl
+    alpha = alpha_max / (ly - 1.0) * qf * (1.0 - s) / (s + qf)
+    dads = -alpha_max / (ly - 1.0) * qf * (1.0 + qf) / (s + qf) ** 2
+    diff = dso + (dfl - dso) * s * (1.0 + qg) / (s + qg)
+    dkds = (dfl - dso) * qg * (1.0 + qg) / (s + qg) ** 2
+    hsrc = np.where(k < plate, heat, 0.0)
+    wall = (j == 0) | (j == ly - 1) | (k == 0) | (k == lz - 1)
+    inlet = ((i == 0) & ~wall).astype(np.int32)
+    outlet = ((i == lx - 1) & ~wall).astype(np.int32)
+    flux = ((i > 0) | wall).astype(np.int32)
+    bottom = (k == 0).astype(np.int32)
+    F = lambda kind, a, dt=np.float64: to_face(kind, lx, ly, lz, a, dt)
+    zf, ones = np.zeros(N), np.ones(N)
+    fw, fin, fout = F(D3Q15, wall.astype(np.int32), np.int32), F(D3Q15, inlet, np.int32), F(D3Q15, outlet, np.int32)
+    fuin, fz, f1 = F(D3Q15, np.full(N, uin)), F(D3Q15, zf), F(D3Q15, ones)
+    gin, gfl, gobj, gz = F(ad_kind, inlet, np.int32), F(ad_kind, flux, np.int32), F(ad_kind, bottom, np.int32), \
+        F(ad_kind, zf)
+    pf, pg = Lattice(D3Q15, lx, ly, lz), Lattice(ad_kind, lx, ly, lz)
+    z = lambda: np.zeros(N)
+    rho, tem = np.ones(N), z()
+    U = [[z(), z(), z()], [z(), z(), z()]]
+    Q = [[z(), z(), z()], [z(), z(), z()]]
+    g = np.zeros((NC[ad_kind], N))
+    O.ol_ns_init(pf.h, ptr(rho), *[ptr(a) for a in U[0]])
+    O.ol_ad_init(pg.h, ptr(tem), *[ptr(a) for a in U[0]])
+    res3 = lambda a, b: O.ol_residual(3, *[ptr(x) for x in a], *[ptr(x) for x in b], N)
+    t = 0
+    while True:
+        t += 1
+        b = t % 2
+        O.ol_ad_collide_fc(pf.h, ptr(rho), *[ptr(a) for a in U[b]], ptr(alpha), D(nu), pg.h, ptr(tem),
+                           *[ptr(a) for a in Q[b]], ptr(diff), 1, ptr(g) if t % check_every == 0 else None, ptr(hsrc))
+        if t % check_every == 0:
+            ru, rq = res3(U[b], U[1 - b]), res3(Q[b], Q[1 - b])
+            if (ru < eps and rq < eps) or t >= max_steps:
+                break
+        O.ol_stream(pf.h)
+        O.ol_stream(pg.h)
+        O.ol_bc(pf.h, ptr(fw), 0)
+        O.ol_ns_set_u(pf.h, ptr(fin), ptr(fuin), ptr(fz), ptr(fz))
+        O.ol_ns_set_rho(pf.h, ptr(fout), ptr(f1), ptr(fz), ptr(fz))
+        O.ol_ad_set_t(pg.h, ptr(gin), ptr(gz), *[ptr(a) for a in U[b]])
+        O.ol_ad_set_q(pg.h, ptr(gfl), ptr(gz), *[ptr(a) for a in U[b]], ptr(diff), D(0.0))
+    fwd_steps, fb = t, t % 2
+    out = {"fwd_f": pf.get(), "fwd_g": pg.get(), "rho": rho.copy(), "tem": tem.copy(), "g": g.copy(),
+           "fwd_steps": fwd_steps, "fwd_res": (ru, rq), "gamma": s}
+    for a, name in enumerate(("ux", "uy", "uz")):
+        out[name] = U[fb][a].copy()
+    uf = U[fb]
+    ip, item = z(), z()
+    IM = [z(), z(), z()]
+    IU = [[z(), z(), z()], [z(), z(), z()]]
+    IQ = [[z(), z(), z()], [z(), z(), z()]]
+    ig = np.zeros((NC[ad_kind], N))
+    O.ol_ans_init(pf.h, *[ptr(a) for a in uf], ptr(ip), *[ptr(a) for a in IU[0]])
+    O.ol_aad_init(pg.h, *[ptr(a) for a in uf], ptr(item), *[ptr(a) for a in IQ[0]])
+    t = 0
+    while True:
+        t += 1
+        b = t % 2
+        O.ol_aad_collide_fc(pf.h, ptr(rho), *[ptr(a) for a in uf], ptr(ip), *[ptr(a) for a in IU[b]],
+                            *[ptr(a) for a in IM], ptr(alpha), D(nu), pg.h, ptr(tem), ptr(item),
+                            *[ptr(a) for a in IQ[b]], ptr(diff), 1, ptr(ig) if t % check_every == 0 else None)
+        if t % check_every == 0:
+            ru, rq = res3(IU[b], IU[1 - b]), res3(IQ[b], IQ[1 - b])
+            if (ru < eps and rq < eps) or t >= max_steps:
+                break
+        O.ol_istream(pf.h)
+        O.ol_istream(pg.h)
+        O.ol_aad_iset_t(pg.h, ptr(gin), *[ptr(a) for a in uf])
+        O.ol_aad_iset_q(pg.h, ptr(gfl), *[ptr(a) for a in uf], D(0.0))
+        O.ol_aad_iset_q(pg.h, ptr(gobj), *[ptr(a) for a in uf], D(1.0))
+        O.ol_bc(pf.h, ptr(fw), 1)
+        O.ol_ans_iset_u(pf.h, ptr(fin), ptr(fuin), ptr(fz), ptr(fz), D(0.0), 1)
+        O.ol_ans_iset_rho(pf.h, ptr(fout))
+    ab = t % 2
+    dfds = z()
+    O.ol_sens_brinkman_diffusivity(pg.h, ptr(dfds), *[ptr(a) for a in uf], *[ptr(a) for a in IM], ptr(dads),
+                                   ptr(tem), ptr(item), *[ptr(a) for a in IQ[ab]], ptr(g), ptr(ig), ptr(diff),
+                                   ptr(dkds))
+    objsum = 0.0
+    for m in np.nonzero(bottom)[0]:
+        objsum += tem[m]
+    out.update({"adj_f": pf.get(), "adj_g": pg.get(), "ip": ip, "imx": IM[0], "imy": IM[1], "imz": IM[2],
+                "item": item, "iqx": IQ[ab][0].copy(), "iqy": IQ[ab][1].copy(), "iqz": IQ[ab][2].copy(), "ig": ig,
+                "dfds": dfds, "objective": objsum / int(bottom.sum()), "adj_steps": t, "adj_res": (ru, rq)})
+    return out

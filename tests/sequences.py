@@ -885,6 +885,8 @@ class NaturalConvectionBGK2D:
 
 SEQUENCES = [
     Cavity("cavity_21x21", 21, 21, 300),
+    # the lid-driven cavity workload: 101 x 101 nodes (L = 100, Re = u0 L / nu = 100), 10,000 steps
+    Cavity("cavity_101x101", 101, 101, 10000),
     ForcedConvection2D("fc2d_24x16", 24, 16, 200, check=100),
     ForcedConvection3D("fc3d_7x5x5", 7, 5, 5, 60, check=30),
     ForcedConvection2D("fc2d_noswap_20x12", 20, 12, 120, swap=False),

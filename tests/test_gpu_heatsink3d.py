@@ -64,3 +64,52 @@ def test_mixed_precision_evaluation(ctx, ns_kind):
     (o64, s64), (o32, s32) = out[pa.F64], out[pa.F32]
     assert abs(o32 - o64) <= 1e-5 * abs(o64)
     assert np.max(np.abs(s32 - s64)) <= 1e-4 * np.max(np.abs(s64))
+
+
+@pytest.mark.parametrize("graphs", [True, False], ids=["graphs", "nographs"])
+def test_heatsink3d_driver_matches_oracle(ctx, graphs):
+    """The 3D heat-sink driver (fused steps, graphs, field rebinding, boundary-only saves) against its
+    CPU-oracle replica (tests/heatsink_oracle.py run3d): populations, fields and the objective
+    value-identical, the sensitivity (pow-based in the reference) within 1e-12 relative."""
+    import heatsink_oracle
+    import panslbm2_amd as pa
+    from panslbm2_amd.heatsink3d import Heatsink3D
+
+    print("heatsink3d 14x10x9 vs oracle", "graphs" if graphs else "eager", flush=True)
+    kw = dict(lx=14, ly=10, lz=9, plate=3, seed=4, max_steps=400, eps=1e-30)
+    hs = Heatsink3D(ctx, use_graphs=graphs, **kw)
+    r = hs.evaluate()
+    want = heatsink_oracle.run3d(ad_kind=pa.D3Q7, **kw)
+    assert r["fwd_steps"] == want["fwd_steps"] == 400 and r["adj_steps"] == want["adj_steps"] == 400
+    np.testing.assert_array_equal(hs.gamma, want["gamma"])
+    d, fb, ab = hs.d, hs.fwd_buf, hs.adj_buf
+    got = {"rho": d["rho"], "tem": d["tem"], "ux": d["ux%d" % fb], "uy": d["uy%d" % fb], "uz": d["uz%d" % fb],
+           "g": d["g"], "ip": d["ip"], "imx": d["imx"], "imy": d["imy"], "imz": d["imz"], "item": d["item"],
+           "iqx": d["iqx%d" % ab], "iqy": d["iqy%d" % ab], "iqz": d["iqz%d" % ab], "ig": d["ig"]}
+    for k, v in got.items():
+        np.testing.assert_array_equal(v.download(), want[k], err_msg=k)
+    np.testing.assert_array_equal(hs.pf.download(), want["adj_f"])
+    np.testing.assert_array_equal(hs.pg.download(), want["adj_g"])
+    assert r["objective"] == want["objective"]
+    dfds, wd = d["dfds"].download(), want["dfds"]
+    assert np.max(np.abs(dfds - wd)) <= 1e-12 * np.max(np.abs(wd))
+    hs.close()
+
+
+def test_heatsink3d_fp16_error_vs_fp64_oracle(ctx):
+    """FP16 population storage (FP32 collide, FP64 from the residual floor on) on the 3D duct: objective
+    and sensitivity against the FP64 oracle replica, both run to the same step counts."""
+    import heatsink_oracle
+    import panslbm2_amd as pa
+    from panslbm2_amd.heatsink3d import Heatsink3D
+
+    kw = dict(lx=14, ly=10, lz=9, plate=3, seed=4, eps=1e-9, max_steps=200000)
+    want = heatsink_oracle.run3d(ad_kind=pa.D3Q7, **kw)
+    hs = Heatsink3D(ctx, prec=pa.F16, **kw)
+    r = hs.evaluate()
+    print("fp16 3d", r["fwd_steps"], r["adj_steps"], r["fp32_steps"], r["objective"], want["objective"], flush=True)
+    assert r["fp32_steps"][0] > 0 and r["fp32_steps"][1] > 0
+    assert abs(r["objective"] / want["objective"] - 1.0) < 1e-6
+    dfds, wd = hs.d["dfds"].download(), want["dfds"]
+    assert np.max(np.abs(dfds - wd)) <= 1e-4 * np.max(np.abs(wd))
+    hs.close()
