@@ -24,7 +24,9 @@
  *  - issave of the collide calls: 0 writes no fields, 1 writes every node (the reference's
  *    `issave = true`), 2 writes the fields of boundary (face) nodes only -- all the next step's
  *    boundary calls read, since they read the saved velocity at their own node
- *    (advection.h:107) -- and no snapshot.  Values written are the same in modes 1 and 2.
+ *    (advection.h:107) -- and no snapshot; 3, for the advection-diffusion collides (lbm_ad_*),
+ *    writes only the velocity of boundary nodes (all the advection-diffusion boundary operators
+ *    read; other collides write nothing for 3).  Values written are the same in every mode.
  *  - Populations are kept on the device in plane order (population c of node idx at c*N + idx).
  *    lbm_lattice_upload/download move them in that order; the *_aos variants use the
  *    reference's f0[N] + f[(nc-1)*idx + c-1] layout (d2q9.h:88-90).

@@ -151,7 +151,7 @@ struct lbm_program {
     bool ordered_ok = true;  // no operator on a lattice after its SmoothCorner
     // device tables
     bool uploaded = false;
-    DevBuf d_slot_entry, d_entry_node, d_entry_ops, d_ops, d_sops;
+    DevBuf d_slot_entry, d_entry_node, d_entry_ops, d_ops, d_sops, d_slot_list;
     DevBuf d_side_of_slot, d_corner2d, d_side, d_side_nodes;  // 2D corner smoothing
     int nentry = 0, nslot = 0, maxops = 0, n_corner2d = 0, n_side = 0;
     bool corner_block_ok = false;  // the corners may be done by the fused pass's last workgroup
@@ -162,6 +162,7 @@ struct lbm_program {
         d_entry_ops.release();
         d_ops.release();
         d_sops.release();
+        d_slot_list.release();
         d_side_of_slot.release();
         d_corner2d.release();
         d_side.release();
@@ -362,15 +363,30 @@ int prog_upload(lbm_program *P) {
         if (s < 0) return fail(LBM_ERR_STATE, "boundary operator on an interior node");
         slot_entry[s] = e;
     }
-    // slot-major copy for the fused kernels: slot s's operators at [s * maxops, ...), OP_NONE-padded
+    // operator lists for the fused kernels: each boundary slot's operators, OP_NONE-padded to maxops,
+    // stored once per distinct list (a face's nodes mostly share theirs), slot -> list index
     int most = 0;
     for (const int2 &r : entry_ops) most = std::max(most, r.y);
     P->maxops = (most + lbm::kOpBatch - 1) / lbm::kOpBatch * lbm::kOpBatch;
-    std::vector<Op> sops((size_t)P->nslot * P->maxops);
+    std::vector<Op> sops;
+    std::vector<int> slot_list(P->nslot, -1);
+    std::map<std::string, int> list_of;
+    std::vector<Op> one(P->maxops);
     for (int s = 0; s < P->nslot; ++s) {
         int e = slot_entry[s];
         if (e < 0) continue;
-        for (int t = 0; t < entry_ops[e].y; ++t) sops[(size_t)s * P->maxops + t] = ops[entry_ops[e].x + t];
+        std::fill(one.begin(), one.end(), Op{});
+        for (int t = 0; t < entry_ops[e].y; ++t) one[t] = ops[entry_ops[e].x + t];
+        // the list's operator count rides in its first operator, so a node stops after its own
+        // batches (0 when it does not fit a byte: the kernels then walk all maxops)
+        one[0].pad[0] = entry_ops[e].y <= 255 ? (uint8_t)entry_ops[e].y : 0;
+        std::string key(reinterpret_cast<const char *>(one.data()), one.size() * sizeof(Op));
+        auto it = list_of.find(key);
+        if (it == list_of.end()) {
+            it = list_of.emplace(std::move(key), (int)(sops.size() / P->maxops)).first;
+            sops.insert(sops.end(), one.begin(), one.end());
+        }
+        slot_list[s] = it->second;
     }
     // 2D corners are done from their neighbours' post-boundary states, by the fused pass's last
     // workgroup or, when the boundary operators read arrays the collide saves, by a corner kernel
@@ -422,6 +438,7 @@ int prog_upload(lbm_program *P) {
     if ((rc = up(P->d_entry_ops, entry_ops.data(), entry_ops.size() * sizeof(int2)))) return rc;
     if ((rc = up(P->d_ops, ops.data(), ops.size() * sizeof(Op)))) return rc;
     if ((rc = up(P->d_sops, sops.data(), sops.size() * sizeof(Op)))) return rc;
+    if ((rc = up(P->d_slot_list, slot_list.data(), slot_list.size() * sizeof(int)))) return rc;
     if ((rc = up(P->d_side_of_slot, side_of_slot.data(), side_of_slot.size() * sizeof(int)))) return rc;
     if ((rc = up(P->d_corner2d, corner2d.data(), corner2d.size() * sizeof(int4)))) return rc;
     if ((rc = up(P->d_side_nodes, side_nodes.data(), side_nodes.size() * sizeof(int)))) return rc;
@@ -437,6 +454,7 @@ ProgView prog_view(const lbm_program *P) {
     v.entry_ops = (const int2 *)P->d_entry_ops.p;
     v.ops = (const Op *)P->d_ops.p;
     v.sops = (const Op *)P->d_sops.p;
+    v.slot_list = (const int *)P->d_slot_list.p;
     v.maxops = P->maxops;
     v.side_of_slot = P->n_corner2d > 0 ? (const int *)P->d_side_of_slot.p : nullptr;
     v.side = P->n_corner2d > 0 ? (double *)P->d_side.p : nullptr;

@@ -109,6 +109,10 @@ struct ADBody<LA, LB, double> {
                 qz[n] = hz;
             }
             if (gsnap && save == 1) store_node<LB>(gsnap, N, n, g);
+        } else if (save == 3 && bnd) {  // the velocity the boundary operators read
+            ux[n] = vx;
+            uy[n] = vy;
+            if constexpr (LA::nd == 3) uz[n] = vz;
         }
         {
             double feq[LA::nc];
@@ -389,6 +393,10 @@ struct ADBody<LA, LB, float> {
                 qz[n] = hz;
             }
             if (gsnap && save == 1) store_snapshot<LB>(gsnap, N, n, g, eb, ob);
+        } else if (save == 3 && bnd) {  // the velocity the boundary operators read
+            ux[n] = vx;
+            uy[n] = vy;
+            if constexpr (LA::nd == 3) uz[n] = vz;
         }
         {
             float feq[LA::nc];
@@ -659,9 +667,10 @@ __global__ void __launch_bounds__(kBlock) k_collide_corner2d(const S *__restrict
     }
     const double *s0 = pv.side + (size_t)e.y * kSideStride, *s1 = pv.side + (size_t)e.z * kSideStride;
     int slot = boundary_slot(i, j, k, g.nx, g.ny, g.nz, LA::nd);
-    const Op *p = pv.sops + (size_t)slot * pv.maxops;
+    const int li = pv.slot_list[slot];
+    const Op *p = pv.sops + (size_t)(li < 0 ? 0 : li) * pv.maxops;
     NodeFieldCache fc = prefetch_fields(pv, n);
-    for (int base = 0; base < pv.maxops; base += kOpBatch) {
+    for (int base = 0; li >= 0 && base < pv.maxops; base += kOpBatch) {
         Op o[kOpBatch];
         sfor<0, kOpBatch>([&](auto q) { o[decltype(q)::value] = p[base + decltype(q)::value]; });
         sfor<0, kOpBatch>([&](auto q) {

@@ -169,15 +169,20 @@ __device__ __forceinline__ void apply_op(const Op &op, double *a, double *b, con
 }
 
 // Run the operators of boundary slot `slot` (node n) except SMOOTH, in record order.  The slot's
-// operators sit at sops[slot * maxops ...], padded with OP_NONE to a multiple of kOpBatch, so a
-// batch is fetched with independent loads together with the node's fields.
+// operator list (shared by every slot with the same operators) sits at sops[list * maxops ...],
+// padded with OP_NONE to a multiple of kOpBatch, its count in the first operator; a batch is
+// fetched with independent loads together with the node's fields.
 template <class LA, class LB>
 __device__ __forceinline__ void run_slot_ops(const ProgView &pv, int slot, int n, double *a, double *b) {
-    const Op *p = pv.sops + (size_t)slot * pv.maxops;
+    const int li = pv.slot_list[slot];
+    if (li < 0) return;
+    const Op *p = pv.sops + (size_t)li * pv.maxops;
     NodeFieldCache fc = prefetch_fields(pv, n);
-    for (int base = 0; base < pv.maxops; base += kOpBatch) {
+    int count = pv.maxops;  // the list's own count arrives with its first batch
+    for (int base = 0; base < count; base += kOpBatch) {
         Op o[kOpBatch];
         sfor<0, kOpBatch>([&](auto q) { o[decltype(q)::value] = p[base + decltype(q)::value]; });
+        if (base == 0 && o[0].pad[0] != 0) count = o[0].pad[0];
         sfor<0, kOpBatch>([&](auto q) {
             const Op &op = o[decltype(q)::value];
             if (op.code != OP_NONE && op.code != OP_SMOOTH) apply_op<LA, LB>(op, a, b, pv, n, fc);

@@ -37,7 +37,7 @@ struct Op {
     uint8_t arg;
     uint8_t fset;  // index of the field set the operator reads
     uint8_t seg;   // number of SmoothCorner calls recorded before this operator
-    uint8_t pad[2];
+    uint8_t pad[2];  // operator lists (sops): pad[0] of a list's first operator = its operator count
     double v[3];
     double v3;     // eps where used
 };
@@ -57,8 +57,9 @@ struct ProgView {
     const int *entry_node;  // entry -> node index
     const int2 *entry_ops;  // entry -> (first op, op count)
     const Op *ops;
-    const Op *sops;         // slot-major copy: operators of slot s at sops[s * maxops + k], OP_NONE-padded
-    int maxops;             // operators per slot in sops, a multiple of kOpBatch (0: no operators)
+    const Op *sops;         // distinct operator lists, maxops each, OP_NONE-padded
+    const int *slot_list;   // boundary slot -> its list in sops, or -1 (size nslot)
+    int maxops;             // operators per list in sops, a multiple of kOpBatch (0: no operators)
     int nentry;
     int nslot;
     FieldSet fs[kMaxFieldSets];

@@ -152,10 +152,13 @@ class Heatsink2D:
 
     # ---- steps ------------------------------------------------------------------------------
     def _save_mode(self, t, adjoint):
+        """1 (every field) on a residual step and the step before it; otherwise the forward step
+        writes only the boundary nodes' velocity its boundary operators read next (3) and the
+        adjoint step nothing."""
         r = t % self.check_every
         if r == 0 or r == self.check_every - 1:
             return 1
-        return 0 if adjoint else 2
+        return 0 if adjoint else 3
 
     def _fwd_step(self, t):
         """Forward step t (t >= 1): saves into buffer t % 2; boundary calls read buffer (t-1) % 2."""
@@ -193,8 +196,15 @@ class Heatsink2D:
         g = self._graphs.get(key)
         if g is None:
             check(self.L.lbm_graph_begin(self.ctx.h))
-            for t in range(t0 + 1, t0 + self.check_every + 1):
-                step(t)
+            try:
+                for t in range(t0 + 1, t0 + self.check_every + 1):
+                    step(t)
+            except BaseException:
+                # end the capture so the context stream works again, and drop the partial graph
+                part = VP()
+                if self.L.lbm_graph_end(self.ctx.h, ctypes.byref(part)) == 0 and part:
+                    self.L.lbm_graph_destroy(part)
+                raise
             g = VP()
             check(self.L.lbm_graph_end(self.ctx.h, ctypes.byref(g)))
             self._graphs[key] = g

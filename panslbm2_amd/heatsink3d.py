@@ -12,8 +12,11 @@ set of a duct heat sink:
   * objective: mean temperature on the bottom face z = 0, injected into the adjoint through the flux
     condition with eps = 1 (heatsink.cpp:202); the adjoint outlet is ANS::iBoundaryConditionSetRho3D
     (the reference's 3D AAD::iBoundaryConditionSetRho does not compile).
-No SmoothCorner: the corner and edge nodes lie on bounce-back walls, and the fused 3D path runs
-every operator but SmoothCorner, so each time step is one kernel pass.
+By default no SmoothCorner: the corner and edge nodes lie on bounce-back walls, and the fused 3D
+path runs every operator but SmoothCorner, so each time step is one kernel pass.  With
+smooth_corners=True both lattices are smoothed after the boundary calls of every forward and adjoint
+step, as the reference's 3D driver does (heatsink3D.cpp:175-176, 215-216); those steps run the
+exact unfused sequence (stream, boundary passes, smoothing, collide).
 The Navier-Stokes lattice is D3Q15 (the reference's) or D3Q27 (ns_kind=D3Q27, no reference
 counterpart; its velocity and pressure boundaries are non-equilibrium bounce-back).
 """
@@ -33,7 +36,7 @@ class Heatsink3D(Heatsink2D):
     def __init__(self, ctx=None, lx=128, ly=128, lz=64, ad_kind=D3Q7, seed=0, nu=0.1, pr=6.0, uin=0.03, heat=1e-3,
                  plate=8, alpha_max=1e4, qf=0.01, qg=1.0, filter_radius=2.0, eps=1e-6, check_every=100,
                  max_steps=400000, use_graphs=True, prec=F64, fp32_floor=None, fused=True, ns_kind=D3Q15,
-                 heaviside_beta=None, design_box=None):
+                 heaviside_beta=None, design_box=None, smooth_corners=False):
         assert ns_kind in (D3Q15, D3Q27)
         self.ctx = ctx or Context(0)
         self.L = self.ctx.lib
@@ -43,6 +46,7 @@ class Heatsink3D(Heatsink2D):
         self.nu, self.eps, self.check_every, self.max_steps = nu, eps, check_every, max_steps
         assert check_every % 2 == 0
         self.use_graphs, self.fused = use_graphs and fused, fused
+        self.smooth_corners = smooth_corners
         self.prec = prec
         self.fp32_floor = fp32_floor if fp32_floor is not None else (1e-3 if prec == F16 else 1e-5)
         c = self.ctx
@@ -200,6 +204,9 @@ class Heatsink3D(Heatsink2D):
                                hptr(fc["zero_f"])))
         check(L.lbm_ad_set_t(self.pg.h, hptr(fc["inlet_g"]), hptr(fc["zero_g"]), ux, uy, uz))
         check(L.lbm_ad_set_q(self.pg.h, hptr(fc["flux_g"]), hptr(fc["zero_g"]), ux, uy, uz, d["diff"].ptr, D(0.0)))
+        if self.smooth_corners:
+            check(L.lbm_smooth_corner(self.pf.h))
+            check(L.lbm_smooth_corner(self.pg.h))
 
     def _adj_boundaries(self, ux, uy, uz):
         L, fc = self.L, self.faces
@@ -212,13 +219,17 @@ class Heatsink3D(Heatsink2D):
         check(L.lbm_ans_iset_u(self.pf.h, hptr(fc["inlet_f"]), hptr(fc["uin_f"]), hptr(fc["zero_f"]),
                                hptr(fc["zero_f"]), D(0.0), 1))
         check(L.lbm_ans_iset_rho(self.pf.h, hptr(fc["outlet_f"])))
+        if self.smooth_corners:
+            check(L.lbm_smooth_corner(self.pf.h))
+            check(L.lbm_smooth_corner(self.pg.h))
 
     def _record(self, calls):
         check(self.L.lbm_record_begin(self.ctx.h))
         calls()
         prog = VP()
         check(self.L.lbm_record_end(self.ctx.h, ctypes.byref(prog)))
-        assert self.L.lbm_program_fusable(prog) == 1
+        # a 3D program with SmoothCorner runs unfused (apply, then collide); any other must fuse
+        assert self.smooth_corners or self.L.lbm_program_fusable(prog) == 1
         return prog
 
     def _record_forward(self):
@@ -426,9 +437,23 @@ class Heatsink3D(Heatsink2D):
             ws.close()
             self.warm = None
 
-    def optimize(self, iterations, vf=0.4, move=0.05, warm=None, x0=None, callback=None, optimizer="mma"):
+    # the reference driver's continuation (heatsink3D.cpp:45,100-118,402-405): (qf, beta) per stage
+    REFERENCE_STAGES = ((1e-2, 1.0), (1e-1, 2.0), (1e-1, 4.0), (1e-1, 8.0), (1e-1, 16.0))
+
+    def optimize(self, iterations, vf=0.4, move=0.05, warm=None, x0=None, callback=None, optimizer="mma",
+                 stages=None, stage_iterations=100, stage_tol=0.01):
         """Design loop from a uniform solid fraction vf above the plate (or x0): evaluate, then
-        design_update.  Returns the per-iteration evaluate() records."""
+        design_update.  Returns the per-iteration evaluate() records.
+
+        stages: a continuation schedule of (qf, beta) pairs as the reference's driver runs it
+        (REFERENCE_STAGES): stage s sets the Brinkman interpolation parameter qf and, with a
+        Heaviside projection, its sharpness beta; the loop moves to the next stage when the largest
+        design change of an update falls below stage_tol or after stage_iterations iterations in
+        the stage, and stops in the last stage once the change is below stage_tol."""
+        if stages:
+            self.qf = stages[0][0]
+            if self.heaviside_beta:
+                self.heaviside_beta = stages[0][1]
         x = np.where(self.k >= self.plate, 1.0 - vf, 0.0) if x0 is None else x0
         self.set_design(x)
         old = getattr(self, "_mma", None)
@@ -436,14 +461,29 @@ class Heatsink3D(Heatsink2D):
             old.close()
         self._mma = None
         hist = []
+        stage, cnt = 0, 0
         for it in range(iterations):
             r = self.evaluate(warm=warm)
             r["iteration"] = it
+            r["stage"] = stage
             hist.append(r)
             if callback is not None:
                 callback(r)
             if it + 1 < iterations:
+                x_old = self.x.copy()
                 self.design_update(vf, move, optimizer)
+                if stages:
+                    cnt += 1
+                    dsmax = float(np.abs(self.x - x_old).max())
+                    r["dsmax"] = dsmax
+                    if stage == len(stages) - 1 and dsmax < stage_tol:
+                        break
+                    if dsmax < stage_tol or cnt % stage_iterations == 0:
+                        stage, cnt = min(stage + 1, len(stages) - 1), 0
+                        self.qf = stages[stage][0]
+                        if self.heaviside_beta:
+                            self.heaviside_beta = stages[stage][1]
+                        self.set_design(self.x)  # the new stage's coefficients for the same design
         ws = getattr(self, "warm", None)
         if ws is not None:
             ws.net.synchronize()

@@ -122,7 +122,7 @@ class HeatsinkBatch2D:
         t = 1
         batches = self._make_batches(step_fn, 2)
         if self.tune and self.tuned is None:
-            quiet = 0 if self.adjoint else 2  # save mode of the steps between checks
+            quiet = self.hs[0]._save_mode(2, self.adjoint)  # save mode of the steps between checks
             arr = (VP * 2)(batches[(0, quiet)], batches[(1, quiet)])
             blk, xcd = I(), I()
             check(self.L.lbm_batch_tune(arr, 2, 3, ctypes.byref(blk), ctypes.byref(xcd)))

@@ -63,13 +63,8 @@ class TransientHeatsink2D:
         self.gx, self.gy = gx, (u * u / (ly - 1.0) if gy is None else gy)
         dfl = nu / pr
         dso = 10.0 * dfl
-        coef = dict(
-            alpha=alpha_max / (ly - 1.0) * qf * (1.0 - s) / (s + qf),
-            dads=-alpha_max / (ly - 1.0) * qf * (1.0 + qf) / (s + qf) ** 2,
-            diff=dso + (dfl - dso) * s * (1.0 + qg) / (s + qg),
-            dkds=(dfl - dso) * qg * (1.0 + qg) / (s + qg) ** 2,
-        )
-        self.d = {k: c.array((N,), data=v) for k, v in coef.items()}
+        self._coef_par = (alpha_max, qf, qg, dfl, dso)
+        self.d = {k: c.array((N,), data=v) for k, v in self._coefficients(s).items()}
         for k in ("qx", "qy", "ip", "iux", "iuy", "imx", "imy", "item", "iqx", "iqy", "dfds"):
             self.d[k] = c.array((N,), fill=0.0)
         self.d["ig"] = c.array((NC[D2Q9], N), fill=0.0)
@@ -106,6 +101,24 @@ class TransientHeatsink2D:
         # leaves the buffers where it found them when nt is even
         self.use_graphs, self.chunk = use_graphs and storage == "full" and nt % 2 == 0, chunk
         self._graphs = {}
+
+    def _coefficients(self, s):
+        """heatsink_transient.cpp:108-113: Brinkman and diffusivity coefficients of design s."""
+        alpha_max, qf, qg, dfl, dso = self._coef_par
+        ly = self.ly
+        return dict(
+            alpha=alpha_max / (ly - 1.0) * qf * (1.0 - s) / (s + qf),
+            dads=-alpha_max / (ly - 1.0) * qf * (1.0 + qf) / (s + qf) ** 2,
+            diff=dso + (dfl - dso) * s * (1.0 + qg) / (s + qg),
+            dkds=(dfl - dso) * qg * (1.0 + qg) / (s + qg) ** 2,
+        )
+
+    def set_design(self, s):
+        """A new design: the four coefficient arrays are rewritten in place, so captured graphs
+        (which hold their addresses) stay valid."""
+        s = np.asarray(s, dtype=np.float64).reshape(self.N)
+        for k, v in self._coefficients(s).items():
+            self.d[k].upload(v)
 
     # ---- addresses --------------------------------------------------------------------------
     def _at(self, arr, row, width=None):
@@ -201,8 +214,15 @@ class TransientHeatsink2D:
         g = self._graphs.get(key)
         if g is None:
             check(self.Lb.lbm_graph_begin(self.ctx.h))
-            for s in steps:
-                s()
+            try:
+                for s in steps:
+                    s()
+            except BaseException:
+                # end the capture so the context stream works again, and drop the partial graph
+                part = VP()
+                if self.Lb.lbm_graph_end(self.ctx.h, ctypes.byref(part)) == 0 and part:
+                    self.Lb.lbm_graph_destroy(part)
+                raise
             g = VP()
             check(self.Lb.lbm_graph_end(self.ctx.h, ctypes.byref(g)))
             self._graphs[key] = g

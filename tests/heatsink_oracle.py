@@ -117,9 +117,10 @@ def run(lx, ly, ad_kind, seed=0, nu=0.1, pr=6.0, umax=0.05, heat=1e-3, alpha_max
 
 
 def run3d(lx, ly, lz, ad_kind, seed=0, nu=0.1, pr=6.0, uin=0.03, heat=1e-3, plate=8, alpha_max=1e4, qf=0.01,
-          qg=1.0, filter_radius=2.0, eps=1e-6, check_every=100, max_steps=400000):
+          qg=1.0, filter_radius=2.0, eps=1e-6, check_every=100, max_steps=400000, smooth_corners=False):
     """CPU-oracle replica of panslbm2_amd/heatsink3d.py (Heatsink3D, D3Q15 NS): the duct's design,
-    coefficients, forward and adjoint step order, residual test and sensitivity."""
+    coefficients, forward and adjoint step order (with smooth_corners, SmoothCorner on both lattices
+    after the boundary calls), residual test and sensitivity."""
     from _oracle import D3Q15
 
     O = lib()
@@ -175,6 +176,9 @@ def run3d(lx, ly, lz, ad_kind, seed=0, nu=0.1, pr=6.0, uin=0.03, heat=1e-3, plat
         O.ol_ns_set_rho(pf.h, ptr(fout), ptr(f1), ptr(fz), ptr(fz))
         O.ol_ad_set_t(pg.h, ptr(gin), ptr(gz), *[ptr(a) for a in U[b]])
         O.ol_ad_set_q(pg.h, ptr(gfl), ptr(gz), *[ptr(a) for a in U[b]], ptr(diff), D(0.0))
+        if smooth_corners:
+            O.ol_smooth_corner(pf.h)
+            O.ol_smooth_corner(pg.h)
     fwd_steps, fb = t, t % 2
     out = {"fwd_f": pf.get(), "fwd_g": pg.get(), "rho": rho.copy(), "tem": tem.copy(), "g": g.copy(),
            "fwd_steps": fwd_steps, "fwd_res": (ru, rq), "gamma": s}
@@ -207,6 +211,9 @@ def run3d(lx, ly, lz, ad_kind, seed=0, nu=0.1, pr=6.0, uin=0.03, heat=1e-3, plat
         O.ol_bc(pf.h, ptr(fw), 1)
         O.ol_ans_iset_u(pf.h, ptr(fin), ptr(fuin), ptr(fz), ptr(fz), D(0.0), 1)
         O.ol_ans_iset_rho(pf.h, ptr(fout))
+        if smooth_corners:
+            O.ol_smooth_corner(pf.h)
+            O.ol_smooth_corner(pg.h)
     ab = t % 2
     dfds = z()
     O.ol_sens_brinkman_diffusivity(pg.h, ptr(dfds), *[ptr(a) for a in uf], *[ptr(a) for a in IM], ptr(dads),

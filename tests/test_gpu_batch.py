@@ -95,7 +95,8 @@ def test_tuner_rounds_are_steps(ctx):
         h.init_forward()
         h._fwd_step(1)
     batches = hb._make_batches(lambda h, t: h._fwd_step(t), 2)
-    arr = (ctypes.c_void_p * 2)(batches[(0, 2)], batches[(1, 2)])
+    mode = hb.hs[0]._save_mode(2, False)  # the save mode of the ordinary (non-residual) steps
+    arr = (ctypes.c_void_p * 2)(batches[(0, mode)], batches[(1, mode)])
     blk, xcd = ctypes.c_int(), ctypes.c_int()
     pa.check(ctx.lib.lbm_batch_tune(arr, 2, 1, ctypes.byref(blk), ctypes.byref(xcd)))
     assert blk.value in (64, 128, 256) and xcd.value in (0, 1)

@@ -66,8 +66,9 @@ def test_mixed_precision_evaluation(ctx, ns_kind):
     assert np.max(np.abs(s32 - s64)) <= 1e-4 * np.max(np.abs(s64))
 
 
-@pytest.mark.parametrize("graphs", [True, False], ids=["graphs", "nographs"])
-def test_heatsink3d_driver_matches_oracle(ctx, graphs):
+@pytest.mark.parametrize("graphs,smooth", [(True, False), (False, False), (True, True)],
+                         ids=["graphs", "nographs", "smoothed"])
+def test_heatsink3d_driver_matches_oracle(ctx, graphs, smooth):
     """The 3D heat-sink driver (fused steps, graphs, field rebinding, boundary-only saves) against its
     CPU-oracle replica (tests/heatsink_oracle.py run3d): populations, fields and the objective
     value-identical, the sensitivity (pow-based in the reference) within 1e-12 relative."""
@@ -75,8 +76,8 @@ def test_heatsink3d_driver_matches_oracle(ctx, graphs):
     import panslbm2_amd as pa
     from panslbm2_amd.heatsink3d import Heatsink3D
 
-    print("heatsink3d 14x10x9 vs oracle", "graphs" if graphs else "eager", flush=True)
-    kw = dict(lx=14, ly=10, lz=9, plate=3, seed=4, max_steps=400, eps=1e-30)
+    print("heatsink3d 14x10x9 vs oracle", "graphs" if graphs else "eager", "smoothed" if smooth else "", flush=True)
+    kw = dict(lx=14, ly=10, lz=9, plate=3, seed=4, max_steps=400, eps=1e-30, smooth_corners=smooth)
     hs = Heatsink3D(ctx, use_graphs=graphs, **kw)
     r = hs.evaluate()
     want = heatsink_oracle.run3d(ad_kind=pa.D3Q7, **kw)
@@ -112,4 +113,30 @@ def test_heatsink3d_fp16_error_vs_fp64_oracle(ctx):
     assert abs(r["objective"] / want["objective"] - 1.0) < 1e-6
     dfds, wd = hs.d["dfds"].download(), want["dfds"]
     assert np.max(np.abs(dfds - wd)) <= 1e-4 * np.max(np.abs(wd))
+    hs.close()
+
+
+def test_continuation_stages(ctx):
+    """The reference driver's (qf, beta) continuation: the stage advances every stage_iterations
+    iterations (or once the design stops changing) and each stage's qf reaches the Brinkman
+    coefficient."""
+    import panslbm2_amd as pa
+    from panslbm2_amd.heatsink3d import Heatsink3D
+
+    hs = Heatsink3D(ctx, **DIMS, seed=4, max_steps=20000, heaviside_beta=1.0)
+    seen = []
+
+    def cb(r):
+        a = hs.d["alpha"].download()
+        s = hs.gamma
+        want = hs.alpha_max / (hs.ly - 1.0) * hs.qf * (1.0 - s) / (s + hs.qf)
+        np.testing.assert_allclose(a, want, rtol=1e-14)
+        seen.append((r["stage"], hs.qf, hs.heaviside_beta))
+
+    hist = hs.optimize(6, stages=Heatsink3D.REFERENCE_STAGES, stage_iterations=2, callback=cb, optimizer="pg")
+    stages = [s for s, _, _ in seen]
+    assert stages == sorted(stages) and stages[0] == 0 and stages[-1] >= 2
+    for st, qf, beta in seen:
+        assert (qf, beta) == Heatsink3D.REFERENCE_STAGES[st]
+    assert len(hist) == len(seen)
     hs.close()

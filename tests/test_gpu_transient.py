@@ -75,3 +75,24 @@ def test_transient_evaluate_twice_is_repeatable(ctx):
     assert f1 == f2
     np.testing.assert_array_equal(g1, g2)
     drv.close()
+
+
+def test_transient_graphs_follow_a_new_design(ctx):
+    """Graphs captured on design A, replayed after set_design(B): the same values as eager launches
+    of design B (the coefficient arrays are rewritten in place, their addresses kept)."""
+    import numpy as np
+
+    rng = np.random.default_rng(11)
+    seq = [s for s in sequences.SEQUENCES if s.name == "transient_nc2d_16x12"][0]
+    N = seq.lx * seq.ly
+    sb = rng.uniform(0.01, 0.99, N)
+    out = {}
+    for graphs in (True, False):
+        drv = _driver(ctx, storage="full", use_graphs=graphs, chunk=16)
+        drv.evaluate()  # design A (captures the graphs when graphs is set)
+        drv.set_design(sb)
+        f, dfds = drv.evaluate()
+        out[graphs] = (f, np.asarray(dfds).copy())
+        drv.close()
+    assert out[True][0] == out[False][0]
+    np.testing.assert_array_equal(out[True][1], out[False][1])
