@@ -36,7 +36,7 @@ class Heatsink3D(Heatsink2D):
     def __init__(self, ctx=None, lx=128, ly=128, lz=64, ad_kind=D3Q7, seed=0, nu=0.1, pr=6.0, uin=0.03, heat=1e-3,
                  plate=8, alpha_max=1e4, qf=0.01, qg=1.0, filter_radius=2.0, eps=1e-6, check_every=100,
                  max_steps=400000, use_graphs=True, prec=F64, fp32_floor=None, fused=True, ns_kind=D3Q15,
-                 heaviside_beta=None, design_box=None, smooth_corners=False):
+                 heaviside_beta=None, design_box=None, smooth_corners=False, inlet_rho=None):
         assert ns_kind in (D3Q15, D3Q27)
         self.ctx = ctx or Context(0)
         self.L = self.ctx.lib
@@ -47,6 +47,10 @@ class Heatsink3D(Heatsink2D):
         assert check_every % 2 == 0
         self.use_graphs, self.fused = use_graphs and fused, fused
         self.smooth_corners = smooth_corners
+        # inlet_rho: a pressure inlet (Zou-He density rho_in, navierstokes.h:261-426, and its adjoint
+        # ANS::iBoundaryConditionSetRho3D) instead of the velocity inlet: the flow rate then follows
+        # the design, so a design that narrows the duct slows the flow instead of accelerating it
+        self.inlet_rho = inlet_rho
         self.prec = prec
         self.fp32_floor = fp32_floor if fp32_floor is not None else (1e-3 if prec == F16 else 1e-5)
         c = self.ctx
@@ -59,6 +63,11 @@ class Heatsink3D(Heatsink2D):
         # the Heaviside filter's design region [0,mx) x [0,my) x [0,mz) (heatsink3D.cpp:87-93); the
         # whole grid by default (the filter's default weights)
         self.design_box = tuple(design_box) if design_box is not None else (lx, ly, lz)
+        # as the reference's driver: nodes outside the box are fluid (heatsink3D.cpp:104-108, 286-290),
+        # and the design variables are the box's nodes above the plate
+        mx, my, mz = self.design_box
+        self.in_box = (i < mx) & (j < my) & (k < mz)
+        self.region = self.in_box & (k >= plate)
         self.alpha_max, self.qf, self.qg = alpha_max, qf, qg
         self.dfl = nu / pr
         self.dso = 10.0 * self.dfl
@@ -92,6 +101,7 @@ class Heatsink3D(Heatsink2D):
         self.faces = dict(
             walls_f=F(D3Q15, wall.astype(np.int32), np.int32), inlet_f=F(D3Q15, inlet, np.int32),
             outlet_f=F(D3Q15, outlet, np.int32), uin_f=F(D3Q15, np.full(N, uin)), zero_f=F(D3Q15, zero),
+            rhoin_f=F(D3Q15, np.full(N, inlet_rho if inlet_rho is not None else 1.0)),
             one_f=F(D3Q15, np.ones(N)), inlet_g=F(ad_kind, inlet, np.int32), flux_g=F(ad_kind, flux, np.int32),
             obj_g=F(ad_kind, bottom, np.int32), zero_g=F(ad_kind, zero),
         )
@@ -107,6 +117,7 @@ class Heatsink3D(Heatsink2D):
         geometry)."""
         c, lx, ly, lz, d = self.ctx, self.lx, self.ly, self.lz, self.d
         self.x = np.clip(np.asarray(x, dtype=np.float64), 0.0, 1.0).copy()
+        self.x[~self.in_box] = 1.0
         d["raw"].upload(self.x)
         if self.heaviside_beta:
             check(self.L.lbm_heaviside_filter_region(c.h, lx, ly, lz, D(self.filter_radius), D(self.heaviside_beta),
@@ -115,6 +126,7 @@ class Heatsink3D(Heatsink2D):
             check(self.L.lbm_density_filter(c.h, lx, ly, lz, D(self.filter_radius), d["raw"].ptr,
                                             d["design"].ptr))
         s = d["design"].download()
+        s[~self.in_box] = 1.0  # fluid outside the design box
         s[self.k < self.plate] = 0.0  # the base plate is solid
         d["design"].upload(s)
         prev = getattr(self, "gamma", None)
@@ -127,7 +139,8 @@ class Heatsink3D(Heatsink2D):
         d["dkds"].upload((dfl - dso) * qg * (1.0 + qg) / (s + qg) ** 2)
 
     def design_update(self, vf=0.4, move=0.05, optimizer="mma"):
-        """One design step over the nodes above the plate.  optimizer="mma": the device MMA
+        """One design step over the design variables (the design box's nodes above the plate).
+        optimizer="mma": the device MMA
         optimiser (panslbm2_amd/optimizer.py) as the reference's driver uses it (heatsink3D.cpp:78-85,
         :120-139, :282): one constraint g = mean(1 - x) / vf - 1 <= 0 (solid fraction at most vf),
         a0 = 1, a = 0, c = 1e4, d = 0, the move limit, and the sensitivity normalised to a largest
@@ -137,7 +150,7 @@ class Heatsink3D(Heatsink2D):
         if optimizer == "mma":
             return self._mma_update(vf, move)
         dfds = self.d["dfds"].download()
-        region = self.k >= self.plate
+        region = self.region
         g = dfds[region]
         g = g / max(np.abs(g).max(), 1e-300)
         x = self.x[region]
@@ -157,7 +170,7 @@ class Heatsink3D(Heatsink2D):
     def _mma_update(self, vf, move):
         from .optimizer import MMA
 
-        region = np.nonzero(self.k >= self.plate)[0]
+        region = np.nonzero(self.region)[0]
         n = len(region)
         c = self.ctx
         if getattr(self, "_mma", None) is None:
@@ -172,7 +185,7 @@ class Heatsink3D(Heatsink2D):
             R, beta = D(self.filter_radius), D(self.heaviside_beta)
             if "dfdx" not in d:
                 d["dfdx"] = c.array((self.N,))
-                d["dgdg"] = c.array((self.N,), data=np.where(self.k >= self.plate, -1.0 / (vf * n), 0.0))
+                d["dgdg"] = c.array((self.N,), data=np.where(self.region, -1.0 / (vf * n), 0.0))
                 d["dgdx"] = c.array((self.N,))
             box = self.design_box
             check(self.L.lbm_heaviside_sensitivity_region(c.h, lx, ly, lz, R, beta, *box, d["raw"].ptr,
@@ -199,7 +212,12 @@ class Heatsink3D(Heatsink2D):
         check(L.lbm_stream(self.pf.h))
         check(L.lbm_stream(self.pg.h))
         check(L.lbm_bc(self.pf.h, hptr(fc["walls_f"])))
-        check(L.lbm_ns_set_u(self.pf.h, hptr(fc["inlet_f"]), hptr(fc["uin_f"]), hptr(fc["zero_f"]), hptr(fc["zero_f"])))
+        if self.inlet_rho is None:
+            check(L.lbm_ns_set_u(self.pf.h, hptr(fc["inlet_f"]), hptr(fc["uin_f"]), hptr(fc["zero_f"]),
+                                 hptr(fc["zero_f"])))
+        else:
+            check(L.lbm_ns_set_rho(self.pf.h, hptr(fc["inlet_f"]), hptr(fc["rhoin_f"]), hptr(fc["zero_f"]),
+                                   hptr(fc["zero_f"])))
         check(L.lbm_ns_set_rho(self.pf.h, hptr(fc["outlet_f"]), hptr(fc["one_f"]), hptr(fc["zero_f"]),
                                hptr(fc["zero_f"])))
         check(L.lbm_ad_set_t(self.pg.h, hptr(fc["inlet_g"]), hptr(fc["zero_g"]), ux, uy, uz))
@@ -216,8 +234,11 @@ class Heatsink3D(Heatsink2D):
         check(L.lbm_aad_iset_q(self.pg.h, hptr(fc["flux_g"]), ux, uy, uz, D(0.0)))
         check(L.lbm_aad_iset_q(self.pg.h, hptr(fc["obj_g"]), ux, uy, uz, D(1.0)))
         check(L.lbm_ibc(self.pf.h, hptr(fc["walls_f"])))
-        check(L.lbm_ans_iset_u(self.pf.h, hptr(fc["inlet_f"]), hptr(fc["uin_f"]), hptr(fc["zero_f"]),
-                               hptr(fc["zero_f"]), D(0.0), 1))
+        if self.inlet_rho is None:
+            check(L.lbm_ans_iset_u(self.pf.h, hptr(fc["inlet_f"]), hptr(fc["uin_f"]), hptr(fc["zero_f"]),
+                                   hptr(fc["zero_f"]), D(0.0), 1))
+        else:
+            check(L.lbm_ans_iset_rho(self.pf.h, hptr(fc["inlet_f"])))
         check(L.lbm_ans_iset_rho(self.pf.h, hptr(fc["outlet_f"])))
         if self.smooth_corners:
             check(L.lbm_smooth_corner(self.pf.h))
@@ -442,8 +463,9 @@ class Heatsink3D(Heatsink2D):
 
     def optimize(self, iterations, vf=0.4, move=0.05, warm=None, x0=None, callback=None, optimizer="mma",
                  stages=None, stage_iterations=100, stage_tol=0.01):
-        """Design loop from a uniform solid fraction vf above the plate (or x0): evaluate, then
-        design_update.  Returns the per-iteration evaluate() records.
+        """Design loop from a uniform solid fraction vf over the design variables (the design box's
+        nodes above the plate), or from x0: evaluate, then design_update.  Returns the per-iteration
+        evaluate() records.
 
         stages: a continuation schedule of (qf, beta) pairs as the reference's driver runs it
         (REFERENCE_STAGES): stage s sets the Brinkman interpolation parameter qf and, with a
@@ -454,7 +476,7 @@ class Heatsink3D(Heatsink2D):
             self.qf = stages[0][0]
             if self.heaviside_beta:
                 self.heaviside_beta = stages[0][1]
-        x = np.where(self.k >= self.plate, 1.0 - vf, 0.0) if x0 is None else x0
+        x = np.where(self.region, 1.0 - vf, 0.0) if x0 is None else x0
         self.set_design(x)
         old = getattr(self, "_mma", None)
         if old is not None:

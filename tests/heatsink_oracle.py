@@ -117,7 +117,8 @@ def run(lx, ly, ad_kind, seed=0, nu=0.1, pr=6.0, umax=0.05, heat=1e-3, alpha_max
 
 
 def run3d(lx, ly, lz, ad_kind, seed=0, nu=0.1, pr=6.0, uin=0.03, heat=1e-3, plate=8, alpha_max=1e4, qf=0.01,
-          qg=1.0, filter_radius=2.0, eps=1e-6, check_every=100, max_steps=400000, smooth_corners=False):
+          qg=1.0, filter_radius=2.0, eps=1e-6, check_every=100, max_steps=400000, smooth_corners=False, gamma=None,
+          inlet_rho=None):
     """CPU-oracle replica of panslbm2_amd/heatsink3d.py (Heatsink3D, D3Q15 NS): the duct's design,
     coefficients, forward and adjoint step order (with smooth_corners, SmoothCorner on both lattices
     after the boundary calls), residual test and sensitivity."""
@@ -132,6 +133,8 @@ def run3d(lx, ly, lz, ad_kind, seed=0, nu=0.1, pr=6.0, uin=0.03, heat=1e-3, plat
     s = np.zeros(N)
     O.ol_density_filter(lx, ly, lz, D(filter_radius), ptr(raw), ptr(s))
     s[k < plate] = 0.0
+    if gamma is not None:  # a given design (e.g. one a design loop produced) instead of the seeded one
+        s = np.ascontiguousarray(gamma, dtype=np.float64).copy()
     dfl = nu / pr
     dso = 10.0 * dfl
     alpha = alpha_max / (ly - 1.0) * qf * (1.0 - s) / (s + qf)
@@ -172,7 +175,10 @@ def run3d(lx, ly, lz, ad_kind, seed=0, nu=0.1, pr=6.0, uin=0.03, heat=1e-3, plat
         O.ol_stream(pf.h)
         O.ol_stream(pg.h)
         O.ol_bc(pf.h, ptr(fw), 0)
-        O.ol_ns_set_u(pf.h, ptr(fin), ptr(fuin), ptr(fz), ptr(fz))
+        if inlet_rho is None:
+            O.ol_ns_set_u(pf.h, ptr(fin), ptr(fuin), ptr(fz), ptr(fz))
+        else:
+            O.ol_ns_set_rho(pf.h, ptr(fin), ptr(F(D3Q15, np.full(N, inlet_rho))), ptr(fz), ptr(fz))
         O.ol_ns_set_rho(pf.h, ptr(fout), ptr(f1), ptr(fz), ptr(fz))
         O.ol_ad_set_t(pg.h, ptr(gin), ptr(gz), *[ptr(a) for a in U[b]])
         O.ol_ad_set_q(pg.h, ptr(gfl), ptr(gz), *[ptr(a) for a in U[b]], ptr(diff), D(0.0))
@@ -209,7 +215,10 @@ def run3d(lx, ly, lz, ad_kind, seed=0, nu=0.1, pr=6.0, uin=0.03, heat=1e-3, plat
         O.ol_aad_iset_q(pg.h, ptr(gfl), *[ptr(a) for a in uf], D(0.0))
         O.ol_aad_iset_q(pg.h, ptr(gobj), *[ptr(a) for a in uf], D(1.0))
         O.ol_bc(pf.h, ptr(fw), 1)
-        O.ol_ans_iset_u(pf.h, ptr(fin), ptr(fuin), ptr(fz), ptr(fz), D(0.0), 1)
+        if inlet_rho is None:
+            O.ol_ans_iset_u(pf.h, ptr(fin), ptr(fuin), ptr(fz), ptr(fz), D(0.0), 1)
+        else:
+            O.ol_ans_iset_rho(pf.h, ptr(fin))
         O.ol_ans_iset_rho(pf.h, ptr(fout))
         if smooth_corners:
             O.ol_smooth_corner(pf.h)

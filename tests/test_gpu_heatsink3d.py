@@ -66,9 +66,10 @@ def test_mixed_precision_evaluation(ctx, ns_kind):
     assert np.max(np.abs(s32 - s64)) <= 1e-4 * np.max(np.abs(s64))
 
 
-@pytest.mark.parametrize("graphs,smooth", [(True, False), (False, False), (True, True)],
-                         ids=["graphs", "nographs", "smoothed"])
-def test_heatsink3d_driver_matches_oracle(ctx, graphs, smooth):
+@pytest.mark.parametrize("graphs,smooth,rho_in", [(True, False, None), (False, False, None), (True, True, None),
+                                                  (True, False, 1.01)],
+                         ids=["graphs", "nographs", "smoothed", "pressure_inlet"])
+def test_heatsink3d_driver_matches_oracle(ctx, graphs, smooth, rho_in):
     """The 3D heat-sink driver (fused steps, graphs, field rebinding, boundary-only saves) against its
     CPU-oracle replica (tests/heatsink_oracle.py run3d): populations, fields and the objective
     value-identical, the sensitivity (pow-based in the reference) within 1e-12 relative."""
@@ -77,7 +78,7 @@ def test_heatsink3d_driver_matches_oracle(ctx, graphs, smooth):
     from panslbm2_amd.heatsink3d import Heatsink3D
 
     print("heatsink3d 14x10x9 vs oracle", "graphs" if graphs else "eager", "smoothed" if smooth else "", flush=True)
-    kw = dict(lx=14, ly=10, lz=9, plate=3, seed=4, max_steps=400, eps=1e-30, smooth_corners=smooth)
+    kw = dict(lx=14, ly=10, lz=9, plate=3, seed=4, max_steps=400, eps=1e-30, smooth_corners=smooth, inlet_rho=rho_in)
     hs = Heatsink3D(ctx, use_graphs=graphs, **kw)
     r = hs.evaluate()
     want = heatsink_oracle.run3d(ad_kind=pa.D3Q7, **kw)
