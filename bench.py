@@ -28,12 +28,15 @@ Also reported:
                    heat source and inlet velocity drawn per design) evaluated together, one kernel
                    launch per time step for all of them (FP32 mixed, autotuned workgroup shape),
                    against evaluating them one after another in plain FP64.
-  warm_start    -- a 3D design loop (D3Q15 + D3Q7 heat sink, 96 x 96 x 48, solid fraction 0.4,
-                   projected-gradient updates) run four ways: cold start in FP64 (the plain GPU
-                   form), previous-solution warm start, learned warm start (U-Net on the matrix
-                   cores predicting the change of the fields, trained beside the solves), and the
-                   learned warm start with FP32 mixed-precision storage; wall seconds and LBM steps
-                   per design iteration after the first, every solve to the 1e-6 residual.
+  warm_start    -- the first --warm-iters iterations of the 3D design loop (D3Q15 + D3Q7 heat sink,
+                   96 x 96 x 48, MMA, solid fraction 0.4, the reference driver's continuation and
+                   design sub-box) run four ways: cold start in FP64 (the plain GPU form),
+                   previous-solution warm start, learned warm start (U-Net on the matrix cores
+                   predicting the change of the fields, trained beside the solves), and the learned
+                   warm start with FP32 mixed-precision storage; wall seconds and LBM steps per
+                   design iteration after the first, every solve to the 1e-6 residual, and each
+                   mode's last objective and sensitivity against cold FP64 solves of the same design
+                   (to 1e-6 and to 1e-9).  --design-loop N runs only this loop, N iterations.
   transient     -- the transient natural-convection heat sink of the reference's transient
                    program (141 x 161, 2000 steps forward and backward, the adjoint over the stored
                    trajectory): seconds per evaluation with the trajectory resident in HBM and with
@@ -67,8 +70,8 @@ BYTES_ADJ_Q15Q7 = 2 * (15 + 7) * 8 + 2 * 8 + 5 * 8
 # cache-sized step includes Infinity Cache hits.
 TRAFFIC_FWD_200x400 = round((5767.5 * 1.920 + 8920.5) * 1024)
 # The same for the fused 3D forward and adjoint steps at 128x128x64 (profiles/k_collide_fused_3d_128x128x64.txt)
-TRAFFIC_FWD_128x128x64 = round((118115.5 * 1.920 + 187170.0) * 1024)
-TRAFFIC_ADJ_128x128x64 = round((137131.1 * 1.920 + 180224.0) * 1024)
+TRAFFIC_FWD_128x128x64 = round((106812.7 * 1.920 + 182828.8) * 1024)
+TRAFFIC_ADJ_128x128x64 = round((119837.8 * 1.920 + 180224.0) * 1024)
 
 
 def kernel_rate(hs, steps=100, reps=5):
@@ -341,37 +344,92 @@ def transient_extra(ctx, nt=2000):
     return out
 
 
-def warm_extra(ctx, iters, train_steps=20, use_graphs=True):
-    """The 96x96x48 3D design loop, cold vs warm-started (see the module docstring)."""
+DESIGN_LOOP_MODES = {"cold_fp64": ("F64", None), "cold_fp32": ("F32", None),
+                     "previous_solution_fp64": ("F64", False), "learned_fp64": ("F64", True),
+                     "learned_fp32": ("F32", True)}
+
+
+def design_loop(ctx, iters, modes, stage_iterations=20, train_steps=20, eps=1e-6):
+    """The 96x96x48 topology-optimisation loop: D3Q15 + D3Q7 heat sink, MMA with one volume
+    constraint (solid fraction <= 0.4), the reference driver's (qf, beta) continuation
+    (heatsink3D.cpp:45) advanced every stage_iterations iterations, its 3/4 design sub-box
+    (heatsink3D.cpp:47) with fluid around it, inlet u = 0.01, all-fluid start, every solve to the
+    residual eps (1e-6, cap 60,000 steps).  Each mode runs the loop from the same start; after the
+    loop the last evaluated design is evaluated again cold in FP64, once to the same residual and
+    once to a 100x-1000x tighter one, and the mode's objective and sensitivity are compared with
+    both."""
     import numpy as np
 
-    from panslbm2_amd import F32, F64
+    import panslbm2_amd as pa
     from panslbm2_amd.heatsink3d import Heatsink3D
 
+    box = tuple(3 * (n - 1) // 4 + 1 for n in (96, 96, 48))
+    cap = 60000 if eps >= 1e-6 else 200000
+    tight = min(1e-9, eps * 1e-2)
+    kw = dict(lx=96, ly=96, lz=48, plate=6, heaviside_beta=1.0, design_box=box, uin=0.01, max_steps=cap, eps=eps)
     out = {"grid": [96, 96, 48], "lattices": "D3Q15 NS + D3Q7 AD", "iterations": iters, "solid_fraction": 0.4,
-           "train_steps_per_iteration": train_steps}
-    for mode, prec, learn in (("cold_fp64", F64, None), ("previous_solution_fp64", F64, False),
-                              ("learned_fp64", F64, True), ("learned_fp32", F32, True)):
-        hs = Heatsink3D(ctx, lx=96, ly=96, lz=48, plate=6, prec=prec, use_graphs=use_graphs)
+           "optimizer": "MMA", "stages": [list(s) for s in Heatsink3D.REFERENCE_STAGES],
+           "stage_iterations": stage_iterations, "design_box": list(box), "inlet_u": 0.01,
+           "train_steps_per_iteration": train_steps, "residual": eps, "tight_residual": tight}
+    for mode in modes:
+        prec, learn = DESIGN_LOOP_MODES[mode]
+        hs = Heatsink3D(ctx, prec=getattr(pa, prec), **kw)
         if learn is not None:
             hs.attach_warm_start(seed=0, steps=train_steps, learn=learn)
+        last = {}
+
+        def cb(r, hs=hs, last=last):
+            last.update(dfds=hs.d["dfds"].download(), x=hs.x.copy(), qf=hs.qf, beta=hs.heaviside_beta,
+                        objective=r["objective"])
+            print("design loop %s it %d stage %d obj %.6f fwd %d adj %d %.3f s%s" % (
+                mode, r["iteration"], r["stage"], r["objective"], r["fwd_steps"], r["adj_steps"],
+                r["seconds"] + r.get("train_seconds", 0.0), "" if r["converged"] else " NOT CONVERGED"),
+                file=sys.stderr, flush=True)
+
         t0 = time.perf_counter()
-        hist = hs.optimize(iters, warm=(False if learn is None else None), optimizer="pg")
+        hist = hs.optimize(iters, vf=0.4, move=0.05, optimizer="mma", x0=np.where(hs.region, 1.0, 0.0),
+                           stages=Heatsink3D.REFERENCE_STAGES, stage_iterations=stage_iterations,
+                           warm=(False if learn is None else None), callback=cb)
+        ctx.sync()
         total = time.perf_counter() - t0
-        later = hist[1:]
-        out[mode] = {"s_per_iteration": round(float(np.mean([r["seconds"] + r["train_seconds"] for r in later])), 4),
-                     "steps_per_iteration": int(np.mean([r["fwd_steps"] + r["adj_steps"] for r in later])),
-                     "total_s": round(total, 2), "objective_last": hist[-1]["objective"],
-                     "fallbacks": int(sum(any(r["fallback"]) for r in hist)),
-                     "all_converged": bool(all(r["converged"] for r in hist))}
         hs.close()
-        print("bench: warm start %s %.3f s/iteration" % (mode, out[mode]["s_per_iteration"]), file=sys.stderr,
+        refs = {}
+        for name, e, c in (("cold_fp64", eps, cap), ("tight_fp64", tight, 600000)):
+            ref = Heatsink3D(ctx, prec=pa.F64, **dict(kw, eps=e, max_steps=c))
+            ref.qf, ref.heaviside_beta = last["qf"], last["beta"]
+            ref.set_design(last["x"])
+            rr = ref.evaluate(warm=False)
+            refs[name] = (rr, ref.d["dfds"].download())
+            ref.close()
+        rr, d64 = refs["cold_fp64"]
+        rt, dt = refs["tight_fp64"]
+
+        def errs(obj, dfds):
+            return {"objective_rel_err": abs(obj - rt["objective"]) / abs(rt["objective"]),
+                    "dfds_max_rel_err": float(np.max(np.abs(dfds - dt)) / np.max(np.abs(dt)))}
+
+        later = hist[1:] or hist
+        out[mode] = {
+            "s_per_iteration": round(float(np.mean([r["seconds"] + r.get("train_seconds", 0.0) for r in later])), 4),
+            "steps_per_iteration": int(np.mean([r["fwd_steps"] + r["adj_steps"] for r in later])),
+            "total_s": round(total, 2), "iterations_run": len(hist), "objective_first": hist[0]["objective"],
+            "objective_last": hist[-1]["objective"], "final_stage": hist[-1]["stage"],
+            "fallbacks": int(sum(any(r["fallback"]) for r in hist)),
+            "all_converged": bool(all(r["converged"] for r in hist)),
+            "last_design_vs_cold_fp64": {
+                "objective_rel_err": abs(last["objective"] - rr["objective"]) / abs(rr["objective"]),
+                "dfds_max_rel_err": float(np.max(np.abs(last["dfds"] - d64)) / np.max(np.abs(d64)))},
+            "last_design_vs_tight_fp64": errs(last["objective"], last["dfds"]),
+            "cold_fp64_vs_tight_fp64": errs(rr["objective"], d64),
+            "tight_fp64_steps": [rt["fwd_steps"], rt["adj_steps"]], "tight_fp64_converged": bool(rt["converged"])}
+        print("bench: design loop %s %.3f s/iteration" % (mode, out[mode]["s_per_iteration"]), file=sys.stderr,
               flush=True)
-    base = out["cold_fp64"]["s_per_iteration"]
-    for mode in ("previous_solution_fp64", "learned_fp64", "learned_fp32"):
-        out[mode]["speedup_vs_cold_fp64"] = round(base / out[mode]["s_per_iteration"], 2)
-    out["note"] = ("per iteration after the first (the first is a cold start in every mode); the designs of the "
-                   "four loops drift apart slightly because each solve stops at the 1e-6 residual")
+    if "cold_fp64" in out:
+        base = out["cold_fp64"]["s_per_iteration"]
+        for mode in modes:
+            out[mode]["speedup_vs_cold_fp64"] = round(base / out[mode]["s_per_iteration"], 2)
+    out["note"] = ("seconds per iteration after the first, the warm-start network's training included; the "
+                   "loops' designs drift apart slightly because each solve stops at the 1e-6 residual")
     return out
 
 
@@ -545,6 +603,13 @@ def main():
     ap.add_argument("--no-transient", action="store_true")
     ap.add_argument("--batch", type=int, default=64, help="designs in the batched evaluation (0: skip it)")
     ap.add_argument("--warm-iters", type=int, default=10, help="3D design-loop iterations per warm-start mode (0: skip)")
+    ap.add_argument("--design-loop", type=int, default=0,
+                    help="run only the 96x96x48 MMA design loop for this many iterations per mode and print "
+                         "its JSON summary")
+    ap.add_argument("--design-modes", default="cold_fp64,previous_solution_fp64,learned_fp64,learned_fp32",
+                    help="comma-separated modes of --design-loop: " + ", ".join(DESIGN_LOOP_MODES))
+    ap.add_argument("--stage-iterations", type=int, default=20, help="--design-loop iterations per continuation stage")
+    ap.add_argument("--design-eps", type=float, default=1e-6, help="--design-loop steady-state residual")
     ap.add_argument("--kernel-only", choices=["fwd", "adj", "stream", "unet", "fwd3", "adj3"],
                     help="launch only the fused 2D forward / adjoint step (--lx x --ly), the pull-stream copy, "
                          "or the fused 3D forward / adjoint step (--lx3 x --ly3 x --lz3) --steps times "
@@ -561,6 +626,15 @@ def main():
     ctx = Context(0)
     if a.kernel_only:
         kernel_only(ctx, a)
+        ctx.close()
+        return
+    if a.design_loop > 0:
+        modes = [m for m in a.design_modes.split(",") if m]
+        for m in modes:
+            if m not in DESIGN_LOOP_MODES:
+                sys.exit("unknown --design-modes entry %r" % m)
+        print(json.dumps({"design_loop": design_loop(ctx, a.design_loop, modes, a.stage_iterations,
+                                                      eps=a.design_eps)}))
         ctx.close()
         return
     hs = Heatsink3D(ctx, a.lx3, a.ly3, a.lz3, ad_kind=D3Q7, max_steps=a.max_steps, use_graphs=not a.no_graphs)
@@ -613,7 +687,8 @@ def main():
     if a.batch > 0:
         extra["batch"] = batch_extra(ctx, a.batch)
     if a.warm_iters > 0:
-        extra["warm_start"] = warm_extra(ctx, a.warm_iters, use_graphs=not a.no_graphs)
+        extra["warm_start"] = design_loop(ctx, a.warm_iters, [m for m in a.design_modes.split(",") if m],
+                                          a.stage_iterations, eps=a.design_eps)
     if not a.no_transient:
         extra["transient"] = transient_extra(ctx)
 
