@@ -345,8 +345,8 @@ def transient_extra(ctx, nt=2000):
 
 
 DESIGN_LOOP_MODES = {"cold_fp64": ("F64", None), "cold_fp32": ("F32", None),
-                     "previous_solution_fp64": ("F64", False), "learned_fp64": ("F64", True),
-                     "learned_fp32": ("F32", True)}
+                     "previous_solution_fp64": ("F64", False), "previous_solution_fp32": ("F32", False),
+                     "learned_fp64": ("F64", True), "learned_fp32": ("F32", True)}
 
 
 def design_loop(ctx, iters, modes, stage_iterations=20, train_steps=20, eps=1e-6):

@@ -52,7 +52,7 @@ class Heatsink3D(Heatsink2D):
         # the design, so a design that narrows the duct slows the flow instead of accelerating it
         self.inlet_rho = inlet_rho
         self.prec = prec
-        self.fp32_floor = fp32_floor if fp32_floor is not None else (1e-3 if prec == F16 else 1e-5)
+        self.fp32_floor = fp32_floor if fp32_floor is not None else (1e-3 if prec == F16 else 1e-7)
         c = self.ctx
         n = np.arange(N)
         i, j, k = n % lx, (n // lx) % ly, n // (lx * ly)
