@@ -416,6 +416,7 @@ def design_loop(ctx, iters, modes, stage_iterations=20, train_steps=20, eps=1e-6
             "objective_last": hist[-1]["objective"], "final_stage": hist[-1]["stage"],
             "fallbacks": int(sum(any(r["fallback"]) for r in hist)),
             "all_converged": bool(all(r["converged"] for r in hist)),
+            "warm_gain_last": hist[-1].get("warm_gain"),
             "last_design_vs_cold_fp64": {
                 "objective_rel_err": abs(last["objective"] - rr["objective"]) / abs(rr["objective"]),
                 "dfds_max_rel_err": float(np.max(np.abs(last["dfds"] - d64)) / np.max(np.abs(d64)))},

@@ -446,7 +446,7 @@ class Heatsink3D(Heatsink2D):
                 "adj_steps": self.adj_steps, "fp32_steps": (self.fwd_steps32, self.adj_steps32),
                 "objective": self.objective, "fwd_residual": self.fwd_res, "adj_residual": self.adj_res,
                 "diverged": not np.all(np.isfinite(res)), "converged": converged, "warm": use,
-                "fallback": fallback}
+                "fallback": fallback, "warm_gain": None if ws is None else ws.gain.tolist()}
 
     def close(self):
         super().close()

@@ -134,13 +134,19 @@ class WarmStart:
         as a training sample and keeps a copy of targets as the base of the next prediction.
     train(): queues `steps` Adam updates over the newest `replay` samples (round robin) on the
         network's stream and returns at once; the solve that follows runs beside it.
-    predict(inputs, outputs): outputs = previous targets + predicted change, each field clamped
-        to the range its observed values spanned (widened by 5 % on either side).  Before any
-        training the network outputs zero, so the first warm start is the previous solution.
+    predict(inputs, outputs): outputs = previous targets + gain x predicted change, each field
+        clamped to the range its observed values spanned (widened by 5 % on either side).  Before
+        any training the network outputs zero, so the first warm start is the previous solution.
+    gain: per field, the least-squares scale of the last prediction's change onto the change the
+        solve then found, <p, t> / <p, p> clipped to [0, 1] -- a held-out check, since the network
+        had not trained on that design.  A network whose change does not point the right way is
+        thereby shrunk toward the previous solution; one that does keeps gain 1.
     The normalisation of the change (per-field mean and scale) is fixed from the first sample.
     learn=False never trains: every prediction is then the previous solution (the plain
     continuation warm start the learned one is measured against).
     """
+
+    GAIN_MAX = 1.0
 
     def __init__(self, ctx, dims, n_in, n_out, seed=0, lr=1e-3, steps=40, replay=4, learn=True):
         self.ctx = ctx
@@ -154,6 +160,9 @@ class WarmStart:
         self.lo = self.hi = None
         self.n_observed = 0
         self.trained_steps = 0
+        self.gain = np.ones(n_out)
+        self.prev_host = None  # host copies of the last observed targets
+        self.raw = None        # host copy of the last prediction's change at gain 1
 
     @property
     def ready(self):
@@ -168,6 +177,12 @@ class WarmStart:
 
     def observe(self, inputs, targets):
         host = [t.download().ravel() for t in targets]
+        if self.raw is not None and self.prev_host is not None:
+            for f, (p, h, b) in enumerate(zip(self.raw, host, self.prev_host)):
+                pp = float(np.dot(p, p))
+                self.gain[f] = min(max(float(np.dot(p, h - b)) / pp, 0.0), self.GAIN_MAX) if pp > 0.0 else 1.0
+        self.raw = None
+        self.prev_host = host
         lo = np.array([h.min() for h in host])
         hi = np.array([h.max() for h in host])
         self.lo = lo if self.lo is None else np.minimum(self.lo, lo)
@@ -213,7 +228,14 @@ class WarmStart:
         self.net.set_input(inputs)
         self.net.forward()
         pad = 0.05 * (self.hi - self.lo)
-        self.net.output(outputs, self.prev, self.mean, self.scale, self.lo - pad, self.hi + pad)
+        lo, hi = self.lo - pad, self.hi + pad
+        if not self.learn or not self.samples:  # untrained: the previous solution
+            self.net.output(outputs, self.prev, self.mean, self.scale, lo, hi)
+            return
+        self.net.output(outputs, self.prev, self.mean, self.scale, lo, hi)
+        self.raw = [o.download().ravel() - b for o, b in zip(outputs, self.prev_host)]
+        if np.any(self.gain != 1.0):
+            self.net.output(outputs, self.prev, self.mean * self.gain, self.scale * self.gain, lo, hi)
 
     def close(self):
         self.net.close()
