@@ -307,8 +307,9 @@ def batch_extra(ctx, n, seed=2024):
         objs.append(q["objective"])
         div.append(q["diverged"])
         h.close()
-    err = max(abs(x - y) / abs(y) for x, y in zip(rb["objectives"], objs))
-    bdiv = [c == -2 for c in rb["fwd_conv"] + rb["adj_conv"]]
+    ok = [i for i, (x, y) in enumerate(zip(rb["objectives"], objs)) if np.isfinite(x) and np.isfinite(y)]
+    err = max(abs(rb["objectives"][i] - objs[i]) / abs(objs[i]) for i in ok)
+    bdiv = [a == -2 or b == -2 for a, b in zip(rb["fwd_conv"], rb["adj_conv"])]  # per design
     return {"designs": n, "seconds": round(rb["seconds"], 3), "sequential_fp64_seconds": round(seq_s, 3),
             "speedup_vs_sequential_fp64": round(seq_s / rb["seconds"], 2),
             "mlups": round(rb["node_steps"] / rb["seconds"] / 1e6, 1),
@@ -317,9 +318,12 @@ def batch_extra(ctx, n, seed=2024):
             "workgroup_and_xcd_remap": list(rb["tuned"]) if rb["tuned"] else None,
             "max_objective_rel_err_vs_fp64": err,
             "diverged_fp64_sequential": int(sum(div)), "diverged_batched": int(sum(bdiv)),
-            "note": "a solve whose residual turns non-finite stops there in both runs (the adjoint of "
-                    "designs with u_max near 0.06 diverges in FP64 too); objectives agree to the "
-                    "spread a 1e-6 step-to-step residual leaves (~1e-4 of the slowest mode)"}
+            "fp64_redo_batched": [len(x) for x in rb["fp64_redo"]],
+            "note": "designs counted as diverged when their forward or adjoint residual turns "
+                    "non-finite (the adjoint of designs with u_max near 0.06 grows without bound in both "
+                    "precisions); FP32 solves that diverge are solved again in FP64, batched together "
+                    "(fp64_redo_batched: forward, adjoint); objectives of the designs that converge in both runs agree to the spread "
+                    "a 1e-6 step-to-step residual leaves (~1e-4 of the slowest mode)"}
 
 
 def transient_extra(ctx, nt=2000):

@@ -279,7 +279,11 @@ int lbm_sensitivity_temperature_at_heat_source_c(lbm_lattice *q, double *dfds, c
                                                  const int *mask_face, const double *qn_face, int compat);
 
 /* ---- reductions (FP64, deterministic fixed-order trees) ---- */
-/* Residual, residual.h:9-51: sqrt(sum|a-b|^2 / sum|a|^2) over ncomp (1..3) components */
+/* Residual, residual.h:9-51: sqrt(sum|a-b|^2 / sum|a|^2) over ncomp (1..3) components.  One
+ * departure: when sum|a|^2 (or sum|a-b|^2) is not finite the result is NaN.  The reference's
+ * quotient is 0 there (a growing field whose squares overflow before its differences do), which a
+ * convergence test reads as converged; NaN reads as diverged.  The same holds for the batched
+ * residuals below. */
 int lbm_residual(lbm_context *ctx, int ncomp, const double *a0, const double *a1, const double *a2, const double *b0,
                  const double *b1, const double *b2, int n, double *out_host);
 int lbm_normalize(lbm_context *ctx, double *v, int n);              /* Normalize, normalize.h:9-24 */

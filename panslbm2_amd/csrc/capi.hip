@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <limits>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -1470,6 +1471,13 @@ int lbm_sensitivity_temperature_at_heat_source(lbm_lattice *q, double *dfds, con
 }
 
 // ---- reductions -------------------------------------------------------------------------------
+// The reference's quotient sqrt(sum (a-b)^2 / sum a^2) (residual.h:9-51), except that sums that
+// are not finite give NaN: a growing field's sum of squares overflows before its differences do,
+// and the quotient would then be 0, which a convergence test takes for converged.
+static double residual_ratio(double num, double den) {
+    if (!std::isfinite(num) || !std::isfinite(den)) return std::numeric_limits<double>::quiet_NaN();
+    return std::sqrt(num / den);
+}
 int lbm_residual(lbm_context *C, int ncomp, const double *a0, const double *a1, const double *a2, const double *b0,
                  const double *b1, const double *b2, int n, double *out) {
     if (!C || !out || ncomp < 1 || ncomp > 3 || !a0 || !b0 || (ncomp > 1 && (!a1 || !b1)) || (ncomp > 2 && (!a2 || !b2)))
@@ -1478,7 +1486,7 @@ int lbm_residual(lbm_context *C, int ncomp, const double *a0, const double *a1, 
     HIPCHK(lbm::launch_residual(ncomp, a, b, n, C->d_part, C->d_out, C->stream));
     HIPCHK(hipMemcpyAsync(C->h_out, C->d_out, 2 * sizeof(double), hipMemcpyDeviceToHost, C->stream));
     HIPCHK(hipStreamSynchronize(C->stream));
-    *out = std::sqrt(C->h_out[0] / C->h_out[1]);
+    *out = residual_ratio(C->h_out[0], C->h_out[1]);
     return LBM_OK;
 }
 int lbm_normalize(lbm_context *C, double *v, int n) {
@@ -1791,7 +1799,7 @@ int lbm_residual_batch(lbm_context *C, int n, int ncomp, const double *const *a,
     std::vector<double> h(2 * n);
     HIPCHK(hipMemcpyAsync(h.data(), res, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, C->stream));
     HIPCHK(hipStreamSynchronize(C->stream));
-    for (int i = 0; i < n; ++i) out[i] = std::sqrt(h[2 * i] / h[2 * i + 1]);
+    for (int i = 0; i < n; ++i) out[i] = residual_ratio(h[2 * i], h[2 * i + 1]);
     return LBM_OK;
 }
 
@@ -1823,7 +1831,7 @@ int lbm_residual_batch_static(lbm_context *C, int n, int ncomp, const double *co
     HIPCHK(lbm::launch_residual_multi(reinterpret_cast<const lbm::ResItem *>(base), n, part, sums, C->stream));
     HIPCHK(hipMemcpyAsync(h, sums, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, C->stream));
     HIPCHK(hipStreamSynchronize(C->stream));
-    for (int i = 0; i < n; ++i) out[i] = std::sqrt(h[2 * i] / h[2 * i + 1]);
+    for (int i = 0; i < n; ++i) out[i] = residual_ratio(h[2 * i], h[2 * i + 1]);
     return LBM_OK;
 }
 

@@ -33,6 +33,18 @@ D = ctypes.c_double
 VP = ctypes.c_void_p
 
 
+# A reduced-precision solve whose residual has stopped falling (not 5 % lower in ten checks) is
+# at that precision's floor once the residual is below this or 100x the switch floor; plateaus
+# of the early transient lie above it.
+STALL_CEILING = 1e-4
+
+
+def switch_ready(worst, floor, stall):
+    """Whether a reduced-precision solve converts to FP64: its residual is below the switch floor,
+    or it has stalled (ten checks without a 5 % drop) below max(100 floor, STALL_CEILING)."""
+    return worst < floor or (stall >= 10 and worst < max(100.0 * floor, STALL_CEILING))
+
+
 class Heatsink2D:
     def __init__(self, ctx=None, lx=200, ly=400, ad_kind=D2Q5, seed=0, nu=0.1, pr=6.0, umax=0.05, heat=1e-3,
                  alpha_max=1e4, qf=0.01, qg=1.0, filter_radius=3.0, eps=1e-6, check_every=100, max_steps=400000,
@@ -227,10 +239,10 @@ class Heatsink2D:
         self._graphs = {}
 
     def _solve(self, key, step, res):
-        """Chunks of check_every steps until both residuals are below eps.  In FP32 the residual is
-        watched: once it is below fp32_floor, or below 100 fp32_floor and not 5 % lower in ten
-        checks (plateaus of the transient above that do not count), the lattices convert to FP64 and
-        the solve continues there.  Returns (steps, steps in FP32, final residuals)."""
+        """Chunks of check_every steps until both residuals are below eps.  In FP32 / FP16 the
+        residual is watched (switch_ready): once it is below fp32_floor, or has stalled below
+        max(100 fp32_floor, STALL_CEILING), the lattices convert to FP64 and the solve continues
+        there.  Returns (steps, steps in reduced precision, final residuals)."""
         ce = self.check_every
         t, t32, best, stall = 0, 0, np.inf, 0
         r = (np.inf, np.inf)
@@ -249,7 +261,7 @@ class Heatsink2D:
                 t32 = t
                 stall = 0 if worst < 0.95 * best else stall + 1
                 best = min(best, worst)
-                if worst < self.fp32_floor or (stall >= 10 and worst < 100.0 * self.fp32_floor):
+                if switch_ready(worst, self.fp32_floor, stall):
                     self._set_prec(F64)
         return t, (t32 if self.prec != F64 else 0), r
 
@@ -328,18 +340,38 @@ class Heatsink2D:
         self.objective = out.value / len(self.obj_nodes)
         return self.objective
 
+    def in_fp64(self, solve, *args):
+        """Run solve(*args) from its cold start in FP64, whatever this driver's storage precision."""
+        prec = self.prec
+        self.prec = F64
+        try:
+            return solve(*args)
+        finally:
+            self.prec = prec
+
     def evaluate(self):
-        """One design evaluation; returns wall seconds and step counts."""
+        """One design evaluation; returns wall seconds and step counts.  A solve in reduced
+        precision whose residual turns non-finite is solved again from its cold start in FP64
+        (fp64_redo): a marginally unstable adjoint mode that FP64 round-off seeds too weakly to
+        grow before the solve converges can grow from FP32 round-off."""
         t0 = time.perf_counter()
+        redo = [False, False]
         self.forward()
+        if self.prec != F64 and not np.all(np.isfinite(self.fwd_res)):
+            redo[0] = True
+            self.in_fp64(self.forward)
         self.adjoint()
+        if self.prec != F64 and not np.all(np.isfinite(self.adj_res)):
+            redo[1] = True
+            self.in_fp64(self.adjoint)
         self.sensitivity()
         self.ctx.sync()
         dt = time.perf_counter() - t0
         diverged = not (np.all(np.isfinite(self.fwd_res)) and np.all(np.isfinite(self.adj_res)))
         return {"seconds": dt, "fwd_steps": self.fwd_steps, "adj_steps": self.adj_steps,
                 "fp32_steps": (self.fwd_steps32, self.adj_steps32), "objective": self.objective,
-                "fwd_residual": self.fwd_res, "adj_residual": self.adj_res, "diverged": diverged}
+                "fwd_residual": self.fwd_res, "adj_residual": self.adj_res, "diverged": diverged,
+                "fp64_redo": redo}
 
     def close(self):
         for g in self._graphs.values():

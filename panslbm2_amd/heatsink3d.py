@@ -412,7 +412,8 @@ class Heatsink3D(Heatsink2D):
 
         warm=None starts both solves from the attached warm start's prediction once it has seen a
         converged solution (warm=False forces the cold start); a warm-started solve whose residual
-        turns non-finite is repeated from the cold start.  With train=True the warm start's
+        turns non-finite is repeated from the cold start, and a reduced-precision one that still
+        does, from the cold start in FP64.  With train=True the warm start's
         training on the stored solutions is queued right after its prediction and runs beside the
         solves, and a converged evaluation is then stored as a new solution.  'seconds' is the
         wall time of the solves (training beside them included), 'train_seconds' the time to
@@ -425,14 +426,21 @@ class Heatsink3D(Heatsink2D):
             ws.predict(self._warm_inputs(), [self.w[n] for n in self.WARM_OUT])
         if ws is not None and train:
             ws.train()
+        redo = [False, False]
         self.forward(use)
         if use and not np.all(np.isfinite(self.fwd_res)):
             fallback[0] = True
             self.forward(False)
+        if self.prec != F64 and not np.all(np.isfinite(self.fwd_res)):
+            redo[0] = True
+            self.in_fp64(self.forward, False)
         self.adjoint(use)
         if use and not np.all(np.isfinite(self.adj_res)):
             fallback[1] = True
             self.adjoint(False)
+        if self.prec != F64 and not np.all(np.isfinite(self.adj_res)):
+            redo[1] = True
+            self.in_fp64(self.adjoint, False)
         self.sensitivity()
         self.ctx.sync()
         t1 = time.perf_counter()
@@ -446,7 +454,7 @@ class Heatsink3D(Heatsink2D):
                 "adj_steps": self.adj_steps, "fp32_steps": (self.fwd_steps32, self.adj_steps32),
                 "objective": self.objective, "fwd_residual": self.fwd_res, "adj_residual": self.adj_res,
                 "diverged": not np.all(np.isfinite(res)), "converged": converged, "warm": use,
-                "fallback": fallback, "warm_gain": None if ws is None else ws.gain.tolist()}
+                "fallback": fallback, "fp64_redo": redo, "warm_gain": None if ws is None else ws.gain.tolist()}
 
     def close(self):
         super().close()

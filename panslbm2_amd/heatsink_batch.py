@@ -8,9 +8,11 @@ The step order per instance is exactly the single-instance driver's, so with FP6
 instance's result is value-identical to solving it alone.
 
 Precision as in Heatsink2D: with prec=F32 all instances run in FP32 until every active one's
-residual has reached the FP32 floor, or is within 100x of it and has stopped falling (not 5 %
-lower in ten checks), then every lattice converts to FP64 and the solves finish at eps.  An instance that has converged is frozen (its workgroups return at once)
-at that check step, so each instance stops where it would have stopped alone.
+residual has reached the FP32 floor, or has stopped falling (not 5 % lower in ten checks) below
+max(100 floor, STALL_CEILING), then every lattice converts to FP64 and the solves finish at eps.
+An instance that has converged is frozen (its workgroups return at once) at that check step, so
+each instance stops where it would have stopped alone.  Instances whose reduced-precision solve
+diverges are solved again from their cold start in FP64, together in one more batched phase.
 """
 import ctypes
 import time
@@ -18,7 +20,7 @@ import time
 import numpy as np
 
 from . import D2Q5, F32, F64, Context, check
-from .heatsink2d import Heatsink2D
+from .heatsink2d import STALL_CEILING, Heatsink2D
 
 VP = ctypes.c_void_p
 I = ctypes.c_int
@@ -27,17 +29,18 @@ D = ctypes.c_double
 
 class HeatsinkBatch2D:
     def __init__(self, ctx=None, specs=(), ad_kind=D2Q5, prec=F32, eps=1e-6, check_every=100, max_steps=400000,
-                 use_graphs=True, tune=True, fp32_floor=1e-5, filter_radius=3.0):
+                 use_graphs=True, tune=True, fp32_floor=None, filter_radius=3.0):
         """specs: one dict per instance with lx, ly and optionally seed, heat, umax."""
         self.ctx = ctx or Context(0)
         self.L = self.ctx.lib
         self.prec, self.eps, self.ce, self.max_steps = prec, eps, check_every, max_steps
-        self.use_graphs, self.tune, self.fp32_floor = use_graphs, tune, fp32_floor
+        self.use_graphs, self.tune = use_graphs, tune
         self.hs = [Heatsink2D(self.ctx, lx=sp["lx"], ly=sp["ly"], ad_kind=ad_kind, seed=sp.get("seed", i),
                               heat=sp.get("heat", 1e-3), umax=sp.get("umax", 0.05), eps=eps, check_every=check_every,
                               max_steps=max_steps, use_graphs=False, prec=prec, filter_radius=filter_radius,
                               fp32_floor=fp32_floor)
                    for i, sp in enumerate(specs)]
+        self.fp32_floor = self.hs[0].fp32_floor if self.hs else fp32_floor
         self.n = len(self.hs)
         self.N = np.array([h.N for h in self.hs])
         self.tuned = None
@@ -151,7 +154,7 @@ class HeatsinkBatch2D:
             if self.hs[0].pf.prec == F32:
                 stall = np.where(r < 0.95 * best, 0, stall + 1)
                 best = np.minimum(best, r)
-                ready = (r < self.fp32_floor) | ((stall >= 10) & (r < 100 * self.fp32_floor))
+                ready = (r < self.fp32_floor) | ((stall >= 10) & (r < max(100 * self.fp32_floor, STALL_CEILING)))
                 if np.all(ready[act]):
                     self.t32 = t
                     if graph is not None:
@@ -180,6 +183,39 @@ class HeatsinkBatch2D:
         self._destroy(batches)
         return t, conv
 
+    def _record(self, conv, t_end, kind, hs=None):
+        """Each instance's step count and buffer parity: a frozen instance's state is that of its
+        convergence step, the others' that of the phase's last step."""
+        for h, c in zip(hs if hs is not None else self.hs, conv):
+            steps = int(c) if c > 0 else t_end
+            setattr(h, kind + "_steps", steps)
+            setattr(h, kind + "_buf", steps % 2)
+
+    def _redo_in_fp64(self, conv, step_fn, init, kind):
+        """Instances whose reduced-precision solve diverged (conv -2) are solved again from their
+        cold start in FP64, together in one batch; conv is updated in place.  Returns their
+        indices."""
+        idx = [i for i, c in enumerate(conv) if c == -2] if self.prec != F64 else []
+        if not idx:
+            return idx
+        saved = (self.hs, self.n, self.N, self.prec)
+        sub = [self.hs[i] for i in idx]
+        precs = [h.prec for h in sub]
+        try:
+            for h in sub:
+                h.prec = F64
+                init(h)
+            self.hs, self.n, self.N, self.prec = sub, len(sub), np.array([h.N for h in sub]), F64
+            t, c2 = self._phase(step_fn)
+        finally:
+            for h, p in zip(sub, precs):
+                h.prec = p
+            self.hs, self.n, self.N, self.prec = saved
+        self._record(c2, t, kind, sub)
+        for i, c in zip(idx, c2):
+            conv[i] = c
+        return idx
+
     def evaluate(self):
         """Forward, adjoint and sensitivity of every instance; returns timings and step counts."""
         t0 = time.perf_counter()
@@ -187,24 +223,27 @@ class HeatsinkBatch2D:
         self.t32 = 0
         for h in self.hs:
             h.init_forward()
-        tf, conv_f = self._phase(lambda h, t: h._fwd_step(t))
+        fwd_step = lambda h, t: h._fwd_step(t)
+        tf, conv_f = self._phase(fwd_step)
         t32f = self.t32
-        for h, c in zip(self.hs, conv_f):  # a frozen instance's state is that of its convergence step
-            h.fwd_steps = int(c) if c > 0 else tf
-            h.fwd_buf = h.fwd_steps % 2
+        self._record(conv_f, tf, "fwd")
+        redo_f = self._redo_in_fp64(conv_f, fwd_step, lambda h: h.init_forward(), "fwd")
         self.adjoint = True
         self.t32 = 0
         for h in self.hs:
             h.init_adjoint()
-        ta, conv_a = self._phase(lambda h, t: h._adj_step(t, h.fwd_buf))
-        for h, c in zip(self.hs, conv_a):
-            h.adj_steps = int(c) if c > 0 else ta
-            h.adj_buf = h.adj_steps % 2
+        adj_step = lambda h, t: h._adj_step(t, h.fwd_buf)
+        ta, conv_a = self._phase(adj_step)
+        t32a = self.t32
+        self._record(conv_a, ta, "adj")
+        redo_a = self._redo_in_fp64(conv_a, adj_step, lambda h: h.init_adjoint(), "adj")
+        self.t32 = t32a
         objectives = [h.sensitivity() for h in self.hs]
         self.ctx.sync()
         dt = time.perf_counter() - t0
         return {"seconds": dt, "fwd_steps": tf, "adj_steps": ta, "fwd_conv": conv_f.tolist(),
                 "adj_conv": conv_a.tolist(), "fp32_steps": (t32f, self.t32), "objectives": objectives,
+                "fp64_redo": (redo_f, redo_a),
                 "tuned": self.tuned,
                 "node_steps": int(sum(h.N * (h.fwd_steps + h.adj_steps) for h in self.hs))}
 

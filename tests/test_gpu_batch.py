@@ -83,6 +83,44 @@ def test_residual_batch_matches_single(ctx):
         np.testing.assert_allclose(out[i], r.value, rtol=1e-12)
 
 
+def test_residual_of_overflowing_field_is_nan(ctx):
+    """A field whose squares overflow while its step-to-step differences do not (a diverging
+    solution near 1e154): the reference's quotient is finite / inf = 0, which a convergence test
+    reads as converged; lbm_residual and the batched forms return NaN.  Ordinary fields keep the
+    reference's value (the oracle's restatement of residual.h)."""
+    import _oracle
+
+    import panslbm2_amd as pa
+
+    m = 1000
+    big = np.full(m, 1e154)  # each square finite, their sum over 2,000 entries not
+    a0, a1 = ctx.array((m,), data=big), ctx.array((m,), data=big)
+    b0, b1 = ctx.array((m,), data=big * (1 - 7e-3)), ctx.array((m,), data=big * (1 + 7e-3))
+    with np.errstate(over="ignore"):
+        want0 = np.sqrt(np.sum((big * 7e-3) ** 2 * 2) / np.sum(big ** 2 * 2))  # inf in the denominator: 0
+    assert want0 == 0.0
+    r = ctypes.c_double()
+    pa.check(ctx.lib.lbm_residual(ctx.h, 2, a0.ptr, a1.ptr, None, b0.ptr, b1.ptr, None, m, ctypes.byref(r)))
+    assert np.isnan(r.value)
+    lens = (ctypes.c_int * 1)(m)
+    a = (ctypes.c_void_p * 3)(a0.ptr, a1.ptr, None)
+    b = (ctypes.c_void_p * 3)(b0.ptr, b1.ptr, None)
+    out = np.zeros(1)
+    pa.check(ctx.lib.lbm_residual_batch(ctx.h, 1, 2, a, b, lens, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    assert np.isnan(out[0])
+    pa.check(ctx.lib.lbm_residual_batch_static(ctx.h, 1, 2, a, b, lens,
+                                               out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    assert np.isnan(out[0])
+    rng = np.random.default_rng(9)
+    x = [rng.standard_normal(m) for _ in range(4)]
+    dx = [ctx.array((m,), data=v) for v in x]
+    pa.check(ctx.lib.lbm_residual(ctx.h, 2, dx[0].ptr, dx[1].ptr, None, dx[2].ptr, dx[3].ptr, None, m,
+                                  ctypes.byref(r)))
+    want = _oracle.lib().ol_residual(2, *[v.ctypes.data_as(ctypes.c_void_p) for v in x[:2]], None,
+                                     *[v.ctypes.data_as(ctypes.c_void_p) for v in x[2:]], None, m)
+    assert abs(r.value - want) <= 1e-12 * want
+
+
 def test_tuner_rounds_are_steps(ctx):
     import panslbm2_amd as pa
     from panslbm2_amd.heatsink_batch import HeatsinkBatch2D
@@ -142,4 +180,68 @@ def test_batched_evaluation_matches_single(ctx, prec_name):
         assert abs(o - obj) <= tol_o * abs(obj)
         d = h.d["dfds"].download()
         assert np.max(np.abs(d - dfds)) <= tol_s * np.max(np.abs(dfds))
+    hb.close()
+
+
+def test_batched_evaluation_matches_oracle(ctx):
+    """The batch in FP64 (uneven grids, graphs, the autotuner's timed rounds included) against the
+    CPU-oracle replica of each design (tests/heatsink_oracle.py run): every instance stops at the
+    check step the oracle stops at, with the same forward temperature and objective, and dfds
+    within 1e-12 (pow-based terms)."""
+    import heatsink_oracle
+
+    import panslbm2_amd as pa
+    from panslbm2_amd.heatsink_batch import HeatsinkBatch2D
+
+    print("batched evaluation vs oracle", flush=True)
+    hb = HeatsinkBatch2D(ctx, SPECS, prec=pa.F64, use_graphs=True, tune=True, filter_radius=2.0, eps=1e-6)
+    r = hb.evaluate()
+    print("batch", r["fwd_conv"], r["adj_conv"], r["tuned"], flush=True)
+    for sp, h, o, cf, ca in zip(SPECS, hb.hs, r["objectives"], r["fwd_conv"], r["adj_conv"]):
+        want = heatsink_oracle.run(sp["lx"], sp["ly"], pa.D2Q5, seed=sp["seed"], heat=sp["heat"], umax=sp["umax"],
+                                   filter_radius=2.0, eps=1e-6)
+        print(sp, "oracle", want["fwd_steps"], want["adj_steps"], flush=True)
+        assert cf == want["fwd_steps"] and ca == want["adj_steps"]
+        np.testing.assert_array_equal(h.d["tem"].download(), want["tem"])
+        assert o == want["objective"]
+        d, wd = h.d["dfds"].download(), want["dfds"]
+        assert np.max(np.abs(d - wd)) <= 1e-12 * np.max(np.abs(wd))
+    hb.close()
+
+
+def test_fp32_divergence_is_solved_again_in_fp64(ctx):
+    """An instance whose FP32 adjoint turns non-finite (forced from the host: one NaN population
+    planted after its FP32 initial condition) is solved again from its cold start in FP64.
+    Every instance converges, the forward (and so the objective) is untouched, and the sensitivities
+    agree with the run without the NaN to the spread of where the solves stop (the batch converts
+    to FP64 when every active instance is ready, so dropping one moves that step)."""
+    import panslbm2_amd as pa
+    from panslbm2_amd.heatsink_batch import HeatsinkBatch2D
+
+    print("batched fp32 with a planted NaN", flush=True)
+    clean = HeatsinkBatch2D(ctx, SPECS, prec=pa.F32, use_graphs=True, tune=False, filter_radius=2.0)
+    rc = clean.evaluate()
+    want = [h.d["dfds"].download() for h in clean.hs]
+    clean.close()
+    hb = HeatsinkBatch2D(ctx, SPECS, prec=pa.F32, use_graphs=True, tune=False, filter_radius=2.0)
+    h1 = hb.hs[1]
+    orig = h1.init_adjoint
+
+    def planted():
+        orig()
+        if h1.pf.prec != pa.F64:
+            f = h1.pf.download()
+            f[3, h1.N // 2] = np.nan
+            h1.pf.upload(f)
+
+    h1.init_adjoint = planted
+    r = hb.evaluate()
+    print("redo", r["fp64_redo"], r["adj_conv"], flush=True)
+    assert r["fp64_redo"] == ([], [1])
+    assert min(r["fwd_conv"]) > 0 and min(r["adj_conv"]) > 0
+    assert r["objectives"] == rc["objectives"]  # the objective is the forward's
+    for i, h in enumerate(hb.hs):
+        d = h.d["dfds"].download()
+        assert np.all(np.isfinite(d))
+        assert np.max(np.abs(d - want[i])) <= 2e-3 * np.max(np.abs(want[i]))
     hb.close()

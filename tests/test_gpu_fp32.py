@@ -165,3 +165,35 @@ def test_fp32_heatsink_sensitivity_within_1e4(ctx):
     assert r32["fp32_steps"][0] >= 0.8 * r32["fwd_steps"] and r32["fp32_steps"][1] >= 0.8 * r32["adj_steps"]
     assert e32[0] <= max(1e-4, e64[0]) and e32[1] <= max(1e-4, e64[1])
     assert e32t[0] <= 2e-5 and e32t[1] <= 1e-5
+
+
+def test_fp32_divergence_is_solved_again_in_fp64(ctx):
+    """Heatsink2D: a reduced-precision adjoint whose residual turns non-finite (forced from the
+    host: one NaN population planted after its FP32 initial condition) is solved again from its
+    cold start in FP64, as the FP64 driver would solve it on the same forward fields."""
+    import panslbm2_amd as pa
+    from panslbm2_amd.heatsink2d import Heatsink2D
+
+    kw = dict(lx=24, ly=40, ad_kind=pa.D2Q5, seed=1, filter_radius=2.0, use_graphs=False)
+    hs = Heatsink2D(ctx, prec=pa.F32, **kw)
+    orig = hs.init_adjoint
+
+    def planted():
+        orig()
+        if hs.pf.prec != pa.F64:
+            f = hs.pf.download()
+            f[2, hs.N // 3] = np.nan
+            hs.pf.upload(f)
+
+    hs.init_adjoint = planted
+    r = hs.evaluate()
+    print("redo", r["fp64_redo"], r["fwd_steps"], r["adj_steps"], r["fp32_steps"], flush=True)
+    assert r["fp64_redo"] == [False, True] and not r["diverged"]
+    assert max(r["adj_residual"]) < 1e-6 and r["fp32_steps"][1] == 0
+    got = hs.d["dfds"].download()
+    # the same forward fields, adjoint solved in FP64 directly
+    hs.init_adjoint = orig
+    hs.in_fp64(hs.adjoint)
+    hs.sensitivity()
+    np.testing.assert_array_equal(got, hs.d["dfds"].download())
+    hs.close()
