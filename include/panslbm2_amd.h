@@ -310,6 +310,11 @@ int lbm_density_filter(lbm_context *ctx, int lx, int ly, int lz, double R, const
 /* ---- recording and fused steps ---- */
 int lbm_record_begin(lbm_context *ctx);
 int lbm_record_end(lbm_context *ctx, lbm_program **out);
+/* No reference counterpart.  lbm_record_end without the upload: the program's device tables are
+ * built at its first use (apply or collide), which must not be inside a graph capture
+ * (LBM_ERR_STATE).  For callers that record every step and keep an equal program recorded before
+ * (lbm_program_equal, host-side): the duplicate is destroyed without device work. */
+int lbm_record_end_deferred(lbm_context *ctx, lbm_program **out);
 int lbm_program_destroy(lbm_program *prog);
 /* run the recorded calls on their own, exactly as if they had been made (no collide) */
 int lbm_program_apply(lbm_program *prog);

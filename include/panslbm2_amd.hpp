@@ -172,7 +172,7 @@ public:
         if (!recording_) return nullptr;
         lbm_program *P = nullptr;
         recording_ = false;
-        check(lbm_record_end(ctx_, &P), "lbm_record_end");
+        check(lbm_record_end_deferred(ctx_, &P), "lbm_record_end_deferred");
         for (auto it = cache_.begin(); it != cache_.end(); ++it) {
             if (lbm_program_equal(it->prog, P) == 1) {
                 check(lbm_program_destroy(P), "lbm_program_destroy");

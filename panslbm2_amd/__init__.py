@@ -150,6 +150,7 @@ def load():
         "lbm_density_filter": [VP, I, I, I, D, VP, VP],
         "lbm_record_begin": [VP],
         "lbm_record_end": [VP, ctypes.POINTER(VP)],
+        "lbm_record_end_deferred": [VP, ctypes.POINTER(VP)],
         "lbm_program_destroy": [VP],
         "lbm_program_apply": [VP],
         "lbm_program_bind_fields": [VP, I] + [VP] * 6,

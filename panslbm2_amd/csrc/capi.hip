@@ -152,6 +152,7 @@ struct lbm_program {
     bool ordered_ok = true;  // no operator on a lattice after its SmoothCorner
     // device tables
     bool uploaded = false;
+    bool dev_used = false;  // device copies exist (a launch may still read them)
     DevBuf d_slot_entry, d_entry_node, d_entry_ops, d_ops, d_sops, d_slot_list;
     DevBuf d_side_of_slot, d_corner2d, d_side, d_side_nodes;  // 2D corner smoothing
     int nentry = 0, nslot = 0, maxops = 0, n_corner2d = 0, n_side = 0;
@@ -343,6 +344,10 @@ int prog_upload(lbm_program *P) {
     if (P->uploaded) return LBM_OK;
     lbm_lattice *L = P->lat[0];
     if (!L) return fail(LBM_ERR_STATE, "empty program");
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    HIPCHK(hipStreamIsCapturing(P->ctx->stream, &cap));
+    if (cap != hipStreamCaptureStatusNone)
+        return fail(LBM_ERR_STATE, "a program's first use is inside a graph capture: use it once before capturing");
     // group operators by node, keeping record order within a node
     std::map<int, std::vector<int>> by_node;
     for (size_t t = 0; t < P->ops.size(); ++t) by_node[P->ops[t].node].push_back((int)t);
@@ -445,6 +450,7 @@ int prog_upload(lbm_program *P) {
     if ((rc = up(P->d_side_nodes, side_nodes.data(), side_nodes.size() * sizeof(int)))) return rc;
     P->n_side = (int)side_nodes.size();
     P->uploaded = true;
+    P->dev_used = true;
     return LBM_OK;
 }
 
@@ -1576,24 +1582,27 @@ int lbm_record_begin(lbm_context *C) {
     C->recording->ctx = C;
     return LBM_OK;
 }
-int lbm_record_end(lbm_context *C, lbm_program **out) {
+int lbm_record_end_deferred(lbm_context *C, lbm_program **out) {
     if (!C || !out) return fail(LBM_ERR_ARG, "null argument");
     if (!C->recording) return fail(LBM_ERR_STATE, "not recording");
-    lbm_program *P = C->recording;
+    *out = C->recording;
     C->recording = nullptr;
-    if (P->nlat > 0) {
-        int rc = prog_upload(P);
-        if (rc) {
-            delete P;
-            return rc;
-        }
+    return LBM_OK;
+}
+int lbm_record_end(lbm_context *C, lbm_program **out) {
+    lbm_program *P = nullptr;
+    int rc = lbm_record_end_deferred(C, &P);
+    if (rc) return rc;
+    if (P->nlat > 0 && (rc = prog_upload(P))) {
+        delete P;
+        return rc;
     }
     *out = P;
     return LBM_OK;
 }
 int lbm_program_destroy(lbm_program *P) {
     if (P) {
-        (void)hipStreamSynchronize(P->ctx->stream);
+        if (P->dev_used) (void)hipStreamSynchronize(P->ctx->stream);
         delete P;
     }
     return LBM_OK;

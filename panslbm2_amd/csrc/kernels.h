@@ -173,8 +173,7 @@ __device__ __forceinline__ void apply_op(const Op &op, double *a, double *b, con
 // padded with OP_NONE to a multiple of kOpBatch, its count in the first operator; a batch is
 // fetched with independent loads together with the node's fields.
 template <class LA, class LB>
-__device__ __forceinline__ void run_slot_ops(const ProgView &pv, int slot, int n, double *a, double *b) {
-    const int li = pv.slot_list[slot];
+__device__ __forceinline__ void run_list_ops(const ProgView &pv, int li, int n, double *a, double *b) {
     if (li < 0) return;
     const Op *p = pv.sops + (size_t)li * pv.maxops;
     NodeFieldCache fc = prefetch_fields(pv, n);
@@ -188,6 +187,17 @@ __device__ __forceinline__ void run_slot_ops(const ProgView &pv, int slot, int n
             if (op.code != OP_NONE && op.code != OP_SMOOTH) apply_op<LA, LB>(op, a, b, pv, n, fc);
         });
     }
+}
+
+template <class LA, class LB>
+__device__ __forceinline__ void run_slot_ops(const ProgView &pv, int slot, int n, double *a, double *b) {
+    run_list_ops<LA, LB>(pv, pv.slot_list[slot], n, a, b);
+}
+// Operator list of node (i, j, k), or -1 (no boundary slot, or a slot without operators).  The
+// fused kernels load it before the node's pull so that its latency overlaps the pull's.
+__device__ __forceinline__ int list_of_node(const ProgView &pv, const Grid &g, int i, int j, int k, int nd) {
+    const int slot = boundary_slot(i, j, k, g.nx, g.ny, g.nz, nd);
+    return (slot >= 0 && slot < pv.nslot) ? pv.slot_list[slot] : -1;
 }
 
 // E_c of a shift, in FP64
@@ -332,10 +342,10 @@ __device__ __forceinline__ void node_state(const double *sa, const double *sb, c
                                            double *a, double *b, const Enc &, const Enc &, const Sh4 &, const Sh4 &) {
     int i, j, k;
     coords(n, g, i, j, k);
+    const int li = list_of_node(pv, g, i, j, k, LA::nd);
     load_pull<LA>(sa, g, i, j, k, n, pv.pull[0], a);
     load_pull<LB>(sb, g, i, j, k, n, pv.pull[1], b);
-    int slot = boundary_slot(i, j, k, g.nx, g.ny, g.nz, LA::nd);
-    if (slot >= 0 && slot < pv.nslot) run_slot_ops<LA, LB>(pv, slot, n, a, b);
+    run_list_ops<LA, LB>(pv, li, n, a, b);
 }
 template <class LA, class LB, class S, std::enable_if_t<!std::is_same_v<S, double>, int> = 0>
 __device__ __forceinline__ void node_state(const S *sa, const S *sb, const Grid &g, const ProgView &pv, int n,
@@ -343,14 +353,14 @@ __device__ __forceinline__ void node_state(const S *sa, const S *sb, const Grid 
                                            const Sh4 &ob) {
     int i, j, k;
     coords(n, g, i, j, k);
+    const int li = list_of_node(pv, g, i, j, k, LA::nd);
     load_pull<LA>(sa, g, i, j, k, n, pv.pull[0], a);
     load_pull<LB>(sb, g, i, j, k, n, pv.pull[1], b);
-    int slot = boundary_slot(i, j, k, g.nx, g.ny, g.nz, LA::nd);
-    if (slot >= 0 && slot < pv.nslot) {
+    if (li >= 0) {
         double xa[LA::nc], xb[LB::nc > 0 ? LB::nc : 1];
         dev_to_f64<LA>(a, xa, ea, oa);
         if constexpr (LB::nc > 0) dev_to_f64<LB>(b, xb, eb, ob);
-        run_slot_ops<LA, LB>(pv, slot, n, xa, xb);
+        run_list_ops<LA, LB>(pv, li, n, xa, xb);
         f64_to_dev<LA>(xa, a, ea, oa);
         if constexpr (LB::nc > 0) f64_to_dev<LB>(xb, b, eb, ob);
     }

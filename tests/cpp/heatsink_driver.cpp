@@ -6,12 +6,15 @@
 //                                                  the golden output (this container only)
 //   g++ -I include ... -lpanslbm2_hip           -> the same calls through panslbm2_amd.hpp on the GPU
 //
-//   heatsink_driver OUT [host|device]
+//   heatsink_driver OUT [host|device] [lx ly nt]
 // OUT receives every field the run leaves on the host, as records (name[32], int64 count, double[count]).
 // "device" (GPU build only) keeps fields on the device between calls and syncs them where the host
-// reads them.
+// reads them.  lx ly nt (default 30 17 240) size the grid and the steps per solve; the wall time of
+// the forward and adjoint loops is printed.
+#include <chrono>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -72,7 +75,13 @@ int main(int argc, char **argv) {
     if (mode == "device") gpu::SetCoherence(gpu::Coherence::Device);
 #endif
 
-    const int lx = 30, ly = 17, nt = 240, dt = 40;
+    const int lx = argc > 5 ? std::atoi(argv[3]) : 30, ly = argc > 5 ? std::atoi(argv[4]) : 17;
+    const int nt = argc > 5 ? std::atoi(argv[5]) : 240, dt = 40;
+    if (lx < 3 || ly < 3 || nt < 1) {
+        std::fprintf(stderr, "bad grid or step count\n");
+        return 2;
+    }
+    double loop_seconds = 0.0;
     const double nu = 0.1, diff_f = 0.1 / 6.0, diff_s = 10.0 * diff_f, u0 = 0.05, qn = 1.0e-2;
     const double qa = 0.01, amax = 40.0, qg = 1.0;
     const int L = lx / 2;  // heat source: j == 0, i < L
@@ -140,6 +149,7 @@ int main(int argc, char **argv) {
             HOST_WROTE(p);
 
         // ---- forward ----
+        auto t0 = std::chrono::steady_clock::now();
         NS::InitialCondition(pf, rho, ux, uy);
         AD::InitialCondition(pg, tem, ux, uy);
         for (int t = 1; t <= nt; ++t) {
@@ -166,11 +176,13 @@ int main(int argc, char **argv) {
             std::swap(qy, qyp);
         }
         HOST_READS();
+        auto t1 = std::chrono::steady_clock::now();
         double f = 0.0;
         for (int i = 0; i < L; ++i) f += tem[pf.Index(i, 0)];
         f /= L;
 
         // ---- adjoint ----
+        auto t2 = std::chrono::steady_clock::now();
         ANS::InitialCondition(pf, ux, uy, irho, iux, iuy);
         AAD::InitialCondition(pg, ux, uy, item, iqx, iqy);
         for (int t = 1; t <= nt; ++t) {
@@ -200,6 +212,10 @@ int main(int argc, char **argv) {
             std::swap(iqx, iqxp);
             std::swap(iqy, iqyp);
         }
+
+        HOST_READS();
+        auto t3 = std::chrono::steady_clock::now();
+        loop_seconds += std::chrono::duration<double>(t1 - t0).count() + std::chrono::duration<double>(t3 - t2).count();
 
         // ---- sensitivities ----
         AAD::SensitivityBrinkmanDiffusivity(pg, dfds.data(), ux, uy, imx, imy, dads.data(), tem, item, iqx, iqy,
@@ -246,5 +262,8 @@ int main(int argc, char **argv) {
     out.put("residuals", residuals.data(), (int64_t)residuals.size());
     std::fclose(fp);
     std::printf("heatsink_driver: wrote %s (%s)\n", argv[1], mode.c_str());
+    std::printf("heatsink_driver: %dx%d D2Q9+D2Q9, 2 designs x (%d forward + %d adjoint steps) in %.4f s: "
+                "%.1f us per step, %.1f MLUPS\n",
+                lx, ly, nt, nt, loop_seconds, loop_seconds / (4.0 * nt) * 1e6, 4.0 * nt * N / loop_seconds / 1e6);
     return 0;
 }
