@@ -33,16 +33,18 @@ D = ctypes.c_double
 VP = ctypes.c_void_p
 
 
-# A reduced-precision solve whose residual has stopped falling (not 5 % lower in ten checks) is
-# at that precision's floor once the residual is below this or 100x the switch floor; plateaus
-# of the early transient lie above it.
+# A reduced-precision solve whose residual has stopped falling (not 5 % lower in ten checks) below
+# 100x the switch floor is at that precision's floor; between that and STALL_CEILING (the FP32
+# heat-flux residual of a high-Peclet channel levels off at ~1e-5) it must stall twice as long, so
+# that a slow stretch of the convergence is not taken for the floor; plateaus of the early
+# transient lie above STALL_CEILING.
 STALL_CEILING = 1e-4
 
 
 def switch_ready(worst, floor, stall):
     """Whether a reduced-precision solve converts to FP64: its residual is below the switch floor,
-    or it has stalled (ten checks without a 5 % drop) below max(100 floor, STALL_CEILING)."""
-    return worst < floor or (stall >= 10 and worst < max(100.0 * floor, STALL_CEILING))
+    or has stalled (no 5 % drop) for ten checks below 100 floor or twenty below STALL_CEILING."""
+    return worst < floor or (stall >= 10 and worst < 100.0 * floor) or (stall >= 20 and worst < STALL_CEILING)
 
 
 class Heatsink2D:
@@ -241,8 +243,8 @@ class Heatsink2D:
     def _solve(self, key, step, res):
         """Chunks of check_every steps until both residuals are below eps.  In FP32 / FP16 the
         residual is watched (switch_ready): once it is below fp32_floor, or has stalled below
-        max(100 fp32_floor, STALL_CEILING), the lattices convert to FP64 and the solve continues
-        there.  Returns (steps, steps in reduced precision, final residuals)."""
+        100 fp32_floor (ten checks) or STALL_CEILING (twenty checks), the lattices convert to FP64
+        and the solve continues there.  Returns (steps, steps in reduced precision, final residuals)."""
         ce = self.check_every
         t, t32, best, stall = 0, 0, np.inf, 0
         r = (np.inf, np.inf)

@@ -8,8 +8,9 @@ The step order per instance is exactly the single-instance driver's, so with FP6
 instance's result is value-identical to solving it alone.
 
 Precision as in Heatsink2D: with prec=F32 all instances run in FP32 until every active one's
-residual has reached the FP32 floor, or has stopped falling (not 5 % lower in ten checks) below
-max(100 floor, STALL_CEILING), then every lattice converts to FP64 and the solves finish at eps.
+residual has reached the FP32 floor or has stopped falling (not 5 % lower in ten checks below
+100x the floor, or in twenty below STALL_CEILING), then every lattice converts to FP64 and the
+solves finish at eps.
 An instance that has converged is frozen (its workgroups return at once) at that check step, so
 each instance stops where it would have stopped alone.  Instances whose reduced-precision solve
 diverges are solved again from their cold start in FP64, together in one more batched phase.
@@ -154,7 +155,8 @@ class HeatsinkBatch2D:
             if self.hs[0].pf.prec == F32:
                 stall = np.where(r < 0.95 * best, 0, stall + 1)
                 best = np.minimum(best, r)
-                ready = (r < self.fp32_floor) | ((stall >= 10) & (r < max(100 * self.fp32_floor, STALL_CEILING)))
+                ready = ((r < self.fp32_floor) | ((stall >= 10) & (r < 100 * self.fp32_floor))
+                         | ((stall >= 20) & (r < STALL_CEILING)))
                 if np.all(ready[act]):
                     self.t32 = t
                     if graph is not None:
