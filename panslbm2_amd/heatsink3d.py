@@ -185,11 +185,15 @@ class Heatsink3D(Heatsink2D):
             R, beta = D(self.filter_radius), D(self.heaviside_beta)
             if "dfdx" not in d:
                 d["dfdx"] = c.array((self.N,))
+                d["dfdg"] = c.array((self.N,))
                 d["dgdg"] = c.array((self.N,), data=np.where(self.region, -1.0 / (vf * n), 0.0))
                 d["dgdx"] = c.array((self.N,))
             box = self.design_box
+            # gamma of the base plate (and outside the box) is set after the filter, whatever x is:
+            # its sensitivity must not flow back into the design variables near the plate
+            d["dfdg"].upload(np.where(self.region, d["dfds"].download(), 0.0))
             check(self.L.lbm_heaviside_sensitivity_region(c.h, lx, ly, lz, R, beta, *box, d["raw"].ptr,
-                                                          d["dfds"].ptr, d["dfdx"].ptr))
+                                                          d["dfdg"].ptr, d["dfdx"].ptr))
             check(self.L.lbm_heaviside_sensitivity_region(c.h, lx, ly, lz, R, beta, *box, d["raw"].ptr,
                                                           d["dgdg"].ptr, d["dgdx"].ptr))
             dfds = d["dfdx"].download()[region]

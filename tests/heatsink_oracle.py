@@ -235,3 +235,89 @@ def run3d(lx, ly, lz, ad_kind, seed=0, nu=0.1, pr=6.0, uin=0.03, heat=1e-3, plat
                 "item": item, "iqx": IQ[ab][0].copy(), "iqy": IQ[ab][1].copy(), "iqz": IQ[ab][2].copy(), "ig": ig,
                 "dfds": dfds, "objective": objsum / int(bottom.sum()), "adj_steps": t, "adj_res": (ru, rq)})
     return out
+
+
+def loop3d(lx, ly, lz, iterations, vf=0.4, move=0.05, plate=3, box=None, filter_radius=2.0, stages=None,
+           stage_iterations=100, stage_tol=0.01, x0=None, **kw):
+    """CPU replica of Heatsink3D.optimize(optimizer="mma") with a Heaviside projection, built from the
+    reference's own pieces: HeavisideFilter with the design-region weights (oracle/_ref
+    ref3_heaviside_*_region, heatsink3D.cpp:87-93,103), run3d for each evaluation, and the
+    reference's MMA (oracle/_ref refmma_*, mma.h) with one volume constraint, as Heatsink3D drives them:
+    gamma = projection of x with the base plate solid and fluid outside the box; dF/dgamma zero where
+    gamma is set after the filter; sensitivities through the filter; the objective's gradient
+    normalised to a largest magnitude of 1; (qf, beta) continuation.  Returns per iteration
+    (objective, x, gamma, dfds)."""
+    from _oracle import D3Q7, ref
+
+    R = ref()
+    if R is None:
+        raise RuntimeError("reference harness not built")
+    VP, I = ctypes.c_void_p, ctypes.c_int
+    R.ref3_heaviside_value_region.argtypes = [I, I, I, D, D, I, I, I, VP, VP]
+    R.ref3_heaviside_sensitivity_region.argtypes = [I, I, I, D, D, I, I, I, VP, VP, VP]
+    R.refmma_create.restype = VP
+    R.refmma_create.argtypes = [I, I, D, VP, VP, VP, VP, VP, D]
+    R.refmma_update.argtypes = [VP, I, I, VP, D, VP, VP, VP]
+    R.refmma_destroy.argtypes = [VP]
+    N = lx * ly * lz
+    idx = np.arange(N)
+    i, j, k = idx % lx, (idx // lx) % ly, idx // (lx * ly)
+    box = tuple(box) if box is not None else (lx, ly, lz)
+    in_box = (i < box[0]) & (j < box[1]) & (k < box[2])
+    region = in_box & (k >= plate)
+    rid = np.nonzero(region)[0]
+    n = len(rid)
+    stages = stages or ((kw.pop("qf", 0.01), 1.0),)
+    stage, cnt = 0, 0
+    qf, beta = stages[0]
+
+    def clean(x):
+        x = np.clip(np.asarray(x, dtype=np.float64), 0.0, 1.0).copy()
+        x[~in_box] = 1.0
+        return x
+
+    def gamma_of(x):
+        s = np.zeros(N)
+        R.ref3_heaviside_value_region(lx, ly, lz, D(filter_radius), D(beta), *box, ptr(x), ptr(s))
+        s[~in_box] = 1.0
+        s[k < plate] = 0.0
+        return s
+
+    def sens(x, d):
+        out = np.zeros(N)
+        R.ref3_heaviside_sensitivity_region(lx, ly, lz, D(filter_radius), D(beta), *box, ptr(x),
+                                            ptr(np.ascontiguousarray(d)), ptr(out))
+        return out
+
+    x = clean(np.where(region, 1.0 - vf, 0.0) if x0 is None else x0)
+    z = np.zeros(1)
+    mma = R.refmma_create(n, 1, D(1.0), ptr(z), ptr(np.full(1, 1e4)), ptr(z), ptr(np.zeros(n)), ptr(np.ones(n)),
+                          D(move))
+    hist = []
+    try:
+        for it in range(iterations):
+            s = gamma_of(x)
+            r = run3d(lx, ly, lz, D3Q7, plate=plate, qf=qf, filter_radius=filter_radius, gamma=s, **kw)
+            hist.append((r["objective"], x.copy(), s, r["dfds"]))
+            if it + 1 == iterations:
+                break
+            dfdx = sens(x, np.where(region, r["dfds"], 0.0))[rid]
+            dgdx = sens(x, np.where(region, -1.0 / (vf * n), 0.0))[rid]
+            g = np.array([(1.0 - s[rid]).sum() / (vf * n) - 1.0])
+            dfdx = dfdx / max(np.abs(dfdx).max(), 1e-300)
+            xr = np.ascontiguousarray(x[rid])
+            R.refmma_update(mma, n, 1, ptr(xr), D(r["objective"]), ptr(dfdx), ptr(g), ptr(np.ascontiguousarray(dgdx)))
+            x_old = x.copy()
+            x[rid] = xr
+            x = clean(x)
+            if len(stages) > 1:
+                cnt += 1
+                dsmax = float(np.abs(x - x_old).max())
+                if stage == len(stages) - 1 and dsmax < stage_tol:
+                    break
+                if dsmax < stage_tol or cnt % stage_iterations == 0:
+                    stage, cnt = min(stage + 1, len(stages) - 1), 0
+                    qf, beta = stages[stage]
+    finally:
+        R.refmma_destroy(mma)
+    return hist

@@ -141,3 +141,34 @@ def test_continuation_stages(ctx):
         assert (qf, beta) == Heatsink3D.REFERENCE_STAGES[st]
     assert len(hist) == len(seen)
     hs.close()
+
+
+def test_design_loop_matches_oracle_loop(ctx):
+    """The 3D MMA design loop (Heaviside projection with the design sub-box, base plate, the
+    reference driver's (qf, beta) continuation advanced every iteration) against its CPU replica
+    built from the reference's HeavisideFilter and MMA and the oracle evaluation
+    (tests/heatsink_oracle.py loop3d).  A given gamma evaluates value-identically; the device
+    projection and MMA agree with the reference's to ~1e-15 per call, so per iteration gamma is held
+    to 1e-13, the objective to 1e-10, dfds to 1e-9 and the design to 1e-9."""
+    import heatsink_oracle
+
+    from panslbm2_amd.heatsink3d import Heatsink3D
+
+    print("3d design loop vs oracle loop", flush=True)
+    kw = dict(lx=14, ly=10, lz=9, plate=3, max_steps=400, eps=1e-30)
+    box = (10, 8, 7)
+    hs = Heatsink3D(ctx, heaviside_beta=1.0, design_box=box, **kw)
+    got = []
+    hs.optimize(3, vf=0.4, move=0.05, optimizer="mma", stages=Heatsink3D.REFERENCE_STAGES, stage_iterations=1,
+                callback=lambda r: got.append((r["objective"], hs.x.copy(), hs.gamma.copy(), hs.d["dfds"].download())))
+    hs.close()
+    want = heatsink_oracle.loop3d(14, 10, 9, 3, vf=0.4, move=0.05, plate=3, box=box, filter_radius=2.0,
+                                  stages=Heatsink3D.REFERENCE_STAGES, stage_iterations=1, max_steps=400, eps=1e-30)
+    assert len(got) == len(want) == 3
+    for it, ((o, x, s, d), (wo, wx, ws, wd)) in enumerate(zip(got, want)):
+        print("iteration", it, o, wo, np.abs(x - wx).max(), np.abs(s - ws).max(), flush=True)
+        assert np.abs(s - ws).max() <= 1e-13
+        assert abs(o - wo) <= 1e-10 * abs(wo)
+        assert np.abs(d - wd).max() <= 1e-9 * np.abs(wd).max()
+        assert np.abs(x - wx).max() <= 1e-9
+    assert np.abs(got[-1][1] - got[0][1]).max() > 1e-3  # the designs moved
