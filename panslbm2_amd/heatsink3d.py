@@ -146,7 +146,12 @@ class Heatsink3D(Heatsink2D):
         a0 = 1, a = 0, c = 1e4, d = 0, the move limit, and the sensitivity normalised to a largest
         magnitude of 1 (normalize.h:9-24).  optimizer="pg": a projected-gradient step
         x <- clip(x - move dF/ds / max|dF/ds| + lam, x - move, x + move, 0, 1) with lam found by
-        bisection so that the solid fraction equals vf."""
+        bisection so that the solid fraction equals vf.
+
+        With heaviside_beta the gradients reach x through the projection's sensitivity (the
+        reference driver's chain, heatsink3D.cpp:249-250), with dF/dgamma zero where gamma is set
+        after the filter (base plate, outside the box).  Without it, the updates take dF/dgamma
+        as the gradient in x: the density filter's own chain is not applied."""
         if optimizer == "mma":
             return self._mma_update(vf, move)
         dfds = self.d["dfds"].download()
