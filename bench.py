@@ -326,6 +326,86 @@ def batch_extra(ctx, n, seed=2024):
                     "a 1e-6 step-to-step residual leaves (~1e-4 of the slowest mode)"}
 
 
+def design_batch(ctx, n, iters, seed=2024, vf=0.4, move=0.05, fast_prec="F32", fast_warm=True, plain=True):
+    """n 2D heat-sink designs (the batch's grid sizes, heat sources and inlet velocities), each
+    optimised for `iters` projected-gradient iterations, two ways: the plain GPU form (one design
+    after another, cold FP64 solves) and the fast form (all designs batched in one launch per step,
+    FP32 storage carrying the solves, each solve warm-started from its design's previous converged
+    solution).  Seconds per design iteration of all n designs, after the first."""
+    import numpy as np
+
+    from panslbm2_amd import D2Q5, F32, F64
+    from panslbm2_amd.heatsink2d import Heatsink2D
+    from panslbm2_amd.heatsink_batch import HeatsinkBatch2D
+
+    rng = np.random.default_rng(seed)
+    sizes = [(64, 128), (128, 256), (200, 400), (256, 512)]
+    specs = []
+    for i in range(n):
+        lx, ly = sizes[rng.integers(0, 4)]
+        specs.append(dict(lx=lx, ly=ly, seed=100 + i, heat=float(rng.uniform(5e-4, 2e-3)),
+                          umax=float(rng.uniform(0.02, 0.06))))
+
+    def run(evaluate, hs, label):
+        times, steps, objs = [], [], []
+        for it in range(iters):
+            t0 = time.perf_counter()
+            rs = evaluate(it)
+            for h, ok in zip(hs, rs["ok"]):
+                if ok:
+                    h.design_update(vf, move)
+            ctx.sync()
+            times.append(time.perf_counter() - t0)
+            steps.append(rs["node_steps"])
+            objs.append(rs["objectives"])
+            print("bench: design batch %s it %d %.3f s" % (label, it, times[-1]), file=sys.stderr, flush=True)
+        return times, steps, objs
+
+    import panslbm2_amd as pa
+
+    fast_label = "batched_%s_%s" % (fast_prec.lower(), "warm" if fast_warm else "cold")
+    hb = HeatsinkBatch2D(ctx, specs, prec=getattr(pa, fast_prec), tune=True)
+
+    def ev_fast(it):
+        r = hb.evaluate(warm=fast_warm and it > 0)
+        return {"ok": [a > 0 and b > 0 for a, b in zip(r["fwd_conv"], r["adj_conv"])], "objectives": r["objectives"],
+                "node_steps": r["node_steps"]}
+
+    tf, sf, of = run(ev_fast, hb.hs, fast_label)
+    hb.close()
+    if plain:
+        hs = [Heatsink2D(ctx, lx=sp["lx"], ly=sp["ly"], ad_kind=D2Q5, seed=sp["seed"], heat=sp["heat"],
+                         umax=sp["umax"], prec=F64) for sp in specs]
+
+        def ev_plain(it):
+            rs = [h.evaluate(warm=False) for h in hs]
+            return {"ok": [not r["diverged"] for r in rs], "objectives": [r["objective"] for r in rs],
+                    "node_steps": sum(h.N * (r["fwd_steps"] + r["adj_steps"]) for h, r in zip(hs, rs))}
+
+        tp, sp_, op = run(ev_plain, hs, "plain_fp64_cold")
+        for h in hs:
+            h.close()
+    else:
+        tp, sp_, op = tf, sf, of
+    later = slice(1, None) if iters > 1 else slice(0, None)
+    op, of = np.array(op), np.array(of)
+    ok = np.all(np.isfinite(op), axis=0) & np.all(np.isfinite(of), axis=0)
+    out = {"designs": n, "iterations": iters, "solid_fraction": vf, "move": move,
+           "plain_fp64_cold": {"s_per_iteration": round(float(np.mean(tp[later])), 4),
+                               "node_steps_per_iteration": int(np.mean(sp_[later]))},
+           fast_label: {"s_per_iteration": round(float(np.mean(tf[later])), 4),
+                        "node_steps_per_iteration": int(np.mean(sf[later]))},
+           "designs_converging_in_both": int(ok.sum()),
+           "mean_objective_first_last": {"plain": [float(op[0, ok].mean()), float(op[-1, ok].mean())],
+                                         "fast": [float(of[0, ok].mean()), float(of[-1, ok].mean())]}}
+    out["speedup"] = round(out["plain_fp64_cold"]["s_per_iteration"] / out[fast_label]["s_per_iteration"], 2)
+    if not plain:
+        del out["plain_fp64_cold"], out["speedup"]
+    out["note"] = ("per design iteration of all designs after the first; designs whose solves diverge (inlet "
+                   "speeds near 0.06) keep their design and are solved again each iteration in both forms")
+    return out
+
+
 def transient_extra(ctx, nt=2000):
     """panslbm2_amd/transient2d.py on the reference program's grid (second evaluation timed)."""
     from panslbm2_amd.transient2d import TransientHeatsink2D
@@ -614,6 +694,12 @@ def main():
     ap.add_argument("--design-modes", default="cold_fp64,previous_solution_fp64,learned_fp64,learned_fp32",
                     help="comma-separated modes of --design-loop: " + ", ".join(DESIGN_LOOP_MODES))
     ap.add_argument("--stage-iterations", type=int, default=20, help="--design-loop iterations per continuation stage")
+    ap.add_argument("--design-batch", type=int, default=0,
+                    help="run only the batched 2D design loops: this many designs, plain vs FP32 + batched + warm")
+    ap.add_argument("--design-batch-iters", type=int, default=8, help="--design-batch iterations")
+    ap.add_argument("--design-batch-prec", choices=["F32", "F64"], default="F32",
+                    help="--design-batch: storage of the batched form")
+    ap.add_argument("--design-batch-cold", action="store_true", help="--design-batch: batched form without warm starts")
     ap.add_argument("--design-eps", type=float, default=1e-6, help="--design-loop steady-state residual")
     ap.add_argument("--kernel-only", choices=["fwd", "adj", "stream", "unet", "fwd3", "adj3"],
                     help="launch only the fused 2D forward / adjoint step (--lx x --ly), the pull-stream copy, "
@@ -631,6 +717,12 @@ def main():
     ctx = Context(0)
     if a.kernel_only:
         kernel_only(ctx, a)
+        ctx.close()
+        return
+    if a.design_batch > 0:
+        print(json.dumps({"design_batch": design_batch(ctx, a.design_batch, a.design_batch_iters,
+                                                        fast_prec=a.design_batch_prec,
+                                                        fast_warm=not a.design_batch_cold)}))
         ctx.close()
         return
     if a.design_loop > 0:

@@ -66,22 +66,18 @@ class Heatsink2D:
 
         # design field: seeded uniform[0,1], density filter on the device
         rng = np.random.default_rng(seed)
-        raw = c.array((N,), data=rng.uniform(0.0, 1.0, N))
+        self.x = rng.uniform(0.0, 1.0, N)
+        raw = c.array((N,), data=self.x)
         gam = c.array((N,), fill=0.0)
         check(self.L.lbm_density_filter(c.h, lx, ly, 1, D(filter_radius), raw.ptr, gam.ptr))
         s = gam.download()
         self.gamma = s
-        dfl = nu / pr
-        dso = 10.0 * dfl
-        coef = {
-            "alpha": alpha_max / (ly - 1.0) * qf * (1.0 - s) / (s + qf),
-            "dads": -alpha_max / (ly - 1.0) * qf * (1.0 + qf) / (s + qf) ** 2,
-            "diff": dso + (dfl - dso) * s * (1.0 + qg) / (s + qg),
-            "dkds": (dfl - dso) * qg * (1.0 + qg) / (s + qg) ** 2,
-            "heat": heat * (1.0 - s),
-            "dhds": np.full(N, -heat),
-        }
-        self.d = {k: c.array((N,), data=v) for k, v in coef.items()}
+        self.alpha_max, self.qf, self.qg, self.heat, self.filter_radius = alpha_max, qf, qg, heat, filter_radius
+        self.dfl = nu / pr
+        self.dso = 10.0 * self.dfl
+        self.d = {k: c.array((N,), data=v) for k, v in self._coefficients(s).items()}
+        self.d["raw"], self.d["design"] = raw, gam
+        self.kept = None  # device copies of the last converged fields (the warm start)
         # fields: saved values alternate between two buffers per quantity (u at t and t-1)
         for k in ("rho", "tem", "ip", "imx", "imy", "item", "dfds"):
             self.d[k] = c.array((N,), fill=0.0)
@@ -122,6 +118,65 @@ class Heatsink2D:
         self.fwd_prog = self._record_forward()
         self.adj_prog = None
         self._graphs = {}
+
+    def _coefficients(self, s):
+        """Brinkman, diffusivity and heat-source coefficients of design gamma = s (heatsink.cpp:104-117
+        interpolation, heat source Q(1 - gamma)) and their design derivatives."""
+        am, qf, qg, dfl, dso, ly = self.alpha_max, self.qf, self.qg, self.dfl, self.dso, self.ly
+        return {
+            "alpha": am / (ly - 1.0) * qf * (1.0 - s) / (s + qf),
+            "dads": -am / (ly - 1.0) * qf * (1.0 + qf) / (s + qf) ** 2,
+            "diff": dso + (dfl - dso) * s * (1.0 + qg) / (s + qg),
+            "dkds": (dfl - dso) * qg * (1.0 + qg) / (s + qg) ** 2,
+            "heat": self.heat * (1.0 - s),
+            "dhds": np.full(self.N, -self.heat),
+        }
+
+    def set_design(self, x):
+        """Design variables x (per node, clipped to [0, 1]) -> gamma = density-filtered x -> the
+        coefficient arrays, rewritten in place (recorded programs and graphs stay valid)."""
+        c, d = self.ctx, self.d
+        self.x = np.clip(np.asarray(x, dtype=np.float64), 0.0, 1.0).copy()
+        d["raw"].upload(self.x)
+        check(self.L.lbm_density_filter(c.h, self.lx, self.ly, 1, D(self.filter_radius), d["raw"].ptr,
+                                        d["design"].ptr))
+        self.gamma = d["design"].download()
+        for k, v in self._coefficients(self.gamma).items():
+            d[k].upload(v)
+
+    def design_update(self, vf=0.4, move=0.05):
+        """One projected-gradient design step over every node, as Heatsink3D's optimizer="pg":
+        x <- clip(x - move dF/dgamma / max|dF/dgamma| + lam, x - move, x + move, 0, 1), lam found by
+        bisection so that the solid fraction mean(1 - x) equals vf.  dF/dgamma serves as the
+        gradient in x (the density filter's chain is not applied)."""
+        g = self.d["dfds"].download()
+        g = g / max(np.abs(g).max(), 1e-300)
+        x = self.x
+        lo_b, hi_b = np.maximum(0.0, x - move), np.minimum(1.0, x + move)
+        trial = lambda lam: np.clip(x - move * g + lam, lo_b, hi_b)
+        lo, hi = -1.0, 1.0
+        for _ in range(60):
+            mid = 0.5 * (lo + hi)
+            if trial(mid).mean() > 1.0 - vf:
+                hi = mid
+            else:
+                lo = mid
+        self.set_design(trial(0.5 * (lo + hi)))
+
+    def keep_solution(self):
+        """Device copies of the converged forward and adjoint fields: the next warm start."""
+        c, d, N = self.ctx, self.d, self.N
+        fb, ab = self.fwd_buf, self.adj_buf
+        src = {"rho": d["rho"], "ux": d["ux%d" % fb], "uy": d["uy%d" % fb], "tem": d["tem"], "ip": d["ip"],
+               "iux": d["iux%d" % ab], "iuy": d["iuy%d" % ab], "item": d["item"], "iqx": d["iqx%d" % ab],
+               "iqy": d["iqy%d" % ab]}
+        if self.kept is None:
+            self.kept = {k: c.array((N,)) for k in src}
+        for k, a in src.items():
+            check(self.L.lbm_device_copy(c.h, self.kept[k].ptr, a.ptr, 8 * N))
+
+    def _copy(self, dst, src):
+        check(self.L.lbm_device_copy(self.ctx.h, dst.ptr, src.ptr, 8 * self.N))
 
     # ---- programs ---------------------------------------------------------------------------
     def _record_forward(self):
@@ -267,14 +322,23 @@ class Heatsink2D:
                     self._set_prec(F64)
         return t, (t32 if self.prec != F64 else 0), r
 
-    def init_forward(self):
-        """Cold start of the forward solve: rest state, T = 0."""
+    def init_forward(self, warm=False):
+        """Cold start of the forward solve (rest state, T = 0), or with warm=True (after
+        keep_solution) the equilibrium of the kept converged fields."""
         L, d = self.L, self.d
         self._set_prec(self.prec)
-        for k in ("ux0", "uy0"):
-            d[k].fill(0.0)
-        d["tem"].fill(0.0)
-        check(L.lbm_ns_initial_condition(self.pf.h, d["one"].ptr, d["ux0"].ptr, d["uy0"].ptr, None))
+        rho = d["one"]
+        if warm and self.kept is not None:
+            k = self.kept
+            self._copy(d["ux0"], k["ux"])
+            self._copy(d["uy0"], k["uy"])
+            self._copy(d["tem"], k["tem"])
+            rho = k["rho"]
+        else:
+            for name in ("ux0", "uy0"):
+                d[name].fill(0.0)
+            d["tem"].fill(0.0)
+        check(L.lbm_ns_initial_condition(self.pf.h, rho.ptr, d["ux0"].ptr, d["uy0"].ptr, None))
         check(L.lbm_ad_initial_condition(self.pg.h, d["tem"].ptr, d["ux0"].ptr, d["uy0"].ptr, None))
 
     def residual_fields(self, adjoint, t):
@@ -285,16 +349,17 @@ class Heatsink2D:
         return ((d["%s%d" % (u[0], b)], d["%s%d" % (u[1], b)], d["%s%d" % (u[0], pb)], d["%s%d" % (u[1], pb)]),
                 (d["%s%d" % (q[0], b)], d["%s%d" % (q[1], b)], d["%s%d" % (q[0], pb)], d["%s%d" % (q[1], pb)]))
 
-    def forward(self):
-        self.init_forward()
+    def forward(self, warm=False):
+        self.init_forward(warm)
         res = lambda t: tuple(self._residual(*f) for f in self.residual_fields(False, t))
         t, self.fwd_steps32, self.fwd_res = self._solve("fwd", self._fwd_step, res)
         self.fwd_steps = t
         self.fwd_buf = t % 2
         return t
 
-    def init_adjoint(self):
-        """Adjoint program for the converged forward velocity, and the adjoint cold start."""
+    def init_adjoint(self, warm=False):
+        """Adjoint program for the converged forward velocity, and the adjoint cold start (or with
+        warm=True, after keep_solution, the equilibrium of the kept adjoint fields)."""
         L, d = self.L, self.d
         self._set_prec(self.prec)
         fb = self.fwd_buf
@@ -306,17 +371,22 @@ class Heatsink2D:
             g = self._graphs.pop("adj", None)
             if g is not None:
                 L.lbm_graph_destroy(g)
-        for k in ("ip", "item"):
-            d[k].fill(0.0)
-        for k in ("iux0", "iuy0", "iqx0", "iqy0"):
-            d[k].fill(0.0)
+        if warm and self.kept is not None:
+            for name, src in (("ip", "ip"), ("item", "item"), ("iux0", "iux"), ("iuy0", "iuy"), ("iqx0", "iqx"),
+                              ("iqy0", "iqy")):
+                self._copy(d[name], self.kept[src])
+        else:
+            for k in ("ip", "item"):
+                d[k].fill(0.0)
+            for k in ("iux0", "iuy0", "iqx0", "iqy0"):
+                d[k].fill(0.0)
         check(L.lbm_ans_initial_condition(self.pf.h, d["ux%d" % fb].ptr, d["uy%d" % fb].ptr, None, d["ip"].ptr,
                                           d["iux0"].ptr, d["iuy0"].ptr, None))
         check(L.lbm_aad_initial_condition(self.pg.h, d["ux%d" % fb].ptr, d["uy%d" % fb].ptr, None, d["item"].ptr,
                                           d["iqx0"].ptr, d["iqy0"].ptr, None))
 
-    def adjoint(self):
-        self.init_adjoint()
+    def adjoint(self, warm=False):
+        self.init_adjoint(warm)
         fb = self.fwd_buf
         res = lambda t: tuple(self._residual(*f) for f in self.residual_fields(True, t))
         t, self.adj_steps32, self.adj_res = self._solve("adj", lambda s: self._adj_step(s, fb), res)
@@ -351,29 +421,42 @@ class Heatsink2D:
         finally:
             self.prec = prec
 
-    def evaluate(self):
-        """One design evaluation; returns wall seconds and step counts.  A solve in reduced
-        precision whose residual turns non-finite is solved again from its cold start in FP64
-        (fp64_redo): a marginally unstable adjoint mode that FP64 round-off seeds too weakly to
-        grow before the solve converges can grow from FP32 round-off."""
+    def evaluate(self, warm=False):
+        """One design evaluation; returns wall seconds and step counts.  warm=True starts both
+        solves from the last converged solution kept (keep_solution, done after every converged
+        evaluation); a warm-started solve whose residual turns non-finite is repeated from the cold
+        start (fallback).  A solve in reduced precision whose residual still turns non-finite is
+        solved again from its cold start in FP64 (fp64_redo): a marginally unstable adjoint mode
+        that FP64 round-off seeds too weakly to grow before the solve converges can grow from FP32
+        round-off."""
         t0 = time.perf_counter()
-        redo = [False, False]
-        self.forward()
+        use = bool(warm) and self.kept is not None
+        redo, fallback = [False, False], [False, False]
+        self.forward(use)
+        if use and not np.all(np.isfinite(self.fwd_res)):
+            fallback[0] = True
+            self.forward()
         if self.prec != F64 and not np.all(np.isfinite(self.fwd_res)):
             redo[0] = True
             self.in_fp64(self.forward)
-        self.adjoint()
+        self.adjoint(use)
+        if use and not np.all(np.isfinite(self.adj_res)):
+            fallback[1] = True
+            self.adjoint()
         if self.prec != F64 and not np.all(np.isfinite(self.adj_res)):
             redo[1] = True
             self.in_fp64(self.adjoint)
         self.sensitivity()
+        res = np.concatenate([np.ravel(self.fwd_res), np.ravel(self.adj_res)])
+        converged = bool(np.all(np.isfinite(res)) and res.max() < self.eps)
+        if converged:
+            self.keep_solution()
         self.ctx.sync()
         dt = time.perf_counter() - t0
-        diverged = not (np.all(np.isfinite(self.fwd_res)) and np.all(np.isfinite(self.adj_res)))
         return {"seconds": dt, "fwd_steps": self.fwd_steps, "adj_steps": self.adj_steps,
                 "fp32_steps": (self.fwd_steps32, self.adj_steps32), "objective": self.objective,
-                "fwd_residual": self.fwd_res, "adj_residual": self.adj_res, "diverged": diverged,
-                "fp64_redo": redo}
+                "fwd_residual": self.fwd_res, "adj_residual": self.adj_res, "diverged": not np.all(np.isfinite(res)),
+                "converged": converged, "warm": use, "fallback": fallback, "fp64_redo": redo}
 
     def close(self):
         for g in self._graphs.values():

@@ -178,8 +178,8 @@ def test_fp32_divergence_is_solved_again_in_fp64(ctx):
     hs = Heatsink2D(ctx, prec=pa.F32, **kw)
     orig = hs.init_adjoint
 
-    def planted():
-        orig()
+    def planted(warm=False):
+        orig(warm)
         if hs.pf.prec != pa.F64:
             f = hs.pf.download()
             f[2, hs.N // 3] = np.nan

@@ -47,3 +47,35 @@ def test_heatsink_driver_matches_oracle(ctx, kind, graphs):
     assert np.max(np.abs(dfds - wd)) <= 1e-12 * np.max(np.abs(wd))
     np.testing.assert_allclose(r["fwd_residual"], want["fwd_res"], rtol=1e-10)
     hs.close()
+
+
+def test_set_design_in_place_and_warm_restart(ctx):
+    """Heatsink2D.set_design rewrites the coefficient arrays in place: a driver whose graphs were
+    captured on design A, given design B, evaluates B exactly as a fresh eager driver given B.  A
+    warm-started evaluation (from the converged solution kept by the previous one) converges in
+    fewer steps to the same objective, to the spread the 1e-6 residual leaves."""
+    import panslbm2_amd as pa
+    from panslbm2_amd.heatsink2d import Heatsink2D
+
+    print("heatsink 2d set_design + warm", flush=True)
+    kw = dict(lx=40, ly=24, ad_kind=pa.D2Q5, seed=3, filter_radius=2.0)
+    a = Heatsink2D(ctx, use_graphs=True, **kw)
+    a.set_design(np.full(a.N, 0.6))  # solid fraction 0.4
+    a.evaluate()
+    a.design_update(vf=0.4, move=0.05)  # a projected-gradient step keeps the solid fraction at vf
+    assert abs((1.0 - a.x).mean() - 0.4) < 1e-6
+    xb = a.x.copy()
+    ra = a.evaluate()  # design B, coefficients rewritten under the graphs captured on design A
+    b = Heatsink2D(ctx, use_graphs=False, **kw)
+    b.set_design(xb)
+    rb = b.evaluate()
+    assert ra["fwd_steps"] == rb["fwd_steps"] and ra["adj_steps"] == rb["adj_steps"]
+    assert ra["objective"] == rb["objective"]
+    np.testing.assert_array_equal(a.d["dfds"].download(), b.d["dfds"].download())
+    rw = a.evaluate(warm=True)
+    print("cold", ra["fwd_steps"], ra["adj_steps"], "warm", rw["fwd_steps"], rw["adj_steps"], flush=True)
+    assert rw["warm"] and rw["converged"]
+    assert rw["fwd_steps"] < ra["fwd_steps"] and rw["adj_steps"] < ra["adj_steps"]
+    assert abs(rw["objective"] - ra["objective"]) <= 1e-3 * abs(ra["objective"])
+    a.close()
+    b.close()
