@@ -70,8 +70,8 @@ BYTES_ADJ_Q15Q7 = 2 * (15 + 7) * 8 + 2 * 8 + 5 * 8
 # cache-sized step includes Infinity Cache hits.
 TRAFFIC_FWD_200x400 = round((5767.5 * 1.920 + 8920.5) * 1024)
 # The same for the fused 3D forward and adjoint steps at 128x128x64 (profiles/k_collide_fused_3d_128x128x64.txt)
-TRAFFIC_FWD_128x128x64 = round((106812.7 * 1.920 + 182828.8) * 1024)
-TRAFFIC_ADJ_128x128x64 = round((119837.8 * 1.920 + 180224.0) * 1024)
+TRAFFIC_FWD_128x128x64 = round((108690.5 * 1.920 + 182828.8) * 1024)
+TRAFFIC_ADJ_128x128x64 = round((120205.3 * 1.920 + 180224.0) * 1024)
 
 
 def kernel_rate(hs, steps=100, reps=5):
