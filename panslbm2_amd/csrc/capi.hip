@@ -548,6 +548,13 @@ int collide_call(lbm_program *P, lbm_lattice *p, lbm_lattice *q, std::initialize
     cl.ea = p->enc();
     cl.eb = q ? q->enc() : lbm::Enc{};
     cl.g = p->grid();
+    {
+        // non-temporal pulls (3D) where the step's populations, read and written, exceed what the
+        // 256 MB Infinity Cache keeps: measured faster from 184 MB (FP32, 128x128x64) up, slower
+        // at 156 MB (FP64, 96x96x48)
+        const size_t bytes = 2 * p->esize() * (size_t)p->N * (size_t)(p->nc + (q ? q->nc : 0));
+        cl.g.nt = (p->nd == 3 && bytes > ((size_t)160 << 20)) ? 1 : 0;
+    }
     cl.stream = C->stream;
     bool fused = false;
     if (P && P->nlat == 0) P = nullptr;

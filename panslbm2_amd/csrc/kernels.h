@@ -217,23 +217,20 @@ __device__ __forceinline__ S enc(double v, const Enc &e) {
     else return S(v - shift_c<L, C>(e));
 }
 
-// One pulled value.  3D lattices load it non-temporal: a step reads each source population once,
-// and streaming them keeps the caches for the populations the step writes, which the next step
-// reads (128x128x64 duct step 98 -> 85 us).  2D lattices reuse a source row from L2 for the rows
-// beside it, which the hint would give up (4096^2 step 773 -> 837 us), so they load normally.
-template <class L, class S>
+// One pulled value, loaded non-temporal with NT (see node_state).
+template <bool NT, class S>
 __device__ __forceinline__ S pull_load(const S *p) {
-    if constexpr (L::nd == 3) return __builtin_nontemporal_load(p);
+    if constexpr (NT) return __builtin_nontemporal_load(p);
     else return *p;
 }
 
 // Pull node n's populations: pull = +1 from x - c (Stream), -1 from x + c (iStream), 0 in place.
 // S: storage type; values are decoded with the source node's shift.
-template <class L, class S, class R>
+template <class L, bool NT = false, class S, class R>
 __device__ __forceinline__ void load_pull(const S *__restrict__ src, const Grid &g, int i, int j, int k, int n,
                                           int pull, R *f, const Enc &e = Enc{}) {
     if constexpr (L::nc > 0) {
-        f[0] = R(dec<L, 0>(pull_load<L>(&src[n]), e));
+        f[0] = R(dec<L, 0>(pull_load<NT>(&src[n]), e));
         sfor<1, L::nc>([&](auto c) {
             constexpr int C = decltype(c)::value;
             int s = n;
@@ -243,7 +240,7 @@ __device__ __forceinline__ void load_pull(const S *__restrict__ src, const Grid 
                 int kk = L::nd == 3 ? wrap(k, -pull * L::cz[C], g.nz) : 0;
                 s = ii + g.nx * (jj + g.ny * kk);
             }
-            f[C] = R(dec<L, C>(pull_load<L>(&src[(size_t)C * g.N + s]), e));
+            f[C] = R(dec<L, C>(pull_load<NT>(&src[(size_t)C * g.N + s]), e));
         });
     }
 }
@@ -344,6 +341,24 @@ __device__ __forceinline__ int nboundary(int i, int j, int k, const Grid &g, int
     return c;
 }
 
+// Both lattices' pulls of node n.  With g.nt (3D launches whose populations do not fit the
+// Infinity Cache, set by the host) the loads are non-temporal: a step reads each source population
+// once, and streaming them keeps the caches for the populations the step writes, which the next
+// step reads (128x128x64 duct step 98 -> 82 us).  Where the step fits the cache (96x96x48: 3.44 ->
+// 3.58 s per design iteration with the hint) or reuses a source row from L2 for the rows beside it
+// (2D, 4096^2 step 773 -> 837 us with the hint) the loads stay normal.
+template <class LA, class LB, class S, class R>
+__device__ __forceinline__ void pull_pair(const S *sa, const S *sb, const Grid &g, int i, int j, int k, int n,
+                                          const ProgView &pv, R *a, R *b, const Enc &, const Enc &) {
+    if (LA::nd == 3 && g.nt) {
+        load_pull<LA, true>(sa, g, i, j, k, n, pv.pull[0], a);
+        load_pull<LB, true>(sb, g, i, j, k, n, pv.pull[1], b);
+    } else {
+        load_pull<LA>(sa, g, i, j, k, n, pv.pull[0], a);
+        load_pull<LB>(sb, g, i, j, k, n, pv.pull[1], b);
+    }
+}
+
 // Post-stream, post-boundary state of node n; SMOOTH operators are skipped (callers that own
 // smoothed nodes apply them).  FP64: the populations.  FP32: their deviations from node n's shift
 // (oa, ob); the boundary operators run on FP64 populations.
@@ -353,8 +368,7 @@ __device__ __forceinline__ void node_state(const double *sa, const double *sb, c
     int i, j, k;
     coords(n, g, i, j, k);
     const int li = list_of_node(pv, g, i, j, k, LA::nd);
-    load_pull<LA>(sa, g, i, j, k, n, pv.pull[0], a);
-    load_pull<LB>(sb, g, i, j, k, n, pv.pull[1], b);
+    pull_pair<LA, LB>(sa, sb, g, i, j, k, n, pv, a, b, Enc{}, Enc{});
     run_list_ops<LA, LB>(pv, li, n, a, b);
 }
 template <class LA, class LB, class S, std::enable_if_t<!std::is_same_v<S, double>, int> = 0>
@@ -364,8 +378,7 @@ __device__ __forceinline__ void node_state(const S *sa, const S *sb, const Grid 
     int i, j, k;
     coords(n, g, i, j, k);
     const int li = list_of_node(pv, g, i, j, k, LA::nd);
-    load_pull<LA>(sa, g, i, j, k, n, pv.pull[0], a);
-    load_pull<LB>(sb, g, i, j, k, n, pv.pull[1], b);
+    pull_pair<LA, LB>(sa, sb, g, i, j, k, n, pv, a, b, Enc{}, Enc{});
     if (li >= 0) {
         double xa[LA::nc], xb[LB::nc > 0 ? LB::nc : 1];
         dev_to_f64<LA>(a, xa, ea, oa);

@@ -130,6 +130,7 @@ __device__ __forceinline__ R cmul(R v) {
 // ---- grid geometry ------------------------------------------------------------------------
 struct Grid {
     int nx, ny, nz, N;
+    int nt;  // 1: the fused kernels load the pulled populations non-temporal (set per launch)
 };
 
 // Periodic neighbour of (i, j, k) shifted by (dx, dy, dz) in {-1, 0, 1}: d2q9.h:80-84.
