@@ -41,13 +41,14 @@ VP = ctypes.c_void_p
 STALL_CEILING = 1e-4
 
 
-def switch_ready(worst, floor, stall):
+def switch_ready(worst, floor, stall, ceiling=STALL_CEILING):
     """Whether a reduced-precision solve converts to FP64: its residual is below the switch floor,
-    or has stalled (no 5 % drop) for ten checks below 100 floor or twenty below STALL_CEILING."""
-    return worst < floor or (stall >= 10 and worst < 100.0 * floor) or (stall >= 20 and worst < STALL_CEILING)
+    or has stalled (no 5 % drop) for ten checks below 100 floor or twenty below ceiling."""
+    return worst < floor or (stall >= 10 and worst < 100.0 * floor) or (stall >= 20 and worst < ceiling)
 
 
 class Heatsink2D:
+    stall_ceiling = STALL_CEILING  # see switch_ready
     def __init__(self, ctx=None, lx=200, ly=400, ad_kind=D2Q5, seed=0, nu=0.1, pr=6.0, umax=0.05, heat=1e-3,
                  alpha_max=1e4, qf=0.01, qg=1.0, filter_radius=3.0, eps=1e-6, check_every=100, max_steps=400000,
                  use_graphs=True, prec=F64, fp32_floor=None):
@@ -318,7 +319,7 @@ class Heatsink2D:
                 t32 = t
                 stall = 0 if worst < 0.95 * best else stall + 1
                 best = min(best, worst)
-                if switch_ready(worst, self.fp32_floor, stall):
+                if switch_ready(worst, self.fp32_floor, stall, self.stall_ceiling):
                     self._set_prec(F64)
         return t, (t32 if self.prec != F64 else 0), r
 
