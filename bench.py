@@ -315,7 +315,7 @@ def batch_extra(ctx, n, seed=2024):
             "mlups": round(rb["node_steps"] / rb["seconds"] / 1e6, 1),
             "sequential_fp64_mlups": round(seq_updates / seq_s / 1e6, 1),
             "steps": [rb["fwd_steps"], rb["adj_steps"]], "steps_in_fp32": list(rb["fp32_steps"]),
-            "workgroup_and_xcd_remap": list(rb["tuned"]) if rb["tuned"] else None,
+            "workgroup_rows_xcd_remap": list(rb["tuned"]) if rb["tuned"] else None,
             "max_objective_rel_err_vs_fp64": err,
             "diverged_fp64_sequential": int(sum(div)), "diverged_batched": int(sum(bdiv)),
             "fp64_redo_batched": [len(x) for x in rb["fp64_redo"]],

@@ -396,6 +396,15 @@ int lbm_batch_configure(lbm_batch *batch, int block, int xcd);
  * the fastest on all of them.  The timed rounds are real steps: the cycle advances 2 * 3 * reps + 6
  * rounds. */
 int lbm_batch_tune(lbm_batch *const *batches, int nbatch, int reps, int *block_out, int *xcd_out);
+/* tile shapes: a workgroup of `block` lanes takes block / rows columns by `rows` rows of one (x, y)
+ * plane (rows 1, 2, 4 or 8, at least 16 columns; rows = 1 is lbm_batch_configure's run of
+ * consecutive nodes), so that a tile's pulls from the rows beside it stay within the tile */
+int lbm_batch_configure_tiles(lbm_batch *batch, int block, int rows, int xcd);
+/* autotuner over workgroup size, tile rows and the XCD remap (22 configurations), as
+ * lbm_batch_tune; *rounds_out = the rounds of the batches it launched (real steps: each
+ * configuration launches reps + 1 rounds) */
+int lbm_batch_tune_tiles(lbm_batch *const *batches, int nbatch, int reps, int *block_out, int *rows_out,
+                         int *xcd_out, int *rounds_out);
 int lbm_batch_destroy(lbm_batch *batch);
 /* Residual (residual.h:24-51) of n array sets at once: a[3*i + k], b[3*i + k] for k < ncomp */
 int lbm_residual_batch(lbm_context *ctx, int n, int ncomp, const double *const *a, const double *const *b,

@@ -88,6 +88,9 @@ def load():
         "lbm_batch_set_active": [VP, VP],
         "lbm_batch_configure": [VP, I, I],
         "lbm_batch_tune": [ctypes.POINTER(VP), I, I, ctypes.POINTER(I), ctypes.POINTER(I)],
+        "lbm_batch_configure_tiles": [VP, I, I, I],
+        "lbm_batch_tune_tiles": [ctypes.POINTER(VP), I, I, ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(I),
+                                 ctypes.POINTER(I)],
         "lbm_batch_destroy": [VP],
         "lbm_residual_batch": [VP, I, I, ctypes.POINTER(VP), ctypes.POINTER(VP), ctypes.POINTER(I),
                                ctypes.POINTER(D)],

@@ -108,6 +108,7 @@ struct lbm_batch {
     lbm::BatchSink sink;
     DevBuf d_items, d_tiles, d_active;
     int block = 256, xcd = 0, total = 0;  // workgroup size, XCD remap, workgroups in the launch
+    int rows = 1;                         // tile rows (fused_block): 1 = runs of consecutive nodes
 };
 static lbm::BatchSink *batch_sink(lbm_context *C) { return C->batching ? &C->batching->sink : nullptr; }
 
@@ -1675,7 +1676,14 @@ static int batch_retile(lbm_batch *B) {
     std::vector<int2> t(n);
     int start = 0;
     for (int i = 0; i < n; ++i) {
-        int nb = (k.nodes[i] + B->block - 1) / B->block + k.corner[i];
+        int nb;
+        if (B->rows > 1) {  // tiles of (block / rows) x rows nodes per (x, y) plane, as fused_block maps them
+            const int4 d = k.dims[i];
+            const int tw = B->block / B->rows;
+            nb = ((d.x + tw - 1) / tw) * ((d.y + B->rows - 1) / B->rows) * d.z + k.corner[i];
+        } else {
+            nb = (k.nodes[i] + B->block - 1) / B->block + k.corner[i];
+        }
         t[i] = make_int2(start, nb);
         start += (nb + 7) & ~7;  // instances start on multiples of 8 workgroups (one per XCD)
     }
@@ -1723,7 +1731,7 @@ int lbm_batch_launch(lbm_batch *B) {
     if (!B) return fail(LBM_ERR_ARG, "null batch");
     if (B->ctx->recording || B->ctx->batching) return fail(LBM_ERR_STATE, "batch launched while recording");
     return hip_result(B->sink.launch(B->d_items.p, (const int2 *)B->d_tiles.p, (const int *)B->d_active.p,
-                                     (int)B->sink.nodes.size(), B->total, B->block, B->xcd, B->ctx->stream),
+                                     (int)B->sink.nodes.size(), B->total, B->block, B->xcd, B->rows, B->ctx->stream),
                       "batch launch");
 }
 int lbm_batch_set_active(lbm_batch *B, const int *active) {
@@ -1734,13 +1742,16 @@ int lbm_batch_set_active(lbm_batch *B, const int *active) {
     HIPCHK(hipMemcpy(B->d_active.p, active, sizeof(int) * n, hipMemcpyHostToDevice));
     return LBM_OK;
 }
-int lbm_batch_configure(lbm_batch *B, int block, int xcd) {
-    if (!B || (block != 64 && block != 128 && block != 256) || (xcd != 0 && xcd != 1))
+int lbm_batch_configure_tiles(lbm_batch *B, int block, int rows, int xcd) {
+    if (!B || (block != 64 && block != 128 && block != 256) || (xcd != 0 && xcd != 1) ||
+        (rows != 1 && rows != 2 && rows != 4 && rows != 8) || block / rows < 16)
         return fail(LBM_ERR_ARG, "bad batch configuration");
     B->block = block;
+    B->rows = rows;
     B->xcd = xcd;
     return batch_retile(B);
 }
+int lbm_batch_configure(lbm_batch *B, int block, int xcd) { return lbm_batch_configure_tiles(B, block, 1, xcd); }
 int lbm_batch_tune(lbm_batch *const *batches, int nbatch, int reps, int *block_out, int *xcd_out) {
     if (!batches || nbatch < 1 || reps < 1) return fail(LBM_ERR_ARG, "bad arguments");
     lbm_context *C = batches[0]->ctx;
@@ -1779,6 +1790,53 @@ int lbm_batch_tune(lbm_batch *const *batches, int nbatch, int reps, int *block_o
     }
     if (block_out) *block_out = bb;
     if (xcd_out) *xcd_out = bx;
+    return LBM_OK;
+}
+int lbm_batch_tune_tiles(lbm_batch *const *batches, int nbatch, int reps, int *block_out, int *rows_out,
+                         int *xcd_out, int *rounds_out) {
+    if (!batches || nbatch < 1 || reps < 1) return fail(LBM_ERR_ARG, "bad arguments");
+    lbm_context *C = batches[0]->ctx;
+    const int blocks[3] = {64, 128, 256}, rowset[4] = {1, 2, 4, 8};
+    float best = 0.0f;
+    int bb = 256, br = 1, bx = 0, rounds = 0;
+    for (int xcd = 0; xcd <= 1; ++xcd)
+        for (int blk : blocks)
+            for (int rows : rowset) {
+                if (blk / rows < 16) continue;
+                for (int i = 0; i < nbatch; ++i) {
+                    int rc = lbm_batch_configure_tiles(batches[i], blk, rows, xcd);
+                    if (rc) return rc;
+                }
+                for (int i = 0; i < nbatch; ++i) {  // warm-up round
+                    int rc = lbm_batch_launch(batches[i]);
+                    if (rc) return rc;
+                }
+                HIPCHK(hipEventRecord(C->ev[6], C->stream));
+                for (int r = 0; r < reps; ++r)
+                    for (int i = 0; i < nbatch; ++i) {
+                        int rc = lbm_batch_launch(batches[i]);
+                        if (rc) return rc;
+                    }
+                HIPCHK(hipEventRecord(C->ev[7], C->stream));
+                HIPCHK(hipEventSynchronize(C->ev[7]));
+                rounds += reps + 1;
+                float ms = 0.0f;
+                HIPCHK(hipEventElapsedTime(&ms, C->ev[6], C->ev[7]));
+                if (best == 0.0f || ms < best) {
+                    best = ms;
+                    bb = blk;
+                    br = rows;
+                    bx = xcd;
+                }
+            }
+    for (int i = 0; i < nbatch; ++i) {
+        int rc = lbm_batch_configure_tiles(batches[i], bb, br, bx);
+        if (rc) return rc;
+    }
+    if (block_out) *block_out = bb;
+    if (rows_out) *rows_out = br;
+    if (xcd_out) *xcd_out = bx;
+    if (rounds_out) *rounds_out = rounds;
     return LBM_OK;
 }
 int lbm_batch_destroy(lbm_batch *B) {

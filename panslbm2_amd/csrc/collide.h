@@ -533,10 +533,12 @@ __global__ void __launch_bounds__(kBlock) k_collide_inplace(S *a, S *b, Grid g, 
 
 // One workgroup's share of a fused step: workgroup blk of the nblk that cover the grid (the last
 // is the corner workgroup when pv.corner_block); xcd: remap workgroups to XCD-contiguous tiles.
+// rows = 1: a tile is blockDim consecutive nodes; rows > 1: blockDim / rows columns by `rows` rows
+// of one (x, y) plane, so that a tile's pulls from the rows beside it stay within the tile.
 template <class LA, class LB, class Body, class S>
 __device__ __forceinline__ void fused_block(const S *__restrict__ sa, const S *__restrict__ sb, S *__restrict__ da,
                                             S *__restrict__ db, const Grid &g, const ProgView &pv, int any_smooth,
-                                            const Body &body, int blk, int nblk, int xcd) {
+                                            const Body &body, int blk, int nblk, int xcd, int rows = 1) {
     using R = RegOf<S>;
     constexpr int NB = LB::nc > 0 ? LB::nc : 1;
     // the corner workgroup (last in the grid when pv.corner_block): lanes [0, nside) build the
@@ -549,8 +551,16 @@ __device__ __forceinline__ void fused_block(const S *__restrict__ sa, const S *_
         nb = 2;
     } else {
         int tile = xcd ? xcd_tile(blk, nblk - pv.corner_block) : blk;
-        n = tile * blockDim.x + threadIdx.x;
-        if (n >= g.N) return;
+        if (rows > 1) {
+            const int tw = (int)blockDim.x / rows, ntx = (g.nx + tw - 1) / tw, nty = (g.ny + rows - 1) / rows;
+            const int z = tile / (ntx * nty), r = tile - z * (ntx * nty);
+            const int x = (r % ntx) * tw + (int)threadIdx.x % tw, y = (r / ntx) * rows + (int)threadIdx.x / tw;
+            if (x >= g.nx || y >= g.ny || z >= g.nz) return;
+            n = x + g.nx * (y + g.ny * z);
+        } else {
+            n = tile * blockDim.x + threadIdx.x;
+            if (n >= g.N) return;
+        }
         int i, j, k;
         coords(n, g, i, j, k);
         nb = nboundary(i, j, k, g, LA::nd);
@@ -614,7 +624,8 @@ struct BatchItem {
 template <class LA, class LB, class Body, class S>
 __global__ void __launch_bounds__(kBlock) k_collide_batch(const BatchItem<Body, S> *__restrict__ items,
                                                           const int2 *__restrict__ tiles,
-                                                          const int *__restrict__ active, int nitem, int xcd) {
+                                                          const int *__restrict__ active, int nitem, int xcd,
+                                                          int rows) {
     const int b = blockIdx.x;
     int lo = 0, hi = nitem - 1;  // last instance whose first workgroup is <= b
     while (lo < hi) {
@@ -626,13 +637,14 @@ __global__ void __launch_bounds__(kBlock) k_collide_batch(const BatchItem<Body, 
     const int blk = b - t.x;
     if (blk >= t.y || !active[lo]) return;  // alignment padding between instances, or frozen
     const BatchItem<Body, S> &it = items[lo];
-    fused_block<LA, LB, Body, S>(it.sa, it.sb, it.da, it.db, it.g, it.pv, it.any_smooth, it.body, blk, t.y, xcd);
+    fused_block<LA, LB, Body, S>(it.sa, it.sb, it.da, it.db, it.g, it.pv, it.any_smooth, it.body, blk, t.y, xcd,
+                                 rows);
 }
 template <class LA, class LB, class Body, class S>
 static hipError_t launch_batch_t(const void *items, const int2 *tiles, const int *active, int nitem, int nblocks_total,
-                                 int block, int xcd, hipStream_t s) {
+                                 int block, int xcd, int rows, hipStream_t s) {
     hipLaunchKernelGGL((k_collide_batch<LA, LB, Body, S>), dim3(nblocks_total), dim3(block), 0, s,
-                       static_cast<const BatchItem<Body, S> *>(items), tiles, active, nitem, xcd);
+                       static_cast<const BatchItem<Body, S> *>(items), tiles, active, nitem, xcd, rows);
     return hipGetLastError();
 }
 
@@ -789,6 +801,7 @@ static hipError_t launch_collide(const CollideLaunch &cl, Make &&make) {
             k.bytes.insert(k.bytes.end(), p, p + sizeof(it));
             k.nodes.push_back(g.N);
             k.corner.push_back(pv.corner_block);
+            k.dims.push_back(make_int4(g.nx, g.ny, g.nz, 0));
             return hipSuccess;
         }
         LBM_LAUNCH((k_collide_fused<LA, LB, B, S>), nblocks(g.N) + pv.corner_block, cl.stream, sa, sb, da, db, g, pv,

@@ -13,10 +13,11 @@ namespace lbm {
 // node count and corner workgroup, and the launcher of the one kernel instantiation they share.
 struct BatchSink {
     using LaunchFn = hipError_t (*)(const void *items, const int2 *tiles, const int *active, int nitem,
-                                    int nblocks_total, int block, int xcd, hipStream_t s);
+                                    int nblocks_total, int block, int xcd, int rows, hipStream_t s);
     std::vector<unsigned char> bytes;
     size_t item_size = 0;
     std::vector<int> nodes, corner;
+    std::vector<int4> dims;  // (nx, ny, nz) of each instance
     LaunchFn launch = nullptr;
 };
 

@@ -114,7 +114,7 @@ class HeatsinkBatch2D:
     def _configure(self, batches):
         if self.tuned is not None:
             for b in batches.values():
-                check(self.L.lbm_batch_configure(b, *self.tuned))
+                check(self.L.lbm_batch_configure_tiles(b, *self.tuned))
 
     def _phase(self, step_fn):
         """Steps until every instance's residuals are below eps; returns (steps, convergence step per
@@ -128,10 +128,13 @@ class HeatsinkBatch2D:
         if self.tune and self.tuned is None:
             quiet = self.hs[0]._save_mode(2, self.adjoint)  # save mode of the steps between checks
             arr = (VP * 2)(batches[(0, quiet)], batches[(1, quiet)])
-            blk, xcd = I(), I()
-            check(self.L.lbm_batch_tune(arr, 2, 3, ctypes.byref(blk), ctypes.byref(xcd)))
-            self.tuned = (blk.value, xcd.value)
-            t += 2 * 6 * (3 + 1)  # the tuner's rounds are real steps (none of them a check step)
+            blk, rows, xcd, rounds = I(), I(), I(), I()
+            check(self.L.lbm_batch_tune_tiles(arr, 2, 3, ctypes.byref(blk), ctypes.byref(rows), ctypes.byref(xcd),
+                                              ctypes.byref(rounds)))
+            self.tuned = (blk.value, rows.value, xcd.value)  # workgroup size, tile rows, XCD remap
+            # the tuner's rounds are real steps, of the kind between checks; the residual checks
+            # resume at the next multiple of check_every after them
+            t += 2 * rounds.value
         else:
             self._configure(batches)
         while t % ce:  # finish the first chunk

@@ -35,13 +35,17 @@ def _state(h):
     return [h.pf.download(), h.pg.download()] + [d[k].download() for k in ("rho", "tem", "ux0", "ux1", "qy0", "qy1")]
 
 
-@pytest.mark.parametrize("prec_name", ["F64", "F32"])
-def test_batched_steps_equal_single_steps(ctx, prec_name):
+@pytest.mark.parametrize("prec_name,tiles", [("F64", (256, 1, 0)), ("F32", (256, 1, 0)), ("F64", (128, 4, 1)),
+                                             ("F64", (64, 2, 0)), ("F32", (256, 8, 1))],
+                         ids=["f64", "f32", "f64-128x4-xcd", "f64-64x2", "f32-256x8-xcd"])
+def test_batched_steps_equal_single_steps(ctx, prec_name, tiles):
+    """Batched steps (workgroup size, tile rows, XCD remap as given) against each instance's own
+    launches: value-identical states across the check steps 99 and 100."""
     import panslbm2_amd as pa
     from panslbm2_amd.heatsink_batch import HeatsinkBatch2D
 
     prec = getattr(pa, prec_name)
-    print("batched vs single steps", prec_name, flush=True)
+    print("batched vs single steps", prec_name, tiles, flush=True)
     single = _instances(ctx, prec)
     hb = HeatsinkBatch2D(ctx, SPECS, prec=prec, use_graphs=False, tune=False, filter_radius=2.0)
     hb.adjoint = False
@@ -49,6 +53,8 @@ def test_batched_steps_equal_single_steps(ctx, prec_name):
         h.init_forward()
         h._fwd_step(1)
     batches = hb._make_batches(lambda h, t: h._fwd_step(t), 2)
+    for b in batches.values():
+        pa.check(ctx.lib.lbm_batch_configure_tiles(b, *tiles))
     for t in range(2, 106):  # crosses the check steps 99 and 100
         hb._launch(batches, t)
         for h in single:
@@ -119,6 +125,40 @@ def test_residual_of_overflowing_field_is_nan(ctx):
     want = _oracle.lib().ol_residual(2, *[v.ctypes.data_as(ctypes.c_void_p) for v in x[:2]], None,
                                      *[v.ctypes.data_as(ctypes.c_void_p) for v in x[2:]], None, m)
     assert abs(r.value - want) <= 1e-12 * want
+
+
+def test_tile_tuner_rounds_are_steps(ctx):
+    """lbm_batch_tune_tiles: every configuration it times is a valid one, the rounds it reports are
+    the steps it ran (the instances advance exactly as many single steps), and it leaves the
+    fastest configuration set."""
+    import panslbm2_amd as pa
+    from panslbm2_amd.heatsink_batch import HeatsinkBatch2D
+
+    print("tile tuner", flush=True)
+    single = _instances(ctx, pa.F64)
+    hb = HeatsinkBatch2D(ctx, SPECS, prec=pa.F64, use_graphs=False, tune=False, filter_radius=2.0)
+    hb.adjoint = False
+    for h in single + hb.hs:
+        h.init_forward()
+        h._fwd_step(1)
+    batches = hb._make_batches(lambda h, t: h._fwd_step(t), 2)
+    mode = hb.hs[0]._save_mode(2, False)
+    arr = (ctypes.c_void_p * 2)(batches[(0, mode)], batches[(1, mode)])
+    blk, rows, xcd, rounds = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    pa.check(ctx.lib.lbm_batch_tune_tiles(arr, 2, 1, ctypes.byref(blk), ctypes.byref(rows), ctypes.byref(xcd),
+                                          ctypes.byref(rounds)))
+    assert blk.value in (64, 128, 256) and rows.value in (1, 2, 4, 8) and blk.value // rows.value >= 16
+    assert xcd.value in (0, 1) and rounds.value == 22 * 2
+    for t in range(2, 2 + 2 * rounds.value):
+        for h in single:
+            h._fwd_step(t)
+    hb._destroy(batches)
+    for a, b in zip(single, hb.hs):
+        for x, y in zip(_state(a), _state(b)):
+            np.testing.assert_array_equal(x, y)
+    for h in single:
+        h.close()
+    hb.close()
 
 
 def test_tuner_rounds_are_steps(ctx):
