@@ -155,8 +155,8 @@ struct lbm_program {
     bool uploaded = false;
     bool dev_used = false;  // device copies exist (a launch may still read them)
     DevBuf d_slot_entry, d_entry_node, d_entry_ops, d_ops, d_sops, d_slot_list;
-    DevBuf d_side_of_slot, d_corner2d, d_side, d_side_nodes;  // 2D corner smoothing
-    int nentry = 0, nslot = 0, maxops = 0, n_corner2d = 0, n_side = 0;
+    DevBuf d_side_of_slot, d_smoothed, d_side, d_side_nodes;  // corner smoothing from side states
+    int nentry = 0, nslot = 0, maxops = 0, n_smoothed = 0, smooth_stride = 1, n_side = 0;
     bool corner_block_ok = false;  // the corners may be done by the fused pass's last workgroup
     bool owned_by_ctx = false;  // eager scratch program
     ~lbm_program() {
@@ -167,7 +167,7 @@ struct lbm_program {
         d_sops.release();
         d_slot_list.release();
         d_side_of_slot.release();
-        d_corner2d.release();
+        d_smoothed.release();
         d_side.release();
         d_side_nodes.release();
     }
@@ -395,25 +395,46 @@ int prog_upload(lbm_program *P) {
         }
         slot_list[s] = it->second;
     }
-    // 2D corners are done from their neighbours' post-boundary states, by the fused pass's last
-    // workgroup or, when the boundary operators read arrays the collide saves, by a corner kernel
-    // after the pass from states the pass stores in a side buffer (3D smoothed programs are not fused)
+    // Smoothed nodes are done from their inward face neighbours' post-boundary states.  2D corners:
+    // by the fused pass's last workgroup or, when the boundary operators read arrays the collide
+    // saves, by a kernel after the pass from states the pass stores in a side buffer.  3D edges and
+    // corners: always by that kernel after the pass (k_collide_smoothed, two int4 per node).
     std::vector<int> side_of_slot, side_nodes;
-    std::vector<int4> corner2d;
+    std::vector<int4> sm_ents;
     P->corner_block_ok = false;
-    if ((P->smoothed[0] || P->smoothed[1]) && L->nd == 2) {
-        std::map<int, int> side;
-        auto side_id = [&](int node) {
-            auto it = side.find(node);
-            if (it != side.end()) return it->second;
-            int id = (int)side.size();
-            side[node] = id;
-            return id;
-        };
+    P->smooth_stride = L->nd == 3 ? 2 : 1;
+    const bool any_smoothed = P->smoothed[0] || P->smoothed[1];
+    std::map<int, int> side;
+    auto side_id = [&](int node) {
+        auto it = side.find(node);
+        if (it != side.end()) return it->second;
+        int id = (int)side.size();
+        side[node] = id;
+        return id;
+    };
+    if (any_smoothed && L->nd == 3) {
+        // edge node -> side states of its two inward face neighbours (axis order)
+        std::map<int, int2> edge_sides;
+        for (const int4 &e : L->sm_edge) {
+            if (nboundary(L, e.y) != 1 || nboundary(L, e.z) != 1)
+                return fail(LBM_ERR_STATE, "a smoothed edge node's neighbour is not a face node");
+            int2 ab = make_int2(side_id(e.y), side_id(e.z));
+            edge_sides[e.x] = ab;
+            sm_ents.push_back(make_int4(e.x, ab.x, ab.y, -1));
+            sm_ents.push_back(make_int4(-1, -1, -1, -1));
+        }
+        for (const int4 &e : L->sm_corner) {  // (corner, x-, y-, z-neighbour): three edge nodes
+            auto ex = edge_sides.find(e.y), ey = edge_sides.find(e.z), ez = edge_sides.find(e.w);
+            if (e.w < 0 || ex == edge_sides.end() || ey == edge_sides.end() || ez == edge_sides.end())
+                return fail(LBM_ERR_STATE, "a smoothed corner's neighbour is not an edge node");
+            sm_ents.push_back(make_int4(e.x, ex->second.x, ex->second.y, -1));
+            sm_ents.push_back(make_int4(ey->second.x, ey->second.y, ez->second.x, ez->second.y));
+        }
+    } else if (any_smoothed && L->nd == 2) {
         const int smoothed_bits = (P->smoothed[0] ? 1 : 0) | (P->smoothed[1] ? 2 : 0);
         for (const int4 &e : L->sm_corner) {
             int a = side_id(e.y), b = side_id(e.z);
-            corner2d.push_back(make_int4(e.x, a, b, smoothed_bits));
+            sm_ents.push_back(make_int4(e.x, a, b, smoothed_bits));
         }
         side_nodes.assign(side.size(), 0);
         for (auto &kv : side) side_nodes[kv.second] = kv.first;
@@ -423,17 +444,21 @@ int prog_upload(lbm_program *P) {
         for (const int4 &e : L->sm_corner)
             for (const HostOp &h : P->ops)
                 if (h.node == e.x && h.op.code == lbm::OP_AAD_ISET_RHO) cross = true;
-        P->corner_block_ok = !cross && (int)side.size() <= lbm::kMaxSide && corner2d.size() + side.size() <= 256;
+        P->corner_block_ok = !cross && (int)side.size() <= lbm::kMaxSide && sm_ents.size() + side.size() <= 256;
+    }
+    if (any_smoothed) {
         side_of_slot.assign(P->nslot, -1);
         for (auto &kv : side) {
             int i, j, k;
             node_coords(L, kv.first, i, j, k);
-            side_of_slot[lbm::boundary_slot(i, j, k, L->lx, L->ly, L->lz, L->nd)] = kv.second;
+            const int slot = lbm::boundary_slot(i, j, k, L->lx, L->ly, L->lz, L->nd);
+            if (slot < 0 || slot >= P->nslot) return fail(LBM_ERR_STATE, "a side-state node is not on the boundary");
+            side_of_slot[slot] = kv.second;
         }
         if (P->d_side.ensure(sizeof(double) * lbm::kSideStride * std::max<size_t>(side.size(), 1)))
             return fail(LBM_ERR_HIP, "device allocation of the side states failed");
     }
-    P->n_corner2d = (int)corner2d.size();
+    P->n_smoothed = (int)sm_ents.size() / P->smooth_stride;
     auto up = [&](DevBuf &b, const void *src, size_t bytes) -> int {
         if (b.ensure(bytes ? bytes : 8)) return fail(LBM_ERR_HIP, "device allocation of %zu bytes failed", bytes);
         if (bytes) HIPCHK(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
@@ -447,7 +472,7 @@ int prog_upload(lbm_program *P) {
     if ((rc = up(P->d_sops, sops.data(), sops.size() * sizeof(Op)))) return rc;
     if ((rc = up(P->d_slot_list, slot_list.data(), slot_list.size() * sizeof(int)))) return rc;
     if ((rc = up(P->d_side_of_slot, side_of_slot.data(), side_of_slot.size() * sizeof(int)))) return rc;
-    if ((rc = up(P->d_corner2d, corner2d.data(), corner2d.size() * sizeof(int4)))) return rc;
+    if ((rc = up(P->d_smoothed, sm_ents.data(), sm_ents.size() * sizeof(int4)))) return rc;
     if ((rc = up(P->d_side_nodes, side_nodes.data(), side_nodes.size() * sizeof(int)))) return rc;
     P->n_side = (int)side_nodes.size();
     P->uploaded = true;
@@ -464,8 +489,8 @@ ProgView prog_view(const lbm_program *P) {
     v.sops = (const Op *)P->d_sops.p;
     v.slot_list = (const int *)P->d_slot_list.p;
     v.maxops = P->maxops;
-    v.side_of_slot = P->n_corner2d > 0 ? (const int *)P->d_side_of_slot.p : nullptr;
-    v.side = P->n_corner2d > 0 ? (double *)P->d_side.p : nullptr;
+    v.side_of_slot = P->n_smoothed > 0 ? (const int *)P->d_side_of_slot.p : nullptr;
+    v.side = P->n_smoothed > 0 ? (double *)P->d_side.p : nullptr;
     v.nentry = P->nentry;
     v.nslot = P->nslot;
     for (size_t k = 0; k < P->fsets.size() && k < (size_t)lbm::kMaxFieldSets; ++k) v.fs[k] = P->fsets[k];
@@ -562,10 +587,10 @@ int collide_call(lbm_program *P, lbm_lattice *p, lbm_lattice *q, std::initialize
     if (P) {
         bool subset = true;
         for (int k = 0; k < P->nlat; ++k) subset &= (P->lat[k] == p || P->lat[k] == q);
-        // SmoothCorner fuses in 2D only (corners from the pass's own neighbour states); a 3D
-        // program with SmoothCorner runs unfused: stream, in-place boundary passes, smoothing, collide.
+        // SmoothCorner fuses with at least 3 nodes per axis: the smoothed nodes' inward neighbours
+        // are then face nodes whose states the pass saves.
         bool smoothed = P->smoothed[0] || P->smoothed[1];
-        bool dims = !smoothed || (p->nd == 2 && p->lx >= 3 && p->ly >= 3);
+        bool dims = !smoothed || (p->lx >= 3 && p->ly >= 3 && (p->nd == 2 || p->lz >= 3));
         fused = subset && P->ordered_ok && dims && P->nlat > 0;
         if (!fused) {
             int rc = prog_apply(P);
@@ -599,9 +624,10 @@ int collide_call(lbm_program *P, lbm_lattice *p, lbm_lattice *q, std::initialize
     cl.sb = q ? q->buf[q->cur] : nullptr;
     cl.db = q ? q->buf[q->cur ^ 1] : nullptr;
     cl.pv = &v;
-    cl.corner2d = (const int4 *)P->d_corner2d.p;
-    cl.n_corner2d = P->n_corner2d;
-    if (P->corner_block_ok && P->n_corner2d > 0) {
+    cl.smoothed = (const int4 *)P->d_smoothed.p;
+    cl.n_smoothed = P->n_smoothed;
+    cl.smooth_stride = P->smooth_stride;
+    if (P->corner_block_ok && P->n_smoothed > 0) {
         bool alias = false;
         for (const FieldSet &fs : P->fsets)
             for (const double *r : {fs.rho, fs.ux, fs.uy, fs.uz, fs.tem, fs.diff})
@@ -609,18 +635,18 @@ int collide_call(lbm_program *P, lbm_lattice *p, lbm_lattice *q, std::initialize
         if (!alias) {
             v.corner_block = 1;
             v.nside = P->n_side;
-            v.ncorner = P->n_corner2d;
+            v.ncorner = P->n_smoothed;
             v.side_nodes = (const int *)P->d_side_nodes.p;
-            v.corners = (const int4 *)P->d_corner2d.p;
+            v.corners = (const int4 *)P->d_smoothed.p;
             v.side_of_slot = nullptr;
             v.side = nullptr;
-            cl.n_corner2d = 0;
+            cl.n_smoothed = 0;
         }
     }
     cl.any_smooth = (P->smoothed[0] || P->smoothed[1]) ? 1 : 0;
     if (sink) {
-        if (cl.n_corner2d > 0)
-            return fail(LBM_ERR_UNSUPPORTED, "a batched collide cannot smooth corners from fields it saves");
+        if (cl.n_smoothed > 0)
+            return fail(LBM_ERR_UNSUPPORTED, "a batched collide cannot run the smoothed nodes' kernel after its pass");
         v.xcd = 0;  // the batch decides
         cl.batch = sink;
     }

@@ -511,8 +511,8 @@ struct AADBody<LA, LB, float> {
 // ------------------------------------------------------------------------------------------
 // Collide kernels.  In place: the reference's own collide (no pending stream / boundaries).
 // Fused: pull + boundary program + collide, reading `s*` and writing `d*`.  2D smoothed corners:
-// by the fused grid's last workgroup, or after the pass by k_collide_corner2d from the neighbour
-// states the pass saved.
+// by the fused grid's last workgroup, or after the pass by k_collide_smoothed from the neighbour
+// states the pass saved; 3D smoothed edges and corners: by k_collide_smoothed after the pass.
 // ------------------------------------------------------------------------------------------
 template <class LA, class LB, class Body, class S>
 __global__ void __launch_bounds__(kBlock) k_collide_inplace(S *a, S *b, Grid g, Body body) {
@@ -648,20 +648,32 @@ static hipError_t launch_batch_t(const void *items, const int2 *tiles, const int
     return hipGetLastError();
 }
 
-// 2D smoothed corners after the fused pass: pull, the corner's own operators in record order, a
-// SMOOTH operator averaging the two inward neighbours' post-boundary states that the fused pass
-// saved (d2q9.h:577-587), then the collide.  The corner's fields are untouched by the fused pass,
-// which skips corner nodes.  The operators run on FP64 populations.
+// Smoothed nodes after the fused pass, which skips them: pull, the node's own operators in record
+// order with each SMOOTH operator replaced by a mean of face-node states that the pass saved (side
+// states: post-boundary, FP64), then the collide.  The node's fields are untouched by the pass.
+// Entries are `stride` int4 each:
+//   stride 1 (2D corner, d2q9.h:577-587): (node, side a, side b, -)  -> 0.5 (a + b);
+//   stride 2 (3D, d3q15.h:199-220, 1241-1303), with at least 3 nodes per axis:
+//     edge node:  (node, a, b, -), (-1, ...)       -> 0.5 (a + b), a and b its two inward face
+//                 neighbours;
+//     corner:     (node, a, b, -), (c, d, e, f)    -> (0.5 (a + b) + 0.5 (c + d) + 0.5 (e + f)) / 3,
+//                 the three means being the smoothed states of its inward neighbours along x, y
+//                 and z (edge nodes).  The reference smooths the edge lines in full and then the
+//                 corners, so a corner's final state is the mean of the three smoothed edge nodes
+//                 beside it, and no edge node reads a corner or another edge node.
+// The operators run on FP64 populations.
 template <class LA, class LB, class Body, class S>
-__global__ void __launch_bounds__(kBlock) k_collide_corner2d(const S *__restrict__ sa, const S *__restrict__ sb,
+__global__ void __launch_bounds__(kBlock) k_collide_smoothed(const S *__restrict__ sa, const S *__restrict__ sb,
                                                              S *__restrict__ da, S *__restrict__ db, Grid g,
-                                                             ProgView pv, const int4 *__restrict__ corners, int count,
-                                                             Body body) {
+                                                             ProgView pv, const int4 *__restrict__ ents, int count,
+                                                             int stride, Body body) {
     using R = RegOf<S>;
     constexpr int NB = LB::nc > 0 ? LB::nc : 1;
+    static_assert(LA::nc + NB <= kSideStride, "side state stride too small for this lattice pair");
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= count) return;
-    int4 e = corners[t];
+    const int4 e = ents[(size_t)t * stride];
+    const int4 e2 = stride > 1 ? ents[(size_t)t * stride + 1] : make_int4(-1, -1, -1, -1);
     int n = e.x;
     int i, j, k;
     coords(n, g, i, j, k);
@@ -678,6 +690,17 @@ __global__ void __launch_bounds__(kBlock) k_collide_corner2d(const S *__restrict
         if constexpr (LB::nc > 0) put_state<LB>(xb, fb, body.eb, ob);
     }
     const double *s0 = pv.side + (size_t)e.y * kSideStride, *s1 = pv.side + (size_t)e.z * kSideStride;
+    const bool corner3 = e2.x >= 0;
+    const double *s2 = pv.side + (size_t)(corner3 ? e2.x : 0) * kSideStride;
+    const double *s3 = pv.side + (size_t)(corner3 ? e2.y : 0) * kSideStride;
+    const double *s4 = pv.side + (size_t)(corner3 ? e2.z : 0) * kSideStride;
+    const double *s5 = pv.side + (size_t)(corner3 ? e2.w : 0) * kSideStride;
+    // the smoothed value of component o of the side states (o: lattice offset + population)
+    auto mean = [&](int o) -> double {
+        const double m = 0.5 * (s0[o] + s1[o]);
+        if (!corner3) return m;
+        return (m + 0.5 * (s2[o] + s3[o]) + 0.5 * (s4[o] + s5[o])) / 3.0;
+    };
     int slot = boundary_slot(i, j, k, g.nx, g.ny, g.nz, LA::nd);
     const int li = pv.slot_list[slot];
     const Op *p = pv.sops + (size_t)(li < 0 ? 0 : li) * pv.maxops;
@@ -693,12 +716,12 @@ __global__ void __launch_bounds__(kBlock) k_collide_corner2d(const S *__restrict
             } else if ((op.lat ^ pv.latswap) == 0) {
                 sfor<0, LA::nc>([&](auto c) {
                     constexpr int C = decltype(c)::value;
-                    xa[C] = 0.5 * (s0[C] + s1[C]);
+                    xa[C] = mean(C);
                 });
             } else if constexpr (LB::nc > 0) {
                 sfor<0, LB::nc>([&](auto c) {
                     constexpr int C = decltype(c)::value;
-                    xb[C] = 0.5 * (s0[LA::nc + C] + s1[LA::nc + C]);
+                    xb[C] = mean(LA::nc + C);
                 });
             }
         });
@@ -790,7 +813,7 @@ static hipError_t launch_collide(const CollideLaunch &cl, Make &&make) {
         const S *sa = static_cast<const S *>(cl.sa), *sb = static_cast<const S *>(cl.sb);
         const ProgView &pv = *cl.pv;
         if (cl.batch) {  // collect the step instead of launching it
-            if (cl.n_corner2d > 0) return hipErrorNotSupported;
+            if (cl.n_smoothed > 0) return hipErrorNotSupported;
             BatchSink &k = *cl.batch;
             BatchItem<B, S> it{sa, sb, da, db, g, pv, body, cl.any_smooth};
             BatchSink::LaunchFn fn = &launch_batch_t<LA, LB, B, S>;
@@ -806,9 +829,9 @@ static hipError_t launch_collide(const CollideLaunch &cl, Make &&make) {
         }
         LBM_LAUNCH((k_collide_fused<LA, LB, B, S>), nblocks(g.N) + pv.corner_block, cl.stream, sa, sb, da, db, g, pv,
                    cl.any_smooth, body);
-        if (cl.n_corner2d > 0) {
-            LBM_LAUNCH((k_collide_corner2d<LA, LB, B, S>), nblocks(cl.n_corner2d), cl.stream, sa, sb, da, db, g, pv,
-                       cl.corner2d, cl.n_corner2d, body);
+        if (cl.n_smoothed > 0) {
+            LBM_LAUNCH((k_collide_smoothed<LA, LB, B, S>), nblocks(cl.n_smoothed), cl.stream, sa, sb, da, db, g, pv,
+                       cl.smoothed, cl.n_smoothed, cl.smooth_stride, body);
         }
         return hipGetLastError();
     };

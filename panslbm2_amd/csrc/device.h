@@ -30,8 +30,9 @@ struct CollideLaunch {
     void *da, *db;             // destinations (fused) or the in-place buffers
     Enc ea, eb;                // FP32 storage shift of each lattice (no shift in FP64)
     const ProgView *pv;        // nullptr: plain in-place collide
-    const int4 *corner2d;      // 2D smoothed corners (node, side index of x-neighbour, of y-neighbour, -1),
-    int n_corner2d;            // done after the fused pass from the side states it saved
+    const int4 *smoothed;      // smoothed nodes done after the fused pass from the side states it saved:
+    int n_smoothed;            // smooth_stride int4 per node (2D corners 1, 3D edges and corners 2;
+    int smooth_stride;         // layout at k_collide_smoothed)
     int any_smooth;
     BatchSink *batch;          // non-null: collect the fused step instead of launching it
 };

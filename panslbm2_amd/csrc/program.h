@@ -66,9 +66,10 @@ struct ProgView {
     int pull[2];            // per kernel lattice: +1 stream (pull from x - c), -1 iStream (x + c), 0 none
     int latswap;            // 1 when the program's lattice 0 is the kernel's second lattice
     int xcd;                // 1: workgroup -> tile remap that keeps consecutive tiles on one XCD
-    // 2D corner smoothing: the fused pass saves the post-boundary state of each corner's inward
-    // neighbours here (kSideStride doubles each: the kernel's first lattice, then its second) and a
-    // corner kernel launched after it averages them.  nullptr when unused.
+    // Corner smoothing after the pass: the fused pass saves the post-boundary state of each
+    // smoothed node's inward face neighbours here (kSideStride doubles each: the kernel's first
+    // lattice, then its second) and a kernel launched after it averages them (2D corners; 3D edges
+    // and corners).  nullptr when unused.
     const int *side_of_slot;  // boundary slot -> side-state index or -1
     double *side;
     // 2D corner smoothing inside the fused pass, used when the boundary operators read no array the
@@ -80,7 +81,7 @@ struct ProgView {
     const int4 *corners;      // (node, state of x-neighbour, state of y-neighbour, smoothed-lattice bits)
 };
 constexpr int kMaxSide = 8;
-constexpr int kSideStride = 32;  // >= nc(first) + nc(second) of any lattice pair
+constexpr int kSideStride = 40;  // >= nc(first) + nc(second) of any lattice pair (D3Q27 + D3Q7: 34)
 
 // Boundary slot of node (i, j, k) on an nx*ny*nz grid, -1 for interior nodes.  Surface nodes are
 // numbered z-faces first, then the y-faces of the remaining layers, then the x-faces.

@@ -12,11 +12,11 @@ set of a duct heat sink:
   * objective: mean temperature on the bottom face z = 0, injected into the adjoint through the flux
     condition with eps = 1 (heatsink.cpp:202); the adjoint outlet is ANS::iBoundaryConditionSetRho3D
     (the reference's 3D AAD::iBoundaryConditionSetRho does not compile).
-By default no SmoothCorner: the corner and edge nodes lie on bounce-back walls, and the fused 3D
-path runs every operator but SmoothCorner, so each time step is one kernel pass.  With
-smooth_corners=True both lattices are smoothed after the boundary calls of every forward and adjoint
-step, as the reference's 3D driver does (heatsink3D.cpp:175-176, 215-216); those steps run the
-exact unfused sequence (stream, boundary passes, smoothing, collide).
+By default no SmoothCorner: the corner and edge nodes lie on bounce-back walls, and each time step
+is one fused kernel pass.  With smooth_corners=True both lattices are smoothed after the boundary
+calls of every forward and adjoint step, as the reference's 3D driver does (heatsink3D.cpp:175-176,
+215-216); the fused pass then skips the edge and corner nodes and saves the post-boundary states of
+their face neighbours, and a second kernel smooths, collides and stores the edges and corners.
 The Navier-Stokes lattice is D3Q15 (the reference's) or D3Q27 (ns_kind=D3Q27, no reference
 counterpart; its velocity and pressure boundaries are non-equilibrium bounce-back).
 """
@@ -263,8 +263,8 @@ class Heatsink3D(Heatsink2D):
         calls()
         prog = VP()
         check(self.L.lbm_record_end(self.ctx.h, ctypes.byref(prog)))
-        # a 3D program with SmoothCorner runs unfused (apply, then collide); any other must fuse
-        assert self.smooth_corners or self.L.lbm_program_fusable(prog) == 1
+        # every program here fuses (with SmoothCorner too: the edges and corners run after the pass)
+        assert self.L.lbm_program_fusable(prog) == 1
         return prog
 
     def _record_forward(self):

@@ -539,12 +539,14 @@ class ForcedConvection3D:
     ANS::iBoundaryConditionSetRho3D (the reference's 3D AAD::iBoundaryConditionSetRho does not
     compile)."""
 
-    def __init__(self, name, lx, ly, lz, nt, nu=0.1, pr=6.0, umax=0.05, qn=1e-2, seed=6, check=None):
+    def __init__(self, name, lx, ly, lz, nt, nu=0.1, pr=6.0, umax=0.05, qn=1e-2, seed=6, check=None,
+                 kinds=(D3Q15, D3Q15)):
         self.name, self.lx, self.ly, self.lz, self.nt = name, lx, ly, lz, nt
         self.nu, self.pr, self.umax, self.qn = nu, pr, umax, qn
         self.L = lx // 2
         self.seed = seed
         self.check = check
+        self.kinds = kinds  # (NS lattice, AD lattice); the reference's is (D3Q15, D3Q15)
 
     def setup(self):
         lx, ly, lz = self.lx, self.ly, self.lz
@@ -579,8 +581,8 @@ class ForcedConvection3D:
         lx, ly, lz = self.lx, self.ly, self.lz
         N = lx * ly * lz
         d = self.setup()
-        pf = b.lattice(D3Q15, lx, ly, lz)
-        pg = b.lattice(D3Q15, lx, ly, lz)
+        pf = b.lattice(self.kinds[0], lx, ly, lz)
+        pg = b.lattice(self.kinds[1], lx, ly, lz)
         for k in ("alpha", "dads", "diff", "dkds"):
             b.put(k, d[k])
         b.put("rho", np.ones(N))
@@ -588,8 +590,8 @@ class ForcedConvection3D:
                   "ip", "iux", "iuy", "iuz", "iuxp", "iuyp", "iuzp", "imx", "imy", "imz",
                   "item", "iqx", "iqy", "iqz", "iqxp", "iqyp", "iqzp", "dfds"):
             b.zeros(k, N)
-        b.zeros("g", (NC[D3Q15], N))
-        b.zeros("ig", (NC[D3Q15], N))
+        b.zeros("g", (NC[self.kinds[1]], N))
+        b.zeros("ig", (NC[self.kinds[1]], N))
         zero = np.zeros(N)
         b.ns_init(pf, "rho", "ux", "uy", "uz")
         b.ad_init(pg, "tem", "ux", "uy", "uz")

@@ -71,3 +71,25 @@ def test_d3q27_fp32_matches_oracle(ctx, case, op):
         assert np.all(np.isfinite(g))
         err = float(np.max(np.abs(g - w))) / max(float(np.max(np.abs(w))), 1e-300)
         assert err <= 1e-5, "%s %s %s: relative error %.3g" % (case.name, op, k, err)
+
+
+@pytest.mark.parametrize("kinds", [("D3Q27", "D3Q7"), ("D3Q15", "D3Q7")], ids=["q27-q7", "q15-q7"])
+def test_fused_smoothed_sequence_matches_eager(ctx, kinds):
+    """The 3D forced-convection sequence with SmoothCorner on both lattices, forward and adjoint
+    (tests/sequences.py ForcedConvection3D) on NS + D3Q7 lattice pairs: the fused steps (pass, then
+    the smoothed edges and corners from the face states it saved) value-identical to the eager
+    sequence (stream, boundary passes, smoothing, collide).  D3Q27 + D3Q7 fills 34 of the side
+    state's components."""
+    import _oracle
+    import sequences
+    from gpu_backend import FusedGpuBackend, GpuBackend
+
+    k = tuple(getattr(_oracle, n) for n in kinds)
+    print("fused smoothed sequence", kinds, flush=True)
+    seq = sequences.ForcedConvection3D("fc3d_%s_%s" % kinds, 7, 5, 5, 30, check=15, kinds=k)
+    want = seq.run(GpuBackend(ctx))
+    fb = FusedGpuBackend(ctx)
+    got = seq.run(fb)
+    assert fb.fused_calls > 0
+    for key, w in want.items():
+        np.testing.assert_array_equal(np.asarray(got[key]), np.asarray(w), err_msg=key)

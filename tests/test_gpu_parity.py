@@ -66,9 +66,6 @@ def test_gpu_function_matches_reference(ctx, case, op):
 def test_gpu_sequence_matches_reference(ctx, seq, fused):
     from gpu_backend import FusedGpuBackend, GpuBackend
 
-    if fused and seq.name.startswith("fc3d"):
-        pytest.skip("the fused 3D SmoothCorner path (fix-up kernels) timed out on the GPU and is switched off; "
-                    "3D programs with SmoothCorner now run unfused, which the eager case covers")
     print("sequence", seq.name, "fused" if fused else "eager", flush=True)
     b = FusedGpuBackend(ctx) if fused else GpuBackend(ctx)
     got = seq.run(b)
