@@ -89,6 +89,12 @@ int lbm_version(void);
 int lbm_context_create(int device, lbm_context **out);
 int lbm_context_destroy(lbm_context *ctx);
 int lbm_context_synchronize(lbm_context *ctx);
+/* No reference counterpart.  3D programs with SmoothCorner: finish the smoothed edges and corners
+ * on a second stream concurrently with the fused pass, each lane rebuilding its face neighbours'
+ * post-boundary states (on = 1, the default), or after the pass from the states it saves (0).
+ * Both give the same values; the concurrent form is used only when no field a boundary operator
+ * reads is one the collide writes. */
+int lbm_context_set_concurrent_smoothing(lbm_context *ctx, int on);
 int lbm_device_alloc(lbm_context *ctx, size_t bytes, void **out);
 int lbm_device_free(lbm_context *ctx, void *ptr);
 int lbm_copy_to_device(lbm_context *ctx, void *dst, const void *src, size_t bytes);

@@ -69,6 +69,7 @@ def load():
         "lbm_context_create": [I, ctypes.POINTER(VP)],
         "lbm_context_destroy": [VP],
         "lbm_context_synchronize": [VP],
+        "lbm_context_set_concurrent_smoothing": [VP, I],
         "lbm_device_alloc": [VP, SZ, ctypes.POINTER(VP)],
         "lbm_device_free": [VP, VP],
         "lbm_copy_to_device": [VP, VP, VP, SZ],

@@ -91,6 +91,11 @@ struct lbm_context {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[8] = {};
+    // second stream (default priority) for 3D smoothed nodes run concurrently with the fused pass,
+    // forked from and joined back to `stream` by events
+    hipStream_t side_stream = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    bool concurrent_smoothing = true;  // lbm_context_set_concurrent_smoothing
     lbm_program *recording = nullptr;
     lbm_batch *batching = nullptr;  // between lbm_batch_begin and lbm_batch_end
     double *d_part = nullptr;  // reduction partials
@@ -156,7 +161,8 @@ struct lbm_program {
     bool dev_used = false;  // device copies exist (a launch may still read them)
     DevBuf d_slot_entry, d_entry_node, d_entry_ops, d_ops, d_sops, d_slot_list;
     DevBuf d_side_of_slot, d_smoothed, d_side, d_side_nodes;  // corner smoothing from side states
-    int nentry = 0, nslot = 0, maxops = 0, n_smoothed = 0, smooth_stride = 1, n_side = 0;
+    DevBuf d_self;  // 3D smoothed nodes with face node indices (k_collide_smoothed_self)
+    int nentry = 0, nslot = 0, maxops = 0, n_smoothed = 0, smooth_stride = 1, n_side = 0, n_self = 0;
     bool corner_block_ok = false;  // the corners may be done by the fused pass's last workgroup
     bool owned_by_ctx = false;  // eager scratch program
     ~lbm_program() {
@@ -168,6 +174,7 @@ struct lbm_program {
         d_slot_list.release();
         d_side_of_slot.release();
         d_smoothed.release();
+        d_self.release();
         d_side.release();
         d_side_nodes.release();
     }
@@ -400,7 +407,7 @@ int prog_upload(lbm_program *P) {
     // saves, by a kernel after the pass from states the pass stores in a side buffer.  3D edges and
     // corners: always by that kernel after the pass (k_collide_smoothed, two int4 per node).
     std::vector<int> side_of_slot, side_nodes;
-    std::vector<int4> sm_ents;
+    std::vector<int4> sm_ents, self_ents;
     P->corner_block_ok = false;
     P->smooth_stride = L->nd == 3 ? 2 : 1;
     const bool any_smoothed = P->smoothed[0] || P->smoothed[1];
@@ -413,22 +420,31 @@ int prog_upload(lbm_program *P) {
         return id;
     };
     if (any_smoothed && L->nd == 3) {
-        // edge node -> side states of its two inward face neighbours (axis order)
-        std::map<int, int2> edge_sides;
+        // edge node -> side states of its two inward face neighbours (axis order), and the two
+        // face nodes themselves (for the concurrent form, which rebuilds their states)
+        std::map<int, int2> edge_sides, edge_faces;
         for (const int4 &e : L->sm_edge) {
-            if (nboundary(L, e.y) != 1 || nboundary(L, e.z) != 1)
+            if (e.x < 0 || e.x >= L->N || e.y < 0 || e.y >= L->N || e.z < 0 || e.z >= L->N ||
+                nboundary(L, e.y) != 1 || nboundary(L, e.z) != 1)
                 return fail(LBM_ERR_STATE, "a smoothed edge node's neighbour is not a face node");
             int2 ab = make_int2(side_id(e.y), side_id(e.z));
             edge_sides[e.x] = ab;
+            edge_faces[e.x] = make_int2(e.y, e.z);
             sm_ents.push_back(make_int4(e.x, ab.x, ab.y, -1));
             sm_ents.push_back(make_int4(-1, -1, -1, -1));
+            self_ents.push_back(make_int4(e.x, e.y, e.z, -1));
+            self_ents.push_back(make_int4(-1, -1, -1, -1));
         }
         for (const int4 &e : L->sm_corner) {  // (corner, x-, y-, z-neighbour): three edge nodes
             auto ex = edge_sides.find(e.y), ey = edge_sides.find(e.z), ez = edge_sides.find(e.w);
-            if (e.w < 0 || ex == edge_sides.end() || ey == edge_sides.end() || ez == edge_sides.end())
+            if (e.x < 0 || e.x >= L->N || e.w < 0 || ex == edge_sides.end() || ey == edge_sides.end() ||
+                ez == edge_sides.end())
                 return fail(LBM_ERR_STATE, "a smoothed corner's neighbour is not an edge node");
             sm_ents.push_back(make_int4(e.x, ex->second.x, ex->second.y, -1));
             sm_ents.push_back(make_int4(ey->second.x, ey->second.y, ez->second.x, ez->second.y));
+            const int2 fx = edge_faces[e.y], fy = edge_faces[e.z], fz = edge_faces[e.w];
+            self_ents.push_back(make_int4(e.x, fx.x, fx.y, -1));
+            self_ents.push_back(make_int4(fy.x, fy.y, fz.x, fz.y));
         }
     } else if (any_smoothed && L->nd == 2) {
         const int smoothed_bits = (P->smoothed[0] ? 1 : 0) | (P->smoothed[1] ? 2 : 0);
@@ -473,6 +489,8 @@ int prog_upload(lbm_program *P) {
     if ((rc = up(P->d_slot_list, slot_list.data(), slot_list.size() * sizeof(int)))) return rc;
     if ((rc = up(P->d_side_of_slot, side_of_slot.data(), side_of_slot.size() * sizeof(int)))) return rc;
     if ((rc = up(P->d_smoothed, sm_ents.data(), sm_ents.size() * sizeof(int4)))) return rc;
+    if ((rc = up(P->d_self, self_ents.data(), self_ents.size() * sizeof(int4)))) return rc;
+    P->n_self = (int)self_ents.size() / 2;
     if ((rc = up(P->d_side_nodes, side_nodes.data(), side_nodes.size() * sizeof(int)))) return rc;
     P->n_side = (int)side_nodes.size();
     P->uploaded = true;
@@ -627,11 +645,25 @@ int collide_call(lbm_program *P, lbm_lattice *p, lbm_lattice *q, std::initialize
     cl.smoothed = (const int4 *)P->d_smoothed.p;
     cl.n_smoothed = P->n_smoothed;
     cl.smooth_stride = P->smooth_stride;
-    if (P->corner_block_ok && P->n_smoothed > 0) {
-        bool alias = false;
+    // The smoothed nodes may be done concurrently with the pass (2D: its corner workgroup; 3D: a
+    // kernel on the second stream) only when no field an operator reads is one the collide writes:
+    // both rebuild the neighbours' post-boundary states while the pass collides those neighbours.
+    bool alias = false;
+    if (P->n_smoothed > 0)
         for (const FieldSet &fs : P->fsets)
             for (const double *r : {fs.rho, fs.ux, fs.uy, fs.uz, fs.tem, fs.diff})
                 for (const double *w : saved) alias |= (r != nullptr && r == w);
+    if (P->n_self > 0 && !alias && !sink && P->ctx->concurrent_smoothing) {
+        cl.self_ents = (const int4 *)P->d_self.p;
+        cl.n_self = P->n_self;
+        cl.self_stream = P->ctx->side_stream;
+        cl.ev_fork = P->ctx->ev_fork;
+        cl.ev_join = P->ctx->ev_join;
+        v.side_of_slot = nullptr;
+        v.side = nullptr;
+        cl.n_smoothed = 0;
+    }
+    if (P->corner_block_ok && P->n_smoothed > 0) {
         if (!alias) {
             v.corner_block = 1;
             v.nside = P->n_side;
@@ -734,6 +766,9 @@ int lbm_context_create(int device, lbm_context **out) {
         return fail(LBM_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
     }
     for (auto &ev : C->ev) HIPCHK(hipEventCreate(&ev));
+    HIPCHK(hipStreamCreateWithFlags(&C->side_stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&C->ev_fork, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&C->ev_join, hipEventDisableTiming));
     HIPCHK(hipMalloc(&C->d_part, sizeof(double) * 2 * lbm::reduction_blocks()));
     HIPCHK(hipMalloc(&C->d_out, sizeof(double) * 4));
     HIPCHK(hipHostMalloc(&C->h_out, sizeof(double) * 4, hipHostMallocDefault));
@@ -745,11 +780,20 @@ int lbm_context_create(int device, lbm_context **out) {
     *out = C;
     return LBM_OK;
 }
+int lbm_context_set_concurrent_smoothing(lbm_context *C, int on) {
+    if (!C) return fail(LBM_ERR_ARG, "null context");
+    C->concurrent_smoothing = on != 0;
+    return LBM_OK;
+}
 int lbm_context_destroy(lbm_context *C) {
     if (!C) return LBM_OK;
     (void)hipSetDevice(C->device);
     (void)hipStreamSynchronize(C->stream);
     for (auto &ev : C->ev) (void)hipEventDestroy(ev);
+    if (C->side_stream) (void)hipStreamSynchronize(C->side_stream);
+    if (C->ev_fork) (void)hipEventDestroy(C->ev_fork);
+    if (C->ev_join) (void)hipEventDestroy(C->ev_join);
+    if (C->side_stream) (void)hipStreamDestroy(C->side_stream);
     (void)hipFree(C->d_part);
     (void)hipFree(C->d_out);
     (void)hipHostFree(C->h_out);

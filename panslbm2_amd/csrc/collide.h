@@ -662,19 +662,15 @@ static hipError_t launch_batch_t(const void *items, const int2 *tiles, const int
 //                 corners, so a corner's final state is the mean of the three smoothed edge nodes
 //                 beside it, and no edge node reads a corner or another edge node.
 // The operators run on FP64 populations.
-template <class LA, class LB, class Body, class S>
-__global__ void __launch_bounds__(kBlock) k_collide_smoothed(const S *__restrict__ sa, const S *__restrict__ sb,
-                                                             S *__restrict__ da, S *__restrict__ db, Grid g,
-                                                             ProgView pv, const int4 *__restrict__ ents, int count,
-                                                             int stride, Body body) {
+// The rest of a smoothed node n: its pull, its operators in record order with each SMOOTH operator
+// replaced by the smoothed FP64 state (mean(o), o = component: the kernel's first lattice, then its
+// second; evaluated only where a SMOOTH operator needs it), the collide and the store.
+template <class LA, class LB, class Body, class S, class Mean>
+__device__ __forceinline__ void finish_smoothed(const S *__restrict__ sa, const S *__restrict__ sb,
+                                                S *__restrict__ da, S *__restrict__ db, const Grid &g,
+                                                const ProgView &pv, int n, const Mean &mean, const Body &body) {
     using R = RegOf<S>;
     constexpr int NB = LB::nc > 0 ? LB::nc : 1;
-    static_assert(LA::nc + NB <= kSideStride, "side state stride too small for this lattice pair");
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= count) return;
-    const int4 e = ents[(size_t)t * stride];
-    const int4 e2 = stride > 1 ? ents[(size_t)t * stride + 1] : make_int4(-1, -1, -1, -1);
-    int n = e.x;
     int i, j, k;
     coords(n, g, i, j, k);
     const Sh4 oa = own_shift<S>(body.ea), ob = own_shift<S>(body.eb);
@@ -689,18 +685,6 @@ __global__ void __launch_bounds__(kBlock) k_collide_smoothed(const S *__restrict
         put_state<LA>(xa, fa, body.ea, oa);
         if constexpr (LB::nc > 0) put_state<LB>(xb, fb, body.eb, ob);
     }
-    const double *s0 = pv.side + (size_t)e.y * kSideStride, *s1 = pv.side + (size_t)e.z * kSideStride;
-    const bool corner3 = e2.x >= 0;
-    const double *s2 = pv.side + (size_t)(corner3 ? e2.x : 0) * kSideStride;
-    const double *s3 = pv.side + (size_t)(corner3 ? e2.y : 0) * kSideStride;
-    const double *s4 = pv.side + (size_t)(corner3 ? e2.z : 0) * kSideStride;
-    const double *s5 = pv.side + (size_t)(corner3 ? e2.w : 0) * kSideStride;
-    // the smoothed value of component o of the side states (o: lattice offset + population)
-    auto mean = [&](int o) -> double {
-        const double m = 0.5 * (s0[o] + s1[o]);
-        if (!corner3) return m;
-        return (m + 0.5 * (s2[o] + s3[o]) + 0.5 * (s4[o] + s5[o])) / 3.0;
-    };
     int slot = boundary_slot(i, j, k, g.nx, g.ny, g.nz, LA::nd);
     const int li = pv.slot_list[slot];
     const Op *p = pv.sops + (size_t)(li < 0 ? 0 : li) * pv.maxops;
@@ -731,6 +715,82 @@ __global__ void __launch_bounds__(kBlock) k_collide_smoothed(const S *__restrict
     body(fa, fb, n, true, oa, ob);
     store_raw<LA>(da, g.N, n, fa);
     store_raw<LB>(db, g.N, n, fb);
+}
+
+template <class LA, class LB, class Body, class S>
+__global__ void __launch_bounds__(kBlock) k_collide_smoothed(const S *__restrict__ sa, const S *__restrict__ sb,
+                                                             S *__restrict__ da, S *__restrict__ db, Grid g,
+                                                             ProgView pv, const int4 *__restrict__ ents, int count,
+                                                             int stride, Body body) {
+    constexpr int NB = LB::nc > 0 ? LB::nc : 1;
+    static_assert(LA::nc + NB <= kSideStride, "side state stride too small for this lattice pair");
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const int4 e = ents[(size_t)t * stride];
+    const int4 e2 = stride > 1 ? ents[(size_t)t * stride + 1] : make_int4(-1, -1, -1, -1);
+    const double *s0 = pv.side + (size_t)e.y * kSideStride, *s1 = pv.side + (size_t)e.z * kSideStride;
+    const bool corner3 = e2.x >= 0;
+    const double *s2 = pv.side + (size_t)(corner3 ? e2.x : 0) * kSideStride;
+    const double *s3 = pv.side + (size_t)(corner3 ? e2.y : 0) * kSideStride;
+    const double *s4 = pv.side + (size_t)(corner3 ? e2.z : 0) * kSideStride;
+    const double *s5 = pv.side + (size_t)(corner3 ? e2.w : 0) * kSideStride;
+    auto mean = [&](int o) -> double {  // a 3D corner: the mean of the three edge means
+        const double m = 0.5 * (s0[o] + s1[o]);
+        if (!corner3) return m;
+        return (m + 0.5 * (s2[o] + s3[o]) + 0.5 * (s4[o] + s5[o])) / 3.0;
+    };
+    finish_smoothed<LA, LB, Body, S>(sa, sb, da, db, g, pv, e.x, mean, body);
+}
+
+// 3D smoothed edges and corners without the pass's side states, so that they run concurrently with
+// the pass (on a second stream): each lane rebuilds the post-boundary states of its node's face
+// neighbours itself (the same node_state the pass runs for them), pair by pair.  Entries are two
+// int4 per node with face NODE indices: edge (node, a, b, -), (-1, ...); corner (node, a, b, -),
+// (c, d, e, f), laid out as for k_collide_smoothed.  Used only when no field an operator reads is
+// one the collide writes (the pass may be writing them meanwhile).
+template <class LA, class LB, class Body, class S>
+__global__ void __launch_bounds__(kBlock) k_collide_smoothed_self(const S *__restrict__ sa, const S *__restrict__ sb,
+                                                                  S *__restrict__ da, S *__restrict__ db, Grid g,
+                                                                  ProgView pv, const int4 *__restrict__ ents,
+                                                                  int count, Body body) {
+    using R = RegOf<S>;
+    constexpr int NB = LB::nc > 0 ? LB::nc : 1;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const int4 e = ents[(size_t)t * 2];
+    const int4 e2 = ents[(size_t)t * 2 + 1];
+    const Sh4 oa = own_shift<S>(body.ea), ob = own_shift<S>(body.eb);
+    const int npair = e2.x >= 0 ? 3 : 1;
+    double sm[LA::nc + NB];
+    for (int q = 0; q < npair; ++q) {
+        const int na = q == 0 ? e.y : (q == 1 ? e2.x : e2.z);
+        const int nb = q == 0 ? e.z : (q == 1 ? e2.y : e2.w);
+        double ta[LA::nc + NB], tb[LA::nc + NB];
+        {
+            R fa[LA::nc], fb[NB];
+            node_state<LA, LB>(sa, sb, g, pv, na, fa, fb, body.ea, body.eb, oa, ob);
+            put_state<LA>(ta, fa, body.ea, oa);
+            if constexpr (LB::nc > 0) put_state<LB>(ta + LA::nc, fb, body.eb, ob);
+        }
+        {
+            R fa[LA::nc], fb[NB];
+            node_state<LA, LB>(sa, sb, g, pv, nb, fa, fb, body.ea, body.eb, oa, ob);
+            put_state<LA>(tb, fa, body.ea, oa);
+            if constexpr (LB::nc > 0) put_state<LB>(tb + LA::nc, fb, body.eb, ob);
+        }
+        sfor<0, LA::nc + NB>([&](auto c) {
+            constexpr int C = decltype(c)::value;
+            const double m = 0.5 * (ta[C] + tb[C]);
+            sm[C] = q == 0 ? m : sm[C] + m;
+        });
+    }
+    if (npair == 3) {
+        sfor<0, LA::nc + NB>([&](auto c) {
+            constexpr int C = decltype(c)::value;
+            sm[C] = sm[C] / 3.0;
+        });
+    }
+    finish_smoothed<LA, LB, Body, S>(sa, sb, da, db, g, pv, e.x, [&](int o) { return sm[o]; }, body);
 }
 
 // Dispatch a functor over the supported lattices (single) or (A, B) lattice pairs (coupled).
@@ -827,8 +887,22 @@ static hipError_t launch_collide(const CollideLaunch &cl, Make &&make) {
             k.dims.push_back(make_int4(g.nx, g.ny, g.nz, 0));
             return hipSuccess;
         }
+        if (cl.n_self > 0) {  // 3D smoothed nodes on the second stream, concurrently with the pass
+            hipError_t er;
+            if ((er = hipEventRecord(cl.ev_fork, cl.stream)) != hipSuccess) return er;
+            if ((er = hipStreamWaitEvent(cl.self_stream, cl.ev_fork, 0)) != hipSuccess) return er;
+            LBM_LAUNCH((k_collide_smoothed_self<LA, LB, B, S>), nblocks(cl.n_self), cl.self_stream, sa, sb, da, db, g,
+                       pv, cl.self_ents, cl.n_self, body);
+            if ((er = hipGetLastError()) != hipSuccess) return er;
+            if ((er = hipEventRecord(cl.ev_join, cl.self_stream)) != hipSuccess) return er;
+        }
         LBM_LAUNCH((k_collide_fused<LA, LB, B, S>), nblocks(g.N) + pv.corner_block, cl.stream, sa, sb, da, db, g, pv,
                    cl.any_smooth, body);
+        if (cl.n_self > 0) {
+            hipError_t er = hipGetLastError();
+            if (er != hipSuccess) return er;
+            if ((er = hipStreamWaitEvent(cl.stream, cl.ev_join, 0)) != hipSuccess) return er;
+        }
         if (cl.n_smoothed > 0) {
             LBM_LAUNCH((k_collide_smoothed<LA, LB, B, S>), nblocks(cl.n_smoothed), cl.stream, sa, sb, da, db, g, pv,
                        cl.smoothed, cl.n_smoothed, cl.smooth_stride, body);

@@ -33,6 +33,10 @@ struct CollideLaunch {
     const int4 *smoothed;      // smoothed nodes done after the fused pass from the side states it saved:
     int n_smoothed;            // smooth_stride int4 per node (2D corners 1, 3D edges and corners 2;
     int smooth_stride;         // layout at k_collide_smoothed)
+    const int4 *self_ents;     // 3D smoothed nodes run concurrently with the pass on self_stream
+    int n_self;                // (k_collide_smoothed_self; 0: not used), forked and joined with
+    hipStream_t self_stream;   // ev_fork / ev_join
+    hipEvent_t ev_fork, ev_join;
     int any_smooth;
     BatchSink *batch;          // non-null: collect the fused step instead of launching it
 };

@@ -73,23 +73,30 @@ def test_d3q27_fp32_matches_oracle(ctx, case, op):
         assert err <= 1e-5, "%s %s %s: relative error %.3g" % (case.name, op, k, err)
 
 
+@pytest.mark.parametrize("concurrent", [1, 0], ids=["concurrent", "after_pass"])
 @pytest.mark.parametrize("kinds", [("D3Q27", "D3Q7"), ("D3Q15", "D3Q7")], ids=["q27-q7", "q15-q7"])
-def test_fused_smoothed_sequence_matches_eager(ctx, kinds):
+def test_fused_smoothed_sequence_matches_eager(ctx, kinds, concurrent):
     """The 3D forced-convection sequence with SmoothCorner on both lattices, forward and adjoint
-    (tests/sequences.py ForcedConvection3D) on NS + D3Q7 lattice pairs: the fused steps (pass, then
-    the smoothed edges and corners from the face states it saved) value-identical to the eager
+    (tests/sequences.py ForcedConvection3D) on NS + D3Q7 lattice pairs: the fused steps (the pass,
+    and the smoothed edges and corners either concurrently with it, from face states each lane
+    rebuilds, or after it, from the face states the pass saved) value-identical to the eager
     sequence (stream, boundary passes, smoothing, collide).  D3Q27 + D3Q7 fills 34 of the side
     state's components."""
     import _oracle
+    import panslbm2_amd as pa
     import sequences
     from gpu_backend import FusedGpuBackend, GpuBackend
 
     k = tuple(getattr(_oracle, n) for n in kinds)
-    print("fused smoothed sequence", kinds, flush=True)
+    print("fused smoothed sequence", kinds, "concurrent" if concurrent else "after the pass", flush=True)
     seq = sequences.ForcedConvection3D("fc3d_%s_%s" % kinds, 7, 5, 5, 30, check=15, kinds=k)
     want = seq.run(GpuBackend(ctx))
     fb = FusedGpuBackend(ctx)
-    got = seq.run(fb)
+    pa.check(ctx.lib.lbm_context_set_concurrent_smoothing(ctx.h, concurrent))
+    try:
+        got = seq.run(fb)
+    finally:
+        pa.check(ctx.lib.lbm_context_set_concurrent_smoothing(ctx.h, 1))
     assert fb.fused_calls > 0
     for key, w in want.items():
         np.testing.assert_array_equal(np.asarray(got[key]), np.asarray(w), err_msg=key)

@@ -67,20 +67,29 @@ def test_mixed_precision_evaluation(ctx, ns_kind):
 
 
 @pytest.mark.parametrize("graphs,smooth,rho_in", [(True, False, None), (False, False, None), (True, True, None),
-                                                  (True, False, 1.01)],
-                         ids=["graphs", "nographs", "smoothed", "pressure_inlet"])
+                                                  (False, "after", None), (True, False, 1.01)],
+                         ids=["graphs", "nographs", "smoothed", "smoothed_after_pass", "pressure_inlet"])
 def test_heatsink3d_driver_matches_oracle(ctx, graphs, smooth, rho_in):
     """The 3D heat-sink driver (fused steps, graphs, field rebinding, boundary-only saves) against its
     CPU-oracle replica (tests/heatsink_oracle.py run3d): populations, fields and the objective
-    value-identical, the sensitivity (pow-based in the reference) within 1e-12 relative."""
+    value-identical, the sensitivity (pow-based in the reference) within 1e-12 relative.  Smoothed:
+    the edges and corners concurrently with the pass (second stream, inside the step graphs), and
+    after it from the pass's side states ("after")."""
     import heatsink_oracle
     import panslbm2_amd as pa
     from panslbm2_amd.heatsink3d import Heatsink3D
 
-    print("heatsink3d 14x10x9 vs oracle", "graphs" if graphs else "eager", "smoothed" if smooth else "", flush=True)
+    print("heatsink3d 14x10x9 vs oracle", "graphs" if graphs else "eager", "smoothed %s" % smooth if smooth else "",
+          flush=True)
+    after = smooth == "after"
+    smooth = bool(smooth)
     kw = dict(lx=14, ly=10, lz=9, plate=3, seed=4, max_steps=400, eps=1e-30, smooth_corners=smooth, inlet_rho=rho_in)
-    hs = Heatsink3D(ctx, use_graphs=graphs, **kw)
-    r = hs.evaluate()
+    pa.check(ctx.lib.lbm_context_set_concurrent_smoothing(ctx.h, 0 if after else 1))
+    try:
+        hs = Heatsink3D(ctx, use_graphs=graphs, **kw)
+        r = hs.evaluate()
+    finally:
+        pa.check(ctx.lib.lbm_context_set_concurrent_smoothing(ctx.h, 1))
     want = heatsink_oracle.run3d(ad_kind=pa.D3Q7, **kw)
     assert r["fwd_steps"] == want["fwd_steps"] == 400 and r["adj_steps"] == want["adj_steps"] == 400
     np.testing.assert_array_equal(hs.gamma, want["gamma"])
