@@ -420,20 +420,35 @@ int prog_upload(lbm_program *P) {
         return id;
     };
     if (any_smoothed && L->nd == 3) {
-        // edge node -> side states of its two inward face neighbours (axis order), and the two
-        // face nodes themselves (for the concurrent form, which rebuilds their states)
-        std::map<int, int2> edge_sides, edge_faces;
+        // edge node -> side states of its two inward face neighbours (axis order)
+        std::map<int, int2> edge_sides;
         for (const int4 &e : L->sm_edge) {
             if (e.x < 0 || e.x >= L->N || e.y < 0 || e.y >= L->N || e.z < 0 || e.z >= L->N ||
                 nboundary(L, e.y) != 1 || nboundary(L, e.z) != 1)
                 return fail(LBM_ERR_STATE, "a smoothed edge node's neighbour is not a face node");
             int2 ab = make_int2(side_id(e.y), side_id(e.z));
             edge_sides[e.x] = ab;
-            edge_faces[e.x] = make_int2(e.y, e.z);
             sm_ents.push_back(make_int4(e.x, ab.x, ab.y, -1));
             sm_ents.push_back(make_int4(-1, -1, -1, -1));
-            self_ents.push_back(make_int4(e.x, e.y, e.z, -1));
-            self_ents.push_back(make_int4(-1, -1, -1, -1));
+        }
+        // Concurrent form (k_collide_smoothed_self): the edge nodes beside a corner publish their
+        // smoothed state in an LDS slot of workgroup 0, where the corners read it; they and the
+        // corners come first, the other edge nodes after them.
+        std::map<int, int> edge_slot;
+        for (const int4 &e : L->sm_corner)
+            for (int nb : {e.y, e.z, e.w})
+                if (edge_slot.find(nb) == edge_slot.end()) {
+                    const int id = (int)edge_slot.size();
+                    edge_slot[nb] = id;
+                }
+        if ((int)edge_slot.size() > lbm::kCornerEdgeSlots)
+            return fail(LBM_ERR_STATE, "too many edge nodes beside the corners");
+        std::vector<int4> rest;
+        for (const int4 &e : L->sm_edge) {
+            auto it = edge_slot.find(e.x);
+            std::vector<int4> &dst = it != edge_slot.end() ? self_ents : rest;
+            dst.push_back(make_int4(e.x, e.y, e.z, it != edge_slot.end() ? it->second : -1));
+            dst.push_back(make_int4(-1, -1, -1, -1));
         }
         for (const int4 &e : L->sm_corner) {  // (corner, x-, y-, z-neighbour): three edge nodes
             auto ex = edge_sides.find(e.y), ey = edge_sides.find(e.z), ez = edge_sides.find(e.w);
@@ -442,10 +457,12 @@ int prog_upload(lbm_program *P) {
                 return fail(LBM_ERR_STATE, "a smoothed corner's neighbour is not an edge node");
             sm_ents.push_back(make_int4(e.x, ex->second.x, ex->second.y, -1));
             sm_ents.push_back(make_int4(ey->second.x, ey->second.y, ez->second.x, ez->second.y));
-            const int2 fx = edge_faces[e.y], fy = edge_faces[e.z], fz = edge_faces[e.w];
-            self_ents.push_back(make_int4(e.x, fx.x, fx.y, -1));
-            self_ents.push_back(make_int4(fy.x, fy.y, fz.x, fz.y));
+            self_ents.push_back(make_int4(e.x, -1, -1, -1));
+            self_ents.push_back(make_int4(edge_slot.at(e.y), edge_slot.at(e.z), edge_slot.at(e.w), 1));
         }
+        if (self_ents.size() / 2 > (size_t)lbm::kSmoothedGroup)
+            return fail(LBM_ERR_STATE, "the corners and their edge nodes do not fit one workgroup");
+        self_ents.insert(self_ents.end(), rest.begin(), rest.end());
     } else if (any_smoothed && L->nd == 2) {
         const int smoothed_bits = (P->smoothed[0] ? 1 : 0) | (P->smoothed[1] ? 2 : 0);
         for (const int4 &e : L->sm_corner) {

@@ -743,11 +743,16 @@ __global__ void __launch_bounds__(kBlock) k_collide_smoothed(const S *__restrict
 }
 
 // 3D smoothed edges and corners without the pass's side states, so that they run concurrently with
-// the pass (on a second stream): each lane rebuilds the post-boundary states of its node's face
-// neighbours itself (the same node_state the pass runs for them), pair by pair.  Entries are two
-// int4 per node with face NODE indices: edge (node, a, b, -), (-1, ...); corner (node, a, b, -),
-// (c, d, e, f), laid out as for k_collide_smoothed.  Used only when no field an operator reads is
-// one the collide writes (the pass may be writing them meanwhile).
+// the pass (on a second stream).  Entries are two int4 per node:
+//   edge node: (node, face a, face b, slot), (-1, -1, -1, -1): the lane rebuilds the post-boundary
+//     states of its two inward face neighbours (the same node_state the pass runs for them) and
+//     takes their mean; slot >= 0 (edge nodes beside a corner) also publishes it in LDS slot `slot`;
+//   corner: (node, -1, -1, -1), (slot x, slot y, slot z, 1): after the workgroup barrier the mean,
+//     in x, y, z order, of the three smoothed edge states beside it.
+// The host puts every corner and every edge node beside one in workgroup 0 (at most 8 + 24 lanes),
+// so the longest chain of a lane is three node states.  Used only when no field an operator reads
+// is one the collide writes (the pass may be writing them meanwhile).
+static_assert(kSmoothedGroup == kBlock, "the corner group must fill at most one workgroup");
 template <class LA, class LB, class Body, class S>
 __global__ void __launch_bounds__(kBlock) k_collide_smoothed_self(const S *__restrict__ sa, const S *__restrict__ sb,
                                                                   S *__restrict__ da, S *__restrict__ db, Grid g,
@@ -755,41 +760,51 @@ __global__ void __launch_bounds__(kBlock) k_collide_smoothed_self(const S *__res
                                                                   int count, Body body) {
     using R = RegOf<S>;
     constexpr int NB = LB::nc > 0 ? LB::nc : 1;
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= count) return;
-    const int4 e = ents[(size_t)t * 2];
-    const int4 e2 = ents[(size_t)t * 2 + 1];
+    constexpr int W = LA::nc + NB;
+    __shared__ double edge_state[kCornerEdgeSlots * W];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = t < count;
+    const int4 e = valid ? ents[(size_t)t * 2] : make_int4(-1, -1, -1, -1);
+    const int4 e2 = valid ? ents[(size_t)t * 2 + 1] : make_int4(-1, -1, -1, -1);
+    const bool corner = e2.w == 1;
     const Sh4 oa = own_shift<S>(body.ea), ob = own_shift<S>(body.eb);
-    const int npair = e2.x >= 0 ? 3 : 1;
-    double sm[LA::nc + NB];
-    for (int q = 0; q < npair; ++q) {
-        const int na = q == 0 ? e.y : (q == 1 ? e2.x : e2.z);
-        const int nb = q == 0 ? e.z : (q == 1 ? e2.y : e2.w);
-        double ta[LA::nc + NB], tb[LA::nc + NB];
+    double sm[W];
+    if (valid && !corner) {
+        double ta[W], tb[W];
         {
             R fa[LA::nc], fb[NB];
-            node_state<LA, LB>(sa, sb, g, pv, na, fa, fb, body.ea, body.eb, oa, ob);
+            node_state<LA, LB>(sa, sb, g, pv, e.y, fa, fb, body.ea, body.eb, oa, ob);
             put_state<LA>(ta, fa, body.ea, oa);
             if constexpr (LB::nc > 0) put_state<LB>(ta + LA::nc, fb, body.eb, ob);
         }
         {
             R fa[LA::nc], fb[NB];
-            node_state<LA, LB>(sa, sb, g, pv, nb, fa, fb, body.ea, body.eb, oa, ob);
+            node_state<LA, LB>(sa, sb, g, pv, e.z, fa, fb, body.ea, body.eb, oa, ob);
             put_state<LA>(tb, fa, body.ea, oa);
             if constexpr (LB::nc > 0) put_state<LB>(tb + LA::nc, fb, body.eb, ob);
         }
-        sfor<0, LA::nc + NB>([&](auto c) {
+        sfor<0, W>([&](auto c) {
             constexpr int C = decltype(c)::value;
-            const double m = 0.5 * (ta[C] + tb[C]);
-            sm[C] = q == 0 ? m : sm[C] + m;
+            sm[C] = 0.5 * (ta[C] + tb[C]);
         });
     }
-    if (npair == 3) {
-        sfor<0, LA::nc + NB>([&](auto c) {
-            constexpr int C = decltype(c)::value;
-            sm[C] = sm[C] / 3.0;
-        });
+    if (blockIdx.x == 0) {  // uniform: every lane of workgroup 0 reaches the barrier
+        if (valid && !corner && e.w >= 0 && e.w < kCornerEdgeSlots) {
+            sfor<0, W>([&](auto c) {
+                constexpr int C = decltype(c)::value;
+                edge_state[e.w * W + C] = sm[C];
+            });
+        }
+        __syncthreads();
+        if (valid && corner) {
+            const double *ex = edge_state + e2.x * W, *ey = edge_state + e2.y * W, *ez = edge_state + e2.z * W;
+            sfor<0, W>([&](auto c) {
+                constexpr int C = decltype(c)::value;
+                sm[C] = (ex[C] + ey[C] + ez[C]) / 3.0;
+            });
+        }
     }
+    if (!valid) return;
     finish_smoothed<LA, LB, Body, S>(sa, sb, da, db, g, pv, e.x, [&](int o) { return sm[o]; }, body);
 }
 
@@ -888,13 +903,17 @@ static hipError_t launch_collide(const CollideLaunch &cl, Make &&make) {
             return hipSuccess;
         }
         if (cl.n_self > 0) {  // 3D smoothed nodes on the second stream, concurrently with the pass
-            hipError_t er;
-            if ((er = hipEventRecord(cl.ev_fork, cl.stream)) != hipSuccess) return er;
-            if ((er = hipStreamWaitEvent(cl.self_stream, cl.ev_fork, 0)) != hipSuccess) return er;
-            LBM_LAUNCH((k_collide_smoothed_self<LA, LB, B, S>), nblocks(cl.n_self), cl.self_stream, sa, sb, da, db, g,
-                       pv, cl.self_ents, cl.n_self, body);
-            if ((er = hipGetLastError()) != hipSuccess) return er;
-            if ((er = hipEventRecord(cl.ev_join, cl.self_stream)) != hipSuccess) return er;
+            if constexpr (LA::nd == 3) {
+                hipError_t er;
+                if ((er = hipEventRecord(cl.ev_fork, cl.stream)) != hipSuccess) return er;
+                if ((er = hipStreamWaitEvent(cl.self_stream, cl.ev_fork, 0)) != hipSuccess) return er;
+                LBM_LAUNCH((k_collide_smoothed_self<LA, LB, B, S>), nblocks(cl.n_self), cl.self_stream, sa, sb, da,
+                           db, g, pv, cl.self_ents, cl.n_self, body);
+                if ((er = hipGetLastError()) != hipSuccess) return er;
+                if ((er = hipEventRecord(cl.ev_join, cl.self_stream)) != hipSuccess) return er;
+            } else {
+                return hipErrorInvalidValue;  // 2D corners never take this form
+            }
         }
         LBM_LAUNCH((k_collide_fused<LA, LB, B, S>), nblocks(g.N) + pv.corner_block, cl.stream, sa, sb, da, db, g, pv,
                    cl.any_smooth, body);

@@ -81,6 +81,8 @@ struct ProgView {
     const int4 *corners;      // (node, state of x-neighbour, state of y-neighbour, smoothed-lattice bits)
 };
 constexpr int kMaxSide = 8;
+constexpr int kCornerEdgeSlots = 24;  // 3D edge nodes beside a corner (k_collide_smoothed_self's LDS)
+constexpr int kSmoothedGroup = 256;   // lanes of k_collide_smoothed_self's workgroup 0 (its block size)
 constexpr int kSideStride = 40;  // >= nc(first) + nc(second) of any lattice pair (D3Q27 + D3Q7: 34)
 
 // Boundary slot of node (i, j, k) on an nx*ny*nz grid, -1 for interior nodes.  Surface nodes are
