@@ -15,8 +15,9 @@ set of a duct heat sink:
 By default no SmoothCorner: the corner and edge nodes lie on bounce-back walls, and each time step
 is one fused kernel pass.  With smooth_corners=True both lattices are smoothed after the boundary
 calls of every forward and adjoint step, as the reference's 3D driver does (heatsink3D.cpp:175-176,
-215-216); the fused pass then skips the edge and corner nodes and saves the post-boundary states of
-their face neighbours, and a second kernel smooths, collides and stores the edges and corners.
+215-216); the fused pass then skips the edge and corner nodes, which a kernel on the context's
+second stream smooths, collides and stores concurrently with the pass (107.8 vs 91.7 us per step at
+128x128x64).
 The Navier-Stokes lattice is D3Q15 (the reference's) or D3Q27 (ns_kind=D3Q27, no reference
 counterpart; its velocity and pressure boundaries are non-equilibrium bounce-back).
 """
