@@ -21,6 +21,9 @@ Also reported:
   c3_fp16, c3_fp32 -- the same evaluation with FP16 / FP32 population storage (collided in FP32,
                    FP64 once the residual reaches its floor), with the objective's and the
                    sensitivity's error against the FP64 evaluation.
+  c3_smoothed   -- the same evaluation with SmoothCorner on both lattices every step (the
+                   reference's 3D driver sequence, heatsink3D.cpp:175-176, 215-216), FP64: the
+                   fused pass plus the edges and corners on a second stream concurrently with it.
   heatsink2d    -- the 2D heat sink (200 x 400, D2Q9 + D2Q5, FP64): MLUPS, its fused step's roofline,
                    the same kernel on 4096^2 (HBM bound), FP32 / FP16 variants, the reference pairing
                    D2Q9 + D2Q9 and the reference CPU path on it.
@@ -192,6 +195,27 @@ def c3_precision_extra(ctx, a, r64, dfds64, prec):
            "objective_rel_err_vs_fp64": abs(r["objective"] - r64["objective"]) / abs(r64["objective"]),
            "dfds_max_err_rel_to_max_vs_fp64": float(np.abs(dfds - dfds64).max() / np.abs(dfds64).max()),
            "residuals": [max(r["fwd_residual"]), max(r["adj_residual"])]}
+    hs.close()
+    return out
+
+
+def c3_smoothed_extra(ctx, a, r64):
+    """The 3D evaluation with SmoothCorner on both lattices (FP64, the default concurrent form)."""
+    from panslbm2_amd import D3Q7
+    from panslbm2_amd.heatsink3d import Heatsink3D
+
+    hs = Heatsink3D(ctx, a.lx3, a.ly3, a.lz3, ad_kind=D3Q7, max_steps=a.max_steps, use_graphs=not a.no_graphs,
+                    smooth_corners=True)
+    hs.evaluate()  # captures the graphs
+    r = hs.evaluate()
+    steps = r["fwd_steps"] + r["adj_steps"]
+    out = {"mlups": round(hs.N * steps / r["seconds"] / 1e6, 1), "seconds": round(r["seconds"], 4),
+           "fwd_steps": r["fwd_steps"], "adj_steps": r["adj_steps"],
+           "us_per_step": round(r["seconds"] / steps * 1e6, 1),
+           "unsmoothed_us_per_step": round(r64["seconds"] / (r64["fwd_steps"] + r64["adj_steps"]) * 1e6, 1),
+           "objective": r["objective"],
+           "note": "edges and corners smoothed every forward and adjoint step; the objective differs from "
+                   "the unsmoothed one because the edge and corner states do"}
     hs.close()
     return out
 
@@ -685,6 +709,7 @@ def main():
     ap.add_argument("--max-steps", type=int, default=400000, help="cap on forward and on adjoint steps")
     ap.add_argument("--no-graphs", action="store_true", help="launch every step (for profilers)")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-smoothed", action="store_true", help="skip the c3_smoothed extra")
     ap.add_argument("--no-transient", action="store_true")
     ap.add_argument("--batch", type=int, default=64, help="designs in the batched evaluation (0: skip it)")
     ap.add_argument("--warm-iters", type=int, default=10, help="3D design-loop iterations per warm-start mode (0: skip)")
@@ -779,6 +804,8 @@ def main():
     if not a.no_fp32:
         extra["c3_fp16"] = c3_precision_extra(ctx, a, r64, dfds64, F16)
         extra["c3_fp32"] = c3_precision_extra(ctx, a, r64, dfds64, F32)
+    if not a.no_smoothed:
+        extra["c3_smoothed"] = c3_smoothed_extra(ctx, a, r64)
     if not a.no_2d:
         extra["heatsink2d"] = heatsink2d_extra(ctx, a)
     if a.batch > 0:
