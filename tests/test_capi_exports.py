@@ -40,6 +40,9 @@ def test_library_reports_errors_without_gpu():
     h = ctypes.c_void_p()
     rc = L.lbm_context_create(10 ** 6, ctypes.byref(h))
     assert rc != 0 and L.lbm_last_error()
+    # entry points that take a context refuse a null one with an argument error
+    assert L.lbm_context_set_concurrent_smoothing(None, 1) == -1  # LBM_ERR_ARG
+    assert b"null" in L.lbm_last_error()
 
 
 def declared_params():
