@@ -181,3 +181,32 @@ def test_design_loop_matches_oracle_loop(ctx):
         assert np.abs(d - wd).max() <= 1e-9 * np.abs(wd).max()
         assert np.abs(x - wx).max() <= 1e-9
     assert np.abs(got[-1][1] - got[0][1]).max() > 1e-3  # the designs moved
+
+
+@pytest.mark.parametrize("prec_name", ["F32", "F16"])
+def test_smoothed_concurrent_equals_after_pass_reduced_precision(ctx, prec_name):
+    """With FP32 / FP16 population storage, the smoothed 3D steps in the concurrent form (second
+    stream, each lane rebuilding its face neighbours' states) and in the after-pass form (the states
+    the pass saved) give identical populations, fields and objective: both take the same FP64 means
+    of the same face states."""
+    import panslbm2_amd as pa
+    from panslbm2_amd.heatsink3d import Heatsink3D
+
+    print("smoothed 3D concurrent vs after the pass", prec_name, flush=True)
+    prec = getattr(pa, prec_name)
+    kw = dict(lx=14, ly=10, lz=9, plate=3, seed=4, max_steps=300, eps=1e-30, smooth_corners=True, prec=prec)
+    out = []
+    for conc in (1, 0):
+        pa.check(ctx.lib.lbm_context_set_concurrent_smoothing(ctx.h, conc))
+        try:
+            hs = Heatsink3D(ctx, **kw)
+            r = hs.evaluate()
+            out.append((r["objective"], hs.pf.download(), hs.pg.download(), hs.d["dfds"].download(),
+                        hs.d["tem"].download()))
+            hs.close()
+        finally:
+            pa.check(ctx.lib.lbm_context_set_concurrent_smoothing(ctx.h, 1))
+    (o1, *a1), (o0, *a0) = out
+    assert np.isfinite(o1) and o1 == o0
+    for x, y in zip(a1, a0):
+        np.testing.assert_array_equal(x, y)
